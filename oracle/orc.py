@@ -1,0 +1,197 @@
+"""ctypes bindings for the oracle (liboracle.so) and the compiled reference
+(_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker, never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+ORDER_REFERENCE, ORDER_COLOUR = 0, 1
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_U = C.POINTER(C.c_uint32)
+_I = C.POINTER(C.c_int32)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _bind(lib, prefix):
+    f = lambda n: getattr(lib, prefix + n)
+    f("destroy").argtypes = [_P]
+    f("add_bodies").argtypes = [_P, C.c_int, _P]
+    f("add_colliders").argtypes = [_P, C.c_int, _P]
+    f("step").argtypes = [_P, C.c_int]
+    f("num_bodies").argtypes = [_P]
+    f("get_bodies").argtypes = [_P, _D, _D, _D, _D]
+    f("set_bodies").argtypes = [_P, _D, _D, _D, _D]
+    f("num_pairs").argtypes = [_P]
+    f("get_pairs").argtypes = [_P, _U]
+    f("num_contacts").argtypes = [_P]
+    f("get_contacts").argtypes = [_P, _U, _D, _D, _D, _D, _D]
+    f("collide").restype = C.c_int
+
+
+_libs = {}
+
+
+def oracle_lib():
+    if "orc" not in _libs:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_SO} (run `make -C oracle`)")
+        lib = C.CDLL(ORACLE_SO)
+        lib.orc_create.restype = _P
+        lib.orc_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int]
+        _bind(lib, "orc_")
+        lib.orc_collide.argtypes = [C.c_int, _P, _D, _P, _D, _D]
+        lib.orc_num_solve_pairs.argtypes = [_P]
+        lib.orc_get_solve_pairs.argtypes = [_P, _U, _I, _I]
+        lib.orc_num_colours.argtypes = [_P]
+        lib.orc_mix32.restype = C.c_uint32
+        lib.orc_mix32.argtypes = [C.c_uint32]
+        _libs["orc"] = lib
+    return _libs["orc"]
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    if "ref" not in _libs:
+        if not ref_available():
+            raise RuntimeError(f"reference build missing: {REF_SO} (run `make -C oracle ref` where the reference is present)")
+        lib = C.CDLL(REF_SO)
+        lib.ref_create.restype = _P
+        lib.ref_create.argtypes = [C.c_double, C.c_int]
+        _bind(lib, "ref_")
+        lib.ref_collide.argtypes = [_P, _D, _P, _D, _D]
+        lib.ref_collision_groups.argtypes = [_P, _I]
+        _libs["ref"] = lib
+    return _libs["ref"]
+
+
+class _WorldBase:
+    prefix = ""
+
+    def __init__(self, lib, handle):
+        self.lib = lib
+        self.h = handle
+        self._fn = lambda n: getattr(lib, self.prefix + n)
+
+    def close(self):
+        if self.h:
+            self._fn("destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, scene):
+        b = np.ascontiguousarray(scene.bodies)
+        c = np.ascontiguousarray(scene.colliders)
+        self._fn("add_bodies")(self.h, len(b), b.ctypes.data_as(_P))
+        self._fn("add_colliders")(self.h, len(c), c.ctypes.data_as(_P))
+        return self
+
+    def step(self, frames=1):
+        self._fn("step")(self.h, frames)
+
+    def state(self):
+        n = self._fn("num_bodies")(self.h)
+        p, q, v, w = (np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3)))
+        self._fn("get_bodies")(self.h, _ptr(p, _D), _ptr(q, _D), _ptr(v, _D), _ptr(w, _D))
+        return {"position": p, "orientation": q, "linear_velocity": v, "angular_velocity": w}
+
+    def set_state(self, st):
+        a = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in st}
+        self._fn("set_bodies")(self.h, _ptr(a.get("position"), _D), _ptr(a.get("orientation"), _D),
+                               _ptr(a.get("linear_velocity"), _D), _ptr(a.get("angular_velocity"), _D))
+
+    def pairs(self):
+        n = self._fn("num_pairs")(self.h)
+        out = np.zeros((n, 2), np.uint32)
+        if n:
+            self._fn("get_pairs")(self.h, _ptr(out, _U))
+        return out
+
+    def contacts(self):
+        n = self._fn("num_contacts")(self.h)
+        ids = np.zeros((n, 4), np.uint32)
+        nrm, ra, rb = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        dep, lam = np.zeros(n), np.zeros((n, 2))
+        if n:
+            self._fn("get_contacts")(self.h, _ptr(ids, _U), _ptr(nrm, _D), _ptr(ra, _D), _ptr(rb, _D),
+                                     _ptr(dep, _D), _ptr(lam, _D))
+        return {"ids": ids, "normal": nrm, "r_a_local": ra, "r_b_local": rb, "depth": dep, "lambda": lam}
+
+
+class OracleWorld(_WorldBase):
+    """The oracle restatement: fp32 selects float arithmetic, order selects
+    the reference's sequential order or the GPU's colour order."""
+    prefix = "orc_"
+
+    def __init__(self, timestep=1.0 / 60.0, substeps=20, fp32=False, order=ORDER_REFERENCE):
+        lib = oracle_lib()
+        super().__init__(lib, lib.orc_create(1 if fp32 else 0, order, timestep, substeps))
+        self.fp32, self.order = fp32, order
+
+    def solve_pairs(self):
+        n = self.lib.orc_num_solve_pairs(self.h)
+        cc, col, cnt = np.zeros((n, 2), np.uint32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        if n:
+            self.lib.orc_get_solve_pairs(self.h, _ptr(cc, _U), _ptr(col, _I), _ptr(cnt, _I))
+        return {"colliders": cc, "colour": col, "count": cnt, "ncolours": self.lib.orc_num_colours(self.h)}
+
+
+class RefWorld(_WorldBase):
+    """The reference engine compiled from its own sources (FP64, sequential)."""
+    prefix = "ref_"
+
+    def __init__(self, timestep=1.0 / 60.0, substeps=20):
+        lib = ref_lib()
+        super().__init__(lib, lib.ref_create(timestep, substeps))
+
+    def collision_groups(self):
+        n = self.lib.ref_num_contacts(self.h)
+        out = np.full(n, -1, np.int32)
+        g = self.lib.ref_collision_groups(self.h, _ptr(out, _I)) if n else 0
+        return out, g
+
+
+def _collide(fn, pre, a, tfa, b, tfb):
+    out = np.zeros(20)
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    tfa = np.ascontiguousarray(tfa, dtype=np.float64)
+    tfb = np.ascontiguousarray(tfb, dtype=np.float64)
+    hit = fn(*pre, a.ctypes.data_as(_P), _ptr(tfa, _D), b.ctypes.data_as(_P), _ptr(tfb, _D), _ptr(out, _D))
+    n = int(out[0])
+    return {"hit": bool(hit), "n": n, "normal": out[1:4].copy(),
+            "points": out[4:4 + 4 * n].reshape(n, 4)[:, :3].copy(), "depth": out[4:4 + 4 * n].reshape(n, 4)[:, 3].copy()}
+
+
+def oracle_collide(a, tfa, b, tfb, fp32=False):
+    return _collide(oracle_lib().orc_collide, (1 if fp32 else 0,), a, tfa, b, tfb)
+
+
+def ref_collide(a, tfa, b, tfb):
+    return _collide(ref_lib().ref_collide, (), a, tfa, b, tfb)
+
+
+def mix32(x: int) -> int:
+    return int(oracle_lib().orc_mix32(x))

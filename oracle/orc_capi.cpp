@@ -1,0 +1,203 @@
+// Oracle C-ABI (ctypes entry points for tests/ and bench.py's cpu_baseline leg).
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the GPU pipeline, never the product.
+// The descriptor layouts match the product's C-ABI descriptors byte for byte so
+// one numpy structured array feeds both.
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include "orc_world.hpp"
+
+extern "C" {
+
+typedef struct {
+    int32_t type, _pad;
+    double mass;
+    double inertia[6];   // xx yy zz xy xz yz
+    double position[3];
+    double orientation[4];   // w x y z
+    double linear_velocity[3];
+    double angular_velocity[3];
+} orc_body_desc;
+
+typedef struct {
+    int32_t shape;
+    uint32_t body_id;
+    int32_t is_static;
+    int32_t hull_index;
+    double dims[4];
+    double local_pos[3];
+    double local_rot[4];
+    double restitution, static_friction, dynamic_friction;
+} orc_collider_desc;
+
+}  // extern "C"
+
+namespace {
+
+struct Any {
+    virtual ~Any() {}
+    virtual void add_body(const orc_body_desc &d) = 0;
+    virtual void add_collider(const orc_collider_desc &d) = 0;
+    virtual void step(int frames) = 0;
+    virtual int nbodies() const = 0;
+    virtual void bodies(double *p, double *q, double *v, double *w) const = 0;
+    virtual int npairs() const = 0;
+    virtual void pairs(uint32_t *out) const = 0;
+    virtual int ncontacts() const = 0;
+    virtual void contacts(uint32_t *ids4, double *n3, double *ra3, double *rb3, double *dep, double *lam2) const = 0;
+    virtual int nspairs() const = 0;
+    virtual void spairs(uint32_t *ca_cb, int32_t *col, int32_t *cnt) const = 0;
+    virtual int ncolours() const = 0;
+    virtual void set_state(const double *p, const double *q, const double *v, const double *w) = 0;
+};
+
+template <class T> struct Impl : Any {
+    orc::World<T> w;
+    void add_body(const orc_body_desc &d) override {
+        orc::SM<T> I(T(d.inertia[0]), T(d.inertia[1]), T(d.inertia[2]), T(d.inertia[3]), T(d.inertia[4]), T(d.inertia[5]));
+        w.add_body(d.type, T(d.mass), I, orc::V3<T>(T(d.position[0]), T(d.position[1]), T(d.position[2])),
+                   orc::Q<T>(T(d.orientation[0]), T(d.orientation[1]), T(d.orientation[2]), T(d.orientation[3])),
+                   orc::V3<T>(T(d.linear_velocity[0]), T(d.linear_velocity[1]), T(d.linear_velocity[2])),
+                   orc::V3<T>(T(d.angular_velocity[0]), T(d.angular_velocity[1]), T(d.angular_velocity[2])));
+    }
+    void add_collider(const orc_collider_desc &d) override {
+        orc::ShapeP<T> s;
+        s.kind = d.shape;
+        s.a = T(d.dims[0]); s.b = T(d.dims[1]); s.c = T(d.dims[2]); s.d = T(d.dims[3]);
+        w.add_collider(s, d.body_id, d.is_static != 0,
+                       orc::V3<T>(T(d.local_pos[0]), T(d.local_pos[1]), T(d.local_pos[2])),
+                       orc::Q<T>(T(d.local_rot[0]), T(d.local_rot[1]), T(d.local_rot[2]), T(d.local_rot[3])),
+                       T(d.restitution), T(d.static_friction), T(d.dynamic_friction));
+    }
+    void step(int frames) override { for (int f = 0; f < frames; ++f) w.step(); }
+    int nbodies() const override { return (int)w.btype.size(); }
+    void bodies(double *p, double *q, double *v, double *av) const override {
+        for (size_t i = 0; i < w.btype.size(); ++i) {
+            if (p) { p[3 * i] = w.pos[i].x; p[3 * i + 1] = w.pos[i].y; p[3 * i + 2] = w.pos[i].z; }
+            if (q) { q[4 * i] = w.q[i].w; q[4 * i + 1] = w.q[i].x; q[4 * i + 2] = w.q[i].y; q[4 * i + 3] = w.q[i].z; }
+            if (v) { v[3 * i] = w.lv[i].x; v[3 * i + 1] = w.lv[i].y; v[3 * i + 2] = w.lv[i].z; }
+            if (av) { av[3 * i] = w.av[i].x; av[3 * i + 1] = w.av[i].y; av[3 * i + 2] = w.av[i].z; }
+        }
+    }
+    void set_state(const double *p, const double *q, const double *v, const double *av) override {
+        for (size_t i = 0; i < w.btype.size(); ++i) {
+            if (p) w.pos[i] = orc::V3<T>(T(p[3 * i]), T(p[3 * i + 1]), T(p[3 * i + 2]));
+            if (q) w.q[i] = orc::Q<T>(T(q[4 * i]), T(q[4 * i + 1]), T(q[4 * i + 2]), T(q[4 * i + 3]));
+            if (v) w.lv[i] = orc::V3<T>(T(v[3 * i]), T(v[3 * i + 1]), T(v[3 * i + 2]));
+            if (av) w.av[i] = orc::V3<T>(T(av[3 * i]), T(av[3 * i + 1]), T(av[3 * i + 2]));
+        }
+    }
+    int npairs() const override { return (int)w.sap_pairs.size(); }
+    void pairs(uint32_t *out) const override {
+        for (size_t i = 0; i < w.sap_pairs.size(); ++i) { out[2 * i] = w.sap_pairs[i].first; out[2 * i + 1] = w.sap_pairs[i].second; }
+    }
+    // Contacts of the last substep, in solve-list order: reference mode = emission
+    // order; colour mode = pair order, points in manifold order.
+    template <class F> void each_contact(F &&f) const {
+        if (w.order == orc::ORDER_REFERENCE) {
+            for (auto &k : w.contacts) f(k);
+        } else {
+            for (size_t p = 0; p < w.spairs.size(); ++p)
+                for (int k = 0; k < w.pcount[p]; ++k) f(w.pcont[p * 4 + k]);
+        }
+    }
+    int ncontacts() const override { int n = 0; each_contact([&](const orc::Contact<T> &) { ++n; }); return n; }
+    void contacts(uint32_t *ids4, double *n3, double *ra3, double *rb3, double *dep, double *lam2) const override {
+        size_t i = 0;
+        each_contact([&](const orc::Contact<T> &k) {
+            if (ids4) { ids4[4 * i] = k.ba; ids4[4 * i + 1] = k.bb; ids4[4 * i + 2] = k.ca; ids4[4 * i + 3] = k.cb; }
+            if (n3) { n3[3 * i] = k.n.x; n3[3 * i + 1] = k.n.y; n3[3 * i + 2] = k.n.z; }
+            if (ra3) { ra3[3 * i] = k.ra.x; ra3[3 * i + 1] = k.ra.y; ra3[3 * i + 2] = k.ra.z; }
+            if (rb3) { rb3[3 * i] = k.rb.x; rb3[3 * i + 1] = k.rb.y; rb3[3 * i + 2] = k.rb.z; }
+            if (dep) dep[i] = k.depth;
+            if (lam2) { lam2[2 * i] = k.lam_n; lam2[2 * i + 1] = k.lam_t; }
+            ++i;
+        });
+    }
+    int nspairs() const override { return (int)w.spairs.size(); }
+    void spairs(uint32_t *ca_cb, int32_t *col, int32_t *cnt) const override {
+        for (size_t p = 0; p < w.spairs.size(); ++p) {
+            if (ca_cb) { ca_cb[2 * p] = w.spairs[p].ca; ca_cb[2 * p + 1] = w.spairs[p].cb; }
+            if (col) col[p] = w.colour[p];
+            if (cnt) cnt[p] = w.pcount[p];
+        }
+    }
+    int ncolours() const override { return w.ncolours; }
+};
+
+// One narrowphase call (same layout as the reference driver's ref_collide).
+template <class T> int collide_t(const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
+                                        const double *tb, double *out) {
+    auto shp = [](const orc_collider_desc *d) {
+        orc::ShapeP<T> s;
+        s.kind = d->shape; s.a = T(d->dims[0]); s.b = T(d->dims[1]); s.c = T(d->dims[2]); s.d = T(d->dims[3]);
+        return s;
+    };
+    auto tf = [](const double *p) {
+        orc::TF<T> t;
+        t.pos = orc::V3<T>(T(p[0]), T(p[1]), T(p[2]));
+        t.rot = orc::Q<T>(T(p[3]), T(p[4]), T(p[5]), T(p[6]));
+        return t;
+    };
+    orc::Manifold<T> m;
+    bool hit = orc::dispatch(shp(a), tf(ta), shp(b), tf(tb), m);
+    out[0] = hit ? m.n : 0;
+    out[1] = m.normal.x; out[2] = m.normal.y; out[3] = m.normal.z;
+    for (int i = 0; i < 4; ++i) {
+        out[4 + 4 * i] = i < m.n ? (double)m.pos[i].x : 0.0;
+        out[5 + 4 * i] = i < m.n ? (double)m.pos[i].y : 0.0;
+        out[6 + 4 * i] = i < m.n ? (double)m.pos[i].z : 0.0;
+        out[7 + 4 * i] = i < m.n ? (double)m.depth[i] : 0.0;
+    }
+    return hit ? 1 : 0;
+}
+}  // namespace
+
+extern "C" {
+
+void *orc_create(int fp32, int order, double timestep, int substeps) {
+    Any *a;
+    if (fp32) {
+        auto *i = new Impl<float>();
+        i->w.timestep = (float)timestep; i->w.substeps = substeps; i->w.order = order;
+        a = i;
+    } else {
+        auto *i = new Impl<double>();
+        i->w.timestep = timestep; i->w.substeps = substeps; i->w.order = order;
+        a = i;
+    }
+    return a;
+}
+void orc_destroy(void *h) { delete static_cast<Any *>(h); }
+int orc_add_bodies(void *h, int n, const orc_body_desc *d) {
+    for (int i = 0; i < n; ++i) static_cast<Any *>(h)->add_body(d[i]);
+    return 0;
+}
+int orc_add_colliders(void *h, int n, const orc_collider_desc *d) {
+    for (int i = 0; i < n; ++i) static_cast<Any *>(h)->add_collider(d[i]);
+    return 0;
+}
+int orc_step(void *h, int frames) { static_cast<Any *>(h)->step(frames); return 0; }
+int orc_num_bodies(void *h) { return static_cast<Any *>(h)->nbodies(); }
+void orc_get_bodies(void *h, double *p, double *q, double *v, double *w) { static_cast<Any *>(h)->bodies(p, q, v, w); }
+void orc_set_bodies(void *h, const double *p, const double *q, const double *v, const double *w) {
+    static_cast<Any *>(h)->set_state(p, q, v, w);
+}
+int orc_num_pairs(void *h) { return static_cast<Any *>(h)->npairs(); }
+void orc_get_pairs(void *h, uint32_t *out) { static_cast<Any *>(h)->pairs(out); }
+int orc_num_contacts(void *h) { return static_cast<Any *>(h)->ncontacts(); }
+void orc_get_contacts(void *h, uint32_t *ids4, double *n3, double *ra3, double *rb3, double *dep, double *lam2) {
+    static_cast<Any *>(h)->contacts(ids4, n3, ra3, rb3, dep, lam2);
+}
+int orc_num_solve_pairs(void *h) { return static_cast<Any *>(h)->nspairs(); }
+void orc_get_solve_pairs(void *h, uint32_t *ca_cb, int32_t *col, int32_t *cnt) { static_cast<Any *>(h)->spairs(ca_cb, col, cnt); }
+int orc_num_colours(void *h) { return static_cast<Any *>(h)->ncolours(); }
+uint32_t orc_mix32(uint32_t x) { return orc::mix32(x); }
+
+int orc_collide(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b, const double *tb,
+                double *out) {
+    return fp32 ? collide_t<float>(a, ta, b, tb, out) : collide_t<double>(a, ta, b, tb, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,484 @@
+// Oracle: CPU restatement of rbps::World::step (broadphase, narrowphase, contact
+// emission, XPBD position/velocity solve), templated on the scalar type.
+//
+// TEST INFRASTRUCTURE ONLY (see orc_math.hpp).
+//
+// Two solve orders:
+//   ORDER_REFERENCE — the reference's sequential Gauss-Seidel in contact emission
+//                     order (pairs in SAP active-set order, then the plane loop).
+//                     With T=double this is the reference step bit-for-bit.
+//   ORDER_COLOUR    — the order the GPU pipeline uses: the frame's pairs in
+//                     canonical sweep order, coloured greedily in descending
+//                     hash priority (what Jones-Plassmann computes), then solved
+//                     colour by colour; a pair's points are solved in sequence.
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+#include "orc_collide.hpp"
+
+namespace orc {
+
+enum { ORDER_REFERENCE = 0, ORDER_COLOUR = 1 };
+static constexpr uint32_t BP_INVALID = 0xffffffffu;
+
+// lowbias32 integer mix (public-domain hash by C. Wellons); JP priorities.
+inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+template <class T> struct Contact {
+    uint32_t ba, bb, ca, cb;
+    V3<T> ra, rb, n;            // body-local lever arms, world normal A->B
+    T rest, sf, df;
+    T lam_n, lam_t;
+    V3<T> pa, pb;               // world points (rebuilt in the position solve)
+    T depth;
+    bool collision;
+    T relv;
+};
+
+template <class T> struct World {
+    T timestep = T(1.0 / 60.0);
+    int substeps = 20;
+    int order = ORDER_REFERENCE;
+
+    // ---- bodies (Body.hpp:52-69)
+    std::vector<int> btype;   // 0 STATIC, 1 DYNAMIC
+    std::vector<T> mass, inv_mass;
+    std::vector<V3<T>> force, torque, pos, lv, av, ppos;
+    std::vector<Q<T>> q, pq;
+    std::vector<SM<T>> I, Iinv, Iw, Iwinv;
+
+    // ---- colliders (Collider.hpp:43-63)
+    std::vector<ShapeP<T>> shape;
+    std::vector<V3<T>> lpos;
+    std::vector<Q<T>> lrot;
+    std::vector<uint32_t> cbody, bph;
+    std::vector<char> cstatic;
+    std::vector<T> rest, sfr, dfr;
+
+    // ---- broadphase (BroadPhase.hpp:78-125)
+    struct Obj { AABB<T> fat, tight; uint32_t uid; bool alive; };
+    struct EP { float v; uint32_t slot; uint32_t is_max; };
+    std::vector<Obj> objs;
+    std::vector<EP> eps;
+    std::vector<uint32_t> ep_at[2];   // slot -> endpoint index (min, max); replaces the O(n) scan
+    bool dirty = false;
+    T fat_margin = T(0.1), reinsert = T(0.05);
+    std::vector<std::pair<uint32_t, uint32_t>> sap_pairs;   // user ids, (lo, hi)
+
+    // ---- contacts
+    std::vector<Contact<T>> contacts;             // ORDER_REFERENCE: emission order
+    struct SPair { uint32_t ca, cb, ba, bb; };
+    std::vector<SPair> spairs;                     // ORDER_COLOUR: frame pair list
+    std::vector<int> colour;
+    int ncolours = 0;
+    std::vector<int> pcount;                       // points per pair this substep
+    std::vector<Contact<T>> pcont;                 // 4 slots per pair
+
+    // ------------------------------------------------------------ creation
+    // BodyAPI.hpp:53-80
+    uint32_t add_body(int type, T m, const SM<T> &inertia, V3<T> p, Q<T> o, V3<T> v, V3<T> w) {
+        uint32_t id = (uint32_t)btype.size();
+        T im = (type == 0) ? T(0) : T(1) / m;
+        SM<T> Ii = (type == 0) ? SM<T>() : inertia.inverse();
+        btype.push_back(type);
+        force.push_back(V3<T>(T(0), T(-9.8f) * m, T(0)));
+        torque.push_back(V3<T>(T(0)));
+        mass.push_back(m);
+        inv_mass.push_back(im);
+        pos.push_back(p); q.push_back(o); lv.push_back(v); av.push_back(w);
+        ppos.push_back(p); pq.push_back(o);
+        I.push_back(inertia); Iinv.push_back(Ii); Iw.push_back(inertia); Iwinv.push_back(Ii);
+        return id;
+    }
+    TF<T> collider_tf(uint32_t c) const {   // Collider.hpp:73-81
+        uint32_t b = cbody[c];
+        return {pos[b] + rotate(q[b], lpos[c]), q[b] * lrot[c]};
+    }
+    // ColliderAPI.hpp:61-91 + BroadPhase.cpp:169-197
+    uint32_t add_collider(const ShapeP<T> &s, uint32_t body, bool is_static, V3<T> lp, Q<T> lr, T e, T fs, T fd) {
+        uint32_t id = (uint32_t)shape.size();
+        shape.push_back(s); lpos.push_back(lp); lrot.push_back(lr); cbody.push_back(body);
+        cstatic.push_back(is_static ? 1 : 0); rest.push_back(e); sfr.push_back(fs); dfr.push_back(fd);
+        uint32_t h = BP_INVALID;
+        if (s.kind != SH_PLANE) {
+            TF<T> tf = collider_tf(id);
+            AABB<T> tight = shape_aabb(s, tf);
+            h = (uint32_t)objs.size();
+            Obj o;
+            o.tight = tight;
+            V3<T> m(fat_margin);
+            o.fat = {tight.mn - m, tight.mx + m};
+            o.uid = id;
+            o.alive = true;
+            objs.push_back(o);
+            ep_at[0].push_back((uint32_t)eps.size());
+            eps.push_back({(float)o.fat.mn.x, h, 0});
+            ep_at[1].push_back((uint32_t)eps.size());
+            eps.push_back({(float)o.fat.mx.x, h, 1});
+            dirty = true;
+        }
+        bph.push_back(h);
+        return id;
+    }
+
+    // ------------------------------------------------------------ broadphase
+    // BroadPhase.cpp:211-232 with the swept box of BroadPhase.hpp:228-249
+    void bp_move(uint32_t h, const AABB<T> &nt) {
+        Obj &o = objs[h];
+        o.tight = nt;
+        const AABB<T> &f = o.fat;
+        if (nt.mn.x >= f.mn.x && nt.mx.x <= f.mx.x && nt.mn.y >= f.mn.y && nt.mx.y <= f.mx.y &&
+            nt.mn.z >= f.mn.z && nt.mx.z <= f.mx.z)
+            return;
+        V3<T> m(fat_margin + reinsert);
+        o.fat = {nt.mn - m, nt.mx + m};
+        eps[ep_at[0][h]].v = (float)o.fat.mn.x;
+        eps[ep_at[1][h]].v = (float)o.fat.mx.x;
+        dirty = true;
+    }
+    // CollisionPipeline.cpp:14-53
+    void refit(T dt) {
+        for (uint32_t i = 0; i < shape.size(); ++i) {
+            if (cstatic[i]) continue;
+            if (bph[i] == BP_INVALID) continue;
+            uint32_t b = cbody[i];
+            AABB<T> tight = shape_aabb(shape[i], collider_tf(i));
+            V3<T> dv = lv[b] * dt;
+            V3<T> emn(fat_margin + (dv.x < 0 ? -dv.x : T(0)), fat_margin + (dv.y < 0 ? -dv.y : T(0)),
+                      fat_margin + (dv.z < 0 ? -dv.z : T(0)));
+            V3<T> emx(fat_margin + (dv.x > 0 ? dv.x : T(0)), fat_margin + (dv.y > 0 ? dv.y : T(0)),
+                      fat_margin + (dv.z > 0 ? dv.z : T(0)));
+            bp_move(bph[i], AABB<T>{tight.mn - emn, tight.mx + emx});
+        }
+    }
+    // BroadPhase.cpp:80-94 (a stable sort on the FP32 key; equal keys keep their previous order)
+    void bp_sort() {
+        std::stable_sort(eps.begin(), eps.end(), [](const EP &a, const EP &b) { return a.v < b.v; });
+        for (uint32_t k = 0; k < eps.size(); ++k) ep_at[eps[k].is_max][eps[k].slot] = k;
+    }
+    static std::pair<uint32_t, uint32_t> ordered(uint32_t a, uint32_t b) {
+        return a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+    }
+    // BroadPhase.cpp:104-152: active-set sweep, pairs in the reference's emission order.
+    void sweep_reference() {
+        sap_pairs.clear();
+        std::vector<uint32_t> active;
+        for (const EP &e : eps) {
+            const Obj &o = objs[e.slot];
+            if (!o.alive) continue;
+            if (!e.is_max) {
+                for (uint32_t s : active) {
+                    const Obj &ot = objs[s];
+                    if (!ot.alive) continue;
+                    if (!overlap(o.fat, ot.fat)) continue;
+                    sap_pairs.push_back(ordered(o.uid, ot.uid));
+                }
+                active.push_back(e.slot);
+            } else {
+                auto it = std::find(active.begin(), active.end(), e.slot);
+                if (it != active.end()) active.erase(it);
+            }
+        }
+    }
+    // Same pair set, canonical order: each min endpoint scans forward to its own max.
+    void sweep_canonical() {
+        sap_pairs.clear();
+        for (uint32_t i = 0; i < eps.size(); ++i) {
+            const EP &e = eps[i];
+            if (e.is_max || !objs[e.slot].alive) continue;
+            const Obj &a = objs[e.slot];
+            for (uint32_t j = i + 1; j < eps.size(); ++j) {
+                const EP &f = eps[j];
+                if (f.slot == e.slot) break;   // own max endpoint
+                if (f.is_max || !objs[f.slot].alive) continue;
+                if (overlap(a.fat, objs[f.slot].fat)) sap_pairs.push_back(ordered(a.uid, objs[f.slot].uid));
+            }
+        }
+    }
+    void bp_update() {   // BroadPhase.cpp:234-245
+        if (dirty) { bp_sort(); dirty = false; }
+        if (order == ORDER_REFERENCE) sweep_reference(); else sweep_canonical();
+    }
+
+    // ------------------------------------------------------------ narrowphase + emission
+    // CollisionPipeline.cpp:59-104
+    void make_contacts(const Manifold<T> &m, uint32_t ba, uint32_t bb, uint32_t ca, uint32_t cb, Contact<T> *out) const {
+        for (int i = 0; i < m.n; ++i) {
+            Contact<T> &k = out[i];
+            T half = m.depth[i] * T(0.5);
+            V3<T> pa = m.pos[i] + m.normal * half;
+            V3<T> pb = m.pos[i] - m.normal * half;
+            k.ba = ba; k.bb = bb; k.ca = ca; k.cb = cb;
+            k.ra = rotate(conj(q[ba]), pa - pos[ba]);
+            k.rb = rotate(conj(q[bb]), pb - pos[bb]);
+            k.n = m.normal;
+            k.rest = smin(rest[ca], rest[cb]);
+            k.sf = (sfr[ca] + sfr[cb]) * T(0.5);
+            k.df = (dfr[ca] + dfr[cb]) * T(0.5);
+            k.lam_n = T(0); k.lam_t = T(0);
+            k.pa = pa; k.pb = pb;
+            k.depth = m.depth[i];
+            k.collision = false;
+            k.relv = T(0);
+        }
+    }
+    bool pair_ok(uint32_t ca, uint32_t cb) const {   // CollisionPipeline.cpp:121-133
+        if (cstatic[ca] && cstatic[cb]) return false;
+        return cbody[ca] != cbody[cb];
+    }
+    // CollisionPipeline.cpp:110-198
+    void narrowphase_reference() {
+        contacts.clear();
+        Contact<T> buf[4];
+        for (auto &pr : sap_pairs) {
+            uint32_t ca = pr.first, cb = pr.second;
+            if (ca >= shape.size() || cb >= shape.size()) continue;
+            if (!pair_ok(ca, cb)) continue;
+            Manifold<T> m;
+            if (dispatch(shape[ca], collider_tf(ca), shape[cb], collider_tf(cb), m)) {
+                make_contacts(m, cbody[ca], cbody[cb], ca, cb, buf);
+                for (int i = 0; i < m.n; ++i) contacts.push_back(buf[i]);
+            }
+        }
+        for (uint32_t pi = 0; pi < shape.size(); ++pi) {
+            if (shape[pi].kind != SH_PLANE) continue;
+            for (uint32_t di = 0; di < shape.size(); ++di) {
+                if (di == pi || shape[di].kind == SH_PLANE) continue;
+                if (!pair_ok(pi, di)) continue;
+                Manifold<T> m;
+                if (dispatch(shape[di], collider_tf(di), shape[pi], collider_tf(pi), m)) {
+                    make_contacts(m, cbody[di], cbody[pi], di, pi, buf);
+                    for (int i = 0; i < m.n; ++i) contacts.push_back(buf[i]);
+                }
+            }
+        }
+    }
+    // Frame pair list for the colour order: canonical SAP pairs then the plane loop.
+    void build_pairs_and_colour() {
+        spairs.clear();
+        for (auto &pr : sap_pairs)
+            if (pair_ok(pr.first, pr.second)) spairs.push_back({pr.first, pr.second, cbody[pr.first], cbody[pr.second]});
+        for (uint32_t pi = 0; pi < shape.size(); ++pi) {
+            if (shape[pi].kind != SH_PLANE) continue;
+            for (uint32_t di = 0; di < shape.size(); ++di) {
+                if (di == pi || shape[di].kind == SH_PLANE) continue;
+                if (!pair_ok(pi, di)) continue;
+                spairs.push_back({di, pi, cbody[di], cbody[pi]});
+            }
+        }
+        // Greedy first-fit in descending (mix32(p), p) priority == Jones-Plassmann result.
+        const uint32_t P = (uint32_t)spairs.size();
+        std::vector<std::vector<uint32_t>> by_body(btype.size());
+        for (uint32_t p = 0; p < P; ++p) {
+            if (btype[spairs[p].ba] == 1) by_body[spairs[p].ba].push_back(p);
+            if (btype[spairs[p].bb] == 1) by_body[spairs[p].bb].push_back(p);
+        }
+        std::vector<uint32_t> ord(P);
+        for (uint32_t p = 0; p < P; ++p) ord[p] = p;
+        std::sort(ord.begin(), ord.end(), [](uint32_t a, uint32_t b) {
+            uint32_t ha = mix32(a), hb = mix32(b);
+            return ha != hb ? ha > hb : a > b;
+        });
+        colour.assign(P, -1);
+        ncolours = 0;
+        std::vector<char> used;
+        for (uint32_t p : ord) {
+            used.assign(ncolours + 1, 0);
+            for (uint32_t b : {spairs[p].ba, spairs[p].bb}) {
+                if (btype[b] != 1) continue;
+                for (uint32_t nb : by_body[b])
+                    if (nb != p && colour[nb] >= 0) used[colour[nb]] = 1;
+            }
+            int c = 0;
+            while (used[c]) ++c;
+            colour[p] = c;
+            if (c + 1 > ncolours) ncolours = c + 1;
+        }
+        pcount.assign(P, 0);
+        pcont.resize((size_t)P * 4);
+    }
+    void narrowphase_colour() {
+        for (uint32_t p = 0; p < spairs.size(); ++p) {
+            const SPair &s = spairs[p];
+            Manifold<T> m;
+            pcount[p] = 0;
+            if (dispatch(shape[s.ca], collider_tf(s.ca), shape[s.cb], collider_tf(s.cb), m)) {
+                make_contacts(m, s.ba, s.bb, s.ca, s.cb, &pcont[(size_t)p * 4]);
+                pcount[p] = m.n;
+            }
+        }
+    }
+
+    // ------------------------------------------------------------ body kernels (Body.cpp)
+    void update_Iw(uint32_t i) {   // Body.cpp:11-16
+        M3<T> R = mat3_cast(q[i]);
+        Iw[i] = world_tensor(R, I[i]);
+        Iwinv[i] = world_tensor(R, Iinv[i]);
+    }
+    void integrate(T dt) {   // Body.cpp:18-39
+        for (uint32_t i = 0; i < btype.size(); ++i) {
+            if (btype[i] == 0) continue;
+            update_Iw(i);
+            ppos[i] = pos[i];
+            lv[i] = lv[i] + force[i] * inv_mass[i] * dt;
+            pos[i] = pos[i] + lv[i] * dt;
+            pq[i] = q[i];
+            av[i] = av[i] + (dt * Iwinv[i]) * (torque[i] - cross(av[i], Iw[i] * av[i]));
+            V3<T> ws = (T(0.5) * dt) * av[i];
+            Q<T> r(T(0.0), ws.x, ws.y, ws.z);
+            q[i] = normalize(q[i] + r * q[i]);
+        }
+    }
+    void update_velocities(T inv_dt) {   // Body.cpp:41-57
+        for (uint32_t i = 0; i < btype.size(); ++i) {
+            if (btype[i] == 0) continue;
+            lv[i] = (pos[i] - ppos[i]) * inv_dt;
+            Q<T> dq = q[i] * conj(pq[i]);
+            V3<T> w = (T(2) * inv_dt) * V3<T>(dq.x, dq.y, dq.z);
+            av[i] = dq.w >= T(0) ? w : -w;
+        }
+    }
+    void apply_pos_impulse(uint32_t i, V3<T> imp, V3<T> r) {   // Body.cpp:59-68
+        if (btype[i] == 0) return;
+        pos[i] = pos[i] + imp * inv_mass[i];
+        V3<T> rot = (Iwinv[i] * cross(r, imp)) * T(0.5);
+        Q<T> rq(T(0.0), rot.x, rot.y, rot.z);
+        q[i] = normalize(q[i] + rq * q[i]);
+    }
+    void apply_vel_impulse(uint32_t i, V3<T> imp, V3<T> r) {   // Body.cpp:79-85
+        if (btype[i] == 0) return;
+        lv[i] = lv[i] + imp * inv_mass[i];
+        av[i] = av[i] + Iwinv[i] * cross(r, imp);
+    }
+    T w_pos(uint32_t i, V3<T> r, V3<T> n) {   // Body.cpp:94-101 (refreshes the world tensors)
+        if (btype[i] == 0) return T(0.0);
+        update_Iw(i);
+        V3<T> c = cross(r, n);
+        return inv_mass[i] + dot(c, Iwinv[i] * c);
+    }
+
+    // ------------------------------------------------------------ contact solve (Contact.cpp)
+    void contact_position(Contact<T> &k, T inv_h) {   // Contact.cpp:35-75
+        (void)inv_h;
+        const uint32_t a = k.ba, b = k.bb;
+        V3<T> pa = pos[a] + rotate(q[a], k.ra);
+        V3<T> pb = pos[b] + rotate(q[b], k.rb);
+        k.pa = pa; k.pb = pb;
+        T d = dot(pa - pb, k.n);
+        if (d <= T(0.0)) { k.collision = false; return; }
+        k.collision = true;
+        V3<T> raw = pa - pos[a], rbw = pb - pos[b];
+        {
+            V3<T> vr = (lv[a] + cross(av[a], raw)) - (lv[b] + cross(av[b], rbw));
+            k.relv = dot(vr, k.n);
+        }
+        // normal (Contact.cpp:81-115)
+        {
+            T wa = w_pos(a, raw, k.n), wb = w_pos(b, rbw, k.n);
+            T den = wa + wb;
+            if (!(den < C<T>::EPS)) {
+                T dl = -d / den;
+                k.lam_n += dl;
+                if (k.lam_n > T(0.0)) k.lam_n = T(0.0);
+                V3<T> imp = dl * k.n;
+                apply_pos_impulse(a, imp, raw);
+                apply_pos_impulse(b, -imp, rbw);
+            }
+        }
+        // tangent (Contact.cpp:121-195)
+        V3<T> ra = rotate(q[a], k.ra), rb = rotate(q[b], k.rb);
+        V3<T> pac = pos[a] + ra, pbc = pos[b] + rb;
+        k.pa = pac; k.pb = pbc;
+        V3<T> ppa = ppos[a] + rotate(pq[a], k.ra);
+        V3<T> ppb = ppos[b] + rotate(pq[b], k.rb);
+        V3<T> dp = (pac - ppa) - (pbc - ppb);
+        V3<T> dpt = dp - dot(dp, k.n) * k.n;
+        T slip = length(dpt);
+        if (slip < C<T>::EPS) return;
+        V3<T> t = normalize(dpt);
+        T wa = w_pos(a, ra, t), wb = w_pos(b, rb, t);
+        T den = wa + wb;
+        if (den < C<T>::EPS) return;
+        T dl = -slip / den;
+        T old = k.lam_t;
+        k.lam_t += dl;
+        T cone = k.sf * k.lam_n;
+        if (k.lam_t < cone) k.lam_t = cone;
+        T act = k.lam_t - old;
+        if (std::abs(act) < C<T>::EPS) return;
+        V3<T> imp = act * t;
+        apply_pos_impulse(a, imp, ra);
+        apply_pos_impulse(b, -imp, rb);
+    }
+    void contact_velocity(Contact<T> &k, T h) {   // Contact.cpp:206-289
+        if (!k.collision) return;
+        const uint32_t a = k.ba, b = k.bb;
+        V3<T> raw = k.pa - pos[a], rbw = k.pb - pos[b];
+        V3<T> vr = (lv[a] + cross(av[a], raw)) - (lv[b] + cross(av[b], rbw));
+        T vn = dot(vr, k.n);
+        V3<T> vt = vr - k.n * vn;
+        T vtm = length(vt);
+        T e = k.rest;
+        if (std::abs(k.relv) <= T(2.0) * T(9.8) * h) e = T(0.0);
+        T vtarget = smin(-e * k.relv, T(0.0));
+        T dvn = vtarget - vn;
+        T dvt = T(0.0);
+        if (vtm > C<T>::EPS) {
+            T lim = k.df * std::abs(k.lam_n) / h;
+            dvt = -smin(lim, vtm);
+        }
+        T wan = w_pos(a, raw, k.n), wbn = w_pos(b, rbw, k.n);
+        T denn = wan + wbn;
+        if (denn > C<T>::EPS && std::abs(dvn) > C<T>::EPS) {
+            V3<T> imp = (dvn / denn) * k.n;
+            apply_vel_impulse(a, imp, raw);
+            apply_vel_impulse(b, -imp, rbw);
+        }
+        if (vtm > C<T>::EPS && std::abs(dvt) > C<T>::EPS) {
+            V3<T> t = vt / vtm;
+            T wat = w_pos(a, raw, t), wbt = w_pos(b, rbw, t);
+            T dent = wat + wbt;
+            if (dent > C<T>::EPS) {
+                V3<T> imp = (dvt / dent) * t;
+                apply_vel_impulse(a, imp, raw);
+                apply_vel_impulse(b, -imp, rbw);
+            }
+        }
+    }
+    template <class F> void for_each_colour_contact(F &&f) {
+        for (int c = 0; c < ncolours; ++c)
+            for (uint32_t p = 0; p < spairs.size(); ++p)
+                if (colour[p] == c)
+                    for (int k = 0; k < pcount[p]; ++k) f(pcont[(size_t)p * 4 + k]);
+    }
+
+    // ------------------------------------------------------------ World::step (World.hpp:192-212)
+    void step() {
+        const T h = timestep / T(substeps);
+        const T inv_h = T(1.0) / h;
+        refit(timestep);
+        bp_update();
+        if (order == ORDER_COLOUR) build_pairs_and_colour();
+        for (int s = 0; s < substeps; ++s) {
+            if (order == ORDER_REFERENCE) narrowphase_reference(); else narrowphase_colour();
+            integrate(h);
+            if (order == ORDER_REFERENCE)
+                for (auto &k : contacts) contact_position(k, inv_h);
+            else
+                for_each_colour_contact([&](Contact<T> &k) { contact_position(k, inv_h); });
+            update_velocities(inv_h);
+            if (order == ORDER_REFERENCE)
+                for (auto &k : contacts) contact_velocity(k, h);
+            else
+                for_each_colour_contact([&](Contact<T> &k) { contact_velocity(k, h); });
+        }
+    }
+};
+
+}  // namespace orc
