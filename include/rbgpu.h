@@ -1,0 +1,131 @@
+/*
+ * rbgpu — C-ABI of the MI355X XPBD substep pipeline (broadphase, narrowphase,
+ * contact generation, constraint colouring, position/velocity solve).
+ *
+ * This is the boundary that replaces the reference's `rbps::World` step path.
+ * Plain pointers and sizes only; all state lives on one GPU in FP32 and the
+ * host sees it through the get/set calls.  Every call returns RBGPU_OK (0) or a
+ * negative status; rbgpu_last_error() describes the last failure.
+ *
+ * Body and collider IDs follow the reference: a 24-bit slot plus 8 generation
+ * bits (include/storage/Soa.hpp:405-416); without removals an ID equals its
+ * dense index, which is how the reference pipeline uses them
+ * (src/rbps/CollisionPipeline.cpp:34-35, 124-125).
+ */
+#ifndef RBGPU_H
+#define RBGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RBGPU_OK = 0,
+    RBGPU_ERR_INVALID = -1,   /* bad argument or handle */
+    RBGPU_ERR_HIP = -2,       /* a HIP runtime call failed */
+    RBGPU_ERR_CAPACITY = -3,  /* a fixed capacity was exceeded (e.g. > 256 colours) */
+    RBGPU_ERR_OOM = -4,       /* device allocation failed */
+    RBGPU_ERR_UNSUPPORTED = -5
+};
+
+/* Body types (include/rbps/Body.hpp:11-15). */
+enum { RBGPU_STATIC = 0, RBGPU_DYNAMIC = 1 };
+
+/* Shape codes = the reference's rbc::Shape variant indices (include/rbc/shapes/ShapeTypes.hpp:69). */
+enum { RBGPU_SPHERE = 0, RBGPU_BOX = 1, RBGPU_CAPSULE = 3, RBGPU_PLANE = 6, RBGPU_HULL = 9 };
+
+/* Fields of rbps::BodyParams (include/rbps/API/BodyAPI.hpp:22-31). 168 bytes. */
+typedef struct {
+    int32_t type;                /* RBGPU_STATIC / RBGPU_DYNAMIC */
+    int32_t _pad;
+    double mass;
+    double inertia[6];           /* local inertia tensor: xx yy zz xy xz yz */
+    double position[3];
+    double orientation[4];       /* quaternion w x y z */
+    double linear_velocity[3];
+    double angular_velocity[3];
+} rbgpu_body_desc;
+
+/* Fields of rbps::ColliderParams (include/rbps/API/ColliderAPI.hpp:17-27). 128 bytes.
+ * dims: sphere {radius} | box {hx, hy, hz} | capsule {half_height, radius} | plane {nx, ny, nz, d}. */
+typedef struct {
+    int32_t shape;
+    uint32_t body_id;
+    int32_t is_static;
+    int32_t hull_index;          /* reserved, -1 */
+    double dims[4];
+    double local_pos[3];
+    double local_rot[4];         /* w x y z */
+    double restitution, static_friction, dynamic_friction;
+} rbgpu_collider_desc;
+
+typedef struct {
+    uint32_t frames;             /* World::step calls executed */
+    uint32_t sap_pairs;          /* broadphase pairs of the last frame (after the pipeline's filters) */
+    uint32_t pairs;              /* + plane pairs = the frame's narrowphase pair list */
+    uint32_t colours;            /* colours of the last frame's pair graph */
+    uint32_t jp_rounds;          /* Jones-Plassmann rounds launched for the last frame */
+    uint32_t contacts;           /* contact points of the last substep */
+    uint32_t bins[10];           /* pairs per narrowphase bin of the last frame */
+} rbgpu_stats;
+
+typedef struct rbgpu_world rbgpu_world;
+
+/* World(timestep, substeps) — include/rbps/API/World.hpp:87-106. */
+int rbgpu_world_create(double timestep, int substeps, rbgpu_world **out);
+int rbgpu_world_destroy(rbgpu_world *w);
+
+/* World::create_body / rbps::create_body — include/rbps/API/World.hpp:113-116,
+ * include/rbps/API/BodyAPI.hpp:53-80.  ids_out may be NULL. */
+int rbgpu_add_bodies(rbgpu_world *w, int n, const rbgpu_body_desc *desc, uint32_t *ids_out);
+
+/* World::create_collider / rbps::create_collider — include/rbps/API/World.hpp:124-132,
+ * include/rbps/API/ColliderAPI.hpp:61-91 (inserts into the broadphase). */
+int rbgpu_add_colliders(rbgpu_world *w, int n, const rbgpu_collider_desc *desc, uint32_t *ids_out);
+
+/* World::step — include/rbps/API/World.hpp:192-212; runs n_frames frames, asynchronously
+ * on the world's stream (the call returns once the work is queued and the per-frame
+ * bookkeeping read-backs are done). */
+int rbgpu_step(rbgpu_world *w, int n_frames);
+
+/* Waits for all queued work of the world. */
+int rbgpu_synchronize(rbgpu_world *w);
+
+/* Reads / writes the public body state (BodyCollection fields position, orientation,
+ * linear_velocity, angular_velocity; include/rbps/Body.hpp:52-69) as doubles,
+ * n*3 / n*4 values; any pointer may be NULL. */
+int rbgpu_get_bodies(rbgpu_world *w, double *position, double *orientation, double *linear_velocity,
+                     double *angular_velocity);
+int rbgpu_set_bodies(rbgpu_world *w, const double *position, const double *orientation,
+                     const double *linear_velocity, const double *angular_velocity);
+
+/* Broadphase output of the last frame (rbc::BroadPhaseState::pairs, include/rbc/BroadPhase.hpp:117)
+ * with the narrowphase driver's filters applied, in canonical sweep order: (id_a, id_b)
+ * with id_a < id_b.  Writes min(cap, n) pairs, *n = total. */
+int rbgpu_get_sap_pairs(rbgpu_world *w, uint32_t *pairs2, uint32_t cap, uint32_t *n);
+
+/* The last frame's pair list (SAP pairs then plane pairs) with each pair's colour. */
+int rbgpu_get_pair_colours(rbgpu_world *w, uint32_t *pairs2, int32_t *colour, uint32_t cap, uint32_t *n);
+
+/* Contacts of the last substep (rbps::ContactList, include/rbps/CollisionPipeline.hpp:46-139),
+ * listed per pair in pair-list order, points in manifold order.  ids4 = body_a, body_b,
+ * collider_a, collider_b; lam2 = normal_lambda, tangent_lambda.  Any pointer may be NULL. */
+int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *normal3, double *r_a_local3, double *r_b_local3,
+                       double *depth, double *lam2, uint32_t cap, uint32_t *n);
+
+int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
+
+/* Kernel timing with HIP events around every launch of one kernel family on the world's
+ * stream.  kernel: 0 off, 1 position solve, 2 velocity solve, 3 narrowphase, 4 integrate,
+ * 5 velocity update, 6 broadphase sweep.  rbgpu_get_timing returns the summed
+ * milliseconds and launches since the last reset (synchronises). */
+int rbgpu_set_timing(rbgpu_world *w, int kernel);
+int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, double *bytes);
+
+const char *rbgpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RBGPU_H */

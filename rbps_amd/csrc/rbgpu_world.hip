@@ -1,0 +1,748 @@
+// Host orchestration of the XPBD substep pipeline and the rbgpu C-ABI.
+//
+// One world = one HIP stream.  Per frame (World::step, include/rbps/API/World.hpp:192-212):
+//   refit + stable endpoint sort + sweep (pairs), plane pairs appended, Jones-Plassmann
+//   colouring of the pair graph, solve slots ordered by colour, narrowphase bins;
+//   then per substep: narrowphase -> integrate -> position solve (per colour) ->
+//   velocity update -> velocity solve (per colour).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/rbgpu.h"
+#include "rbgpu_kernels.hpp"
+
+using namespace rb;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) return fail(RBGPU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int TPB = 256;
+inline uint32_t nblk(uint32_t n, uint32_t t = TPB) { return (n + t - 1) / t; }
+
+template <class T> int dalloc(T *&p, size_t n) {
+    p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
+    if (e != hipSuccess) return fail(RBGPU_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return RBGPU_OK;
+}
+template <class T> void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Readback block (pinned host memory).
+struct Readback {
+    uint32_t n_sap;
+    uint32_t n_pairs;
+    uint32_t uncoloured;
+    int32_t dirty, overflow, colour_err;
+    uint32_t col_off[MAX_COLOURS + 1];
+    uint32_t bin_off[NBINS + 1];
+    uint64_t points;
+};
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+__global__ void k_offsets(const uint32_t *hoff, uint32_t ntiles, uint32_t nd, const uint32_t *n, uint32_t *out) {
+    uint32_t d = threadIdx.x + blockIdx.x * blockDim.x;
+    if (d < nd) out[d] = hoff[d * ntiles];
+    if (d == 0) out[nd] = *n;
+}
+
+__global__ void k_count_points(const int *cnt, uint32_t n, unsigned long long *acc) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t v = s < n ? (uint32_t)cnt[s] : 0u;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc, (unsigned long long)v);
+}
+
+}  // namespace
+
+struct rbgpu_world {
+    double timestep = 1.0 / 60.0;
+    int substeps = 20;
+    float margin = 0.1f, reinsert = 0.05f;
+    hipStream_t st = nullptr;
+
+    // host-side scene (descriptors) and broadphase bookkeeping for (re)uploads
+    std::vector<rbgpu_body_desc> hb;
+    std::vector<rbgpu_collider_desc> hc;
+    std::vector<float4> h_fmin, h_fmax;
+    std::vector<uint32_t> h_uid, h_key, h_val;
+    std::vector<uint2> h_eppos;
+    bool built = false, sorted_once = false;
+
+    uint32_t N = 0, M = 0, S = 0, NEP = 0;
+    Bodies B{};
+    Colliders C{};
+    Contacts K{};
+    float4 *fmin = nullptr, *fmax = nullptr;
+    uint32_t *uid = nullptr;
+    uint32_t *ep_key[2] = {nullptr, nullptr}, *ep_val[2] = {nullptr, nullptr};
+    uint2 *ep_pos = nullptr;
+    uint32_t *sw_meta = nullptr;
+    float4 *sw_a = nullptr;
+    float2 *sw_b = nullptr;
+    uint32_t *ep_cnt = nullptr, *ep_off = nullptr;
+    uint32_t *ep_hist = nullptr;
+    int *flags = nullptr;            // [0] dirty [1] overflow [2] colour error
+    uint32_t *scal = nullptr;        // [0] n_sap [1] n_pairs [2] uncoloured
+    uint32_t *tile_sums = nullptr;   // scan scratch
+    uint32_t *body_deg = nullptr, *body_off = nullptr, *body_cur = nullptr;
+    uint2 *plane_pairs = nullptr;
+    uint32_t n_plane = 0;
+    unsigned long long *d_points = nullptr;
+
+    // pair stage (capacity pcap)
+    uint32_t pcap = 0;
+    uint2 *pairs = nullptr, *pbod = nullptr;
+    uint32_t *adj = nullptr;
+    int *colour = nullptr;
+    uint32_t *ka = nullptr, *va = nullptr, *kb = nullptr, *slot_pair = nullptr, *bin_slot = nullptr;
+    uint32_t *p_hist = nullptr, *offs = nullptr;
+
+    Readback *rb = nullptr;          // pinned
+    uint32_t n_pairs = 0, n_sap = 0, ncol = 0, jp_rounds = 0;
+    std::vector<uint32_t> col_off, bin_off;
+    uint32_t frames = 0;
+
+    // timing
+    int timing_kernel = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    size_t ev_used = 0;
+    double timing_bytes = 0;
+    bool timing_points_pending = false;
+
+    ~rbgpu_world() { release(); }
+
+    void release_pairs() {
+        dfree(pairs); dfree(pbod); dfree(adj); dfree(colour); dfree(ka); dfree(va); dfree(kb);
+        dfree(slot_pair); dfree(bin_slot); dfree(p_hist);
+        dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
+        dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
+        pcap = 0;
+    }
+    void release() {
+        if (st) (void)hipStreamSynchronize(st);
+        release_pairs();
+        dfree(B.x); dfree(B.q); dfree(B.v); dfree(B.w); dfree(B.px); dfree(B.pq); dfree(B.f); dfree(B.t);
+        dfree(B.I0); dfree(B.I1); dfree(B.Ii0); dfree(B.Ii1);
+        dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat);
+        dfree(fmin); dfree(fmax); dfree(uid); dfree(ep_key[0]); dfree(ep_key[1]); dfree(ep_val[0]); dfree(ep_val[1]);
+        dfree(ep_pos); dfree(sw_meta); dfree(sw_a); dfree(sw_b); dfree(ep_cnt); dfree(ep_off); dfree(ep_hist);
+        dfree(flags); dfree(scal); dfree(tile_sums); dfree(body_deg); dfree(body_off); dfree(body_cur);
+        dfree(plane_pairs); dfree(offs); dfree(d_points);
+        for (auto &e : ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        ev_pool.clear();
+        if (rb) (void)hipHostFree(rb);
+        rb = nullptr;
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+        built = false;
+    }
+
+    // ------------------------------------------------------------ primitives
+    int scan(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total) {
+        uint32_t nt = std::max<uint32_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+        if (nt > (uint32_t)(SCAN_TOP_THREADS * SCAN_TOP_ITEMS)) return fail(RBGPU_ERR_CAPACITY, "scan too large");
+        k_scan_reduce<<<nt, SCAN_THREADS, 0, st>>>(in, nullptr, n, tile_sums);
+        k_scan_top<<<1, SCAN_TOP_THREADS, 0, st>>>(tile_sums, nt, total);
+        k_scan_apply<<<nt, SCAN_THREADS, 0, st>>>(in, nullptr, n, tile_sums, out);
+        return RBGPU_OK;
+    }
+    // One stable 8-bit radix pass (k0,v0) -> (k1,v1) on bits [shift, shift+8).
+    int radix_pass(const uint32_t *k0, const uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, int shift,
+                   uint32_t *hist, const int *skip) {
+        uint32_t nt = std::max<uint32_t>(1, (n + RS_TILE - 1) / RS_TILE);
+        k_rs_hist<<<nt, RS_THREADS, 0, st>>>(k0, nullptr, n, shift, nt, hist, skip);
+        int rc = scan(hist, hist, nt * RS_BINS, nullptr);
+        if (rc) return rc;
+        k_rs_scatter<<<nt, RS_THREADS, 0, st>>>(k0, v0, k1, v1, nullptr, n, shift, nt, hist, skip);
+        return RBGPU_OK;
+    }
+
+    hipEvent_t *timed_begin(int kernel) {
+        if (timing_kernel != kernel) return nullptr;
+        if (ev_used == ev_pool.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
+            ev_pool.push_back({a, b});
+        }
+        auto &e = ev_pool[ev_used++];
+        (void)hipEventRecord(e.first, st);
+        return &e.second;
+    }
+    void timed_end(hipEvent_t *e) {
+        if (e) (void)hipEventRecord(*e, st);
+    }
+
+    // ------------------------------------------------------------ upload
+    int build();
+    int download_state();
+    int ensure_pair_capacity(uint32_t need);
+    int frame();
+};
+
+// ----------------------------------------------------------------- build / upload
+int rbgpu_world::ensure_pair_capacity(uint32_t need) {
+    if (need <= pcap && pairs) return RBGPU_OK;
+    HIPCHK(hipStreamSynchronize(st));
+    release_pairs();
+    uint32_t cap = std::max<uint32_t>(need + need / 2, 4096);
+    int rc = 0;
+    rc |= dalloc(pairs, cap); rc |= dalloc(pbod, cap); rc |= dalloc(adj, 2 * (size_t)cap); rc |= dalloc(colour, cap);
+    rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(slot_pair, cap);
+    rc |= dalloc(bin_slot, cap);
+    uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
+    rc |= dalloc(p_hist, (size_t)nt * RS_BINS);
+    rc |= dalloc(K.cnt, cap); rc |= dalloc(K.cc, cap); rc |= dalloc(K.bb, cap); rc |= dalloc(K.nr, cap);
+    rc |= dalloc(K.fr, cap);
+    rc |= dalloc(K.ra, 4 * (size_t)cap); rc |= dalloc(K.rb, 4 * (size_t)cap); rc |= dalloc(K.pa, 4 * (size_t)cap);
+    rc |= dalloc(K.pb, 4 * (size_t)cap); rc |= dalloc(K.lam, 4 * (size_t)cap);
+    if (rc) return RBGPU_ERR_OOM;
+    HIPCHK(hipMemsetAsync(K.cnt, 0, cap * sizeof(int), st));
+    pcap = cap;
+    return RBGPU_OK;
+}
+
+int rbgpu_world::build() {
+    // Keep broadphase history; bodies come from the descriptors (or the downloaded state).
+    const uint32_t n = (uint32_t)hb.size(), m = (uint32_t)hc.size();
+    std::vector<float4> x(n), q(n), v(n), w(n), f(n), t(n), I0(n), I1(n), Ii0(n), Ii1(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const rbgpu_body_desc &d = hb[i];
+        const bool dyn = d.type != RBGPU_STATIC;
+        const float mass = (float)d.mass;
+        const float im = dyn ? 1.0f / mass : 0.0f;
+        SM I{(float)d.inertia[0], (float)d.inertia[1], (float)d.inertia[2], (float)d.inertia[3], (float)d.inertia[4],
+             (float)d.inertia[5]};
+        SM Ii = dyn ? sinverse(I) : SM{0, 0, 0, 0, 0, 0};
+        x[i] = make_float4((float)d.position[0], (float)d.position[1], (float)d.position[2], im);
+        q[i] = make_float4((float)d.orientation[0], (float)d.orientation[1], (float)d.orientation[2], (float)d.orientation[3]);
+        v[i] = make_float4((float)d.linear_velocity[0], (float)d.linear_velocity[1], (float)d.linear_velocity[2], 0.0f);
+        w[i] = make_float4((float)d.angular_velocity[0], (float)d.angular_velocity[1], (float)d.angular_velocity[2], 0.0f);
+        f[i] = make_float4(0.0f, -9.8f * mass, 0.0f, mass);   // BodyAPI.hpp:61
+        t[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        I0[i] = make_float4(I.xx, I.yy, I.zz, I.xy);
+        I1[i] = make_float4(I.xz, I.yz, 0.0f, 0.0f);
+        Ii0[i] = make_float4(Ii.xx, Ii.yy, Ii.zz, Ii.xy);
+        Ii1[i] = make_float4(Ii.xz, Ii.yz, dyn ? 1.0f : 0.0f, 0.0f);
+    }
+    std::vector<int4> meta(m);
+    std::vector<float4> dims(m), lpos(m), lrot(m), mat(m);
+    std::vector<uint2> planes;
+    uint32_t slot = 0;
+    for (uint32_t c = 0; c < m; ++c) {
+        const rbgpu_collider_desc &d = hc[c];
+        const bool plane = d.shape == RBGPU_PLANE;
+        meta[c] = make_int4(d.shape, (int)d.body_id, d.is_static ? 1 : 0, plane ? (int)BP_INVALID : (int)slot);
+        if (!plane) ++slot;
+        dims[c] = make_float4((float)d.dims[0], (float)d.dims[1], (float)d.dims[2], (float)d.dims[3]);
+        lpos[c] = make_float4((float)d.local_pos[0], (float)d.local_pos[1], (float)d.local_pos[2], 0.0f);
+        lrot[c] = make_float4((float)d.local_rot[0], (float)d.local_rot[1], (float)d.local_rot[2], (float)d.local_rot[3]);
+        mat[c] = make_float4((float)d.restitution, (float)d.static_friction, (float)d.dynamic_friction, 0.0f);
+    }
+    // Plane loop of the narrowphase driver (CollisionPipeline.cpp:156-197): constant per scene.
+    for (uint32_t pi = 0; pi < m; ++pi) {
+        if (hc[pi].shape != RBGPU_PLANE) continue;
+        for (uint32_t di = 0; di < m; ++di) {
+            if (di == pi || hc[di].shape == RBGPU_PLANE) continue;
+            if (hc[pi].is_static && hc[di].is_static) continue;
+            if (hc[pi].body_id == hc[di].body_id) continue;
+            planes.push_back(make_uint2(di, pi));
+        }
+    }
+    if (slot != S) return fail(RBGPU_ERR_INVALID, "broadphase slot bookkeeping mismatch");
+
+    release();
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void **)&rb, sizeof(Readback), hipHostMallocDefault));
+    N = n; M = m; NEP = 2 * S;
+    int rc = 0;
+    rc |= dalloc(B.x, n); rc |= dalloc(B.q, n); rc |= dalloc(B.v, n); rc |= dalloc(B.w, n); rc |= dalloc(B.px, n);
+    rc |= dalloc(B.pq, n); rc |= dalloc(B.f, n); rc |= dalloc(B.t, n); rc |= dalloc(B.I0, n); rc |= dalloc(B.I1, n);
+    rc |= dalloc(B.Ii0, n); rc |= dalloc(B.Ii1, n);
+    rc |= dalloc(C.meta, m); rc |= dalloc(C.dims, m); rc |= dalloc(C.lpos, m); rc |= dalloc(C.lrot, m); rc |= dalloc(C.mat, m);
+    rc |= dalloc(fmin, S); rc |= dalloc(fmax, S); rc |= dalloc(uid, S);
+    rc |= dalloc(ep_key[0], NEP); rc |= dalloc(ep_key[1], NEP); rc |= dalloc(ep_val[0], NEP); rc |= dalloc(ep_val[1], NEP);
+    rc |= dalloc(ep_pos, S); rc |= dalloc(sw_meta, NEP); rc |= dalloc(sw_a, NEP); rc |= dalloc(sw_b, NEP);
+    rc |= dalloc(ep_cnt, NEP + 1); rc |= dalloc(ep_off, NEP + 1);
+    rc |= dalloc(ep_hist, (size_t)((NEP + RS_TILE - 1) / RS_TILE) * RS_BINS);
+    rc |= dalloc(flags, 4); rc |= dalloc(scal, 4);
+    rc |= dalloc(tile_sums, SCAN_TOP_THREADS * SCAN_TOP_ITEMS);
+    rc |= dalloc(body_deg, n + 1); rc |= dalloc(body_off, n + 1); rc |= dalloc(body_cur, n + 1);
+    rc |= dalloc(plane_pairs, planes.size()); rc |= dalloc(offs, MAX_COLOURS + NBINS + 4);
+    rc |= dalloc(d_points, 1);
+    if (rc) return RBGPU_ERR_OOM;
+    auto up = [&](void *dst, const void *src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st); };
+    if (n) {
+        HIPCHK(up(B.x, x.data(), n * 16)); HIPCHK(up(B.q, q.data(), n * 16)); HIPCHK(up(B.v, v.data(), n * 16));
+        HIPCHK(up(B.w, w.data(), n * 16)); HIPCHK(up(B.px, x.data(), n * 16)); HIPCHK(up(B.pq, q.data(), n * 16));
+        HIPCHK(up(B.f, f.data(), n * 16)); HIPCHK(up(B.t, t.data(), n * 16)); HIPCHK(up(B.I0, I0.data(), n * 16));
+        HIPCHK(up(B.I1, I1.data(), n * 16)); HIPCHK(up(B.Ii0, Ii0.data(), n * 16)); HIPCHK(up(B.Ii1, Ii1.data(), n * 16));
+    }
+    if (m) {
+        HIPCHK(up(C.meta, meta.data(), m * 16)); HIPCHK(up(C.dims, dims.data(), m * 16));
+        HIPCHK(up(C.lpos, lpos.data(), m * 16)); HIPCHK(up(C.lrot, lrot.data(), m * 16)); HIPCHK(up(C.mat, mat.data(), m * 16));
+    }
+    if (S) {
+        HIPCHK(up(fmin, h_fmin.data(), S * 16)); HIPCHK(up(fmax, h_fmax.data(), S * 16)); HIPCHK(up(uid, h_uid.data(), S * 4));
+        HIPCHK(up(ep_key[0], h_key.data(), NEP * 4)); HIPCHK(up(ep_val[0], h_val.data(), NEP * 4));
+        HIPCHK(up(ep_pos, h_eppos.data(), S * 8));
+    }
+    n_plane = (uint32_t)planes.size();
+    if (n_plane) HIPCHK(up(plane_pairs, planes.data(), n_plane * 8));
+    int init_flags[4] = {1, 0, 0, 0};   // dirty: the endpoint array has never been sorted
+    HIPCHK(up(flags, init_flags, sizeof(init_flags)));
+    HIPCHK(hipMemsetAsync(scal, 0, 16, st));
+    HIPCHK(hipMemsetAsync(d_points, 0, 8, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int rc2 = ensure_pair_capacity(std::max<uint32_t>(8 * S + n_plane, 4096));
+    if (rc2) return rc2;
+    built = true;
+    return RBGPU_OK;
+}
+
+// Pull the evolving state back into the host descriptors and broadphase mirrors so that
+// a later build() (after bodies or colliders were added) continues from it.
+int rbgpu_world::download_state() {
+    if (!built) return RBGPU_OK;
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<float4> x(N), q(N), v(N), w(N);
+    if (N) {
+        HIPCHK(hipMemcpy(x.data(), B.x, N * 16, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(q.data(), B.q, N * 16, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(v.data(), B.v, N * 16, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(w.data(), B.w, N * 16, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < N; ++i) {
+        rbgpu_body_desc &d = hb[i];
+        d.position[0] = x[i].x; d.position[1] = x[i].y; d.position[2] = x[i].z;
+        d.orientation[0] = q[i].x; d.orientation[1] = q[i].y; d.orientation[2] = q[i].z; d.orientation[3] = q[i].w;
+        d.linear_velocity[0] = v[i].x; d.linear_velocity[1] = v[i].y; d.linear_velocity[2] = v[i].z;
+        d.angular_velocity[0] = w[i].x; d.angular_velocity[1] = w[i].y; d.angular_velocity[2] = w[i].z;
+    }
+    if (S) {
+        HIPCHK(hipMemcpy(h_fmin.data(), fmin, S * 16, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h_fmax.data(), fmax, S * 16, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h_key.data(), ep_key[0], NEP * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h_val.data(), ep_val[0], NEP * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h_eppos.data(), ep_pos, S * 8, hipMemcpyDeviceToHost));
+    }
+    // a pending dirty flag is re-established by build()
+    return RBGPU_OK;
+}
+
+// ----------------------------------------------------------------- one frame
+int rbgpu_world::frame() {
+    const float dt = (float)timestep;
+    const float h = dt / (float)substeps;
+    const float inv_h = 1.0f / h;
+    int *dirty = flags, *overflow = flags + 1, *colerr = flags + 2;
+
+    // 1. refit (CollisionPipeline.cpp:14-53)
+    if (M) {
+        hipEvent_t *e = timed_begin(6);
+        k_refit<<<nblk(M), TPB, 0, st>>>(C, B, M, dt, margin, reinsert, fmin, fmax, ep_key[0], ep_pos, dirty);
+        timed_end(e);
+    }
+    // 2. stable sort of the endpoints if any fat box changed (BroadPhase.cpp:234-245)
+    if (NEP) {
+        for (int pass = 0; pass < 4; ++pass) {
+            int s = pass & 1;
+            int rc = radix_pass(ep_key[s], ep_val[s], ep_key[s ^ 1], ep_val[s ^ 1], NEP, 8 * pass, ep_hist, dirty);
+            if (rc) return rc;
+        }
+        k_sweep_prep<<<nblk(NEP), TPB, 0, st>>>(ep_val[0], NEP, fmin, fmax, ep_pos, sw_meta, sw_a, sw_b, dirty);
+        // 3. sweep: count, scan, readback of the pair count
+        k_sweep<false><<<nblk(NEP), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, NEP, ep_cnt, nullptr, nullptr, 0,
+                                                  overflow);
+        int rc = scan(ep_cnt, ep_off, NEP, scal + 0);
+        if (rc) return rc;
+        k_set_int<<<1, 1, 0, st>>>(dirty, 0);
+    } else {
+        HIPCHK(hipMemsetAsync(scal, 0, 4, st));
+    }
+    HIPCHK(hipMemcpyAsync(&rb->n_sap, scal, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    n_sap = rb->n_sap;
+    n_pairs = n_sap + n_plane;
+    if (n_pairs > pcap) {
+        int rc = ensure_pair_capacity(n_pairs);
+        if (rc) return rc;
+    }
+    const uint32_t P = n_pairs;
+    if (NEP)
+        k_sweep<true><<<nblk(NEP), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, NEP, nullptr, ep_off, pairs, pcap,
+                                                 overflow);
+    k_append<<<nblk(std::max<uint32_t>(n_plane, 1)), TPB, 0, st>>>(pairs, scal + 0, plane_pairs, n_plane, pcap, scal + 1,
+                                                                    overflow);
+
+    // 4. Jones-Plassmann colouring of the pair graph (edges = shared dynamic bodies)
+    ncol = 0;
+    jp_rounds = 0;
+    if (P) {
+        k_fill_u32<<<nblk(N + 1), TPB, 0, st>>>(body_deg, N + 1, 0u);
+        k_fill_u32<<<nblk(N + 1), TPB, 0, st>>>(body_cur, N + 1, 0u);
+        k_pair_bodies<<<nblk(P), TPB, 0, st>>>(pairs, scal + 1, C, B, pbod, body_deg, colour, scal + 2);
+        int rc = scan(body_deg, body_off, N + 1, nullptr);
+        if (rc) return rc;
+        k_fill_adj<<<nblk(P), TPB, 0, st>>>(pbod, scal + 1, B, body_off, body_cur, adj);
+        for (;;) {
+            for (int r = 0; r < 8; ++r)
+                k_jp_round<<<nblk(P), TPB, 0, st>>>(pbod, scal + 1, B, body_off, body_deg, adj, colour, scal + 2, colerr);
+            jp_rounds += 8;
+            HIPCHK(hipMemcpyAsync(&rb->uncoloured, scal + 2, 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (rb->uncoloured == 0) break;
+            if (jp_rounds > 4096) return fail(RBGPU_ERR_CAPACITY, "colouring did not converge");
+        }
+        // 5. solve slots = pairs stably sorted by colour; narrowphase bins = slots sorted by pair type
+        k_colour_keys<<<nblk(P), TPB, 0, st>>>(colour, scal + 1, ka, va);
+        rc = radix_pass(ka, va, kb, slot_pair, P, 0, p_hist, nullptr);
+        if (rc) return rc;
+        uint32_t nt = (P + RS_TILE - 1) / RS_TILE;
+        k_offsets<<<1, MAX_COLOURS, 0, st>>>(p_hist, nt, MAX_COLOURS, scal + 1, offs);
+        k_slot_gather<<<nblk(P), TPB, 0, st>>>(slot_pair, scal + 1, pairs, pbod, C, K, ka, va);
+        rc = radix_pass(ka, va, kb, bin_slot, P, 0, p_hist, nullptr);
+        if (rc) return rc;
+        k_offsets<<<1, 64, 0, st>>>(p_hist, nt, NBINS, scal + 1, offs + MAX_COLOURS + 1);
+        HIPCHK(hipMemcpyAsync(rb->col_off, offs, (MAX_COLOURS + 1) * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(rb->bin_off, offs + MAX_COLOURS + 1, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&rb->overflow, overflow, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (rb->overflow) return fail(RBGPU_ERR_CAPACITY, "pair buffer overflow");
+        if (rb->colour_err) return fail(RBGPU_ERR_CAPACITY, "more than 256 colours needed");
+        col_off.assign(rb->col_off, rb->col_off + MAX_COLOURS + 1);
+        bin_off.assign(rb->bin_off, rb->bin_off + NBINS + 1);
+        for (int c = 0; c < MAX_COLOURS; ++c)
+            if (col_off[c + 1] > col_off[c]) ncol = c + 1;
+    }
+
+    // 6. substeps (World.hpp:203-211)
+    for (int s = 0; s < substeps; ++s) {
+        if (P) {
+            hipEvent_t *e = timed_begin(3);
+            for (int b = 0; b < NBINS; ++b) {
+                uint32_t o = bin_off[b], n = bin_off[b + 1] - bin_off[b];
+                if (!n) continue;
+                const uint32_t g = nblk(n, 128);
+                switch (b) {
+                    case BIN_SS: k_narrow<BIN_SS><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_SB: k_narrow<BIN_SB><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_BB: k_narrow<BIN_BB><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_SP: k_narrow<BIN_SP><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                }
+            }
+            timed_end(e);
+        }
+        if (N) {
+            hipEvent_t *e = timed_begin(4);
+            k_integrate<<<nblk(N), TPB, 0, st>>>(B, N, h);
+            timed_end(e);
+        }
+        for (uint32_t c = 0; c < ncol; ++c) {
+            uint32_t o = col_off[c], n = col_off[c + 1] - col_off[c];
+            if (!n) continue;
+            hipEvent_t *e = timed_begin(1);
+            k_solve_pos<<<nblk(n), TPB, 0, st>>>(B, K, o, n, inv_h);
+            timed_end(e);
+        }
+        if (N) {
+            hipEvent_t *e = timed_begin(5);
+            k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h);
+            timed_end(e);
+        }
+        for (uint32_t c = 0; c < ncol; ++c) {
+            uint32_t o = col_off[c], n = col_off[c + 1] - col_off[c];
+            if (!n) continue;
+            hipEvent_t *e = timed_begin(2);
+            k_solve_vel<<<nblk(n), TPB, 0, st>>>(B, K, o, n, h);
+            timed_end(e);
+        }
+        if (timing_kernel == 1 || timing_kernel == 2) {
+            // algorithmic bytes need the substep's contact count: accumulate on the device
+            if (P) k_count_points<<<nblk(P), TPB, 0, st>>>(K.cnt, P, d_points);
+        }
+    }
+    ++frames;
+    return RBGPU_OK;
+}
+
+// ----------------------------------------------------------------- C-ABI
+extern "C" {
+
+int rbgpu_world_create(double timestep, int substeps, rbgpu_world **out) {
+    if (!out || substeps <= 0 || !(timestep > 0.0)) return fail(RBGPU_ERR_INVALID, "invalid timestep/substeps");
+    auto *w = new rbgpu_world();
+    w->timestep = timestep;
+    w->substeps = substeps;
+    *out = w;
+    return RBGPU_OK;
+}
+
+int rbgpu_world_destroy(rbgpu_world *w) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    delete w;
+    return RBGPU_OK;
+}
+
+int rbgpu_add_bodies(rbgpu_world *w, int n, const rbgpu_body_desc *desc, uint32_t *ids_out) {
+    if (!w || n < 0 || (n && !desc)) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (w->hb.size() + (size_t)n >= (1u << 24)) return fail(RBGPU_ERR_CAPACITY, "more than 2^24 bodies");
+    int rc = w->download_state();
+    if (rc) return rc;
+    for (int i = 0; i < n; ++i) {
+        if (desc[i].type == RBGPU_DYNAMIC && !(desc[i].mass > 0.0)) return fail(RBGPU_ERR_INVALID, "dynamic body needs mass > 0");
+        if (ids_out) ids_out[i] = (uint32_t)w->hb.size();
+        w->hb.push_back(desc[i]);
+    }
+    w->built = false;
+    return RBGPU_OK;
+}
+
+int rbgpu_add_colliders(rbgpu_world *w, int n, const rbgpu_collider_desc *desc, uint32_t *ids_out) {
+    if (!w || n < 0 || (n && !desc)) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    int rc = w->download_state();
+    if (rc) return rc;
+    for (int i = 0; i < n; ++i) {
+        const rbgpu_collider_desc &d = desc[i];
+        if (d.body_id >= w->hb.size()) return fail(RBGPU_ERR_INVALID, "collider body_id out of range");
+        if (d.shape != RBGPU_SPHERE && d.shape != RBGPU_BOX && d.shape != RBGPU_CAPSULE && d.shape != RBGPU_PLANE)
+            return fail(RBGPU_ERR_UNSUPPORTED, "shape kind not supported by this build");
+        const uint32_t id = (uint32_t)w->hc.size();
+        w->hc.push_back(d);
+        if (ids_out) ids_out[i] = id;
+        if (d.shape == RBGPU_PLANE) continue;   // planes bypass the broadphase (ColliderAPI.hpp:29-33)
+        // initial tight AABB at the body's current pose, fattened by the margin (BroadPhase.cpp:169-197)
+        const rbgpu_body_desc &b = w->hb[d.body_id];
+        TF bt{V3{(float)b.position[0], (float)b.position[1], (float)b.position[2]},
+              Q{(float)b.orientation[0], (float)b.orientation[1], (float)b.orientation[2], (float)b.orientation[3]}};
+        V3 lp{(float)d.local_pos[0], (float)d.local_pos[1], (float)d.local_pos[2]};
+        Q lr{(float)d.local_rot[0], (float)d.local_rot[1], (float)d.local_rot[2], (float)d.local_rot[3]};
+        TF tf{bt.pos + rotate(bt.rot, lp), qmul(bt.rot, lr)};
+        Shape s{d.shape, (float)d.dims[0], (float)d.dims[1], (float)d.dims[2], (float)d.dims[3]};
+        Box3 t = shape_aabb(s, tf);
+        const float mg = w->margin;
+        V3 mn = t.mn - V3{mg, mg, mg}, mx = t.mx + V3{mg, mg, mg};
+        const uint32_t slot = w->S++;
+        w->h_fmin.push_back(make_float4(mn.x, mn.y, mn.z, 0.0f));
+        w->h_fmax.push_back(make_float4(mx.x, mx.y, mx.z, 0.0f));
+        w->h_uid.push_back(id);
+        w->h_eppos.push_back(make_uint2((uint32_t)w->h_key.size(), (uint32_t)w->h_key.size() + 1));
+        w->h_key.push_back(float_key(mn.x));
+        w->h_val.push_back(slot << 1);
+        w->h_key.push_back(float_key(mx.x));
+        w->h_val.push_back((slot << 1) | 1u);
+    }
+    w->built = false;
+    return RBGPU_OK;
+}
+
+int rbgpu_step(rbgpu_world *w, int n_frames) {
+    if (!w || n_frames < 0) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (!w->built) {
+        int rc = w->build();
+        if (rc) return rc;
+    }
+    for (int f = 0; f < n_frames; ++f) {
+        int rc = w->frame();
+        if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    return RBGPU_OK;
+}
+
+int rbgpu_synchronize(rbgpu_world *w) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (w->st) HIPCHK(hipStreamSynchronize(w->st));
+    return RBGPU_OK;
+}
+
+int rbgpu_get_bodies(rbgpu_world *w, double *p, double *q, double *v, double *av) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    const uint32_t n = (uint32_t)w->hb.size();
+    if (!w->built) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const rbgpu_body_desc &d = w->hb[i];
+            for (int k = 0; k < 3; ++k) {
+                if (p) p[3 * i + k] = (float)d.position[k];
+                if (v) v[3 * i + k] = (float)d.linear_velocity[k];
+                if (av) av[3 * i + k] = (float)d.angular_velocity[k];
+            }
+            for (int k = 0; k < 4; ++k)
+                if (q) q[4 * i + k] = (float)d.orientation[k];
+        }
+        return RBGPU_OK;
+    }
+    HIPCHK(hipStreamSynchronize(w->st));
+    std::vector<float4> buf(n);
+    auto pull = [&](float4 *src, double *dst, bool quat) -> int {
+        if (!dst || !n) return RBGPU_OK;
+        HIPCHK(hipMemcpy(buf.data(), src, n * 16, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (quat) { dst[4 * i] = buf[i].x; dst[4 * i + 1] = buf[i].y; dst[4 * i + 2] = buf[i].z; dst[4 * i + 3] = buf[i].w; }
+            else { dst[3 * i] = buf[i].x; dst[3 * i + 1] = buf[i].y; dst[3 * i + 2] = buf[i].z; }
+        }
+        return RBGPU_OK;
+    };
+    int rc = pull(w->B.x, p, false);
+    if (!rc) rc = pull(w->B.q, q, true);
+    if (!rc) rc = pull(w->B.v, v, false);
+    if (!rc) rc = pull(w->B.w, av, false);
+    return rc;
+}
+
+int rbgpu_set_bodies(rbgpu_world *w, const double *p, const double *q, const double *v, const double *av) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    int rc = w->download_state();
+    if (rc) return rc;
+    for (size_t i = 0; i < w->hb.size(); ++i) {
+        rbgpu_body_desc &d = w->hb[i];
+        for (int k = 0; k < 3; ++k) {
+            if (p) d.position[k] = p[3 * i + k];
+            if (v) d.linear_velocity[k] = v[3 * i + k];
+            if (av) d.angular_velocity[k] = av[3 * i + k];
+        }
+        for (int k = 0; k < 4; ++k)
+            if (q) d.orientation[k] = q[4 * i + k];
+    }
+    w->built = false;
+    return RBGPU_OK;
+}
+
+int rbgpu_get_sap_pairs(rbgpu_world *w, uint32_t *pairs2, uint32_t cap, uint32_t *n) {
+    if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    *n = w->built ? w->n_sap : 0;
+    if (!w->built || !pairs2) return RBGPU_OK;
+    HIPCHK(hipStreamSynchronize(w->st));
+    uint32_t k = std::min(cap, w->n_sap);
+    if (k) HIPCHK(hipMemcpy(pairs2, w->pairs, (size_t)k * 8, hipMemcpyDeviceToHost));
+    return RBGPU_OK;
+}
+
+int rbgpu_get_pair_colours(rbgpu_world *w, uint32_t *pairs2, int32_t *colour, uint32_t cap, uint32_t *n) {
+    if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    *n = w->built ? w->n_pairs : 0;
+    if (!w->built) return RBGPU_OK;
+    HIPCHK(hipStreamSynchronize(w->st));
+    uint32_t k = std::min(cap, w->n_pairs);
+    if (k && pairs2) HIPCHK(hipMemcpy(pairs2, w->pairs, (size_t)k * 8, hipMemcpyDeviceToHost));
+    if (k && colour) HIPCHK(hipMemcpy(colour, w->colour, (size_t)k * 4, hipMemcpyDeviceToHost));
+    return RBGPU_OK;
+}
+
+int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *n3, double *ra3, double *rb3, double *dep, double *lam2,
+                       uint32_t cap, uint32_t *n) {
+    if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    *n = 0;
+    if (!w->built || !w->n_pairs) return RBGPU_OK;
+    HIPCHK(hipStreamSynchronize(w->st));
+    const uint32_t P = w->n_pairs;
+    std::vector<uint32_t> sp(P);
+    std::vector<int> cnt(P);
+    std::vector<uint2> cc(P), bb(P);
+    std::vector<float4> nr(P), ra(4 * (size_t)P), rbv(4 * (size_t)P);
+    std::vector<float2> lam(4 * (size_t)P);
+    HIPCHK(hipMemcpy(sp.data(), w->slot_pair, P * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), w->K.cnt, P * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cc.data(), w->K.cc, P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bb.data(), w->K.bb, P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nr.data(), w->K.nr, P * 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ra.data(), w->K.ra, (size_t)P * 64, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rbv.data(), w->K.rb, (size_t)P * 64, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lam.data(), w->K.lam, (size_t)P * 32, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> slot_of(P);
+    for (uint32_t s = 0; s < P; ++s) slot_of[sp[s]] = s;
+    uint32_t k = 0;
+    for (uint32_t p = 0; p < P; ++p) {
+        uint32_t s = slot_of[p];
+        for (int i = 0; i < cnt[s]; ++i, ++k) {
+            if (k >= cap) continue;
+            size_t j = (size_t)s * 4 + i;
+            if (ids4) { ids4[4 * k] = bb[s].x; ids4[4 * k + 1] = bb[s].y; ids4[4 * k + 2] = cc[s].x; ids4[4 * k + 3] = cc[s].y; }
+            if (n3) { n3[3 * k] = nr[s].x; n3[3 * k + 1] = nr[s].y; n3[3 * k + 2] = nr[s].z; }
+            if (ra3) { ra3[3 * k] = ra[j].x; ra3[3 * k + 1] = ra[j].y; ra3[3 * k + 2] = ra[j].z; }
+            if (rb3) { rb3[3 * k] = rbv[j].x; rb3[3 * k + 1] = rbv[j].y; rb3[3 * k + 2] = rbv[j].z; }
+            if (dep) dep[k] = rbv[j].w;
+            if (lam2) { lam2[2 * k] = lam[j].x; lam2[2 * k + 1] = lam[j].y; }
+        }
+    }
+    *n = k;
+    return RBGPU_OK;
+}
+
+int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out) {
+    if (!w || !out) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    std::memset(out, 0, sizeof(*out));
+    out->frames = w->frames;
+    out->sap_pairs = w->n_sap;
+    out->pairs = w->n_pairs;
+    out->colours = w->ncol;
+    out->jp_rounds = w->jp_rounds;
+    for (int b = 0; b < NBINS && (size_t)b + 1 < w->bin_off.size(); ++b) out->bins[b] = w->bin_off[b + 1] - w->bin_off[b];
+    if (w->built && w->n_pairs) {
+        HIPCHK(hipStreamSynchronize(w->st));
+        std::vector<int> cnt(w->n_pairs);
+        HIPCHK(hipMemcpy(cnt.data(), w->K.cnt, w->n_pairs * 4, hipMemcpyDeviceToHost));
+        uint32_t s = 0;
+        for (int c : cnt) s += (uint32_t)c;
+        out->contacts = s;
+    }
+    return RBGPU_OK;
+}
+
+int rbgpu_set_timing(rbgpu_world *w, int kernel) {
+    if (!w || kernel < 0 || kernel > 6) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (w->st) HIPCHK(hipStreamSynchronize(w->st));
+    w->timing_kernel = kernel;
+    w->ev_used = 0;
+    if (w->d_points) HIPCHK(hipMemset(w->d_points, 0, 8));
+    return RBGPU_OK;
+}
+
+int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, double *bytes) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (w->st) HIPCHK(hipStreamSynchronize(w->st));
+    double ms = 0.0;
+    for (size_t i = 0; i < w->ev_used; ++i) {
+        float e = 0.0f;
+        HIPCHK(hipEventElapsedTime(&e, w->ev_pool[i].first, w->ev_pool[i].second));
+        ms += e;
+    }
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = w->ev_used;
+    if (bytes) {
+        unsigned long long pts = 0;
+        if (w->d_points) HIPCHK(hipMemcpy(&pts, w->d_points, 8, hipMemcpyDeviceToHost));
+        // algorithmic bytes per contact point: position solve 296 B, velocity solve 280 B
+        *bytes = (double)pts * (w->timing_kernel == 1 ? 296.0 : w->timing_kernel == 2 ? 280.0 : 0.0);
+    }
+    return RBGPU_OK;
+}
+
+const char *rbgpu_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+"""Python mirror of ``rbps::World`` (include/rbps/API/World.hpp:87-212) over the
+rbgpu C-ABI (include/rbgpu.h).
+
+Same names and argument meaning as the reference API: ``World(timestep,
+substeps)``, ``create_body``, ``create_collider``, ``step``; bodies are read back
+as the reference's ``BodyCollection`` fields.  All computation runs in the HIP
+library; if ``librbgpu.so`` is missing this module raises instead of falling
+back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import scenes
+from .scenes import BODY_DTYPE, COLLIDER_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librbgpu.so")
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_U = C.POINTER(C.c_uint32)
+_I = C.POINTER(C.c_int32)
+
+EXPORTS = [
+    "rbgpu_world_create", "rbgpu_world_destroy", "rbgpu_add_bodies", "rbgpu_add_colliders", "rbgpu_step",
+    "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
+    "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_last_error",
+]
+
+
+class RbgpuError(RuntimeError):
+    pass
+
+
+class Stats(C.Structure):
+    _fields_ = [("frames", C.c_uint32), ("sap_pairs", C.c_uint32), ("pairs", C.c_uint32), ("colours", C.c_uint32),
+                ("jp_rounds", C.c_uint32), ("contacts", C.c_uint32), ("bins", C.c_uint32 * 10)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads librbgpu.so (built by ``python rbps_amd/build.py``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RbgpuError(f"{LIB_PATH} is not built; run `python rbps_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        L.rbgpu_world_create.argtypes = [C.c_double, C.c_int, C.POINTER(_P)]
+        L.rbgpu_world_destroy.argtypes = [_P]
+        L.rbgpu_add_bodies.argtypes = [_P, C.c_int, _P, _U]
+        L.rbgpu_add_colliders.argtypes = [_P, C.c_int, _P, _U]
+        L.rbgpu_step.argtypes = [_P, C.c_int]
+        L.rbgpu_synchronize.argtypes = [_P]
+        L.rbgpu_get_bodies.argtypes = [_P, _D, _D, _D, _D]
+        L.rbgpu_set_bodies.argtypes = [_P, _D, _D, _D, _D]
+        L.rbgpu_get_sap_pairs.argtypes = [_P, _U, C.c_uint32, _U]
+        L.rbgpu_get_pair_colours.argtypes = [_P, _U, _I, C.c_uint32, _U]
+        L.rbgpu_get_contacts.argtypes = [_P, _U, _D, _D, _D, _D, _D, C.c_uint32, _U]
+        L.rbgpu_get_stats.argtypes = [_P, C.POINTER(Stats)]
+        L.rbgpu_set_timing.argtypes = [_P, C.c_int]
+        L.rbgpu_get_timing.argtypes = [_P, _D, C.POINTER(C.c_uint64), _D]
+        L.rbgpu_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise RbgpuError(f"rbgpu error {rc}: {lib().rbgpu_last_error().decode()}")
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "sweep": 6}
+
+
+class World:
+    """A world on the GPU: ``World(timestep, substeps)`` as in World.hpp:89-106."""
+
+    def __init__(self, timestep: float = 1.0 / 60.0, substeps: int = 20):
+        self.timestep, self.substeps = timestep, substeps
+        h = _P()
+        _check(lib().rbgpu_world_create(timestep, substeps, C.byref(h)))
+        self.h = h
+        self.n_bodies = 0
+        self.n_colliders = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rbgpu_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- creation (BodyAPI / ColliderAPI)
+    def add_bodies(self, desc: np.ndarray) -> np.ndarray:
+        desc = np.ascontiguousarray(desc, dtype=BODY_DTYPE)
+        ids = np.zeros(len(desc), np.uint32)
+        _check(lib().rbgpu_add_bodies(self.h, len(desc), desc.ctypes.data_as(_P), _ptr(ids, _U)))
+        self.n_bodies += len(desc)
+        return ids
+
+    def add_colliders(self, desc: np.ndarray) -> np.ndarray:
+        desc = np.ascontiguousarray(desc, dtype=COLLIDER_DTYPE)
+        ids = np.zeros(len(desc), np.uint32)
+        _check(lib().rbgpu_add_colliders(self.h, len(desc), desc.ctypes.data_as(_P), _ptr(ids, _U)))
+        self.n_colliders += len(desc)
+        return ids
+
+    def create_body(self, type=scenes.DYNAMIC, mass=1.0, inertia_tensor=(1.0, 1.0, 1.0, 0.0, 0.0, 0.0),
+                    position=(0, 0, 0), orientation=(1, 0, 0, 0), linear_velocity=(0, 0, 0),
+                    angular_velocity=(0, 0, 0)) -> int:
+        """rbps::create_body (BodyAPI.hpp:53-80); defaults of BodyParams (BodyAPI.hpp:22-31)."""
+        sb = scenes.SceneBuilder()
+        sb.body(type, mass, inertia_tensor, position, orientation, linear_velocity, angular_velocity)
+        return int(self.add_bodies(sb.build("b").bodies)[0])
+
+    def create_collider(self, shape, dims, body_id=0, is_static=False, local_pos=(0, 0, 0),
+                        local_rot=(1, 0, 0, 0), restitution=0.5, static_friction=0.5, dynamic_friction=0.5) -> int:
+        """rbps::create_collider (ColliderAPI.hpp:61-91); defaults of ColliderParams (ColliderAPI.hpp:17-27)."""
+        sb = scenes.SceneBuilder()
+        sb.collider(shape, body_id, dims, is_static, local_pos, local_rot, restitution, static_friction,
+                    dynamic_friction)
+        return int(self.add_colliders(sb.build("c").colliders)[0])
+
+    def load(self, scene) -> "World":
+        self.add_bodies(scene.bodies)
+        self.add_colliders(scene.colliders)
+        return self
+
+    # ---------------------------------------------------------------- step
+    def step(self, frames: int = 1):
+        _check(lib().rbgpu_step(self.h, frames))
+
+    def synchronize(self):
+        _check(lib().rbgpu_synchronize(self.h))
+
+    # ---------------------------------------------------------------- state
+    def state(self) -> dict:
+        n = self.n_bodies
+        p, q, v, w = np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3))
+        _check(lib().rbgpu_get_bodies(self.h, _ptr(p, _D), _ptr(q, _D), _ptr(v, _D), _ptr(w, _D)))
+        return {"position": p, "orientation": q, "linear_velocity": v, "angular_velocity": w}
+
+    def set_state(self, st: dict):
+        a = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in st}
+        _check(lib().rbgpu_set_bodies(self.h, _ptr(a.get("position"), _D), _ptr(a.get("orientation"), _D),
+                                      _ptr(a.get("linear_velocity"), _D), _ptr(a.get("angular_velocity"), _D)))
+
+    def sap_pairs(self) -> np.ndarray:
+        n = C.c_uint32()
+        _check(lib().rbgpu_get_sap_pairs(self.h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 2), np.uint32)
+        if n.value:
+            _check(lib().rbgpu_get_sap_pairs(self.h, _ptr(out, _U), n.value, C.byref(n)))
+        return out
+
+    def pair_colours(self):
+        n = C.c_uint32()
+        _check(lib().rbgpu_get_pair_colours(self.h, None, None, 0, C.byref(n)))
+        pr, col = np.zeros((n.value, 2), np.uint32), np.zeros(n.value, np.int32)
+        if n.value:
+            _check(lib().rbgpu_get_pair_colours(self.h, _ptr(pr, _U), _ptr(col, _I), n.value, C.byref(n)))
+        return pr, col
+
+    def contacts(self) -> dict:
+        n = C.c_uint32()
+        _check(lib().rbgpu_get_contacts(self.h, None, None, None, None, None, None, 0, C.byref(n)))
+        k = n.value
+        ids = np.zeros((k, 4), np.uint32)
+        nrm, ra, rb = np.zeros((k, 3)), np.zeros((k, 3)), np.zeros((k, 3))
+        dep, lam = np.zeros(k), np.zeros((k, 2))
+        if k:
+            _check(lib().rbgpu_get_contacts(self.h, _ptr(ids, _U), _ptr(nrm, _D), _ptr(ra, _D), _ptr(rb, _D),
+                                            _ptr(dep, _D), _ptr(lam, _D), k, C.byref(n)))
+        return {"ids": ids, "normal": nrm, "r_a_local": ra, "r_b_local": rb, "depth": dep, "lambda": lam}
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().rbgpu_get_stats(self.h, C.byref(s)))
+        return {"frames": s.frames, "sap_pairs": s.sap_pairs, "pairs": s.pairs, "colours": s.colours,
+                "jp_rounds": s.jp_rounds, "contacts": s.contacts, "bins": list(s.bins)}
+
+    # ---------------------------------------------------------------- timing
+    def set_timing(self, kernel: str):
+        _check(lib().rbgpu_set_timing(self.h, TIMING[kernel]))
+
+    def timing(self) -> dict:
+        ms, n, b = C.c_double(), C.c_uint64(), C.c_double()
+        _check(lib().rbgpu_get_timing(self.h, C.byref(ms), C.byref(n), C.byref(b)))
+        return {"ms": ms.value, "launches": n.value, "bytes": b.value}
