@@ -1,0 +1,101 @@
+"""Throughput of the XPBD substep pipeline on one MI355X: body-substeps/s.
+
+Workload: the mixed convex pile — 65,536 dynamic bodies (boxes, spheres, capsules,
+type = index mod 3) on a seeded 64x16x64 grid over a static plane, dt = 1/60 s,
+10 substeps per World::step, FP32 state.  A "step" is one World::step (one
+frame); --steps frames are timed after --warmup untimed frames, inputs resident
+in HBM.  value = dynamic bodies * substeps * steps / seconds.
+
+Also reported:
+  roofline     — the dominant kernel family's algorithmic bytes per launch / its
+                 average launch duration (HIP events on the world's stream around
+                 every launch in the timed region), against 8 TB/s HBM.
+  cpu_baseline — the reference engine (compiled from its sources into
+                 oracle/_ref/libref.so; FP64, serial, 1 core) or, without it, the
+                 oracle restatement, on frames 1..k of the same scene (about 10 s).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(scene_fn, budget_s=10.0, max_frames=30):
+    from oracle import orc
+    sc = scene_fn()
+    kind = "reference" if orc.ref_available() else "port"
+    w = orc.RefWorld(sc.timestep, sc.substeps) if kind == "reference" else orc.OracleWorld(sc.timestep, sc.substeps)
+    w.load(sc)
+    t0 = time.perf_counter()
+    frames = 0
+    while frames < max_frames:
+        w.step(1)
+        frames += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": sc.n_dynamic * sc.substeps * frames / dt, "unit": "body-substeps/s", "cores": 1, "kind": kind,
+            "sample": f"frames 1..{frames} of the same {sc.n_dynamic}-body scene ({dt:.1f} s, FP64, serial)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--timing-kernel", default="solve_pos",
+                    choices=["off", "solve_pos", "solve_vel", "narrowphase", "integrate", "update_vel", "sweep"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(independent scenes would shard with no per-substep traffic, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from rbps_amd import scenes, world
+    scene_fn = scenes.pile_mixed
+    sc = scene_fn()
+    w = world.World(sc.timestep, sc.substeps).load(sc)
+    w.step(args.warmup)
+    w.synchronize()
+    w.set_timing(args.timing_kernel)
+    t0 = time.perf_counter()
+    w.step(args.steps)
+    w.synchronize()
+    dt = time.perf_counter() - t0
+    tm = w.timing()
+    st = w.stats()
+    value = sc.n_dynamic * sc.substeps * args.steps / dt
+
+    roof = None
+    if args.timing_kernel != "off" and tm["launches"]:
+        avg_s = tm["ms"] / 1e3 / tm["launches"]
+        bytes_per_launch = tm["bytes"] / tm["launches"]
+        achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": args.timing_kernel,
+                "avg_launch_us": round(avg_s * 1e6, 3), "launches": tm["launches"],
+                "algorithmic_bytes_per_launch": round(bytes_per_launch, 1)}
+
+    out = {
+        "metric": "body-substeps/s on 1 MI355X (plain GPU pipeline, mixed convex pile)",
+        "value": round(value, 1), "unit": "body-substeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
+        "config": {"workload": sc.name, "bodies": sc.n_dynamic, "substeps": sc.substeps, "timestep": sc.timestep,
+                   "shapes": "box/sphere/capsule + static plane", "last_frame": st},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene_fn)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -89,6 +89,10 @@ int rbgpu_add_colliders(rbgpu_world *w, int n, const rbgpu_collider_desc *desc, 
  * bookkeeping read-backs are done). */
 int rbgpu_step(rbgpu_world *w, int n_frames);
 
+/* Changes World::timestep / World::substeps (public members of the reference World,
+ * include/rbps/API/World.hpp:89-90) for the following steps. */
+int rbgpu_set_step(rbgpu_world *w, double timestep, int substeps);
+
 /* Waits for all queued work of the world. */
 int rbgpu_synchronize(rbgpu_world *w);
 
@@ -122,6 +126,13 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  * milliseconds and launches since the last reset (synchronises). */
 int rbgpu_set_timing(rbgpu_world *w, int kernel);
 int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, double *bytes);
+
+/* Narrowphase of n independent shape pairs, no world (rbc::test_narrow_phase,
+ * src/rbc/narrow_phase.cpp:8-13, dispatching as include/rbc/Dispatcher.hpp:212-217).
+ * kind: shape codes; dims: 4 per shape; tf: 7 per shape (px py pz qw qx qy qz).
+ * out: 20 per pair = n_points, normal xyz, then (point xyz, depth) x 4 (zeros when no hit). */
+int rbgpu_collide_pairs(int n, const int32_t *kind_a, const double *dims_a, const double *tf_a, const int32_t *kind_b,
+                        const double *dims_b, const double *tf_b, double *out);
 
 const char *rbgpu_last_error(void);
 

@@ -329,6 +329,40 @@ __global__ void __launch_bounds__(128) k_narrow(const uint32_t *bin_slot, uint32
     else K.cnt[s] = 0;
 }
 
+// Single-pair narrowphase over a batch (the analytic dispatch of Dispatcher.hpp:212-217
+// without a world): out row = n, normal xyz, then (point xyz, depth) x 4.
+__global__ void k_collide_pairs(uint32_t n, const int *kind_a, const float4 *dims_a, const float4 *pos_a,
+                                const float4 *rot_a, const int *kind_b, const float4 *dims_b, const float4 *pos_b,
+                                const float4 *rot_b, float *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Shape a = shape_of(kind_a[i], dims_a[i]), b = shape_of(kind_b[i], dims_b[i]);
+    TF ta{xyz(pos_a[i]), quat(rot_a[i])}, tb{xyz(pos_b[i]), quat(rot_b[i])};
+    Manifold m;
+    m.np = 0;
+    bool hit = false;
+    switch (bin_of(a.kind, b.kind)) {
+        case BIN_SS: hit = collide_bin<BIN_SS>(a, ta, b, tb, m); break;
+        case BIN_SB: hit = collide_bin<BIN_SB>(a, ta, b, tb, m); break;
+        case BIN_BB: hit = collide_bin<BIN_BB>(a, ta, b, tb, m); break;
+        case BIN_SC: hit = collide_bin<BIN_SC>(a, ta, b, tb, m); break;
+        case BIN_CC: hit = collide_bin<BIN_CC>(a, ta, b, tb, m); break;
+        case BIN_CB: hit = collide_bin<BIN_CB>(a, ta, b, tb, m); break;
+        case BIN_SP: hit = collide_bin<BIN_SP>(a, ta, b, tb, m); break;
+        case BIN_BP: hit = collide_bin<BIN_BP>(a, ta, b, tb, m); break;
+        case BIN_CP: hit = collide_bin<BIN_CP>(a, ta, b, tb, m); break;
+        default: break;
+    }
+    float *o = out + (size_t)i * 20;
+    for (int k = 0; k < 20; ++k) o[k] = 0.0f;
+    if (!hit) return;
+    o[0] = (float)m.np;
+    o[1] = m.n.x; o[2] = m.n.y; o[3] = m.n.z;
+    for (int k = 0; k < m.np; ++k) {
+        o[4 + 4 * k] = m.p[k].x; o[5 + 4 * k] = m.p[k].y; o[6 + 4 * k] = m.p[k].z; o[7 + 4 * k] = m.dep[k];
+    }
+}
+
 // ------------------------------------------------------------------ bodies (Body.cpp)
 __global__ void k_integrate(Bodies B, uint32_t N, float dt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

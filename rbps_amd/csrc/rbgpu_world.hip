@@ -575,6 +575,13 @@ int rbgpu_step(rbgpu_world *w, int n_frames) {
     return RBGPU_OK;
 }
 
+int rbgpu_set_step(rbgpu_world *w, double timestep, int substeps) {
+    if (!w || substeps <= 0 || !(timestep > 0.0)) return fail(RBGPU_ERR_INVALID, "invalid timestep/substeps");
+    w->timestep = timestep;
+    w->substeps = substeps;
+    return RBGPU_OK;
+}
+
 int rbgpu_synchronize(rbgpu_world *w) {
     if (!w) return fail(RBGPU_ERR_INVALID, "null world");
     if (w->st) HIPCHK(hipStreamSynchronize(w->st));
@@ -740,6 +747,54 @@ int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, doubl
         // algorithmic bytes per contact point: position solve 296 B, velocity solve 280 B
         *bytes = (double)pts * (w->timing_kernel == 1 ? 296.0 : w->timing_kernel == 2 ? 280.0 : 0.0);
     }
+    return RBGPU_OK;
+}
+
+int rbgpu_collide_pairs(int n, const int32_t *kind_a, const double *dims_a, const double *tf_a, const int32_t *kind_b,
+                        const double *dims_b, const double *tf_b, double *out) {
+    if (n < 0 || (n && (!kind_a || !dims_a || !tf_a || !kind_b || !dims_b || !tf_b || !out)))
+        return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (n == 0) return RBGPU_OK;
+    std::vector<float4> da(n), db(n), pa(n), pb(n), ra(n), rbq(n);
+    for (int i = 0; i < n; ++i) {
+        da[i] = make_float4((float)dims_a[4 * i], (float)dims_a[4 * i + 1], (float)dims_a[4 * i + 2], (float)dims_a[4 * i + 3]);
+        db[i] = make_float4((float)dims_b[4 * i], (float)dims_b[4 * i + 1], (float)dims_b[4 * i + 2], (float)dims_b[4 * i + 3]);
+        const double *a = tf_a + 7 * i, *b = tf_b + 7 * i;
+        pa[i] = make_float4((float)a[0], (float)a[1], (float)a[2], 0.0f);
+        ra[i] = make_float4((float)a[3], (float)a[4], (float)a[5], (float)a[6]);
+        pb[i] = make_float4((float)b[0], (float)b[1], (float)b[2], 0.0f);
+        rbq[i] = make_float4((float)b[3], (float)b[4], (float)b[5], (float)b[6]);
+    }
+    int *dka = nullptr, *dkb = nullptr;
+    float4 *dda = nullptr, *ddb = nullptr, *dpa = nullptr, *dpb = nullptr, *dra = nullptr, *drb = nullptr;
+    float *dout = nullptr;
+    int rc = 0;
+    rc |= dalloc(dka, n); rc |= dalloc(dkb, n); rc |= dalloc(dda, n); rc |= dalloc(ddb, n); rc |= dalloc(dpa, n);
+    rc |= dalloc(dpb, n); rc |= dalloc(dra, n); rc |= dalloc(drb, n); rc |= dalloc(dout, 20 * (size_t)n);
+    std::vector<float> fo(20 * (size_t)n);
+    auto cleanup = [&]() {
+        dfree(dka); dfree(dkb); dfree(dda); dfree(ddb); dfree(dpa); dfree(dpb); dfree(dra); dfree(drb); dfree(dout);
+    };
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        if (e == hipSuccess) e = hipMemcpy(dka, kind_a, n * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dkb, kind_b, n * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dda, da.data(), n * 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(ddb, db.data(), n * 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dpa, pa.data(), n * 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dpb, pb.data(), n * 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dra, ra.data(), n * 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(drb, rbq.data(), n * 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            k_collide_pairs<<<nblk(n), TPB>>>(n, dka, dda, dpa, dra, dkb, ddb, dpb, drb, dout);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpy(fo.data(), dout, 20 * (size_t)n * 4, hipMemcpyDeviceToHost);
+    }
+    cleanup();
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(RBGPU_ERR_HIP, std::string("rbgpu_collide_pairs: ") + hipGetErrorString(e));
+    for (size_t k = 0; k < fo.size(); ++k) out[k] = fo[k];
     return RBGPU_OK;
 }
 

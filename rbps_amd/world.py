@@ -27,8 +27,9 @@ _I = C.POINTER(C.c_int32)
 
 EXPORTS = [
     "rbgpu_world_create", "rbgpu_world_destroy", "rbgpu_add_bodies", "rbgpu_add_colliders", "rbgpu_step",
-    "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
-    "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_last_error",
+    "rbgpu_set_step", "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
+    "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_collide_pairs",
+    "rbgpu_last_error",
 ]
 
 
@@ -56,6 +57,7 @@ def lib():
         L.rbgpu_add_bodies.argtypes = [_P, C.c_int, _P, _U]
         L.rbgpu_add_colliders.argtypes = [_P, C.c_int, _P, _U]
         L.rbgpu_step.argtypes = [_P, C.c_int]
+        L.rbgpu_set_step.argtypes = [_P, C.c_double, C.c_int]
         L.rbgpu_synchronize.argtypes = [_P]
         L.rbgpu_get_bodies.argtypes = [_P, _D, _D, _D, _D]
         L.rbgpu_set_bodies.argtypes = [_P, _D, _D, _D, _D]
@@ -65,6 +67,7 @@ def lib():
         L.rbgpu_get_stats.argtypes = [_P, C.POINTER(Stats)]
         L.rbgpu_set_timing.argtypes = [_P, C.c_int]
         L.rbgpu_get_timing.argtypes = [_P, _D, C.POINTER(C.c_uint64), _D]
+        L.rbgpu_collide_pairs.argtypes = [C.c_int, _I, _D, _D, _I, _D, _D, _D]
         L.rbgpu_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -77,6 +80,18 @@ def _check(rc: int):
 
 def _ptr(a, t):
     return a.ctypes.data_as(t) if a is not None else None
+
+
+def collide_pairs(kind_a, dims_a, tf_a, kind_b, dims_b, tf_b) -> np.ndarray:
+    """Single-pair narrowphase of a batch on the GPU; rows = n, normal, (point, depth) x 4."""
+    ka = np.ascontiguousarray(kind_a, np.int32)
+    kb = np.ascontiguousarray(kind_b, np.int32)
+    da, db = np.ascontiguousarray(dims_a, np.float64), np.ascontiguousarray(dims_b, np.float64)
+    ta, tb = np.ascontiguousarray(tf_a, np.float64), np.ascontiguousarray(tf_b, np.float64)
+    out = np.zeros((len(ka), 20))
+    _check(lib().rbgpu_collide_pairs(len(ka), _ptr(ka, _I), _ptr(da, _D), _ptr(ta, _D), _ptr(kb, _I), _ptr(db, _D),
+                                     _ptr(tb, _D), _ptr(out, _D)))
+    return out
 
 
 TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "sweep": 6}
@@ -143,6 +158,11 @@ class World:
     # ---------------------------------------------------------------- step
     def step(self, frames: int = 1):
         _check(lib().rbgpu_step(self.h, frames))
+
+    def set_step(self, timestep: float, substeps: int):
+        """Changes World::timestep / World::substeps for the following steps."""
+        _check(lib().rbgpu_set_step(self.h, timestep, substeps))
+        self.timestep, self.substeps = timestep, substeps
 
     def synchronize(self):
         _check(lib().rbgpu_synchronize(self.h))

@@ -1,0 +1,128 @@
+// The reference's World tests (tests/rbps/API/test_world.cpp) written against
+// rbps_amd::World, the drop-in facade over the rbgpu C-ABI.  Built and run by
+// tests/test_gpu_facade.py; prints one line per case and exits non-zero on failure.
+#include <cmath>
+#include <cstdio>
+#include "rbps_amd/World.hpp"
+
+using namespace rbps_amd;
+
+static int failures = 0;
+#define CHECK(cond)                                                        \
+    do {                                                                   \
+        if (!(cond)) {                                                     \
+            std::printf("  FAILED %s:%d %s\n", __FILE__, __LINE__, #cond); \
+            ++failures;                                                    \
+        }                                                                  \
+    } while (0)
+
+static uint32_t add_dynamic_sphere(World &w, vec3 pos, scalar r = 0.5, scalar m = 1.0, vec3 v = vec3(0.0)) {
+    BodyParams bp;
+    bp.position = pos;
+    bp.linear_velocity = v;
+    bp.mass = m;
+    bp.type = DYNAMIC;
+    uint32_t id = w.create_body(bp);
+    ColliderParams cp;
+    cp.body_id = id;
+    cp.shape = Sphere(r);
+    w.create_collider(cp);
+    return id;
+}
+static uint32_t add_static_box(World &w, vec3 pos, vec3 half = vec3(10, 2, 10)) {
+    BodyParams bp;
+    bp.position = pos;
+    bp.mass = 0.0;
+    bp.type = STATIC;
+    uint32_t id = w.create_body(bp);
+    ColliderParams cp;
+    cp.body_id = id;
+    cp.shape = Box(half);
+    w.create_collider(cp);
+    return id;
+}
+static scalar mag(vec3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+int main() {
+    {
+        std::printf("world_parameterized_construction\n");
+        World w(1.0 / 120.0, 10);
+        CHECK(std::fabs(w.timestep - 1.0 / 120.0) < 1e-9 && w.substeps == 10);
+    }
+    {
+        std::printf("world_step_empty_world\n");
+        World w;
+        w.step();
+        w.step();
+    }
+    {
+        std::printf("world_free_fall\n");
+        World w(1.0 / 60.0, 20);
+        uint32_t s = w.bodies.index_of(add_dynamic_sphere(w, vec3(0, 10, 0)));
+        scalar y0 = w.bodies.position[s].y;
+        w.step();
+        scalar vy1 = w.bodies.linear_velocity[s].y;
+        CHECK(w.bodies.position[s].y < y0);
+        w.step();
+        CHECK(w.bodies.linear_velocity[s].y < vy1);
+    }
+    {
+        std::printf("world_static_body_does_not_move\n");
+        World w;
+        uint32_t s = w.bodies.index_of(add_static_box(w, vec3(0, 0, 0)));
+        for (int i = 0; i < 10; ++i) w.step();
+        CHECK(std::fabs(w.bodies.position[s].y) < 1e-6);
+    }
+    {
+        std::printf("world_sphere_lands_on_ground\n");
+        World w(1.0 / 60.0, 20);
+        add_static_box(w, vec3(0, -3, 0));
+        uint32_t s = w.bodies.index_of(add_dynamic_sphere(w, vec3(0, 1, 0), 0.5));
+        w.auto_sync(false);
+        for (int i = 0; i < 180; ++i) w.step();
+        w.sync_bodies();
+        CHECK(mag(w.bodies.linear_velocity[s]) < 2.0);
+        CHECK(w.bodies.position[s].y > -4.0);
+    }
+    {
+        std::printf("world_two_spheres_collide_and_separate\n");
+        World w(1.0 / 60.0, 20);
+        uint32_t a = add_dynamic_sphere(w, vec3(-2, 0, 0), 0.5, 1.0, vec3(5, 0, 0));
+        uint32_t b = add_dynamic_sphere(w, vec3(2, 0, 0), 0.5, 1.0, vec3(-5, 0, 0));
+        for (int i = 0; i < 30; ++i) w.step();
+        CHECK(w.bodies.linear_velocity[a].x < 0.0 || w.bodies.linear_velocity[b].x > 0.0);
+    }
+    {
+        std::printf("world_timestep_change_between_steps\n");
+        World w(1.0 / 60.0, 20);
+        uint32_t s = add_dynamic_sphere(w, vec3(0, 10, 0));
+        w.step();
+        w.substeps = 10;
+        w.timestep = 1.0 / 120.0;
+        w.step();
+        CHECK(w.bodies.position[s].y < 10.0);
+    }
+    {
+        std::printf("world_no_contacts_when_separated\n");
+        World w;
+        add_dynamic_sphere(w, vec3(0, 100, 0));
+        add_dynamic_sphere(w, vec3(100, 0, 0));
+        w.step();
+        CHECK(w.contact_count() == 0);
+    }
+    {
+        std::printf("world_error_is_an_exception\n");
+        World w;
+        bool thrown = false;
+        try {
+            ColliderParams cp;
+            cp.body_id = 42;   // no such body
+            w.create_collider(cp);
+        } catch (const Error &e) {
+            thrown = true;
+        }
+        CHECK(thrown);
+    }
+    std::printf("%s (%d failures)\n", failures ? "FAILED" : "ALL PASSED", failures);
+    return failures ? 1 : 0;
+}

@@ -1,0 +1,159 @@
+"""GPU pipeline (HIP, through the rbgpu C-ABI) against the oracle.
+
+Bar: the GPU runs the colour order in FP32; its results are compared with the FP32
+colour-order oracle — bit-exact for pair lists, colours and contact sets, and states
+within 1e-5 relative (they are bit-identical in practice: same float expressions, no
+FMA contraction, IEEE division/sqrt).  Pair sets are also checked against the FP64
+reference's broadphase output on the same input (golden fixtures), and the GPU
+narrowphase against the reference's manifolds.
+"""
+import numpy as np
+import pytest
+
+from oracle import orc
+from rbps_amd import scenes
+from tests.conftest import golden
+from tests.helpers import colouring_valid, filtered_ref_pairs, sorted_pairs, state_close
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-5
+
+
+def _world(sc):
+    from rbps_amd import world
+    return world.World(sc.timestep, sc.substeps).load(sc)
+
+
+def _oracle(sc):
+    return orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
+
+
+def _assert_frame_parity(g, o, sc, tag):
+    ok, worst = state_close(g.state(), o.state(), RTOL)
+    assert ok, f"{tag}: state rel err {worst}"
+    assert np.array_equal(g.sap_pairs(), o.pairs()), f"{tag}: SAP pairs"
+    pr, col = g.pair_colours()
+    sp = o.solve_pairs()
+    assert np.array_equal(pr, sp["colliders"]), f"{tag}: pair list"
+    assert np.array_equal(col, sp["colour"]), f"{tag}: colours"
+    dyn = sc.bodies["type"] == scenes.DYNAMIC
+    assert colouring_valid(sc.colliders["body_id"][pr] if len(pr) else np.zeros((0, 2)), col, dyn), tag
+    cg, co = g.contacts(), o.contacts()
+    assert np.array_equal(cg["ids"], co["ids"]), f"{tag}: contact ids"
+    for k in ("normal", "r_a_local", "r_b_local", "depth", "lambda"):
+        assert np.allclose(cg[k], co[k], rtol=RTOL, atol=RTOL), f"{tag}: contact {k}"
+    return worst
+
+
+@pytest.mark.parametrize("make,frames", [
+    (lambda: scenes.pile_small(nx=4, ny=4, nz=4), 30),
+    (lambda: scenes.pile_mixed_dense(), 12),
+    (lambda: scenes.pile_small(nx=6, ny=5, nz=6, seed=99), 6),
+])
+def test_gpu_matches_fp32_colour_oracle(make, frames):
+    sc = make()
+    print("case", sc.name, flush=True)
+    g, o = _world(sc), _oracle(sc)
+    for f in range(frames):
+        g.step(1)
+        o.step(1)
+        _assert_frame_parity(g, o, sc, f"{sc.name} frame {f + 1}")
+
+
+@pytest.mark.parametrize("name", ["world_pile_small_4x4x4", "world_pile_mixed_dense"])
+def test_gpu_first_frame_pairs_equal_reference(name):
+    """Same input state -> the GPU's filtered SAP pair set equals the FP64 reference's."""
+    g_ = golden(name)
+    sc = scenes.Scene(name, g_["bodies"], g_["colliders"], float(g_["timestep"]), int(g_["substeps"]))
+    w = _world(sc)
+    w.step(1)
+    ref = filtered_ref_pairs(g_["pairs_1"], sc.colliders)
+    assert np.array_equal(sorted_pairs(w.sap_pairs()), sorted_pairs(ref))
+
+
+def test_gpu_one_step_vs_fp64_reference():
+    """One frame of the 64-body pile vs the FP64 reference (golden): FP32 + colour order vs FP64 +
+    sequential order; the bound is the FP32 resolution at |x| ~ 5 m over 20 substeps."""
+    g_ = golden("world_pile_small_4x4x4")
+    sc = scenes.Scene("p", g_["bodies"], g_["colliders"], float(g_["timestep"]), int(g_["substeps"]))
+    w = _world(sc)
+    w.step(1)
+    st = w.state()
+    err = np.abs(st["position"] - g_["position_1"]).max()
+    print("one-frame max |dx| vs FP64 reference:", err)
+    assert err < 1e-3
+
+
+def test_gpu_narrowphase_vs_reference_manifolds():
+    """rbgpu_collide_pairs vs the reference's rbc::dispatch outputs (golden), FP32 tolerance."""
+    from rbps_amd import world
+    g_ = golden("narrowphase_pairs")
+    out = world.collide_pairs(g_["kind_a"], g_["dims_a"], g_["tfa"], g_["kind_b"], g_["dims_b"], g_["tfb"])
+    exp = g_["result"]
+    n_ok = 0
+    for i in range(len(exp)):
+        if int(exp[i, 0]) != int(out[i, 0]):
+            # a pair within FP32 rounding of touching may flip; allow only if the reference depth is tiny
+            assert exp[i, 0] == 0 or np.min(exp[i, 7:4 + 4 * int(exp[i, 0]):4]) < 1e-5, i
+            continue
+        n = int(exp[i, 0])
+        if n:
+            assert np.allclose(out[i, 1:4], exp[i, 1:4], atol=1e-4), i
+            assert np.allclose(out[i, 4:4 + 4 * n], exp[i, 4:4 + 4 * n], atol=1e-4), i
+        n_ok += 1
+    assert n_ok >= len(exp) - 3
+
+
+def test_gpu_narrowphase_vs_fp32_oracle_bit_exact():
+    from rbps_amd import world
+    g_ = golden("narrowphase_pairs")
+    out = world.collide_pairs(g_["kind_a"], g_["dims_a"], g_["tfa"], g_["kind_b"], g_["dims_b"], g_["tfb"])
+    for i in range(len(out)):
+        a = np.zeros((), scenes.COLLIDER_DTYPE)
+        b = np.zeros((), scenes.COLLIDER_DTYPE)
+        a["shape"], a["dims"] = g_["kind_a"][i], g_["dims_a"][i]
+        b["shape"], b["dims"] = g_["kind_b"][i], g_["dims_b"][i]
+        r = orc.oracle_collide(a, g_["tfa"][i], b, g_["tfb"][i], fp32=True)
+        assert int(out[i, 0]) == r["n"], i
+        if r["n"]:
+            assert np.array_equal(out[i, 1:4], r["normal"]), i
+            for k in range(r["n"]):
+                assert np.array_equal(out[i, 4 + 4 * k:7 + 4 * k], r["points"][k]), i
+                assert out[i, 7 + 4 * k] == r["depth"][k], i
+
+
+def test_gpu_full_size_pile_first_frames():
+    """65,536-body mixed pile at full size: 2 frames bit-compared with the FP32 colour oracle,
+    then size-independent properties (finite state, valid colouring, pair symmetry)."""
+    sc = scenes.pile_mixed()
+    g, o = _world(sc), _oracle(sc)
+    for f in range(2):
+        g.step(1)
+        o.step(1)
+    worst = _assert_frame_parity(g, o, sc, "pile_mixed")
+    print("65k pile: worst rel err after 2 frames", worst)
+    g.step(40)
+    st = g.state()
+    assert all(np.isfinite(v).all() for v in st.values())
+    pr, col = g.pair_colours()
+    dyn = sc.bodies["type"] == scenes.DYNAMIC
+    assert colouring_valid(sc.colliders["body_id"][pr], col, dyn)
+    p = g.sap_pairs()
+    assert (p[:, 0] < p[:, 1]).all() and len(np.unique(p, axis=0)) == len(p)
+
+
+def test_gpu_empty_and_degenerate_worlds():
+    from rbps_amd import world
+    w = world.World(1 / 60, 20)
+    w.step(2)                                   # empty world (test_world.cpp:50-55)
+    w2 = world.World(1 / 60, 20)
+    w2.create_body(position=(0, 10, 0))         # body without collider (test_world.cpp:57-67)
+    w2.step(1)
+    assert w2.state()["position"][0, 1] < 10
+    w3 = world.World(1 / 60, 20)
+    b = w3.create_body(position=(0, 100, 0))
+    w3.create_collider(scenes.SPHERE, (0.5,), body_id=b)
+    b2 = w3.create_body(position=(100, 0, 0))
+    w3.create_collider(scenes.SPHERE, (0.5,), body_id=b2)
+    w3.step(1)                                   # separated: no contacts (test_world.cpp:178-186)
+    assert w3.stats()["contacts"] == 0 and len(w3.sap_pairs()) == 0

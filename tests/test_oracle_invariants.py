@@ -1,0 +1,109 @@
+"""Properties that pin the parts of the oracle the reference does not exercise:
+the canonical sweep order, the colour order and its colouring."""
+import numpy as np
+import pytest
+
+from oracle import orc
+from rbps_amd import scenes
+from tests.helpers import colouring_valid, greedy_by_priority, sorted_pairs
+
+
+def test_canonical_sweep_same_set_as_active_set_sweep():
+    """Forward scan from each min endpoint finds exactly the active-set sweep's pairs."""
+    sc = scenes.pile_mixed_dense()
+    a = orc.OracleWorld(sc.timestep, sc.substeps, order=orc.ORDER_REFERENCE).load(sc)
+    b = orc.OracleWorld(sc.timestep, sc.substeps, order=orc.ORDER_COLOUR).load(sc)
+    a.step(1)
+    b.step(1)
+    assert np.array_equal(sorted_pairs(a.pairs()), sorted_pairs(b.pairs()))
+    assert len(a.pairs()) > 100
+
+
+def test_pair_set_brute_force_first_frame():
+    """At frame 1 the bodies are at rest, so the fat boxes are the insert-time boxes (tight +- 0.1);
+    the SAP pair set equals all colliding-box pairs found by brute force."""
+    sc = scenes.pile_mixed_dense(nx=5, ny=3, nz=5)
+    w = orc.OracleWorld(sc.timestep, sc.substeps).load(sc)
+    w.step(1)
+    # brute force on the insert-time fat AABBs (computed from the oracle's own shape AABB through a
+    # second world stepped zero frames is not exposed; recompute in numpy for the three kinds)
+    boxes = []
+    for c in sc.colliders:
+        if c["shape"] == scenes.PLANE:
+            boxes.append(None)
+            continue
+        b = sc.bodies[c["body_id"]]
+        p = b["position"]
+        w_, x, y, z = b["orientation"]
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w_ * z), 2 * (x * z + w_ * y)],
+                      [2 * (x * y + w_ * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w_ * x)],
+                      [2 * (x * z - w_ * y), 2 * (y * z + w_ * x), 1 - 2 * (x * x + y * y)]])
+        d = c["dims"]
+        if c["shape"] == scenes.SPHERE:
+            e = np.array([d[0]] * 3)
+        elif c["shape"] == scenes.BOX:
+            e = np.abs(R) @ d[:3]
+        else:
+            ax = R @ np.array([0, d[0], 0])
+            e = np.abs(ax) + d[1]
+        boxes.append((p - e - 0.1, p + e + 0.1))
+    bf = []
+    for i in range(len(boxes)):
+        for j in range(i + 1, len(boxes)):
+            if boxes[i] is None or boxes[j] is None:
+                continue
+            (ai, bi), (aj, bj) = boxes[i], boxes[j]
+            if np.all(ai <= bj) and np.all(bi >= aj):
+                bf.append((i, j))
+    got = sorted_pairs(w.pairs())
+    # boxes within 1e-12 of touching could differ by rounding; the seeded scene has none
+    assert np.array_equal(got, sorted_pairs(bf))
+
+
+def test_colour_order_colouring_is_valid_and_greedy():
+    sc = scenes.pile_mixed_dense()
+    w = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
+    w.step(2)
+    sp = w.solve_pairs()
+    cols = sp["colliders"]
+    bodies = sc.colliders["body_id"][cols]
+    dyn = sc.bodies["type"] == scenes.DYNAMIC
+    assert colouring_valid(bodies, sp["colour"], dyn)
+    assert np.array_equal(sp["colour"], greedy_by_priority(bodies, dyn, orc.mix32))
+    assert sp["ncolours"] == sp["colour"].max() + 1
+
+
+def test_mix32_known_values():
+    """lowbias32 reference values (0 -> 0; 1 -> 0x8ee0d535 per its published constants)."""
+    assert orc.mix32(0) == 0
+    x = 1
+    x ^= x >> 16; x = (x * 0x7feb352d) & 0xffffffff
+    x ^= x >> 15; x = (x * 0x846ca68b) & 0xffffffff
+    x ^= x >> 16
+    assert orc.mix32(1) == x
+
+
+@pytest.mark.skipif(not orc.ref_available(), reason="reference build not present")
+def test_reference_greedy_groups_invariants():
+    """get_collision_groups (CollisionPipeline.cpp:223-293): every contact in one group, no two
+    contacts of a group share a dynamic body (test_collision_pipeline_graph_coloring.cpp:26-68)."""
+    sc = scenes.pile_mixed_dense()
+    r = orc.RefWorld(sc.timestep, sc.substeps).load(sc)
+    r.step(3)
+    col, ng = r.collision_groups()
+    c = r.contacts()
+    assert len(col) == len(c["ids"]) and (col >= 0).all()
+    dyn = sc.bodies["type"] == scenes.DYNAMIC
+    assert colouring_valid(c["ids"][:, :2], col, dyn)
+
+
+def test_colour_order_vs_reference_order_divergence_reported():
+    """Colour order changes the Gauss-Seidel order; after one frame of the dense pile the two FP64
+    runs differ by an algorithmic (not rounding) amount, both stay finite."""
+    sc = scenes.pile_mixed_dense()
+    a = orc.OracleWorld(sc.timestep, sc.substeps, order=orc.ORDER_REFERENCE).load(sc)
+    b = orc.OracleWorld(sc.timestep, sc.substeps, order=orc.ORDER_COLOUR).load(sc)
+    a.step(1)
+    b.step(1)
+    d = np.abs(a.state()["position"] - b.state()["position"]).max()
+    assert np.isfinite(d) and d < 0.5
