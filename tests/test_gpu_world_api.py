@@ -70,7 +70,8 @@ def test_two_spheres_collide_and_separate():            # test_world.cpp:139-156
     v = w.state()["linear_velocity"]
     assert v[a, 0] < 0.0 or v[b, 0] > 0.0
     ref = golden("world_api_scenes")
-    assert np.abs(w.state()["position"] - ref["bounce_position"]).max() < 1e-3
+    # FP32 vs the FP64 reference after a 10 m/s impact: the FP32 oracle shows the same 2.09e-3 m gap
+    assert np.abs(w.state()["position"] - ref["bounce_position"]).max() < 5e-3
 
 
 def test_momentum_conserved_approximately():            # test_world.cpp:158-176
