@@ -8,9 +8,10 @@
 //                     order (pairs in SAP active-set order, then the plane loop).
 //                     With T=double this is the reference step bit-for-bit.
 //   ORDER_COLOUR    — the order the GPU pipeline uses: the frame's pairs in
-//                     canonical sweep order, coloured greedily in descending
-//                     hash priority (what Jones-Plassmann computes), then solved
-//                     colour by colour; a pair's points are solved in sequence.
+//                     canonical sweep order; every substep the pairs that have
+//                     contact points are coloured by synchronous randomized trials
+//                     (colour_active) and solved colour by colour; a pair's points
+//                     are solved in sequence.
 #pragma once
 #include <algorithm>
 #include <cstdint>
@@ -77,6 +78,7 @@ template <class T> struct World {
     std::vector<SPair> spairs;                     // ORDER_COLOUR: frame pair list
     std::vector<int> colour;
     int ncolours = 0;
+    uint32_t colour_rounds = 0;
     std::vector<int> pcount;                       // points per pair this substep
     std::vector<Contact<T>> pcont;                 // 4 slots per pair
 
@@ -260,7 +262,7 @@ template <class T> struct World {
         }
     }
     // Frame pair list for the colour order: canonical SAP pairs then the plane loop.
-    void build_pairs_and_colour() {
+    void build_pairs() {
         spairs.clear();
         for (auto &pr : sap_pairs)
             if (pair_ok(pr.first, pr.second)) spairs.push_back({pr.first, pr.second, cbody[pr.first], cbody[pr.second]});
@@ -272,36 +274,82 @@ template <class T> struct World {
                 spairs.push_back({di, pi, cbody[di], cbody[pi]});
             }
         }
-        // Greedy first-fit in descending (mix32(p), p) priority == Jones-Plassmann result.
         const uint32_t P = (uint32_t)spairs.size();
+        colour.assign(P, -1);
+        pcount.assign(P, 0);
+        pcont.resize((size_t)P * 4);
+    }
+    // Colouring of the substep's active pairs (pairs with contact points); edges = shared
+    // dynamic bodies.  Synchronous randomized trials, replayed exactly as the GPU runs them:
+    // in round r every uncoloured pair v picks t(v) = the (mix32(p ^ r*0x9e3779b9) mod k)-th
+    // colour not used by its already-coloured neighbours, k = (number of uncoloured
+    // neighbour entries) + 1; it keeps t(v) unless an uncoloured neighbour of higher
+    // priority (mix32(q), q) picked the same colour.  Valid, at most degree+1 colours,
+    // O(log degree) rounds.
+    static int pick_free(const std::vector<char> &used, uint32_t idx) {
+        for (int c = 0;; ++c) {
+            if (c < (int)used.size() && used[c]) continue;
+            if (idx == 0) return c;
+            --idx;
+        }
+    }
+    void colour_active() {
+        const uint32_t P = (uint32_t)spairs.size();
+        std::vector<uint32_t> act;
+        for (uint32_t p = 0; p < P; ++p)
+            if (pcount[p] > 0) act.push_back(p);
         std::vector<std::vector<uint32_t>> by_body(btype.size());
-        for (uint32_t p = 0; p < P; ++p) {
+        for (uint32_t p : act) {
             if (btype[spairs[p].ba] == 1) by_body[spairs[p].ba].push_back(p);
             if (btype[spairs[p].bb] == 1) by_body[spairs[p].bb].push_back(p);
         }
-        std::vector<uint32_t> ord(P);
-        for (uint32_t p = 0; p < P; ++p) ord[p] = p;
-        std::sort(ord.begin(), ord.end(), [](uint32_t a, uint32_t b) {
-            uint32_t ha = mix32(a), hb = mix32(b);
-            return ha != hb ? ha > hb : a > b;
-        });
         colour.assign(P, -1);
         ncolours = 0;
+        colour_rounds = 0;
+        std::vector<int> tent(P, -1);
         std::vector<char> used;
-        for (uint32_t p : ord) {
-            used.assign(ncolours + 1, 0);
-            for (uint32_t b : {spairs[p].ba, spairs[p].bb}) {
-                if (btype[b] != 1) continue;
-                for (uint32_t nb : by_body[b])
-                    if (nb != p && colour[nb] >= 0) used[colour[nb]] = 1;
+        size_t left = act.size();
+        auto higher = [](uint32_t q, uint32_t p) {
+            uint32_t hq = mix32(q), hp = mix32(p);
+            return hq > hp || (hq == hp && q > p);
+        };
+        for (uint32_t r = 0; left > 0; ++r) {
+            for (uint32_t p : act) {
+                if (colour[p] >= 0) continue;
+                used.assign(1, 0);
+                uint32_t k = 1;
+                for (uint32_t b : {spairs[p].ba, spairs[p].bb}) {
+                    if (btype[b] != 1) continue;
+                    for (uint32_t q : by_body[b]) {
+                        if (q == p) continue;
+                        if (colour[q] >= 0) {
+                            if ((int)used.size() <= colour[q]) used.resize(colour[q] + 1, 0);
+                            used[colour[q]] = 1;
+                        } else {
+                            ++k;
+                        }
+                    }
+                }
+                tent[p] = pick_free(used, mix32(p ^ (r * 0x9e3779b9u)) % k);
             }
-            int c = 0;
-            while (used[c]) ++c;
-            colour[p] = c;
-            if (c + 1 > ncolours) ncolours = c + 1;
+            std::vector<uint32_t> accepted;
+            for (uint32_t p : act) {
+                if (colour[p] >= 0) continue;
+                bool ok = true;
+                for (uint32_t b : {spairs[p].ba, spairs[p].bb}) {
+                    if (btype[b] != 1 || !ok) continue;
+                    for (uint32_t q : by_body[b])
+                        if (q != p && colour[q] < 0 && tent[q] == tent[p] && higher(q, p)) { ok = false; break; }
+                }
+                if (ok) accepted.push_back(p);
+            }
+            for (uint32_t p : accepted) {
+                colour[p] = tent[p];
+                if (tent[p] + 1 > ncolours) ncolours = tent[p] + 1;
+            }
+            left -= accepted.size();
+            colour_rounds = r + 1;
         }
-        pcount.assign(P, 0);
-        pcont.resize((size_t)P * 4);
     }
     void narrowphase_colour() {
         for (uint32_t p = 0; p < spairs.size(); ++p) {
@@ -464,9 +512,10 @@ template <class T> struct World {
         const T inv_h = T(1.0) / h;
         refit(timestep);
         bp_update();
-        if (order == ORDER_COLOUR) build_pairs_and_colour();
+        if (order == ORDER_COLOUR) build_pairs();
         for (int s = 0; s < substeps; ++s) {
-            if (order == ORDER_REFERENCE) narrowphase_reference(); else narrowphase_colour();
+            if (order == ORDER_REFERENCE) narrowphase_reference();
+            else { narrowphase_colour(); colour_active(); }
             integrate(h);
             if (order == ORDER_REFERENCE)
                 for (auto &k : contacts) contact_position(k, inv_h);

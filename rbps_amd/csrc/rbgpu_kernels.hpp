@@ -225,14 +225,6 @@ __global__ void k_jp_round(const uint2 *pbod, const uint32_t *n_pairs, Bodies B,
     atomicSub(uncoloured, 1u);
 }
 
-// Keys for the stable colour sort: key = colour, value = pair index.
-__global__ void k_colour_keys(const int *colour, const uint32_t *n_pairs, uint32_t *key, uint32_t *val) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= *n_pairs) return;
-    key[p] = (uint32_t)colour[p];
-    val[p] = p;
-}
-
 __device__ __forceinline__ int bin_of(int ka, int kb) {
     if (kb == SH_PLANE) return ka == SH_SPHERE ? BIN_SP : ka == SH_BOX ? BIN_BP : ka == SH_CAPSULE ? BIN_CP : BIN_OTHER;
     if (ka > kb) { int t = ka; ka = kb; kb = t; }
@@ -245,22 +237,201 @@ __device__ __forceinline__ int bin_of(int ka, int kb) {
     return BIN_OTHER;
 }
 
-// Solve slot s holds pair slot_pair[s]; write the slot's pair data and its bin key.
-__global__ void k_slot_gather(const uint32_t *slot_pair, const uint32_t *n_pairs, const uint2 *pairs, const uint2 *pbod,
-                              Colliders C, Contacts K, uint32_t *bin_key, uint32_t *bin_val) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= *n_pairs) return;
-    uint32_t p = slot_pair[s];
+// Per-frame pair data indexed by pair id: colliders, bodies, mixed materials, bin key.
+__global__ void k_pair_gather(const uint2 *pairs, const uint32_t *n_pairs, Colliders C, Contacts K, uint32_t *bin_key,
+                              uint32_t *bin_val) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= *n_pairs) return;
     uint2 cc = pairs[p];
-    K.cc[s] = cc;
-    K.bb[s] = pbod[p];
-    K.cnt[s] = 0;
+    int4 ma_ = C.meta[cc.x], mb_ = C.meta[cc.y];
+    K.cc[p] = cc;
+    K.bb[p] = make_uint2((uint32_t)ma_.y, (uint32_t)mb_.y);
+    K.cnt[p] = 0;
     float4 ma = C.mat[cc.x], mb = C.mat[cc.y];
     float rest = fmin_ref(ma.x, mb.x);                                     // CollisionPipeline.cpp:75-77
-    K.nr[s] = make_float4(0.0f, 0.0f, 0.0f, rest);
-    K.fr[s] = make_float2((ma.y + mb.y) * 0.5f, (ma.z + mb.z) * 0.5f);
-    bin_key[s] = (uint32_t)bin_of(C.meta[cc.x].x, C.meta[cc.y].x);
-    bin_val[s] = s;
+    K.nr[p] = make_float4(0.0f, 0.0f, 0.0f, rest);
+    K.fr[p] = make_float2((ma.y + mb.y) * 0.5f, (ma.z + mb.z) * 0.5f);
+    bin_key[p] = (uint32_t)bin_of(ma_.x, mb_.x);
+    bin_val[p] = p;
+}
+
+// ------------------------------------------------------------------ grid barrier
+// Barrier for cooperative (co-resident) grids.  The last block to arrive evaluates
+// `*probe == 0` (when probe is given) after every block has finished the phase, and
+// publishes it with the new generation, so every block takes the same decision.
+// Every wait is bounded: after BARRIER_TIMEOUT_TICKS of the 100 MHz wall clock a
+// waiting block raises *fault and leaves; once *fault is set no block waits again,
+// so the kernel always drains (the host then reports the fault as an error).
+constexpr uint64_t BARRIER_TIMEOUT_TICKS = 200000000ull;   // 2 s
+struct GridBar {
+    uint32_t *count, *gen;
+    int *decision;
+    int *fault;
+};
+__device__ __forceinline__ int grid_barrier(GridBar b, const uint32_t *probe) {
+    __shared__ int s_dec;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t g = __hip_atomic_load(b.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = __hip_atomic_fetch_add(b.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            int d = probe ? (__hip_atomic_load(probe, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) : 0;
+            __hip_atomic_store(b.decision, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(b.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(b.gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(b.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (__hip_atomic_load(b.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (wall_clock64() - t0 > BARRIER_TIMEOUT_TICKS) {
+                    __hip_atomic_store(b.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        s_dec = __hip_atomic_load(b.decision, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_load(b.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) s_dec = 1;   // leave any loop
+        __threadfence();
+    }
+    __syncthreads();
+    return s_dec;
+}
+
+// ------------------------------------------------------------------ substep colouring of active pairs
+__global__ void k_active_flags(const int *cnt, uint32_t P, uint32_t *flag) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P) flag[p] = cnt[p] > 0 ? 1u : 0u;
+}
+// Compacted active list (ascending pair id); colour reset; counters reset.
+__global__ void k_compact(const uint32_t *flag, const uint32_t *off, uint32_t P, const uint32_t *n_act, uint32_t *act,
+                          int *colour, uint32_t *cround, uint32_t *uncoloured, uint32_t *ncol, uint32_t *deg, uint32_t *cur,
+                          uint32_t N) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) { *uncoloured = *n_act; *ncol = 0; }
+    if (p <= N) { deg[p] = 0; cur[p] = 0; }
+    if (p < P && flag[p]) { act[off[p]] = p; colour[off[p]] = -1; cround[off[p]] = 0xffffffffu; }
+}
+__global__ void k_act_deg(const uint32_t *act, const uint32_t *n_act, const uint2 *pbod, Bodies B, uint32_t *deg) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_act) return;
+    uint2 b = pbod[act[i]];
+    if (B.Ii1[b.x].z != 0.0f) atomicAdd(&deg[b.x], 1u);
+    if (B.Ii1[b.y].z != 0.0f) atomicAdd(&deg[b.y], 1u);
+}
+__global__ void k_act_adj(const uint32_t *act, const uint32_t *n_act, const uint2 *pbod, Bodies B, const uint32_t *aoff,
+                          uint32_t *cur, uint32_t *adj) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_act) return;
+    uint2 b = pbod[act[i]];
+    if (B.Ii1[b.x].z != 0.0f) adj[aoff[b.x] + atomicAdd(&cur[b.x], 1u)] = i;
+    if (B.Ii1[b.y].z != 0.0f) adj[aoff[b.y] + atomicAdd(&cur[b.y], 1u)] = i;
+}
+
+__device__ __forceinline__ bool higher_priority(uint32_t q, uint32_t p) {
+    const uint32_t hq = mix32(q), hp = mix32(p);
+    return hq > hp || (hq == hp && q > p);
+}
+// The idx-th colour not set in `used`.
+__device__ __forceinline__ int nth_free(const uint64_t used[MAX_COLOURS / 64], uint32_t idx) {
+    for (int w = 0; w < MAX_COLOURS / 64; ++w) {
+        uint64_t fr = ~used[w];
+        const uint32_t n = (uint32_t)__popcll(fr);
+        if (idx < n) {
+            for (; idx; --idx) fr &= fr - 1;
+            return w * 64 + __builtin_ctzll(fr);
+        }
+        idx -= n;
+    }
+    return -1;
+}
+
+// Colouring of the active pairs by synchronous randomized trials, all rounds in one
+// cooperative launch.  Round r: (A) every uncoloured vertex i (pair p = act[i]) picks
+// t = the (mix32(p ^ r*0x9e3779b9) mod k)-th colour unused by neighbours coloured in
+// earlier rounds, k = uncoloured neighbour entries + 1; (B) it keeps t unless an
+// uncoloured neighbour of higher priority (mix32(q), q) picked t too.  cround[i] holds
+// the round a vertex was coloured in (0xffffffff = not yet), so "uncoloured at the start
+// of round r" is cround >= r in both phases.  Every round colours at least the highest-
+// priority uncoloured vertex; the expected number of rounds is O(log degree).
+__global__ void __launch_bounds__(256) k_colour_coop(const uint32_t *act, const uint32_t *n_act, const uint2 *pbod, Bodies B,
+                                                     const uint32_t *aoff, const uint32_t *adeg, const uint32_t *adj,
+                                                     int *colour, int *tent, uint32_t *cround, uint32_t *uncoloured,
+                                                     uint32_t *ncol, int *err, GridBar bar, uint32_t max_rounds,
+                                                     uint32_t *rounds_out) {
+    const uint32_t n = *n_act;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t r = 0;; ++r) {
+        // phase A: tentative colours
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            if (cround[i] < r) continue;
+            const uint32_t p = act[i];
+            uint64_t used[MAX_COLOURS / 64] = {0, 0, 0, 0};
+            uint32_t k = 1;
+            const uint2 bb = pbod[p];
+            for (int sd = 0; sd < 2; ++sd) {
+                const uint32_t b = sd ? bb.y : bb.x;
+                if (B.Ii1[b].z == 0.0f) continue;
+                const uint32_t o = aoff[b], nn = adeg[b];
+                for (uint32_t e = 0; e < nn; ++e) {
+                    const uint32_t j = adj[o + e];
+                    if (j == i) continue;
+                    if (cround[j] < r) {
+                        const int c = colour[j];
+                        used[c >> 6] |= 1ull << (c & 63);
+                    } else {
+                        ++k;
+                    }
+                }
+            }
+            int t = nth_free(used, mix32(p ^ (r * 0x9e3779b9u)) % k);
+            if (t < 0) { *err = 1; t = MAX_COLOURS - 1; }
+            tent[i] = t;
+        }
+        grid_barrier(bar, nullptr);
+        // phase B: keep the tentative colour unless a higher-priority neighbour chose it too
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            if (cround[i] < r) continue;
+            const uint32_t p = act[i];
+            const int t = tent[i];
+            const uint2 bb = pbod[p];
+            bool ok = true;
+            for (int sd = 0; sd < 2 && ok; ++sd) {
+                const uint32_t b = sd ? bb.y : bb.x;
+                if (B.Ii1[b].z == 0.0f) continue;
+                const uint32_t o = aoff[b], nn = adeg[b];
+                for (uint32_t e = 0; e < nn; ++e) {
+                    const uint32_t j = adj[o + e];
+                    if (j == i) continue;
+                    if (__hip_atomic_load(&cround[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= r && tent[j] == t &&
+                        higher_priority(act[j], p)) { ok = false; break; }
+                }
+            }
+            if (ok) {
+                colour[i] = t;
+                __hip_atomic_store(&cround[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicSub(uncoloured, 1u);
+                atomicMax(ncol, (uint32_t)(t + 1));
+            }
+        }
+        const int done = grid_barrier(bar, uncoloured);
+        if (done) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) *rounds_out = r + 1;
+            break;
+        }
+        if (r + 1 >= max_rounds) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { *err = 2; *rounds_out = r + 1; }
+            break;
+        }
+    }
+}
+
+__global__ void k_colour_keys(const int *colour, const uint32_t *act, const uint32_t *n_act, uint32_t *key, uint32_t *val) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_act) return;
+    key[i] = (uint32_t)colour[i];
+    val[i] = act[i];
 }
 
 // ------------------------------------------------------------------ narrowphase
@@ -434,10 +605,7 @@ __device__ __forceinline__ void apply_pos(BodyReg &r, V3 imp, V3 rr) {   // Body
     r.q = qstep(r.q, rot);
 }
 
-__global__ void __launch_bounds__(256) k_solve_pos(Bodies B, Contacts K, uint32_t off, uint32_t count, float inv_h) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const uint32_t s = off + t;
+__device__ __forceinline__ void solve_pos_pair(const Bodies &B, Contacts &K, uint32_t s) {
     const int np = K.cnt[s];
     if (np == 0) return;
     const uint2 bb = K.bb[s];
@@ -544,10 +712,7 @@ __device__ __forceinline__ void apply_vel(VelReg &r, V3 imp, V3 rr) {   // Body.
     r.w = r.w + smv(r.Iwi, cross(rr, imp));
 }
 
-__global__ void __launch_bounds__(256) k_solve_vel(Bodies B, Contacts K, uint32_t off, uint32_t count, float h) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const uint32_t s = off + t;
+__device__ __forceinline__ void solve_vel_pair(const Bodies &B, Contacts &K, uint32_t s, float h) {
     const int np = K.cnt[s];
     if (np == 0) return;
     const uint2 bb = K.bb[s];
@@ -602,6 +767,29 @@ __global__ void __launch_bounds__(256) k_solve_vel(Bodies B, Contacts K, uint32_
     if (!any) return;
     if (A.dyn) { B.v[bb.x] = f4(A.v, 0.0f); B.w[bb.x] = f4(A.w, 0.0f); }
     if (Bd.dyn) { B.v[bb.y] = f4(Bd.v, 0.0f); B.w[bb.y] = f4(Bd.w, 0.0f); }
+}
+
+// Position / velocity solve of one substep: colour after colour, all in one cooperative
+// launch; pairs of one colour touch disjoint dynamic bodies (grid barrier between colours).
+__global__ void __launch_bounds__(256) k_solve_pos_coop(Bodies B, Contacts K, const uint32_t *order, const uint32_t *col_off,
+                                                        const uint32_t *ncol, GridBar bar) {
+    const uint32_t nc = *ncol;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t c = 0; c < nc; ++c) {
+        const uint32_t o = col_off[c], e = col_off[c + 1];
+        for (uint32_t t = o + blockIdx.x * blockDim.x + threadIdx.x; t < e; t += stride) solve_pos_pair(B, K, order[t]);
+        if (c + 1 < nc) grid_barrier(bar, nullptr);
+    }
+}
+__global__ void __launch_bounds__(256) k_solve_vel_coop(Bodies B, Contacts K, const uint32_t *order, const uint32_t *col_off,
+                                                        const uint32_t *ncol, float h, GridBar bar) {
+    const uint32_t nc = *ncol;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t c = 0; c < nc; ++c) {
+        const uint32_t o = col_off[c], e = col_off[c + 1];
+        for (uint32_t t = o + blockIdx.x * blockDim.x + threadIdx.x; t < e; t += stride) solve_vel_pair(B, K, order[t], h);
+        if (c + 1 < nc) grid_barrier(bar, nullptr);
+    }
 }
 
 }  // namespace rb

@@ -52,7 +52,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const uint32_t *__restrict__ in, const uint32_t *n_dev,
                                                               uint32_t n_host, uint32_t *__restrict__ tile_sums) {
     __shared__ uint32_t sh[SCAN_THREADS / 64 + 1];
-    const uint32_t n = n_dev ? *n_dev : n_host;
+    const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t s = 0;
     for (int k = 0; k < SCAN_ITEMS; ++k)
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(SCAN_TOP_THREADS) k_scan_top(uint32_t *__restr
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t *in, const uint32_t *n_dev, uint32_t n_host,
                                                              const uint32_t *__restrict__ tile_sums, uint32_t *out) {
     __shared__ uint32_t sh[SCAN_THREADS / 64 + 1];
-    const uint32_t n = n_dev ? *n_dev : n_host;
+    const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint32_t *__restri
                                                         const int *skip) {
     if (skip && *skip == 0) return;
     __shared__ uint32_t h[RS_BINS];
-    const uint32_t n = n_dev ? *n_dev : n_host;
+    const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     for (int i = threadIdx.x; i < RS_BINS; i += RS_THREADS) h[i] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * RS_TILE;
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint32_t *__res
     if (skip && *skip == 0) return;
     __shared__ uint32_t base_d[RS_BINS];           // global offset of this tile's next element per digit
     __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];   // this round's per-wave digit counts
-    const uint32_t n = n_dev ? *n_dev : n_host;
+    const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     for (int d = t; d < RS_BINS; d += RS_THREADS) {
         base_d[d] = hoff[d * ntiles + blockIdx.x];

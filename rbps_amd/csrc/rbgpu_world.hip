@@ -51,7 +51,7 @@ struct Readback {
     uint32_t n_sap;
     uint32_t n_pairs;
     uint32_t uncoloured;
-    int32_t dirty, overflow, colour_err;
+    int32_t dirty, overflow, colour_err, barrier_fault;
     uint32_t col_off[MAX_COLOURS + 1];
     uint32_t bin_off[NBINS + 1];
     uint64_t points;
@@ -101,21 +101,26 @@ struct rbgpu_world {
     float2 *sw_b = nullptr;
     uint32_t *ep_cnt = nullptr, *ep_off = nullptr;
     uint32_t *ep_hist = nullptr;
-    int *flags = nullptr;            // [0] dirty [1] overflow [2] colour error
-    uint32_t *scal = nullptr;        // [0] n_sap [1] n_pairs [2] uncoloured
+    int *flags = nullptr;            // [0] dirty [1] overflow [2] colour error [3] barrier fault
+    uint32_t *scal = nullptr;        // [0] n_sap [1] n_pairs [2] uncoloured [3] active pairs [4] colours [5] JP rounds
     uint32_t *tile_sums = nullptr;   // scan scratch
     uint32_t *body_deg = nullptr, *body_off = nullptr, *body_cur = nullptr;
     uint2 *plane_pairs = nullptr;
     uint32_t n_plane = 0;
     unsigned long long *d_points = nullptr;
 
-    // pair stage (capacity pcap)
+    // pair stage (capacity pcap); contact rows are indexed by pair id
     uint32_t pcap = 0;
-    uint2 *pairs = nullptr, *pbod = nullptr;
+    uint2 *pairs = nullptr;
     uint32_t *adj = nullptr;
-    int *colour = nullptr;
-    uint32_t *ka = nullptr, *va = nullptr, *kb = nullptr, *slot_pair = nullptr, *bin_slot = nullptr;
-    uint32_t *p_hist = nullptr, *offs = nullptr;
+    int *colour = nullptr;           // per active index
+    int *tent = nullptr;             // tentative colour per active index
+    uint32_t *cround = nullptr;      // colouring round per active index
+    uint32_t *ka = nullptr, *va = nullptr, *kb = nullptr, *solve_list = nullptr, *bin_list = nullptr;
+    uint32_t *act_flag = nullptr, *act_off = nullptr, *act_list = nullptr;
+    uint32_t *p_hist = nullptr, *offs = nullptr;   // offs: [0..256] colour offsets, [257..267] bin offsets
+    uint32_t *bar = nullptr;         // grid barrier: count, generation, decision
+    uint32_t coop_jp = 0, coop_pos = 0, coop_vel = 0;   // cooperative grid sizes
 
     Readback *rb = nullptr;          // pinned
     uint32_t n_pairs = 0, n_sap = 0, ncol = 0, jp_rounds = 0;
@@ -132,8 +137,8 @@ struct rbgpu_world {
     ~rbgpu_world() { release(); }
 
     void release_pairs() {
-        dfree(pairs); dfree(pbod); dfree(adj); dfree(colour); dfree(ka); dfree(va); dfree(kb);
-        dfree(slot_pair); dfree(bin_slot); dfree(p_hist);
+        dfree(pairs); dfree(adj); dfree(colour); dfree(tent); dfree(cround); dfree(ka); dfree(va); dfree(kb);
+        dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(act_flag); dfree(act_off); dfree(act_list);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
         pcap = 0;
@@ -147,7 +152,7 @@ struct rbgpu_world {
         dfree(fmin); dfree(fmax); dfree(uid); dfree(ep_key[0]); dfree(ep_key[1]); dfree(ep_val[0]); dfree(ep_val[1]);
         dfree(ep_pos); dfree(sw_meta); dfree(sw_a); dfree(sw_b); dfree(ep_cnt); dfree(ep_off); dfree(ep_hist);
         dfree(flags); dfree(scal); dfree(tile_sums); dfree(body_deg); dfree(body_off); dfree(body_cur);
-        dfree(plane_pairs); dfree(offs); dfree(d_points);
+        dfree(plane_pairs); dfree(offs); dfree(d_points); dfree(bar);
         for (auto &e : ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         ev_pool.clear();
         if (rb) (void)hipHostFree(rb);
@@ -166,14 +171,24 @@ struct rbgpu_world {
         k_scan_apply<<<nt, SCAN_THREADS, 0, st>>>(in, nullptr, n, tile_sums, out);
         return RBGPU_OK;
     }
-    // One stable 8-bit radix pass (k0,v0) -> (k1,v1) on bits [shift, shift+8).
+    // Scan of a device-counted array (n_max bounds the launch).
+    int scan_dev(const uint32_t *in, uint32_t *out, const uint32_t *n_dev, uint32_t n_max, uint32_t *total) {
+        uint32_t nt = std::max<uint32_t>(1, (n_max + SCAN_TILE - 1) / SCAN_TILE);
+        if (nt > (uint32_t)(SCAN_TOP_THREADS * SCAN_TOP_ITEMS)) return fail(RBGPU_ERR_CAPACITY, "scan too large");
+        k_scan_reduce<<<nt, SCAN_THREADS, 0, st>>>(in, n_dev, n_max, tile_sums);
+        k_scan_top<<<1, SCAN_TOP_THREADS, 0, st>>>(tile_sums, nt, total);
+        k_scan_apply<<<nt, SCAN_THREADS, 0, st>>>(in, n_dev, n_max, tile_sums, out);
+        return RBGPU_OK;
+    }
+    // One stable 8-bit radix pass (k0,v0) -> (k1,v1) on bits [shift, shift+8); the element
+    // count is n (or *n_dev when given, bounded by n).
     int radix_pass(const uint32_t *k0, const uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, int shift,
-                   uint32_t *hist, const int *skip) {
+                   uint32_t *hist, const int *skip, const uint32_t *n_dev = nullptr) {
         uint32_t nt = std::max<uint32_t>(1, (n + RS_TILE - 1) / RS_TILE);
-        k_rs_hist<<<nt, RS_THREADS, 0, st>>>(k0, nullptr, n, shift, nt, hist, skip);
+        k_rs_hist<<<nt, RS_THREADS, 0, st>>>(k0, n_dev, n, shift, nt, hist, skip);
         int rc = scan(hist, hist, nt * RS_BINS, nullptr);
         if (rc) return rc;
-        k_rs_scatter<<<nt, RS_THREADS, 0, st>>>(k0, v0, k1, v1, nullptr, n, shift, nt, hist, skip);
+        k_rs_scatter<<<nt, RS_THREADS, 0, st>>>(k0, v0, k1, v1, n_dev, n, shift, nt, hist, skip);
         return RBGPU_OK;
     }
 
@@ -206,9 +221,10 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     release_pairs();
     uint32_t cap = std::max<uint32_t>(need + need / 2, 4096);
     int rc = 0;
-    rc |= dalloc(pairs, cap); rc |= dalloc(pbod, cap); rc |= dalloc(adj, 2 * (size_t)cap); rc |= dalloc(colour, cap);
-    rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(slot_pair, cap);
-    rc |= dalloc(bin_slot, cap);
+    rc |= dalloc(pairs, cap); rc |= dalloc(adj, 2 * (size_t)cap); rc |= dalloc(colour, cap);
+    rc |= dalloc(tent, cap); rc |= dalloc(cround, cap);
+    rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
+    rc |= dalloc(bin_list, cap); rc |= dalloc(act_flag, cap); rc |= dalloc(act_off, cap); rc |= dalloc(act_list, cap);
     uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
     rc |= dalloc(p_hist, (size_t)nt * RS_BINS);
     rc |= dalloc(K.cnt, cap); rc |= dalloc(K.cc, cap); rc |= dalloc(K.bb, cap); rc |= dalloc(K.nr, cap);
@@ -284,12 +300,30 @@ int rbgpu_world::build() {
     rc |= dalloc(ep_pos, S); rc |= dalloc(sw_meta, NEP); rc |= dalloc(sw_a, NEP); rc |= dalloc(sw_b, NEP);
     rc |= dalloc(ep_cnt, NEP + 1); rc |= dalloc(ep_off, NEP + 1);
     rc |= dalloc(ep_hist, (size_t)((NEP + RS_TILE - 1) / RS_TILE) * RS_BINS);
-    rc |= dalloc(flags, 4); rc |= dalloc(scal, 4);
+    rc |= dalloc(flags, 4); rc |= dalloc(scal, 8);
     rc |= dalloc(tile_sums, SCAN_TOP_THREADS * SCAN_TOP_ITEMS);
     rc |= dalloc(body_deg, n + 1); rc |= dalloc(body_off, n + 1); rc |= dalloc(body_cur, n + 1);
     rc |= dalloc(plane_pairs, planes.size()); rc |= dalloc(offs, MAX_COLOURS + NBINS + 4);
-    rc |= dalloc(d_points, 1);
+    rc |= dalloc(d_points, 1); rc |= dalloc(bar, 4);
     if (rc) return RBGPU_ERR_OOM;
+    {
+        int dev = 0, ncu = 0, coop = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+        if (!coop) return fail(RBGPU_ERR_UNSUPPORTED, "device does not support cooperative launches");
+        auto grid_for = [&](const void *fn, uint32_t &out) -> int {
+            int nb = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0));
+            if (nb < 1) return fail(RBGPU_ERR_UNSUPPORTED, "cooperative kernel does not fit on a CU");
+            out = (uint32_t)ncu * (uint32_t)std::min(nb, 2);
+            return RBGPU_OK;
+        };
+        int r = grid_for((const void *)k_colour_coop, coop_jp);
+        if (!r) r = grid_for((const void *)k_solve_pos_coop, coop_pos);
+        if (!r) r = grid_for((const void *)k_solve_vel_coop, coop_vel);
+        if (r) return r;
+    }
     auto up = [&](void *dst, const void *src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st); };
     if (n) {
         HIPCHK(up(B.x, x.data(), n * 16)); HIPCHK(up(B.q, q.data(), n * 16)); HIPCHK(up(B.v, v.data(), n * 16));
@@ -310,8 +344,9 @@ int rbgpu_world::build() {
     if (n_plane) HIPCHK(up(plane_pairs, planes.data(), n_plane * 8));
     int init_flags[4] = {1, 0, 0, 0};   // dirty: the endpoint array has never been sorted
     HIPCHK(up(flags, init_flags, sizeof(init_flags)));
-    HIPCHK(hipMemsetAsync(scal, 0, 16, st));
+    HIPCHK(hipMemsetAsync(scal, 0, 32, st));
     HIPCHK(hipMemsetAsync(d_points, 0, 8, st));
+    HIPCHK(hipMemsetAsync(bar, 0, 16, st));
     HIPCHK(hipStreamSynchronize(st));
     int rc2 = ensure_pair_capacity(std::max<uint32_t>(8 * S + n_plane, 4096));
     if (rc2) return rc2;
@@ -379,63 +414,41 @@ int rbgpu_world::frame() {
     } else {
         HIPCHK(hipMemsetAsync(scal, 0, 4, st));
     }
-    HIPCHK(hipMemcpyAsync(&rb->n_sap, scal, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    n_sap = rb->n_sap;
-    n_pairs = n_sap + n_plane;
-    if (n_pairs > pcap) {
-        int rc = ensure_pair_capacity(n_pairs);
+    // 3. pair list: SAP pairs (canonical order) then the plane pairs; per-pair data and
+    //    narrowphase bins.  One host synchronisation per frame: the pair count (capacity
+    //    check) and the bin offsets.
+    for (int attempt = 0;; ++attempt) {
+        HIPCHK(hipMemsetAsync(overflow, 0, 4, st));
+        if (NEP)
+            k_sweep<true><<<nblk(NEP), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, NEP, nullptr, ep_off, pairs,
+                                                     pcap, overflow);
+        k_append<<<nblk(std::max<uint32_t>(n_plane, 1)), TPB, 0, st>>>(pairs, scal + 0, plane_pairs, n_plane, pcap,
+                                                                        scal + 1, overflow);
+        k_pair_gather<<<nblk(pcap), TPB, 0, st>>>(pairs, scal + 1, C, K, ka, va);
+        int rc = radix_pass(ka, va, kb, bin_list, pcap, 0, p_hist, nullptr, scal + 1);
         if (rc) return rc;
-    }
-    const uint32_t P = n_pairs;
-    if (NEP)
-        k_sweep<true><<<nblk(NEP), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, NEP, nullptr, ep_off, pairs, pcap,
-                                                 overflow);
-    k_append<<<nblk(std::max<uint32_t>(n_plane, 1)), TPB, 0, st>>>(pairs, scal + 0, plane_pairs, n_plane, pcap, scal + 1,
-                                                                    overflow);
-
-    // 4. Jones-Plassmann colouring of the pair graph (edges = shared dynamic bodies)
-    ncol = 0;
-    jp_rounds = 0;
-    if (P) {
-        k_fill_u32<<<nblk(N + 1), TPB, 0, st>>>(body_deg, N + 1, 0u);
-        k_fill_u32<<<nblk(N + 1), TPB, 0, st>>>(body_cur, N + 1, 0u);
-        k_pair_bodies<<<nblk(P), TPB, 0, st>>>(pairs, scal + 1, C, B, pbod, body_deg, colour, scal + 2);
-        int rc = scan(body_deg, body_off, N + 1, nullptr);
-        if (rc) return rc;
-        k_fill_adj<<<nblk(P), TPB, 0, st>>>(pbod, scal + 1, B, body_off, body_cur, adj);
-        for (;;) {
-            for (int r = 0; r < 8; ++r)
-                k_jp_round<<<nblk(P), TPB, 0, st>>>(pbod, scal + 1, B, body_off, body_deg, adj, colour, scal + 2, colerr);
-            jp_rounds += 8;
-            HIPCHK(hipMemcpyAsync(&rb->uncoloured, scal + 2, 4, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (rb->uncoloured == 0) break;
-            if (jp_rounds > 4096) return fail(RBGPU_ERR_CAPACITY, "colouring did not converge");
-        }
-        // 5. solve slots = pairs stably sorted by colour; narrowphase bins = slots sorted by pair type
-        k_colour_keys<<<nblk(P), TPB, 0, st>>>(colour, scal + 1, ka, va);
-        rc = radix_pass(ka, va, kb, slot_pair, P, 0, p_hist, nullptr);
-        if (rc) return rc;
-        uint32_t nt = (P + RS_TILE - 1) / RS_TILE;
-        k_offsets<<<1, MAX_COLOURS, 0, st>>>(p_hist, nt, MAX_COLOURS, scal + 1, offs);
-        k_slot_gather<<<nblk(P), TPB, 0, st>>>(slot_pair, scal + 1, pairs, pbod, C, K, ka, va);
-        rc = radix_pass(ka, va, kb, bin_slot, P, 0, p_hist, nullptr);
-        if (rc) return rc;
-        k_offsets<<<1, 64, 0, st>>>(p_hist, nt, NBINS, scal + 1, offs + MAX_COLOURS + 1);
-        HIPCHK(hipMemcpyAsync(rb->col_off, offs, (MAX_COLOURS + 1) * 4, hipMemcpyDeviceToHost, st));
+        k_offsets<<<1, 64, 0, st>>>(p_hist, (pcap + RS_TILE - 1) / RS_TILE, NBINS, scal + 1, offs + MAX_COLOURS + 1);
+        HIPCHK(hipMemcpyAsync(&rb->n_sap, scal, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&rb->overflow, overflow, 12, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(rb->bin_off, offs + MAX_COLOURS + 1, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(&rb->overflow, overflow, 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (rb->overflow) return fail(RBGPU_ERR_CAPACITY, "pair buffer overflow");
-        if (rb->colour_err) return fail(RBGPU_ERR_CAPACITY, "more than 256 colours needed");
-        col_off.assign(rb->col_off, rb->col_off + MAX_COLOURS + 1);
-        bin_off.assign(rb->bin_off, rb->bin_off + NBINS + 1);
-        for (int c = 0; c < MAX_COLOURS; ++c)
-            if (col_off[c + 1] > col_off[c]) ncol = c + 1;
+        if (rb->barrier_fault) return fail(RBGPU_ERR_HIP, "grid barrier timed out in a cooperative kernel");
+        if (rb->colour_err) return fail(RBGPU_ERR_CAPACITY, rb->colour_err == 1 ? "more than 256 colours needed"
+                                                                                 : "colouring did not converge");
+        if (!rb->overflow) break;
+        if (attempt > 0) return fail(RBGPU_ERR_CAPACITY, "pair buffer overflow");
+        int rc2 = ensure_pair_capacity(rb->n_pairs);
+        if (rc2) return rc2;
     }
+    n_sap = rb->n_sap;
+    n_pairs = rb->n_pairs;
+    bin_off.assign(rb->bin_off, rb->bin_off + NBINS + 1);
+    const uint32_t P = n_pairs;
+    const uint32_t ntp = (std::max<uint32_t>(P, 1) + RS_TILE - 1) / RS_TILE;
+    const uint32_t colour_rounds_cap = 256;
+    GridBar gb{bar, bar + 1, (int *)(bar + 2), flags + 3};
 
-    // 6. substeps (World.hpp:203-211)
+    // 4. substeps (World.hpp:203-211)
     for (int s = 0; s < substeps; ++s) {
         if (P) {
             hipEvent_t *e = timed_begin(3);
@@ -444,30 +457,61 @@ int rbgpu_world::frame() {
                 if (!n) continue;
                 const uint32_t g = nblk(n, 128);
                 switch (b) {
-                    case BIN_SS: k_narrow<BIN_SS><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    case BIN_SB: k_narrow<BIN_SB><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    case BIN_BB: k_narrow<BIN_BB><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    case BIN_SP: k_narrow<BIN_SP><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
-                    default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_slot, o, n, C, B, K); break;
+                    case BIN_SS: k_narrow<BIN_SS><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_SB: k_narrow<BIN_SB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_BB: k_narrow<BIN_BB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_SP: k_narrow<BIN_SP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                 }
             }
             timed_end(e);
+            // colouring of the active pairs (edges = shared dynamic bodies)
+            k_active_flags<<<nblk(P), TPB, 0, st>>>(K.cnt, P, act_flag);
+            int rc = scan(act_flag, act_off, P, scal + 3);
+            if (rc) return rc;
+            k_compact<<<nblk(std::max(P, N + 1)), TPB, 0, st>>>(act_flag, act_off, P, scal + 3, act_list, colour, cround,
+                                                                 scal + 2, scal + 4, body_deg, body_cur, N);
+            k_act_deg<<<nblk(P), TPB, 0, st>>>(act_list, scal + 3, K.bb, B, body_deg);
+            rc = scan(body_deg, body_off, N + 1, nullptr);
+            if (rc) return rc;
+            k_act_adj<<<nblk(P), TPB, 0, st>>>(act_list, scal + 3, K.bb, B, body_off, body_cur, adj);
+            {
+                const uint32_t *a0 = act_list, *a1 = scal + 3;
+                const uint2 *a2 = K.bb;
+                Bodies a3 = B;
+                const uint32_t *a4 = body_off, *a5 = body_deg, *a6 = adj;
+                int *a7 = colour, *a8 = tent;
+                uint32_t *a9 = cround, *a10 = scal + 2, *a11 = scal + 4;
+                int *a12 = colerr;
+                GridBar a13 = gb;
+                uint32_t a14 = colour_rounds_cap;
+                uint32_t *a15 = scal + 5;
+                void *args[] = {&a0, &a1, &a2, &a3, &a4, &a5, &a6, &a7, &a8, &a9, &a10, &a11, &a12, &a13, &a14, &a15};
+                HIPCHK(hipLaunchCooperativeKernel((const void *)k_colour_coop, dim3(coop_jp), dim3(256), args, 0, st));
+            }
+            k_colour_keys<<<nblk(P), TPB, 0, st>>>(colour, act_list, scal + 3, ka, va);
+            rc = radix_pass(ka, va, kb, solve_list, P, 0, p_hist, nullptr, scal + 3);
+            if (rc) return rc;
+            k_offsets<<<1, MAX_COLOURS, 0, st>>>(p_hist, ntp, MAX_COLOURS, scal + 3, offs);
         }
         if (N) {
             hipEvent_t *e = timed_begin(4);
             k_integrate<<<nblk(N), TPB, 0, st>>>(B, N, h);
             timed_end(e);
         }
-        for (uint32_t c = 0; c < ncol; ++c) {
-            uint32_t o = col_off[c], n = col_off[c + 1] - col_off[c];
-            if (!n) continue;
+        if (P) {
+            const uint32_t *a2 = solve_list, *a3 = offs, *a4 = scal + 4;
+            Bodies a0 = B;
+            Contacts a1 = K;
+            GridBar a5 = gb;
+            void *args[] = {&a0, &a1, &a2, &a3, &a4, &a5};
             hipEvent_t *e = timed_begin(1);
-            k_solve_pos<<<nblk(n), TPB, 0, st>>>(B, K, o, n, inv_h);
+            HIPCHK(hipLaunchCooperativeKernel((const void *)k_solve_pos_coop, dim3(coop_pos), dim3(256), args, 0, st));
             timed_end(e);
         }
         if (N) {
@@ -475,11 +519,15 @@ int rbgpu_world::frame() {
             k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h);
             timed_end(e);
         }
-        for (uint32_t c = 0; c < ncol; ++c) {
-            uint32_t o = col_off[c], n = col_off[c + 1] - col_off[c];
-            if (!n) continue;
+        if (P) {
+            const uint32_t *a2 = solve_list, *a3 = offs, *a4 = scal + 4;
+            Bodies a0 = B;
+            Contacts a1 = K;
+            float a5 = h;
+            GridBar a6 = gb;
+            void *args[] = {&a0, &a1, &a2, &a3, &a4, &a5, &a6};
             hipEvent_t *e = timed_begin(2);
-            k_solve_vel<<<nblk(n), TPB, 0, st>>>(B, K, o, n, h);
+            HIPCHK(hipLaunchCooperativeKernel((const void *)k_solve_vel_coop, dim3(coop_vel), dim3(256), args, 0, st));
             timed_end(e);
         }
         if (timing_kernel == 1 || timing_kernel == 2) {
@@ -655,9 +703,22 @@ int rbgpu_get_pair_colours(rbgpu_world *w, uint32_t *pairs2, int32_t *colour, ui
     *n = w->built ? w->n_pairs : 0;
     if (!w->built) return RBGPU_OK;
     HIPCHK(hipStreamSynchronize(w->st));
-    uint32_t k = std::min(cap, w->n_pairs);
+    const uint32_t k = std::min(cap, w->n_pairs);
     if (k && pairs2) HIPCHK(hipMemcpy(pairs2, w->pairs, (size_t)k * 8, hipMemcpyDeviceToHost));
-    if (k && colour) HIPCHK(hipMemcpy(colour, w->colour, (size_t)k * 4, hipMemcpyDeviceToHost));
+    if (k && colour) {
+        // colour of each pair in the last substep's colouring of the active pairs; -1 = inactive
+        uint32_t na = 0;
+        HIPCHK(hipMemcpy(&na, w->scal + 3, 4, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> act(na);
+        std::vector<int> col(na);
+        if (na) {
+            HIPCHK(hipMemcpy(act.data(), w->act_list, (size_t)na * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(col.data(), w->colour, (size_t)na * 4, hipMemcpyDeviceToHost));
+        }
+        for (uint32_t p = 0; p < k; ++p) colour[p] = -1;
+        for (uint32_t i = 0; i < na; ++i)
+            if (act[i] < k) colour[act[i]] = col[i];
+    }
     return RBGPU_OK;
 }
 
@@ -668,12 +729,10 @@ int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *n3, double *ra3, 
     if (!w->built || !w->n_pairs) return RBGPU_OK;
     HIPCHK(hipStreamSynchronize(w->st));
     const uint32_t P = w->n_pairs;
-    std::vector<uint32_t> sp(P);
     std::vector<int> cnt(P);
     std::vector<uint2> cc(P), bb(P);
     std::vector<float4> nr(P), ra(4 * (size_t)P), rbv(4 * (size_t)P);
     std::vector<float2> lam(4 * (size_t)P);
-    HIPCHK(hipMemcpy(sp.data(), w->slot_pair, P * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(cnt.data(), w->K.cnt, P * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(cc.data(), w->K.cc, P * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(bb.data(), w->K.bb, P * 8, hipMemcpyDeviceToHost));
@@ -681,11 +740,8 @@ int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *n3, double *ra3, 
     HIPCHK(hipMemcpy(ra.data(), w->K.ra, (size_t)P * 64, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(rbv.data(), w->K.rb, (size_t)P * 64, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(lam.data(), w->K.lam, (size_t)P * 32, hipMemcpyDeviceToHost));
-    std::vector<uint32_t> slot_of(P);
-    for (uint32_t s = 0; s < P; ++s) slot_of[sp[s]] = s;
     uint32_t k = 0;
-    for (uint32_t p = 0; p < P; ++p) {
-        uint32_t s = slot_of[p];
+    for (uint32_t s = 0; s < P; ++s) {
         for (int i = 0; i < cnt[s]; ++i, ++k) {
             if (k >= cap) continue;
             size_t j = (size_t)s * 4 + i;
@@ -707,8 +763,13 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out) {
     out->frames = w->frames;
     out->sap_pairs = w->n_sap;
     out->pairs = w->n_pairs;
-    out->colours = w->ncol;
-    out->jp_rounds = w->jp_rounds;
+    if (w->built) {
+        HIPCHK(hipStreamSynchronize(w->st));
+        uint32_t sc[8];
+        HIPCHK(hipMemcpy(sc, w->scal, sizeof(sc), hipMemcpyDeviceToHost));
+        out->colours = sc[4];
+        out->jp_rounds = sc[5];
+    }
     for (int b = 0; b < NBINS && (size_t)b + 1 < w->bin_off.size(); ++b) out->bins[b] = w->bin_off[b + 1] - w->bin_off[b];
     if (w->built && w->n_pairs) {
         HIPCHK(hipStreamSynchronize(w->st));

@@ -5,7 +5,7 @@ import pytest
 
 from oracle import orc
 from rbps_amd import scenes
-from tests.helpers import colouring_valid, greedy_by_priority, sorted_pairs
+from tests.helpers import colouring_valid, sorted_pairs, trial_colouring
 
 
 def test_canonical_sweep_same_set_as_active_set_sweep():
@@ -60,17 +60,26 @@ def test_pair_set_brute_force_first_frame():
     assert np.array_equal(got, sorted_pairs(bf))
 
 
-def test_colour_order_colouring_is_valid_and_greedy():
+@pytest.mark.parametrize("frames", [2, 6])
+def test_colour_order_colouring_is_valid_and_replayed(frames):
+    """The oracle's substep colouring equals an independent Python replay of the
+    randomized-trial spec, is valid, and uses at most (max degree + 1) colours."""
     sc = scenes.pile_mixed_dense()
     w = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
-    w.step(2)
+    w.step(frames)
     sp = w.solve_pairs()
     cols = sp["colliders"]
     bodies = sc.colliders["body_id"][cols]
     dyn = sc.bodies["type"] == scenes.DYNAMIC
+    active = sp["count"] > 0
+    assert active.sum() > 20
     assert colouring_valid(bodies, sp["colour"], dyn)
-    assert np.array_equal(sp["colour"], greedy_by_priority(bodies, dyn, orc.mix32))
+    replay, rounds = trial_colouring(bodies, dyn, active, orc.mix32)
+    assert np.array_equal(sp["colour"], replay)
+    assert (sp["colour"][~active] == -1).all()
     assert sp["ncolours"] == sp["colour"].max() + 1
+    body_deg = np.bincount(bodies[active].ravel(), minlength=len(dyn))[dyn].max()
+    assert sp["ncolours"] <= 2 * body_deg - 1       # a pair has at most 2*(body degree - 1) neighbours
 
 
 def test_mix32_known_values():
