@@ -24,7 +24,7 @@ enum {
     RBGPU_OK = 0,
     RBGPU_ERR_INVALID = -1,   /* bad argument or handle */
     RBGPU_ERR_HIP = -2,       /* a HIP runtime call failed */
-    RBGPU_ERR_CAPACITY = -3,  /* a fixed capacity was exceeded (e.g. > 256 colours) */
+    RBGPU_ERR_CAPACITY = -3,  /* a fixed capacity was exceeded (e.g. > 1024 colours) */
     RBGPU_ERR_OOM = -4,       /* device allocation failed */
     RBGPU_ERR_UNSUPPORTED = -5
 };
@@ -64,10 +64,11 @@ typedef struct {
     uint32_t frames;             /* World::step calls executed */
     uint32_t sap_pairs;          /* broadphase pairs of the last frame (after the pipeline's filters) */
     uint32_t pairs;              /* + plane pairs = the frame's narrowphase pair list */
-    uint32_t colours;            /* colours of the last frame's pair graph */
-    uint32_t jp_rounds;          /* Jones-Plassmann rounds launched for the last frame */
+    uint32_t colours;            /* colours of the last substep's colouring */
+    uint32_t colour_rounds;      /* colouring rounds of the last substep */
     uint32_t contacts;           /* contact points of the last substep */
     uint32_t bins[10];           /* pairs per narrowphase bin of the last frame */
+    uint32_t active_pairs;       /* pairs with contact points in the last substep */
 } rbgpu_stats;
 
 typedef struct rbgpu_world rbgpu_world;
@@ -121,9 +122,34 @@ int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *normal3, double *
 int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
 
 /* Kernel timing with HIP events around every launch of one kernel family on the world's
- * stream.  kernel: 0 off, 1 position solve, 2 velocity solve, 3 narrowphase, 4 integrate,
- * 5 velocity update, 6 broadphase sweep.  rbgpu_get_timing returns the summed
- * milliseconds and launches since the last reset (synchronises). */
+ * stream.  kernel: 0 off, 1 position solve (per-colour launches), 2 velocity solve
+ * (per-colour launches), 3 narrowphase, 4 integrate, 5 velocity update, 6 broadphase
+ * (refit, endpoint sort, sweep, pair list; one span per frame), 7 colouring (one span per
+ * substep), 8 position solve runs of small colours (single workgroup), 9 velocity solve
+ * runs.  rbgpu_get_timing returns the summed milliseconds and launches since the last
+ * reset (synchronises); bytes = algorithmic bytes of the timed launches for 1 and 2. */
+/* Pipeline options (no counterpart in the reference, whose step is sequential).
+ *   RBGPU_OPT_PALETTE_MAX  colouring: a pair draws its tentative colour from at most this
+ *                          many free colours (default 8).  Changes the colouring, hence the
+ *                          solve order and the trajectory; the other options do not change
+ *                          any result.
+ *   RBGPU_OPT_HEAD_ROUNDS  colouring: most grid-wide rounds per substep (default 96, <= 256)
+ *   RBGPU_OPT_ROUND_BATCH  colouring: grid-wide rounds launched between host checks (12)
+ *   RBGPU_OPT_TAIL_MIN     colouring: at most this many uncoloured pairs go to the
+ *                          single-workgroup rounds (2048)
+ *   RBGPU_OPT_RUN_MIN      solve: colours with fewer pairs run in single-workgroup runs (256)
+ *   RBGPU_OPT_TIMING_STRIDE  kernel timing (rbgpu_set_timing) covers every n-th frame (1) */
+enum {
+    RBGPU_OPT_PALETTE_MAX = 0,
+    RBGPU_OPT_HEAD_ROUNDS = 1,
+    RBGPU_OPT_ROUND_BATCH = 2,
+    RBGPU_OPT_TAIL_MIN = 3,
+    RBGPU_OPT_RUN_MIN = 4,
+    RBGPU_OPT_TIMING_STRIDE = 5
+};
+int rbgpu_set_option(rbgpu_world *w, int option, int value);
+int rbgpu_get_option(rbgpu_world *w, int option, int *value);
+
 int rbgpu_set_timing(rbgpu_world *w, int kernel);
 int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, double *bytes);
 

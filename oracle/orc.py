@@ -56,8 +56,10 @@ def oracle_lib():
         _bind(lib, "orc_")
         lib.orc_collide.argtypes = [C.c_int, _P, _D, _P, _D, _D]
         lib.orc_num_solve_pairs.argtypes = [_P]
-        lib.orc_get_solve_pairs.argtypes = [_P, _U, _I, _I]
+        lib.orc_get_solve_pairs.argtypes = [_P, _U, _I, _I, _I]
         lib.orc_num_colours.argtypes = [_P]
+        lib.orc_colour_rounds.argtypes = [_P]
+        lib.orc_set_palette_max.argtypes = [_P, C.c_int]
         lib.orc_mix32.restype = C.c_uint32
         lib.orc_mix32.argtypes = [C.c_uint32]
         _libs["orc"] = lib
@@ -109,7 +111,8 @@ class _WorldBase:
         return self
 
     def step(self, frames=1):
-        self._fn("step")(self.h, frames)
+        if self._fn("step")(self.h, frames) != 0:
+            raise RuntimeError(f"{self.prefix}step failed (colour capacity exceeded)")
 
     def state(self):
         n = self._fn("num_bodies")(self.h)
@@ -150,12 +153,18 @@ class OracleWorld(_WorldBase):
         super().__init__(lib, lib.orc_create(1 if fp32 else 0, order, timestep, substeps))
         self.fp32, self.order = fp32, order
 
+    def set_palette_max(self, m):
+        if self.lib.orc_set_palette_max(self.h, m) != 0:
+            raise ValueError("palette maximum must be >= 1")
+
     def solve_pairs(self):
         n = self.lib.orc_num_solve_pairs(self.h)
         cc, col, cnt = np.zeros((n, 2), np.uint32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        kind = np.zeros(n, np.int32)
         if n:
-            self.lib.orc_get_solve_pairs(self.h, _ptr(cc, _U), _ptr(col, _I), _ptr(cnt, _I))
-        return {"colliders": cc, "colour": col, "count": cnt, "ncolours": self.lib.orc_num_colours(self.h)}
+            self.lib.orc_get_solve_pairs(self.h, _ptr(cc, _U), _ptr(col, _I), _ptr(cnt, _I), _ptr(kind, _I))
+        return {"colliders": cc, "colour": col, "count": cnt, "kind": kind, "ncolours": self.lib.orc_num_colours(self.h),
+                "rounds": self.lib.orc_colour_rounds(self.h)}
 
 
 class RefWorld(_WorldBase):

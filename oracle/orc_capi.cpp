@@ -3,6 +3,7 @@
 // TEST INFRASTRUCTURE ONLY: the checker for the GPU pipeline, never the product.
 // The descriptor layouts match the product's C-ABI descriptors byte for byte so
 // one numpy structured array feeds both.
+#include <exception>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -47,8 +48,10 @@ struct Any {
     virtual int ncontacts() const = 0;
     virtual void contacts(uint32_t *ids4, double *n3, double *ra3, double *rb3, double *dep, double *lam2) const = 0;
     virtual int nspairs() const = 0;
-    virtual void spairs(uint32_t *ca_cb, int32_t *col, int32_t *cnt) const = 0;
+    virtual void spairs(uint32_t *ca_cb, int32_t *col, int32_t *cnt, int32_t *kind) const = 0;
     virtual int ncolours() const = 0;
+    virtual int colour_rounds() const = 0;
+    virtual void set_palette_max(uint32_t m) = 0;
     virtual void set_state(const double *p, const double *q, const double *v, const double *w) = 0;
 };
 
@@ -116,14 +119,20 @@ template <class T> struct Impl : Any {
         });
     }
     int nspairs() const override { return (int)w.spairs.size(); }
-    void spairs(uint32_t *ca_cb, int32_t *col, int32_t *cnt) const override {
+    // colour: active pairs -> colour of the last substep, inactive -> -1;
+    // kind: active -> -1, inactive -> -2.
+    void spairs(uint32_t *ca_cb, int32_t *col, int32_t *cnt, int32_t *kind) const override {
         for (size_t p = 0; p < w.spairs.size(); ++p) {
             if (ca_cb) { ca_cb[2 * p] = w.spairs[p].ca; ca_cb[2 * p + 1] = w.spairs[p].cb; }
-            if (col) col[p] = w.colour[p];
+            const bool act = w.pcount[p] > 0;
+            if (col) col[p] = act ? w.colour[p] : -1;
             if (cnt) cnt[p] = w.pcount[p];
+            if (kind) kind[p] = act ? -1 : -2;
         }
     }
     int ncolours() const override { return w.ncolours; }
+    int colour_rounds() const override { return (int)w.colour_rounds; }
+    void set_palette_max(uint32_t m) override { w.palette_max = m; }
 };
 
 // One narrowphase call (same layout as the reference driver's ref_collide).
@@ -178,7 +187,14 @@ int orc_add_colliders(void *h, int n, const orc_collider_desc *d) {
     for (int i = 0; i < n; ++i) static_cast<Any *>(h)->add_collider(d[i]);
     return 0;
 }
-int orc_step(void *h, int frames) { static_cast<Any *>(h)->step(frames); return 0; }
+int orc_step(void *h, int frames) {
+    try {
+        static_cast<Any *>(h)->step(frames);
+    } catch (const std::exception &) {
+        return -1;
+    }
+    return 0;
+}
 int orc_num_bodies(void *h) { return static_cast<Any *>(h)->nbodies(); }
 void orc_get_bodies(void *h, double *p, double *q, double *v, double *w) { static_cast<Any *>(h)->bodies(p, q, v, w); }
 void orc_set_bodies(void *h, const double *p, const double *q, const double *v, const double *w) {
@@ -191,8 +207,16 @@ void orc_get_contacts(void *h, uint32_t *ids4, double *n3, double *ra3, double *
     static_cast<Any *>(h)->contacts(ids4, n3, ra3, rb3, dep, lam2);
 }
 int orc_num_solve_pairs(void *h) { return static_cast<Any *>(h)->nspairs(); }
-void orc_get_solve_pairs(void *h, uint32_t *ca_cb, int32_t *col, int32_t *cnt) { static_cast<Any *>(h)->spairs(ca_cb, col, cnt); }
+void orc_get_solve_pairs(void *h, uint32_t *ca_cb, int32_t *col, int32_t *cnt, int32_t *kind) {
+    static_cast<Any *>(h)->spairs(ca_cb, col, cnt, kind);
+}
 int orc_num_colours(void *h) { return static_cast<Any *>(h)->ncolours(); }
+int orc_colour_rounds(void *h) { return static_cast<Any *>(h)->colour_rounds(); }
+int orc_set_palette_max(void *h, int m) {
+    if (m < 1) return -1;
+    static_cast<Any *>(h)->set_palette_max((uint32_t)m);
+    return 0;
+}
 uint32_t orc_mix32(uint32_t x) { return orc::mix32(x); }
 
 int orc_collide(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b, const double *tb,

@@ -15,6 +15,7 @@
 #pragma once
 #include <algorithm>
 #include <cstdint>
+#include <stdexcept>
 #include <vector>
 #include "orc_collide.hpp"
 
@@ -279,75 +280,91 @@ template <class T> struct World {
         pcount.assign(P, 0);
         pcont.resize((size_t)P * 4);
     }
-    // Colouring of the substep's active pairs (pairs with contact points); edges = shared
-    // dynamic bodies.  Synchronous randomized trials, replayed exactly as the GPU runs them:
-    // in round r every uncoloured pair v picks t(v) = the (mix32(p ^ r*0x9e3779b9) mod k)-th
-    // colour not used by its already-coloured neighbours, k = (number of uncoloured
-    // neighbour entries) + 1; it keeps t(v) unless an uncoloured neighbour of higher
-    // priority (mix32(q), q) picked the same colour.  Valid, at most degree+1 colours,
-    // O(log degree) rounds.
-    static int pick_free(const std::vector<char> &used, uint32_t idx) {
-        for (int c = 0;; ++c) {
-            if (c < (int)used.size() && used[c]) continue;
-            if (idx == 0) return c;
-            --idx;
-        }
+    // Colouring of the substep's active pairs (pairs with contact points); two pairs
+    // conflict when they share a dynamic body.  Synchronous randomized rounds whose cost is
+    // independent of body degree, replayed exactly as the GPU runs them.  Per dynamic body
+    // v: used(v) = colours taken by its pairs, u(v) = its uncoloured pairs.  Round r, for
+    // every uncoloured pair p with dynamic bodies D(p):
+    //   claim: t(p) = the (tent_hash(p, r) mod min(max(ceil(k/4), 1), palette_max))-th colour outside the
+    //          union of used(v), k = sum of u(v) over D(p); key(p) = rank_hash(p, r); each v
+    //          in D(p) records the largest key that targets its slot t mod COLOUR_SLOTS;
+    //   settle: p takes t(p) if its key is the recorded one at every v in D(p).
+    // Two pairs of one body never settle on one colour in one round (one key per slot), nor
+    // on a colour used earlier (used(v) is excluded), so the colouring is valid; the pair
+    // with the largest key overall always settles, so every round makes progress.
+    static constexpr uint32_t COLOUR_SLOTS = 16, MAX_COLOURS = 1024;
+    uint32_t palette_max = 8;
+    static uint32_t tent_hash(uint32_t p, uint32_t r) { return mix32(p ^ (r * 0x9e3779b9u)); }
+    static uint32_t rank_hash(uint32_t p, uint32_t r) { return mix32(p ^ (r * 0x85ebca6bu) ^ 0xc2b2ae35u); }
+    uint32_t palette(uint32_t k) const {
+        uint32_t m = (k + 3u) / 4u;
+        return m < 1u ? 1u : (m > palette_max ? palette_max : m);
     }
     void colour_active() {
-        const uint32_t P = (uint32_t)spairs.size();
+        const uint32_t P = (uint32_t)spairs.size(), N = (uint32_t)btype.size();
+        constexpr uint32_t W = MAX_COLOURS / 32;
         std::vector<uint32_t> act;
         for (uint32_t p = 0; p < P; ++p)
             if (pcount[p] > 0) act.push_back(p);
-        std::vector<std::vector<uint32_t>> by_body(btype.size());
+        std::vector<uint32_t> ucnt(N, 0), used((size_t)N * W, 0);
+        std::vector<uint64_t> claim((size_t)N * COLOUR_SLOTS, 0);
+        auto dyn = [&](uint32_t b) { return btype[b] == 1; };
         for (uint32_t p : act) {
-            if (btype[spairs[p].ba] == 1) by_body[spairs[p].ba].push_back(p);
-            if (btype[spairs[p].bb] == 1) by_body[spairs[p].bb].push_back(p);
+            if (dyn(spairs[p].ba)) ++ucnt[spairs[p].ba];
+            if (dyn(spairs[p].bb)) ++ucnt[spairs[p].bb];
         }
         colour.assign(P, -1);
         ncolours = 0;
         colour_rounds = 0;
-        std::vector<int> tent(P, -1);
-        std::vector<char> used;
+        std::vector<uint32_t> tent(P, 0);
+        std::vector<uint64_t> key(P, 0);
         size_t left = act.size();
-        auto higher = [](uint32_t q, uint32_t p) {
-            uint32_t hq = mix32(q), hp = mix32(p);
-            return hq > hp || (hq == hp && q > p);
-        };
         for (uint32_t r = 0; left > 0; ++r) {
             for (uint32_t p : act) {
                 if (colour[p] >= 0) continue;
-                used.assign(1, 0);
-                uint32_t k = 1;
-                for (uint32_t b : {spairs[p].ba, spairs[p].bb}) {
-                    if (btype[b] != 1) continue;
-                    for (uint32_t q : by_body[b]) {
-                        if (q == p) continue;
-                        if (colour[q] >= 0) {
-                            if ((int)used.size() <= colour[q]) used.resize(colour[q] + 1, 0);
-                            used[colour[q]] = 1;
-                        } else {
-                            ++k;
+                const uint32_t bs[2] = {spairs[p].ba, spairs[p].bb};
+                uint32_t k = 0;
+                for (uint32_t b : bs)
+                    if (dyn(b)) k += ucnt[b];
+                uint32_t idx = tent_hash(p, r) % palette(k), t = MAX_COLOURS;
+                for (uint32_t w = 0; w < W && t == MAX_COLOURS; ++w) {
+                    uint32_t fr = ~0u;
+                    for (uint32_t b : bs)
+                        if (dyn(b)) fr &= ~used[(size_t)b * W + w];
+                    for (uint32_t j = 0; j < 32; ++j)
+                        if ((fr >> j) & 1u) {
+                            if (idx == 0) { t = w * 32 + j; break; }
+                            --idx;
                         }
-                    }
                 }
-                tent[p] = pick_free(used, mix32(p ^ (r * 0x9e3779b9u)) % k);
+                if (t == MAX_COLOURS) throw std::runtime_error("more than 1024 colours needed");
+                tent[p] = t;
+                key[p] = ((uint64_t)(r + 1) << 32) | rank_hash(p, r);
+                for (uint32_t b : bs)
+                    if (dyn(b)) {
+                        uint64_t &c = claim[(size_t)b * COLOUR_SLOTS + t % COLOUR_SLOTS];
+                        if (key[p] > c) c = key[p];
+                    }
             }
-            std::vector<uint32_t> accepted;
+            std::vector<uint32_t> won;
             for (uint32_t p : act) {
                 if (colour[p] >= 0) continue;
                 bool ok = true;
-                for (uint32_t b : {spairs[p].ba, spairs[p].bb}) {
-                    if (btype[b] != 1 || !ok) continue;
-                    for (uint32_t q : by_body[b])
-                        if (q != p && colour[q] < 0 && tent[q] == tent[p] && higher(q, p)) { ok = false; break; }
-                }
-                if (ok) accepted.push_back(p);
+                for (uint32_t b : {spairs[p].ba, spairs[p].bb})
+                    if (dyn(b) && claim[(size_t)b * COLOUR_SLOTS + tent[p] % COLOUR_SLOTS] != key[p]) ok = false;
+                if (ok) won.push_back(p);
             }
-            for (uint32_t p : accepted) {
-                colour[p] = tent[p];
-                if (tent[p] + 1 > ncolours) ncolours = tent[p] + 1;
+            for (uint32_t p : won) {
+                const uint32_t t = tent[p];
+                colour[p] = (int)t;
+                for (uint32_t b : {spairs[p].ba, spairs[p].bb})
+                    if (dyn(b)) {
+                        used[(size_t)b * W + t / 32] |= 1u << (t % 32);
+                        --ucnt[b];
+                    }
+                if ((int)t + 1 > ncolours) ncolours = (int)t + 1;
             }
-            left -= accepted.size();
+            left -= won.size();
             colour_rounds = r + 1;
         }
     }
@@ -499,11 +516,20 @@ template <class T> struct World {
             }
         }
     }
+    // Solve order of the colour mode: active pairs stably sorted by colour.
+    std::vector<uint32_t> solve_order;
+    void build_solve_order() {
+        const uint32_t P = (uint32_t)spairs.size();
+        std::vector<std::pair<int, uint32_t>> keyed;
+        for (uint32_t p = 0; p < P; ++p)
+            if (pcount[p] > 0) keyed.push_back({colour[p], p});
+        std::stable_sort(keyed.begin(), keyed.end(), [](auto &a, auto &b) { return a.first < b.first; });
+        solve_order.clear();
+        for (auto &k : keyed) solve_order.push_back(k.second);
+    }
     template <class F> void for_each_colour_contact(F &&f) {
-        for (int c = 0; c < ncolours; ++c)
-            for (uint32_t p = 0; p < spairs.size(); ++p)
-                if (colour[p] == c)
-                    for (int k = 0; k < pcount[p]; ++k) f(pcont[(size_t)p * 4 + k]);
+        for (uint32_t p : solve_order)
+            for (int k = 0; k < pcount[p]; ++k) f(pcont[(size_t)p * 4 + k]);
     }
 
     // ------------------------------------------------------------ World::step (World.hpp:192-212)
@@ -515,7 +541,7 @@ template <class T> struct World {
         if (order == ORDER_COLOUR) build_pairs();
         for (int s = 0; s < substeps; ++s) {
             if (order == ORDER_REFERENCE) narrowphase_reference();
-            else { narrowphase_colour(); colour_active(); }
+            else { narrowphase_colour(); colour_active(); build_solve_order(); }
             integrate(h);
             if (order == ORDER_REFERENCE)
                 for (auto &k : contacts) contact_position(k, inv_h);

@@ -18,7 +18,7 @@
 namespace rb {
 
 constexpr uint32_t BP_INVALID = 0xffffffffu;
-constexpr int MAX_COLOURS = 256;
+constexpr int MAX_COLOURS = 1024;
 constexpr int NBINS = 10;
 enum Bin { BIN_SS = 0, BIN_SB, BIN_BB, BIN_SC, BIN_CC, BIN_CB, BIN_SP, BIN_BP, BIN_CP, BIN_OTHER };
 
@@ -158,73 +158,6 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {   // lowbias32 (C. Wello
     return x;
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// Pair -> bodies, dynamic-body degree counts, colour reset.
-__global__ void k_pair_bodies(const uint2 *pairs, const uint32_t *n_pairs, Colliders C, Bodies B, uint2 *pbod,
-                              uint32_t *deg, int *colour, uint32_t *uncoloured) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t P = *n_pairs;
-    if (p == 0) *uncoloured = P;
-    if (p >= P) return;
-    uint2 pr = pairs[p];
-    uint32_t ba = (uint32_t)C.meta[pr.x].y, bb = (uint32_t)C.meta[pr.y].y;
-    pbod[p] = make_uint2(ba, bb);
-    colour[p] = -1;
-    if (B.Ii1[ba].z != 0.0f) atomicAdd(&deg[ba], 1u);
-    if (B.Ii1[bb].z != 0.0f) atomicAdd(&deg[bb], 1u);
-}
-
-__global__ void k_fill_adj(const uint2 *pbod, const uint32_t *n_pairs, Bodies B, const uint32_t *aoff, uint32_t *cursor,
-                           uint32_t *adj) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= *n_pairs) return;
-    uint2 b = pbod[p];
-    if (B.Ii1[b.x].z != 0.0f) adj[aoff[b.x] + atomicAdd(&cursor[b.x], 1u)] = p;
-    if (B.Ii1[b.y].z != 0.0f) adj[aoff[b.y] + atomicAdd(&cursor[b.y], 1u)] = p;
-}
-
-// One Jones-Plassmann round: a pair whose uncoloured neighbours all have lower
-// priority takes the smallest colour unused by its coloured neighbours (which are
-// then exactly its higher-priority neighbours, so the result equals sequential
-// first-fit in descending priority whatever the timing).
-__global__ void k_jp_round(const uint2 *pbod, const uint32_t *n_pairs, Bodies B, const uint32_t *aoff,
-                           const uint32_t *adeg, const uint32_t *adj, int *colour, uint32_t *uncoloured, int *err) {
-    if (__atomic_load_n(uncoloured, __ATOMIC_RELAXED) == 0) return;
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= *n_pairs) return;
-    if (__atomic_load_n(&colour[p], __ATOMIC_RELAXED) >= 0) return;
-    const uint32_t hp = mix32(p);
-    uint64_t used[MAX_COLOURS / 64] = {0, 0, 0, 0};
-    uint2 bb = pbod[p];
-    uint32_t bodies[2] = {bb.x, bb.y};
-    for (int s = 0; s < 2; ++s) {
-        uint32_t b = bodies[s];
-        if (B.Ii1[b].z == 0.0f) continue;
-        uint32_t o = aoff[b], n = adeg[b];
-        for (uint32_t k = 0; k < n; ++k) {
-            uint32_t q = adj[o + k];
-            if (q == p) continue;
-            int c = __atomic_load_n(&colour[q], __ATOMIC_RELAXED);
-            if (c < 0) {
-                uint32_t hq = mix32(q);
-                if (hq > hp || (hq == hp && q > p)) return;   // a higher-priority neighbour is uncoloured
-            } else {
-                used[c >> 6] |= 1ull << (c & 63);
-            }
-        }
-    }
-    int c = -1;
-    for (int w = 0; w < MAX_COLOURS / 64; ++w)
-        if (~used[w]) { c = w * 64 + __builtin_ctzll(~used[w]); break; }
-    if (c < 0) { *err = 1; c = MAX_COLOURS - 1; }
-    __atomic_store_n(&colour[p], c, __ATOMIC_RELAXED);
-    atomicSub(uncoloured, 1u);
-}
-
 __device__ __forceinline__ int bin_of(int ka, int kb) {
     if (kb == SH_PLANE) return ka == SH_SPHERE ? BIN_SP : ka == SH_BOX ? BIN_BP : ka == SH_CAPSULE ? BIN_CP : BIN_OTHER;
     if (ka > kb) { int t = ka; ka = kb; kb = t; }
@@ -255,183 +188,284 @@ __global__ void k_pair_gather(const uint2 *pairs, const uint32_t *n_pairs, Colli
     bin_val[p] = p;
 }
 
-// ------------------------------------------------------------------ grid barrier
-// Barrier for cooperative (co-resident) grids.  The last block to arrive evaluates
-// `*probe == 0` (when probe is given) after every block has finished the phase, and
-// publishes it with the new generation, so every block takes the same decision.
-// Every wait is bounded: after BARRIER_TIMEOUT_TICKS of the 100 MHz wall clock a
-// waiting block raises *fault and leaves; once *fault is set no block waits again,
-// so the kernel always drains (the host then reports the fault as an error).
-constexpr uint64_t BARRIER_TIMEOUT_TICKS = 200000000ull;   // 2 s
-struct GridBar {
-    uint32_t *count, *gen;
-    int *decision;
-    int *fault;
-};
-__device__ __forceinline__ int grid_barrier(GridBar b, const uint32_t *probe) {
-    __shared__ int s_dec;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t g = __hip_atomic_load(b.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t arrived = __hip_atomic_fetch_add(b.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
-            int d = probe ? (__hip_atomic_load(probe, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) : 0;
-            __hip_atomic_store(b.decision, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(b.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(b.gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(b.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                if (__hip_atomic_load(b.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (wall_clock64() - t0 > BARRIER_TIMEOUT_TICKS) {
-                    __hip_atomic_store(b.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        s_dec = __hip_atomic_load(b.decision, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (__hip_atomic_load(b.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) s_dec = 1;   // leave any loop
-        __threadfence();
-    }
-    __syncthreads();
-    return s_dec;
-}
-
 // ------------------------------------------------------------------ substep colouring of active pairs
-__global__ void k_active_flags(const int *cnt, uint32_t P, uint32_t *flag) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < P) flag[p] = cnt[p] > 0 ? 1u : 0u;
+// Pairs with contact points are coloured every substep; two pairs conflict when they share a
+// dynamic body.  Synchronous randomized rounds whose cost does not depend on body degree.
+// Per dynamic body v: used(v) = colours taken by its pairs (bit mask), u(v) = its uncoloured
+// pairs, claim(v, slot) = the largest key that targeted the slot.  Round r, uncoloured pair p:
+//   claim:  t = the (tent_hash(p, r) mod min(max(ceil(k/4), 1), pal_max))-th colour outside the
+//           union of used(v), k = sum of u(v); key = (r + 1) << 32 | rank_hash(p, r);
+//           atomicMax(claim(v, t mod COLOUR_SLOTS), key) for each dynamic body v;
+//   settle: p takes t if its key is the claim at each of its dynamic bodies; then it sets
+//           t in used(v) and decrements u(v).
+// One key per slot means two pairs of one body never settle on one colour in one round, and
+// used(v) keeps later rounds off taken colours: the colouring is valid.  The largest key
+// always settles, so every round makes progress.  Keys carry the round, so claims of earlier
+// rounds lose without clearing.  Outcomes depend only on the keys, not on thread or list
+// order, so work lists may be built with atomics.  Arrays are indexed by pair id.
+constexpr uint32_t COLOUR_SLOTS = 16;
+constexpr uint32_t COLOUR_WORDS = MAX_COLOURS / 32;
+constexpr uint32_t MAX_HEAD_ROUNDS = 256;
+constexpr uint32_t TAIL_THREADS = 1024;
+constexpr uint32_t LEFT_STRIPES = 64;   // the uncoloured count is striped over 64 words
+// ctr[]: counters of the colouring (cleared every substep)
+enum {
+    CTR_ACT = 0,      // active pairs (coloured pairs, written with the offsets)
+    CTR_REST = 1,     // rest list length
+    CTR_ROUNDS = 2,   // rounds used
+    CTR_NCOL = 3,     // colours used
+    CTR_ERR = 4,      // 1: colour capacity exceeded, 2: no convergence
+    CTR_LEFT = 64,    // LEFT_STRIPES words; their sum (mod 2^32) = uncoloured pairs
+    CTR_FLAGS = CTR_LEFT + LEFT_STRIPES,   // one word per grid-wide round: had pairs
+    CTR_SIZE = CTR_FLAGS + MAX_HEAD_ROUNDS
+};
+constexpr int32_t COLOUR_NONE = -2;     // a pair without points this substep
+constexpr int32_t COLOUR_OPEN = -1;     // an active pair not coloured yet
+
+struct ColourBufs {
+    int *acol;                  // colour per pair id: >= 0, COLOUR_OPEN or COLOUR_NONE
+    uint32_t *tent;             // tentative colour per pair id
+    uint32_t *rest;             // pair ids left to the single-workgroup rounds
+    uint32_t *used;             // N x COLOUR_WORDS colour masks
+    uint32_t *ucnt;             // N uncoloured-pair counts
+    unsigned long long *claim;  // N x COLOUR_SLOTS
+    uint32_t *hist, *off, *cur; // per colour: size, start (MAX_COLOURS + 1), scatter cursor
+    uint32_t *ctr;              // CTR_*
+    uint32_t pal_max;           // largest palette a pair draws its tentative colour from
+};
+
+__device__ __forceinline__ uint32_t tent_hash(uint32_t p, uint32_t r) { return mix32(p ^ (r * 0x9e3779b9u)); }
+__device__ __forceinline__ uint32_t rank_hash(uint32_t p, uint32_t r) { return mix32(p ^ (r * 0x85ebca6bu) ^ 0xc2b2ae35u); }
+__device__ __forceinline__ uint32_t palette(uint32_t k, uint32_t pal_max) {
+    const uint32_t m = (k + 3u) / 4u;
+    return m < 1u ? 1u : (m > pal_max ? pal_max : m);
 }
-// Compacted active list (ascending pair id); colour reset; counters reset.
-__global__ void k_compact(const uint32_t *flag, const uint32_t *off, uint32_t P, const uint32_t *n_act, uint32_t *act,
-                          int *colour, uint32_t *cround, uint32_t *uncoloured, uint32_t *ncol, uint32_t *deg, uint32_t *cur,
-                          uint32_t N) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) { *uncoloured = *n_act; *ncol = 0; }
-    if (p <= N) { deg[p] = 0; cur[p] = 0; }
-    if (p < P && flag[p]) { act[off[p]] = p; colour[off[p]] = -1; cround[off[p]] = 0xffffffffu; }
+__device__ __forceinline__ bool is_dynamic(const Bodies &B, uint32_t b) { return B.Ii1[b].z != 0.0f; }
+// Loads of words that other threads update with atomics in the same kernel.
+__device__ __forceinline__ uint32_t ld_shared(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__global__ void k_act_deg(const uint32_t *act, const uint32_t *n_act, const uint2 *pbod, Bodies B, uint32_t *deg) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_act) return;
-    uint2 b = pbod[act[i]];
-    if (B.Ii1[b.x].z != 0.0f) atomicAdd(&deg[b.x], 1u);
-    if (B.Ii1[b.y].z != 0.0f) atomicAdd(&deg[b.y], 1u);
+__device__ __forceinline__ unsigned long long ld_shared(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__global__ void k_act_adj(const uint32_t *act, const uint32_t *n_act, const uint2 *pbod, Bodies B, const uint32_t *aoff,
-                          uint32_t *cur, uint32_t *adj) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_act) return;
-    uint2 b = pbod[act[i]];
-    if (B.Ii1[b.x].z != 0.0f) adj[aoff[b.x] + atomicAdd(&cur[b.x], 1u)] = i;
-    if (B.Ii1[b.y].z != 0.0f) adj[aoff[b.y] + atomicAdd(&cur[b.y], 1u)] = i;
+// Rank of this lane among the set lanes of a wave-wide ballot.
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// Wave-aggregated append: one atomic per wave; every lane of the wave must call it.
+__device__ __forceinline__ void wave_append(bool take, uint32_t val, uint32_t *counter, uint32_t *list) {
+    const uint64_t m = __ballot(take);
+    if (!m) return;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (take) list[base + lane_rank(m)] = val;
+}
+// Block-wide count of `v` (256 threads) added to one stripe of the uncoloured count.
+__device__ __forceinline__ void block_count_to_left(uint32_t *ctr, bool v, bool add, uint32_t *sh) {
+    if (threadIdx.x == 0) *sh = 0u;
+    __syncthreads();
+    const uint64_t m = __ballot(v);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(sh, (uint32_t)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0 && *sh) {
+        uint32_t *stripe = &ctr[CTR_LEFT + blockIdx.x % LEFT_STRIPES];
+        if (add) atomicAdd(stripe, *sh); else atomicSub(stripe, *sh);
+    }
+}
+// Sum of the stripes, by one wave (every lane gets it).
+__device__ __forceinline__ uint32_t uncoloured_left(const uint32_t *ctr) {
+    uint32_t v = ld_shared(&ctr[CTR_LEFT + (threadIdx.x & 63)]);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-__device__ __forceinline__ bool higher_priority(uint32_t q, uint32_t p) {
-    const uint32_t hq = mix32(q), hp = mix32(p);
-    return hq > hp || (hq == hp && q > p);
+// Clears the per-body colouring state and the counters.  Grid covers N * COLOUR_WORDS / 4.
+__global__ void k_colour_reset(uint32_t N, ColourBufs cb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    if (i < N * (COLOUR_WORDS / 4)) reinterpret_cast<uint4 *>(cb.used)[i] = z;
+    if (i < N * (COLOUR_SLOTS / 2)) reinterpret_cast<uint4 *>(cb.claim)[i] = z;
+    if (i < N) cb.ucnt[i] = 0u;
+    if (i < (uint32_t)MAX_COLOURS) cb.hist[i] = 0u;
+    if (i < (uint32_t)CTR_SIZE) cb.ctr[i] = 0u;
 }
-// The idx-th colour not set in `used`.
-__device__ __forceinline__ int nth_free(const uint64_t used[MAX_COLOURS / 64], uint32_t idx) {
-    for (int w = 0; w < MAX_COLOURS / 64; ++w) {
-        uint64_t fr = ~used[w];
-        const uint32_t n = (uint32_t)__popcll(fr);
+
+// Marks the pairs with points as open, counts them per dynamic body and in total.
+__global__ void __launch_bounds__(256) k_colour_activate(const int *cnt, uint32_t P, const uint2 *pbod, Bodies B,
+                                                         ColourBufs cb) {
+    __shared__ uint32_t sh;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool a = p < P && cnt[p] > 0;
+    if (p < P) cb.acol[p] = a ? COLOUR_OPEN : COLOUR_NONE;
+    if (a) {
+        const uint2 bb = pbod[p];
+        if (is_dynamic(B, bb.x)) atomicAdd(&cb.ucnt[bb.x], 1u);
+        if (is_dynamic(B, bb.y)) atomicAdd(&cb.ucnt[bb.y], 1u);
+    }
+    block_count_to_left(cb.ctr, a, true, &sh);
+}
+
+__device__ __forceinline__ void colour_claim(const ColourBufs &cb, const uint2 *pbod, const Bodies &B, uint32_t p,
+                                             uint32_t r) {
+    const uint2 bb = pbod[p];
+    const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
+    const uint32_t k = (da ? ld_shared(&cb.ucnt[bb.x]) : 0u) + (db ? ld_shared(&cb.ucnt[bb.y]) : 0u);
+    uint32_t idx = tent_hash(p, r) % palette(k, cb.pal_max);
+    uint32_t t = MAX_COLOURS;
+    for (uint32_t w = 0; w < COLOUR_WORDS; ++w) {
+        uint32_t fr = ~((da ? ld_shared(&cb.used[bb.x * COLOUR_WORDS + w]) : 0u) |
+                        (db ? ld_shared(&cb.used[bb.y * COLOUR_WORDS + w]) : 0u));
+        const uint32_t n = (uint32_t)__popc(fr);
         if (idx < n) {
-            for (; idx; --idx) fr &= fr - 1;
-            return w * 64 + __builtin_ctzll(fr);
+            for (; idx; --idx) fr &= fr - 1u;
+            t = w * 32u + (uint32_t)__builtin_ctz(fr);
+            break;
         }
         idx -= n;
     }
-    return -1;
+    if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&cb.ctr[CTR_ERR], 1u); t = MAX_COLOURS - 1; }
+    cb.tent[p] = t;
+    const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(p, r);
+    if (da) atomicMax(&cb.claim[bb.x * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
+    if (db) atomicMax(&cb.claim[bb.y * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
 }
 
-// Colouring of the active pairs by synchronous randomized trials, all rounds in one
-// cooperative launch.  Round r: (A) every uncoloured vertex i (pair p = act[i]) picks
-// t = the (mix32(p ^ r*0x9e3779b9) mod k)-th colour unused by neighbours coloured in
-// earlier rounds, k = uncoloured neighbour entries + 1; (B) it keeps t unless an
-// uncoloured neighbour of higher priority (mix32(q), q) picked t too.  cround[i] holds
-// the round a vertex was coloured in (0xffffffff = not yet), so "uncoloured at the start
-// of round r" is cround >= r in both phases.  Every round colours at least the highest-
-// priority uncoloured vertex; the expected number of rounds is O(log degree).
-__global__ void __launch_bounds__(256) k_colour_coop(const uint32_t *act, const uint32_t *n_act, const uint2 *pbod, Bodies B,
-                                                     const uint32_t *aoff, const uint32_t *adeg, const uint32_t *adj,
-                                                     int *colour, int *tent, uint32_t *cround, uint32_t *uncoloured,
-                                                     uint32_t *ncol, int *err, GridBar bar, uint32_t max_rounds,
-                                                     uint32_t *rounds_out) {
-    const uint32_t n = *n_act;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t r = 0;; ++r) {
-        // phase A: tentative colours
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-            if (cround[i] < r) continue;
-            const uint32_t p = act[i];
-            uint64_t used[MAX_COLOURS / 64] = {0, 0, 0, 0};
-            uint32_t k = 1;
-            const uint2 bb = pbod[p];
-            for (int sd = 0; sd < 2; ++sd) {
-                const uint32_t b = sd ? bb.y : bb.x;
-                if (B.Ii1[b].z == 0.0f) continue;
-                const uint32_t o = aoff[b], nn = adeg[b];
-                for (uint32_t e = 0; e < nn; ++e) {
-                    const uint32_t j = adj[o + e];
-                    if (j == i) continue;
-                    if (cround[j] < r) {
-                        const int c = colour[j];
-                        used[c >> 6] |= 1ull << (c & 63);
-                    } else {
-                        ++k;
-                    }
-                }
-            }
-            int t = nth_free(used, mix32(p ^ (r * 0x9e3779b9u)) % k);
-            if (t < 0) { *err = 1; t = MAX_COLOURS - 1; }
-            tent[i] = t;
+__device__ __forceinline__ bool colour_settle(const ColourBufs &cb, const uint2 *pbod, const Bodies &B, uint32_t p,
+                                              uint32_t r) {
+    const uint2 bb = pbod[p];
+    const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
+    const uint32_t t = cb.tent[p];
+    const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(p, r);
+    if (da && ld_shared(&cb.claim[bb.x * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) return false;
+    if (db && ld_shared(&cb.claim[bb.y * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) return false;
+    cb.acol[p] = (int)t;
+    const uint32_t bit = 1u << (t % 32u);
+    if (da) { atomicOr(&cb.used[bb.x * COLOUR_WORDS + t / 32u], bit); atomicSub(&cb.ucnt[bb.x], 1u); }
+    if (db) { atomicOr(&cb.used[bb.y * COLOUR_WORDS + t / 32u], bit); atomicSub(&cb.ucnt[bb.y], 1u); }
+    return true;
+}
+
+// Grid-wide round, claim phase, over all pairs.  Skipped once at most `tail_min` pairs are
+// uncoloured (the uncoloured count changes only in settle launches, so every block of one
+// claim launch decides alike).  Marks round r as run.
+__global__ void k_colour_claim(const uint2 *pbod, Bodies B, ColourBufs cb, uint32_t P, uint32_t r, uint32_t tail_min) {
+    if (r > 0 && uncoloured_left(cb.ctr) <= tail_min) return;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = p < P && cb.acol[p] == COLOUR_OPEN;
+    if (__ballot(live) && (threadIdx.x & 63) == 0) cb.ctr[CTR_FLAGS + r] = 1u;
+    if (live) colour_claim(cb, pbod, B, p, r);
+}
+// Grid-wide round, settle phase (runs exactly when its claim phase had pairs).
+__global__ void __launch_bounds__(256) k_colour_settle(const uint2 *pbod, Bodies B, ColourBufs cb, uint32_t P, uint32_t r) {
+    __shared__ uint32_t sh;
+    if (!cb.ctr[CTR_FLAGS + r]) return;   // the claim phase of round r was skipped or empty
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = p < P && cb.acol[p] == COLOUR_OPEN;
+    const bool ok = live && colour_settle(cb, pbod, B, p, r);
+    block_count_to_left(cb.ctr, ok, false, &sh);
+}
+// Collects the uncoloured pairs into the rest list (after the grid-wide rounds).
+__global__ void k_colour_collect(ColourBufs cb, uint32_t P) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    wave_append(p < P && cb.acol[p] == COLOUR_OPEN, p, &cb.ctr[CTR_REST], cb.rest);
+}
+
+// The remaining rounds in one workgroup: the rest list is small after the grid-wide rounds,
+// and a workgroup barrier costs far less than a kernel boundary.  Starts at the first round
+// the grid-wide launches skipped (r_max when none did) and runs until every pair is
+// coloured (or `cap` rounds: error 2).  Records the round count.
+__global__ void __launch_bounds__(TAIL_THREADS) k_colour_tail(const uint2 *pbod, Bodies B, ColourBufs cb, uint32_t r_max,
+                                                                uint32_t cap) {
+    __shared__ uint32_t won;
+    const uint32_t n = cb.ctr[CTR_REST];
+    uint32_t r0 = r_max;
+    for (uint32_t q = 0; q < r_max; ++q)
+        if (!cb.ctr[CTR_FLAGS + q]) { r0 = q; break; }
+    uint32_t left = n, r = r0;
+    for (; left > 0 && r < cap; ++r) {
+        if (threadIdx.x == 0) won = 0u;
+        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+            const uint32_t p = cb.rest[j];
+            if (cb.acol[p] == COLOUR_OPEN) colour_claim(cb, pbod, B, p, r);
         }
-        grid_barrier(bar, nullptr);
-        // phase B: keep the tentative colour unless a higher-priority neighbour chose it too
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-            if (cround[i] < r) continue;
-            const uint32_t p = act[i];
-            const int t = tent[i];
-            const uint2 bb = pbod[p];
-            bool ok = true;
-            for (int sd = 0; sd < 2 && ok; ++sd) {
-                const uint32_t b = sd ? bb.y : bb.x;
-                if (B.Ii1[b].z == 0.0f) continue;
-                const uint32_t o = aoff[b], nn = adeg[b];
-                for (uint32_t e = 0; e < nn; ++e) {
-                    const uint32_t j = adj[o + e];
-                    if (j == i) continue;
-                    if (__hip_atomic_load(&cround[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= r && tent[j] == t &&
-                        higher_priority(act[j], p)) { ok = false; break; }
-                }
-            }
-            if (ok) {
-                colour[i] = t;
-                __hip_atomic_store(&cround[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                atomicSub(uncoloured, 1u);
-                atomicMax(ncol, (uint32_t)(t + 1));
-            }
+        __syncthreads();
+        uint32_t w = 0;
+        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+            const uint32_t p = cb.rest[j];
+            if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r)) ++w;
         }
-        const int done = grid_barrier(bar, uncoloured);
-        if (done) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) *rounds_out = r + 1;
-            break;
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+        if ((threadIdx.x & 63) == 0 && w) atomicAdd(&won, w);
+        __syncthreads();
+        left -= won;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint32_t rounds = r;
+        if (n == 0) {   // finished in the grid-wide rounds: count the rounds that had work
+            rounds = 0;
+            for (uint32_t q = 0; q < r0; ++q)
+                if (cb.ctr[CTR_FLAGS + q]) rounds = q + 1;
         }
-        if (r + 1 >= max_rounds) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { *err = 2; *rounds_out = r + 1; }
-            break;
-        }
+        cb.ctr[CTR_ROUNDS] = rounds;
+        if (left) atomicOr(&cb.ctr[CTR_ERR], 2u);
     }
 }
 
-__global__ void k_colour_keys(const int *colour, const uint32_t *act, const uint32_t *n_act, uint32_t *key, uint32_t *val) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_act) return;
-    key[i] = (uint32_t)colour[i];
-    val[i] = act[i];
+// Colour sizes (LDS histogram per block, grid-stride over the pairs).
+__global__ void __launch_bounds__(256) k_colour_hist(ColourBufs cb, uint32_t P) {
+    __shared__ uint32_t h[MAX_COLOURS];
+    for (uint32_t c = threadIdx.x; c < (uint32_t)MAX_COLOURS; c += blockDim.x) h[c] = 0u;
+    __syncthreads();
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const int c = cb.acol[p];
+        if (c >= 0) atomicAdd(&h[c], 1u);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < (uint32_t)MAX_COLOURS; c += blockDim.x)
+        if (h[c]) atomicAdd(&cb.hist[c], h[c]);
+}
+// Colour offsets (one block of MAX_COLOURS threads): off = exclusive scan, cur = off, ncol,
+// active pairs.
+__global__ void __launch_bounds__(MAX_COLOURS) k_colour_offsets(ColourBufs cb) {
+    __shared__ uint32_t sh[MAX_COLOURS / 64 + 1];
+    __shared__ uint32_t top;
+    const uint32_t c = threadIdx.x;
+    if (c == 0) top = 0u;
+    const uint32_t v = cb.hist[c];
+    uint32_t tot;
+    const uint32_t o = block_exclusive_scan(v, sh, &tot);
+    cb.off[c] = o;
+    cb.cur[c] = o;
+    if (c == 0) { cb.off[MAX_COLOURS] = tot; cb.ctr[CTR_ACT] = tot; }
+    if (v) atomicMax(&top, c + 1u);
+    __syncthreads();
+    if (c == 0) cb.ctr[CTR_NCOL] = top;
+}
+// Solve list ordered by colour: each block counts its chunk per colour in LDS, reserves
+// ranges with one atomic per (block, colour), then scatters.
+__global__ void __launch_bounds__(256) k_colour_scatter(ColourBufs cb, uint32_t P, uint32_t *solve_list, uint32_t chunk) {
+    __shared__ uint32_t h[MAX_COLOURS];
+    __shared__ uint32_t base[MAX_COLOURS];
+    for (uint32_t c = threadIdx.x; c < (uint32_t)MAX_COLOURS; c += blockDim.x) h[c] = 0u;
+    __syncthreads();
+    const uint32_t p0 = blockIdx.x * chunk, p1 = min(P, p0 + chunk);
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const int c = cb.acol[p];
+        if (c >= 0) atomicAdd(&h[c], 1u);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < (uint32_t)MAX_COLOURS; c += blockDim.x) {
+        base[c] = h[c] ? atomicAdd(&cb.cur[c], h[c]) : 0u;
+        h[c] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const int c = cb.acol[p];
+        if (c >= 0) solve_list[base[c] + atomicAdd(&h[c], 1u)] = p;
+    }
 }
 
 // ------------------------------------------------------------------ narrowphase
@@ -769,26 +803,44 @@ __device__ __forceinline__ void solve_vel_pair(const Bodies &B, Contacts &K, uin
     if (Bd.dyn) { B.v[bb.y] = f4(Bd.v, 0.0f); B.w[bb.y] = f4(Bd.w, 0.0f); }
 }
 
-// Position / velocity solve of one substep: colour after colour, all in one cooperative
-// launch; pairs of one colour touch disjoint dynamic bodies (grid barrier between colours).
-__global__ void __launch_bounds__(256) k_solve_pos_coop(Bodies B, Contacts K, const uint32_t *order, const uint32_t *col_off,
-                                                        const uint32_t *ncol, GridBar bar) {
-    const uint32_t nc = *ncol;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t c = 0; c < nc; ++c) {
-        const uint32_t o = col_off[c], e = col_off[c + 1];
-        for (uint32_t t = o + blockIdx.x * blockDim.x + threadIdx.x; t < e; t += stride) solve_pos_pair(B, K, order[t]);
-        if (c + 1 < nc) grid_barrier(bar, nullptr);
+// Position / velocity solve of one colour: its pairs touch disjoint dynamic bodies.
+// One colour per launch.  With `pts`, also counts the contact points solved (algorithmic
+// bytes of the timed launches): one atomic per wave.
+__device__ __forceinline__ void count_points(const Contacts &K, uint32_t s, bool valid, unsigned long long *pts) {
+    uint32_t v = valid ? (uint32_t)K.cnt[s] : 0u;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(pts, (unsigned long long)v);
+}
+__global__ void __launch_bounds__(256) k_solve_pos(Bodies B, Contacts K, const uint32_t *order, uint32_t off, uint32_t count,
+                                                   unsigned long long *pts) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = t < count ? order[off + t] : 0u;
+    if (pts) count_points(K, s, t < count, pts);
+    if (t < count) solve_pos_pair(B, K, s);
+}
+__global__ void __launch_bounds__(256) k_solve_vel(Bodies B, Contacts K, const uint32_t *order, uint32_t off, uint32_t count,
+                                                   float h, unsigned long long *pts) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = t < count ? order[off + t] : 0u;
+    if (pts) count_points(K, s, t < count, pts);
+    if (t < count) solve_vel_pair(B, K, s, h);
+}
+// A run of small colours [c0, c1) in one workgroup: colours are solved in order with a
+// workgroup barrier between them (pairs of one colour share no dynamic body).
+__global__ void __launch_bounds__(TAIL_THREADS) k_solve_pos_run(Bodies B, Contacts K, const uint32_t *order,
+                                                                  const uint32_t *coff, uint32_t c0, uint32_t c1) {
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t e = coff[c + 1];
+        for (uint32_t i = coff[c] + threadIdx.x; i < e; i += blockDim.x) solve_pos_pair(B, K, order[i]);
+        __syncthreads();
     }
 }
-__global__ void __launch_bounds__(256) k_solve_vel_coop(Bodies B, Contacts K, const uint32_t *order, const uint32_t *col_off,
-                                                        const uint32_t *ncol, float h, GridBar bar) {
-    const uint32_t nc = *ncol;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t c = 0; c < nc; ++c) {
-        const uint32_t o = col_off[c], e = col_off[c + 1];
-        for (uint32_t t = o + blockIdx.x * blockDim.x + threadIdx.x; t < e; t += stride) solve_vel_pair(B, K, order[t], h);
-        if (c + 1 < nc) grid_barrier(bar, nullptr);
+__global__ void __launch_bounds__(TAIL_THREADS) k_solve_vel_run(Bodies B, Contacts K, const uint32_t *order,
+                                                                  const uint32_t *coff, uint32_t c0, uint32_t c1, float h) {
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t e = coff[c + 1];
+        for (uint32_t i = coff[c] + threadIdx.x; i < e; i += blockDim.x) solve_vel_pair(B, K, order[i], h);
+        __syncthreads();
     }
 }
 

@@ -1,10 +1,12 @@
 // Host orchestration of the XPBD substep pipeline and the rbgpu C-ABI.
 //
 // One world = one HIP stream.  Per frame (World::step, include/rbps/API/World.hpp:192-212):
-//   refit + stable endpoint sort + sweep (pairs), plane pairs appended, Jones-Plassmann
-//   colouring of the pair graph, solve slots ordered by colour, narrowphase bins;
-//   then per substep: narrowphase -> integrate -> position solve (per colour) ->
-//   velocity update -> velocity solve (per colour).
+//   refit + stable endpoint sort + sweep (pairs), plane pairs appended, narrowphase bins
+//   (one host synchronisation: pair count and bin offsets);
+//   then per substep: narrowphase -> colouring of the pairs with points (grid-wide rounds,
+//   then one workgroup for the rest; one host synchronisation for the colour offsets) ->
+//   integrate -> position solve -> velocity update -> velocity solve.  Each solve runs a
+//   large colour as its own launch and a run of small colours in one workgroup.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
@@ -50,8 +52,9 @@ template <class T> void dfree(T *&p) {
 struct Readback {
     uint32_t n_sap;
     uint32_t n_pairs;
-    uint32_t uncoloured;
-    int32_t dirty, overflow, colour_err, barrier_fault;
+    uint32_t pad0_;
+    int32_t dirty, overflow, pad1_, pad2_;
+    uint32_t ctr[CTR_SIZE];
     uint32_t col_off[MAX_COLOURS + 1];
     uint32_t bin_off[NBINS + 1];
     uint64_t points;
@@ -63,13 +66,6 @@ __global__ void k_offsets(const uint32_t *hoff, uint32_t ntiles, uint32_t nd, co
     uint32_t d = threadIdx.x + blockIdx.x * blockDim.x;
     if (d < nd) out[d] = hoff[d * ntiles];
     if (d == 0) out[nd] = *n;
-}
-
-__global__ void k_count_points(const int *cnt, uint32_t n, unsigned long long *acc) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t v = s < n ? (uint32_t)cnt[s] : 0u;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc, (unsigned long long)v);
 }
 
 }  // namespace
@@ -101,10 +97,10 @@ struct rbgpu_world {
     float2 *sw_b = nullptr;
     uint32_t *ep_cnt = nullptr, *ep_off = nullptr;
     uint32_t *ep_hist = nullptr;
-    int *flags = nullptr;            // [0] dirty [1] overflow [2] colour error [3] barrier fault
-    uint32_t *scal = nullptr;        // [0] n_sap [1] n_pairs [2] uncoloured [3] active pairs [4] colours [5] JP rounds
+    int *flags = nullptr;            // [0] dirty [1] overflow
+    uint32_t *scal = nullptr;        // [0] n_sap [1] n_pairs
     uint32_t *tile_sums = nullptr;   // scan scratch
-    uint32_t *body_deg = nullptr, *body_off = nullptr, *body_cur = nullptr;
+    ColourBufs cb{};                 // substep colouring (per-body state sized N, lists sized pcap)
     uint2 *plane_pairs = nullptr;
     uint32_t n_plane = 0;
     unsigned long long *d_points = nullptr;
@@ -112,23 +108,22 @@ struct rbgpu_world {
     // pair stage (capacity pcap); contact rows are indexed by pair id
     uint32_t pcap = 0;
     uint2 *pairs = nullptr;
-    uint32_t *adj = nullptr;
-    int *colour = nullptr;           // per active index
-    int *tent = nullptr;             // tentative colour per active index
-    uint32_t *cround = nullptr;      // colouring round per active index
     uint32_t *ka = nullptr, *va = nullptr, *kb = nullptr, *solve_list = nullptr, *bin_list = nullptr;
-    uint32_t *act_flag = nullptr, *act_off = nullptr, *act_list = nullptr;
-    uint32_t *p_hist = nullptr, *offs = nullptr;   // offs: [0..256] colour offsets, [257..267] bin offsets
-    uint32_t *bar = nullptr;         // grid barrier: count, generation, decision
-    uint32_t coop_jp = 0, coop_pos = 0, coop_vel = 0;   // cooperative grid sizes
+    uint32_t *p_hist = nullptr, *offs = nullptr;   // offs: bin offsets
 
     Readback *rb = nullptr;          // pinned
-    uint32_t n_pairs = 0, n_sap = 0, ncol = 0, jp_rounds = 0;
+    uint32_t n_pairs = 0, n_sap = 0, ncol = 0;
     std::vector<uint32_t> col_off, bin_off;
+    struct Run { uint32_t c0, c1, group; };
+    std::vector<Run> runs;          // this substep's solve launches
     uint32_t frames = 0;
+    // tuning: grid-wide colouring rounds before the single-workgroup rounds; colours with
+    // fewer pairs than run_min are solved in single-workgroup runs
+    uint32_t head_rounds = 96, round_batch = 12, tail_min = 2048, run_min = 256, palette_max = 8;
 
     // timing
     int timing_kernel = 0;
+    uint32_t timing_stride = 1;     // time every timing_stride-th frame
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
     size_t ev_used = 0;
     double timing_bytes = 0;
@@ -137,8 +132,9 @@ struct rbgpu_world {
     ~rbgpu_world() { release(); }
 
     void release_pairs() {
-        dfree(pairs); dfree(adj); dfree(colour); dfree(tent); dfree(cround); dfree(ka); dfree(va); dfree(kb);
-        dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(act_flag); dfree(act_off); dfree(act_list);
+        dfree(pairs); dfree(ka); dfree(va); dfree(kb);
+        dfree(solve_list); dfree(bin_list); dfree(p_hist);
+        dfree(cb.acol); dfree(cb.tent); dfree(cb.rest);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
         pcap = 0;
@@ -151,8 +147,9 @@ struct rbgpu_world {
         dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat);
         dfree(fmin); dfree(fmax); dfree(uid); dfree(ep_key[0]); dfree(ep_key[1]); dfree(ep_val[0]); dfree(ep_val[1]);
         dfree(ep_pos); dfree(sw_meta); dfree(sw_a); dfree(sw_b); dfree(ep_cnt); dfree(ep_off); dfree(ep_hist);
-        dfree(flags); dfree(scal); dfree(tile_sums); dfree(body_deg); dfree(body_off); dfree(body_cur);
-        dfree(plane_pairs); dfree(offs); dfree(d_points); dfree(bar);
+        dfree(flags); dfree(scal); dfree(tile_sums);
+        dfree(cb.used); dfree(cb.ucnt); dfree(cb.claim); dfree(cb.hist); dfree(cb.off); dfree(cb.cur); dfree(cb.ctr);
+        dfree(plane_pairs); dfree(offs); dfree(d_points);
         for (auto &e : ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         ev_pool.clear();
         if (rb) (void)hipHostFree(rb);
@@ -193,7 +190,7 @@ struct rbgpu_world {
     }
 
     hipEvent_t *timed_begin(int kernel) {
-        if (timing_kernel != kernel) return nullptr;
+        if (timing_kernel != kernel || frames % timing_stride) return nullptr;
         if (ev_used == ev_pool.size()) {
             hipEvent_t a, b;
             if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
@@ -221,10 +218,10 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     release_pairs();
     uint32_t cap = std::max<uint32_t>(need + need / 2, 4096);
     int rc = 0;
-    rc |= dalloc(pairs, cap); rc |= dalloc(adj, 2 * (size_t)cap); rc |= dalloc(colour, cap);
-    rc |= dalloc(tent, cap); rc |= dalloc(cround, cap);
+    rc |= dalloc(pairs, cap);
     rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
-    rc |= dalloc(bin_list, cap); rc |= dalloc(act_flag, cap); rc |= dalloc(act_off, cap); rc |= dalloc(act_list, cap);
+    rc |= dalloc(bin_list, cap);
+    rc |= dalloc(cb.acol, cap); rc |= dalloc(cb.tent, cap); rc |= dalloc(cb.rest, cap);
     uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
     rc |= dalloc(p_hist, (size_t)nt * RS_BINS);
     rc |= dalloc(K.cnt, cap); rc |= dalloc(K.cc, cap); rc |= dalloc(K.bb, cap); rc |= dalloc(K.nr, cap);
@@ -302,28 +299,12 @@ int rbgpu_world::build() {
     rc |= dalloc(ep_hist, (size_t)((NEP + RS_TILE - 1) / RS_TILE) * RS_BINS);
     rc |= dalloc(flags, 4); rc |= dalloc(scal, 8);
     rc |= dalloc(tile_sums, SCAN_TOP_THREADS * SCAN_TOP_ITEMS);
-    rc |= dalloc(body_deg, n + 1); rc |= dalloc(body_off, n + 1); rc |= dalloc(body_cur, n + 1);
-    rc |= dalloc(plane_pairs, planes.size()); rc |= dalloc(offs, MAX_COLOURS + NBINS + 4);
-    rc |= dalloc(d_points, 1); rc |= dalloc(bar, 4);
+    rc |= dalloc(cb.used, (size_t)n * COLOUR_WORDS); rc |= dalloc(cb.ucnt, n);
+    rc |= dalloc(cb.claim, (size_t)n * COLOUR_SLOTS); rc |= dalloc(cb.hist, MAX_COLOURS);
+    rc |= dalloc(cb.off, MAX_COLOURS + 1); rc |= dalloc(cb.cur, MAX_COLOURS); rc |= dalloc(cb.ctr, CTR_SIZE);
+    rc |= dalloc(plane_pairs, planes.size()); rc |= dalloc(offs, NBINS + 4);
+    rc |= dalloc(d_points, 1);
     if (rc) return RBGPU_ERR_OOM;
-    {
-        int dev = 0, ncu = 0, coop = 0;
-        HIPCHK(hipGetDevice(&dev));
-        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-        if (!coop) return fail(RBGPU_ERR_UNSUPPORTED, "device does not support cooperative launches");
-        auto grid_for = [&](const void *fn, uint32_t &out) -> int {
-            int nb = 0;
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0));
-            if (nb < 1) return fail(RBGPU_ERR_UNSUPPORTED, "cooperative kernel does not fit on a CU");
-            out = (uint32_t)ncu * (uint32_t)std::min(nb, 2);
-            return RBGPU_OK;
-        };
-        int r = grid_for((const void *)k_colour_coop, coop_jp);
-        if (!r) r = grid_for((const void *)k_solve_pos_coop, coop_pos);
-        if (!r) r = grid_for((const void *)k_solve_vel_coop, coop_vel);
-        if (r) return r;
-    }
     auto up = [&](void *dst, const void *src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st); };
     if (n) {
         HIPCHK(up(B.x, x.data(), n * 16)); HIPCHK(up(B.q, q.data(), n * 16)); HIPCHK(up(B.v, v.data(), n * 16));
@@ -345,8 +326,8 @@ int rbgpu_world::build() {
     int init_flags[4] = {1, 0, 0, 0};   // dirty: the endpoint array has never been sorted
     HIPCHK(up(flags, init_flags, sizeof(init_flags)));
     HIPCHK(hipMemsetAsync(scal, 0, 32, st));
+    HIPCHK(hipMemsetAsync(cb.ctr, 0, CTR_SIZE * 4, st));
     HIPCHK(hipMemsetAsync(d_points, 0, 8, st));
-    HIPCHK(hipMemsetAsync(bar, 0, 16, st));
     HIPCHK(hipStreamSynchronize(st));
     int rc2 = ensure_pair_capacity(std::max<uint32_t>(8 * S + n_plane, 4096));
     if (rc2) return rc2;
@@ -389,14 +370,11 @@ int rbgpu_world::frame() {
     const float dt = (float)timestep;
     const float h = dt / (float)substeps;
     const float inv_h = 1.0f / h;
-    int *dirty = flags, *overflow = flags + 1, *colerr = flags + 2;
+    int *dirty = flags, *overflow = flags + 1;
 
     // 1. refit (CollisionPipeline.cpp:14-53)
-    if (M) {
-        hipEvent_t *e = timed_begin(6);
-        k_refit<<<nblk(M), TPB, 0, st>>>(C, B, M, dt, margin, reinsert, fmin, fmax, ep_key[0], ep_pos, dirty);
-        timed_end(e);
-    }
+    hipEvent_t *ebp = timed_begin(6);
+    if (M) k_refit<<<nblk(M), TPB, 0, st>>>(C, B, M, dt, margin, reinsert, fmin, fmax, ep_key[0], ep_pos, dirty);
     // 2. stable sort of the endpoints if any fat box changed (BroadPhase.cpp:234-245)
     if (NEP) {
         for (int pass = 0; pass < 4; ++pass) {
@@ -427,29 +405,27 @@ int rbgpu_world::frame() {
         k_pair_gather<<<nblk(pcap), TPB, 0, st>>>(pairs, scal + 1, C, K, ka, va);
         int rc = radix_pass(ka, va, kb, bin_list, pcap, 0, p_hist, nullptr, scal + 1);
         if (rc) return rc;
-        k_offsets<<<1, 64, 0, st>>>(p_hist, (pcap + RS_TILE - 1) / RS_TILE, NBINS, scal + 1, offs + MAX_COLOURS + 1);
+        k_offsets<<<1, 64, 0, st>>>(p_hist, (pcap + RS_TILE - 1) / RS_TILE, NBINS, scal + 1, offs);
         HIPCHK(hipMemcpyAsync(&rb->n_sap, scal, 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(&rb->overflow, overflow, 12, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(rb->bin_off, offs + MAX_COLOURS + 1, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&rb->overflow, overflow, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(rb->bin_off, offs, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (rb->barrier_fault) return fail(RBGPU_ERR_HIP, "grid barrier timed out in a cooperative kernel");
-        if (rb->colour_err) return fail(RBGPU_ERR_CAPACITY, rb->colour_err == 1 ? "more than 256 colours needed"
-                                                                                 : "colouring did not converge");
         if (!rb->overflow) break;
         if (attempt > 0) return fail(RBGPU_ERR_CAPACITY, "pair buffer overflow");
         int rc2 = ensure_pair_capacity(rb->n_pairs);
         if (rc2) return rc2;
     }
+    timed_end(ebp);
     n_sap = rb->n_sap;
     n_pairs = rb->n_pairs;
     bin_off.assign(rb->bin_off, rb->bin_off + NBINS + 1);
     const uint32_t P = n_pairs;
-    const uint32_t ntp = (std::max<uint32_t>(P, 1) + RS_TILE - 1) / RS_TILE;
-    const uint32_t colour_rounds_cap = 256;
-    GridBar gb{bar, bar + 1, (int *)(bar + 2), flags + 3};
+    const uint32_t colour_rounds_cap = 1u << 16;
 
     // 4. substeps (World.hpp:203-211)
+    cb.pal_max = palette_max;
     for (int s = 0; s < substeps; ++s) {
+        runs.clear();
         if (P) {
             hipEvent_t *e = timed_begin(3);
             for (int b = 0; b < NBINS; ++b) {
@@ -470,69 +446,83 @@ int rbgpu_world::frame() {
                 }
             }
             timed_end(e);
-            // colouring of the active pairs (edges = shared dynamic bodies)
-            k_active_flags<<<nblk(P), TPB, 0, st>>>(K.cnt, P, act_flag);
-            int rc = scan(act_flag, act_off, P, scal + 3);
-            if (rc) return rc;
-            k_compact<<<nblk(std::max(P, N + 1)), TPB, 0, st>>>(act_flag, act_off, P, scal + 3, act_list, colour, cround,
-                                                                 scal + 2, scal + 4, body_deg, body_cur, N);
-            k_act_deg<<<nblk(P), TPB, 0, st>>>(act_list, scal + 3, K.bb, B, body_deg);
-            rc = scan(body_deg, body_off, N + 1, nullptr);
-            if (rc) return rc;
-            k_act_adj<<<nblk(P), TPB, 0, st>>>(act_list, scal + 3, K.bb, B, body_off, body_cur, adj);
-            {
-                const uint32_t *a0 = act_list, *a1 = scal + 3;
-                const uint2 *a2 = K.bb;
-                Bodies a3 = B;
-                const uint32_t *a4 = body_off, *a5 = body_deg, *a6 = adj;
-                int *a7 = colour, *a8 = tent;
-                uint32_t *a9 = cround, *a10 = scal + 2, *a11 = scal + 4;
-                int *a12 = colerr;
-                GridBar a13 = gb;
-                uint32_t a14 = colour_rounds_cap;
-                uint32_t *a15 = scal + 5;
-                void *args[] = {&a0, &a1, &a2, &a3, &a4, &a5, &a6, &a7, &a8, &a9, &a10, &a11, &a12, &a13, &a14, &a15};
-                HIPCHK(hipLaunchCooperativeKernel((const void *)k_colour_coop, dim3(coop_jp), dim3(256), args, 0, st));
+            // colouring of the pairs with points (conflict = a shared dynamic body)
+            hipEvent_t *ec = timed_begin(7);
+            k_colour_reset<<<nblk(std::max<uint32_t>(N * (COLOUR_WORDS / 4), MAX_COLOURS)), TPB, 0, st>>>(N, cb);
+            k_colour_activate<<<nblk(P), TPB, 0, st>>>(K.cnt, P, K.bb, B, cb);
+            // grid-wide rounds in batches (they skip themselves once at most tail_min pairs are
+            // left); a host check between batches
+            uint32_t launched = 0;
+            for (;;) {
+                const uint32_t end = std::min(head_rounds, launched + round_batch);
+                for (; launched < end; ++launched) {
+                    k_colour_claim<<<nblk(P), TPB, 0, st>>>(K.bb, B, cb, P, launched, tail_min);
+                    k_colour_settle<<<nblk(P), TPB, 0, st>>>(K.bb, B, cb, P, launched);
+                }
+                if (launched >= head_rounds) break;
+                HIPCHK(hipMemcpyAsync(rb->ctr, cb.ctr, CTR_SIZE * 4, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                uint32_t left = 0;
+                for (uint32_t q = 0; q < LEFT_STRIPES; ++q) left += rb->ctr[CTR_LEFT + q];
+                if (left <= tail_min || !rb->ctr[CTR_FLAGS + launched - 1]) break;
             }
-            k_colour_keys<<<nblk(P), TPB, 0, st>>>(colour, act_list, scal + 3, ka, va);
-            rc = radix_pass(ka, va, kb, solve_list, P, 0, p_hist, nullptr, scal + 3);
-            if (rc) return rc;
-            k_offsets<<<1, MAX_COLOURS, 0, st>>>(p_hist, ntp, MAX_COLOURS, scal + 3, offs);
+            k_colour_collect<<<nblk(P), TPB, 0, st>>>(cb, P);
+            k_colour_tail<<<1, TAIL_THREADS, 0, st>>>(K.bb, B, cb, launched, colour_rounds_cap);
+            const uint32_t hist_blocks = std::min<uint32_t>(nblk(P), 512);
+            k_colour_hist<<<hist_blocks, 256, 0, st>>>(cb, P);
+            k_colour_offsets<<<1, MAX_COLOURS, 0, st>>>(cb);
+            const uint32_t chunk = 4096;
+            k_colour_scatter<<<nblk(P, chunk), 256, 0, st>>>(cb, P, solve_list, chunk);
+            HIPCHK(hipMemcpyAsync(rb->ctr, cb.ctr, CTR_SIZE * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(rb->col_off, cb.off, (MAX_COLOURS + 1) * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (rb->ctr[CTR_ERR] & 1u) return fail(RBGPU_ERR_CAPACITY, "more than 1024 colours needed");
+            if (rb->ctr[CTR_ERR] & 2u) return fail(RBGPU_ERR_CAPACITY, "colouring did not converge");
+            ncol = rb->ctr[CTR_NCOL];
+            col_off.assign(rb->col_off, rb->col_off + ncol + 1);
+            runs.clear();   // [c0, c1, single-workgroup?]
+            for (uint32_t c = 0; c < ncol;) {
+                if (col_off[c + 1] - col_off[c] >= run_min) { runs.push_back({c, c + 1, 0}); ++c; continue; }
+                uint32_t c1 = c + 1;
+                while (c1 < ncol && col_off[c1 + 1] - col_off[c1] < run_min) ++c1;
+                runs.push_back({c, c1, 1});
+                c = c1;
+            }
+            timed_end(ec);
         }
         if (N) {
             hipEvent_t *e = timed_begin(4);
             k_integrate<<<nblk(N), TPB, 0, st>>>(B, N, h);
             timed_end(e);
         }
-        if (P) {
-            const uint32_t *a2 = solve_list, *a3 = offs, *a4 = scal + 4;
-            Bodies a0 = B;
-            Contacts a1 = K;
-            GridBar a5 = gb;
-            void *args[] = {&a0, &a1, &a2, &a3, &a4, &a5};
-            hipEvent_t *e = timed_begin(1);
-            HIPCHK(hipLaunchCooperativeKernel((const void *)k_solve_pos_coop, dim3(coop_pos), dim3(256), args, 0, st));
-            timed_end(e);
+        for (const Run &u : runs) {
+            if (u.group) {
+                hipEvent_t *e = timed_begin(8);
+                k_solve_pos_run<<<1, TAIL_THREADS, 0, st>>>(B, K, solve_list, cb.off, u.c0, u.c1);
+                timed_end(e);
+            } else {
+                const uint32_t o = col_off[u.c0], n = col_off[u.c1] - o;
+                hipEvent_t *e = timed_begin(1);
+                k_solve_pos<<<nblk(n), TPB, 0, st>>>(B, K, solve_list, o, n, e ? d_points : nullptr);
+                timed_end(e);
+            }
         }
         if (N) {
             hipEvent_t *e = timed_begin(5);
             k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h);
             timed_end(e);
         }
-        if (P) {
-            const uint32_t *a2 = solve_list, *a3 = offs, *a4 = scal + 4;
-            Bodies a0 = B;
-            Contacts a1 = K;
-            float a5 = h;
-            GridBar a6 = gb;
-            void *args[] = {&a0, &a1, &a2, &a3, &a4, &a5, &a6};
-            hipEvent_t *e = timed_begin(2);
-            HIPCHK(hipLaunchCooperativeKernel((const void *)k_solve_vel_coop, dim3(coop_vel), dim3(256), args, 0, st));
-            timed_end(e);
-        }
-        if (timing_kernel == 1 || timing_kernel == 2) {
-            // algorithmic bytes need the substep's contact count: accumulate on the device
-            if (P) k_count_points<<<nblk(P), TPB, 0, st>>>(K.cnt, P, d_points);
+        for (const Run &u : runs) {
+            if (u.group) {
+                hipEvent_t *e = timed_begin(9);
+                k_solve_vel_run<<<1, TAIL_THREADS, 0, st>>>(B, K, solve_list, cb.off, u.c0, u.c1, h);
+                timed_end(e);
+            } else {
+                const uint32_t o = col_off[u.c0], n = col_off[u.c1] - o;
+                hipEvent_t *e = timed_begin(2);
+                k_solve_vel<<<nblk(n), TPB, 0, st>>>(B, K, solve_list, o, n, h, e ? d_points : nullptr);
+                timed_end(e);
+            }
         }
     }
     ++frames;
@@ -706,18 +696,10 @@ int rbgpu_get_pair_colours(rbgpu_world *w, uint32_t *pairs2, int32_t *colour, ui
     const uint32_t k = std::min(cap, w->n_pairs);
     if (k && pairs2) HIPCHK(hipMemcpy(pairs2, w->pairs, (size_t)k * 8, hipMemcpyDeviceToHost));
     if (k && colour) {
-        // colour of each pair in the last substep's colouring of the active pairs; -1 = inactive
-        uint32_t na = 0;
-        HIPCHK(hipMemcpy(&na, w->scal + 3, 4, hipMemcpyDeviceToHost));
-        std::vector<uint32_t> act(na);
-        std::vector<int> col(na);
-        if (na) {
-            HIPCHK(hipMemcpy(act.data(), w->act_list, (size_t)na * 4, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(col.data(), w->colour, (size_t)na * 4, hipMemcpyDeviceToHost));
-        }
-        for (uint32_t p = 0; p < k; ++p) colour[p] = -1;
-        for (uint32_t i = 0; i < na; ++i)
-            if (act[i] < k) colour[act[i]] = col[i];
+        // colour of each pair in the last substep's colouring; -1 = no points
+        std::vector<int> col(w->n_pairs);
+        HIPCHK(hipMemcpy(col.data(), w->cb.acol, (size_t)w->n_pairs * 4, hipMemcpyDeviceToHost));
+        for (uint32_t p = 0; p < k; ++p) colour[p] = col[p] >= 0 ? col[p] : -1;
     }
     return RBGPU_OK;
 }
@@ -765,10 +747,11 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out) {
     out->pairs = w->n_pairs;
     if (w->built) {
         HIPCHK(hipStreamSynchronize(w->st));
-        uint32_t sc[8];
-        HIPCHK(hipMemcpy(sc, w->scal, sizeof(sc), hipMemcpyDeviceToHost));
-        out->colours = sc[4];
-        out->jp_rounds = sc[5];
+        uint32_t sc[CTR_SIZE];
+        HIPCHK(hipMemcpy(sc, w->cb.ctr, sizeof(sc), hipMemcpyDeviceToHost));
+        out->colours = sc[CTR_NCOL];
+        out->colour_rounds = sc[CTR_ROUNDS];
+        out->active_pairs = sc[CTR_ACT];
     }
     for (int b = 0; b < NBINS && (size_t)b + 1 < w->bin_off.size(); ++b) out->bins[b] = w->bin_off[b + 1] - w->bin_off[b];
     if (w->built && w->n_pairs) {
@@ -782,8 +765,46 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out) {
     return RBGPU_OK;
 }
 
+int rbgpu_set_option(rbgpu_world *w, int option, int value) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    switch (option) {
+        case RBGPU_OPT_PALETTE_MAX:
+            if (value < 1 || value > 64) return fail(RBGPU_ERR_INVALID, "palette maximum must be in [1, 64]");
+            w->palette_max = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_HEAD_ROUNDS:
+            if (value < 1 || value > (int)MAX_HEAD_ROUNDS) return fail(RBGPU_ERR_INVALID, "head rounds must be in [1, 256]");
+            w->head_rounds = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_ROUND_BATCH:
+            if (value < 1) return fail(RBGPU_ERR_INVALID, "round batch must be >= 1");
+            w->round_batch = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_TAIL_MIN:
+            if (value < 0) return fail(RBGPU_ERR_INVALID, "tail minimum must be >= 0");
+            w->tail_min = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_RUN_MIN:
+            if (value < 0) return fail(RBGPU_ERR_INVALID, "run minimum must be >= 0");
+            w->run_min = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_TIMING_STRIDE:
+            if (value < 1) return fail(RBGPU_ERR_INVALID, "timing stride must be >= 1");
+            w->timing_stride = (uint32_t)value; return RBGPU_OK;
+        default: return fail(RBGPU_ERR_INVALID, "unknown option");
+    }
+}
+
+int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
+    if (!w || !value) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    switch (option) {
+        case RBGPU_OPT_PALETTE_MAX: *value = (int)w->palette_max; return RBGPU_OK;
+        case RBGPU_OPT_HEAD_ROUNDS: *value = (int)w->head_rounds; return RBGPU_OK;
+        case RBGPU_OPT_ROUND_BATCH: *value = (int)w->round_batch; return RBGPU_OK;
+        case RBGPU_OPT_TAIL_MIN: *value = (int)w->tail_min; return RBGPU_OK;
+        case RBGPU_OPT_RUN_MIN: *value = (int)w->run_min; return RBGPU_OK;
+        case RBGPU_OPT_TIMING_STRIDE: *value = (int)w->timing_stride; return RBGPU_OK;
+        default: return fail(RBGPU_ERR_INVALID, "unknown option");
+    }
+}
+
 int rbgpu_set_timing(rbgpu_world *w, int kernel) {
-    if (!w || kernel < 0 || kernel > 6) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (!w || kernel < 0 || kernel > 9) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     if (w->st) HIPCHK(hipStreamSynchronize(w->st));
     w->timing_kernel = kernel;
     w->ev_used = 0;
@@ -805,7 +826,8 @@ int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, doubl
     if (bytes) {
         unsigned long long pts = 0;
         if (w->d_points) HIPCHK(hipMemcpy(&pts, w->d_points, 8, hipMemcpyDeviceToHost));
-        // algorithmic bytes per contact point: position solve 296 B, velocity solve 280 B
+        // algorithmic bytes per contact point of the timed per-colour launches: position
+        // solve 296 B, velocity solve 280 B
         *bytes = (double)pts * (w->timing_kernel == 1 ? 296.0 : w->timing_kernel == 2 ? 280.0 : 0.0);
     }
     return RBGPU_OK;

@@ -29,7 +29,7 @@ EXPORTS = [
     "rbgpu_world_create", "rbgpu_world_destroy", "rbgpu_add_bodies", "rbgpu_add_colliders", "rbgpu_step",
     "rbgpu_set_step", "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
     "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_collide_pairs",
-    "rbgpu_last_error",
+    "rbgpu_set_option", "rbgpu_get_option", "rbgpu_last_error",
 ]
 
 
@@ -39,7 +39,8 @@ class RbgpuError(RuntimeError):
 
 class Stats(C.Structure):
     _fields_ = [("frames", C.c_uint32), ("sap_pairs", C.c_uint32), ("pairs", C.c_uint32), ("colours", C.c_uint32),
-                ("jp_rounds", C.c_uint32), ("contacts", C.c_uint32), ("bins", C.c_uint32 * 10)]
+                ("colour_rounds", C.c_uint32), ("contacts", C.c_uint32), ("bins", C.c_uint32 * 10),
+                ("active_pairs", C.c_uint32)]
 
 
 _lib = None
@@ -66,6 +67,8 @@ def lib():
         L.rbgpu_get_contacts.argtypes = [_P, _U, _D, _D, _D, _D, _D, C.c_uint32, _U]
         L.rbgpu_get_stats.argtypes = [_P, C.POINTER(Stats)]
         L.rbgpu_set_timing.argtypes = [_P, C.c_int]
+        L.rbgpu_set_option.argtypes = [_P, C.c_int, C.c_int]
+        L.rbgpu_get_option.argtypes = [_P, C.c_int, _I]
         L.rbgpu_get_timing.argtypes = [_P, _D, C.POINTER(C.c_uint64), _D]
         L.rbgpu_collide_pairs.argtypes = [C.c_int, _I, _D, _D, _I, _D, _D, _D]
         L.rbgpu_last_error.restype = C.c_char_p
@@ -94,7 +97,10 @@ def collide_pairs(kind_a, dims_a, tf_a, kind_b, dims_b, tf_b) -> np.ndarray:
     return out
 
 
-TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "sweep": 6}
+OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, "run_min": 4, "timing_stride": 5}
+
+TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "broadphase": 6,
+          "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9}
 
 
 class World:
@@ -211,7 +217,18 @@ class World:
         s = Stats()
         _check(lib().rbgpu_get_stats(self.h, C.byref(s)))
         return {"frames": s.frames, "sap_pairs": s.sap_pairs, "pairs": s.pairs, "colours": s.colours,
-                "jp_rounds": s.jp_rounds, "contacts": s.contacts, "bins": list(s.bins)}
+                "colour_rounds": s.colour_rounds, "active_pairs": s.active_pairs, "contacts": s.contacts,
+                "bins": list(s.bins)}
+
+    # ---------------------------------------------------------------- options
+    def set_option(self, name: str, value: int):
+        """Pipeline options (OPTIONS); palette_max changes the colouring, the others only speed."""
+        _check(lib().rbgpu_set_option(self.h, OPTIONS[name], int(value)))
+
+    def option(self, name: str) -> int:
+        v = C.c_int32()
+        _check(lib().rbgpu_get_option(self.h, OPTIONS[name], C.byref(v)))
+        return v.value
 
     # ---------------------------------------------------------------- timing
     def set_timing(self, kernel: str):

@@ -86,42 +86,48 @@ def sorted_pairs(p):
     return p[np.lexsort((p[:, 1], p[:, 0]))]
 
 
-def trial_colouring(pairs_bodies, dynamic, active, mix32):
-    """Independent replay of the active-pair colouring (synchronous randomized trials):
-    round r, uncoloured pair p picks the (mix32(p ^ r*0x9e3779b9) mod k)-th colour unused by
-    neighbours coloured in earlier rounds (k = uncoloured neighbour entries + 1) and keeps it
-    unless an uncoloured neighbour of higher priority (mix32(q), q) picked the same."""
+def trial_colouring(pairs_bodies, dynamic, active, mix32, slots=16, palette_max=8):
+    """Independent replay of the active-pair colouring (synchronous randomized rounds with
+    per-body slot claims).  Round r, every uncoloured pair p with dynamic bodies D(p):
+    t(p) = the (tent mod min(max(ceil(k/4), 1), 8))-th colour used by no pair of D(p) so far,
+    k = uncoloured pairs at D(p) summed; key(p) = (r + 1, rank); p keeps t(p) iff its key is
+    the largest among this round's uncoloured pairs of each v in D(p) whose t has the
+    same residue mod `slots`."""
+    M = 0xffffffff
+    tent_h = lambda p, r: mix32(p ^ ((r * 0x9e3779b9) & M))
+    rank_h = lambda p, r: mix32(p ^ ((r * 0x85ebca6b) & M) ^ 0xc2b2ae35)
     P = len(pairs_bodies)
     act = [p for p in range(P) if active[p]]
-    by_body = {}
+    dyn_of = {p: [int(b) for b in pairs_bodies[p] if dynamic[int(b)]] for p in act}
+    used, unc = {}, {}
     for p in act:
-        for b in (int(pairs_bodies[p][0]), int(pairs_bodies[p][1])):
-            if dynamic[b]:
-                by_body.setdefault(b, []).append(p)
-    nbrs = {p: [q for b in (int(pairs_bodies[p][0]), int(pairs_bodies[p][1])) if dynamic[b]
-                for q in by_body[b] if q != p] for p in act}
+        for b in dyn_of[p]:
+            used.setdefault(b, set())
+            unc[b] = unc.get(b, 0) + 1
     col = {p: -1 for p in act}
     r = 0
     while any(c < 0 for c in col.values()):
-        tent = {}
+        tent, key, best = {}, {}, {}
         for p in act:
             if col[p] >= 0:
                 continue
-            used = {col[q] for q in nbrs[p] if col[q] >= 0}
-            k = 1 + sum(1 for q in nbrs[p] if col[q] < 0)
-            idx = mix32(p ^ ((r * 0x9e3779b9) & 0xffffffff)) % k
+            taken = set().union(*[used[b] for b in dyn_of[p]]) if dyn_of[p] else set()
+            k = sum(unc[b] for b in dyn_of[p])
+            idx = tent_h(p, r) % min(max((k + 3) // 4, 1), palette_max)
             c = 0
-            while True:
-                if c not in used:
-                    if idx == 0:
-                        break
+            while c in taken or idx > 0:
+                if c not in taken:
                     idx -= 1
                 c += 1
-            tent[p] = c
-        acc = [p for p in tent if not any(col[q] < 0 and tent[q] == tent[p] and (mix32(q), q) > (mix32(p), p)
-                                          for q in nbrs[p])]
-        for p in acc:
+            tent[p], key[p] = c, ((r + 1) << 32) | rank_h(p, r)
+            for b in dyn_of[p]:
+                best[(b, c % slots)] = max(best.get((b, c % slots), 0), key[p])
+        won = [p for p in tent if all(best[(b, tent[p] % slots)] == key[p] for b in dyn_of[p])]
+        for p in won:
             col[p] = tent[p]
+            for b in dyn_of[p]:
+                used[b].add(tent[p])
+                unc[b] -= 1
         r += 1
     out = np.full(P, -1, np.int32)
     for p, c in col.items():

@@ -63,7 +63,7 @@ def test_pair_set_brute_force_first_frame():
 @pytest.mark.parametrize("frames", [2, 6])
 def test_colour_order_colouring_is_valid_and_replayed(frames):
     """The oracle's substep colouring equals an independent Python replay of the
-    randomized-trial spec, is valid, and uses at most (max degree + 1) colours."""
+    randomized-round spec, is valid, and uses at most 2*(max degree - 1) + 8 colours."""
     sc = scenes.pile_mixed_dense()
     w = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
     w.step(frames)
@@ -79,7 +79,9 @@ def test_colour_order_colouring_is_valid_and_replayed(frames):
     assert (sp["colour"][~active] == -1).all()
     assert sp["ncolours"] == sp["colour"].max() + 1
     body_deg = np.bincount(bodies[active].ravel(), minlength=len(dyn))[dyn].max()
-    assert sp["ncolours"] <= 2 * body_deg - 1       # a pair has at most 2*(body degree - 1) neighbours
+    # a pair has at most 2*(body degree - 1) neighbours and picks among the first 8 free colours
+    assert sp["ncolours"] <= 2 * (body_deg - 1) + 8
+    assert rounds == sp["rounds"]
 
 
 def test_mix32_known_values():
