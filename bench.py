@@ -47,6 +47,27 @@ def cpu_baseline(scene_fn, budget_s=10.0, max_frames=30):
             "sample": f"frames 1..{frames} of the same {sc.n_dynamic}-body scene ({dt:.1f} s, FP64, serial)"}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_pile_mixed_65k_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return round(json.load(f)["traffic_bytes_per_launch"], 1)
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20)
+    x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(10):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 10}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,7 +77,13 @@ def main():
                     choices=["off", "solve_pos", "solve_vel", "narrowphase", "integrate", "update_vel", "broadphase", "colouring",
                              "solve_pos_runs", "solve_vel_runs"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB ten times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a --pmc run to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(independent scenes would shard with no per-substep traffic, see DESIGN.md)", file=sys.stderr)
@@ -72,7 +99,12 @@ def main():
     w.set_option("timing_stride", 5)
     w.set_timing(args.timing_kernel)
     t0 = time.perf_counter()
-    w.step(args.steps)
+    done = 0
+    while done < args.steps:   # chunks only to print progress; the work is the same
+        k = min(50, args.steps - done)
+        w.step(k)
+        done += k
+        print(f"bench: {done}/{args.steps} frames", file=sys.stderr, flush=True)
     w.synchronize()
     dt = time.perf_counter() - t0
     tm = w.timing()
@@ -85,7 +117,8 @@ def main():
         bytes_per_launch = tm["bytes"] / tm["launches"]
         achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": args.timing_kernel,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.timing_kernel),
+                "kernel": args.timing_kernel,
                 "avg_launch_us": round(avg_s * 1e6, 3), "launches": tm["launches"],
                 "algorithmic_bytes_per_launch": round(bytes_per_launch, 1)}
 
