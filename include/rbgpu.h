@@ -60,6 +60,37 @@ typedef struct {
     double restitution, static_friction, dynamic_friction;
 } rbgpu_collider_desc;
 
+/* Joint types and actuation: the reference's JointType and JointActuationType order
+ * (include/rbps/constraints/Joint.hpp:25-47); SPHERICAL is this build's extension
+ * (a ball joint: one positional attachment row, plus a swing limit on `axis` when
+ * limited, upper_limit = cone half-angle). */
+enum {
+    RBGPU_JOINT_PRISMATIC = 0,
+    RBGPU_JOINT_REVOLUTE = 1,
+    RBGPU_JOINT_FIXED = 2,
+    RBGPU_JOINT_SPHERICAL = 3
+};
+enum { RBGPU_ACT_FREE = 0, RBGPU_ACT_POSITION = 1, RBGPU_ACT_SPEED = 2 };
+
+/* Fields of rbps::PrismaticJointParams / RevoluteJointParams / FixedJointParams
+ * (include/rbps/API/JointAPI.hpp:22-68) plus the joint's public target fields
+ * (JointCollection::target_position / target_speed, Joint.hpp:82-106).  168 bytes.
+ * axis: prismatic moving_axis | revolute aligned_axis | spherical swing axis;
+ * limit_axis: revolute only.  compliance lands on the row the reference gives it
+ * (prismatic drive, revolute drive, spherical attachment). */
+typedef struct {
+    int32_t type;
+    int32_t actuation;
+    uint32_t body_1, body_2;
+    double r_1[3], r_2[3];       /* body-local attachment points */
+    double axis[3];
+    double limit_axis[3];
+    double compliance, damping;
+    int32_t limited, _pad;
+    double lower_limit, upper_limit;
+    double target_position, target_speed;
+} rbgpu_joint_desc;
+
 typedef struct {
     uint32_t frames;             /* World::step calls executed */
     uint32_t sap_pairs;          /* broadphase pairs of the last frame (after the pipeline's filters) */
@@ -84,6 +115,22 @@ int rbgpu_add_bodies(rbgpu_world *w, int n, const rbgpu_body_desc *desc, uint32_
 /* World::create_collider / rbps::create_collider — include/rbps/API/World.hpp:124-132,
  * include/rbps/API/ColliderAPI.hpp:61-91 (inserts into the broadphase). */
 int rbgpu_add_colliders(rbgpu_world *w, int n, const rbgpu_collider_desc *desc, uint32_t *ids_out);
+
+/* World::create_prismatic_joint / create_revolute_joint / create_fixed_joint —
+ * include/rbps/API/World.hpp:141-156, include/rbps/API/JointAPI.hpp:160-265 — and the
+ * spherical extension.  Joint ids are creation indices.  ids_out may be NULL. */
+int rbgpu_add_joints(rbgpu_world *w, int n, const rbgpu_joint_desc *desc, uint32_t *ids_out);
+
+/* Public joint state after the last step (JointCollection::current_position,
+ * target_position; ConstraintCollection::lambda of the joint's rows, 4 per joint, unused
+ * rows 0) and the joint's solve colour.  Any pointer may be NULL. */
+int rbgpu_get_joints(rbgpu_world *w, double *current_position, double *target_position, double *lambda4,
+                     int32_t *colour);
+
+/* Writes JointCollection::target_position / target_speed of the given joints (the
+ * reference's callers assign these public fields directly).  Either array may be NULL. */
+int rbgpu_set_joint_targets(rbgpu_world *w, int n, const uint32_t *ids, const double *target_position,
+                            const double *target_speed);
 
 /* World::step — include/rbps/API/World.hpp:192-212; runs n_frames frames, asynchronously
  * on the world's stream (the call returns once the work is queued and the per-frame
@@ -138,14 +185,24 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *   RBGPU_OPT_TAIL_MIN     colouring: at most this many uncoloured pairs go to the
  *                          single-workgroup rounds (2048)
  *   RBGPU_OPT_RUN_MIN      solve: colours with fewer pairs run in single-workgroup runs (256)
- *   RBGPU_OPT_TIMING_STRIDE  kernel timing (rbgpu_set_timing) covers every n-th frame (1) */
+ *   RBGPU_OPT_TIMING_STRIDE  kernel timing (rbgpu_set_timing) covers every n-th frame (1)
+ *   RBGPU_OPT_JOINT_MODE     joint solver (0 = the reference's, default).  The reference
+ *                          evaluates every joint's errors once per substep before any row
+ *                          is solved, never resets lambda, gives rows with errors in
+ *                          (1e-9, ~3.2e-5) a zero direction (a ROTATIONAL row of compliance 0
+ *                          then evaluates 0/0: NaN poses) and builds the revolute alignment
+ *                          row as a1 x a2, which turns the axes apart; chains come apart and
+ *                          hinges diverge.  1 = XPBD rows (Mueller et al. 2020): each row's
+ *                          error re-evaluated just before it is solved, lambda reset every
+ *                          substep, direction v/|v| whenever |v| > 1e-9, alignment a2 x a1. */
 enum {
     RBGPU_OPT_PALETTE_MAX = 0,
     RBGPU_OPT_HEAD_ROUNDS = 1,
     RBGPU_OPT_ROUND_BATCH = 2,
     RBGPU_OPT_TAIL_MIN = 3,
     RBGPU_OPT_RUN_MIN = 4,
-    RBGPU_OPT_TIMING_STRIDE = 5
+    RBGPU_OPT_TIMING_STRIDE = 5,
+    RBGPU_OPT_JOINT_MODE = 6
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);

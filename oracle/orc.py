@@ -41,6 +41,10 @@ def _bind(lib, prefix):
     f("num_contacts").argtypes = [_P]
     f("get_contacts").argtypes = [_P, _U, _D, _D, _D, _D, _D]
     f("collide").restype = C.c_int
+    f("add_joints").argtypes = [_P, C.c_int, _P]
+    f("num_joints").argtypes = [_P]
+    f("get_joints").argtypes = [_P, _D, _D, _D, _I]
+    f("set_joint_targets").argtypes = [_P, C.c_int, _U, _D, _D]
 
 
 _libs = {}
@@ -60,6 +64,13 @@ def oracle_lib():
         lib.orc_num_colours.argtypes = [_P]
         lib.orc_colour_rounds.argtypes = [_P]
         lib.orc_set_palette_max.argtypes = [_P, C.c_int]
+        lib.orc_set_joint_mode.argtypes = [_P, C.c_int]
+        lib.orc_joint_errors.argtypes = [C.c_int, _P, _D, _D, C.c_double, _D]
+        for f in ("orc_sinf", "orc_cosf"):
+            getattr(lib, f).restype = C.c_float
+            getattr(lib, f).argtypes = [C.c_float]
+        lib.orc_atan2f.restype = C.c_float
+        lib.orc_atan2f.argtypes = [C.c_float, C.c_float]
         lib.orc_mix32.restype = C.c_uint32
         lib.orc_mix32.argtypes = [C.c_uint32]
         _libs["orc"] = lib
@@ -80,6 +91,9 @@ def ref_lib():
         _bind(lib, "ref_")
         lib.ref_collide.argtypes = [_P, _D, _P, _D, _D]
         lib.ref_collision_groups.argtypes = [_P, _I]
+        lib.ref_add_joints.restype = C.c_int
+        lib.ref_joint_errors.restype = C.c_int
+        lib.ref_joint_errors.argtypes = [_P, _D, _D, C.c_double, _D]
         _libs["ref"] = lib
     return _libs["ref"]
 
@@ -108,7 +122,24 @@ class _WorldBase:
         c = np.ascontiguousarray(scene.colliders)
         self._fn("add_bodies")(self.h, len(b), b.ctypes.data_as(_P))
         self._fn("add_colliders")(self.h, len(c), c.ctypes.data_as(_P))
+        j = np.ascontiguousarray(getattr(scene, "joints", np.zeros(0)))
+        if len(j):
+            if self._fn("add_joints")(self.h, len(j), j.ctypes.data_as(_P)):
+                raise RuntimeError(f"{self.prefix}add_joints: unsupported joint type")
         return self
+
+    def joints(self):
+        n = self._fn("num_joints")(self.h)
+        cur, tgt, lam, col = np.zeros(n), np.zeros(n), np.zeros((n, 4)), np.zeros(n, np.int32)
+        if n:
+            self._fn("get_joints")(self.h, _ptr(cur, _D), _ptr(tgt, _D), _ptr(lam, _D), _ptr(col, _I))
+        return {"current_position": cur, "target_position": tgt, "lambda": lam, "colour": col}
+
+    def set_joint_targets(self, ids, target_position=None, target_speed=None):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        tp = None if target_position is None else np.ascontiguousarray(target_position, np.float64)
+        ts = None if target_speed is None else np.ascontiguousarray(target_speed, np.float64)
+        self._fn("set_joint_targets")(self.h, len(ids), _ptr(ids, _U), _ptr(tp, _D), _ptr(ts, _D))
 
     def step(self, frames=1):
         if self._fn("step")(self.h, frames) != 0:
@@ -152,6 +183,10 @@ class OracleWorld(_WorldBase):
         lib = oracle_lib()
         super().__init__(lib, lib.orc_create(1 if fp32 else 0, order, timestep, substeps))
         self.fp32, self.order = fp32, order
+
+    def set_joint_mode(self, mode):
+        """0: the reference's joint solver; 1: XPBD rows (see orc_world.hpp)."""
+        self.lib.orc_set_joint_mode(self.h, int(mode))
 
     def set_palette_max(self, m):
         if self.lib.orc_set_palette_max(self.h, m) != 0:
@@ -200,6 +235,39 @@ def oracle_collide(a, tfa, b, tfb, fp32=False):
 
 def ref_collide(a, tfa, b, tfb):
     return _collide(ref_lib().ref_collide, (), a, tfa, b, tfb)
+
+
+def _joint_err(fn, pre, desc, pose1, pose2, h):
+    out = np.zeros(18)
+    d = np.ascontiguousarray(desc)
+    p1 = np.ascontiguousarray(pose1, np.float64)
+    p2 = np.ascontiguousarray(pose2, np.float64)
+    rc = fn(*pre, d.ctypes.data_as(_P), _ptr(p1, _D), _ptr(p2, _D), h, _ptr(out, _D))
+    if rc not in (None, 0):
+        raise RuntimeError("joint type not supported")
+    return {"direction": out[:16].reshape(4, 4)[:, :3].copy(), "magnitude": out[:16].reshape(4, 4)[:, 3].copy(),
+            "current_position": out[16], "target_position": out[17]}
+
+
+def oracle_joint_errors(desc, pose1, pose2, h, fp32=False):
+    """Rows of one joint between two bodies at poses (x y z qw qx qy qz)."""
+    return _joint_err(oracle_lib().orc_joint_errors, (1 if fp32 else 0,), desc, pose1, pose2, h)
+
+
+def ref_joint_errors(desc, pose1, pose2, h):
+    return _joint_err(ref_lib().ref_joint_errors, (), desc, pose1, pose2, h)
+
+
+def sinf(x):
+    return oracle_lib().orc_sinf(x)
+
+
+def cosf(x):
+    return oracle_lib().orc_cosf(x)
+
+
+def atan2f(y, x):
+    return oracle_lib().orc_atan2f(y, x)
 
 
 def mix32(x: int) -> int:

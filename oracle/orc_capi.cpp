@@ -32,6 +32,17 @@ typedef struct {
     double restitution, static_friction, dynamic_friction;
 } orc_collider_desc;
 
+typedef struct {
+    int32_t type, actuation;   // JointType / JointActuationType order (Joint.hpp), 3 = spherical
+    uint32_t body_1, body_2;
+    double r_1[3], r_2[3];
+    double axis[3], limit_axis[3];
+    double compliance, damping;
+    int32_t limited, _pad;
+    double lower_limit, upper_limit;
+    double target_position, target_speed;
+} orc_joint_desc;
+
 }  // extern "C"
 
 namespace {
@@ -52,8 +63,23 @@ struct Any {
     virtual int ncolours() const = 0;
     virtual int colour_rounds() const = 0;
     virtual void set_palette_max(uint32_t m) = 0;
+    virtual void set_joint_mode(int m) = 0;
     virtual void set_state(const double *p, const double *q, const double *v, const double *w) = 0;
+    virtual void add_joint(const orc_joint_desc &d) = 0;
+    virtual int njoints() const = 0;
+    virtual void joint_state(double *cur, double *tgt, double *lam4, int32_t *colour) const = 0;
+    virtual void set_joint_targets(int n, const uint32_t *ids, const double *tgt, const double *speed) = 0;
 };
+
+template <class T> typename orc::World<T>::Joint joint_from(const orc_joint_desc &d) {
+    typename orc::World<T>::Joint j;
+    auto v3 = [](const double *a) { return orc::V3<T>(T(a[0]), T(a[1]), T(a[2])); };
+    j.type = d.type; j.act = d.actuation; j.b1 = d.body_1; j.b2 = d.body_2;
+    j.r1 = v3(d.r_1); j.r2 = v3(d.r_2); j.axis = v3(d.axis); j.laxis = v3(d.limit_axis);
+    j.limited = d.limited != 0; j.lower = T(d.lower_limit); j.upper = T(d.upper_limit);
+    j.target = T(d.target_position); j.speed = T(d.target_speed); j.damping = T(d.damping);
+    return j;
+}
 
 template <class T> struct Impl : Any {
     orc::World<T> w;
@@ -133,7 +159,48 @@ template <class T> struct Impl : Any {
     int ncolours() const override { return w.ncolours; }
     int colour_rounds() const override { return (int)w.colour_rounds; }
     void set_palette_max(uint32_t m) override { w.palette_max = m; }
+    void set_joint_mode(int m) override { w.joint_mode = m; }
+    void add_joint(const orc_joint_desc &d) override { w.add_joint(joint_from<T>(d), T(d.compliance)); }
+    int njoints() const override { return (int)w.joints.size(); }
+    void joint_state(double *cur, double *tgt, double *lam4, int32_t *colour) const override {
+        for (size_t i = 0; i < w.joints.size(); ++i) {
+            const auto &j = w.joints[i];
+            if (cur) cur[i] = j.current;
+            if (tgt) tgt[i] = j.target;
+            if (lam4) for (int k = 0; k < 4; ++k) lam4[4 * i + k] = k < j.nrows ? double(j.lam[k]) : 0.0;
+            if (colour) colour[i] = i < w.jcolour.size() ? w.jcolour[i] : -1;
+        }
+    }
+    void set_joint_targets(int n, const uint32_t *ids, const double *tgt, const double *speed) override {
+        for (int k = 0; k < n; ++k) {
+            auto &j = w.joints[ids[k]];
+            if (tgt) j.target = T(tgt[k]);
+            if (speed) j.speed = T(speed[k]);
+        }
+    }
 };
+
+// Errors of one joint between two bodies at given poses (Joint.cpp:18-182): out =
+// 4 rows x (direction xyz, magnitude), current position, target position.
+template <class T> void joint_errors_t(const orc_joint_desc *d, const double *pose1, const double *pose2, double h,
+                                       double *out) {
+    orc::World<T> w;
+    orc::SM<T> I(T(1), T(1), T(1), T(0), T(0), T(0));
+    for (const double *p : {pose1, pose2})
+        w.add_body(1, T(1), I, orc::V3<T>(T(p[0]), T(p[1]), T(p[2])), orc::Q<T>(T(p[3]), T(p[4]), T(p[5]), T(p[6])),
+                   orc::V3<T>(), orc::V3<T>());
+    orc_joint_desc dd = *d;
+    dd.body_1 = 0; dd.body_2 = 1;
+    w.add_joint(joint_from<T>(dd), T(d->compliance));
+    auto &j = w.joints[0];
+    w.joint_errors(j, T(h));
+    for (int k = 0; k < 4; ++k) {
+        out[4 * k] = j.dir[k].x; out[4 * k + 1] = j.dir[k].y; out[4 * k + 2] = j.dir[k].z;
+        out[4 * k + 3] = k < j.nrows ? double(j.mag[k]) : 0.0;
+    }
+    out[16] = j.current;
+    out[17] = j.target;
+}
 
 // One narrowphase call (same layout as the reference driver's ref_collide).
 template <class T> int collide_t(const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
@@ -212,6 +279,28 @@ void orc_get_solve_pairs(void *h, uint32_t *ca_cb, int32_t *col, int32_t *cnt, i
 }
 int orc_num_colours(void *h) { return static_cast<Any *>(h)->ncolours(); }
 int orc_colour_rounds(void *h) { return static_cast<Any *>(h)->colour_rounds(); }
+int orc_add_joints(void *h, int n, const orc_joint_desc *d) {
+    for (int i = 0; i < n; ++i) {
+        if (d[i].type < 0 || d[i].type > 3) return -1;
+        static_cast<Any *>(h)->add_joint(d[i]);
+    }
+    return 0;
+}
+int orc_num_joints(void *h) { return static_cast<Any *>(h)->njoints(); }
+void orc_get_joints(void *h, double *cur, double *tgt, double *lam4, int32_t *colour) {
+    static_cast<Any *>(h)->joint_state(cur, tgt, lam4, colour);
+}
+void orc_set_joint_targets(void *h, int n, const uint32_t *ids, const double *tgt, const double *speed) {
+    static_cast<Any *>(h)->set_joint_targets(n, ids, tgt, speed);
+}
+void orc_set_joint_mode(void *h, int m) { static_cast<Any *>(h)->set_joint_mode(m); }
+void orc_joint_errors(int fp32, const orc_joint_desc *d, const double *pose1, const double *pose2, double h, double *out) {
+    if (fp32) joint_errors_t<float>(d, pose1, pose2, h, out);
+    else joint_errors_t<double>(d, pose1, pose2, h, out);
+}
+float orc_sinf(float x) { return orc::rb_sin(x); }
+float orc_cosf(float x) { return orc::rb_cos(x); }
+float orc_atan2f(float y, float x) { return orc::rb_atan2(y, x); }
 int orc_set_palette_max(void *h, int m) {
     if (m < 1) return -1;
     static_cast<Any *>(h)->set_palette_max((uint32_t)m);

@@ -18,6 +18,7 @@
 #include <stdexcept>
 #include <vector>
 #include "orc_collide.hpp"
+#include "orc_trig.hpp"
 
 namespace orc {
 
@@ -83,6 +84,38 @@ template <class T> struct World {
     std::vector<int> pcount;                       // points per pair this substep
     std::vector<Contact<T>> pcont;                 // 4 slots per pair
 
+    // ---- joints (Joint.hpp:82-106) with their constraint rows (Constraint.hpp:31-53)
+    enum { J_PRISMATIC = 0, J_REVOLUTE = 1, J_FIXED = 2, J_SPHERICAL = 3 };
+    enum { ACT_FREE = 0, ACT_POSITION = 1, ACT_SPEED = 2 };
+    struct Joint {
+        int type = 0, act = 0;
+        uint32_t b1 = 0, b2 = 0;
+        V3<T> r1, r2, axis, laxis;
+        bool limited = false;
+        T lower = 0, upper = 0, target = 0, speed = 0, current = 0, damping = 0;
+        int nrows = 0;
+        bool rot[4] = {false, false, false, false};   // ROTATIONAL row (else POSITIONAL)
+        T comp[4] = {0, 0, 0, 0};                       // row compliance
+        V3<T> dir[4];
+        T mag[4] = {0, 0, 0, 0}, lam[4] = {0, 0, 0, 0};
+    };
+    std::vector<Joint> joints;
+    std::vector<int> jcolour;
+    std::vector<uint32_t> joint_order;   // colour mode: joints by colour (stable)
+    int njcolours = 0;
+    bool joints_dirty = false;
+    // Joint solver mode.
+    // 0 (the reference): errors of every joint evaluated once per substep before any row is
+    //   solved (World.hpp:168-170); lambda accumulates over the whole run (never reset);
+    //   row direction = normalize(v) (Constraint.cpp:5-12), which is zero for errors in
+    //   (1e-9, ~3.2e-5) although the row is kept, so a ROTATIONAL row with compliance 0
+    //   then divides 0 by 0; the revolute alignment row is a1 x a2 (Joint.cpp:101), whose
+    //   correction turns the axes apart.  Chains diverge and hinges turn NaN.
+    // 1 (XPBD, Mueller et al. 2020): each row's error evaluated from the current state just
+    //   before the row is solved, lambda reset every substep, direction v/|v| whenever
+    //   |v| > EPS, revolute alignment a2 x a1.  Equal to mode 0 in everything else.
+    int joint_mode = 0;
+
     // ------------------------------------------------------------ creation
     // BodyAPI.hpp:53-80
     uint32_t add_body(int type, T m, const SM<T> &inertia, V3<T> p, Q<T> o, V3<T> v, V3<T> w) {
@@ -128,6 +161,28 @@ template <class T> struct World {
         }
         bph.push_back(h);
         return id;
+    }
+
+    // JointAPI.hpp:160-265 (rows per type); the spherical joint is this build's extension:
+    // one POSITIONAL attachment row carrying the compliance, plus a ROTATIONAL swing-limit
+    // row when limited.
+    uint32_t add_joint(const Joint &in, T compliance) {
+        Joint j = in;
+        for (int k = 0; k < 4; ++k) { j.rot[k] = false; j.comp[k] = T(0); j.dir[k] = V3<T>(T(1), T(0), T(0)); }
+        switch (j.type) {
+            case J_PRISMATIC: j.nrows = 3; j.rot[0] = true; j.comp[2] = compliance; j.laxis = V3<T>(); break;
+            case J_REVOLUTE: j.nrows = 4; j.rot[0] = j.rot[2] = j.rot[3] = true; j.comp[3] = compliance; break;
+            case J_FIXED:
+                j.nrows = 2; j.rot[0] = true; j.act = ACT_FREE; j.limited = false; j.lower = j.upper = T(0);
+                j.axis = j.laxis = V3<T>(); j.damping = T(0);
+                break;
+            default:
+                j.nrows = j.limited ? 2 : 1; j.rot[1] = true; j.comp[0] = compliance; break;
+        }
+        j.current = T(0);   // target and speed: the reference's public fields, 0 unless set
+        joints.push_back(j);
+        joints_dirty = true;
+        return (uint32_t)joints.size() - 1;
     }
 
     // ------------------------------------------------------------ broadphase
@@ -428,6 +483,184 @@ template <class T> struct World {
         return inv_mass[i] + dot(c, Iwinv[i] * c);
     }
 
+    // ------------------------------------------------------------ joints (Joint.cpp, Constraint.cpp)
+    T w_rot(uint32_t i, V3<T> n) {   // Body.cpp:103-109
+        if (btype[i] == 0) return T(0.0);
+        update_Iw(i);
+        return dot(n, Iwinv[i] * n);
+    }
+    void apply_rot_impulse(uint32_t i, V3<T> imp) {   // Body.cpp:70-77
+        if (btype[i] == 0) return;
+        V3<T> rot = (Iwinv[i] * imp) * T(0.5);
+        Q<T> rq(T(0.0), rot.x, rot.y, rot.z);
+        q[i] = normalize(q[i] + rq * q[i]);
+    }
+    void apply_rot_vel_impulse(uint32_t i, V3<T> imp) {   // Body.cpp:87-92
+        if (btype[i] == 0) return;
+        av[i] = av[i] + Iwinv[i] * imp;
+    }
+    static Q<T> from_axis_angle(V3<T> axis, T angle) {   // quat.hpp:52-58
+        T half = angle * T(0.5);
+        T sn = t_sin(half);
+        V3<T> n = normalize(axis);
+        return Q<T>(t_cos(half), n.x * sn, n.y * sn, n.z * sn);
+    }
+    void set_row(Joint &j, int k, V3<T> v) const {   // Constraint.cpp:5-12
+        j.mag[k] = length(v);
+        if (!(j.mag[k] > C<T>::EPS)) j.dir[k] = V3<T>(T(1.0), T(0.0), T(0.0));
+        else if (joint_mode == 0) j.dir[k] = normalize(v);
+        else j.dir[k] = v * (T(1.0) / std::sqrt(length_sq(v)));
+    }
+    // Joint.cpp:18-71 (prismatic), 92-159 (revolute), 161-182 (fixed); spherical: attachment
+    // row, and a swing limit built like the revolute angle limit (Joint.cpp:73-90).
+    void joint_errors(Joint &j, T h, bool advance_target = true) {
+        const uint32_t b1 = j.b1, b2 = j.b2;
+        const V3<T> r1w = rotate(q[b1], j.r1), r2w = rotate(q[b2], j.r2);
+        const V3<T> p1 = pos[b1] + r1w, p2 = pos[b2] + r2w;
+        if (j.type == J_PRISMATIC || j.type == J_FIXED) {
+            Q<T> dq = q[b1] * conj(q[b2]);
+            set_row(j, 0, V3<T>(dq.x, dq.y, dq.z) * T(2.0));
+        }
+        if (j.type == J_FIXED) { set_row(j, 1, p1 - p2); return; }
+        if (j.type == J_PRISMATIC) {
+            V3<T> dp = p1 - p2;
+            V3<T> axis = rotate(q[b1], j.axis);
+            T ps = dot(dp, axis);
+            T over = T(0);
+            if (j.limited) {
+                T cl = sclamp(ps, j.lower, j.upper);
+                over = ps - cl;
+            }
+            set_row(j, 1, (dp - axis * ps) + axis * over);
+            V3<T> drive(T(0.0), T(0.0), T(0.0));
+            if (j.act == ACT_SPEED && advance_target) j.target = j.target + j.speed * h;
+            if (j.act == ACT_SPEED || j.act == ACT_POSITION) {
+                V3<T> ddp = p2 - p1;
+                T curr = dot(ddp, axis);
+                drive = axis * (j.target - curr);
+            }
+            set_row(j, 2, drive);
+            j.current = ps;
+            return;
+        }
+        if (j.type == J_REVOLUTE) {
+            V3<T> a1 = rotate(q[b1], j.axis), a2 = rotate(q[b2], j.axis);
+            set_row(j, 0, joint_mode == 0 ? cross(a1, a2) : cross(a2, a1));
+            set_row(j, 1, p1 - p2);
+            V3<T> l1 = rotate(q[b1], j.laxis), l2 = rotate(q[b2], j.laxis);
+            T phi = t_atan2(dot(cross(l1, l2), a1), dot(l1, l2));
+            V3<T> lim(T(0.0), T(0.0), T(0.0));
+            if (j.limited && (phi < j.lower || phi > j.upper)) {
+                phi = sclamp(phi, j.lower, j.upper);
+                V3<T> n1 = rotate(from_axis_angle(a1, phi), l1);
+                lim = cross(n1, l2);
+            }
+            set_row(j, 2, -lim);
+            V3<T> drive(T(0.0), T(0.0), T(0.0));
+            if (j.act == ACT_SPEED && advance_target) j.target = j.target + j.speed * h;
+            if (j.act == ACT_SPEED || j.act == ACT_POSITION) {
+                T theta = j.target + T(3.14159265358979323846);
+                V3<T> bt = (l1 * t_cos(theta) + cross(a1, l1) * t_sin(theta)) + a1 * dot(a1, l1) * (T(1) - t_cos(theta));
+                drive = cross(bt, l2);
+            }
+            set_row(j, 3, drive);
+            j.current = phi;
+            return;
+        }
+        // spherical
+        set_row(j, 0, p1 - p2);
+        if (j.limited) {
+            V3<T> a1 = rotate(q[b1], j.axis), a2 = rotate(q[b2], j.axis);
+            V3<T> c = cross(a1, a2);
+            T th = t_atan2(length(c), dot(a1, a2));
+            V3<T> lim(T(0.0), T(0.0), T(0.0));
+            if (th > j.upper) {
+                V3<T> n1 = rotate(from_axis_angle(c, j.upper), a1);
+                lim = cross(n1, a2);
+            }
+            set_row(j, 1, -lim);
+            j.current = th;
+        }
+    }
+    T joint_delta_lambda(const Joint &j, int k, T w1, T w2, T inv_h) const {   // Constraint.cpp:14-18
+        T ap = j.comp[k] * inv_h * inv_h;
+        return (-j.mag[k] - ap * j.lam[k]) / (w1 + w2 + ap);
+    }
+    // Constraint.cpp:20-46 for the joint's rows in order (mode 1: each row's error first
+    // re-evaluated from the current state).
+    void joint_solve(Joint &j, T inv_h, T h) {
+        for (int k = 0; k < j.nrows; ++k) {
+            if (joint_mode == 1) joint_errors(j, h, false);
+            const V3<T> n = j.dir[k];
+            if (j.rot[k]) {
+                T w1 = w_rot(j.b1, n), w2 = w_rot(j.b2, n);
+                T dl = joint_delta_lambda(j, k, w1, w2, inv_h);
+                j.lam[k] = j.lam[k] + dl;
+                V3<T> imp = dl * n;
+                apply_rot_impulse(j.b1, imp);
+                apply_rot_impulse(j.b2, -imp);
+            } else {
+                V3<T> r1w = rotate(q[j.b1], j.r1), r2w = rotate(q[j.b2], j.r2);
+                T w1 = w_pos(j.b1, r1w, n), w2 = w_pos(j.b2, r2w, n);
+                T dl = joint_delta_lambda(j, k, w1, w2, inv_h);
+                j.lam[k] = j.lam[k] + dl;
+                V3<T> imp = dl * n;
+                apply_pos_impulse(j.b1, imp, r1w);
+                apply_pos_impulse(j.b2, -imp, r2w);
+            }
+        }
+    }
+    // Joint.cpp:184-220 (prismatic and revolute only)
+    void joint_damp(Joint &j, T h) {
+        const uint32_t b1 = j.b1, b2 = j.b2;
+        if (j.type == J_PRISMATIC) {
+            V3<T> dv = (lv[b2] - lv[b1]) * smin(j.damping * h, T(1.0));
+            if (length_sq(dv) < C<T>::EPS) return;
+            V3<T> r1w = rotate(q[b1], j.r1), r2w = rotate(q[b2], j.r2);
+            V3<T> n = normalize(dv);
+            T w1 = w_pos(b1, r1w, n), w2 = w_pos(b2, r2w, n);
+            V3<T> imp = dv / (w1 + w2);
+            apply_vel_impulse(b1, imp, r1w);
+            apply_vel_impulse(b2, -imp, r2w);
+        } else if (j.type == J_REVOLUTE) {
+            V3<T> dw = (av[b2] - av[b1]) * smin(j.damping * h, T(1.0));
+            if (length_sq(dw) < C<T>::EPS) return;
+            V3<T> n = normalize(dw);
+            T w1 = w_rot(b1, n), w2 = w_rot(b2, n);
+            V3<T> imp = dw / (w1 + w2);
+            apply_rot_vel_impulse(b1, imp);
+            apply_rot_vel_impulse(b2, -imp);
+        }
+    }
+    // Colour mode: joints coloured once (first fit in joint order; conflict = a shared
+    // dynamic body), solved colour by colour.
+    void colour_joints() {
+        const uint32_t J = (uint32_t)joints.size();
+        jcolour.assign(J, 0);
+        njcolours = 0;
+        std::vector<std::vector<int>> body_cols(btype.size());
+        for (uint32_t i = 0; i < J; ++i) {
+            const uint32_t bs[2] = {joints[i].b1, joints[i].b2};
+            int c = 0;
+            for (;; ++c) {
+                bool taken = false;
+                for (uint32_t b : bs)
+                    if (btype[b] == 1)
+                        for (int u : body_cols[b]) taken = taken || u == c;
+                if (!taken) break;
+            }
+            jcolour[i] = c;
+            for (uint32_t b : bs)
+                if (btype[b] == 1) body_cols[b].push_back(c);
+            if (c + 1 > njcolours) njcolours = c + 1;
+        }
+        joint_order.resize(J);
+        for (uint32_t i = 0; i < J; ++i) joint_order[i] = i;
+        std::stable_sort(joint_order.begin(), joint_order.end(),
+                         [&](uint32_t a, uint32_t b) { return jcolour[a] < jcolour[b]; });
+        joints_dirty = false;
+    }
+
     // ------------------------------------------------------------ contact solve (Contact.cpp)
     void contact_position(Contact<T> &k, T inv_h) {   // Contact.cpp:35-75
         (void)inv_h;
@@ -539,6 +772,7 @@ template <class T> struct World {
         refit(timestep);
         bp_update();
         if (order == ORDER_COLOUR) build_pairs();
+        if (joints_dirty) colour_joints();
         for (int s = 0; s < substeps; ++s) {
             if (order == ORDER_REFERENCE) narrowphase_reference();
             else { narrowphase_colour(); colour_active(); build_solve_order(); }
@@ -547,11 +781,26 @@ template <class T> struct World {
                 for (auto &k : contacts) contact_position(k, inv_h);
             else
                 for_each_colour_contact([&](Contact<T> &k) { contact_position(k, inv_h); });
+            // World.hpp:168-170: joint errors from the state after the contact solve, then the rows
+            for (auto &j : joints) {
+                joint_errors(j, h);   // mode 1: targets advance here; rows re-evaluated in the solve
+                if (joint_mode == 1)
+                    for (int k = 0; k < 4; ++k) j.lam[k] = T(0);
+            }
+            if (order == ORDER_REFERENCE)
+                for (auto &j : joints) joint_solve(j, inv_h, h);
+            else
+                for (uint32_t i : joint_order) joint_solve(joints[i], inv_h, h);
             update_velocities(inv_h);
             if (order == ORDER_REFERENCE)
                 for (auto &k : contacts) contact_velocity(k, h);
             else
                 for_each_colour_contact([&](Contact<T> &k) { contact_velocity(k, h); });
+            // World.hpp:174-178
+            if (order == ORDER_REFERENCE)
+                for (auto &j : joints) joint_damp(j, h);
+            else
+                for (uint32_t i : joint_order) joint_damp(joints[i], h);
         }
     }
 };

@@ -32,9 +32,52 @@ typedef struct {
     double restitution, static_friction, dynamic_friction;
 } ref_collider_desc;
 
+typedef struct {
+    int32_t type, actuation;
+    uint32_t body_1, body_2;
+    double r_1[3], r_2[3];
+    double axis[3], limit_axis[3];
+    double compliance, damping;
+    int32_t limited, _pad;
+    double lower_limit, upper_limit;
+    double target_position, target_speed;
+} ref_joint_desc;
+
 }  // extern "C"
 
 namespace {
+
+m3d::vec3 v3(const double *a) { return m3d::vec3(a[0], a[1], a[2]); }
+
+// Creates one joint through the reference's World API; target fields are the public
+// joint data a caller sets after creation.  Spherical (3) has no reference counterpart.
+int add_joint(rbps::World &w, const ref_joint_desc &d) {
+    const auto act = static_cast<rbps::JointActuationType>(d.actuation);
+    uint32_t id;
+    if (d.type == 0) {
+        rbps::PrismaticJointParams p;
+        p.body_1 = d.body_1; p.body_2 = d.body_2; p.moving_axis = v3(d.axis); p.r_1 = v3(d.r_1); p.r_2 = v3(d.r_2);
+        p.actuation_type = act; p.compliance = d.compliance; p.damping = d.damping; p.limited = d.limited != 0;
+        p.lower_limit = d.lower_limit; p.upper_limit = d.upper_limit;
+        id = w.create_prismatic_joint(p);
+    } else if (d.type == 1) {
+        rbps::RevoluteJointParams p;
+        p.body_1 = d.body_1; p.body_2 = d.body_2; p.aligned_axis = v3(d.axis); p.limit_axis = v3(d.limit_axis);
+        p.r_1 = v3(d.r_1); p.r_2 = v3(d.r_2); p.actuation_type = act; p.compliance = d.compliance;
+        p.damping = d.damping; p.limited = d.limited != 0; p.lower_limit = d.lower_limit; p.upper_limit = d.upper_limit;
+        id = w.create_revolute_joint(p);
+    } else if (d.type == 2) {
+        rbps::FixedJointParams p;
+        p.body_1 = d.body_1; p.body_2 = d.body_2; p.r_1 = v3(d.r_1); p.r_2 = v3(d.r_2);
+        id = w.create_fixed_joint(p);
+    } else {
+        return -1;
+    }
+    const uint32_t i = w.joints.index_of(id);
+    w.joints.target_position[i] = d.target_position;
+    w.joints.target_speed[i] = d.target_speed;
+    return 0;
+}
 
 rbc::Shape make_shape(const ref_collider_desc &d) {
     switch (d.shape) {
@@ -162,6 +205,65 @@ int ref_collide(const ref_collider_desc *a, const double *tfa, const ref_collide
 
 // The reference's greedy colouring (CollisionPipeline.cpp:223-293) on the current contact list.
 // out_colour[i] = group index of contact i (groups sorted by size as the reference returns them).
+int ref_add_joints(void *h, int n, const ref_joint_desc *d) {
+    auto *w = static_cast<rbps::World *>(h);
+    for (int i = 0; i < n; ++i)
+        if (add_joint(*w, d[i])) return -1;
+    return 0;
+}
+int ref_num_joints(void *h) { return (int)static_cast<rbps::World *>(h)->joints.count(); }
+void ref_get_joints(void *h, double *cur, double *tgt, double *lam4, int32_t *colour) {
+    auto *w = static_cast<rbps::World *>(h);
+    auto &jc = w->joints;
+    auto &cc = w->constraints;
+    for (uint32_t i = 0; i < jc.count(); ++i) {
+        if (cur) cur[i] = jc.current_position[i];
+        if (tgt) tgt[i] = jc.target_position[i];
+        if (lam4)
+            for (uint32_t k = 0; k < 4; ++k)
+                lam4[4 * i + k] = k < jc.constraint_count[i] ? cc.lambda[cc.index_of(jc.constraints[i].ids[k])] : 0.0;
+        if (colour) colour[i] = -1;
+    }
+}
+void ref_set_joint_targets(void *h, int n, const uint32_t *ids, const double *tgt, const double *speed) {
+    auto &jc = static_cast<rbps::World *>(h)->joints;
+    for (int k = 0; k < n; ++k) {
+        if (tgt) jc.target_position[ids[k]] = tgt[k];
+        if (speed) jc.target_speed[ids[k]] = speed[k];
+    }
+}
+// compute_*_joint_errors (Joint.cpp:18-182) for one joint between two bodies at given poses.
+int ref_joint_errors(const ref_joint_desc *d, const double *pose1, const double *pose2, double h, double *out) {
+    rbps::World w(1.0 / 60.0, 1);
+    for (const double *p : {pose1, pose2}) {
+        rbps::BodyParams bp;
+        bp.type = rbps::DYNAMIC;
+        bp.mass = 1.0;
+        bp.inertia_tensor = m3d::smat3(1, 1, 1, 0, 0, 0);
+        bp.position = v3(p);
+        bp.orientation = m3d::quat(p[3], p[4], p[5], p[6]);
+        w.create_body(bp);
+    }
+    ref_joint_desc dd = *d;
+    dd.body_1 = 0; dd.body_2 = 1;
+    if (add_joint(w, dd)) return -1;
+    rbps::compute_joint_errors(w.joints, w.bodies, w.constraints, h);
+    auto &jc = w.joints;
+    auto &cc = w.constraints;
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k < jc.constraint_count[0]) {
+            const uint32_t r = cc.index_of(jc.constraints[0].ids[k]);
+            out[4 * k] = cc.direction[r].x; out[4 * k + 1] = cc.direction[r].y; out[4 * k + 2] = cc.direction[r].z;
+            out[4 * k + 3] = cc.magnitude[r];
+        } else {
+            out[4 * k] = 1.0; out[4 * k + 1] = 0.0; out[4 * k + 2] = 0.0; out[4 * k + 3] = 0.0;
+        }
+    }
+    out[16] = jc.current_position[0];
+    out[17] = jc.target_position[0];
+    return 0;
+}
+
 int ref_collision_groups(void *h, int32_t *out_colour) {
     auto *w = static_cast<rbps::World *>(h);
     auto groups = rbps::get_collision_groups(w->contacts, w->bodies);

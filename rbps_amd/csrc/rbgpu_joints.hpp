@@ -1,0 +1,269 @@
+// Joints on the GPU: error evaluation, XPBD row solve and velocity damping.
+//
+// Reference: src/rbps/constraints/Joint.cpp (errors 18-182, damping 184-220),
+// src/rbps/constraints/Constraint.cpp (rows 5-46), include/rbps/API/JointAPI.hpp
+// (rows per joint type 160-265), World.hpp:166-178 (order inside a substep).
+//
+// A joint is one unit of work: one thread walks its 1-4 rows in order, exactly as the
+// reference walks a joint's consecutive constraint rows.  Joints sharing a dynamic body
+// are separated by a colouring computed once when the joints are uploaded; one launch
+// per colour.  All errors of a substep are evaluated before any row is solved (the
+// reference's compute_joint_errors pass precedes solve_constraints).
+//
+// Layout (one row per joint, 16-byte rows):
+//   jm   int4   (type | actuation << 8 | limited << 16 | rows << 24, body 1, body 2, -)
+//   jr1  float4 (r_1 local, damping)      jr2  float4 (r_2 local, compliance)
+//   jax  float4 (axis, lower limit)        jlx  float4 (limit axis, upper limit)
+//   jtg  float4 (target position, target speed, current position, -)
+//   jrow float4 x 4 per joint (direction, magnitude), jlam float x 4 per joint (lambda,
+//   never reset, as in the reference).
+// Row kinds per type (JointAPI.hpp): prismatic R P P(compliance), revolute R P R R(c),
+// fixed R P; spherical (this build's extension): P(c) and, when limited, a swing-limit R.
+#pragma once
+#include "rbgpu_math.hpp"
+
+namespace rb {
+
+enum JointType { JT_PRISMATIC = 0, JT_REVOLUTE = 1, JT_FIXED = 2, JT_SPHERICAL = 3 };
+enum JointAct { JA_FREE = 0, JA_POSITION = 1, JA_SPEED = 2 };
+
+struct Joints {
+    int4 *jm;
+    float4 *jr1, *jr2, *jax, *jlx, *jtg;
+    float4 *jrow;
+    float *jlam;
+};
+
+__device__ __forceinline__ int jt_type(int4 m) { return m.x & 0xff; }
+__device__ __forceinline__ int jt_act(int4 m) { return (m.x >> 8) & 0xff; }
+__device__ __forceinline__ bool jt_limited(int4 m) { return ((m.x >> 16) & 0xff) != 0; }
+__device__ __forceinline__ int jt_rows(int4 m) { return (m.x >> 24) & 0xff; }
+// ROTATIONAL row?  (POSITIONAL otherwise)
+__device__ __forceinline__ bool jt_rot(int type, int k) {
+    switch (type) {
+        case JT_PRISMATIC: return k == 0;
+        case JT_REVOLUTE: return k != 1;
+        case JT_FIXED: return k == 0;
+        default: return k == 1;
+    }
+}
+// The row carrying the joint's compliance (the others have 0).
+__device__ __forceinline__ int jt_comp_row(int type) {
+    return type == JT_PRISMATIC ? 2 : (type == JT_REVOLUTE ? 3 : (type == JT_SPHERICAL ? 0 : -1));
+}
+
+// Joint solver modes.
+//   0 (the reference): errors of every joint evaluated once per substep before any row is
+//     solved (World.hpp:168-170); lambda never reset; row direction normalize(v)
+//     (Constraint.cpp:5-12), which is zero for errors in (1e-9, ~3.2e-5) although the row is
+//     kept, so a ROTATIONAL row with compliance 0 then divides 0 by 0; revolute alignment
+//     a1 x a2 (Joint.cpp:101), whose correction turns the axes apart.
+//   1 (XPBD, Mueller et al. 2020): each row's error re-evaluated from the current state just
+//     before it is solved, lambda reset every substep, direction v/|v| whenever |v| > EPS,
+//     revolute alignment a2 x a1.
+__device__ __forceinline__ float4 row_value(V3 v, int mode) {
+    const float m = len(v);
+    V3 d{1.0f, 0.0f, 0.0f};
+    if (m > EPS) d = mode == 0 ? normalize(v) : v * (1.0f / sqrtf(len2(v)));
+    return make_float4(d.x, d.y, d.z, m);
+}
+
+// Rows of one joint from the two bodies' poses (Joint.cpp:18-182; spherical: attachment row
+// and a swing limit built like the revolute angle limit, Joint.cpp:73-90).  `advance`
+// moves a speed-driven target by speed * h.  tg.z receives the joint's current position.
+__device__ __forceinline__ void joint_rows(int4 m, float4 r1, float4 r2, float4 ax, float4 lx, float4 &tg, V3 x1, Q q1,
+                                           V3 x2, Q q2, float h, bool advance, int mode, float4 row[4]) {
+    const int type = jt_type(m), act = jt_act(m);
+    const V3 p1 = x1 + rotate(q1, xyz(r1)), p2 = x2 + rotate(q2, xyz(r2));
+    if (type == JT_PRISMATIC || type == JT_FIXED) {   // Joint.cpp:27-29, 170-172
+        const Q dq = qmul(q1, conj(q2));
+        row[0] = row_value(V3{dq.x, dq.y, dq.z} * 2.0f, mode);
+    }
+    if (type == JT_FIXED) {
+        row[1] = row_value(p1 - p2, mode);
+        return;
+    }
+    if (type == JT_PRISMATIC) {   // Joint.cpp:32-70
+        const V3 dp = p1 - p2;
+        const V3 axis = rotate(q1, xyz(ax));
+        const float ps = dot(dp, axis);
+        float over = 0.0f;
+        if (jt_limited(m)) over = ps - clampf(ps, ax.w, lx.w);
+        row[1] = row_value((dp - axis * ps) + axis * over, mode);
+        V3 drive{0.0f, 0.0f, 0.0f};
+        if (act == JA_SPEED && advance) tg.x = tg.x + tg.y * h;
+        if (act == JA_SPEED || act == JA_POSITION) {
+            const float curr = dot(p2 - p1, axis);
+            drive = axis * (tg.x - curr);
+        }
+        row[2] = row_value(drive, mode);
+        tg.z = ps;
+        return;
+    }
+    if (type == JT_REVOLUTE) {   // Joint.cpp:92-159
+        const V3 a1 = rotate(q1, xyz(ax)), a2 = rotate(q2, xyz(ax));
+        row[0] = row_value(mode == 0 ? cross(a1, a2) : cross(a2, a1), mode);
+        row[1] = row_value(p1 - p2, mode);
+        const V3 l1 = rotate(q1, xyz(lx)), l2 = rotate(q2, xyz(lx));
+        float phi = rb_atan2(dot(cross(l1, l2), a1), dot(l1, l2));
+        V3 lim{0.0f, 0.0f, 0.0f};
+        if (jt_limited(m) && (phi < ax.w || phi > lx.w)) {
+            phi = clampf(phi, ax.w, lx.w);
+            const V3 n1 = rotate(from_axis_angle(a1, phi), l1);
+            lim = cross(n1, l2);
+        }
+        row[2] = row_value(-lim, mode);
+        V3 drive{0.0f, 0.0f, 0.0f};
+        if (act == JA_SPEED && advance) tg.x = tg.x + tg.y * h;
+        if (act == JA_SPEED || act == JA_POSITION) {
+            const float theta = tg.x + 3.14159265358979323846f;
+            const V3 bt = (l1 * rb_cos(theta) + cross(a1, l1) * rb_sin(theta)) + a1 * dot(a1, l1) * (1.0f - rb_cos(theta));
+            drive = cross(bt, l2);
+        }
+        row[3] = row_value(drive, mode);
+        tg.z = phi;
+        return;
+    }
+    // spherical
+    row[0] = row_value(p1 - p2, mode);
+    if (jt_limited(m)) {
+        const V3 a1 = rotate(q1, xyz(ax)), a2 = rotate(q2, xyz(ax));
+        const V3 c = cross(a1, a2);
+        const float th = rb_atan2(len(c), dot(a1, a2));
+        V3 lim{0.0f, 0.0f, 0.0f};
+        if (th > lx.w) {
+            const V3 n1 = rotate(from_axis_angle(c, lx.w), a1);
+            lim = cross(n1, a2);
+        }
+        row[1] = row_value(-lim, mode);
+        tg.z = th;
+    }
+}
+
+// compute_joint_errors (Joint.cpp:222-244) for every joint; mode 1 also resets lambda.
+__global__ void k_joint_errors(Bodies B, Joints J, uint32_t nj, float h, int mode) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nj) return;
+    const int4 m = J.jm[i];
+    const uint32_t b1 = (uint32_t)m.y, b2 = (uint32_t)m.z;
+    float4 tg = J.jtg[i];
+    float4 row[4];
+    for (int k = 0; k < 4; ++k) row[k] = J.jrow[(size_t)i * 4 + k];
+    joint_rows(m, J.jr1[i], J.jr2[i], J.jax[i], J.jlx[i], tg, xyz(B.x[b1]), quat(B.q[b1]), xyz(B.x[b2]),
+               quat(B.q[b2]), h, true, mode, row);
+    const int nrows = jt_rows(m);
+    for (int k = 0; k < nrows; ++k) {
+        J.jrow[(size_t)i * 4 + k] = row[k];
+        if (mode == 1) J.jlam[(size_t)i * 4 + k] = 0.0f;
+    }
+    J.jtg[i] = tg;
+}
+
+// Body.cpp:103-109 (refreshes the world tensor)
+__device__ __forceinline__ float w_rot(BodyReg &r, V3 n) {
+    if (!r.dyn) return 0.0f;
+    r.Iwi = world_tensor(mat3_cast(r.q), r.Ii);
+    return dot(n, smv(r.Iwi, n));
+}
+__device__ __forceinline__ void apply_rot(BodyReg &r, V3 imp) {   // Body.cpp:70-77
+    if (!r.dyn) return;
+    const V3 rot = smv(r.Iwi, imp) * 0.5f;
+    r.q = qstep(r.q, rot);
+}
+
+// solve_constraints over one joint's rows (Constraint.cpp:14-46, 89-106); mode 1 first
+// re-evaluates the rows from the bodies' current poses.
+__device__ __forceinline__ void joint_solve(Bodies &B, Joints &J, uint32_t i, float inv_h, float h, int mode) {
+    const int4 m = J.jm[i];
+    const int type = jt_type(m), nrows = jt_rows(m), crow = jt_comp_row(type);
+    const uint32_t b1 = (uint32_t)m.y, b2 = (uint32_t)m.z;
+    const float4 r1f = J.jr1[i], r2f = J.jr2[i], axf = J.jax[i], lxf = J.jlx[i];
+    const V3 r1 = xyz(r1f), r2 = xyz(r2f);
+    float4 tg = J.jtg[i];
+    BodyReg A, Bd;
+    load_body(B, b1, A);
+    load_body(B, b2, Bd);
+    float4 row[4];
+    for (int k = 0; k < nrows; ++k) row[k] = J.jrow[(size_t)i * 4 + k];
+    for (int k = 0; k < nrows; ++k) {
+        if (mode == 1) joint_rows(m, r1f, r2f, axf, lxf, tg, A.x, A.q, Bd.x, Bd.q, h, false, mode, row);
+        const float4 rv = row[k];
+        const V3 n = xyz(rv);
+        const float comp = (k == crow) ? r2f.w : 0.0f;
+        float lam = J.jlam[(size_t)i * 4 + k];
+        if (jt_rot(type, k)) {
+            const float w1 = w_rot(A, n), w2 = w_rot(Bd, n);
+            const float ap = comp * inv_h * inv_h;
+            const float dl = (-rv.w - ap * lam) / (w1 + w2 + ap);
+            lam = lam + dl;
+            const V3 imp = dl * n;
+            apply_rot(A, imp);
+            apply_rot(Bd, -imp);
+        } else {
+            const V3 r1w = rotate(A.q, r1), r2w = rotate(Bd.q, r2);
+            const float w1 = w_pos(A, r1w, n), w2 = w_pos(Bd, r2w, n);
+            const float ap = comp * inv_h * inv_h;
+            const float dl = (-rv.w - ap * lam) / (w1 + w2 + ap);
+            lam = lam + dl;
+            const V3 imp = dl * n;
+            apply_pos(A, imp, r1w);
+            apply_pos(Bd, -imp, r2w);
+        }
+        J.jlam[(size_t)i * 4 + k] = lam;
+    }
+    if (mode == 1) J.jtg[i] = tg;
+    if (A.dyn) { B.x[b1] = f4(A.x, A.im); B.q[b1] = f4q(A.q); }
+    if (Bd.dyn) { B.x[b2] = f4(Bd.x, Bd.im); B.q[b2] = f4q(Bd.q); }
+}
+
+__global__ void __launch_bounds__(256) k_joint_solve(Bodies B, Joints J, const uint32_t *order, uint32_t off,
+                                                     uint32_t count, float inv_h, float h, int mode) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) joint_solve(B, J, order[off + t], inv_h, h, mode);
+}
+
+// Joint.cpp:184-220: prismatic (linear) and revolute (angular) damping; others none.
+__device__ __forceinline__ float w_rot_vel(VelReg &r, V3 n) {
+    if (!r.dyn) return 0.0f;
+    r.Iwi = world_tensor(mat3_cast(r.q), r.Ii);
+    return dot(n, smv(r.Iwi, n));
+}
+__device__ __forceinline__ void joint_damp(Bodies &B, Joints &J, uint32_t i, float h) {
+    const int4 m = J.jm[i];
+    const int type = jt_type(m);
+    if (type != JT_PRISMATIC && type != JT_REVOLUTE) return;
+    const uint32_t b1 = (uint32_t)m.y, b2 = (uint32_t)m.z;
+    const float4 r1f = J.jr1[i];
+    VelReg A, Bd;
+    load_vel(B, b1, A);
+    load_vel(B, b2, Bd);
+    const float f = fmin_ref(r1f.w * h, 1.0f);
+    if (type == JT_PRISMATIC) {
+        const V3 dv = (Bd.v - A.v) * f;
+        if (len2(dv) < EPS) return;
+        const V3 r1w = rotate(A.q, xyz(r1f)), r2w = rotate(Bd.q, xyz(J.jr2[i]));
+        const V3 n = normalize(dv);
+        const float w1 = w_vel(A, r1w, n), w2 = w_vel(Bd, r2w, n);
+        const V3 imp = vdiv(dv, w1 + w2);
+        apply_vel(A, imp, r1w);
+        apply_vel(Bd, -imp, r2w);
+    } else {
+        const V3 dw = (Bd.w - A.w) * f;
+        if (len2(dw) < EPS) return;
+        const V3 n = normalize(dw);
+        const float w1 = w_rot_vel(A, n), w2 = w_rot_vel(Bd, n);
+        const V3 imp = vdiv(dw, w1 + w2);
+        if (A.dyn) A.w = A.w + smv(A.Iwi, imp);
+        if (Bd.dyn) Bd.w = Bd.w + smv(Bd.Iwi, -imp);
+    }
+    if (A.dyn) { B.v[b1] = f4(A.v, 0.0f); B.w[b1] = f4(A.w, 0.0f); }
+    if (Bd.dyn) { B.v[b2] = f4(Bd.v, 0.0f); B.w[b2] = f4(Bd.w, 0.0f); }
+}
+
+__global__ void __launch_bounds__(256) k_joint_damp(Bodies B, Joints J, const uint32_t *order, uint32_t off,
+                                                    uint32_t count, float h) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) joint_damp(B, J, order[off + t], h);
+}
+
+}  // namespace rb

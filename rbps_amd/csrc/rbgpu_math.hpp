@@ -130,4 +130,86 @@ struct Box3 {
     V3 mn, mx;
 };
 
+// ---------------------------------------------------------------- sin, cos, atan2
+// Built only from IEEE +, -, *, / and comparisons so that a CPU evaluation of the same
+// expressions reproduces them bit for bit.  Argument reduction x = k*(pi/2) + r with
+// pi/2 split in three parts (Cody & Waite 1980): 1.5703125 + 4.837512969970703e-4 +
+// 7.549790126404332e-8 (k*P1, k*P2 exact for |k| < 2^13); Taylor polynomials of sin
+// and cos on |r| <= pi/4 through r^11 / r^12; atan via its Taylor series through t^19
+// for t <= tan(pi/8) and atan(t) = pi/4 + atan((t-1)/(t+1)) above, then the octant
+// fix-up.  A few ulp from the correctly rounded values.
+RB_HD float rb_sin_poly(float r) {
+    const float r2 = r * r;
+    float p = -1.0f / 39916800.0f;
+    p = p * r2 + 1.0f / 362880.0f;
+    p = p * r2 + -1.0f / 5040.0f;
+    p = p * r2 + 1.0f / 120.0f;
+    p = p * r2 + -1.0f / 6.0f;
+    return r + (r * r2) * p;
+}
+RB_HD float rb_cos_poly(float r) {
+    const float r2 = r * r;
+    float p = 1.0f / 479001600.0f;
+    p = p * r2 + -1.0f / 3628800.0f;
+    p = p * r2 + 1.0f / 40320.0f;
+    p = p * r2 + -1.0f / 720.0f;
+    p = p * r2 + 1.0f / 24.0f;
+    return (1.0f - 0.5f * r2) + (r2 * r2) * p;
+}
+RB_HD float rb_reduce(float x, int &k) {
+    const float kf = rintf(x * 0.636619772f);
+    k = (int)kf;
+    return ((x - kf * 1.5703125f) - kf * 4.837512969970703e-4f) - kf * 7.549790126404332e-8f;
+}
+RB_HD float rb_sin(float x) {
+    int k;
+    const float r = rb_reduce(x, k);
+    switch (k & 3) {
+        case 0: return rb_sin_poly(r);
+        case 1: return rb_cos_poly(r);
+        case 2: return -rb_sin_poly(r);
+        default: return -rb_cos_poly(r);
+    }
+}
+RB_HD float rb_cos(float x) {
+    int k;
+    const float r = rb_reduce(x, k);
+    switch (k & 3) {
+        case 0: return rb_cos_poly(r);
+        case 1: return -rb_sin_poly(r);
+        case 2: return -rb_cos_poly(r);
+        default: return rb_sin_poly(r);
+    }
+}
+RB_HD float rb_atan_core(float u) {
+    const float u2 = u * u;
+    float p = 1.0f / 19.0f;
+    p = p * u2 + -1.0f / 17.0f;
+    p = p * u2 + 1.0f / 15.0f;
+    p = p * u2 + -1.0f / 13.0f;
+    p = p * u2 + 1.0f / 11.0f;
+    p = p * u2 + -1.0f / 9.0f;
+    p = p * u2 + 1.0f / 7.0f;
+    p = p * u2 + -1.0f / 5.0f;
+    p = p * u2 + 1.0f / 3.0f;
+    return u - (u * u2) * p;
+}
+RB_HD float rb_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    const float t = mn / mx;
+    float a = t > 0.414213562f ? 0.785398163f + rb_atan_core((t - 1.0f) / (t + 1.0f)) : rb_atan_core(t);
+    if (ay > ax) a = 1.57079633f - a;
+    if (x < 0.0f) a = 3.14159265f - a;
+    return y < 0.0f ? -a : a;
+}
+// quat.hpp:52-58
+RB_HD Q from_axis_angle(V3 axis, float angle) {
+    const float half = angle * 0.5f;
+    const float s = rb_sin(half);
+    const V3 n = normalize(axis);
+    return Q{rb_cos(half), n.x * s, n.y * s, n.z * s};
+}
+
 }  // namespace rb

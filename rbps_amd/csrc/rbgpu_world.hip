@@ -15,6 +15,7 @@
 #include <vector>
 #include "../../include/rbgpu.h"
 #include "rbgpu_kernels.hpp"
+#include "rbgpu_joints.hpp"
 
 using namespace rb;
 
@@ -79,12 +80,19 @@ struct rbgpu_world {
     // host-side scene (descriptors) and broadphase bookkeeping for (re)uploads
     std::vector<rbgpu_body_desc> hb;
     std::vector<rbgpu_collider_desc> hc;
+    std::vector<rbgpu_joint_desc> hj;
+    std::vector<float4> h_jtg;          // joint target / speed / current (survives rebuilds)
+    std::vector<float> h_jlam;          // joint row lambdas (never reset)
     std::vector<float4> h_fmin, h_fmax;
     std::vector<uint32_t> h_uid, h_key, h_val;
     std::vector<uint2> h_eppos;
     bool built = false, sorted_once = false;
 
-    uint32_t N = 0, M = 0, S = 0, NEP = 0;
+    uint32_t N = 0, M = 0, S = 0, NEP = 0, NJ = 0;
+    Joints Jt{};
+    uint32_t *jorder = nullptr;         // joints by colour
+    std::vector<uint32_t> jcol_off;     // joint colour offsets
+    std::vector<int32_t> h_jcolour;
     Bodies B{};
     Colliders C{};
     Contacts K{};
@@ -120,6 +128,7 @@ struct rbgpu_world {
     // tuning: grid-wide colouring rounds before the single-workgroup rounds; colours with
     // fewer pairs than run_min are solved in single-workgroup runs
     uint32_t head_rounds = 96, round_batch = 12, tail_min = 2048, run_min = 256, palette_max = 8;
+    int joint_mode = 0;             // joint solver: 0 the reference's, 1 XPBD rows (rbgpu_joints.hpp)
 
     // timing
     int timing_kernel = 0;
@@ -145,6 +154,8 @@ struct rbgpu_world {
         dfree(B.x); dfree(B.q); dfree(B.v); dfree(B.w); dfree(B.px); dfree(B.pq); dfree(B.f); dfree(B.t);
         dfree(B.I0); dfree(B.I1); dfree(B.Ii0); dfree(B.Ii1);
         dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat);
+        dfree(Jt.jm); dfree(Jt.jr1); dfree(Jt.jr2); dfree(Jt.jax); dfree(Jt.jlx); dfree(Jt.jtg); dfree(Jt.jrow);
+        dfree(Jt.jlam); dfree(jorder);
         dfree(fmin); dfree(fmax); dfree(uid); dfree(ep_key[0]); dfree(ep_key[1]); dfree(ep_val[0]); dfree(ep_val[1]);
         dfree(ep_pos); dfree(sw_meta); dfree(sw_a); dfree(sw_b); dfree(ep_cnt); dfree(ep_off); dfree(ep_hist);
         dfree(flags); dfree(scal); dfree(tile_sums);
@@ -206,6 +217,7 @@ struct rbgpu_world {
 
     // ------------------------------------------------------------ upload
     int build();
+    int build_joints();
     int download_state();
     int ensure_pair_capacity(uint32_t need);
     int frame();
@@ -323,6 +335,8 @@ int rbgpu_world::build() {
     }
     n_plane = (uint32_t)planes.size();
     if (n_plane) HIPCHK(up(plane_pairs, planes.data(), n_plane * 8));
+    int rcj = build_joints();
+    if (rcj) return rcj;
     int init_flags[4] = {1, 0, 0, 0};   // dirty: the endpoint array has never been sorted
     HIPCHK(up(flags, init_flags, sizeof(init_flags)));
     HIPCHK(hipMemsetAsync(scal, 0, 32, st));
@@ -332,6 +346,67 @@ int rbgpu_world::build() {
     int rc2 = ensure_pair_capacity(std::max<uint32_t>(8 * S + n_plane, 4096));
     if (rc2) return rc2;
     built = true;
+    return RBGPU_OK;
+}
+
+// Joint upload and colouring.  Colouring: first fit in joint order, conflict = a shared
+// dynamic body (computed once per upload; the joint graph does not change while stepping).
+int rbgpu_world::build_joints() {
+    NJ = (uint32_t)hj.size();
+    h_jtg.resize(NJ);
+    h_jlam.resize((size_t)NJ * 4, 0.0f);
+    std::vector<int4> jm(NJ);
+    std::vector<float4> jr1(NJ), jr2(NJ), jax(NJ), jlx(NJ), jrow((size_t)NJ * 4, make_float4(1.0f, 0.0f, 0.0f, 0.0f));
+    h_jcolour.assign(NJ, 0);
+    std::vector<std::vector<int32_t>> body_cols(hb.size());
+    int ncolj = 0;
+    for (uint32_t i = 0; i < NJ; ++i) {
+        const rbgpu_joint_desc &d = hj[i];
+        int rows = d.type == RBGPU_JOINT_PRISMATIC ? 3 : d.type == RBGPU_JOINT_REVOLUTE ? 4
+                 : d.type == RBGPU_JOINT_FIXED ? 2 : (d.limited ? 2 : 1);
+        const bool fixed = d.type == RBGPU_JOINT_FIXED;
+        const int act = fixed ? RBGPU_ACT_FREE : d.actuation;
+        const int lim = fixed ? 0 : (d.limited ? 1 : 0);
+        jm[i] = make_int4(d.type | (act << 8) | (lim << 16) | (rows << 24), (int)d.body_1, (int)d.body_2, 0);
+        jr1[i] = make_float4((float)d.r_1[0], (float)d.r_1[1], (float)d.r_1[2], fixed ? 0.0f : (float)d.damping);
+        jr2[i] = make_float4((float)d.r_2[0], (float)d.r_2[1], (float)d.r_2[2], (float)d.compliance);
+        const bool lax = d.type == RBGPU_JOINT_REVOLUTE;   // the others carry a zero limit axis (JointAPI.hpp)
+        jax[i] = make_float4(fixed ? 0.0f : (float)d.axis[0], fixed ? 0.0f : (float)d.axis[1], fixed ? 0.0f : (float)d.axis[2],
+                             fixed ? 0.0f : (float)d.lower_limit);
+        jlx[i] = make_float4(lax ? (float)d.limit_axis[0] : 0.0f, lax ? (float)d.limit_axis[1] : 0.0f,
+                             lax ? (float)d.limit_axis[2] : 0.0f, fixed ? 0.0f : (float)d.upper_limit);
+        const uint32_t bs[2] = {d.body_1, d.body_2};
+        int c = 0;
+        for (;; ++c) {
+            bool taken = false;
+            for (uint32_t b : bs)
+                if (hb[b].type == RBGPU_DYNAMIC)
+                    for (int32_t u : body_cols[b]) taken = taken || u == c;
+            if (!taken) break;
+        }
+        h_jcolour[i] = c;
+        for (uint32_t b : bs)
+            if (hb[b].type == RBGPU_DYNAMIC) body_cols[b].push_back(c);
+        ncolj = std::max(ncolj, c + 1);
+    }
+    std::vector<uint32_t> order(NJ);
+    for (uint32_t i = 0; i < NJ; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return h_jcolour[a] < h_jcolour[b]; });
+    jcol_off.assign(ncolj + 1, 0);
+    for (uint32_t i = 0; i < NJ; ++i) ++jcol_off[h_jcolour[i] + 1];
+    for (int c = 0; c < ncolj; ++c) jcol_off[c + 1] += jcol_off[c];
+    int rc = 0;
+    rc |= dalloc(Jt.jm, NJ); rc |= dalloc(Jt.jr1, NJ); rc |= dalloc(Jt.jr2, NJ); rc |= dalloc(Jt.jax, NJ);
+    rc |= dalloc(Jt.jlx, NJ); rc |= dalloc(Jt.jtg, NJ); rc |= dalloc(Jt.jrow, (size_t)NJ * 4);
+    rc |= dalloc(Jt.jlam, (size_t)NJ * 4); rc |= dalloc(jorder, NJ);
+    if (rc) return RBGPU_ERR_OOM;
+    if (!NJ) return RBGPU_OK;
+    auto up = [&](void *dst, const void *src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st); };
+    HIPCHK(up(Jt.jm, jm.data(), NJ * 16)); HIPCHK(up(Jt.jr1, jr1.data(), NJ * 16)); HIPCHK(up(Jt.jr2, jr2.data(), NJ * 16));
+    HIPCHK(up(Jt.jax, jax.data(), NJ * 16)); HIPCHK(up(Jt.jlx, jlx.data(), NJ * 16));
+    HIPCHK(up(Jt.jtg, h_jtg.data(), NJ * 16)); HIPCHK(up(Jt.jrow, jrow.data(), (size_t)NJ * 64));
+    HIPCHK(up(Jt.jlam, h_jlam.data(), (size_t)NJ * 16)); HIPCHK(up(jorder, order.data(), NJ * 4));
+    HIPCHK(hipStreamSynchronize(st));
     return RBGPU_OK;
 }
 
@@ -360,6 +435,10 @@ int rbgpu_world::download_state() {
         HIPCHK(hipMemcpy(h_key.data(), ep_key[0], NEP * 4, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(h_val.data(), ep_val[0], NEP * 4, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(h_eppos.data(), ep_pos, S * 8, hipMemcpyDeviceToHost));
+    }
+    if (NJ) {
+        HIPCHK(hipMemcpy(h_jtg.data(), Jt.jtg, NJ * 16, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h_jlam.data(), Jt.jlam, (size_t)NJ * 16, hipMemcpyDeviceToHost));
     }
     // a pending dirty flag is re-established by build()
     return RBGPU_OK;
@@ -507,6 +586,17 @@ int rbgpu_world::frame() {
                 timed_end(e);
             }
         }
+        // joints: errors from the state after the contact solve, then rows per colour
+        // (World.hpp:166-171)
+        if (NJ) {
+            hipEvent_t *e = timed_begin(10);
+            k_joint_errors<<<nblk(NJ), TPB, 0, st>>>(B, Jt, NJ, h, joint_mode);
+            for (size_t c = 0; c + 1 < jcol_off.size(); ++c) {
+                const uint32_t o = jcol_off[c], n = jcol_off[c + 1] - o;
+                if (n) k_joint_solve<<<nblk(n), TPB, 0, st>>>(B, Jt, jorder, o, n, inv_h, h, joint_mode);
+            }
+            timed_end(e);
+        }
         if (N) {
             hipEvent_t *e = timed_begin(5);
             k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h);
@@ -523,6 +613,15 @@ int rbgpu_world::frame() {
                 k_solve_vel<<<nblk(n), TPB, 0, st>>>(B, K, solve_list, o, n, h, e ? d_points : nullptr);
                 timed_end(e);
             }
+        }
+        // joint damping (World.hpp:174-178)
+        if (NJ) {
+            hipEvent_t *e = timed_begin(10);
+            for (size_t c = 0; c + 1 < jcol_off.size(); ++c) {
+                const uint32_t o = jcol_off[c], n = jcol_off[c + 1] - o;
+                if (n) k_joint_damp<<<nblk(n), TPB, 0, st>>>(B, Jt, jorder, o, n, h);
+            }
+            timed_end(e);
         }
     }
     ++frames;
@@ -596,6 +695,63 @@ int rbgpu_add_colliders(rbgpu_world *w, int n, const rbgpu_collider_desc *desc, 
         w->h_val.push_back((slot << 1) | 1u);
     }
     w->built = false;
+    return RBGPU_OK;
+}
+
+int rbgpu_add_joints(rbgpu_world *w, int n, const rbgpu_joint_desc *desc, uint32_t *ids_out) {
+    if (!w || n < 0 || (n && !desc)) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    for (int i = 0; i < n; ++i) {
+        const rbgpu_joint_desc &d = desc[i];
+        if (d.type < RBGPU_JOINT_PRISMATIC || d.type > RBGPU_JOINT_SPHERICAL) return fail(RBGPU_ERR_INVALID, "unknown joint type");
+        if (d.actuation < RBGPU_ACT_FREE || d.actuation > RBGPU_ACT_SPEED) return fail(RBGPU_ERR_INVALID, "unknown actuation");
+        if (d.body_1 >= w->hb.size() || d.body_2 >= w->hb.size()) return fail(RBGPU_ERR_INVALID, "joint body out of range");
+        if (d.body_1 == d.body_2) return fail(RBGPU_ERR_INVALID, "joint needs two different bodies");
+    }
+    int rc = w->download_state();
+    if (rc) return rc;
+    for (int i = 0; i < n; ++i) {
+        if (ids_out) ids_out[i] = (uint32_t)w->hj.size();
+        w->hj.push_back(desc[i]);
+        w->h_jtg.push_back(make_float4((float)desc[i].target_position, (float)desc[i].target_speed, 0.0f, 0.0f));
+        for (int k = 0; k < 4; ++k) w->h_jlam.push_back(0.0f);
+    }
+    w->built = false;
+    return RBGPU_OK;
+}
+
+int rbgpu_get_joints(rbgpu_world *w, double *cur, double *tgt, double *lam4, int32_t *colour) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    const size_t nj = w->hj.size();
+    std::vector<float4> tg(w->h_jtg.begin(), w->h_jtg.end());
+    std::vector<float> lam(w->h_jlam.begin(), w->h_jlam.end());
+    if (w->built && w->NJ) {
+        HIPCHK(hipStreamSynchronize(w->st));
+        HIPCHK(hipMemcpy(tg.data(), w->Jt.jtg, w->NJ * 16, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(lam.data(), w->Jt.jlam, (size_t)w->NJ * 16, hipMemcpyDeviceToHost));
+    }
+    for (size_t i = 0; i < nj; ++i) {
+        if (cur) cur[i] = tg[i].z;
+        if (tgt) tgt[i] = tg[i].x;
+        if (lam4) for (int k = 0; k < 4; ++k) lam4[4 * i + k] = lam[4 * i + k];
+        if (colour) colour[i] = (w->built && i < w->h_jcolour.size()) ? w->h_jcolour[i] : -1;
+    }
+    return RBGPU_OK;
+}
+
+int rbgpu_set_joint_targets(rbgpu_world *w, int n, const uint32_t *ids, const double *tgt, const double *speed) {
+    if (!w || n < 0 || (n && !ids)) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    for (int k = 0; k < n; ++k)
+        if (ids[k] >= w->hj.size()) return fail(RBGPU_ERR_INVALID, "joint id out of range");
+    int rc = w->download_state();
+    if (rc) return rc;
+    for (int k = 0; k < n; ++k) {
+        float4 &t = w->h_jtg[ids[k]];
+        if (tgt) t.x = (float)tgt[k];
+        if (speed) t.y = (float)speed[k];
+    }
+    if (w->built && w->NJ) {
+        HIPCHK(hipMemcpy(w->Jt.jtg, w->h_jtg.data(), w->NJ * 16, hipMemcpyHostToDevice));
+    }
     return RBGPU_OK;
 }
 
@@ -786,6 +942,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_TIMING_STRIDE:
             if (value < 1) return fail(RBGPU_ERR_INVALID, "timing stride must be >= 1");
             w->timing_stride = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_JOINT_MODE:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "joint mode is 0 or 1");
+            w->joint_mode = value; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
@@ -799,12 +958,13 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_TAIL_MIN: *value = (int)w->tail_min; return RBGPU_OK;
         case RBGPU_OPT_RUN_MIN: *value = (int)w->run_min; return RBGPU_OK;
         case RBGPU_OPT_TIMING_STRIDE: *value = (int)w->timing_stride; return RBGPU_OK;
+        case RBGPU_OPT_JOINT_MODE: *value = w->joint_mode; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
 
 int rbgpu_set_timing(rbgpu_world *w, int kernel) {
-    if (!w || kernel < 0 || kernel > 9) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (!w || kernel < 0 || kernel > 10) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     if (w->st) HIPCHK(hipStreamSynchronize(w->st));
     w->timing_kernel = kernel;
     w->ev_used = 0;

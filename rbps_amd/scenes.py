@@ -1,9 +1,11 @@
 """Scene descriptors and seeded scene generators.
 
-The descriptor dtypes mirror the C structs ``rbgpu_body_desc`` and
-``rbgpu_collider_desc`` declared in ``include/rbgpu.h`` (the fields of the
-reference's ``rbps::BodyParams``, include/rbps/API/BodyAPI.hpp:22-31, and
-``rbps::ColliderParams``, include/rbps/API/ColliderAPI.hpp:17-27).
+The descriptor dtypes mirror the C structs ``rbgpu_body_desc``,
+``rbgpu_collider_desc`` and ``rbgpu_joint_desc`` declared in ``include/rbgpu.h``
+(the fields of the reference's ``rbps::BodyParams``, include/rbps/API/BodyAPI.hpp:22-31,
+``rbps::ColliderParams``, include/rbps/API/ColliderAPI.hpp:17-27, and the
+``Prismatic/Revolute/FixedJointParams`` of include/rbps/API/JointAPI.hpp:22-68 plus the
+joint's public target fields).
 
 Shape codes are the reference's ``rbc::Shape`` variant indices
 (include/rbc/shapes/ShapeTypes.hpp:69): sphere 0, box 1, capsule 3, plane 6.
@@ -16,6 +18,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 STATIC, DYNAMIC = 0, 1
+PRISMATIC, REVOLUTE, FIXED, SPHERICAL = 0, 1, 2, 3      # Joint.hpp JointType (+ spherical)
+ACT_FREE, ACT_POSITION, ACT_SPEED = 0, 1, 2            # Joint.hpp JointActuationType
 SPHERE, BOX, CAPSULE, PLANE, HULL = 0, 1, 3, 6, 9
 
 BODY_DTYPE = np.dtype([
@@ -31,6 +35,14 @@ COLLIDER_DTYPE = np.dtype([
     ("restitution", "<f8"), ("static_friction", "<f8"), ("dynamic_friction", "<f8"),
 ], align=True)
 assert COLLIDER_DTYPE.itemsize == 128
+
+JOINT_DTYPE = np.dtype([
+    ("type", "<i4"), ("actuation", "<i4"), ("body_1", "<u4"), ("body_2", "<u4"),
+    ("r_1", "<f8", (3,)), ("r_2", "<f8", (3,)), ("axis", "<f8", (3,)), ("limit_axis", "<f8", (3,)),
+    ("compliance", "<f8"), ("damping", "<f8"), ("limited", "<i4"), ("_pad", "<i4"),
+    ("lower_limit", "<f8"), ("upper_limit", "<f8"), ("target_position", "<f8"), ("target_speed", "<f8"),
+], align=True)
+assert JOINT_DTYPE.itemsize == 168
 
 
 # ---------------------------------------------------------------- unit-density mass properties
@@ -80,6 +92,7 @@ class Scene:
     substeps: int = 20
     frames: int = 1
     meta: dict = field(default_factory=dict)
+    joints: np.ndarray = field(default_factory=lambda: np.zeros(0, JOINT_DTYPE))
 
     @property
     def n_dynamic(self) -> int:
@@ -90,6 +103,7 @@ class SceneBuilder:
     def __init__(self):
         self.b = []
         self.c = []
+        self.j = []
 
     def body(self, type_=DYNAMIC, mass=1.0, inertia=(1.0, 1.0, 1.0, 0.0, 0.0, 0.0), pos=(0, 0, 0),
              quat=(1, 0, 0, 0), vel=(0, 0, 0), angvel=(0, 0, 0)) -> int:
@@ -121,10 +135,23 @@ class SceneBuilder:
         self.c.append(r)
         return len(self.c) - 1
 
+    def joint(self, type_, body_1, body_2, r_1=(0, 0, 0), r_2=(0, 0, 0), axis=(1, 0, 0), limit_axis=(0, 1, 0),
+              actuation=ACT_FREE, compliance=0.0, damping=0.0, limited=False, lower=0.0, upper=0.0,
+              target_position=0.0, target_speed=0.0) -> int:
+        r = np.zeros((), JOINT_DTYPE)
+        r["type"], r["actuation"], r["body_1"], r["body_2"] = type_, actuation, body_1, body_2
+        r["r_1"], r["r_2"], r["axis"], r["limit_axis"] = r_1, r_2, axis, limit_axis
+        r["compliance"], r["damping"], r["limited"] = compliance, damping, 1 if limited else 0
+        r["lower_limit"], r["upper_limit"] = lower, upper
+        r["target_position"], r["target_speed"] = target_position, target_speed
+        self.j.append(r)
+        return len(self.j) - 1
+
     def build(self, name, timestep=1.0 / 60.0, substeps=20, frames=1, **meta) -> Scene:
         bodies = np.array(self.b, dtype=BODY_DTYPE) if self.b else np.zeros(0, BODY_DTYPE)
         cols = np.array(self.c, dtype=COLLIDER_DTYPE) if self.c else np.zeros(0, COLLIDER_DTYPE)
-        return Scene(name, bodies, cols, timestep, substeps, frames, dict(meta))
+        joints = np.array(self.j, dtype=JOINT_DTYPE) if self.j else np.zeros(0, JOINT_DTYPE)
+        return Scene(name, bodies, cols, timestep, substeps, frames, dict(meta), joints)
 
 
 def _rand_quat(rng) -> tuple:
@@ -218,3 +245,155 @@ def pile_mixed_dense(seed=7, nx=8, ny=6, nz=8, spacing=1.05, y0=0.7, **kw) -> Sc
     """A small, initially interpenetrating mixed pile: every pair type and many
     contacts from the first frame (parity stress case)."""
     return pile_mixed(seed=seed, nx=nx, ny=ny, nz=nz, spacing=spacing, y0=y0, **kw)
+
+
+# ---------------------------------------------------------------- jointed scenes
+def _box(sb, pos, half, density=1000.0, quat=(1, 0, 0, 0), vel=(0, 0, 0), type_=DYNAMIC, friction=0.5):
+    m = density * box_volume(half)
+    inertia = tuple(density * x for x in box_inertia(half))
+    b = sb.body(type_, m if type_ == DYNAMIC else 0.0, inertia, pos, quat, vel)
+    sb.collider(BOX, b, half, is_static=type_ == STATIC, restitution=0.0, static_friction=friction,
+                dynamic_friction=friction)
+    return b
+
+
+def _anchor(sb, pos):
+    """A static body without a collider (a joint anchor nothing collides with)."""
+    return sb.body(STATIC, 0.0, (1.0, 1.0, 1.0, 0.0, 0.0, 0.0), pos)
+
+
+def _capsule(sb, pos, hh, r, density=1000.0, quat=(1, 0, 0, 0), friction=0.5):
+    m = density * capsule_volume(hh, r)
+    inertia = tuple(density * x for x in capsule_inertia(hh, r))
+    b = sb.body(DYNAMIC, m, inertia, pos, quat)
+    sb.collider(CAPSULE, b, (hh, r), restitution=0.0, static_friction=friction, dynamic_friction=friction)
+    return b
+
+
+def _chain(sb, anchor, anchor_pos, n, hh, r, compliance, limited_every=0, swing=math.radians(45)):
+    """n capsules hanging sideways from `anchor` (capsule axis = local y, laid along +x),
+    linked end to end by spherical joints half a radius beyond the capsule tips (the
+    reference does not filter collisions between jointed bodies, so links must not touch
+    when straight)."""
+    qz = (math.cos(math.pi / 4), 0.0, 0.0, -math.sin(math.pi / 4))   # local y -> world +x
+    tip = hh + 1.5 * r
+    link = 2.0 * tip
+    prev, prev_tip = anchor, (0.0, 0.0, 0.0)
+    x0, y0, z0 = anchor_pos
+    for k in range(n):
+        c = _capsule(sb, (x0 + link * (k + 0.5), y0, z0), hh, r, quat=qz)
+        lim = limited_every and (k % limited_every == limited_every - 1)
+        sb.joint(SPHERICAL, prev, c, r_1=prev_tip, r_2=(0.0, -tip, 0.0), axis=(0, 1, 0),
+                 compliance=compliance, limited=bool(lim), upper=swing)
+        prev, prev_tip = c, (0.0, tip, 0.0)
+    return prev
+
+
+def joint_zoo(seed=31, spherical=True) -> Scene:
+    """Every joint type and feature next to contacts: limited and driven revolutes,
+    a limited and driven prismatic, a fixed pair falling on the plane, damping, and
+    (with `spherical`) a capsule chain on spherical joints with one swing limit."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    add_ground_plane(sb)
+    anchor = _box(sb, (0.0, 4.0, 0.0), (0.2, 0.2, 0.2), type_=STATIC)
+    # pendulum on a limited, damped revolute (hinge about z), released sideways
+    p = _box(sb, (1.0, 4.0, 0.0), (0.4, 0.1, 0.1))
+    sb.joint(REVOLUTE, anchor, p, r_1=(0.0, 0.0, 0.0), r_2=(-1.0, 0.0, 0.0), axis=(0, 0, 1), limit_axis=(1, 0, 0),
+             damping=0.2, limited=True, lower=-math.radians(60), upper=math.radians(60))
+    # motor: speed-driven revolute about y with a compliant drive
+    a2 = _box(sb, (3.0, 3.0, 0.0), (0.2, 0.2, 0.2), type_=STATIC)
+    m = _box(sb, (3.0, 2.4, 0.0), (0.5, 0.05, 0.1))
+    sb.joint(REVOLUTE, a2, m, r_1=(0.0, -0.3, 0.0), r_2=(0.0, 0.3, 0.0), axis=(0, 1, 0), limit_axis=(1, 0, 0),
+             actuation=ACT_SPEED, target_speed=2.0, compliance=1e-4)
+    # position-driven revolute with a target the limit cuts off
+    a3 = _box(sb, (5.0, 3.0, 0.0), (0.2, 0.2, 0.2), type_=STATIC)
+    m3 = _box(sb, (5.8, 3.0, 0.0), (0.5, 0.1, 0.1))
+    sb.joint(REVOLUTE, a3, m3, r_2=(-0.8, 0.0, 0.0), axis=(0, 0, 1), limit_axis=(1, 0, 0), actuation=ACT_POSITION,
+             target_position=1.2, limited=True, lower=-0.5, upper=0.8, damping=0.5)
+    # slider: limited prismatic, position-driven, damped, between two dynamic boxes
+    s1 = _box(sb, (-3.0, 1.0, 0.0), (0.3, 0.3, 0.3))
+    s2 = _box(sb, (-2.0, 1.0, 0.0), (0.2, 0.2, 0.2))
+    sb.joint(PRISMATIC, s1, s2, axis=(1, 0, 0), actuation=ACT_POSITION, target_position=0.4, compliance=1e-5,
+             damping=0.5, limited=True, lower=-1.5, upper=-0.3)
+    # speed-driven prismatic on a static rail
+    rail = _box(sb, (-6.0, 2.0, 0.0), (0.1, 0.1, 0.1), type_=STATIC)
+    car = _box(sb, (-6.0, 1.5, 0.0), (0.2, 0.2, 0.2))
+    sb.joint(PRISMATIC, rail, car, axis=(0, 1, 0), actuation=ACT_SPEED, target_speed=-0.5, damping=0.1)
+    # fixed pair tumbling onto the plane
+    f1 = _box(sb, (0.0, 1.5, 3.0), (0.3, 0.3, 0.3), quat=_rand_quat(rng))
+    f2 = _box(sb, (0.5, 1.9, 3.0), (0.2, 0.2, 0.2))
+    sb.joint(FIXED, f1, f2, r_1=(0.25, 0.2, 0.0), r_2=(-0.25, -0.2, 0.0))
+    # loose spheres for contacts with the jointed bodies
+    for k in range(6):
+        rr = 0.2 + 0.1 * rng.random()
+        pos = (-1.0 + 0.7 * k, 2.0 + rng.random(), 3.0 + 0.2 * rng.standard_normal())
+        b = sb.body(DYNAMIC, 1000.0 * sphere_volume(rr), tuple(1000.0 * x for x in sphere_inertia(rr)), pos)
+        sb.collider(SPHERE, b, (rr,), restitution=0.0)
+    if spherical:
+        a4 = _anchor(sb, (0.0, 3.0, -3.0))
+        _chain(sb, a4, (0.0, 3.0, -3.0), 6, 0.2, 0.08, 1e-6, limited_every=3)
+    return sb.build(f"joint_zoo{'_sph' if spherical else ''}", 1.0 / 60.0, 20, 120, seed=seed)
+
+
+def stack_chain(n_bodies=64, seed=0, origin=(0.0, 0.0, 0.0), chain_links=16, density=1000.0) -> Scene:
+    """One batched-workload scene: a seeded box stack of (n_bodies - chain_links) boxes
+    plus a chain of `chain_links` capsules on spherical joints (compliance 1e-6) hanging
+    from a static anchor, over a static plane."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    add_ground_plane(sb)
+    ox, oy, oz = origin
+    nb = max(n_bodies - chain_links, 0)
+    cols = max(1, int(math.ceil(math.sqrt(nb / 6.0))))
+    for k in range(nb):
+        c, lvl = k % (cols * cols), k // (cols * cols)
+        h = 0.25 + 0.05 * rng.random()
+        x = ox + (c % cols) * 0.7 + rng.uniform(-0.02, 0.02)
+        z = oz + (c // cols) * 0.7 + rng.uniform(-0.02, 0.02)
+        _box(sb, (x, oy + 0.3 + lvl * 0.62, z), (h, h, h), density)
+    a = _anchor(sb, (ox - 1.5, oy + 6.0, oz))
+    _chain(sb, a, (ox - 1.5, oy + 6.0, oz), chain_links, 0.15, 0.06, 1e-6)
+    return sb.build(f"stack_chain_{n_bodies}", 1.0 / 60.0, 10, 100, seed=seed)
+
+
+def planar_joints(seed=5) -> Scene:
+    """Joints whose motion stays in the xy plane (hinges about z, sliders along x or y with
+    attachments at the body centres): the reference's rotational rows then keep exactly
+    zero out-of-plane errors and its step stays finite (see joint_zoo for the general case)."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    add_ground_plane(sb)
+    piv = [_box(sb, (4.0 * k, 5.0, 0.0), (0.1, 0.1, 0.1), type_=STATIC) for k in range(4)]
+    # free pendulum, limited + damped pendulum, speed motor, position servo (hinges about z)
+    bobs = [_box(sb, (4.0 * k + 1.2, 5.0 - 0.3 * k, 0.0), (0.35, 0.1, 0.1)) for k in range(4)]
+    for k in range(4):
+        sb.joint(REVOLUTE, piv[k], bobs[k], r_2=(-1.2 if k != 1 else -1.2, 0.3 * k, 0.0), axis=(0, 0, 1),
+                 limit_axis=(1, 0, 0), damping=(0.3 if k == 1 else 0.0), limited=(k == 1),
+                 lower=-math.radians(50), upper=math.radians(50),
+                 actuation=(ACT_SPEED if k == 2 else ACT_POSITION if k == 3 else ACT_FREE),
+                 target_speed=(1.5 if k == 2 else 0.0), target_position=(0.6 if k == 3 else 0.0),
+                 compliance=(1e-3 if k >= 2 else 0.0))
+    # a double pendulum in the plane
+    d1 = _box(sb, (17.0, 5.0, 0.0), (0.5, 0.08, 0.08))
+    d2 = _box(sb, (18.0, 5.0, 0.0), (0.5, 0.08, 0.08))
+    p5 = _box(sb, (16.5, 5.0, 0.0), (0.1, 0.1, 0.1), type_=STATIC)
+    sb.joint(REVOLUTE, p5, d1, r_2=(-0.5, 0.0, 0.0), axis=(0, 0, 1), limit_axis=(1, 0, 0))
+    sb.joint(REVOLUTE, d1, d2, r_1=(0.5, 0.0, 0.0), r_2=(-0.5, 0.0, 0.0), axis=(0, 0, 1), limit_axis=(1, 0, 0))
+    # sliders: a limited, position-driven, damped one between two free boxes on the plane, and
+    # a speed-driven vertical one on a static rail
+    s1 = _box(sb, (-3.0, 0.3, 0.0), (0.3, 0.3, 0.3))
+    s2 = _box(sb, (-2.0, 0.2, 0.0), (0.2, 0.2, 0.2))
+    sb.joint(PRISMATIC, s1, s2, axis=(1, 0, 0), actuation=ACT_POSITION, target_position=0.5, compliance=1e-5,
+             damping=0.5, limited=True, lower=-1.5, upper=-0.3)
+    rail = _box(sb, (-6.0, 3.0, 0.0), (0.1, 0.1, 0.1), type_=STATIC)
+    car = _box(sb, (-6.0, 2.0, 0.0), (0.2, 0.2, 0.2))
+    sb.joint(PRISMATIC, rail, car, axis=(0, 1, 0), actuation=ACT_SPEED, target_speed=-0.5, damping=0.1,
+             limited=True, lower=-2.5, upper=2.5)
+    # spheres dropped on the slider pair (contacts with jointed bodies, still planar)
+    for k in range(3):
+        rr = 0.2 + 0.05 * k
+        b = sb.body(DYNAMIC, 1000.0 * sphere_volume(rr), tuple(1000.0 * x for x in sphere_inertia(rr)),
+                    (-3.2 + 0.6 * k, 1.5 + 0.5 * k + 0.1 * rng.random(), 0.0))
+        sb.collider(SPHERE, b, (rr,), restitution=0.0)
+    return sb.build("planar_joints", 1.0 / 60.0, 20, 120, seed=seed)

@@ -29,7 +29,8 @@ EXPORTS = [
     "rbgpu_world_create", "rbgpu_world_destroy", "rbgpu_add_bodies", "rbgpu_add_colliders", "rbgpu_step",
     "rbgpu_set_step", "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
     "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_collide_pairs",
-    "rbgpu_set_option", "rbgpu_get_option", "rbgpu_last_error",
+    "rbgpu_set_option", "rbgpu_get_option", "rbgpu_add_joints", "rbgpu_get_joints", "rbgpu_set_joint_targets",
+    "rbgpu_last_error",
 ]
 
 
@@ -67,6 +68,9 @@ def lib():
         L.rbgpu_get_contacts.argtypes = [_P, _U, _D, _D, _D, _D, _D, C.c_uint32, _U]
         L.rbgpu_get_stats.argtypes = [_P, C.POINTER(Stats)]
         L.rbgpu_set_timing.argtypes = [_P, C.c_int]
+        L.rbgpu_add_joints.argtypes = [_P, C.c_int, _P, _U]
+        L.rbgpu_get_joints.argtypes = [_P, _D, _D, _D, _I]
+        L.rbgpu_set_joint_targets.argtypes = [_P, C.c_int, _U, _D, _D]
         L.rbgpu_set_option.argtypes = [_P, C.c_int, C.c_int]
         L.rbgpu_get_option.argtypes = [_P, C.c_int, _I]
         L.rbgpu_get_timing.argtypes = [_P, _D, C.POINTER(C.c_uint64), _D]
@@ -97,10 +101,11 @@ def collide_pairs(kind_a, dims_a, tf_a, kind_b, dims_b, tf_b) -> np.ndarray:
     return out
 
 
-OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, "run_min": 4, "timing_stride": 5}
+OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, "run_min": 4, "timing_stride": 5,
+           "joint_mode": 6}
 
 TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "broadphase": 6,
-          "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9}
+          "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9, "joints": 10}
 
 
 class World:
@@ -113,6 +118,7 @@ class World:
         self.h = h
         self.n_bodies = 0
         self.n_colliders = 0
+        self.n_joints = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -156,9 +162,34 @@ class World:
                     dynamic_friction)
         return int(self.add_colliders(sb.build("c").colliders)[0])
 
+    def add_joints(self, desc) -> np.ndarray:
+        """World::create_{prismatic,revolute,fixed}_joint (JointAPI.hpp:160-265) and the
+        spherical extension, from JOINT_DTYPE rows; returns the joint ids."""
+        desc = np.ascontiguousarray(desc, dtype=scenes.JOINT_DTYPE)
+        ids = np.zeros(len(desc), np.uint32)
+        _check(lib().rbgpu_add_joints(self.h, len(desc), desc.ctypes.data_as(_P), _ptr(ids, _U)))
+        self.n_joints += len(desc)
+        return ids
+
+    def joints(self) -> dict:
+        n = self.n_joints
+        cur, tgt, lam, col = np.zeros(n), np.zeros(n), np.zeros((n, 4)), np.zeros(n, np.int32)
+        if n:
+            _check(lib().rbgpu_get_joints(self.h, _ptr(cur, _D), _ptr(tgt, _D), _ptr(lam, _D), _ptr(col, _I)))
+        return {"current_position": cur, "target_position": tgt, "lambda": lam, "colour": col}
+
+    def set_joint_targets(self, ids, target_position=None, target_speed=None):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        tp = None if target_position is None else np.ascontiguousarray(target_position, np.float64)
+        ts = None if target_speed is None else np.ascontiguousarray(target_speed, np.float64)
+        _check(lib().rbgpu_set_joint_targets(self.h, len(ids), _ptr(ids, _U), _ptr(tp, _D), _ptr(ts, _D)))
+
     def load(self, scene) -> "World":
         self.add_bodies(scene.bodies)
         self.add_colliders(scene.colliders)
+        js = getattr(scene, "joints", None)
+        if js is not None and len(js):
+            self.add_joints(js)
         return self
 
     # ---------------------------------------------------------------- step

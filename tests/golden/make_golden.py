@@ -30,6 +30,8 @@ def world_fixture(name, sc, frames):
     w = orc.RefWorld(sc.timestep, sc.substeps).load(sc)
     out = {"bodies": sc.bodies, "colliders": sc.colliders, "timestep": sc.timestep, "substeps": sc.substeps,
            "frames": np.array(frames)}
+    if len(sc.joints):
+        out["joints"] = sc.joints
     done = 0
     for f in frames:
         w.step(f - done)
@@ -42,6 +44,10 @@ def world_fixture(name, sc, frames):
         out[f"contact_ids_{f}"] = c["ids"]
         out[f"contact_normal_{f}"] = c["normal"]
         out[f"contact_lambda_{f}"] = c["lambda"]
+        if len(sc.joints):
+            j = w.joints()
+            for k in ("current_position", "target_position", "lambda"):
+                out[f"joint_{k}_{f}"] = j[k]
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
     print(name, {k: getattr(v, "shape", v) for k, v in out.items() if k.startswith(("position", "pairs"))})
 
@@ -85,6 +91,43 @@ def narrowphase_fixture(name, n=1500, seed=5):
     print(name, "hits", int((res[:, 0] > 0).sum()), "of", n)
 
 
+def joint_errors_fixture(name, n=900, seed=11):
+    """Random joints between two bodies at random poses through the reference's
+    compute_{prismatic,revolute,fixed}_joint_errors (Joint.cpp:18-182)."""
+    rng = np.random.default_rng(seed)
+    d = np.zeros(n, scenes.JOINT_DTYPE)
+    p1, p2 = np.zeros((n, 7)), np.zeros((n, 7))
+    hs = np.zeros(n)
+    res = np.zeros((n, 18))
+
+    def unit(v):
+        return v / np.linalg.norm(v)
+
+    for i in range(n):
+        t = int(rng.integers(3))
+        d[i]["type"] = t
+        d[i]["actuation"] = int(rng.integers(3))
+        d[i]["r_1"], d[i]["r_2"] = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        d[i]["axis"] = unit(rng.standard_normal(3)) if rng.random() < 0.7 else np.eye(3)[rng.integers(3)]
+        d[i]["limit_axis"] = unit(np.cross(d[i]["axis"], rng.standard_normal(3)))
+        d[i]["limited"] = int(rng.random() < 0.6)
+        lo = -rng.uniform(0.0, 2.0)
+        d[i]["lower_limit"], d[i]["upper_limit"] = lo, lo + rng.uniform(0.1, 3.0)
+        d[i]["target_position"], d[i]["target_speed"] = rng.uniform(-2, 2), rng.uniform(-3, 3)
+        d[i]["compliance"], d[i]["damping"] = rng.choice([0.0, 1e-4]), rng.uniform(0, 1)
+        q1, q2 = unit(rng.standard_normal(4)), unit(rng.standard_normal(4))
+        if rng.random() < 0.3:   # small relative rotation (limits near their bounds)
+            q2 = unit(q1 + 0.2 * rng.standard_normal(4))
+        p1[i] = [*rng.uniform(-1, 1, 3), *q1]
+        p2[i] = [*rng.uniform(-1, 1, 3), *q2]
+        hs[i] = rng.choice([1.0 / 1200.0, 0.01, 0.1])
+        r = orc.ref_joint_errors(d[i], p1[i], p2[i], hs[i])
+        res[i, :16] = np.concatenate([r["direction"], r["magnitude"][:, None]], axis=1).ravel()
+        res[i, 16], res[i, 17] = r["current_position"], r["target_position"]
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), desc=d, pose1=p1, pose2=p2, h=hs, result=res)
+    print(name, "written", n)
+
+
 def api_scene_fixture(name):
     """Scenes of the reference's World tests (tests/rbps/API/test_world.cpp) run for their frame counts."""
     out = {}
@@ -123,3 +166,6 @@ if __name__ == "__main__":
     world_fixture("world_pile_mixed_dense", scenes.pile_mixed_dense(), [1, 10])
     narrowphase_fixture("narrowphase_pairs")
     api_scene_fixture("world_api_scenes")
+    joint_errors_fixture("joint_errors")
+    world_fixture("world_joint_zoo", scenes.joint_zoo(spherical=False), [1, 10, 60])
+    world_fixture("world_planar_joints", scenes.planar_joints(), [1, 30, 120])
