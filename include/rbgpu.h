@@ -38,7 +38,7 @@ enum { RBGPU_SPHERE = 0, RBGPU_BOX = 1, RBGPU_CAPSULE = 3, RBGPU_PLANE = 6, RBGP
 /* Fields of rbps::BodyParams (include/rbps/API/BodyAPI.hpp:22-31). 168 bytes. */
 typedef struct {
     int32_t type;                /* RBGPU_STATIC / RBGPU_DYNAMIC */
-    int32_t _pad;
+    uint32_t scene;              /* independent scene of a batch (0 = single world); see below */
     double mass;
     double inertia[6];           /* local inertia tensor: xx yy zz xy xz yz */
     double position[3];
@@ -107,6 +107,12 @@ typedef struct rbgpu_world rbgpu_world;
 /* World(timestep, substeps) — include/rbps/API/World.hpp:87-106. */
 int rbgpu_world_create(double timestep, int substeps, rbgpu_world **out);
 int rbgpu_world_destroy(rbgpu_world *w);
+
+/* Batches: bodies carry a scene index (rbgpu_body_desc.scene).  Scenes never interact:
+ * the broadphase sorts endpoints by (scene, value), a plane pairs only with its own
+ * scene's colliders, joints must join bodies of one scene, and the colouring hashes a
+ * pair's index within its own scene, so every scene evolves exactly as it would alone
+ * in a world of its own, while every kernel runs once over all scenes. */
 
 /* World::create_body / rbps::create_body — include/rbps/API/World.hpp:113-116,
  * include/rbps/API/BodyAPI.hpp:53-80.  ids_out may be NULL. */

@@ -98,6 +98,44 @@ struct ColliderParams {
     scalar dynamic_friction = 0.5;
 };
 
+// Joints (include/rbps/constraints/Joint.hpp:25-47, include/rbps/API/JointAPI.hpp:22-68)
+enum JointType { PRISMATIC = 0, REVOLUTE = 1, FIXED = 2, SPHERICAL = 3 };   // SPHERICAL: extension
+enum JointActuationType { FREE = 0, POSITION = 1, SPEED = 2 };
+
+struct PrismaticJointParams {
+    size_t body_1 = 0, body_2 = 0;
+    vec3 moving_axis = vec3(1, 0, 0);
+    vec3 r_1 = vec3(0, 0, 0), r_2 = vec3(0, 0, 0);
+    JointActuationType actuation_type = FREE;
+    scalar compliance = 0.0, damping = 0.0;
+    bool limited = false;
+    scalar lower_limit = 0.0, upper_limit = 0.0;
+};
+struct RevoluteJointParams {
+    size_t body_1 = 0, body_2 = 0;
+    vec3 aligned_axis = vec3(1, 0, 0);
+    vec3 limit_axis = vec3(0, 1, 0);
+    vec3 r_1 = vec3(0, 0, 0), r_2 = vec3(0, 0, 0);
+    JointActuationType actuation_type = FREE;
+    scalar compliance = 0.0, damping = 0.0;
+    bool limited = false;
+    scalar lower_limit = 0.0, upper_limit = 0.0;
+};
+struct FixedJointParams {
+    size_t body_1 = 0, body_2 = 0;
+    vec3 r_1 = vec3(0, 0, 0), r_2 = vec3(0, 0, 0);
+};
+// Ball joint (no reference counterpart): attachment row with `compliance`; with `limited`,
+// the angle between the bodies' `swing_axis` stays below `swing_limit`.
+struct SphericalJointParams {
+    size_t body_1 = 0, body_2 = 0;
+    vec3 r_1 = vec3(0, 0, 0), r_2 = vec3(0, 0, 0);
+    vec3 swing_axis = vec3(0, 1, 0);
+    scalar compliance = 0.0;
+    bool limited = false;
+    scalar swing_limit = 0.0;
+};
+
 struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
@@ -175,6 +213,48 @@ struct World {
         return id;
     }
 
+    // World::create_prismatic_joint / create_revolute_joint / create_fixed_joint
+    // (include/rbps/API/World.hpp:141-156) and the spherical extension.
+    uint32_t create_prismatic_joint(const PrismaticJointParams &p) {
+        rbgpu_joint_desc d = joint_base(RBGPU_JOINT_PRISMATIC, p.body_1, p.body_2, p.r_1, p.r_2);
+        set3(d.axis, p.moving_axis);
+        d.actuation = p.actuation_type; d.compliance = p.compliance; d.damping = p.damping;
+        d.limited = p.limited ? 1 : 0; d.lower_limit = p.lower_limit; d.upper_limit = p.upper_limit;
+        return add_joint(d);
+    }
+    uint32_t create_revolute_joint(const RevoluteJointParams &p) {
+        rbgpu_joint_desc d = joint_base(RBGPU_JOINT_REVOLUTE, p.body_1, p.body_2, p.r_1, p.r_2);
+        set3(d.axis, p.aligned_axis);
+        set3(d.limit_axis, p.limit_axis);
+        d.actuation = p.actuation_type; d.compliance = p.compliance; d.damping = p.damping;
+        d.limited = p.limited ? 1 : 0; d.lower_limit = p.lower_limit; d.upper_limit = p.upper_limit;
+        return add_joint(d);
+    }
+    uint32_t create_fixed_joint(const FixedJointParams &p) {
+        return add_joint(joint_base(RBGPU_JOINT_FIXED, p.body_1, p.body_2, p.r_1, p.r_2));
+    }
+    uint32_t create_spherical_joint(const SphericalJointParams &p) {
+        rbgpu_joint_desc d = joint_base(RBGPU_JOINT_SPHERICAL, p.body_1, p.body_2, p.r_1, p.r_2);
+        set3(d.axis, p.swing_axis);
+        d.compliance = p.compliance;
+        d.limited = p.limited ? 1 : 0;
+        d.upper_limit = p.swing_limit;
+        return add_joint(d);
+    }
+    // Pipeline options (rbgpu_set_option): RBGPU_OPT_* of rbgpu.h.
+    void set_option(int option, int value) { check(rbgpu_set_option(h_, option, value)); }
+
+    // JointCollection::target_position / target_speed are public fields in the reference.
+    void set_joint_target(uint32_t joint_id, scalar target_position, scalar target_speed) {
+        check(rbgpu_set_joint_targets(h_, 1, &joint_id, &target_position, &target_speed));
+    }
+    // JointCollection::current_position of every joint after the last step.
+    std::vector<scalar> joint_positions() {
+        std::vector<scalar> cur(n_joints_);
+        if (n_joints_) check(rbgpu_get_joints(h_, cur.data(), nullptr, nullptr, nullptr));
+        return cur;
+    }
+
     // World::step (World.hpp:192-212): one frame of `substeps` substeps of timestep/substeps.
     void step() {
         if (timestep != ts_ || substeps != sub_) {
@@ -228,6 +308,24 @@ struct World {
     rbgpu_world *handle() { return h_; }
 
 private:
+    uint32_t n_joints_ = 0;
+    static void set3(double *o, const vec3 &v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+    static rbgpu_joint_desc joint_base(int type, size_t b1, size_t b2, const vec3 &r1, const vec3 &r2) {
+        rbgpu_joint_desc d{};
+        d.type = type;
+        d.actuation = RBGPU_ACT_FREE;
+        d.body_1 = (uint32_t)b1;
+        d.body_2 = (uint32_t)b2;
+        set3(d.r_1, r1);
+        set3(d.r_2, r2);
+        return d;
+    }
+    uint32_t add_joint(const rbgpu_joint_desc &d) {
+        uint32_t id = 0;
+        check(rbgpu_add_joints(h_, 1, &d, &id));
+        ++n_joints_;
+        return id;
+    }
     void create() {
         check(rbgpu_world_create(timestep, substeps, &h_));
         ts_ = timestep;

@@ -12,7 +12,8 @@
 extern "C" {
 
 typedef struct {
-    int32_t type, _pad;
+    int32_t type;
+    uint32_t scene;   // independent scene of a batch
     double mass;
     double inertia[6];   // xx yy zz xy xz yz
     double position[3];
@@ -88,7 +89,7 @@ template <class T> struct Impl : Any {
         w.add_body(d.type, T(d.mass), I, orc::V3<T>(T(d.position[0]), T(d.position[1]), T(d.position[2])),
                    orc::Q<T>(T(d.orientation[0]), T(d.orientation[1]), T(d.orientation[2]), T(d.orientation[3])),
                    orc::V3<T>(T(d.linear_velocity[0]), T(d.linear_velocity[1]), T(d.linear_velocity[2])),
-                   orc::V3<T>(T(d.angular_velocity[0]), T(d.angular_velocity[1]), T(d.angular_velocity[2])));
+                   orc::V3<T>(T(d.angular_velocity[0]), T(d.angular_velocity[1]), T(d.angular_velocity[2])), d.scene);
     }
     void add_collider(const orc_collider_desc &d) override {
         orc::ShapeP<T> s;

@@ -51,6 +51,7 @@ template <class T> struct World {
 
     // ---- bodies (Body.hpp:52-69)
     std::vector<int> btype;   // 0 STATIC, 1 DYNAMIC
+    std::vector<uint32_t> bscene;   // independent scene of a batch (0 for a single world)
     std::vector<T> mass, inv_mass;
     std::vector<V3<T>> force, torque, pos, lv, av, ppos;
     std::vector<Q<T>> q, pq;
@@ -65,7 +66,8 @@ template <class T> struct World {
     std::vector<T> rest, sfr, dfr;
 
     // ---- broadphase (BroadPhase.hpp:78-125)
-    struct Obj { AABB<T> fat, tight; uint32_t uid; bool alive; };
+    struct Obj { AABB<T> fat, tight; uint32_t uid; bool alive; uint32_t scene; };
+    bool batched = false;   // some body has scene > 0
     struct EP { float v; uint32_t slot; uint32_t is_max; };
     std::vector<Obj> objs;
     std::vector<EP> eps;
@@ -118,7 +120,9 @@ template <class T> struct World {
 
     // ------------------------------------------------------------ creation
     // BodyAPI.hpp:53-80
-    uint32_t add_body(int type, T m, const SM<T> &inertia, V3<T> p, Q<T> o, V3<T> v, V3<T> w) {
+    uint32_t add_body(int type, T m, const SM<T> &inertia, V3<T> p, Q<T> o, V3<T> v, V3<T> w, uint32_t scene = 0) {
+        bscene.push_back(scene);
+        if (scene) batched = true;
         uint32_t id = (uint32_t)btype.size();
         T im = (type == 0) ? T(0) : T(1) / m;
         SM<T> Ii = (type == 0) ? SM<T>() : inertia.inverse();
@@ -151,6 +155,7 @@ template <class T> struct World {
             V3<T> m(fat_margin);
             o.fat = {tight.mn - m, tight.mx + m};
             o.uid = id;
+            o.scene = bscene[body];
             o.alive = true;
             objs.push_back(o);
             ep_at[0].push_back((uint32_t)eps.size());
@@ -216,8 +221,18 @@ template <class T> struct World {
         }
     }
     // BroadPhase.cpp:80-94 (a stable sort on the FP32 key; equal keys keep their previous order)
+    // Batches sort by (scene, value): a scene's endpoints are contiguous, so the canonical
+    // sweep (min endpoint to its own max) never leaves the scene.  Ties keep the previous
+    // frame's order in both keys.
     void bp_sort() {
-        std::stable_sort(eps.begin(), eps.end(), [](const EP &a, const EP &b) { return a.v < b.v; });
+        if (!batched) {
+            std::stable_sort(eps.begin(), eps.end(), [](const EP &a, const EP &b) { return a.v < b.v; });
+        } else {
+            std::stable_sort(eps.begin(), eps.end(), [&](const EP &a, const EP &b) {
+                const uint32_t sa = objs[a.slot].scene, sb = objs[b.slot].scene;
+                return sa < sb || (sa == sb && a.v < b.v);
+            });
+        }
         for (uint32_t k = 0; k < eps.size(); ++k) ep_at[eps[k].is_max][eps[k].slot] = k;
     }
     static std::pair<uint32_t, uint32_t> ordered(uint32_t a, uint32_t b) {
@@ -235,7 +250,7 @@ template <class T> struct World {
                     const Obj &ot = objs[s];
                     if (!ot.alive) continue;
                     if (!overlap(o.fat, ot.fat)) continue;
-                    sap_pairs.push_back(ordered(o.uid, ot.uid));
+                    if (o.scene == ot.scene) sap_pairs.push_back(ordered(o.uid, ot.uid));
                 }
                 active.push_back(e.slot);
             } else {
@@ -317,19 +332,37 @@ template <class T> struct World {
             }
         }
     }
-    // Frame pair list for the colour order: canonical SAP pairs then the plane loop.
+    // Frame pair list for the colour order: canonical SAP pairs then the plane loop (a
+    // plane pairs with the colliders of its own scene).  lid = the pair's index in its
+    // scene's own list (its SAP pairs, then its plane pairs): the colouring hashes lid, so
+    // each scene of a batch is coloured exactly as it would be alone.
+    std::vector<uint32_t> lid;
     void build_pairs() {
         spairs.clear();
+        lid.clear();
+        std::vector<uint32_t> nsap(bscene.empty() ? 1 : *std::max_element(bscene.begin(), bscene.end()) + 1, 0);
+        std::vector<uint32_t> nplane(nsap.size(), 0);
         for (auto &pr : sap_pairs)
-            if (pair_ok(pr.first, pr.second)) spairs.push_back({pr.first, pr.second, cbody[pr.first], cbody[pr.second]});
+            if (pair_ok(pr.first, pr.second)) {
+                const uint32_t sc = bscene[cbody[pr.first]];
+                spairs.push_back({pr.first, pr.second, cbody[pr.first], cbody[pr.second]});
+                lid.push_back(nsap[sc]++);
+            }
+        std::vector<uint32_t> plane_sc;
+        std::vector<std::vector<uint32_t>> by_scene(nsap.size());
+        for (uint32_t c = 0; c < shape.size(); ++c) by_scene[bscene[cbody[c]]].push_back(c);
         for (uint32_t pi = 0; pi < shape.size(); ++pi) {
             if (shape[pi].kind != SH_PLANE) continue;
-            for (uint32_t di = 0; di < shape.size(); ++di) {
+            for (uint32_t di : by_scene[bscene[cbody[pi]]]) {
                 if (di == pi || shape[di].kind == SH_PLANE) continue;
                 if (!pair_ok(pi, di)) continue;
+                const uint32_t sc = bscene[cbody[di]];
                 spairs.push_back({di, pi, cbody[di], cbody[pi]});
+                lid.push_back(nplane[sc]++);
+                plane_sc.push_back(sc);
             }
         }
+        for (size_t k = 0; k < plane_sc.size(); ++k) lid[lid.size() - plane_sc.size() + k] += nsap[plane_sc[k]];
         const uint32_t P = (uint32_t)spairs.size();
         colour.assign(P, -1);
         pcount.assign(P, 0);
@@ -381,7 +414,7 @@ template <class T> struct World {
                 uint32_t k = 0;
                 for (uint32_t b : bs)
                     if (dyn(b)) k += ucnt[b];
-                uint32_t idx = tent_hash(p, r) % palette(k), t = MAX_COLOURS;
+                uint32_t idx = tent_hash(lid[p], r) % palette(k), t = MAX_COLOURS;
                 for (uint32_t w = 0; w < W && t == MAX_COLOURS; ++w) {
                     uint32_t fr = ~0u;
                     for (uint32_t b : bs)
@@ -394,7 +427,7 @@ template <class T> struct World {
                 }
                 if (t == MAX_COLOURS) throw std::runtime_error("more than 1024 colours needed");
                 tent[p] = t;
-                key[p] = ((uint64_t)(r + 1) << 32) | rank_hash(p, r);
+                key[p] = ((uint64_t)(r + 1) << 32) | rank_hash(lid[p], r);
                 for (uint32_t b : bs)
                     if (dyn(b)) {
                         uint64_t &c = claim[(size_t)b * COLOUR_SLOTS + t % COLOUR_SLOTS];

@@ -79,16 +79,26 @@ __global__ void k_refit(Colliders C, Bodies B, uint32_t M, float dt, float margi
     *dirty = 1;
 }
 
-// Sorted sweep rows and endpoint positions after a sort.
+// Scene of each endpoint (key of the batch sort passes that follow the value passes).
+__global__ void k_ep_scene_key(const uint32_t *ep_val, const uint32_t *slot_scene, uint32_t nep, uint32_t *key,
+                               const int *dirty) {
+    if (*dirty == 0) return;
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nep) key[j] = slot_scene[ep_val[j] >> 1];
+}
+
+// Sorted sweep rows, endpoint positions and endpoint keys after a sort (the batch sort
+// reorders values without their keys; a key is float_key of its fat-box bound).
 // row (float4): min endpoint -> (x_max, y_min, y_max, z_min) and zmx separately.
 __global__ void k_sweep_prep(const uint32_t *ep_val, uint32_t nep, const float4 *fmin, const float4 *fmax, uint2 *ep_pos,
-                             uint32_t *sw_meta, float4 *sw_a, float2 *sw_b, const int *dirty) {
+                             uint32_t *sw_meta, float4 *sw_a, float2 *sw_b, uint32_t *ep_key, const int *dirty) {
     if (*dirty == 0) return;
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nep) return;
     uint32_t v = ep_val[j];
     uint32_t slot = v >> 1, is_max = v & 1u;
     if (is_max) ep_pos[slot].y = j; else ep_pos[slot].x = j;
+    ep_key[j] = float_key(is_max ? fmax[slot].x : fmin[slot].x);
     sw_meta[j] = v;
     float4 a = fmin[slot], b = fmax[slot];
     sw_a[j] = make_float4(a.x, b.x, a.y, b.y);
@@ -188,6 +198,27 @@ __global__ void k_pair_gather(const uint2 *pairs, const uint32_t *n_pairs, Colli
     bin_val[p] = p;
 }
 
+// Local pair ids (a pair's index in its scene's own list: SAP pairs, then plane pairs).
+// SAP pairs are scene-major (the batch sort makes every scene's endpoints contiguous), so
+// a scene's SAP pairs form one run [start, end).
+__global__ void k_scene_bounds(const uint2 *pairs, const uint32_t *n_sap_dev, const uint32_t *cscene, uint32_t *sap_start,
+                               uint32_t *sap_end) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *n_sap_dev;
+    if (p >= n) return;
+    const uint32_t s = cscene[pairs[p].x];
+    if (p == 0 || cscene[pairs[p - 1].x] != s) sap_start[s] = p;
+    if (p + 1 == n || cscene[pairs[p + 1].x] != s) sap_end[s] = p + 1;
+}
+__global__ void k_pair_lid(const uint2 *pairs, const uint32_t *n_sap_dev, const uint32_t *n_pairs_dev, const uint32_t *cscene,
+                           const uint32_t *sap_start, const uint32_t *sap_end, const uint32_t *plane_lid, uint32_t *lid) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ns = *n_sap_dev;
+    if (p >= *n_pairs_dev) return;
+    const uint32_t s = cscene[pairs[p].x];
+    lid[p] = p < ns ? p - sap_start[s] : (sap_end[s] - sap_start[s]) + plane_lid[p - ns];
+}
+
 // ------------------------------------------------------------------ substep colouring of active pairs
 // Pairs with contact points are coloured every substep; two pairs conflict when they share a
 // dynamic body.  Synchronous randomized rounds whose cost does not depend on body degree.
@@ -202,7 +233,9 @@ __global__ void k_pair_gather(const uint2 *pairs, const uint32_t *n_pairs, Colli
 // used(v) keeps later rounds off taken colours: the colouring is valid.  The largest key
 // always settles, so every round makes progress.  Keys carry the round, so claims of earlier
 // rounds lose without clearing.  Outcomes depend only on the keys, not on thread or list
-// order, so work lists may be built with atomics.  Arrays are indexed by pair id.
+// order, so work lists may be built with atomics.  Arrays are indexed by pair id; the
+// hashes take the pair's local id (its index in its own scene's list), so every scene of
+// a batch is coloured exactly as it would be alone.
 constexpr uint32_t COLOUR_SLOTS = 16;
 constexpr uint32_t COLOUR_WORDS = MAX_COLOURS / 32;
 constexpr uint32_t MAX_HEAD_ROUNDS = 256;
@@ -231,6 +264,7 @@ struct ColourBufs {
     unsigned long long *claim;  // N x COLOUR_SLOTS
     uint32_t *hist, *off, *cur; // per colour: size, start (MAX_COLOURS + 1), scatter cursor
     uint32_t *ctr;              // CTR_*
+    const uint32_t *lid;        // local pair id per pair (the hashes' input)
     uint32_t pal_max;           // largest palette a pair draws its tentative colour from
 };
 
@@ -312,7 +346,8 @@ __device__ __forceinline__ void colour_claim(const ColourBufs &cb, const uint2 *
     const uint2 bb = pbod[p];
     const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
     const uint32_t k = (da ? ld_shared(&cb.ucnt[bb.x]) : 0u) + (db ? ld_shared(&cb.ucnt[bb.y]) : 0u);
-    uint32_t idx = tent_hash(p, r) % palette(k, cb.pal_max);
+    const uint32_t q = cb.lid[p];
+    uint32_t idx = tent_hash(q, r) % palette(k, cb.pal_max);
     uint32_t t = MAX_COLOURS;
     for (uint32_t w = 0; w < COLOUR_WORDS; ++w) {
         uint32_t fr = ~((da ? ld_shared(&cb.used[bb.x * COLOUR_WORDS + w]) : 0u) |
@@ -327,7 +362,7 @@ __device__ __forceinline__ void colour_claim(const ColourBufs &cb, const uint2 *
     }
     if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&cb.ctr[CTR_ERR], 1u); t = MAX_COLOURS - 1; }
     cb.tent[p] = t;
-    const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(p, r);
+    const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(q, r);
     if (da) atomicMax(&cb.claim[bb.x * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
     if (db) atomicMax(&cb.claim[bb.y * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
 }
@@ -337,7 +372,7 @@ __device__ __forceinline__ bool colour_settle(const ColourBufs &cb, const uint2 
     const uint2 bb = pbod[p];
     const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
     const uint32_t t = cb.tent[p];
-    const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(p, r);
+    const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(cb.lid[p], r);
     if (da && ld_shared(&cb.claim[bb.x * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) return false;
     if (db && ld_shared(&cb.claim[bb.y * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) return false;
     cb.acol[p] = (int)t;

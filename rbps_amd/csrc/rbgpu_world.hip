@@ -89,6 +89,9 @@ struct rbgpu_world {
     bool built = false, sorted_once = false;
 
     uint32_t N = 0, M = 0, S = 0, NEP = 0, NJ = 0;
+    uint32_t nscenes = 1;               // independent scenes (batch) in this world
+    uint32_t *cscene = nullptr, *slot_scene = nullptr, *sap_start = nullptr, *sap_end = nullptr;
+    uint32_t *plane_lid = nullptr, *pair_lid = nullptr;
     Joints Jt{};
     uint32_t *jorder = nullptr;         // joints by colour
     std::vector<uint32_t> jcol_off;     // joint colour offsets
@@ -142,7 +145,7 @@ struct rbgpu_world {
 
     void release_pairs() {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
-        dfree(solve_list); dfree(bin_list); dfree(p_hist);
+        dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
@@ -161,6 +164,7 @@ struct rbgpu_world {
         dfree(flags); dfree(scal); dfree(tile_sums);
         dfree(cb.used); dfree(cb.ucnt); dfree(cb.claim); dfree(cb.hist); dfree(cb.off); dfree(cb.cur); dfree(cb.ctr);
         dfree(plane_pairs); dfree(offs); dfree(d_points);
+        dfree(cscene); dfree(slot_scene); dfree(sap_start); dfree(sap_end); dfree(plane_lid);
         for (auto &e : ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         ev_pool.clear();
         if (rb) (void)hipHostFree(rb);
@@ -232,7 +236,7 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     int rc = 0;
     rc |= dalloc(pairs, cap);
     rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
-    rc |= dalloc(bin_list, cap);
+    rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap);
     rc |= dalloc(cb.acol, cap); rc |= dalloc(cb.tent, cap); rc |= dalloc(cb.rest, cap);
     uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
     rc |= dalloc(p_hist, (size_t)nt * RS_BINS);
@@ -283,14 +287,31 @@ int rbgpu_world::build() {
         lrot[c] = make_float4((float)d.local_rot[0], (float)d.local_rot[1], (float)d.local_rot[2], (float)d.local_rot[3]);
         mat[c] = make_float4((float)d.restitution, (float)d.static_friction, (float)d.dynamic_friction, 0.0f);
     }
-    // Plane loop of the narrowphase driver (CollisionPipeline.cpp:156-197): constant per scene.
-    for (uint32_t pi = 0; pi < m; ++pi) {
-        if (hc[pi].shape != RBGPU_PLANE) continue;
-        for (uint32_t di = 0; di < m; ++di) {
-            if (di == pi || hc[di].shape == RBGPU_PLANE) continue;
-            if (hc[pi].is_static && hc[di].is_static) continue;
-            if (hc[pi].body_id == hc[di].body_id) continue;
-            planes.push_back(make_uint2(di, pi));
+    // Plane loop of the narrowphase driver (CollisionPipeline.cpp:156-197): constant per scene;
+    // a plane pairs with the colliders of its own scene.  plane_lid: index of the pair among
+    // its scene's plane pairs.
+    std::vector<uint32_t> csc(m), ssc(S), plids;
+    uint32_t nsc = 1;
+    for (uint32_t c = 0; c < m; ++c) {
+        csc[c] = hb[hc[c].body_id].scene;
+        nsc = std::max(nsc, csc[c] + 1);
+        if ((uint32_t)meta[c].w != BP_INVALID) ssc[(uint32_t)meta[c].w] = csc[c];
+    }
+    for (const rbgpu_body_desc &d : hb) nsc = std::max(nsc, d.scene + 1);
+    if (nsc > 65536) return fail(RBGPU_ERR_CAPACITY, "more than 65536 scenes in one world");
+    {
+        std::vector<std::vector<uint32_t>> by_scene(nsc);
+        for (uint32_t c = 0; c < m; ++c) by_scene[csc[c]].push_back(c);
+        std::vector<uint32_t> nplane(nsc, 0);
+        for (uint32_t pi = 0; pi < m; ++pi) {
+            if (hc[pi].shape != RBGPU_PLANE) continue;
+            for (uint32_t di : by_scene[csc[pi]]) {
+                if (di == pi || hc[di].shape == RBGPU_PLANE) continue;
+                if (hc[pi].is_static && hc[di].is_static) continue;
+                if (hc[pi].body_id == hc[di].body_id) continue;
+                planes.push_back(make_uint2(di, pi));
+                plids.push_back(nplane[csc[pi]]++);
+            }
         }
     }
     if (slot != S) return fail(RBGPU_ERR_INVALID, "broadphase slot bookkeeping mismatch");
@@ -315,6 +336,8 @@ int rbgpu_world::build() {
     rc |= dalloc(cb.claim, (size_t)n * COLOUR_SLOTS); rc |= dalloc(cb.hist, MAX_COLOURS);
     rc |= dalloc(cb.off, MAX_COLOURS + 1); rc |= dalloc(cb.cur, MAX_COLOURS); rc |= dalloc(cb.ctr, CTR_SIZE);
     rc |= dalloc(plane_pairs, planes.size()); rc |= dalloc(offs, NBINS + 4);
+    rc |= dalloc(cscene, m); rc |= dalloc(slot_scene, S); rc |= dalloc(sap_start, nsc); rc |= dalloc(sap_end, nsc);
+    rc |= dalloc(plane_lid, planes.size());
     rc |= dalloc(d_points, 1);
     if (rc) return RBGPU_ERR_OOM;
     auto up = [&](void *dst, const void *src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st); };
@@ -335,6 +358,10 @@ int rbgpu_world::build() {
     }
     n_plane = (uint32_t)planes.size();
     if (n_plane) HIPCHK(up(plane_pairs, planes.data(), n_plane * 8));
+    if (n_plane) HIPCHK(up(plane_lid, plids.data(), n_plane * 4));
+    nscenes = nsc;
+    if (m) HIPCHK(up(cscene, csc.data(), m * 4));
+    if (S) HIPCHK(up(slot_scene, ssc.data(), S * 4));
     int rcj = build_joints();
     if (rcj) return rcj;
     int init_flags[4] = {1, 0, 0, 0};   // dirty: the endpoint array has never been sorted
@@ -461,7 +488,13 @@ int rbgpu_world::frame() {
             int rc = radix_pass(ep_key[s], ep_val[s], ep_key[s ^ 1], ep_val[s ^ 1], NEP, 8 * pass, ep_hist, dirty);
             if (rc) return rc;
         }
-        k_sweep_prep<<<nblk(NEP), TPB, 0, st>>>(ep_val[0], NEP, fmin, fmax, ep_pos, sw_meta, sw_a, sw_b, dirty);
+        if (nscenes > 1) {   // then by scene (stable): (scene, value) order, scenes contiguous
+            k_ep_scene_key<<<nblk(NEP), TPB, 0, st>>>(ep_val[0], slot_scene, NEP, ep_cnt, dirty);
+            int rc = radix_pass(ep_cnt, ep_val[0], ep_key[1], ep_val[1], NEP, 0, ep_hist, dirty);
+            if (!rc) rc = radix_pass(ep_key[1], ep_val[1], ep_cnt, ep_val[0], NEP, 8, ep_hist, dirty);
+            if (rc) return rc;
+        }
+        k_sweep_prep<<<nblk(NEP), TPB, 0, st>>>(ep_val[0], NEP, fmin, fmax, ep_pos, sw_meta, sw_a, sw_b, ep_key[0], dirty);
         // 3. sweep: count, scan, readback of the pair count
         k_sweep<false><<<nblk(NEP), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, NEP, ep_cnt, nullptr, nullptr, 0,
                                                   overflow);
@@ -482,6 +515,11 @@ int rbgpu_world::frame() {
         k_append<<<nblk(std::max<uint32_t>(n_plane, 1)), TPB, 0, st>>>(pairs, scal + 0, plane_pairs, n_plane, pcap,
                                                                         scal + 1, overflow);
         k_pair_gather<<<nblk(pcap), TPB, 0, st>>>(pairs, scal + 1, C, K, ka, va);
+        // local pair ids (colouring hash input): a scene's SAP pairs are one run
+        HIPCHK(hipMemsetAsync(sap_start, 0, nscenes * 4, st));
+        HIPCHK(hipMemsetAsync(sap_end, 0, nscenes * 4, st));
+        k_scene_bounds<<<nblk(pcap), TPB, 0, st>>>(pairs, scal + 0, cscene, sap_start, sap_end);
+        k_pair_lid<<<nblk(pcap), TPB, 0, st>>>(pairs, scal + 0, scal + 1, cscene, sap_start, sap_end, plane_lid, pair_lid);
         int rc = radix_pass(ka, va, kb, bin_list, pcap, 0, p_hist, nullptr, scal + 1);
         if (rc) return rc;
         k_offsets<<<1, 64, 0, st>>>(p_hist, (pcap + RS_TILE - 1) / RS_TILE, NBINS, scal + 1, offs);
@@ -503,6 +541,7 @@ int rbgpu_world::frame() {
 
     // 4. substeps (World.hpp:203-211)
     cb.pal_max = palette_max;
+    cb.lid = pair_lid;
     for (int s = 0; s < substeps; ++s) {
         runs.clear();
         if (P) {
@@ -706,6 +745,7 @@ int rbgpu_add_joints(rbgpu_world *w, int n, const rbgpu_joint_desc *desc, uint32
         if (d.actuation < RBGPU_ACT_FREE || d.actuation > RBGPU_ACT_SPEED) return fail(RBGPU_ERR_INVALID, "unknown actuation");
         if (d.body_1 >= w->hb.size() || d.body_2 >= w->hb.size()) return fail(RBGPU_ERR_INVALID, "joint body out of range");
         if (d.body_1 == d.body_2) return fail(RBGPU_ERR_INVALID, "joint needs two different bodies");
+        if (w->hb[d.body_1].scene != w->hb[d.body_2].scene) return fail(RBGPU_ERR_INVALID, "joint across two scenes");
     }
     int rc = w->download_state();
     if (rc) return rc;

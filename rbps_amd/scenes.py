@@ -23,7 +23,7 @@ ACT_FREE, ACT_POSITION, ACT_SPEED = 0, 1, 2            # Joint.hpp JointActuatio
 SPHERE, BOX, CAPSULE, PLANE, HULL = 0, 1, 3, 6, 9
 
 BODY_DTYPE = np.dtype([
-    ("type", "<i4"), ("_pad", "<i4"), ("mass", "<f8"), ("inertia", "<f8", (6,)),
+    ("type", "<i4"), ("scene", "<u4"), ("mass", "<f8"), ("inertia", "<f8", (6,)),
     ("position", "<f8", (3,)), ("orientation", "<f8", (4,)),
     ("linear_velocity", "<f8", (3,)), ("angular_velocity", "<f8", (3,)),
 ], align=True)
@@ -397,3 +397,37 @@ def planar_joints(seed=5) -> Scene:
                     (-3.2 + 0.6 * k, 1.5 + 0.5 * k + 0.1 * rng.random(), 0.0))
         sb.collider(SPHERE, b, (rr,), restitution=0.0)
     return sb.build("planar_joints", 1.0 / 60.0, 20, 120, seed=seed)
+
+
+def batch(parts, name="batch") -> Scene:
+    """Independent scenes as one batched world: every body carries its scene index;
+    colliders and joints are re-indexed.  Scenes may overlap in space, they never
+    interact.  Timestep and substeps come from the first scene."""
+    bodies, cols, joints = [], [], []
+    nb = 0
+    for s, sc in enumerate(parts):
+        b = sc.bodies.copy()
+        b["scene"] = s
+        c = sc.colliders.copy()
+        c["body_id"] += nb
+        j = sc.joints.copy()
+        j["body_1"] += nb
+        j["body_2"] += nb
+        bodies.append(b)
+        cols.append(c)
+        joints.append(j)
+        nb += len(b)
+    first = parts[0]
+    return Scene(name, np.concatenate(bodies), np.concatenate(cols), first.timestep, first.substeps, first.frames,
+                 {"scenes": len(parts), "bodies_per_scene": [len(p.bodies) for p in parts]},
+                 np.concatenate(joints) if joints else np.zeros(0, JOINT_DTYPE))
+
+
+def batched_stack_chains(n_scenes=4096, seed=77, n_min=32, n_max=512) -> Scene:
+    """The batched workload: n_scenes independent scenes with body counts uniform in
+    [n_min, n_max], each a seeded box stack plus a 16-link capsule chain on spherical
+    joints (compliance 1e-6) over its own ground plane."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(n_min, n_max + 1, n_scenes)
+    parts = [stack_chain(int(n), seed=seed + 1 + s) for s, n in enumerate(counts)]
+    return batch(parts, f"batched_stack_chains_{n_scenes}")

@@ -123,6 +123,62 @@ int main() {
         }
         CHECK(thrown);
     }
+    {
+        // the reference demo's pendulum (src/visr_demo/main.cpp:168-185) on the XPBD joint mode
+        std::printf("revolute_pendulum_demo\n");
+        World w;
+        w.set_option(RBGPU_OPT_JOINT_MODE, 1);
+        BodyParams pp;
+        pp.type = STATIC;
+        pp.mass = 0.0;
+        pp.position = vec3(0.0, 4.0, 0.0);
+        uint32_t pivot = w.create_body(pp);
+        uint32_t bob = add_dynamic_sphere(w, vec3(1.8, 4.0, 0.0), 0.3, 20.0);
+        RevoluteJointParams rj;
+        rj.body_1 = pivot;
+        rj.body_2 = bob;
+        rj.aligned_axis = vec3(0, 0, 1);
+        rj.limit_axis = vec3(1, 0, 0);
+        rj.r_2 = vec3(-1.8, 0, 0);
+        w.create_revolute_joint(rj);
+        scalar lowest = 10.0, worst = 0.0;
+        for (int f = 0; f < 60; ++f) {
+            w.step();
+            vec3 p = w.bodies.position[w.bodies.index_of(bob)];
+            vec3 d(p.x, p.y - 4.0, p.z);
+            worst = std::fmax(worst, std::fabs(mag(d) - 1.8));
+            lowest = std::fmin(lowest, p.y);
+        }
+        CHECK(worst < 0.02);
+        CHECK(lowest < 2.5);
+        CHECK(w.joint_positions().size() == 1);
+    }
+    {
+        std::printf("spherical_chain\n");
+        World w;
+        w.set_option(RBGPU_OPT_JOINT_MODE, 1);
+        BodyParams ap;
+        ap.type = STATIC;
+        ap.mass = 0.0;
+        ap.position = vec3(0.0, 5.0, 0.0);
+        uint32_t prev = w.create_body(ap);
+        vec3 tip(0.0, 0.0, 0.0);
+        for (int k = 0; k < 4; ++k) {
+            uint32_t b = add_dynamic_sphere(w, vec3(0.6 * (k + 1), 5.0, 0.0), 0.1);
+            SphericalJointParams sj;
+            sj.body_1 = prev;
+            sj.body_2 = b;
+            sj.r_1 = tip;
+            sj.r_2 = vec3(-0.3, 0.0, 0.0);
+            w.create_spherical_joint(sj);
+            prev = b;
+            tip = vec3(0.3, 0.0, 0.0);
+        }
+        for (int f = 0; f < 60; ++f) w.step();
+        vec3 end = w.bodies.position[w.bodies.index_of(prev)];
+        CHECK(end.y < 4.0);
+        CHECK(std::sqrt(end.x * end.x + (end.y - 5.0) * (end.y - 5.0) + end.z * end.z) < 2.45);
+    }
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "ALL PASSED", failures);
     return failures ? 1 : 0;
 }
