@@ -17,6 +17,8 @@ Also reported:
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -29,12 +31,18 @@ METRIC = "body-substeps/s on 1 MI355X (learned+batched vs plain), pos err \u2264
 WORKLOAD = "plain GPU pipeline, large pile: 65,536 mixed convex bodies, 10 substeps, 500 steps"
 
 
-def cpu_baseline(scene_fn, budget_s=10.0, max_frames=30):
+def _cpu_world(sc):
+    """The reference engine where it can build the scene, else the oracle restatement in
+    the reference's order (FP64); the reference has no spherical joint."""
     from oracle import orc
-    sc = scene_fn()
-    kind = "reference" if orc.ref_available() else "port"
+    kind = "reference" if orc.ref_available() and not (sc.joints["type"] == 3).any() else "port"
     w = orc.RefWorld(sc.timestep, sc.substeps) if kind == "reference" else orc.OracleWorld(sc.timestep, sc.substeps)
-    w.load(sc)
+    return w.load(sc), kind
+
+
+def cpu_baseline(scene_fn, budget_s=10.0, max_frames=30):
+    sc = scene_fn()
+    w, kind = _cpu_world(sc)
     t0 = time.perf_counter()
     frames = 0
     while frames < max_frames:
@@ -45,6 +53,27 @@ def cpu_baseline(scene_fn, budget_s=10.0, max_frames=30):
     dt = time.perf_counter() - t0
     return {"value": sc.n_dynamic * sc.substeps * frames / dt, "unit": "body-substeps/s", "cores": 1, "kind": kind,
             "sample": f"frames 1..{frames} of the same {sc.n_dynamic}-body scene ({dt:.1f} s, FP64, serial)"}
+
+
+def cpu_baseline_batched(budget_s=10.0, frames=10):
+    """The reference runs one World per scene: scenes of the same batch (same seeds),
+    each for `frames` frames, one after another on one core, until the budget is spent."""
+    from rbps_amd import scenes
+    rng = np.random.default_rng(77)
+    counts = rng.integers(32, 513, 4096)
+    t0 = time.perf_counter()
+    work = done = 0
+    while time.perf_counter() - t0 < budget_s and done < len(counts):
+        sc = scenes.stack_chain(int(counts[done]), seed=77 + 1 + done)
+        w, kind = _cpu_world(sc)
+        w.step(frames)
+        work += sc.n_dynamic * sc.substeps * frames
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": work / dt, "unit": "body-substeps/s", "cores": 1, "kind": kind,
+            "sample": f"scenes 0..{done - 1} of the batch, {frames} frames each, one World per scene "
+                      f"({dt:.1f} s incl. set-up, FP64, serial; the oracle restatement in the "
+                      f"reference's order because the reference has no spherical joint)"}
 
 
 def measured_traffic(kernel):
@@ -77,6 +106,9 @@ def main():
                     choices=["off", "solve_pos", "solve_vel", "narrowphase", "integrate", "update_vel", "broadphase", "colouring",
                              "solve_pos_runs", "solve_vel_runs"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="pile", choices=["pile", "batched"],
+                    help="pile: the 65,536-body mixed pile (default); batched: 4,096 independent "
+                         "box-stack + capsule-chain scenes (1.1M bodies) in one world")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB ten times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a --pmc run to bytes)")
@@ -90,9 +122,17 @@ def main():
         sys.exit(2)
 
     from rbps_amd import scenes, world
-    scene_fn = scenes.pile_mixed
+    if args.workload == "batched":
+        scene_fn = scenes.batched_stack_chains
+        workload = ("batched scenes: 4,096 independent scenes of 32-512 bodies (box stack + 16-capsule chain "
+                    "on spherical joints, compliance 1e-6), 10 substeps, XPBD joint mode")
+    else:
+        scene_fn = scenes.pile_mixed
+        workload = WORKLOAD
     sc = scene_fn()
     w = world.World(sc.timestep, sc.substeps).load(sc)
+    if args.workload == "batched":
+        w.set_option("joint_mode", 1)
     w.step(args.warmup)
     w.synchronize()
     # per-launch HIP events on every 5th frame: events between launches cost time themselves
@@ -127,12 +167,12 @@ def main():
         "value": round(value, 1), "unit": "body-substeps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": WORKLOAD, "scene": sc.name, "bodies": sc.n_dynamic, "substeps": sc.substeps, "timestep": sc.timestep,
+        "config": {"workload": workload, "scene": sc.name, "bodies": sc.n_dynamic, "substeps": sc.substeps, "timestep": sc.timestep,
                    "shapes": "box/sphere/capsule + static plane", "last_frame": st},
-        "roofline": roof,
+        "roofline": roof if args.workload == "pile" else dict(roof or {}, traffic=None),
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(scene_fn)
+        out["cpu_baseline"] = cpu_baseline_batched() if args.workload == "batched" else cpu_baseline(scene_fn)
     print(json.dumps(out))
 
 

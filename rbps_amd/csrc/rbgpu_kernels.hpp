@@ -243,7 +243,7 @@ constexpr uint32_t TAIL_THREADS = 1024;
 constexpr uint32_t LEFT_STRIPES = 64;   // the uncoloured count is striped over 64 words
 // ctr[]: counters of the colouring (cleared every substep)
 enum {
-    CTR_ACT = 0,      // active pairs (coloured pairs, written with the offsets)
+    CTR_ACT = 0,      // active pairs (the scan total of the per-block counts)
     CTR_REST = 1,     // rest list length
     CTR_ROUNDS = 2,   // rounds used
     CTR_NCOL = 3,     // colours used
@@ -256,6 +256,8 @@ constexpr int32_t COLOUR_NONE = -2;     // a pair without points this substep
 constexpr int32_t COLOUR_OPEN = -1;     // an active pair not coloured yet
 
 struct ColourBufs {
+    uint32_t *act;              // active pair ids, ascending (CTR_ACT of them)
+    uint32_t *blk;              // per-block active counts, then their exclusive scan
     int *acol;                  // colour per pair id: >= 0, COLOUR_OPEN or COLOUR_NONE
     uint32_t *tent;             // tentative colour per pair id
     uint32_t *rest;             // pair ids left to the single-workgroup rounds
@@ -296,18 +298,6 @@ __device__ __forceinline__ void wave_append(bool take, uint32_t val, uint32_t *c
     base = __shfl(base, leader, 64);
     if (take) list[base + lane_rank(m)] = val;
 }
-// Block-wide count of `v` (256 threads) added to one stripe of the uncoloured count.
-__device__ __forceinline__ void block_count_to_left(uint32_t *ctr, bool v, bool add, uint32_t *sh) {
-    if (threadIdx.x == 0) *sh = 0u;
-    __syncthreads();
-    const uint64_t m = __ballot(v);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(sh, (uint32_t)__popcll(m));
-    __syncthreads();
-    if (threadIdx.x == 0 && *sh) {
-        uint32_t *stripe = &ctr[CTR_LEFT + blockIdx.x % LEFT_STRIPES];
-        if (add) atomicAdd(stripe, *sh); else atomicSub(stripe, *sh);
-    }
-}
 // Sum of the stripes, by one wave (every lane gets it).
 __device__ __forceinline__ uint32_t uncoloured_left(const uint32_t *ctr) {
     uint32_t v = ld_shared(&ctr[CTR_LEFT + (threadIdx.x & 63)]);
@@ -326,7 +316,7 @@ __global__ void k_colour_reset(uint32_t N, ColourBufs cb) {
     if (i < (uint32_t)CTR_SIZE) cb.ctr[i] = 0u;
 }
 
-// Marks the pairs with points as open, counts them per dynamic body and in total.
+// Marks the pairs with points as open, counts them per dynamic body and per block.
 __global__ void __launch_bounds__(256) k_colour_activate(const int *cnt, uint32_t P, const uint2 *pbod, Bodies B,
                                                          ColourBufs cb) {
     __shared__ uint32_t sh;
@@ -338,7 +328,27 @@ __global__ void __launch_bounds__(256) k_colour_activate(const int *cnt, uint32_
         if (is_dynamic(B, bb.x)) atomicAdd(&cb.ucnt[bb.x], 1u);
         if (is_dynamic(B, bb.y)) atomicAdd(&cb.ucnt[bb.y], 1u);
     }
-    block_count_to_left(cb.ctr, a, true, &sh);
+    if (threadIdx.x == 0) sh = 0u;
+    __syncthreads();
+    const uint64_t m = __ballot(a);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&sh, (uint32_t)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) cb.blk[blockIdx.x] = sh;
+}
+// Active list in ascending pair order from the scanned block counts; the uncoloured
+// count starts at the total.
+__global__ void __launch_bounds__(256) k_colour_compact(uint32_t P, ColourBufs cb) {
+    __shared__ uint32_t wc[4];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool a = p < P && cb.acol[p] == COLOUR_OPEN;
+    const uint64_t m = __ballot(a);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wc[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = cb.blk[blockIdx.x];
+    for (int w = 0; w < wave; ++w) base += wc[w];
+    if (a) cb.act[base + lane_rank(m)] = p;
+    if (p == 0) cb.ctr[CTR_LEFT] = cb.ctr[CTR_ACT];
 }
 
 __device__ __forceinline__ void colour_claim(const ColourBufs &cb, const uint2 *pbod, const Bodies &B, uint32_t p,
@@ -382,29 +392,49 @@ __device__ __forceinline__ bool colour_settle(const ColourBufs &cb, const uint2 
     return true;
 }
 
-// Grid-wide round, claim phase, over all pairs.  Skipped once at most `tail_min` pairs are
-// uncoloured (the uncoloured count changes only in settle launches, so every block of one
-// claim launch decides alike).  Marks round r as run.
-__global__ void k_colour_claim(const uint2 *pbod, Bodies B, ColourBufs cb, uint32_t P, uint32_t r, uint32_t tail_min) {
+// Grid-wide round, claim phase, over the active list (grid-stride).  Skipped once at most
+// `tail_min` pairs are uncoloured (the uncoloured count changes only in settle launches,
+// so every block of one claim launch decides alike).  Marks round r as run.
+__global__ void __launch_bounds__(256) k_colour_claim(const uint2 *pbod, Bodies B, ColourBufs cb, uint32_t r,
+                                                      uint32_t tail_min) {
     if (r > 0 && uncoloured_left(cb.ctr) <= tail_min) return;
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = p < P && cb.acol[p] == COLOUR_OPEN;
-    if (__ballot(live) && (threadIdx.x & 63) == 0) cb.ctr[CTR_FLAGS + r] = 1u;
-    if (live) colour_claim(cb, pbod, B, p, r);
+    const uint32_t n = cb.ctr[CTR_ACT];
+    bool any = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = cb.act[i];
+        if (cb.acol[p] == COLOUR_OPEN) {
+            any = true;
+            colour_claim(cb, pbod, B, p, r);
+        }
+    }
+    if (__ballot(any) && (threadIdx.x & 63) == 0) cb.ctr[CTR_FLAGS + r] = 1u;
 }
 // Grid-wide round, settle phase (runs exactly when its claim phase had pairs).
-__global__ void __launch_bounds__(256) k_colour_settle(const uint2 *pbod, Bodies B, ColourBufs cb, uint32_t P, uint32_t r) {
+__global__ void __launch_bounds__(256) k_colour_settle(const uint2 *pbod, Bodies B, ColourBufs cb, uint32_t r) {
     __shared__ uint32_t sh;
     if (!cb.ctr[CTR_FLAGS + r]) return;   // the claim phase of round r was skipped or empty
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = p < P && cb.acol[p] == COLOUR_OPEN;
-    const bool ok = live && colour_settle(cb, pbod, B, p, r);
-    block_count_to_left(cb.ctr, ok, false, &sh);
+    const uint32_t n = cb.ctr[CTR_ACT];
+    uint32_t won = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = cb.act[i];
+        if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r)) ++won;
+    }
+    for (int o = 32; o > 0; o >>= 1) won += __shfl_xor(won, o, 64);
+    if (threadIdx.x == 0) sh = 0u;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && won) atomicAdd(&sh, won);
+    __syncthreads();
+    if (threadIdx.x == 0 && sh) atomicSub(&cb.ctr[CTR_LEFT + blockIdx.x % LEFT_STRIPES], sh);
 }
 // Collects the uncoloured pairs into the rest list (after the grid-wide rounds).
-__global__ void k_colour_collect(ColourBufs cb, uint32_t P) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    wave_append(p < P && cb.acol[p] == COLOUR_OPEN, p, &cb.ctr[CTR_REST], cb.rest);
+__global__ void k_colour_collect(ColourBufs cb) {
+    const uint32_t n = cb.ctr[CTR_ACT];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {   // every lane of a wave iterates alike
+        const uint32_t i = i0 + threadIdx.x;
+        const uint32_t p = i < n ? cb.act[i] : 0u;
+        wave_append(i < n && cb.acol[p] == COLOUR_OPEN, p, &cb.ctr[CTR_REST], cb.rest);
+    }
 }
 
 // The remaining rounds in one workgroup: the rest list is small after the grid-wide rounds,
@@ -450,12 +480,13 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_colour_tail(const uint2 *pbod,
 }
 
 // Colour sizes (LDS histogram per block, grid-stride over the pairs).
-__global__ void __launch_bounds__(256) k_colour_hist(ColourBufs cb, uint32_t P) {
+__global__ void __launch_bounds__(256) k_colour_hist(ColourBufs cb) {
     __shared__ uint32_t h[MAX_COLOURS];
     for (uint32_t c = threadIdx.x; c < (uint32_t)MAX_COLOURS; c += blockDim.x) h[c] = 0u;
     __syncthreads();
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
-        const int c = cb.acol[p];
+    const uint32_t n = cb.ctr[CTR_ACT];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c = cb.acol[cb.act[i]];
         if (c >= 0) atomicAdd(&h[c], 1u);
     }
     __syncthreads();
@@ -474,21 +505,22 @@ __global__ void __launch_bounds__(MAX_COLOURS) k_colour_offsets(ColourBufs cb) {
     const uint32_t o = block_exclusive_scan(v, sh, &tot);
     cb.off[c] = o;
     cb.cur[c] = o;
-    if (c == 0) { cb.off[MAX_COLOURS] = tot; cb.ctr[CTR_ACT] = tot; }
+    if (c == 0) cb.off[MAX_COLOURS] = tot;
     if (v) atomicMax(&top, c + 1u);
     __syncthreads();
     if (c == 0) cb.ctr[CTR_NCOL] = top;
 }
 // Solve list ordered by colour: each block counts its chunk per colour in LDS, reserves
 // ranges with one atomic per (block, colour), then scatters.
-__global__ void __launch_bounds__(256) k_colour_scatter(ColourBufs cb, uint32_t P, uint32_t *solve_list, uint32_t chunk) {
+__global__ void __launch_bounds__(256) k_colour_scatter(ColourBufs cb, uint32_t *solve_list, uint32_t chunk) {
     __shared__ uint32_t h[MAX_COLOURS];
     __shared__ uint32_t base[MAX_COLOURS];
     for (uint32_t c = threadIdx.x; c < (uint32_t)MAX_COLOURS; c += blockDim.x) h[c] = 0u;
     __syncthreads();
-    const uint32_t p0 = blockIdx.x * chunk, p1 = min(P, p0 + chunk);
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        const int c = cb.acol[p];
+    const uint32_t n = cb.ctr[CTR_ACT];
+    const uint32_t i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const int c = cb.acol[cb.act[i]];
         if (c >= 0) atomicAdd(&h[c], 1u);
     }
     __syncthreads();
@@ -497,7 +529,8 @@ __global__ void __launch_bounds__(256) k_colour_scatter(ColourBufs cb, uint32_t 
         h[c] = 0u;
     }
     __syncthreads();
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const uint32_t p = cb.act[i];
         const int c = cb.acol[p];
         if (c >= 0) solve_list[base[c] + atomicAdd(&h[c], 1u)] = p;
     }
@@ -552,6 +585,24 @@ __device__ __forceinline__ bool collide_bin(const Shape &a, const TF &ta, const 
     return hit;
 }
 
+// Bounding-sphere rejection ahead of the analytic routines whose separation test is exact
+// (box-box: the 15-axis SAT is complete for boxes; sphere-box, sphere-capsule, capsule-
+// capsule: closest points): when the bounding spheres are more than SEP_MARGIN apart the
+// shapes are disjoint, these routines report no contact, and the margin keeps the decision
+// far above rounding (and above box-box's 0.001 edge bias), so contact sets are unchanged.
+// Not for capsule-box: the reference tests only 6 SAT axes there (CapsuleBox.hpp:51-97) and
+// reports contacts for shapes several centimetres apart near box edges, which must be kept.
+constexpr float SEP_MARGIN = 1e-2f;
+__device__ __forceinline__ float bound_radius(const Shape &s) {
+    return s.kind == SH_SPHERE ? s.a : (s.kind == SH_BOX ? sqrtf(s.a * s.a + s.b * s.b + s.c * s.c) : s.a + s.b);
+}
+template <int BIN>
+__device__ __forceinline__ bool far_apart(const Shape &a, const TF &ta, const Shape &b, const TF &tb) {
+    if (BIN != BIN_BB && BIN != BIN_SB && BIN != BIN_SC && BIN != BIN_CC) return false;
+    const float r = bound_radius(a) + bound_radius(b) + SEP_MARGIN;
+    return len2(ta.pos - tb.pos) > r * r;
+}
+
 template <int BIN>
 __global__ void __launch_bounds__(128) k_narrow(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B,
                                                 Contacts K) {
@@ -563,6 +614,10 @@ __global__ void __launch_bounds__(128) k_narrow(const uint32_t *bin_slot, uint32
     int4 ma = C.meta[cc.x], mb = C.meta[cc.y];
     Shape a = shape_of(ma.x, C.dims[cc.x]), b = shape_of(mb.x, C.dims[cc.y]);
     TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+    if (far_apart<BIN>(a, ta, b, tb)) {
+        K.cnt[s] = 0;
+        return;
+    }
     Manifold m;
     m.np = 0;
     if (collide_bin<BIN>(a, ta, b, tb, m) && m.np > 0) emit(m, B, K, s, bb);

@@ -146,7 +146,7 @@ struct rbgpu_world {
     void release_pairs() {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid);
-        dfree(cb.acol); dfree(cb.tent); dfree(cb.rest);
+        dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
         pcap = 0;
@@ -237,7 +237,8 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     rc |= dalloc(pairs, cap);
     rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
     rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap);
-    rc |= dalloc(cb.acol, cap); rc |= dalloc(cb.tent, cap); rc |= dalloc(cb.rest, cap);
+    rc |= dalloc(cb.acol, cap); rc |= dalloc(cb.tent, cap); rc |= dalloc(cb.rest, cap); rc |= dalloc(cb.act, cap);
+    rc |= dalloc(cb.blk, nblk(cap) + 1);
     uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
     rc |= dalloc(p_hist, (size_t)nt * RS_BINS);
     rc |= dalloc(K.cnt, cap); rc |= dalloc(K.cc, cap); rc |= dalloc(K.bb, cap); rc |= dalloc(K.nr, cap);
@@ -568,14 +569,18 @@ int rbgpu_world::frame() {
             hipEvent_t *ec = timed_begin(7);
             k_colour_reset<<<nblk(std::max<uint32_t>(N * (COLOUR_WORDS / 4), MAX_COLOURS)), TPB, 0, st>>>(N, cb);
             k_colour_activate<<<nblk(P), TPB, 0, st>>>(K.cnt, P, K.bb, B, cb);
+            int rcs = scan(cb.blk, cb.blk, nblk(P), cb.ctr + CTR_ACT);
+            if (rcs) return rcs;
+            k_colour_compact<<<nblk(P), TPB, 0, st>>>(P, cb);
+            const uint32_t G = std::min<uint32_t>(nblk(P), 1024);
             // grid-wide rounds in batches (they skip themselves once at most tail_min pairs are
             // left); a host check between batches
             uint32_t launched = 0;
             for (;;) {
                 const uint32_t end = std::min(head_rounds, launched + round_batch);
                 for (; launched < end; ++launched) {
-                    k_colour_claim<<<nblk(P), TPB, 0, st>>>(K.bb, B, cb, P, launched, tail_min);
-                    k_colour_settle<<<nblk(P), TPB, 0, st>>>(K.bb, B, cb, P, launched);
+                    k_colour_claim<<<G, TPB, 0, st>>>(K.bb, B, cb, launched, tail_min);
+                    k_colour_settle<<<G, TPB, 0, st>>>(K.bb, B, cb, launched);
                 }
                 if (launched >= head_rounds) break;
                 HIPCHK(hipMemcpyAsync(rb->ctr, cb.ctr, CTR_SIZE * 4, hipMemcpyDeviceToHost, st));
@@ -584,13 +589,13 @@ int rbgpu_world::frame() {
                 for (uint32_t q = 0; q < LEFT_STRIPES; ++q) left += rb->ctr[CTR_LEFT + q];
                 if (left <= tail_min || !rb->ctr[CTR_FLAGS + launched - 1]) break;
             }
-            k_colour_collect<<<nblk(P), TPB, 0, st>>>(cb, P);
+            k_colour_collect<<<G, TPB, 0, st>>>(cb);
             k_colour_tail<<<1, TAIL_THREADS, 0, st>>>(K.bb, B, cb, launched, colour_rounds_cap);
             const uint32_t hist_blocks = std::min<uint32_t>(nblk(P), 512);
-            k_colour_hist<<<hist_blocks, 256, 0, st>>>(cb, P);
+            k_colour_hist<<<hist_blocks, 256, 0, st>>>(cb);
             k_colour_offsets<<<1, MAX_COLOURS, 0, st>>>(cb);
             const uint32_t chunk = 4096;
-            k_colour_scatter<<<nblk(P, chunk), 256, 0, st>>>(cb, P, solve_list, chunk);
+            k_colour_scatter<<<nblk(P, chunk), 256, 0, st>>>(cb, solve_list, chunk);
             HIPCHK(hipMemcpyAsync(rb->ctr, cb.ctr, CTR_SIZE * 4, hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(rb->col_off, cb.off, (MAX_COLOURS + 1) * 4, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));

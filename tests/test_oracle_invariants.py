@@ -118,3 +118,62 @@ def test_colour_order_vs_reference_order_divergence_reported():
     b.step(1)
     d = np.abs(a.state()["position"] - b.state()["position"]).max()
     assert np.isfinite(d) and d < 0.5
+
+
+def test_bounding_sphere_rejection_never_drops_a_reference_contact():
+    """The GPU narrowphase skips pairs whose bounding spheres are more than 1e-2 apart
+    (rbgpu_kernels.hpp, far_apart).  On the reference's own manifolds for random pairs
+    (tests/golden/narrowphase_pairs.npz) and on a random sweep through the oracle at gaps
+    just beyond the margin, no such pair has a contact point."""
+    from tests.conftest import golden
+    def radius(kind, d):
+        return d[0] if kind == scenes.SPHERE else (np.linalg.norm(d[:3]) if kind == scenes.BOX else d[0] + d[1])
+    g = golden("narrowphase_pairs")
+    CB = {scenes.CAPSULE, scenes.BOX}   # not rejected on the GPU: the reference's 6-axis SAT
+    checked = 0
+    for i in range(len(g["kind_a"])):
+        ka, kb = int(g["kind_a"][i]), int(g["kind_b"][i])
+        if scenes.PLANE in (ka, kb) or (ka == scenes.SPHERE and kb == scenes.SPHERE) or {ka, kb} == CB:
+            continue
+        dist = np.linalg.norm(g["tfa"][i][:3] - g["tfb"][i][:3])
+        if dist > radius(ka, g["dims_a"][i]) + radius(kb, g["dims_b"][i]) + 1e-2:
+            checked += 1
+            assert g["result"][i][0] == 0, i
+    rng = np.random.default_rng(8)
+    kinds = [scenes.SPHERE, scenes.BOX, scenes.CAPSULE]
+    for _ in range(4000):
+        a, b = np.zeros((), scenes.COLLIDER_DTYPE), np.zeros((), scenes.COLLIDER_DTYPE)
+        for rec in (a, b):
+            k = kinds[rng.integers(3)]
+            rec["shape"] = k
+            rec["dims"] = ((rng.uniform(0.2, 0.5), 0, 0, 0) if k == scenes.SPHERE else
+                           (*rng.uniform(0.2, 0.6, 3), 0) if k == scenes.BOX else (rng.uniform(0.2, 0.6), rng.uniform(0.1, 0.3), 0, 0))
+        if {int(a["shape"]), int(b["shape"])} == CB:
+            continue
+        ra, rb = radius(int(a["shape"]), a["dims"]), radius(int(b["shape"]), b["dims"])
+        qa, qb = rng.standard_normal(4), rng.standard_normal(4)
+        u = rng.standard_normal(3)
+        gap = rng.uniform(1e-2, 2e-2)   # just beyond the rejection margin
+        pb = u / np.linalg.norm(u) * (ra + rb + gap)
+        r = orc.oracle_collide(a, [0, 0, 0, *(qa / np.linalg.norm(qa))], b, [*pb, *(qb / np.linalg.norm(qb))], fp32=True)
+        checked += 1
+        assert r["n"] == 0
+    assert checked > 2500
+
+
+def test_reference_capsule_box_reports_contacts_for_separated_shapes():
+    """Why capsule-box keeps its full routine on the GPU: the reference's capsule-box SAT
+    (CapsuleBox.hpp:51-97) tests 3 face and 3 edge-segment axes only.  A configuration from
+    the dense mixed pile whose bounding spheres are 5.7 cm apart (so the shapes cannot
+    touch) still gets a contact point from the reference itself and from the oracle."""
+    cap = np.zeros((), scenes.COLLIDER_DTYPE)
+    cap["shape"], cap["dims"] = scenes.CAPSULE, (0.37779128, 0.2904079, 0, 0)
+    box = np.zeros((), scenes.COLLIDER_DTYPE)
+    box["shape"], box["dims"] = scenes.BOX, (0.53864865, 0.2337594, 0.30970263, 0)
+    tc = [-3.09105396, 0.73405004, 2.41632104, 0.46798345, -0.52098435, 0.70266134, 0.12583251]
+    tb = [-3.65475941, 1.71166432, 1.60650361, -0.95267385, -0.04297998, 0.18746893, 0.23541585]
+    gap = np.linalg.norm(np.array(tc[:3]) - tb[:3]) - (0.37779128 + 0.2904079) - np.linalg.norm(box["dims"][:3])
+    assert gap > 0.05
+    assert orc.oracle_collide(cap, tc, box, tb)["n"] == 1
+    if orc.ref_available():
+        assert orc.ref_collide(cap, tc, box, tb)["n"] == 1
