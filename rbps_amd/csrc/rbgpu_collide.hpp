@@ -257,6 +257,24 @@ RB_HD int reduce4(const V3 *p, const float *dep, int n, V3 *op, float *od) {
     op[3] = p[i3]; od[3] = dep[i3];
     return 4;
 }
+// The separating-axis part of box_box alone (same axes, order and arithmetic): false
+// exactly when box_box returns false before building a manifold.
+RB_HD bool box_box_overlap(const Shape &a, const TF &ta, const Shape &b, const TF &tb) {
+    V3 cd = tb.pos - ta.pos;
+    V3 ax[3] = {tf_vec(ta, V3{1, 0, 0}), tf_vec(ta, V3{0, 1, 0}), tf_vec(ta, V3{0, 0, 1})};
+    V3 bx[3] = {tf_vec(tb, V3{1, 0, 0}), tf_vec(tb, V3{0, 1, 0}), tf_vec(tb, V3{0, 0, 1})};
+    float best = __FLT_MAX__;
+    V3 bn{1.0f, 0.0f, 0.0f};
+    int bf = 0, feat = 0;
+    for (int i = 0; i < 3; ++i, ++feat)
+        if (!bb_axis(ax[i], ax, a, bx, b, cd, best, bn, feat, bf, 0.0f)) return false;
+    for (int i = 0; i < 3; ++i, ++feat)
+        if (!bb_axis(bx[i], ax, a, bx, b, cd, best, bn, feat, bf, 0.0f)) return false;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j, ++feat)
+            if (!bb_axis(cross(ax[i], bx[j]), ax, a, bx, b, cd, best, bn, feat, bf, 0.001f)) return false;
+    return true;
+}
 RB_HD bool box_box(const Shape &a, const TF &ta, const Shape &b, const TF &tb, Manifold &m) {
     V3 cd = tb.pos - ta.pos;
     V3 ax[3] = {tf_vec(ta, V3{1, 0, 0}), tf_vec(ta, V3{0, 1, 0}), tf_vec(ta, V3{0, 0, 1})};

@@ -624,6 +624,45 @@ __global__ void __launch_bounds__(128) k_narrow(const uint32_t *bin_slot, uint32
     else K.cnt[s] = 0;
 }
 
+// Box-box in two launches: the separating-axis test for every pair of the bin (registers
+// only, full occupancy), then the full routine, whose clipping buffers live in scratch
+// memory, for the overlapping pairs alone (it recomputes the same test, so results are
+// unchanged).  Each block of the first launch lists its overlapping pairs in its own
+// 128-entry segment; block b of the second launch takes segment b (no atomics).
+__global__ void __launch_bounds__(128) k_narrow_bb_sat(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C,
+                                                       Bodies B, Contacts K, uint32_t *hits, uint32_t *blk_hits) {
+    __shared__ uint32_t wc[2];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    uint32_t s = 0;
+    if (t < n) {
+        s = bin_slot[off + t];
+        const uint2 cc = K.cc[s], bb = K.bb[s];
+        const Shape a = shape_of(SH_BOX, C.dims[cc.x]), b = shape_of(SH_BOX, C.dims[cc.y]);
+        const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+        hit = !far_apart<BIN_BB>(a, ta, b, tb) && box_box_overlap(a, ta, b, tb);
+        if (!hit) K.cnt[s] = 0;
+    }
+    const uint64_t m = __ballot(hit);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wc[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (hit) hits[blockIdx.x * 128 + (wave ? wc[0] : 0u) + lane_rank(m)] = s;
+    if (threadIdx.x == 0) blk_hits[blockIdx.x] = wc[0] + wc[1];
+}
+__global__ void __launch_bounds__(128) k_narrow_bb_full(const uint32_t *hits, const uint32_t *blk_hits, Colliders C,
+                                                        Bodies B, Contacts K) {
+    if (threadIdx.x >= blk_hits[blockIdx.x]) return;
+    const uint32_t s = hits[blockIdx.x * 128 + threadIdx.x];
+    const uint2 cc = K.cc[s], bb = K.bb[s];
+    const Shape a = shape_of(SH_BOX, C.dims[cc.x]), b = shape_of(SH_BOX, C.dims[cc.y]);
+    const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+    Manifold m;
+    m.np = 0;
+    if (collide_bin<BIN_BB>(a, ta, b, tb, m) && m.np > 0) emit(m, B, K, s, bb);
+    else K.cnt[s] = 0;
+}
+
 // Single-pair narrowphase over a batch (the analytic dispatch of Dispatcher.hpp:212-217
 // without a world): out row = n, normal xyz, then (point xyz, depth) x 4.
 __global__ void k_collide_pairs(uint32_t n, const int *kind_a, const float4 *dims_a, const float4 *pos_a,

@@ -92,6 +92,7 @@ struct rbgpu_world {
     uint32_t nscenes = 1;               // independent scenes (batch) in this world
     uint32_t *cscene = nullptr, *slot_scene = nullptr, *sap_start = nullptr, *sap_end = nullptr;
     uint32_t *plane_lid = nullptr, *pair_lid = nullptr;
+    uint32_t *bb_hits = nullptr, *bb_blk = nullptr;   // box-box overlap lists (per 128-pair block)
     Joints Jt{};
     uint32_t *jorder = nullptr;         // joints by colour
     std::vector<uint32_t> jcol_off;     // joint colour offsets
@@ -145,7 +146,7 @@ struct rbgpu_world {
 
     void release_pairs() {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
-        dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid);
+        dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
@@ -237,6 +238,7 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     rc |= dalloc(pairs, cap);
     rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
     rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap);
+    rc |= dalloc(bb_hits, nblk(cap, 128) * 128); rc |= dalloc(bb_blk, nblk(cap, 128));
     rc |= dalloc(cb.acol, cap); rc |= dalloc(cb.tent, cap); rc |= dalloc(cb.rest, cap); rc |= dalloc(cb.act, cap);
     rc |= dalloc(cb.blk, nblk(cap) + 1);
     uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
@@ -554,7 +556,10 @@ int rbgpu_world::frame() {
                 switch (b) {
                     case BIN_SS: k_narrow<BIN_SS><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                     case BIN_SB: k_narrow<BIN_SB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    case BIN_BB: k_narrow<BIN_BB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                    case BIN_BB:
+                        k_narrow_bb_sat<<<g, 128, 0, st>>>(bin_list, o, n, C, B, K, bb_hits, bb_blk);
+                        k_narrow_bb_full<<<g, 128, 0, st>>>(bb_hits, bb_blk, C, B, K);
+                        break;
                     case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                     case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                     case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
