@@ -45,9 +45,73 @@ def _bind(lib, prefix):
     f("num_joints").argtypes = [_P]
     f("get_joints").argtypes = [_P, _D, _D, _D, _I]
     f("set_joint_targets").argtypes = [_P, C.c_int, _U, _D, _D]
+    f("hull_register").restype = C.c_int
+    f("hull_register").argtypes = [_D, C.c_int, _U, C.c_int]
+    f("hull_sizes").argtypes = [C.c_int, _I]
+    f("hull_tables").argtypes = [C.c_int, _D, _U, _U, _U, _U, _U, _U, _U, _U]
+    f("hull_mass").argtypes = [C.c_int, _D]
 
 
 _libs = {}
+_hull_ids = {}   # (lib prefix, content key) -> registered index
+
+
+def _hull_key(verts, faces):
+    v = np.ascontiguousarray(verts, np.float64)
+    f = None if faces is None else np.ascontiguousarray(faces, np.uint32)
+    return v, f, (v.tobytes(), None if f is None else f.tobytes())
+
+
+def register_hull(prefix, verts, faces=None):
+    """Registers a hull (vertices (n,3), CCW triangles (m,3) or None) in the oracle ("orc_")
+    or the reference build ("ref_"); the same content maps to the same index."""
+    lib = oracle_lib() if prefix == "orc_" else ref_lib()
+    v, f, key = _hull_key(verts, faces)
+    k = (prefix, key)
+    if k not in _hull_ids:
+        i = getattr(lib, prefix + "hull_register")(_ptr(v, _D), len(v), _ptr(f, _U), 0 if f is None else len(f))
+        if i < 0:
+            raise ValueError("invalid hull")
+        _hull_ids[k] = i
+    return _hull_ids[k]
+
+
+def hull_tables(prefix, verts, faces=None):
+    """Merged polygons and adjacency tables of a hull as the library built them."""
+    lib = oracle_lib() if prefix == "orc_" else ref_lib()
+    i = register_hull(prefix, verts, faces)
+    sz = np.zeros(6, np.int32)
+    getattr(lib, prefix + "hull_sizes")(i, _ptr(sz, _I))
+    P, NI, N2P, N2N, NP2P = (int(x) for x in sz[:5])
+    nv = len(verts)
+    out = {"pn": np.zeros((P, 3)), "poff": np.zeros(P + 1 if P else 0, np.uint32), "pidx": np.zeros(NI, np.uint32),
+           "v2p_o": np.zeros(nv + 1 if P else 0, np.uint32), "v2p_i": np.zeros(N2P, np.uint32),
+           "v2n_o": np.zeros(nv + 1 if P else 0, np.uint32), "v2n_i": np.zeros(N2N, np.uint32),
+           "p2p_o": np.zeros(P + 1 if P else 0, np.uint32), "p2p_i": np.zeros(NP2P, np.uint32)}
+    getattr(lib, prefix + "hull_tables")(i, *(_ptr(out[k], _D if k == "pn" else _U) for k in
+                                              ("pn", "poff", "pidx", "v2p_o", "v2p_i", "v2n_o", "v2n_i", "p2p_o", "p2p_i")))
+    return out
+
+
+def hull_mass(prefix, verts, faces=None):
+    """(volume, inertia xx yy zz xy xz yz) at unit density."""
+    lib = oracle_lib() if prefix == "orc_" else ref_lib()
+    out = np.zeros(7)
+    getattr(lib, prefix + "hull_mass")(register_hull(prefix, verts, faces), _ptr(out, _D))
+    return out[0], out[1:]
+
+
+def map_hulls(prefix, colliders, hulls):
+    """Copy of collider descriptors whose hull_index (into the scene's hull list) is
+    replaced by the library's registered index."""
+    c = np.array(colliders, copy=True)
+    if hulls is None or not len(c):
+        return c
+    sel = c["shape"] == 9
+    for k in np.nonzero(sel)[0]:
+        hv, hf = hulls[int(c["hull_index"][k])]
+        c["hull_index"][k] = register_hull(prefix, hv, hf)
+    return c
 
 
 def oracle_lib():
@@ -59,6 +123,8 @@ def oracle_lib():
         lib.orc_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int]
         _bind(lib, "orc_")
         lib.orc_collide.argtypes = [C.c_int, _P, _D, _P, _D, _D]
+        lib.orc_gjk_epa.restype = C.c_int
+        lib.orc_gjk_epa.argtypes = [C.c_int, _P, _D, _P, _D, _D, _D]
         lib.orc_num_solve_pairs.argtypes = [_P]
         lib.orc_get_solve_pairs.argtypes = [_P, _U, _I, _I, _I]
         lib.orc_num_colours.argtypes = [_P]
@@ -90,6 +156,8 @@ def ref_lib():
         lib.ref_create.argtypes = [C.c_double, C.c_int]
         _bind(lib, "ref_")
         lib.ref_collide.argtypes = [_P, _D, _P, _D, _D]
+        lib.ref_gjk_epa.restype = C.c_int
+        lib.ref_gjk_epa.argtypes = [_P, _D, _P, _D, _D]
         lib.ref_collision_groups.argtypes = [_P, _I]
         lib.ref_add_joints.restype = C.c_int
         lib.ref_joint_errors.restype = C.c_int
@@ -119,9 +187,10 @@ class _WorldBase:
 
     def load(self, scene):
         b = np.ascontiguousarray(scene.bodies)
-        c = np.ascontiguousarray(scene.colliders)
+        c = np.ascontiguousarray(map_hulls(self.prefix, scene.colliders, getattr(scene, "hulls", None)))
         self._fn("add_bodies")(self.h, len(b), b.ctypes.data_as(_P))
-        self._fn("add_colliders")(self.h, len(c), c.ctypes.data_as(_P))
+        if self._fn("add_colliders")(self.h, len(c), c.ctypes.data_as(_P)):
+            raise RuntimeError(f"{self.prefix}add_colliders: invalid collider")
         j = np.ascontiguousarray(getattr(scene, "joints", np.zeros(0)))
         if len(j):
             if self._fn("add_joints")(self.h, len(j), j.ctypes.data_as(_P)):
@@ -217,24 +286,60 @@ class RefWorld(_WorldBase):
         return out, g
 
 
-def _collide(fn, pre, a, tfa, b, tfb):
+def _collide(fn, pre, a, tfa, b, tfb, hulls=None, prefix="orc_"):
     out = np.zeros(20)
-    a = np.ascontiguousarray(a)
-    b = np.ascontiguousarray(b)
+    a = np.ascontiguousarray(map_hulls(prefix, np.atleast_1d(a), hulls))
+    b = np.ascontiguousarray(map_hulls(prefix, np.atleast_1d(b), hulls))
     tfa = np.ascontiguousarray(tfa, dtype=np.float64)
     tfb = np.ascontiguousarray(tfb, dtype=np.float64)
     hit = fn(*pre, a.ctypes.data_as(_P), _ptr(tfa, _D), b.ctypes.data_as(_P), _ptr(tfb, _D), _ptr(out, _D))
+    if hit < 0:
+        raise RuntimeError("collide: invalid shape")
     n = int(out[0])
     return {"hit": bool(hit), "n": n, "normal": out[1:4].copy(),
             "points": out[4:4 + 4 * n].reshape(n, 4)[:, :3].copy(), "depth": out[4:4 + 4 * n].reshape(n, 4)[:, 3].copy()}
 
 
-def oracle_collide(a, tfa, b, tfb, fp32=False):
-    return _collide(oracle_lib().orc_collide, (1 if fp32 else 0,), a, tfa, b, tfb)
+def oracle_collide(a, tfa, b, tfb, fp32=False, hulls=None):
+    return _collide(oracle_lib().orc_collide, (1 if fp32 else 0,), a, tfa, b, tfb, hulls, "orc_")
 
 
-def ref_collide(a, tfa, b, tfb):
-    return _collide(ref_lib().ref_collide, (), a, tfa, b, tfb)
+def ref_collide(a, tfa, b, tfb, hulls=None):
+    return _collide(ref_lib().ref_collide, (), a, tfa, b, tfb, hulls, "ref_")
+
+
+GJK_VALID, GJK_INSIDE, GJK_FAILED = 0, 1, 2
+EPA_VALID, EPA_FAILED, EPA_OUT_OF_FACES, EPA_OUT_OF_VERTICES, EPA_NONLINEAR = 0, 1, 2, 3, 4
+
+
+def _gjk_epa_out(out, hit):
+    return {"hit": bool(hit), "gjk_status": int(out[0]), "gjk_iters": int(out[1]), "epa_status": int(out[2]),
+            "epa_iters": int(out[3]), "normal": out[4:7].copy(), "depth": float(out[7]), "point": out[8:11].copy()}
+
+
+def oracle_gjk_epa(a, tfa, b, tfb, fp32=False, hulls=None, guess=None):
+    """GJK + EPA of one pair with statuses and iteration counts (guess: GJK seed direction)."""
+    out = np.zeros(11)
+    a = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(a), hulls))
+    b = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(b), hulls))
+    g = None if guess is None else np.ascontiguousarray(guess, np.float64)
+    hit = oracle_lib().orc_gjk_epa(1 if fp32 else 0, a.ctypes.data_as(_P), _ptr(np.ascontiguousarray(tfa, np.float64), _D),
+                                   b.ctypes.data_as(_P), _ptr(np.ascontiguousarray(tfb, np.float64), _D), _ptr(g, _D),
+                                   _ptr(out, _D))
+    if hit < 0:
+        raise RuntimeError("gjk_epa: invalid shape")
+    return _gjk_epa_out(out, hit)
+
+
+def ref_gjk_epa(a, tfa, b, tfb, hulls=None):
+    out = np.zeros(11)
+    a = np.ascontiguousarray(map_hulls("ref_", np.atleast_1d(a), hulls))
+    b = np.ascontiguousarray(map_hulls("ref_", np.atleast_1d(b), hulls))
+    hit = ref_lib().ref_gjk_epa(a.ctypes.data_as(_P), _ptr(np.ascontiguousarray(tfa, np.float64), _D),
+                                b.ctypes.data_as(_P), _ptr(np.ascontiguousarray(tfb, np.float64), _D), _ptr(out, _D))
+    if hit < 0:
+        raise RuntimeError("gjk_epa: invalid shape")
+    return _gjk_epa_out(out, hit)
 
 
 def _joint_err(fn, pre, desc, pose1, pose2, h):

@@ -6,7 +6,9 @@
 #include <exception>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <stdexcept>
 #include "orc_world.hpp"
 
 extern "C" {
@@ -47,6 +49,27 @@ typedef struct {
 }  // extern "C"
 
 namespace {
+
+// Registered hulls (process lifetime): tables in double, and the hull in each precision.
+std::deque<orc::HullTables> g_tables;
+std::deque<orc::Hull<float>> g_hull_f;
+std::deque<orc::Hull<double>> g_hull_d;
+template <class T> const orc::Hull<T> *hull_ref(int i);
+template <> const orc::Hull<float> *hull_ref<float>(int i) {
+    if (i < 0 || (size_t)i >= g_hull_f.size()) throw std::out_of_range("hull index");
+    return &g_hull_f[i];
+}
+template <> const orc::Hull<double> *hull_ref<double>(int i) {
+    if (i < 0 || (size_t)i >= g_hull_d.size()) throw std::out_of_range("hull index");
+    return &g_hull_d[i];
+}
+template <class T> orc::ShapeP<T> shape_from(const orc_collider_desc &d) {
+    orc::ShapeP<T> s;
+    s.kind = d.shape;
+    s.a = T(d.dims[0]); s.b = T(d.dims[1]); s.c = T(d.dims[2]); s.d = T(d.dims[3]);
+    if (d.shape == orc::SH_HULL) s.hull = hull_ref<T>(d.hull_index);
+    return s;
+}
 
 struct Any {
     virtual ~Any() {}
@@ -92,9 +115,7 @@ template <class T> struct Impl : Any {
                    orc::V3<T>(T(d.angular_velocity[0]), T(d.angular_velocity[1]), T(d.angular_velocity[2])), d.scene);
     }
     void add_collider(const orc_collider_desc &d) override {
-        orc::ShapeP<T> s;
-        s.kind = d.shape;
-        s.a = T(d.dims[0]); s.b = T(d.dims[1]); s.c = T(d.dims[2]); s.d = T(d.dims[3]);
+        const orc::ShapeP<T> s = shape_from<T>(d);
         w.add_collider(s, d.body_id, d.is_static != 0,
                        orc::V3<T>(T(d.local_pos[0]), T(d.local_pos[1]), T(d.local_pos[2])),
                        orc::Q<T>(T(d.local_rot[0]), T(d.local_rot[1]), T(d.local_rot[2]), T(d.local_rot[3])),
@@ -203,22 +224,37 @@ template <class T> void joint_errors_t(const orc_joint_desc *d, const double *po
     out[17] = j.target;
 }
 
+template <class T> orc::TF<T> tf7(const double *p) {
+    orc::TF<T> t;
+    t.pos = orc::V3<T>(T(p[0]), T(p[1]), T(p[2]));
+    t.rot = orc::Q<T>(T(p[3]), T(p[4]), T(p[5]), T(p[6]));
+    return t;
+}
+
+// GJK + EPA of one pair with statistics: out = gjk status, gjk iterations, epa status,
+// epa iterations, normal xyz, depth, EPA witness xyz (11 values).  guess3 may be null
+// (the reference's seed tf_b.pos - tf_a.pos).
+template <class T> int gjk_epa_t(const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
+                                 const double *tb, const double *guess3, double *out) {
+    const orc::ShapeP<T> sa = shape_from<T>(*a), sb = shape_from<T>(*b);
+    const orc::TF<T> A = tf7<T>(ta), Bt = tf7<T>(tb);
+    orc::Manifold<T> m;
+    orc::GjkEpaStats st{};
+    orc::EpaOut<T> e;
+    orc::V3<T> g;
+    if (guess3) g = orc::V3<T>(T(guess3[0]), T(guess3[1]), T(guess3[2]));
+    const bool hit = orc::gjk_epa_manifold(sa, A, sb, Bt, m, guess3 ? &g : nullptr, &st, &e);
+    out[0] = st.gjk_status; out[1] = st.gjk_iters; out[2] = st.epa_status; out[3] = st.epa_iters;
+    out[4] = e.n.x; out[5] = e.n.y; out[6] = e.n.z; out[7] = e.depth;
+    out[8] = e.p.x; out[9] = e.p.y; out[10] = e.p.z;
+    return hit ? 1 : 0;
+}
+
 // One narrowphase call (same layout as the reference driver's ref_collide).
 template <class T> int collide_t(const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
                                         const double *tb, double *out) {
-    auto shp = [](const orc_collider_desc *d) {
-        orc::ShapeP<T> s;
-        s.kind = d->shape; s.a = T(d->dims[0]); s.b = T(d->dims[1]); s.c = T(d->dims[2]); s.d = T(d->dims[3]);
-        return s;
-    };
-    auto tf = [](const double *p) {
-        orc::TF<T> t;
-        t.pos = orc::V3<T>(T(p[0]), T(p[1]), T(p[2]));
-        t.rot = orc::Q<T>(T(p[3]), T(p[4]), T(p[5]), T(p[6]));
-        return t;
-    };
     orc::Manifold<T> m;
-    bool hit = orc::dispatch(shp(a), tf(ta), shp(b), tf(tb), m);
+    bool hit = orc::dispatch(shape_from<T>(*a), tf7<T>(ta), shape_from<T>(*b), tf7<T>(tb), m);
     out[0] = hit ? m.n : 0;
     out[1] = m.normal.x; out[2] = m.normal.y; out[3] = m.normal.z;
     for (int i = 0; i < 4; ++i) {
@@ -252,7 +288,11 @@ int orc_add_bodies(void *h, int n, const orc_body_desc *d) {
     return 0;
 }
 int orc_add_colliders(void *h, int n, const orc_collider_desc *d) {
-    for (int i = 0; i < n; ++i) static_cast<Any *>(h)->add_collider(d[i]);
+    try {
+        for (int i = 0; i < n; ++i) static_cast<Any *>(h)->add_collider(d[i]);
+    } catch (const std::exception &) {
+        return -1;
+    }
     return 0;
 }
 int orc_step(void *h, int frames) {
@@ -311,7 +351,57 @@ uint32_t orc_mix32(uint32_t x) { return orc::mix32(x); }
 
 int orc_collide(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b, const double *tb,
                 double *out) {
-    return fp32 ? collide_t<float>(a, ta, b, tb, out) : collide_t<double>(a, ta, b, tb, out);
+    try {
+        return fp32 ? collide_t<float>(a, ta, b, tb, out) : collide_t<double>(a, ta, b, tb, out);
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+int orc_gjk_epa(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b, const double *tb,
+                const double *guess3, double *out) {
+    try {
+        return fp32 ? gjk_epa_t<float>(a, ta, b, tb, guess3, out) : gjk_epa_t<double>(a, ta, b, tb, guess3, out);
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+// Registers a hull (ConvexHull.hpp:149-420 tables); returns its index for
+// orc_collider_desc.hull_index.  faces may be null (vertex-only hull).
+int orc_hull_register(const double *verts, int nv, const uint32_t *faces, int nf) {
+    if (nv <= 0) return -1;
+    for (int i = 0; i < 3 * nf; ++i)
+        if (faces[i] >= (uint32_t)nv) return -1;
+    g_tables.push_back(orc::build_hull_tables(verts, (uint32_t)nv, faces, faces ? (uint32_t)nf : 0u));
+    g_hull_f.push_back(orc::Hull<float>::from(g_tables.back()));
+    g_hull_d.push_back(orc::Hull<double>::from(g_tables.back()));
+    return (int)g_tables.size() - 1;
+}
+// Table sizes: polygons, polygon indices, v2p, v2n, p2p entries; adjacency flag.
+int orc_hull_sizes(int i, int32_t *out6) {
+    if (i < 0 || (size_t)i >= g_tables.size()) return -1;
+    const auto &t = g_tables[i];
+    out6[0] = (int32_t)t.pn.size(); out6[1] = (int32_t)t.pidx.size(); out6[2] = (int32_t)t.v2p_i.size();
+    out6[3] = (int32_t)t.v2n_i.size(); out6[4] = (int32_t)t.p2p_i.size(); out6[5] = g_hull_d[i].adj ? 1 : 0;
+    return 0;
+}
+// Tables: poly normals (3 per poly), poly offsets (P+1), indices, and the three adjacency
+// lists as offsets (nv+1 / nv+1 / P+1) plus indices.
+int orc_hull_tables(int i, double *pn, uint32_t *poff, uint32_t *pidx, uint32_t *v2p_o, uint32_t *v2p_i,
+                    uint32_t *v2n_o, uint32_t *v2n_i, uint32_t *p2p_o, uint32_t *p2p_i) {
+    if (i < 0 || (size_t)i >= g_tables.size()) return -1;
+    const auto &t = g_tables[i];
+    for (size_t k = 0; k < t.pn.size(); ++k) { pn[3 * k] = t.pn[k].x; pn[3 * k + 1] = t.pn[k].y; pn[3 * k + 2] = t.pn[k].z; }
+    auto cp = [](const std::vector<uint32_t> &v, uint32_t *o) { if (o) for (size_t k = 0; k < v.size(); ++k) o[k] = v[k]; };
+    cp(t.poff, poff); cp(t.pidx, pidx); cp(t.v2p_o, v2p_o); cp(t.v2p_i, v2p_i);
+    cp(t.v2n_o, v2n_o); cp(t.v2n_i, v2n_i); cp(t.p2p_o, p2p_o); cp(t.p2p_i, p2p_i);
+    return 0;
+}
+// Volume and unit-density inertia (xx yy zz xy xz yz) of a registered hull.
+int orc_hull_mass(int i, double *out7) {
+    if (i < 0 || (size_t)i >= g_tables.size()) return -1;
+    orc::hull_mass_properties(g_tables[i], out7);
+    return 0;
 }
 
 }  // extern "C"

@@ -11,12 +11,17 @@ namespace orc {
 // Shape kinds use the reference's Shape::v variant indices (ShapeTypes.hpp:69).
 enum ShapeKind { SH_SPHERE = 0, SH_BOX = 1, SH_CAPSULE = 3, SH_PLANE = 6, SH_HULL = 9 };
 
+template <class T> struct Hull;   // orc_gjk.hpp
+
 template <class T> struct ShapeP {
     int kind = SH_SPHERE;
     // sphere: a=radius | box: half extents (a,b,c) | capsule: a=half_height b=radius
-    // plane: normal (a,b,c), d
+    // plane: normal (a,b,c), d | hull: the hull's tables
     T a = 0, b = 0, c = 0, d = 0;
+    const Hull<T> *hull = nullptr;
 };
+
+template <class T> inline AABB<T> hull_aabb(const Hull<T> &h, const TF<T> &tf);
 
 template <class T> struct Manifold {
     V3<T> normal;
@@ -46,6 +51,8 @@ template <class T> inline AABB<T> shape_aabb(const ShapeP<T> &s, const TF<T> &tf
         V3<T> rr(s.b, s.b, s.b);
         r.mn = V3<T>(smin(p1.x, p2.x), smin(p1.y, p2.y), smin(p1.z, p2.z)) - rr;
         r.mx = V3<T>(smax(p1.x, p2.x), smax(p1.y, p2.y), smax(p1.z, p2.z)) + rr;
+    } else if (s.kind == SH_HULL) {
+        r = hull_aabb(*s.hull, tf);
     } else {
         T inf = std::numeric_limits<T>::infinity();
         r.mn = V3<T>(-inf);
@@ -533,6 +540,10 @@ inline bool capsule_plane(const ShapeP<T> &cp, const TF<T> &tc, const ShapeP<T> 
 }
 
 // Dispatcher.hpp:104-116 and 212-217: analytic table; (B,A) = (A,B) with the normal flipped.
+}  // namespace orc
+#include "orc_gjk.hpp"
+namespace orc {
+
 template <class T>
 inline bool dispatch(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, const TF<T> &tb, Manifold<T> &m) {
     const int ka = a.kind, kb = b.kind;
@@ -546,6 +557,11 @@ inline bool dispatch(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, co
     if (ka == SH_CAPSULE && kb == SH_CAPSULE) return capsule_capsule(a, ta, b, tb, m);
     if (ka == SH_CAPSULE && kb == SH_BOX) return capsule_box(a, ta, b, tb, m);
     if (ka == SH_BOX && kb == SH_CAPSULE) return sym(capsule_box(b, tb, a, ta, m));
+    // hulls: hull-plane analytic (PlaneCollision.hpp:90-289); every other pair with a hull
+    // takes the GJK/EPA primary template, in the given order (Dispatcher.hpp:69-91)
+    if (ka == SH_HULL && kb == SH_PLANE) return hull_plane(a, ta, b, tb, m);
+    if (ka == SH_PLANE && kb == SH_HULL) return sym(hull_plane(b, tb, a, ta, m));
+    if ((ka == SH_HULL || kb == SH_HULL) && ka != SH_PLANE && kb != SH_PLANE) return gjk_epa_manifold(a, ta, b, tb, m);
     if (kb == SH_PLANE) {
         if (ka == SH_SPHERE) return sphere_plane(a, ta, b, tb, m);
         if (ka == SH_BOX) return box_plane(a, ta, b, tb, m);

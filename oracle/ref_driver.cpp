@@ -6,6 +6,8 @@
 // the same as oracle/orc_capi.cpp.
 #include <cstdint>
 #include <cstring>
+#include <deque>
+#include <vector>
 #include "rbps/API/World.hpp"
 #include "rbc/Dispatcher.hpp"
 
@@ -79,8 +81,18 @@ int add_joint(rbps::World &w, const ref_joint_desc &d) {
     return 0;
 }
 
+// Registered hulls: the caller-owned arrays ConvexHullData points at, kept for the
+// process lifetime, and the data made by convex_hull_data_create.
+struct HullStore {
+    std::vector<m3d::vec3> v;
+    std::vector<uint32_t> f;
+    rbc::ConvexHullData *data = nullptr;
+};
+std::deque<HullStore> g_hulls;
+
 rbc::Shape make_shape(const ref_collider_desc &d) {
     switch (d.shape) {
+        case 9: return rbc::Shape(rbc::ConvexHull(g_hulls.at(d.hull_index).data));
         case 0: return rbc::Shape(rbc::Sphere(d.dims[0]));
         case 1: return rbc::Shape(rbc::Box(m3d::vec3(d.dims[0], d.dims[1], d.dims[2])));
         case 3: return rbc::Shape(rbc::Capsule(d.dims[0], d.dims[1]));
@@ -191,7 +203,12 @@ void ref_get_contacts(void *h, uint32_t *ids4, double *n3, double *ra3, double *
 // out: [n, nx, ny, nz, (px, py, pz, depth) x4]; returns hit.
 int ref_collide(const ref_collider_desc *a, const double *tfa, const ref_collider_desc *b, const double *tfb, double *out) {
     rbc::ContactManifold m;
-    bool hit = rbc::dispatch(make_shape(*a), make_tf(tfa), make_shape(*b), make_tf(tfb), m);
+    bool hit;
+    try {
+        hit = rbc::dispatch(make_shape(*a), make_tf(tfa), make_shape(*b), make_tf(tfb), m);
+    } catch (const std::exception &) {
+        return -1;
+    }
     out[0] = hit ? m.num_points : 0;
     out[1] = m.normal.x; out[2] = m.normal.y; out[3] = m.normal.z;
     for (int i = 0; i < 4; ++i) {
@@ -262,6 +279,81 @@ int ref_joint_errors(const ref_joint_desc *d, const double *pose1, const double 
     out[16] = jc.current_position[0];
     out[17] = jc.target_position[0];
     return 0;
+}
+
+// convex_hull_data_create (ConvexHull.hpp:149-420); returns the index for hull_index.
+int ref_hull_register(const double *verts, int nv, const uint32_t *faces, int nf) {
+    g_hulls.emplace_back();
+    HullStore &h = g_hulls.back();
+    for (int i = 0; i < nv; ++i) h.v.push_back(m3d::vec3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]));
+    if (faces && nf > 0) h.f.assign(faces, faces + 3 * nf);
+    h.data = rbc::convex_hull_data_create(h.v.data(), (uint32_t)nv, h.f.empty() ? nullptr : h.f.data(),
+                                          h.f.empty() ? 0u : (uint32_t)nf);
+    return (int)g_hulls.size() - 1;
+}
+int ref_hull_sizes(int i, int32_t *out6) {
+    if (i < 0 || (size_t)i >= g_hulls.size()) return -1;
+    const rbc::ConvexHullData *d = g_hulls[i].data;
+    const uint32_t P = d->poly_count, nv = d->vert_count;
+    out6[0] = (int32_t)P;
+    out6[1] = P ? (int32_t)d->poly_offsets[P] : 0;
+    out6[2] = P ? (int32_t)d->vertex_to_polys_offsets[nv] : 0;
+    out6[3] = P ? (int32_t)d->vertex_to_neighbors_offsets[nv] : 0;
+    out6[4] = P ? (int32_t)d->poly_to_polys_offsets[P] : 0;
+    out6[5] = -1;
+    return 0;
+}
+int ref_hull_tables(int i, double *pn, uint32_t *poff, uint32_t *pidx, uint32_t *v2p_o, uint32_t *v2p_i,
+                    uint32_t *v2n_o, uint32_t *v2n_i, uint32_t *p2p_o, uint32_t *p2p_i) {
+    if (i < 0 || (size_t)i >= g_hulls.size()) return -1;
+    const rbc::ConvexHullData *d = g_hulls[i].data;
+    const uint32_t P = d->poly_count, nv = d->vert_count;
+    if (!P) return 0;
+    for (uint32_t k = 0; k < P; ++k) { pn[3 * k] = d->poly_normals[k].x; pn[3 * k + 1] = d->poly_normals[k].y; pn[3 * k + 2] = d->poly_normals[k].z; }
+    auto cp = [](const uint32_t *src, uint32_t n, uint32_t *o) { for (uint32_t k = 0; k < n; ++k) o[k] = src[k]; };
+    cp(d->poly_offsets, P + 1, poff);
+    cp(d->poly_indices, d->poly_offsets[P], pidx);
+    cp(d->vertex_to_polys_offsets, nv + 1, v2p_o);
+    cp(d->vertex_to_polys_indices, d->vertex_to_polys_offsets[nv], v2p_i);
+    cp(d->vertex_to_neighbors_offsets, nv + 1, v2n_o);
+    cp(d->vertex_to_neighbors_indices, d->vertex_to_neighbors_offsets[nv], v2n_i);
+    cp(d->poly_to_polys_offsets, P + 1, p2p_o);
+    cp(d->poly_to_polys_indices, d->poly_to_polys_offsets[P], p2p_i);
+    return 0;
+}
+// compute_volume / compute_inertia_tensor (ConvexHull.hpp:575-745): volume, xx yy zz xy xz yz.
+int ref_hull_mass(int i, double *out7) {
+    if (i < 0 || (size_t)i >= g_hulls.size()) return -1;
+    rbc::ConvexHull h(g_hulls[i].data);
+    out7[0] = rbc::compute_volume(h);
+    const m3d::smat3 I = rbc::compute_inertia_tensor(h);
+    out7[1] = I.xx; out7[2] = I.yy; out7[3] = I.zz; out7[4] = I.xy; out7[5] = I.xz; out7[6] = I.yz;
+    return 0;
+}
+// GJK + EPA of one pair as gjk_epa_manifold runs them (ContactManifoldGenerator.cpp:415-438):
+// out = gjk status, -1, epa status (-1 if not run), -1, normal xyz, depth, witness xyz.
+int ref_gjk_epa(const ref_collider_desc *a, const double *tfa, const ref_collider_desc *b, const double *tfb,
+                double *out) {
+    try {
+        const rbc::Shape sa = make_shape(*a), sb = make_shape(*b);
+        const m3d::tf ta = make_tf(tfa), tb = make_tf(tfb);
+        rbc::MinkowskiDiff md(&sa, &sb, ta, tb);
+        m3d::vec3 guess = tb.pos - ta.pos;
+        if (m3d::length_sq(guess) < m3d::EPSILON) guess = m3d::vec3(1.0, 0.0, 0.0);
+        rbc::GJK gjk;
+        const int gs = gjk.evaluate(md, guess);
+        for (int k = 0; k < 11; ++k) out[k] = 0.0;
+        out[0] = gs; out[1] = -1; out[2] = -1; out[3] = -1;
+        if (gs != rbc::GJK::Inside) return 0;
+        rbc::EPA epa;
+        const int es = epa.evaluate(gjk, md);
+        out[2] = es;
+        out[4] = epa.normal.x; out[5] = epa.normal.y; out[6] = epa.normal.z; out[7] = epa.depth;
+        out[8] = epa.contact_point.x; out[9] = epa.contact_point.y; out[10] = epa.contact_point.z;
+        return es == rbc::EPA::Valid ? 1 : 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
 }
 
 int ref_collision_groups(void *h, int32_t *out_colour) {

@@ -8,7 +8,10 @@ The descriptor dtypes mirror the C structs ``rbgpu_body_desc``,
 joint's public target fields).
 
 Shape codes are the reference's ``rbc::Shape`` variant indices
-(include/rbc/shapes/ShapeTypes.hpp:69): sphere 0, box 1, capsule 3, plane 6.
+(include/rbc/shapes/ShapeTypes.hpp:69): sphere 0, box 1, capsule 3, plane 6, convex
+hull 9.  A hull collider's ``hull_index`` indexes the scene's ``hulls`` list of
+(vertices (n,3), CCW triangles (m,3) or None), the arrays a caller hands to
+``rbc::convex_hull_data_create`` (include/rbc/shapes/ConvexHull.hpp:149).
 """
 from __future__ import annotations
 
@@ -82,6 +85,68 @@ def capsule_inertia(hh, r):
     return (ixx, iyy, ixx, 0.0, 0.0, 0.0)
 
 
+def hull_mass_properties(verts, faces=None):
+    """Volume and unit-density inertia (xx yy zz xy xz yz) about the centroid of a hull
+    (ConvexHull.hpp:575-745): tetrahedra from the origin per CCW triangle, moved to the
+    centroid; a vertex-only hull uses its local AABB as a box."""
+    v = np.asarray(verts, np.float64)
+    if faces is None or len(faces) == 0:
+        e = v.max(0) - v.min(0)
+        m = e[0] * e[1] * e[2]
+        x2, y2, z2 = e * e
+        return m, (m * (y2 + z2) / 12.0, m * (x2 + z2) / 12.0, m * (x2 + y2) / 12.0, 0.0, 0.0, 0.0)
+    f = np.asarray(faces, np.int64)
+    A, B, Cc = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    det = np.einsum("ij,ij->i", A, np.cross(B, Cc))
+    vol = det.sum() / 6.0
+    if abs(vol) < 1e-9:
+        return 0.0, (0.0,) * 6
+    cen = ((A + B + Cc) * (det / 24.0)[:, None]).sum(0) / vol
+
+    def sq(a, b, c):
+        return (det / 60.0) * (a * a + b * b + c * c + a * b + a * c + b * c)
+
+    def pr(i, j):
+        return (det / 120.0) * (2.0 * (A[:, i] * A[:, j] + B[:, i] * B[:, j] + Cc[:, i] * Cc[:, j])
+                                + A[:, i] * B[:, j] + A[:, j] * B[:, i] + A[:, i] * Cc[:, j] + A[:, j] * Cc[:, i]
+                                + B[:, i] * Cc[:, j] + B[:, j] * Cc[:, i])
+    mxx, myy, mzz = (sq(A[:, k], B[:, k], Cc[:, k]).sum() for k in range(3))
+    mxy, mxz, myz = pr(0, 1).sum(), pr(0, 2).sum(), pr(1, 2).sum()
+    cx, cy, cz = cen
+    return vol, (myy + mzz - vol * (cy * cy + cz * cz), mxx + mzz - vol * (cx * cx + cz * cz),
+                 mxx + myy - vol * (cx * cx + cy * cy), -mxy + vol * cx * cy, -mxz + vol * cx * cz,
+                 -myz + vol * cy * cz)
+
+
+def convex_hull_faces(points):
+    """Vertices and CCW-outward triangles of the convex hull of a point set (Qhull via
+    scipy; vertices not on the hull are dropped)."""
+    from scipy.spatial import ConvexHull
+    p = np.asarray(points, np.float64)
+    h = ConvexHull(p)
+    keep = np.unique(h.simplices)
+    remap = np.full(len(p), -1, np.int64)
+    remap[keep] = np.arange(len(keep))
+    v = p[keep]
+    f = remap[h.simplices]
+    n = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    flip = np.einsum("ij,ij->i", n, h.equations[:, :3]) < 0
+    f[flip] = f[flip][:, [0, 2, 1]]
+    return v, f.astype(np.uint32)
+
+
+def ellipsoid_hull(rng, n_min=16, n_max=64, axis_min=0.2, axis_max=0.8, faces=True):
+    """A random convex hull: n ~ U{n_min..n_max} points on an ellipsoid with semi-axes
+    U(axis_min, axis_max), with its triangles (or vertex-only)."""
+    n = int(rng.integers(n_min, n_max + 1))
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pts = d * rng.uniform(axis_min, axis_max, 3)
+    if not faces:
+        return pts, None
+    return convex_hull_faces(pts)
+
+
 @dataclass
 class Scene:
     """One world: bodies, colliders and the World(timestep, substeps) settings."""
@@ -93,6 +158,7 @@ class Scene:
     frames: int = 1
     meta: dict = field(default_factory=dict)
     joints: np.ndarray = field(default_factory=lambda: np.zeros(0, JOINT_DTYPE))
+    hulls: list = field(default_factory=list)
 
     @property
     def n_dynamic(self) -> int:
@@ -104,6 +170,14 @@ class SceneBuilder:
         self.b = []
         self.c = []
         self.j = []
+        self.h = []
+
+    def hull(self, verts, faces=None) -> int:
+        """Adds hull data (vertices, optional CCW triangles); returns its hull_index."""
+        v = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+        f = None if faces is None else np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        self.h.append((v, f))
+        return len(self.h) - 1
 
     def body(self, type_=DYNAMIC, mass=1.0, inertia=(1.0, 1.0, 1.0, 0.0, 0.0, 0.0), pos=(0, 0, 0),
              quat=(1, 0, 0, 0), vel=(0, 0, 0), angvel=(0, 0, 0)) -> int:
@@ -119,12 +193,12 @@ class SceneBuilder:
         return len(self.b) - 1
 
     def collider(self, shape, body, dims, is_static=False, local_pos=(0, 0, 0), local_rot=(1, 0, 0, 0),
-                 restitution=0.5, static_friction=0.5, dynamic_friction=0.5) -> int:
+                 restitution=0.5, static_friction=0.5, dynamic_friction=0.5, hull_index=-1) -> int:
         r = np.zeros((), COLLIDER_DTYPE)
         r["shape"] = shape
         r["body_id"] = body
         r["is_static"] = 1 if is_static else 0
-        r["hull_index"] = -1
+        r["hull_index"] = hull_index
         d = list(dims) + [0.0] * (4 - len(dims))
         r["dims"] = d
         r["local_pos"] = local_pos
@@ -151,7 +225,7 @@ class SceneBuilder:
         bodies = np.array(self.b, dtype=BODY_DTYPE) if self.b else np.zeros(0, BODY_DTYPE)
         cols = np.array(self.c, dtype=COLLIDER_DTYPE) if self.c else np.zeros(0, COLLIDER_DTYPE)
         joints = np.array(self.j, dtype=JOINT_DTYPE) if self.j else np.zeros(0, JOINT_DTYPE)
-        return Scene(name, bodies, cols, timestep, substeps, frames, dict(meta), joints)
+        return Scene(name, bodies, cols, timestep, substeps, frames, dict(meta), joints, list(self.h))
 
 
 def _rand_quat(rng) -> tuple:
@@ -403,13 +477,15 @@ def batch(parts, name="batch") -> Scene:
     """Independent scenes as one batched world: every body carries its scene index;
     colliders and joints are re-indexed.  Scenes may overlap in space, they never
     interact.  Timestep and substeps come from the first scene."""
-    bodies, cols, joints = [], [], []
+    bodies, cols, joints, hulls = [], [], [], []
     nb = 0
     for s, sc in enumerate(parts):
         b = sc.bodies.copy()
         b["scene"] = s
         c = sc.colliders.copy()
         c["body_id"] += nb
+        c["hull_index"][c["shape"] == HULL] += len(hulls)
+        hulls.extend(sc.hulls)
         j = sc.joints.copy()
         j["body_1"] += nb
         j["body_2"] += nb
@@ -420,7 +496,7 @@ def batch(parts, name="batch") -> Scene:
     first = parts[0]
     return Scene(name, np.concatenate(bodies), np.concatenate(cols), first.timestep, first.substeps, first.frames,
                  {"scenes": len(parts), "bodies_per_scene": [len(p.bodies) for p in parts]},
-                 np.concatenate(joints) if joints else np.zeros(0, JOINT_DTYPE))
+                 np.concatenate(joints) if joints else np.zeros(0, JOINT_DTYPE), hulls)
 
 
 def batched_stack_chains(n_scenes=4096, seed=77, n_min=32, n_max=512) -> Scene:
@@ -431,3 +507,103 @@ def batched_stack_chains(n_scenes=4096, seed=77, n_min=32, n_max=512) -> Scene:
     counts = rng.integers(n_min, n_max + 1, n_scenes)
     parts = [stack_chain(int(n), seed=seed + 1 + s) for s, n in enumerate(counts)]
     return batch(parts, f"batched_stack_chains_{n_scenes}")
+
+
+# ---------------------------------------------------------------- hulls
+def _hull_body(sb, pos, hidx, density=1000.0, quat=(1, 0, 0, 0), friction=0.5, restitution=0.0, vel=(0, 0, 0)):
+    v, f = sb.h[hidx]
+    vol, inert = hull_mass_properties(v, f)
+    b = sb.body(DYNAMIC, density * vol, tuple(density * x for x in inert), tuple(pos), quat, vel)
+    sb.collider(HULL, b, (), hull_index=hidx, restitution=restitution, static_friction=friction,
+                dynamic_friction=friction)
+    return b
+
+
+def tetrahedron_hull():
+    """The demo tetrahedron (src/visr_demo/main.cpp:12-30): 4 vertices, 4 CCW triangles."""
+    v = np.array([[0.6, 0.6, 0.6], [-0.6, -0.6, 0.6], [-0.6, 0.6, -0.6], [0.6, -0.6, -0.6]])
+    f = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], np.uint32)
+    return v, f
+
+
+def hex_prism_hull(radius=0.8, half_height=0.5):
+    """A hexagonal prism whose caps are fans of four triangles (merged into hexagons by
+    the coplanar merge) and whose sides are quads of two triangles."""
+    ang = np.arange(6) * (2.0 * math.pi / 6.0)
+    top = np.stack([np.cos(ang) * radius, np.full(6, half_height), np.sin(ang) * radius], 1)
+    bot = top * np.array([1.0, -1.0, 1.0])
+    v = np.concatenate([top, bot])
+    f = []
+    for i in range(1, 5):
+        f.append((0, i + 1, i))
+        f.append((6, i + 6, i + 7))
+    for i in range(6):
+        j = (i + 1) % 6
+        f.append((i, j, i + 6))
+        f.append((j, j + 6, i + 6))
+    return v, np.array(f, np.uint32)
+
+
+def box_hull(half=(0.5, 0.5, 0.5)):
+    """A box as a hull of 12 triangles (merged into 6 quads)."""
+    pts = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float64) * np.asarray(half)
+    return convex_hull_faces(pts)
+
+
+def hull_pile(n=32768, seed=4, faces=True, spacing=1.7, y0=1.0, jitter=0.05, density=1000.0, friction=0.5,
+              substeps=10, frames=500) -> Scene:
+    """The GJK/EPA-heavy workload: n random hulls (16-64 vertices on ellipsoids with
+    semi-axes U(0.2, 0.8)) on a seeded, jittered, square-based grid pile over a plane,
+    random orientations, each hull its own data."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    add_ground_plane(sb, friction, 0.0)
+    side = max(1, int(round(math.sqrt(n / 8.0))))
+    layers = (n + side * side - 1) // (side * side)
+    k = 0
+    for j in range(layers):
+        for i in range(side):
+            for l in range(side):
+                if k == n:
+                    break
+                v, f = ellipsoid_hull(rng, faces=faces)
+                h = sb.hull(v, f)
+                p = np.array([(i - (side - 1) / 2) * spacing, y0 + j * spacing, (l - (side - 1) / 2) * spacing])
+                p = p + rng.uniform(-jitter, jitter, 3)
+                _hull_body(sb, p, h, density, _rand_quat(rng), friction)
+                k += 1
+    return sb.build(f"hull_pile_{n}{'' if faces else '_vertex_only'}", 1.0 / 60.0, substeps, frames, seed=seed)
+
+
+def hull_zoo(seed=9) -> Scene:
+    """Hulls (tetrahedra, hexagonal prisms, box hulls, random ellipsoid hulls with and
+    without faces) dropped among boxes, spheres and capsules on a plane: every hull pair
+    kind (hull-hull manifold, hull-hull single point, hull vs primitive, hull-plane)."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    add_ground_plane(sb, 0.5, 0.0)
+    hs = [sb.hull(*tetrahedron_hull()), sb.hull(*hex_prism_hull()), sb.hull(*box_hull((0.4, 0.3, 0.5)))]
+    hs += [sb.hull(*ellipsoid_hull(rng, faces=True)) for _ in range(3)]
+    hs += [sb.hull(*ellipsoid_hull(rng, faces=False)) for _ in range(2)]
+    k = 0
+    for j in range(4):
+        for i in range(4):
+            for l in range(3):
+                p = np.array([(i - 1.5) * 1.3, 0.9 + j * 1.25, (l - 1.0) * 1.3]) + rng.uniform(-0.05, 0.05, 3)
+                q = _rand_quat(rng)
+                kind = k % 4
+                if kind == 3:
+                    which = int(rng.integers(0, 3))
+                    if which == 0:
+                        _box(sb, tuple(p), (0.35, 0.3, 0.4), quat=q)
+                    elif which == 1:
+                        r = 0.4
+                        b = sb.body(DYNAMIC, 1000.0 * sphere_volume(r), tuple(1000.0 * x for x in sphere_inertia(r)),
+                                    tuple(p), q)
+                        sb.collider(SPHERE, b, (r,), restitution=0.0)
+                    else:
+                        _capsule(sb, tuple(p), 0.35, 0.2, quat=q)
+                else:
+                    _hull_body(sb, p, hs[int(rng.integers(0, len(hs)))], quat=q)
+                k += 1
+    return sb.build("hull_zoo", 1.0 / 60.0, 10, 1, seed=seed)

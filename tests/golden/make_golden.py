@@ -26,12 +26,24 @@ from rbps_amd import scenes  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 
 
+def pack_hulls(hulls, prefix="hull_"):
+    """Hull list -> flat arrays: vertices with offsets, triangles with offsets (a
+    vertex-only hull has no triangles and a flag)."""
+    v = [np.asarray(h[0], np.float64) for h in hulls]
+    f = [np.zeros((0, 3), np.uint32) if h[1] is None else np.asarray(h[1], np.uint32) for h in hulls]
+    return {prefix + "v": np.concatenate(v), prefix + "voff": np.cumsum([0] + [len(x) for x in v]),
+            prefix + "f": np.concatenate(f), prefix + "foff": np.cumsum([0] + [len(x) for x in f]),
+            prefix + "has_faces": np.array([h[1] is not None for h in hulls])}
+
+
 def world_fixture(name, sc, frames):
     w = orc.RefWorld(sc.timestep, sc.substeps).load(sc)
     out = {"bodies": sc.bodies, "colliders": sc.colliders, "timestep": sc.timestep, "substeps": sc.substeps,
            "frames": np.array(frames)}
     if len(sc.joints):
         out["joints"] = sc.joints
+    if sc.hulls:
+        out.update(pack_hulls(sc.hulls))
     done = 0
     for f in frames:
         w.step(f - done)
@@ -128,6 +140,63 @@ def joint_errors_fixture(name, n=900, seed=11):
     print(name, "written", n)
 
 
+def hull_pairs_fixture(name, n=1500, seed=21):
+    """Hull tables and mass properties (ConvexHull.hpp:149-745) of a hull set, and random
+    pairs with at least one hull through rbc::dispatch: hull-plane analytic, every other
+    hull pair GJK + EPA + manifold (Dispatcher.hpp:69-91, ContactManifoldGenerator.cpp:415-438),
+    with the reference's GJK and EPA statuses."""
+    rng = np.random.default_rng(seed)
+    hulls = [scenes.tetrahedron_hull(), scenes.hex_prism_hull(), scenes.box_hull((0.4, 0.3, 0.5))]
+    hulls += [scenes.ellipsoid_hull(rng, faces=i % 2 == 0) for i in range(10)]
+    out = pack_hulls(hulls)
+    for i, (v, f) in enumerate(hulls):
+        t = orc.hull_tables("ref_", v, f)
+        for k, a in t.items():
+            out[f"t{i}_{k}"] = a
+        vol, inert = orc.hull_mass("ref_", v, f)
+        out[f"t{i}_mass"] = np.concatenate([[vol], inert])
+    shapes = [(scenes.HULL, (), i) for i in range(len(hulls))]
+    prims = [(scenes.SPHERE, (0.4,), -1), (scenes.BOX, (0.3, 0.4, 0.5), -1), (scenes.CAPSULE, (0.3, 0.2), -1)]
+    a = np.zeros(n, scenes.COLLIDER_DTYPE)
+    b = np.zeros(n, scenes.COLLIDER_DTYPE)
+    tfa, tfb = np.zeros((n, 7)), np.zeros((n, 7))
+    res, st = np.zeros((n, 20)), np.full((n, 2), -1, np.int32)
+
+    def unit(x):
+        return x / np.linalg.norm(x)
+
+    for i in range(n):
+        u = rng.random()
+        sa = shapes[rng.integers(len(shapes))]
+        if u < 0.15:
+            sb_ = (scenes.PLANE, (0.0, 1.0, 0.0, 0.0), -1)
+        elif u < 0.45:
+            sb_ = prims[rng.integers(3)]
+        else:
+            sb_ = shapes[rng.integers(len(shapes))]
+        if sb_[0] != scenes.PLANE and rng.random() < 0.5:
+            sa, sb_ = sb_, sa
+        for rec, (k, dims, h) in ((a[i], sa), (b[i], sb_)):
+            rec["shape"], rec["hull_index"] = k, h
+            rec["dims"] = list(dims) + [0.0] * (4 - len(dims))
+        tfa[i] = [*rng.uniform(-0.6, 0.6, 3), *unit(rng.standard_normal(4))]
+        tfb[i] = [*rng.uniform(-0.6, 0.6, 3), *unit(rng.standard_normal(4))]
+        if sb_[0] == scenes.PLANE:
+            tfb[i] = [0.0, rng.uniform(-0.5, 0.5), 0.0, 1.0, 0.0, 0.0, 0.0]
+        r = orc.ref_collide(a[i], tfa[i], b[i], tfb[i], hulls=hulls)
+        res[i, 0] = r["n"]
+        res[i, 1:4] = r["normal"]
+        for k in range(r["n"]):
+            res[i, 4 + 4 * k:7 + 4 * k] = r["points"][k]
+            res[i, 7 + 4 * k] = r["depth"][k]
+        if sb_[0] != scenes.PLANE:
+            g = orc.ref_gjk_epa(a[i], tfa[i], b[i], tfb[i], hulls=hulls)
+            st[i] = (g["gjk_status"], g["epa_status"])
+    out.update(a=a, b=b, tfa=tfa, tfb=tfb, result=res, status=st)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+    print(name, "hits", int((res[:, 0] > 0).sum()), "of", n)
+
+
 def api_scene_fixture(name):
     """Scenes of the reference's World tests (tests/rbps/API/test_world.cpp) run for their frame counts."""
     out = {}
@@ -169,3 +238,6 @@ if __name__ == "__main__":
     joint_errors_fixture("joint_errors")
     world_fixture("world_joint_zoo", scenes.joint_zoo(spherical=False), [1, 10, 60])
     world_fixture("world_planar_joints", scenes.planar_joints(), [1, 30, 120])
+    hull_pairs_fixture("hull_pairs")
+    world_fixture("world_hull_zoo", scenes.hull_zoo(), [1, 10, 60])
+    world_fixture("world_hull_pile_64", scenes.hull_pile(64, seed=3), [1, 30])

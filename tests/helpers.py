@@ -133,3 +133,43 @@ def trial_colouring(pairs_bodies, dynamic, active, mix32, slots=16, palette_max=
     for p, c in col.items():
         out[p] = c
     return out, r
+
+
+def unpack_hulls(g, prefix="hull_"):
+    """Hull list of a fixture written by make_golden.pack_hulls."""
+    v, voff, f, foff, has = (g[prefix + k] for k in ("v", "voff", "f", "foff", "has_faces"))
+    return [(v[voff[i]:voff[i + 1]], f[foff[i]:foff[i + 1]] if has[i] else None) for i in range(len(has))]
+
+
+def scene_from_golden(g):
+    """The scene of a world fixture (bodies, colliders, joints and hulls when present)."""
+    sc = scenes.Scene("g", g["bodies"], g["colliders"], float(g["timestep"]), int(g["substeps"]))
+    if "joints" in g:
+        sc.joints = g["joints"]
+    if "hull_v" in g:
+        sc.hulls = unpack_hulls(g)
+    return sc
+
+
+def sat_depth(va, fa, vb, fb):
+    """Penetration depth of two convex polytopes (world vertices, triangles) by brute
+    force: the minimum over the Minkowski-difference face normals (the triangle normals of
+    both hulls and the cross products of their edge directions, both signs) of
+    max(a.n) - min(b.n).  Negative: separated (then a separating axis exists among them)."""
+    def tri_normals(v, f):
+        n = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+        return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+    def edges(v, f):
+        e = np.concatenate([v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 1]], v[f[:, 0]] - v[f[:, 2]]])
+        return e / np.linalg.norm(e, axis=1, keepdims=True)
+    ax = [tri_normals(va, fa), tri_normals(vb, fb)]
+    ea, eb = edges(va, fa), edges(vb, fb)
+    c = np.cross(ea[:, None, :], eb[None, :, :]).reshape(-1, 3)
+    ln = np.linalg.norm(c, axis=1)
+    ax.append(c[ln > 1e-9] / ln[ln > 1e-9, None])
+    n = np.concatenate(ax)
+    n = np.concatenate([n, -n])
+    d = (va @ n.T).max(0) - (vb @ n.T).min(0)
+    k = int(np.argmin(d))
+    return float(d[k]), n[k]
