@@ -53,7 +53,7 @@ typedef struct {
     int32_t shape;
     uint32_t body_id;
     int32_t is_static;
-    int32_t hull_index;          /* reserved, -1 */
+    int32_t hull_index;          /* RBGPU_HULL: id from rbgpu_add_hull (else -1) */
     double dims[4];
     double local_pos[3];
     double local_rot[4];         /* w x y z */
@@ -98,7 +98,7 @@ typedef struct {
     uint32_t colours;            /* colours of the last substep's colouring */
     uint32_t colour_rounds;      /* colouring rounds of the last substep */
     uint32_t contacts;           /* contact points of the last substep */
-    uint32_t bins[10];           /* pairs per narrowphase bin of the last frame */
+    uint32_t bins[16];           /* pairs per narrowphase bin of the last frame (SS SB BB SC CC CB SP BP CP HH HX HP other) */
     uint32_t active_pairs;       /* pairs with contact points in the last substep */
 } rbgpu_stats;
 
@@ -222,6 +222,38 @@ int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, doubl
  * out: 20 per pair = n_points, normal xyz, then (point xyz, depth) x 4 (zeros when no hit). */
 int rbgpu_collide_pairs(int n, const int32_t *kind_a, const double *dims_a, const double *tf_a, const int32_t *kind_b,
                         const double *dims_b, const double *tf_b, double *out);
+
+/* ---- convex hulls ------------------------------------------------------------------
+ * rbc::convex_hull_data_create (include/rbc/shapes/ConvexHull.hpp:149-420): vertices
+ * (3 doubles each, local frame) and optional CCW outward triangles; the world copies them,
+ * merges coplanar triangles into polygons and builds the adjacency tables the hull-hull
+ * manifold uses.  A hull collider (shape RBGPU_HULL) names the returned id in hull_index.
+ * Limits: 1..256 vertices, merged polygons of at most 64 vertices and 64 neighbours
+ * (RBGPU_ERR_CAPACITY otherwise).  Pairs: hull-plane is analytic (PlaneCollision.hpp:90-285);
+ * any other pair with a hull runs GJK + EPA + manifold (ContactManifoldGenerator.cpp:415-438),
+ * with a face/edge manifold when both hulls have triangles, else one point. */
+int rbgpu_add_hull(rbgpu_world *w, const double *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                   uint32_t n_triangles, uint32_t *hull_id_out);
+/* compute_volume / compute_inertia_tensor (ConvexHull.hpp:575-745): unit-density volume
+ * and inertia about the centroid (xx yy zz xy xz yz); vertex-only hulls use the local AABB. */
+int rbgpu_hull_mass_properties(const double *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                               uint32_t n_triangles, double *volume, double *inertia6);
+/* The tables rbgpu_add_hull builds (ConvexHullData's poly_* and adjacency arrays,
+ * ConvexHull.hpp:79-98).  sizes6 = polygons, polygon indices, v2p, v2n and p2p entries,
+ * adjacency flag; call with null arrays first to size them.  Offsets arrays hold
+ * polygons + 1 (poly, p2p) or vertices + 1 (v2p, v2n) entries. */
+int rbgpu_hull_tables(const double *vertices, uint32_t n_vertices, const uint32_t *triangles, uint32_t n_triangles,
+                      uint32_t *sizes6, double *poly_normals, uint32_t *poly_offsets, uint32_t *poly_indices,
+                      uint32_t *v2p_offsets, uint32_t *v2p_indices, uint32_t *v2n_offsets, uint32_t *v2n_indices,
+                      uint32_t *p2p_offsets, uint32_t *p2p_indices);
+/* The narrowphase (rbc::dispatch, Dispatcher.hpp:212-217) of given collider pairs at the
+ * world's current state, through the pipeline's own binned kernels; a plane must be the
+ * second collider (as in the pipeline's plane loop, CollisionPipeline.cpp:156-197).
+ * out32 per pair: points, normal xyz, then per point the two surface points p_a, p_b
+ * (CollisionPipeline.cpp:59-104) and the depth (7 values) x 4.  gjk_stats (optional) per
+ * pair: GJK iterations | EPA iterations << 8 | GJK status << 16 | EPA status << 20 (EPA
+ * status 15 = not run; 0xffffffff = not a GJK pair).  Replaces the last substep's contacts. */
+int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2, double *out32, uint32_t *gjk_stats);
 
 const char *rbgpu_last_error(void);
 

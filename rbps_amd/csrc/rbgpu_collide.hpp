@@ -50,6 +50,18 @@ RB_HD Box3 shape_aabb(const Shape &s, const TF &tf) {
     return r;
 }
 
+// ConvexHull.hpp:484-499: centre and half extent of the local AABB, projected like a box.
+RB_HD Box3 hull_aabb(V3 mn, V3 mx, const TF &tf) {
+    const V3 c = (mn + mx) * 0.5f;
+    const V3 e = (mx - mn) * 0.5f;
+    const M3 R = mat3_cast(tf.rot);
+    const V3 wc = tf.pos + tf_vec(tf, c);
+    const V3 ext{absf(R.c0.x) * e.x + absf(R.c1.x) * e.y + absf(R.c2.x) * e.z,
+                 absf(R.c0.y) * e.x + absf(R.c1.y) * e.y + absf(R.c2.y) * e.z,
+                 absf(R.c0.z) * e.x + absf(R.c1.z) * e.y + absf(R.c2.z) * e.z};
+    return Box3{wc - ext, wc + ext};
+}
+
 RB_HD void capsule_ends(const Shape &c, const TF &tf, V3 &p1, V3 &p2) {   // Capsule.hpp:43-49
     V3 ax = tf_vec(tf, V3{0.0f, c.a, 0.0f});
     p1 = tf.pos + ax;

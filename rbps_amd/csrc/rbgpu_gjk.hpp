@@ -1,0 +1,885 @@
+// Convex hulls on CDNA4: GJK, EPA and the hull contact manifolds, one 64-lane
+// workgroup (one wavefront) per pair.
+//
+// Work split inside the wave:
+//   * support mappings: each lane holds up to four hull vertices in registers (vertex
+//     lane + 64k); a support is a lane-local scan plus a butterfly arg-max with the
+//     lowest vertex index winning ties, which is the reference's first-maximum scan
+//     (ConvexHull.hpp:457-473);
+//   * GJK's simplex logic is uniform scalar code every lane executes alike;
+//   * EPA keeps its polytope in LDS (vertices: w and the A-side witness; faces: normal,
+//     distance, vertex indices, obsolete flag).  The closest face is a parallel arg-min
+//     (lowest face index on ties, like the reference's first-minimum scan); the visible
+//     faces are tested in parallel and listed in face order; the horizon is computed in
+//     parallel from that list: the reference's add_edge (EPA.cpp:102-114) toggles an
+//     undirected edge in an ordered list, so the final list holds the edges that occur an
+//     odd number of times, each in the direction and at the place of its last occurrence;
+//     new faces are built one per lane and packed in horizon order;
+//   * the hull-hull manifold (ContactManifoldGenerator.cpp:176-379): supports and the
+//     edge-pair search are wave-parallel; the polygon clipping and the 4-point reduction
+//     run on lane 0 over LDS buffers.
+// Every floating-point expression keeps the reference's operand order, so the FP32
+// oracle's restatement reproduces these results bit for bit.
+#pragma once
+#include "rbgpu_kernels.hpp"
+
+namespace rb {
+
+constexpr int HULL_MAX_VERTS = 256;
+constexpr int HULL_VPL = HULL_MAX_VERTS / 64;     // vertices per lane
+constexpr int HULL_MAX_POLY = 64;                 // vertices of one polygon, neighbours of one polygon
+constexpr int EPA_FACES = 256, EPA_VERTS = 128, EPA_ITERS = 128, GJK_ITERS = 128;
+constexpr int EPA_SEQ = 3 * EPA_FACES;            // directed edges of the visible faces
+constexpr int HCLIP_CAP = 2 * HULL_MAX_POLY + 8;   // Sutherland-Hodgman output bound (n + planes)
+constexpr int HP_MAX_POINTS = 32;                 // hull-plane candidates (PlaneCollision.hpp:118)
+
+enum { GJK_VALID = 0, GJK_INSIDE = 1, GJK_FAILED = 2 };
+enum { EPA_VALID = 0, EPA_FAILED = 1, EPA_OUT_OF_FACES = 2, EPA_OUT_OF_VERTICES = 3, EPA_NONLINEAR = 4 };
+
+__device__ __forceinline__ int lane64() { return (int)(threadIdx.x & 63); }
+
+// ------------------------------------------------------------------ wave reductions
+// Arg-max over (value, index): larger value, then lower index.
+__device__ __forceinline__ void wave_argmax(float &d, int &i) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float od = __shfl_xor(d, o, 64);
+        const int oi = __shfl_xor(i, o, 64);
+        if (od > d || (od == d && oi < i)) { d = od; i = oi; }
+    }
+}
+// Arg-min over (value, index): smaller value, then lower index.
+__device__ __forceinline__ void wave_argmin(float &d, int &i) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float od = __shfl_xor(d, o, 64);
+        const int oi = __shfl_xor(i, o, 64);
+        if (od < d || (od == d && oi < i)) { d = od; i = oi; }
+    }
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------ shapes in a wave
+struct WaveHull {   // this lane's vertices lane + 64k
+    float x[HULL_VPL], y[HULL_VPL], z[HULL_VPL];
+    int nv;
+    uint32_t v0;
+};
+__device__ __forceinline__ void load_wave_hull(WaveHull &w, const HullSet &hs, uint32_t h) {
+    const HullMeta m = hs.meta[h];
+    w.nv = (int)m.nv;
+    w.v0 = m.v0;
+#pragma unroll
+    for (int k = 0; k < HULL_VPL; ++k) {
+        const int i = lane64() + 64 * k;
+        float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i < w.nv) p = hs.v[m.v0 + i];
+        w.x[k] = p.x; w.y[k] = p.y; w.z[k] = p.z;
+    }
+}
+// Index of the first vertex with the largest dot(v, d) (uniform result).
+__device__ __forceinline__ int wave_support_index(const WaveHull &w, V3 d) {
+    float best = -__builtin_inff();
+    int bi = 0x7fffffff;
+    bool nan0 = false;
+#pragma unroll
+    for (int k = 0; k < HULL_VPL; ++k) {
+        const int i = lane64() + 64 * k;
+        if (i < w.nv) {
+            float x = w.x[k] * d.x + w.y[k] * d.y + w.z[k] * d.z;
+            if (x != x) {
+                nan0 |= i == 0;
+                x = -__builtin_inff();
+            }
+            if (x > best || (x == best && i < bi)) { best = x; bi = i; }
+        }
+    }
+    wave_argmax(best, bi);
+    // a NaN first value is never beaten in the sequential scan
+    return __shfl((int)nan0, 0, 64) ? 0 : bi;
+}
+
+struct Side {
+    int kind;
+    float a, b, c;   // sphere r | box half extents | capsule half_height, radius
+    TF tf;
+    WaveHull wh;
+};
+__device__ __forceinline__ V3 hull_vertex(const HullSet &hs, const WaveHull &w, int i) {
+    const float4 p = hs.v[w.v0 + i];
+    return V3{p.x, p.y, p.z};
+}
+// Local support (Sphere.hpp:48-51, Box.hpp:49-55, Capsule.hpp:72-80, ConvexHull.hpp:457-473).
+__device__ __forceinline__ V3 local_support(const Side &s, const HullSet &hs, V3 d) {
+    if (s.kind == SH_HULL) return hull_vertex(hs, s.wh, wave_support_index(s.wh, d));
+    if (s.kind == SH_SPHERE) return normalize(d) * s.a;
+    if (s.kind == SH_BOX) return V3{d.x > 0.0f ? s.a : -s.a, d.y > 0.0f ? s.b : -s.b, d.z > 0.0f ? s.c : -s.c};
+    const V3 e{0.0f, d.y >= 0.0f ? s.a : -s.a, 0.0f};   // capsule
+    const float l = len(d);
+    if (l < EPS) return e;
+    return e + vdiv(d, l) * s.b;
+}
+// World support of one side (MinkowskiDiff.hpp:52-63).
+__device__ __forceinline__ V3 side_support(const Side &s, const HullSet &hs, V3 dir) {
+    return tf_point(s.tf, local_support(s, hs, tf_inv_vec(s.tf, dir)));
+}
+
+struct SV {
+    V3 w0, w1, w;
+};
+__device__ __forceinline__ SV md_sample(const Side &A, const Side &Bs, const HullSet &hs, V3 d) {
+    SV s;
+    s.w0 = side_support(A, hs, d);
+    s.w1 = side_support(Bs, hs, -d);
+    s.w = s.w0 - s.w1;
+    return s;
+}
+
+// ------------------------------------------------------------------ GJK (GJK.cpp:42-96)
+struct Simplex {
+    SV v[4];   // v[0] newest
+    int rank;
+};
+__device__ __forceinline__ void sx_push(Simplex &s, const SV &n) {   // DoSimplex.cpp:5-17
+    const int r = s.rank;
+    if (r >= 3) s.v[3] = s.v[2];
+    if (r >= 2) s.v[2] = s.v[1];
+    if (r >= 1) s.v[1] = s.v[0];
+    s.v[0] = n;
+    if (r < 4) s.rank = r + 1;
+}
+__device__ __forceinline__ V3 triple(V3 a, V3 b, V3 c) { return cross(cross(a, b), c); }
+
+__device__ __forceinline__ void sx_line(Simplex &s, V3 &dir) {   // Line.cpp:6-48
+    const V3 a = s.v[0].w, b = s.v[1].w;
+    const V3 ao = -a, ab = b - a;
+    if (dot(ab, ao) >= 0.0f) {
+        s.rank = 2;
+        dir = triple(ab, ao, ab);
+        if (len2(dir) < 1e-6f) {
+            const float ax = absf(ab.x), ay = absf(ab.y), az = absf(ab.z);
+            const V3 axis = (ax <= ay && ax <= az) ? V3{1.0f, 0.0f, 0.0f}
+                                                   : (ay <= az ? V3{0.0f, 1.0f, 0.0f} : V3{0.0f, 0.0f, 1.0f});
+            dir = cross(ab, axis);
+            if (len2(dir) < 1e-6f) dir = cross(ab, V3{0.0f, 0.0f, 1.0f});
+        }
+    } else {
+        s.rank = 1;
+        dir = ao;
+    }
+}
+__device__ __forceinline__ void sx_triangle(Simplex &s, V3 &dir) {   // Triangle.cpp:6-80
+    const V3 a = s.v[0].w, b = s.v[1].w, c = s.v[2].w;
+    const V3 ao = -a, ab = b - a, ac = c - a, abc = cross(ab, ac);
+    if (dot(cross(abc, ac), ao) >= 0.0f) {
+        if (dot(ac, ao) >= 0.0f) { s.v[1] = s.v[2]; s.rank = 2; dir = triple(ac, ao, ac); }
+        else if (dot(ab, ao) >= 0.0f) { s.rank = 2; dir = triple(ab, ao, ab); }
+        else { s.rank = 1; dir = ao; }
+    } else if (dot(cross(ab, abc), ao) >= 0.0f) {
+        if (dot(ab, ao) >= 0.0f) { s.rank = 2; dir = triple(ab, ao, ab); }
+        else { s.rank = 1; dir = ao; }
+    } else if (dot(abc, ao) >= 0.0f) {
+        s.rank = 3;
+        dir = abc;
+    } else {
+        const SV t = s.v[1];
+        s.v[1] = s.v[2];
+        s.v[2] = t;
+        s.rank = 3;
+        dir = -abc;
+    }
+}
+// One face (A, B, C) of the tetrahedron with outward normal n (Tetrahedron.cpp:55-165).
+__device__ __forceinline__ void sx_tet_face(Simplex &s, V3 &dir, const SV &A, const SV &Bv, const SV &Cv, V3 ab, V3 ac,
+                                            V3 n, V3 ao) {
+    if (dot(cross(n, ac), ao) >= 0.0f) {
+        if (dot(ac, ao) >= 0.0f) { s.v[0] = A; s.v[1] = Cv; s.rank = 2; dir = triple(ac, ao, ac); }
+        else if (dot(ab, ao) >= 0.0f) { s.v[0] = A; s.v[1] = Bv; s.rank = 2; dir = triple(ab, ao, ab); }
+        else { s.v[0] = A; s.rank = 1; dir = ao; }
+    } else if (dot(cross(ab, n), ao) >= 0.0f) {
+        if (dot(ab, ao) >= 0.0f) { s.v[0] = A; s.v[1] = Bv; s.rank = 2; dir = triple(ab, ao, ab); }
+        else { s.v[0] = A; s.rank = 1; dir = ao; }
+    } else {
+        s.v[0] = A; s.v[1] = Bv; s.v[2] = Cv; s.rank = 3; dir = n;
+    }
+}
+__device__ __forceinline__ bool sx_tetra(Simplex &s, V3 &dir) {   // Tetrahedron.cpp:6-217
+    const SV va = s.v[0], vb = s.v[1], vc = s.v[2], vd = s.v[3];
+    const V3 ao = -va.w, ab = vb.w - va.w, ac = vc.w - va.w, ad = vd.w - va.w;
+    const V3 abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab);
+    int pi = 0;
+    if (dot(abc, ao) > 1e-10f) pi |= 1;
+    if (dot(acd, ao) > 1e-10f) pi |= 2;
+    if (dot(adb, ao) > 1e-10f) pi |= 4;
+    switch (pi) {
+        case 0: return true;
+        case 1: sx_tet_face(s, dir, va, vb, vc, ab, ac, abc, ao); break;
+        case 2: sx_tet_face(s, dir, va, vc, vd, ac, ad, acd, ao); break;
+        case 4: sx_tet_face(s, dir, va, vd, vb, ad, ab, adb, ao); break;
+        case 3: case 5: case 6: {
+            const SV &X = pi == 3 ? vc : (pi == 5 ? vb : vd);
+            const V3 ax = pi == 3 ? ac : (pi == 5 ? ab : ad);
+            if (dot(ax, ao) >= 0.0f) { s.v[0] = va; s.v[1] = X; s.rank = 2; dir = triple(ax, ao, ax); }
+            else { s.v[0] = va; s.rank = 1; dir = ao; }
+            break;
+        }
+        default: s.v[0] = va; s.rank = 1; dir = ao; break;
+    }
+    return false;
+}
+__device__ __forceinline__ int gjk(const Side &A, const Side &Bs, const HullSet &hs, V3 guess, Simplex &s, int &iters) {
+    s.rank = 0;
+    V3 dir = (len2(guess) > EPS) ? guess : V3{1.0f, 0.0f, 0.0f};
+    const SV v0 = md_sample(A, Bs, hs, dir);
+    sx_push(s, v0);
+    dir = -v0.w;
+    for (int it = 0; it < GJK_ITERS; ++it) {
+        iters = it + 1;
+        const SV nx = md_sample(A, Bs, hs, dir);
+        if (dot(nx.w, dir) < 0.0f) return GJK_VALID;
+        sx_push(s, nx);
+        bool inside = false;
+        if (s.rank == 2) sx_line(s, dir);
+        else if (s.rank == 3) sx_triangle(s, dir);
+        else inside = sx_tetra(s, dir);
+        if (inside) return GJK_INSIDE;
+    }
+    return GJK_FAILED;
+}
+
+// ------------------------------------------------------------------ EPA (EPA.cpp:141-378)
+struct EpaLds {
+    float wx[EPA_VERTS], wy[EPA_VERTS], wz[EPA_VERTS];     // polytope vertices (w)
+    float ax[EPA_VERTS], ay[EPA_VERTS], az[EPA_VERTS];     // their A-side supports (w0)
+    float nx[EPA_FACES], ny[EPA_FACES], nz[EPA_FACES], nd[EPA_FACES];
+    uint32_t fv[EPA_FACES];                                // v0 | v1 << 8 | v2 << 16 | obsolete << 24
+    uint16_t vis[EPA_FACES];                               // visible faces, face order
+    uint16_t seq[EPA_SEQ];                                 // their directed edges (a | b << 8)
+    uint16_t hor[EPA_SEQ];                                 // horizon edges, in order
+};
+// Manifold buffers, overlaid on the EPA polytope once EPA is done.
+struct ClipLds {
+    float px[2][HCLIP_CAP], py[2][HCLIP_CAP], pz[2][HCLIP_CAP];
+    float qx[HULL_MAX_POLY], qy[HULL_MAX_POLY], qz[HULL_MAX_POLY];   // clip planes: point
+    float mx[HULL_MAX_POLY], my[HULL_MAX_POLY], mz[HULL_MAX_POLY];   // and normal
+    float dep[HCLIP_CAP];
+};
+union PairLds {
+    EpaLds e;
+    ClipLds c;
+};
+
+__device__ __forceinline__ V3 lds_w(const EpaLds &L, int i) { return V3{L.wx[i], L.wy[i], L.wz[i]}; }
+__device__ __forceinline__ V3 lds_a(const EpaLds &L, int i) { return V3{L.ax[i], L.ay[i], L.az[i]}; }
+
+// compute_face_normal (EPA.cpp:37-87) over the first nall polytope vertices.
+__device__ __forceinline__ bool epa_face_normal(const EpaLds &L, int i0, int i1, int i2, int nall, V3 &n_out, float &d_out) {
+    const V3 a = lds_w(L, i0), b = lds_w(L, i1), c = lds_w(L, i2);
+    V3 n = cross(b - a, c - a);
+    const float nl = len(n);
+    if (nl < EPS) return false;
+    n = vdiv(n, nl);
+    float d = dot(n, a);
+    if (d < -1e-12f) {
+        n = -n;
+        d = -d;
+    } else if (d <= 1e-12f) {
+        bool fixed = false;
+        for (int i = 0; i < nall; ++i) {
+            const float t = dot(n, lds_w(L, i));
+            if (t > 1e-12f) { n = -n; d = 0.0f; fixed = true; break; }
+            if (t < -1e-12f) { d = 0.0f; fixed = true; break; }
+        }
+        if (!fixed) return false;
+    }
+    n_out = n;
+    d_out = d;
+    return true;
+}
+
+struct EpaResult {
+    int status, iters;
+    V3 n, p;
+    float depth;
+};
+
+__device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side &A, const Side &Bs, const HullSet &hs,
+                                         EpaLds &L) {
+    EpaResult r{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
+    const int lane = lane64();
+    if (seed.rank != 4) { r.status = EPA_NONLINEAR; return r; }
+    {   // coplanar seed: a touching contact (EPA.cpp:174-212)
+        const V3 va = seed.v[0].w, vb = seed.v[1].w, vc = seed.v[2].w, vd = seed.v[3].w;
+        const float e2 = EPS * EPS;
+        V3 pn = cross(vb - va, vc - va);
+        if (len2(pn) < e2) pn = cross(vb - va, vd - va);
+        if (len2(pn) < e2) pn = cross(vc - va, vd - va);
+        if (len2(pn) > e2) {
+            pn = normalize(pn);
+            const float dd = absf(dot(vd - va, pn));
+            if (dd < 1e-5f) {
+                const V3 sp = side_support(A, hs, pn) - side_support(Bs, hs, -pn);
+                const V3 sn = side_support(A, hs, -pn) - side_support(Bs, hs, pn);
+                const float dp = dot(sp - va, pn);
+                const float dn = dot(sn - va, -pn);
+                if (dp < 1e-4f || dn < 1e-4f) {
+                    r.n = (dp <= dn) ? pn : -pn;
+                    r.depth = 0.0f;
+                    r.p = (((seed.v[0].w0 + seed.v[1].w0) + seed.v[2].w0) + seed.v[3].w0) * 0.25f;
+                    r.status = EPA_VALID;
+                    return r;
+                }
+            }
+        }
+    }
+    if (lane < 4) {
+        const SV &s = lane == 0 ? seed.v[0] : (lane == 1 ? seed.v[1] : (lane == 2 ? seed.v[2] : seed.v[3]));
+        L.wx[lane] = s.w.x; L.wy[lane] = s.w.y; L.wz[lane] = s.w.z;
+        L.ax[lane] = s.w0.x; L.ay[lane] = s.w0.y; L.az[lane] = s.w0.z;
+    }
+    __syncthreads();
+    int nvtx = 4, nf = 0;
+    {   // seed faces ABC, ACD, ADB, BCD (every lane evaluates them alike)
+        const int fa[4] = {0, 0, 0, 1}, fb[4] = {1, 2, 3, 2}, fc[4] = {2, 3, 1, 3};
+        for (int f = 0; f < 4; ++f) {
+            V3 n;
+            float d;
+            if (!epa_face_normal(L, fa[f], fb[f], fc[f], 4, n, d)) { r.status = EPA_NONLINEAR; return r; }
+            if (lane == f) {
+                L.nx[f] = n.x; L.ny[f] = n.y; L.nz[f] = n.z; L.nd[f] = d;
+                L.fv[f] = (uint32_t)fa[f] | ((uint32_t)fb[f] << 8) | ((uint32_t)fc[f] << 16);
+            }
+        }
+        nf = 4;
+    }
+    __syncthreads();
+    for (int it = 0; it < EPA_ITERS; ++it) {
+        r.iters = it + 1;
+        // 1. closest live face
+        float best = 3.402823466e+38f;
+        int bi = 0x7fffffff;
+        for (int f = lane; f < nf; f += 64) {
+            const float d = L.nd[f];
+            if (!(L.fv[f] >> 24) && d < best) { best = d; bi = f; }
+        }
+        wave_argmin(best, bi);
+        if (bi == 0x7fffffff) { r.status = EPA_FAILED; return r; }
+        const int ci = bi;
+        const float mind = best;
+        const V3 cn{L.nx[ci], L.ny[ci], L.nz[ci]};
+        const uint32_t cv = L.fv[ci];
+        const int c0 = (int)(cv & 255u), c1 = (int)((cv >> 8) & 255u), c2 = (int)((cv >> 16) & 255u);
+        // 2-3. support along its normal; convergence
+        const SV w = md_sample(A, Bs, hs, cn);
+        const float dproj = dot(w.w, cn);
+        if (dproj - mind < 1e-6f) {
+            r.n = cn;
+            r.depth = mind;
+            const V3 q = cn * mind;
+            const V3 p0 = lds_w(L, c0), p1 = lds_w(L, c1), p2 = lds_w(L, c2);
+            const V3 e0 = p1 - p0, e1 = p2 - p0, rr = q - p0;
+            const float d00 = dot(e0, e0), d01 = dot(e0, e1), d11 = dot(e1, e1), d20 = dot(rr, e0), d21 = dot(rr, e1);
+            const float den = d00 * d11 - d01 * d01;
+            if (absf(den) < EPS) {
+                r.p = lds_a(L, c0);
+            } else {
+                const float vw = (d11 * d20 - d01 * d21) / den;
+                const float ww = (d00 * d21 - d01 * d20) / den;
+                const float uw = 1.0f - vw - ww;
+                r.p = (lds_a(L, c0) * uw + lds_a(L, c1) * vw) + lds_a(L, c2) * ww;
+            }
+            r.status = EPA_VALID;
+            return r;
+        }
+        // duplicate support: stop on the closest face
+        bool dup = false;
+        for (int i = lane; i < nvtx; i += 64) dup |= len2(lds_w(L, i) - w.w) < EPS * EPS;
+        if (__ballot(dup)) {
+            r.n = cn;
+            r.depth = mind;
+            r.p = lds_a(L, c0);
+            r.status = EPA_VALID;
+            return r;
+        }
+        if (nvtx >= EPA_VERTS) { r.status = EPA_OUT_OF_VERTICES; return r; }
+        const int wi = nvtx;
+        if (lane == 0) {
+            L.wx[wi] = w.w.x; L.wy[wi] = w.w.y; L.wz[wi] = w.w.z;
+            L.ax[wi] = w.w0.x; L.ay[wi] = w.w0.y; L.az[wi] = w.w0.z;
+        }
+        ++nvtx;
+        // 4. visible faces, listed in face order
+        int nvis = 0;
+        for (int base = 0; base < nf; base += 64) {
+            const int f = base + lane;
+            bool v = false;
+            if (f < nf && !(L.fv[f] >> 24)) {
+                const uint32_t fv = L.fv[f];
+                const V3 cen = ((lds_w(L, (int)(fv & 255u)) + lds_w(L, (int)((fv >> 8) & 255u))) +
+                                lds_w(L, (int)((fv >> 16) & 255u))) * (1.0f / 3.0f);
+                const V3 fn{L.nx[f], L.ny[f], L.nz[f]};
+                v = dot(fn, w.w - cen) > 0.0f;
+            }
+            const uint64_t m = __ballot(v);
+            if (v) {
+                L.fv[f] |= 1u << 24;
+                L.vis[nvis + (int)lane_rank(m)] = (uint16_t)f;
+            }
+            nvis += __popcll(m);
+        }
+        __syncthreads();
+        // directed edges (v0,v1) (v1,v2) (v2,v0) of each visible face
+        const int ns = 3 * nvis;
+        for (int i = lane; i < ns; i += 64) {
+            const uint32_t fv = L.fv[L.vis[i / 3]];
+            const int k = i % 3;
+            const uint32_t a = (fv >> (8 * k)) & 255u, b = (fv >> (8 * ((k + 1) % 3))) & 255u;
+            L.seq[i] = (uint16_t)(a | (b << 8));
+        }
+        __syncthreads();
+        // horizon: odd occurrence count, kept at the last occurrence
+        int nh = 0;
+        for (int base = 0; base < ns; base += 64) {
+            const int i = base + lane;
+            bool keep = false;
+            if (i < ns) {
+                const uint32_t e = L.seq[i];
+                const uint32_t a = e & 255u, b = e >> 8;
+                const uint32_t key = a < b ? (a | (b << 8)) : (b | (a << 8));
+                bool later = false;
+                for (int j = i + 1; j < ns && !later; ++j) {
+                    const uint32_t f = L.seq[j];
+                    const uint32_t fa = f & 255u, fb = f >> 8;
+                    later = (fa < fb ? (fa | (fb << 8)) : (fb | (fa << 8))) == key;
+                }
+                int before = 0;
+                if (!later)
+                    for (int j = 0; j < i; ++j) {
+                        const uint32_t f = L.seq[j];
+                        const uint32_t fa = f & 255u, fb = f >> 8;
+                        before += (fa < fb ? (fa | (fb << 8)) : (fb | (fa << 8))) == key;
+                    }
+                keep = !later && (before & 1) == 0;
+            }
+            const uint64_t m = __ballot(keep);
+            if (keep) L.hor[nh + (int)lane_rank(m)] = L.seq[i];
+            nh += __popcll(m);
+        }
+        __syncthreads();
+        // stitch: one candidate face per horizon edge, packed in horizon order
+        int made_before_last = 0, made = 0;
+        for (int base = 0; base < nh; base += 64) {
+            const int k = base + lane;
+            bool ok = false;
+            V3 n{0.0f, 0.0f, 0.0f};
+            float d = 0.0f;
+            uint32_t e = 0;
+            if (k < nh) {
+                e = L.hor[k];
+                ok = epa_face_normal(L, (int)(e & 255u), (int)(e >> 8), wi, nvtx, n, d);
+            }
+            const uint64_t m = __ballot(ok);
+            const int slot = nf + made + (int)lane_rank(m);
+            if (ok && k != nh - 1) ++made_before_last;   // counted per lane, summed below
+            if (ok && slot < EPA_FACES) {
+                L.nx[slot] = n.x; L.ny[slot] = n.y; L.nz[slot] = n.z; L.nd[slot] = d;
+                L.fv[slot] = e | ((uint32_t)wi << 16);
+            }
+            made += __popcll(m);
+        }
+        // the reference checks the pool before each new face, failing ones included
+        if (nh > 0 && nf + (int)wave_sum((uint32_t)made_before_last) >= EPA_FACES) {
+            r.status = EPA_OUT_OF_FACES;
+            return r;
+        }
+        nf += made;
+        __syncthreads();
+    }
+    r.status = EPA_FAILED;
+    return r;
+}
+
+// ------------------------------------------------------------------ manifolds
+struct HullTabs {   // one hull's tables
+    const float4 *v, *pn;
+    const uint32_t *poff, *pidx, *v2p_o, *v2p_i, *v2n_o, *v2n_i, *p2p_o, *p2p_i;
+};
+__device__ __forceinline__ HullTabs hull_tabs(const HullSet &hs, uint32_t h) {
+    const HullMeta m = hs.meta[h];
+    return HullTabs{hs.v + m.v0, hs.pn + m.pn0, hs.tab + m.poff, hs.tab + m.pidx, hs.tab + m.v2p_o,
+                    hs.tab + m.v2p_i, hs.tab + m.v2n_o, hs.tab + m.v2n_i, hs.tab + m.p2p_o, hs.tab + m.p2p_i};
+}
+__device__ __forceinline__ V3 tv(const float4 *p, uint32_t i) {
+    const float4 q = p[i];
+    return V3{q.x, q.y, q.z};
+}
+
+// Polygon at a support vertex most aligned with wd (ContactManifoldGenerator.cpp:73-95).
+__device__ __forceinline__ uint32_t best_polygon(const HullTabs &h, const TF &tf, uint32_t sv, V3 wd, float &al) {
+    const uint32_t s = h.v2p_o[sv], e = h.v2p_o[sv + 1];
+    uint32_t best = (s < e) ? h.v2p_i[s] : 0u;
+    float bd = -3.402823466e+38f;
+    for (uint32_t i = s; i < e; ++i) {
+        const uint32_t p = h.v2p_i[i];
+        const float x = dot(tf_vec(tf, tv(h.pn, p)), wd);
+        if (x > bd) { bd = x; best = p; }
+    }
+    al = bd;
+    return best;
+}
+
+// Clipping.cpp:6-36
+__device__ __forceinline__ bool clip_in(V3 pp, V3 pn, V3 p) { return dot(p, pn) + (-dot(pn, pp)) >= 0.0f; }
+__device__ __forceinline__ bool clip_edge(V3 pp, V3 pn, V3 s, V3 e, V3 &out) {
+    const V3 ab = e - s;
+    const float abp = dot(pn, ab);
+    if (absf(abp) <= 1e-6f) return false;
+    const float dist = -dot(pn, pp);
+    const V3 pco = pn * (-dist);
+    float fac = -dot(pn, s - pco) / abp;
+    if (fac < 0.0f) fac = 0.0f;
+    if (fac > 1.0f) fac = 1.0f;
+    out = s + ab * fac;
+    return true;
+}
+// Sutherland-Hodgman of buffer `src` (n points) against one plane into buffer 1 - src.
+// A convex n-gon gains at most one point per plane, so HCLIP_CAP never binds; the bound
+// only keeps a numerically non-convex input inside the buffer.
+__device__ __forceinline__ int clip_pass(ClipLds &C, int src, int n, V3 pp, V3 pn, bool remove_only) {
+    const int dst = 1 - src;
+    int m = 0;
+    auto put = [&](V3 p) {
+        if (m < HCLIP_CAP) { C.px[dst][m] = p.x; C.py[dst][m] = p.y; C.pz[dst][m] = p.z; ++m; }
+    };
+    V3 sp{C.px[src][n - 1], C.py[src][n - 1], C.pz[src][n - 1]};
+    for (int j = 0; j < n; ++j) {
+        const V3 ep{C.px[src][j], C.py[src][j], C.pz[src][j]};
+        const bool si = clip_in(pp, pn, sp), ei = clip_in(pp, pn, ep);
+        V3 cut;
+        if (remove_only) {
+            if (ei) put(ep);
+        } else if (si && ei) {
+            put(ep);
+        } else if (si && !ei) {
+            if (clip_edge(pp, pn, sp, ep, cut)) put(cut);
+        } else if (!si && ei) {
+            if (clip_edge(pp, pn, sp, ep, cut)) put(cut);
+            put(ep);
+        }
+        sp = ep;
+    }
+    return m;
+}
+// Clipping.cpp:104-128
+__device__ __forceinline__ bool skew_lines(V3 p1, V3 d1, V3 p2, V3 d2, V3 &l1, V3 &l2) {
+    const float n1 = d1.x * d2.x + d1.y * d2.y + d1.z * d2.z;
+    const float n2 = d2.x * d2.x + d2.y * d2.y + d2.z * d2.z;
+    const float m1 = -d1.x * d1.x - d1.y * d1.y - d1.z * d1.z;
+    const float m2 = -d2.x * d1.x - d2.y * d1.y - d2.z * d1.z;
+    const float r1 = -d1.x * p2.x + d1.x * p1.x - d1.y * p2.y + d1.y * p1.y - d1.z * p2.z + d1.z * p1.z;
+    const float r2 = -d2.x * p2.x + d2.x * p1.x - d2.y * p2.y + d2.y * p1.y - d2.z * p2.z + d2.z * p1.z;
+    const float den = (n1 * m2) - (n2 * m1);
+    if (absf(den) < EPS) return false;
+    const float n = ((r1 * m2) - (r2 * m1)) / den;
+    const float m = ((n1 * r2) - (n2 * r1)) / den;
+    l1 = p1 + d1 * m;
+    l2 = p2 + d2 * n;
+    return true;
+}
+// ContactManifoldGenerator.hpp:88-154 over the kept points in C.px[buf]/C.dep (lane 0).
+__device__ __forceinline__ int reduce4_area(const ClipLds &C, int buf, int n, Manifold &m) {
+    auto P = [&](int i) { return V3{C.px[buf][i], C.py[buf][i], C.pz[buf][i]}; };
+    if (n <= 4) {
+        for (int i = 0; i < n; ++i) { m.p[i] = P(i); m.dep[i] = C.dep[i]; }
+        return n;
+    }
+    int i0 = 0;
+    for (int i = 1; i < n; ++i) if (C.dep[i] > C.dep[i0]) i0 = i;
+    int i1 = (i0 == 0) ? 1 : 0;
+    float best = len2(P(i1) - P(i0));
+    for (int i = 0; i < n; ++i) {
+        if (i == i0) continue;
+        const float x = len2(P(i) - P(i0));
+        if (x > best) { best = x; i1 = i; }
+    }
+    int i2 = -1;
+    best = 0.0f;
+    const V3 seg = P(i1) - P(i0);
+    for (int i = 0; i < n; ++i) {
+        if (i == i0 || i == i1) continue;
+        const float x = len2(cross(P(i) - P(i0), seg));
+        if (x > best) { best = x; i2 = i; }
+    }
+    if (i2 < 0) for (int i = 0; i < n; ++i) if (i != i0 && i != i1) { i2 = i; break; }
+    int i3 = -1;
+    best = 0.0f;
+    if (i2 >= 0) {
+        const V3 tn = cross(P(i1) - P(i0), P(i2) - P(i0));
+        for (int i = 0; i < n; ++i) {
+            if (i == i0 || i == i1 || i == i2) continue;
+            const float x = absf(dot(P(i) - P(i0), tn));
+            if (x > best) { best = x; i3 = i; }
+        }
+        if (i3 < 0) for (int i = 0; i < n; ++i) if (i != i0 && i != i1 && i != i2) { i3 = i; break; }
+    }
+    m.p[0] = P(i0); m.dep[0] = C.dep[i0];
+    m.p[1] = P(i1); m.dep[1] = C.dep[i1];
+    if (i2 >= 0) { m.p[2] = P(i2); m.dep[2] = C.dep[i2]; }
+    if (i3 >= 0) { m.p[3] = P(i3); m.dep[3] = C.dep[i3]; return 4; }
+    return (i2 >= 0) ? 3 : 2;
+}
+
+// Hull-hull manifold from the EPA result (ContactManifoldGenerator.cpp:176-379).  The
+// result is valid on lane 0.
+__device__ __forceinline__ void manifold_hulls(const EpaResult &e, const Side &A, const Side &Bs, const HullSet &hs,
+                                               uint32_t ha_id, uint32_t hb_id, ClipLds &C, Manifold &m) {
+    const int lane = lane64();
+    const float lowest = -3.402823466e+38f;
+    m.n = e.n;
+    m.np = 0;
+    const HullTabs ha = hull_tabs(hs, ha_id), hb = hull_tabs(hs, hb_id);
+    const uint32_t sa = (uint32_t)wave_support_index(A.wh, tf_inv_vec(A.tf, e.n));
+    const uint32_t sb = (uint32_t)wave_support_index(Bs.wh, tf_inv_vec(Bs.tf, -e.n));
+    float al_a, al_b;
+    const uint32_t pa = best_polygon(ha, A.tf, sa, e.n, al_a);
+    const uint32_t pb = best_polygon(hb, Bs.tf, sb, -e.n, al_b);
+    // edge pairs at the two support vertices, in (i, j, sign) order
+    const V3 wsa = tf_point(A.tf, tv(ha.v, sa)), wsb = tf_point(Bs.tf, tv(hb.v, sb));
+    const uint32_t as = ha.v2n_o[sa], ae = ha.v2n_o[sa + 1], bs = hb.v2n_o[sb], be = hb.v2n_o[sb + 1];
+    const int dega = (int)(ae - as), degb = (int)(be - bs);
+    float al_e = lowest;
+    int key = 0x7fffffff;
+    if (dega > 0 && degb > 0) {
+        for (int c = lane; c < dega * degb; c += 64) {
+            const uint32_t na = ha.v2n_i[as + c / degb], nb = hb.v2n_i[bs + c % degb];
+            const V3 ea = wsa - tf_point(A.tf, tv(ha.v, na));
+            const V3 eb = wsb - tf_point(Bs.tf, tv(hb.v, nb));
+            const V3 cc = cross(ea, eb);
+            const float cl = len(cc);
+            if (cl < EPS) continue;
+            const V3 cn = vdiv(cc, cl);
+            const float dp = dot(cn, e.n);
+            if (dp > al_e) { al_e = dp; key = 2 * c; }
+            const float dn = -dp;
+            if (dn > al_e) { al_e = dn; key = 2 * c + 1; }
+        }
+    }
+    wave_argmax(al_e, key);
+    const bool has_edge = key != 0x7fffffff;
+    if (lane != 0) return;
+    if (has_edge && al_e > al_a + 1e-4f && al_e > al_b + 1e-4f) {
+        const int c = key / 2;
+        const uint32_t na = ha.v2n_i[as + c / degb], nb = hb.v2n_i[bs + c % degb];
+        const V3 p1 = tf_point(A.tf, tv(ha.v, sa));
+        const V3 d1 = tf_point(A.tf, tv(ha.v, na)) - p1;
+        const V3 p2 = tf_point(Bs.tf, tv(hb.v, sb));
+        const V3 d2 = tf_point(Bs.tf, tv(hb.v, nb)) - p2;
+        V3 l1, l2;
+        m.np = 1;
+        m.dep[0] = e.depth;
+        m.p[0] = skew_lines(p1, d1, p2, d2, l1, l2) ? (l1 + l2) * 0.5f : e.p;
+        return;
+    }
+    const bool ref_a = al_a > al_b;
+    const HullTabs &hr = ref_a ? ha : hb;
+    const HullTabs &hi = ref_a ? hb : ha;
+    const TF &tr = ref_a ? A.tf : Bs.tf;
+    const TF &ti = ref_a ? Bs.tf : A.tf;
+    const uint32_t rp = ref_a ? pa : pb, ip = ref_a ? pb : pa;
+    const V3 rn = tf_vec(tr, tv(hr.pn, rp));
+    const uint32_t is = hi.poff[ip], ie = hi.poff[ip + 1];
+    const int inc_n = (int)(ie - is);
+    if (inc_n < 3) {
+        m.np = 1; m.p[0] = e.p; m.dep[0] = e.depth;
+        return;
+    }
+    for (int i = 0; i < inc_n; ++i) {
+        const V3 p = tf_point(ti, tv(hi.v, hi.pidx[is + i]));
+        C.px[0][i] = p.x; C.py[0][i] = p.y; C.pz[0][i] = p.z;
+    }
+    const V3 rpt = tf_point(tr, tv(hr.v, hr.pidx[hr.poff[rp]]));
+    int n = inc_n, buf = 0;
+    for (uint32_t k = hr.p2p_o[rp]; k < hr.p2p_o[rp + 1] && n > 0; ++k) {
+        const uint32_t q = hr.p2p_i[k];
+        const V3 pp = tf_point(tr, tv(hr.v, hr.pidx[hr.poff[q]]));
+        const V3 pn = -tf_vec(tr, tv(hr.pn, q));
+        n = clip_pass(C, buf, n, pp, pn, false);
+        buf = 1 - buf;
+    }
+    if (n > 0) {
+        n = clip_pass(C, buf, n, rpt, -rn, true);
+        buf = 1 - buf;
+    }
+    int kept = 0;   // compacted in place
+    for (int i = 0; i < n; ++i) {
+        const V3 p{C.px[buf][i], C.py[buf][i], C.pz[buf][i]};
+        const float dtp = dot(p - rpt, rn);
+        const V3 diff = rn * dtp;
+        const float pen = ref_a ? dot(diff, e.n) : -dot(diff, e.n);
+        if (pen <= 0.0f) {
+            C.px[buf][kept] = p.x; C.py[buf][kept] = p.y; C.pz[buf][kept] = p.z;
+            C.dep[kept] = -pen;
+            ++kept;
+        }
+    }
+    if (!kept) {
+        m.np = 1; m.p[0] = e.p; m.dep[0] = e.depth;
+        return;
+    }
+    m.np = reduce4_area(C, buf, kept, m);
+}
+
+// ------------------------------------------------------------------ kernels
+__device__ __forceinline__ void load_side(Side &s, const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf) {
+    const int kind = C.meta[c].x;
+    const float4 d = C.dims[c];
+    s.kind = kind;
+    s.a = d.x; s.b = d.y; s.c = d.z;
+    s.tf = tf;
+    s.wh.nv = 0;
+    s.wh.v0 = 0;
+    if (kind == SH_HULL) load_wave_hull(s.wh, hs, (uint32_t)C.hull[c]);
+}
+
+// GJK + EPA + manifold for one pair of the hull-hull / hull-primitive bins
+// (ContactManifoldGenerator.cpp:415-438).  One 64-lane workgroup per pair.
+// gstat (optional): per slot, GJK iterations | EPA iterations << 8 | GJK status << 16 |
+// EPA status << 20 (EPA status 15: not run).  guess (optional): GJK seed direction per slot.
+__global__ void __launch_bounds__(64) k_narrow_gjk(const uint32_t *bin_slot, uint32_t off, Colliders C, Bodies B,
+                                                   Contacts K, HullSet hs, uint32_t *gstat, const float4 *guess) {
+    __shared__ PairLds L;
+    const uint32_t s = bin_slot[off + blockIdx.x];
+    const uint2 cc = K.cc[s], bb = K.bb[s];
+    const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+    Side A, Bs;
+    load_side(A, C, hs, cc.x, ta);
+    load_side(Bs, C, hs, cc.y, tb);
+    V3 g = tb.pos - ta.pos;
+    if (guess) g = xyz(guess[s]);
+    else if (len2(g) < EPS) g = V3{1.0f, 0.0f, 0.0f};
+    Simplex sx;
+    int giters = 0;
+    const int gs = gjk(A, Bs, hs, g, sx, giters);
+    EpaResult e{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
+    if (gs == GJK_INSIDE) e = epa(sx, A, Bs, hs, L.e);
+    if (gstat && lane64() == 0)
+        gstat[s] = (uint32_t)giters | ((uint32_t)e.iters << 8) | ((uint32_t)gs << 16) |
+                   ((uint32_t)(gs == GJK_INSIDE ? e.status : 15) << 20);
+    Manifold m;
+    m.np = 0;
+    if (gs == GJK_INSIDE && e.status == EPA_VALID) {
+        const bool adj = A.kind == SH_HULL && Bs.kind == SH_HULL && hs.meta[C.hull[cc.x]].adj && hs.meta[C.hull[cc.y]].adj;
+        if (adj) {
+            __syncthreads();   // the manifold buffers overlay the polytope
+            manifold_hulls(e, A, Bs, hs, (uint32_t)C.hull[cc.x], (uint32_t)C.hull[cc.y], L.c, m);
+        } else {
+            m.n = e.n;
+            m.np = 1;
+            m.p[0] = e.p;
+            m.dep[0] = e.depth;
+        }
+    }
+    if (lane64() != 0) return;
+    if (m.np > 0) emit(m, B, K, s, bb);
+    else K.cnt[s] = 0;
+}
+
+// Hull against a plane (PlaneCollision.hpp:90-285): deepest support, then up to 32
+// vertices within 1e-4 of the plane in vertex order, reduced to 4 by area.
+__global__ void __launch_bounds__(64) k_narrow_hull_plane(const uint32_t *bin_slot, uint32_t off, Colliders C, Bodies B,
+                                                          Contacts K, HullSet hs) {
+    __shared__ float tx[HP_MAX_POINTS], ty[HP_MAX_POINTS], tz[HP_MAX_POINTS], td[HP_MAX_POINTS];
+    const int lane = lane64();
+    const uint32_t s = bin_slot[off + blockIdx.x];
+    const uint2 cc = K.cc[s], bb = K.bb[s];
+    const TF ta = collider_tf(C, B, cc.x, bb.x), tp = collider_tf(C, B, cc.y, bb.y);
+    WaveHull wh;
+    load_wave_hull(wh, hs, (uint32_t)C.hull[cc.x]);
+    const float4 pd = C.dims[cc.y];
+    const V3 wn = tf_vec(tp, V3{pd.x, pd.y, pd.z});   // Plane.hpp:47-57
+    const float wd = pd.w + dot(wn, tp.pos);
+    const V3 ln = tf_inv_vec(ta, wn);
+    const float ld = wd - dot(ta.pos, wn);
+    const V3 deep = hull_vertex(hs, wh, wave_support_index(wh, -ln));
+    if (dot(deep, ln) - ld > 1e-4f) {
+        if (lane == 0) K.cnt[s] = 0;
+        return;
+    }
+    int tc = 0;
+#pragma unroll
+    for (int k = 0; k < HULL_VPL; ++k) {
+        const int i = lane + 64 * k;
+        float dist = 0.0f;
+        bool pen = false;
+        if (i < wh.nv) {
+            dist = wh.x[k] * ln.x + wh.y[k] * ln.y + wh.z[k] * ln.z - ld;
+            pen = dist <= 1e-4f;
+        }
+        const uint64_t m = __ballot(pen);
+        const int slot = tc + (int)lane_rank(m);
+        if (pen && slot < HP_MAX_POINTS) {
+            tx[slot] = wh.x[k]; ty[slot] = wh.y[k]; tz[slot] = wh.z[k]; td[slot] = -dist;
+        }
+        tc += __popcll(m);
+    }
+    if (tc > HP_MAX_POINTS) tc = HP_MAX_POINTS;
+    __syncthreads();
+    if (lane != 0) return;
+    if (tc == 0) { K.cnt[s] = 0; return; }
+    auto P = [&](int i) { return V3{tx[i], ty[i], tz[i]}; };
+    Manifold m;
+    m.n = -wn;
+    if (tc <= 4) {
+        for (int i = 0; i < tc; ++i) { m.p[i] = tf_point(ta, P(i)); m.dep[i] = td[i]; }
+        m.np = tc;
+        emit(m, B, K, s, bb);
+        return;
+    }
+    const float low = -1e18f;
+    int idx[4] = {-1, -1, -1, -1};
+    float best = low;
+    for (int i = 0; i < tc; ++i) if (td[i] > best) { best = td[i]; idx[0] = i; }
+    best = low;
+    for (int i = 0; i < tc; ++i) {
+        if (i == idx[0]) continue;
+        const V3 d = P(i) - P(idx[0]);
+        const float x = dot(d, d);
+        if (x > best) { best = x; idx[1] = i; }
+    }
+    best = low;
+    {
+        const V3 e = P(idx[1]) - P(idx[0]);
+        for (int i = 0; i < tc; ++i) {
+            if (i == idx[0] || i == idx[1]) continue;
+            const V3 c = cross(e, P(i) - P(idx[0]));
+            const float x = dot(c, c);
+            if (x > best) { best = x; idx[2] = i; }
+        }
+    }
+    best = low;
+    {
+        const V3 p0 = P(idx[0]), p1 = P(idx[1]), p2 = P(idx[2]);
+        const V3 e01 = p1 - p0, e12 = p2 - p1, e20 = p0 - p2;
+        for (int i = 0; i < tc; ++i) {
+            if (i == idx[0] || i == idx[1] || i == idx[2]) continue;
+            const V3 x = P(i);
+            const float a0 = dot(cross(e01, x - p0), ln);
+            const float a1 = dot(cross(e12, x - p1), ln);
+            const float a2 = dot(cross(e20, x - p2), ln);
+            float sc = a0;
+            if (a1 > sc) sc = a1;
+            if (a2 > sc) sc = a2;
+            if (sc > best) { best = sc; idx[3] = i; }
+        }
+    }
+    const int ne = (idx[3] >= 0) ? 4 : 3;
+    for (int i = 0; i < ne; ++i) { m.p[i] = tf_point(ta, P(idx[i])); m.dep[i] = td[idx[i]]; }
+    m.np = ne;
+    emit(m, B, K, s, bb);
+}
+
+}  // namespace rb

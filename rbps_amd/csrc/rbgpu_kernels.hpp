@@ -19,8 +19,10 @@ namespace rb {
 
 constexpr uint32_t BP_INVALID = 0xffffffffu;
 constexpr int MAX_COLOURS = 1024;
-constexpr int NBINS = 10;
-enum Bin { BIN_SS = 0, BIN_SB, BIN_BB, BIN_SC, BIN_CC, BIN_CB, BIN_SP, BIN_BP, BIN_CP, BIN_OTHER };
+constexpr int NBINS = 13;
+// pair-type bins: analytic pairs, shape-plane, hull-hull, hull with a sphere / box /
+// capsule (either order), hull-plane
+enum Bin { BIN_SS = 0, BIN_SB, BIN_BB, BIN_SC, BIN_CC, BIN_CB, BIN_SP, BIN_BP, BIN_CP, BIN_HH, BIN_HX, BIN_HP, BIN_OTHER };
 
 struct Bodies {
     float4 *x, *q, *v, *w, *px, *pq, *f, *t, *I0, *I1, *Ii0, *Ii1;
@@ -28,6 +30,23 @@ struct Bodies {
 struct Colliders {
     int4 *meta;      // kind, body, is_static, bp slot
     float4 *dims, *lpos, *lrot, *mat;   // mat: restitution, static friction, dynamic friction
+    int *hull;       // hull index of a hull collider (-1 otherwise)
+};
+
+// Convex hull tables of a world (built on the host from the caller's vertices and
+// triangles; see rbgpu_hull.hpp).
+struct HullMeta {   // 64 bytes; table offsets index HullSet::tab
+    uint32_t v0, nv, pn0, np;
+    uint32_t poff, pidx, v2p_o, v2p_i;
+    uint32_t v2n_o, v2n_i, p2p_o, p2p_i;
+    uint32_t adj, pad0, pad1, pad2;
+};
+struct HullSet {
+    const float4 *v;       // local vertices (x y z 0), all hulls
+    const float4 *pn;      // polygon normals
+    const uint32_t *tab;   // polygon loops and adjacency lists (CSR, hull-local indices)
+    const HullMeta *meta;
+    const float4 *amin, *amax;   // local AABB
 };
 struct Contacts {
     int *cnt;        // points per slot this substep
@@ -55,14 +74,21 @@ __device__ __forceinline__ TF collider_tf(const Colliders &C, const Bodies &B, u
 // ------------------------------------------------------------------ broadphase
 // Refit of one collider's swept AABB with fat-AABB hysteresis
 // (CollisionPipeline.cpp:14-53, BroadPhase.hpp:228-249, BroadPhase.cpp:211-232).
-__global__ void k_refit(Colliders C, Bodies B, uint32_t M, float dt, float margin, float reinsert, float4 *fmin,
-                        float4 *fmax, uint32_t *ep_key, const uint2 *ep_pos, int *dirty) {
+__global__ void k_refit(Colliders C, Bodies B, HullSet hs, uint32_t M, float dt, float margin, float reinsert,
+                        float4 *fmin, float4 *fmax, uint32_t *ep_key, const uint2 *ep_pos, int *dirty) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     int4 m = C.meta[i];
     if (m.z != 0 || (uint32_t)m.w == BP_INVALID) return;
     const uint32_t body = (uint32_t)m.y, slot = (uint32_t)m.w;
-    Box3 t = shape_aabb(shape_of(m.x, C.dims[i]), collider_tf(C, B, i, body));
+    const TF tf = collider_tf(C, B, i, body);
+    Box3 t;
+    if (m.x == SH_HULL) {
+        const float4 a = hs.amin[C.hull[i]], b = hs.amax[C.hull[i]];
+        t = hull_aabb(V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, tf);
+    } else {
+        t = shape_aabb(shape_of(m.x, C.dims[i]), tf);
+    }
     V3 dv = xyz(B.v[body]) * dt;
     V3 emn{margin + (dv.x < 0.0f ? -dv.x : 0.0f), margin + (dv.y < 0.0f ? -dv.y : 0.0f), margin + (dv.z < 0.0f ? -dv.z : 0.0f)};
     V3 emx{margin + (dv.x > 0.0f ? dv.x : 0.0f), margin + (dv.y > 0.0f ? dv.y : 0.0f), margin + (dv.z > 0.0f ? dv.z : 0.0f)};
@@ -169,6 +195,10 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {   // lowbias32 (C. Wello
 }
 
 __device__ __forceinline__ int bin_of(int ka, int kb) {
+    if (ka == SH_HULL || kb == SH_HULL) {
+        if (ka == SH_PLANE || kb == SH_PLANE) return kb == SH_PLANE ? BIN_HP : BIN_OTHER;   // plane always B
+        return (ka == SH_HULL && kb == SH_HULL) ? BIN_HH : BIN_HX;
+    }
     if (kb == SH_PLANE) return ka == SH_SPHERE ? BIN_SP : ka == SH_BOX ? BIN_BP : ka == SH_CAPSULE ? BIN_CP : BIN_OTHER;
     if (ka > kb) { int t = ka; ka = kb; kb = t; }
     if (ka == SH_SPHERE && kb == SH_SPHERE) return BIN_SS;

@@ -16,6 +16,8 @@
 #include "../../include/rbgpu.h"
 #include "rbgpu_kernels.hpp"
 #include "rbgpu_joints.hpp"
+#include "rbgpu_gjk.hpp"
+#include "rbgpu_hull.hpp"
 
 using namespace rb;
 
@@ -81,6 +83,11 @@ struct rbgpu_world {
     std::vector<rbgpu_body_desc> hb;
     std::vector<rbgpu_collider_desc> hc;
     std::vector<rbgpu_joint_desc> hj;
+    struct HostHull {
+        rbh::HullTables t;
+        float amin[3], amax[3];   // local AABB of the FP32 vertices
+    };
+    std::vector<HostHull> hh;           // convex hull data (rbgpu_add_hull)
     std::vector<float4> h_jtg;          // joint target / speed / current (survives rebuilds)
     std::vector<float> h_jlam;          // joint row lambdas (never reset)
     std::vector<float4> h_fmin, h_fmax;
@@ -93,6 +100,12 @@ struct rbgpu_world {
     uint32_t *cscene = nullptr, *slot_scene = nullptr, *sap_start = nullptr, *sap_end = nullptr;
     uint32_t *plane_lid = nullptr, *pair_lid = nullptr;
     uint32_t *bb_hits = nullptr, *bb_blk = nullptr;   // box-box overlap lists (per 128-pair block)
+    // hull tables on the device and GJK/EPA statistics per pair slot
+    float4 *hv = nullptr, *hpn = nullptr, *hamin = nullptr, *hamax = nullptr;
+    uint32_t *htab = nullptr;
+    HullMeta *hmeta = nullptr;
+    HullSet hs{};
+    uint32_t *gstat = nullptr;
     Joints Jt{};
     uint32_t *jorder = nullptr;         // joints by colour
     std::vector<uint32_t> jcol_off;     // joint colour offsets
@@ -146,7 +159,7 @@ struct rbgpu_world {
 
     void release_pairs() {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
-        dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk);
+        dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
@@ -157,7 +170,9 @@ struct rbgpu_world {
         release_pairs();
         dfree(B.x); dfree(B.q); dfree(B.v); dfree(B.w); dfree(B.px); dfree(B.pq); dfree(B.f); dfree(B.t);
         dfree(B.I0); dfree(B.I1); dfree(B.Ii0); dfree(B.Ii1);
-        dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat);
+        dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat); dfree(C.hull);
+        dfree(hv); dfree(hpn); dfree(hamin); dfree(hamax); dfree(htab); dfree(hmeta);
+        hs = HullSet{};
         dfree(Jt.jm); dfree(Jt.jr1); dfree(Jt.jr2); dfree(Jt.jax); dfree(Jt.jlx); dfree(Jt.jtg); dfree(Jt.jrow);
         dfree(Jt.jlam); dfree(jorder);
         dfree(fmin); dfree(fmax); dfree(uid); dfree(ep_key[0]); dfree(ep_key[1]); dfree(ep_val[0]); dfree(ep_val[1]);
@@ -205,6 +220,33 @@ struct rbgpu_world {
         return RBGPU_OK;
     }
 
+    // Narrowphase of every bin of the current pair list (bin_off): one launch per bin.
+    void launch_narrowphase() {
+        for (int b = 0; b < NBINS; ++b) {
+            uint32_t o = bin_off[b], n = bin_off[b + 1] - bin_off[b];
+            if (!n) continue;
+            const uint32_t g = nblk(n, 128);
+            switch (b) {
+                case BIN_SS: k_narrow<BIN_SS><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_SB: k_narrow<BIN_SB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_BB:
+                    k_narrow_bb_sat<<<g, 128, 0, st>>>(bin_list, o, n, C, B, K, bb_hits, bb_blk);
+                    k_narrow_bb_full<<<g, 128, 0, st>>>(bb_hits, bb_blk, C, B, K);
+                    break;
+                case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_SP: k_narrow<BIN_SP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_HH:
+                case BIN_HX: k_narrow_gjk<<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs, gstat, nullptr); break;
+                case BIN_HP: k_narrow_hull_plane<<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs); break;
+                default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+            }
+        }
+    }
+
     hipEvent_t *timed_begin(int kernel) {
         if (timing_kernel != kernel || frames % timing_stride) return nullptr;
         if (ev_used == ev_pool.size()) {
@@ -238,7 +280,7 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     rc |= dalloc(pairs, cap);
     rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
     rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap);
-    rc |= dalloc(bb_hits, nblk(cap, 128) * 128); rc |= dalloc(bb_blk, nblk(cap, 128));
+    rc |= dalloc(bb_hits, nblk(cap, 128) * 128); rc |= dalloc(bb_blk, nblk(cap, 128)); rc |= dalloc(gstat, cap);
     rc |= dalloc(cb.acol, cap); rc |= dalloc(cb.tent, cap); rc |= dalloc(cb.rest, cap); rc |= dalloc(cb.act, cap);
     rc |= dalloc(cb.blk, nblk(cap) + 1);
     uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
@@ -278,6 +320,7 @@ int rbgpu_world::build() {
     }
     std::vector<int4> meta(m);
     std::vector<float4> dims(m), lpos(m), lrot(m), mat(m);
+    std::vector<int> chull(m);
     std::vector<uint2> planes;
     uint32_t slot = 0;
     for (uint32_t c = 0; c < m; ++c) {
@@ -289,6 +332,31 @@ int rbgpu_world::build() {
         lpos[c] = make_float4((float)d.local_pos[0], (float)d.local_pos[1], (float)d.local_pos[2], 0.0f);
         lrot[c] = make_float4((float)d.local_rot[0], (float)d.local_rot[1], (float)d.local_rot[2], (float)d.local_rot[3]);
         mat[c] = make_float4((float)d.restitution, (float)d.static_friction, (float)d.dynamic_friction, 0.0f);
+        chull[c] = d.shape == RBGPU_HULL ? d.hull_index : -1;
+    }
+    // hull tables, concatenated: vertices, polygon normals (rounded from double), index tables
+    std::vector<float4> hvv, hpnv, hminv, hmaxv;
+    std::vector<uint32_t> htabv;
+    std::vector<HullMeta> hmv;
+    for (const HostHull &h : hh) {
+        HullMeta mt{};
+        mt.v0 = (uint32_t)hvv.size();
+        mt.nv = (uint32_t)h.t.v.size();
+        mt.pn0 = (uint32_t)hpnv.size();
+        mt.np = (uint32_t)h.t.pn.size();
+        for (const auto &p : h.t.v) hvv.push_back(make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f));
+        for (const auto &p : h.t.pn) hpnv.push_back(make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f));
+        auto put = [&](const std::vector<uint32_t> &a) {
+            const uint32_t o = (uint32_t)htabv.size();
+            htabv.insert(htabv.end(), a.begin(), a.end());
+            return o;
+        };
+        mt.poff = put(h.t.poff); mt.pidx = put(h.t.pidx); mt.v2p_o = put(h.t.v2p_o); mt.v2p_i = put(h.t.v2p_i);
+        mt.v2n_o = put(h.t.v2n_o); mt.v2n_i = put(h.t.v2n_i); mt.p2p_o = put(h.t.p2p_o); mt.p2p_i = put(h.t.p2p_i);
+        mt.adj = h.t.adjacency() ? 1u : 0u;
+        hmv.push_back(mt);
+        hminv.push_back(make_float4(h.amin[0], h.amin[1], h.amin[2], 0.0f));
+        hmaxv.push_back(make_float4(h.amax[0], h.amax[1], h.amax[2], 0.0f));
     }
     // Plane loop of the narrowphase driver (CollisionPipeline.cpp:156-197): constant per scene;
     // a plane pairs with the colliders of its own scene.  plane_lid: index of the pair among
@@ -328,6 +396,9 @@ int rbgpu_world::build() {
     rc |= dalloc(B.pq, n); rc |= dalloc(B.f, n); rc |= dalloc(B.t, n); rc |= dalloc(B.I0, n); rc |= dalloc(B.I1, n);
     rc |= dalloc(B.Ii0, n); rc |= dalloc(B.Ii1, n);
     rc |= dalloc(C.meta, m); rc |= dalloc(C.dims, m); rc |= dalloc(C.lpos, m); rc |= dalloc(C.lrot, m); rc |= dalloc(C.mat, m);
+    rc |= dalloc(C.hull, m);
+    rc |= dalloc(hv, hvv.size()); rc |= dalloc(hpn, hpnv.size()); rc |= dalloc(htab, htabv.size());
+    rc |= dalloc(hmeta, hmv.size()); rc |= dalloc(hamin, hminv.size()); rc |= dalloc(hamax, hmaxv.size());
     rc |= dalloc(fmin, S); rc |= dalloc(fmax, S); rc |= dalloc(uid, S);
     rc |= dalloc(ep_key[0], NEP); rc |= dalloc(ep_key[1], NEP); rc |= dalloc(ep_val[0], NEP); rc |= dalloc(ep_val[1], NEP);
     rc |= dalloc(ep_pos, S); rc |= dalloc(sw_meta, NEP); rc |= dalloc(sw_a, NEP); rc |= dalloc(sw_b, NEP);
@@ -353,7 +424,17 @@ int rbgpu_world::build() {
     if (m) {
         HIPCHK(up(C.meta, meta.data(), m * 16)); HIPCHK(up(C.dims, dims.data(), m * 16));
         HIPCHK(up(C.lpos, lpos.data(), m * 16)); HIPCHK(up(C.lrot, lrot.data(), m * 16)); HIPCHK(up(C.mat, mat.data(), m * 16));
+        HIPCHK(up(C.hull, chull.data(), m * 4));
     }
+    if (!hmv.empty()) {
+        HIPCHK(up(hv, hvv.data(), hvv.size() * 16));
+        if (!hpnv.empty()) HIPCHK(up(hpn, hpnv.data(), hpnv.size() * 16));
+        if (!htabv.empty()) HIPCHK(up(htab, htabv.data(), htabv.size() * 4));
+        HIPCHK(up(hmeta, hmv.data(), hmv.size() * sizeof(HullMeta)));
+        HIPCHK(up(hamin, hminv.data(), hminv.size() * 16));
+        HIPCHK(up(hamax, hmaxv.data(), hmaxv.size() * 16));
+    }
+    hs = HullSet{hv, hpn, htab, hmeta, hamin, hamax};
     if (S) {
         HIPCHK(up(fmin, h_fmin.data(), S * 16)); HIPCHK(up(fmax, h_fmax.data(), S * 16)); HIPCHK(up(uid, h_uid.data(), S * 4));
         HIPCHK(up(ep_key[0], h_key.data(), NEP * 4)); HIPCHK(up(ep_val[0], h_val.data(), NEP * 4));
@@ -483,7 +564,7 @@ int rbgpu_world::frame() {
 
     // 1. refit (CollisionPipeline.cpp:14-53)
     hipEvent_t *ebp = timed_begin(6);
-    if (M) k_refit<<<nblk(M), TPB, 0, st>>>(C, B, M, dt, margin, reinsert, fmin, fmax, ep_key[0], ep_pos, dirty);
+    if (M) k_refit<<<nblk(M), TPB, 0, st>>>(C, B, hs, M, dt, margin, reinsert, fmin, fmax, ep_key[0], ep_pos, dirty);
     // 2. stable sort of the endpoints if any fat box changed (BroadPhase.cpp:234-245)
     if (NEP) {
         for (int pass = 0; pass < 4; ++pass) {
@@ -549,26 +630,7 @@ int rbgpu_world::frame() {
         runs.clear();
         if (P) {
             hipEvent_t *e = timed_begin(3);
-            for (int b = 0; b < NBINS; ++b) {
-                uint32_t o = bin_off[b], n = bin_off[b + 1] - bin_off[b];
-                if (!n) continue;
-                const uint32_t g = nblk(n, 128);
-                switch (b) {
-                    case BIN_SS: k_narrow<BIN_SS><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    case BIN_SB: k_narrow<BIN_SB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    case BIN_BB:
-                        k_narrow_bb_sat<<<g, 128, 0, st>>>(bin_list, o, n, C, B, K, bb_hits, bb_blk);
-                        k_narrow_bb_full<<<g, 128, 0, st>>>(bb_hits, bb_blk, C, B, K);
-                        break;
-                    case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    case BIN_SP: k_narrow<BIN_SP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                    default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                }
-            }
+            launch_narrowphase();
             timed_end(e);
             // colouring of the pairs with points (conflict = a shared dynamic body)
             hipEvent_t *ec = timed_begin(7);
@@ -716,8 +778,11 @@ int rbgpu_add_colliders(rbgpu_world *w, int n, const rbgpu_collider_desc *desc, 
     for (int i = 0; i < n; ++i) {
         const rbgpu_collider_desc &d = desc[i];
         if (d.body_id >= w->hb.size()) return fail(RBGPU_ERR_INVALID, "collider body_id out of range");
-        if (d.shape != RBGPU_SPHERE && d.shape != RBGPU_BOX && d.shape != RBGPU_CAPSULE && d.shape != RBGPU_PLANE)
+        if (d.shape != RBGPU_SPHERE && d.shape != RBGPU_BOX && d.shape != RBGPU_CAPSULE && d.shape != RBGPU_PLANE &&
+            d.shape != RBGPU_HULL)
             return fail(RBGPU_ERR_UNSUPPORTED, "shape kind not supported by this build");
+        if (d.shape == RBGPU_HULL && (d.hull_index < 0 || (size_t)d.hull_index >= w->hh.size()))
+            return fail(RBGPU_ERR_INVALID, "hull collider: hull_index does not name a hull of this world");
         const uint32_t id = (uint32_t)w->hc.size();
         w->hc.push_back(d);
         if (ids_out) ids_out[i] = id;
@@ -730,7 +795,13 @@ int rbgpu_add_colliders(rbgpu_world *w, int n, const rbgpu_collider_desc *desc, 
         Q lr{(float)d.local_rot[0], (float)d.local_rot[1], (float)d.local_rot[2], (float)d.local_rot[3]};
         TF tf{bt.pos + rotate(bt.rot, lp), qmul(bt.rot, lr)};
         Shape s{d.shape, (float)d.dims[0], (float)d.dims[1], (float)d.dims[2], (float)d.dims[3]};
-        Box3 t = shape_aabb(s, tf);
+        Box3 t;
+        if (d.shape == RBGPU_HULL) {
+            const auto &h = w->hh[d.hull_index];
+            t = hull_aabb(V3{h.amin[0], h.amin[1], h.amin[2]}, V3{h.amax[0], h.amax[1], h.amax[2]}, tf);
+        } else {
+            t = shape_aabb(s, tf);
+        }
         const float mg = w->margin;
         V3 mn = t.mn - V3{mg, mg, mg}, mx = t.mx + V3{mg, mg, mg};
         const uint32_t slot = w->S++;
@@ -1088,6 +1159,132 @@ int rbgpu_collide_pairs(int n, const int32_t *kind_a, const double *dims_a, cons
     if (rc) return rc;
     if (e != hipSuccess) return fail(RBGPU_ERR_HIP, std::string("rbgpu_collide_pairs: ") + hipGetErrorString(e));
     for (size_t k = 0; k < fo.size(); ++k) out[k] = fo[k];
+    return RBGPU_OK;
+}
+
+// ---------------------------------------------------------------- convex hulls
+namespace {
+int check_hull_input(const double *v, uint32_t nv, const uint32_t *tri, uint32_t nt) {
+    if (!v || nv == 0) return fail(RBGPU_ERR_INVALID, "hull needs at least one vertex");
+    if (nv > (uint32_t)HULL_MAX_VERTS) return fail(RBGPU_ERR_CAPACITY, "hull has more than 256 vertices");
+    if (nt && !tri) return fail(RBGPU_ERR_INVALID, "hull triangles missing");
+    for (uint32_t i = 0; i < 3 * nt; ++i)
+        if (tri[i] >= nv) return fail(RBGPU_ERR_INVALID, "hull triangle index out of range");
+    return RBGPU_OK;
+}
+}  // namespace
+
+int rbgpu_add_hull(rbgpu_world *w, const double *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                   uint32_t n_triangles, uint32_t *hull_id_out) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    int rc = check_hull_input(vertices, n_vertices, triangles, n_triangles);
+    if (rc) return rc;
+    rbgpu_world::HostHull h;
+    h.t = rbh::prepare_hull(vertices, n_vertices, triangles, n_triangles);
+    const uint32_t P = (uint32_t)h.t.pn.size();
+    for (uint32_t p = 0; p < P; ++p) {
+        if (h.t.poff[p + 1] - h.t.poff[p] > (uint32_t)HULL_MAX_POLY)
+            return fail(RBGPU_ERR_CAPACITY, "hull polygon with more than 64 vertices");
+        if (h.t.p2p_o[p + 1] - h.t.p2p_o[p] > (uint32_t)HULL_MAX_POLY)
+            return fail(RBGPU_ERR_CAPACITY, "hull polygon with more than 64 neighbours");
+    }
+    for (int k = 0; k < 3; ++k) { h.amin[k] = 1e30f; h.amax[k] = -1e30f; }
+    for (const auto &p : h.t.v) {
+        const float c[3] = {(float)p.x, (float)p.y, (float)p.z};
+        for (int k = 0; k < 3; ++k) {
+            h.amin[k] = c[k] < h.amin[k] ? c[k] : h.amin[k];
+            h.amax[k] = h.amax[k] < c[k] ? c[k] : h.amax[k];
+        }
+    }
+    rc = w->download_state();
+    if (rc) return rc;
+    if (hull_id_out) *hull_id_out = (uint32_t)w->hh.size();
+    w->hh.push_back(std::move(h));
+    w->built = false;
+    return RBGPU_OK;
+}
+
+int rbgpu_hull_mass_properties(const double *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                               uint32_t n_triangles, double *volume, double *inertia6) {
+    int rc = check_hull_input(vertices, n_vertices, triangles, n_triangles);
+    if (rc) return rc;
+    const rbh::HullTables t = rbh::prepare_hull(vertices, n_vertices, nullptr, 0);
+    double vol, I[6];
+    rbh::hull_mass(t, triangles, n_triangles, vol, I);
+    if (volume) *volume = vol;
+    if (inertia6) for (int k = 0; k < 6; ++k) inertia6[k] = I[k];
+    return RBGPU_OK;
+}
+
+int rbgpu_hull_tables(const double *vertices, uint32_t n_vertices, const uint32_t *triangles, uint32_t n_triangles,
+                      uint32_t *sizes6, double *poly_normals, uint32_t *poly_offsets, uint32_t *poly_indices,
+                      uint32_t *v2p_offsets, uint32_t *v2p_indices, uint32_t *v2n_offsets, uint32_t *v2n_indices,
+                      uint32_t *p2p_offsets, uint32_t *p2p_indices) {
+    int rc = check_hull_input(vertices, n_vertices, triangles, n_triangles);
+    if (rc) return rc;
+    const rbh::HullTables t = rbh::prepare_hull(vertices, n_vertices, triangles, n_triangles);
+    if (sizes6) {
+        sizes6[0] = (uint32_t)t.pn.size(); sizes6[1] = (uint32_t)t.pidx.size(); sizes6[2] = (uint32_t)t.v2p_i.size();
+        sizes6[3] = (uint32_t)t.v2n_i.size(); sizes6[4] = (uint32_t)t.p2p_i.size(); sizes6[5] = t.adjacency() ? 1u : 0u;
+    }
+    if (poly_normals)
+        for (size_t k = 0; k < t.pn.size(); ++k) {
+            poly_normals[3 * k] = t.pn[k].x; poly_normals[3 * k + 1] = t.pn[k].y; poly_normals[3 * k + 2] = t.pn[k].z;
+        }
+    auto cp = [](const std::vector<uint32_t> &a, uint32_t *o) { if (o) std::copy(a.begin(), a.end(), o); };
+    cp(t.poff, poly_offsets); cp(t.pidx, poly_indices); cp(t.v2p_o, v2p_offsets); cp(t.v2p_i, v2p_indices);
+    cp(t.v2n_o, v2n_offsets); cp(t.v2n_i, v2n_indices); cp(t.p2p_o, p2p_offsets); cp(t.p2p_i, p2p_indices);
+    return RBGPU_OK;
+}
+
+int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2, double *out32, uint32_t *gjk_stats) {
+    if (!w || (n && (!pairs2 || !out32))) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (!w->built) {
+        int rc = w->build();
+        if (rc) return rc;
+    }
+    for (uint32_t i = 0; i < 2 * n; ++i)
+        if (pairs2[i] >= w->M) return fail(RBGPU_ERR_INVALID, "pair collider out of range");
+    if (!n) return RBGPU_OK;
+    int rc = w->ensure_pair_capacity(n);
+    if (rc) return rc;
+    hipStream_t st = w->st;
+    HIPCHK(hipMemcpyAsync(w->pairs, pairs2, (size_t)n * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(w->scal + 1, &n, 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(w->gstat, 0xff, (size_t)n * 4, st));
+    k_pair_gather<<<nblk(n), TPB, 0, st>>>(w->pairs, w->scal + 1, w->C, w->K, w->ka, w->va);
+    rc = w->radix_pass(w->ka, w->va, w->kb, w->bin_list, w->pcap, 0, w->p_hist, nullptr, w->scal + 1);
+    if (rc) return rc;
+    k_offsets<<<1, 64, 0, st>>>(w->p_hist, (w->pcap + RS_TILE - 1) / RS_TILE, NBINS, w->scal + 1, w->offs);
+    std::vector<uint32_t> off(NBINS + 1);
+    HIPCHK(hipMemcpyAsync(off.data(), w->offs, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    w->bin_off = off;
+    w->launch_narrowphase();
+    HIPCHK(hipGetLastError());
+    std::vector<int> cnt(n);
+    std::vector<float4> nr(n), pa(4 * (size_t)n), pb(4 * (size_t)n), rbv(4 * (size_t)n);
+    std::vector<uint32_t> gs(n);
+    HIPCHK(hipMemcpyAsync(cnt.data(), w->K.cnt, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nr.data(), w->K.nr, (size_t)n * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(pa.data(), w->K.pa, (size_t)n * 64, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(pb.data(), w->K.pb, (size_t)n * 64, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(rbv.data(), w->K.rb, (size_t)n * 64, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(gs.data(), w->gstat, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < n; ++i) {
+        double *o = out32 + (size_t)i * 32;
+        for (int k = 0; k < 32; ++k) o[k] = 0.0;
+        o[0] = cnt[i];
+        if (!cnt[i]) continue;
+        o[1] = nr[i].x; o[2] = nr[i].y; o[3] = nr[i].z;
+        for (int k = 0; k < cnt[i]; ++k) {
+            const float4 a = pa[4 * i + k], b = pb[4 * i + k];
+            double *q = o + 4 + 7 * k;
+            q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = b.x; q[4] = b.y; q[5] = b.z; q[6] = rbv[4 * i + k].w;
+        }
+    }
+    if (gjk_stats) std::copy(gs.begin(), gs.end(), gjk_stats);
     return RBGPU_OK;
 }
 

@@ -30,6 +30,7 @@ EXPORTS = [
     "rbgpu_set_step", "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
     "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_collide_pairs",
     "rbgpu_set_option", "rbgpu_get_option", "rbgpu_add_joints", "rbgpu_get_joints", "rbgpu_set_joint_targets",
+    "rbgpu_add_hull", "rbgpu_hull_mass_properties", "rbgpu_hull_tables", "rbgpu_collide_world_pairs",
     "rbgpu_last_error",
 ]
 
@@ -40,7 +41,7 @@ class RbgpuError(RuntimeError):
 
 class Stats(C.Structure):
     _fields_ = [("frames", C.c_uint32), ("sap_pairs", C.c_uint32), ("pairs", C.c_uint32), ("colours", C.c_uint32),
-                ("colour_rounds", C.c_uint32), ("contacts", C.c_uint32), ("bins", C.c_uint32 * 10),
+                ("colour_rounds", C.c_uint32), ("contacts", C.c_uint32), ("bins", C.c_uint32 * 16),
                 ("active_pairs", C.c_uint32)]
 
 
@@ -75,6 +76,10 @@ def lib():
         L.rbgpu_get_option.argtypes = [_P, C.c_int, _I]
         L.rbgpu_get_timing.argtypes = [_P, _D, C.POINTER(C.c_uint64), _D]
         L.rbgpu_collide_pairs.argtypes = [C.c_int, _I, _D, _D, _I, _D, _D, _D]
+        L.rbgpu_add_hull.argtypes = [_P, _D, C.c_uint32, _U, C.c_uint32, _U]
+        L.rbgpu_hull_mass_properties.argtypes = [_D, C.c_uint32, _U, C.c_uint32, _D, _D]
+        L.rbgpu_hull_tables.argtypes = [_D, C.c_uint32, _U, C.c_uint32, _U, _D] + [_U] * 8
+        L.rbgpu_collide_world_pairs.argtypes = [_P, C.c_uint32, _U, _D, _U]
         L.rbgpu_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -101,6 +106,39 @@ def collide_pairs(kind_a, dims_a, tf_a, kind_b, dims_b, tf_b) -> np.ndarray:
     return out
 
 
+def _hull_arrays(verts, faces):
+    v = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+    f = None if faces is None else np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+    return v, f
+
+
+def hull_mass_properties(verts, faces=None):
+    """(volume, inertia xx yy zz xy xz yz) at unit density, ConvexHull.hpp:575-745."""
+    v, f = _hull_arrays(verts, faces)
+    vol, inert = np.zeros(1), np.zeros(6)
+    _check(lib().rbgpu_hull_mass_properties(_ptr(v, _D), len(v), _ptr(f, _U), 0 if f is None else len(f),
+                                            _ptr(vol, _D), _ptr(inert, _D)))
+    return float(vol[0]), inert
+
+
+def hull_tables(verts, faces=None) -> dict:
+    """Merged polygons and adjacency tables as rbgpu_add_hull builds them."""
+    v, f = _hull_arrays(verts, faces)
+    nf = 0 if f is None else len(f)
+    sz = np.zeros(6, np.uint32)
+    _check(lib().rbgpu_hull_tables(_ptr(v, _D), len(v), _ptr(f, _U), nf, _ptr(sz, _U), *([None] * 9)))
+    P, NI, N2P, N2N, NP2P = (int(x) for x in sz[:5])
+    nv = len(v)
+    out = {"pn": np.zeros((P, 3)), "poff": np.zeros(P + 1 if P else 0, np.uint32), "pidx": np.zeros(NI, np.uint32),
+           "v2p_o": np.zeros(nv + 1 if P else 0, np.uint32), "v2p_i": np.zeros(N2P, np.uint32),
+           "v2n_o": np.zeros(nv + 1 if P else 0, np.uint32), "v2n_i": np.zeros(N2N, np.uint32),
+           "p2p_o": np.zeros(P + 1 if P else 0, np.uint32), "p2p_i": np.zeros(NP2P, np.uint32)}
+    _check(lib().rbgpu_hull_tables(_ptr(v, _D), len(v), _ptr(f, _U), nf, _ptr(sz, _U), _ptr(out["pn"], _D),
+                                   *(_ptr(out[k], _U) for k in ("poff", "pidx", "v2p_o", "v2p_i", "v2n_o", "v2n_i",
+                                                                "p2p_o", "p2p_i"))))
+    return out
+
+
 OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, "run_min": 4, "timing_stride": 5,
            "joint_mode": 6}
 
@@ -119,6 +157,7 @@ class World:
         self.n_bodies = 0
         self.n_colliders = 0
         self.n_joints = 0
+        self.n_hulls = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -184,13 +223,46 @@ class World:
         ts = None if target_speed is None else np.ascontiguousarray(target_speed, np.float64)
         _check(lib().rbgpu_set_joint_targets(self.h, len(ids), _ptr(ids, _U), _ptr(tp, _D), _ptr(ts, _D)))
 
+    def add_hull(self, verts, faces=None) -> int:
+        """rbc::convex_hull_data_create (ConvexHull.hpp:149-420): the world copies the hull;
+        returns the id a hull collider names in hull_index."""
+        v, f = _hull_arrays(verts, faces)
+        hid = np.zeros(1, np.uint32)
+        _check(lib().rbgpu_add_hull(self.h, _ptr(v, _D), len(v), _ptr(f, _U), 0 if f is None else len(f), _ptr(hid, _U)))
+        self.n_hulls += 1
+        return int(hid[0])
+
     def load(self, scene) -> "World":
         self.add_bodies(scene.bodies)
-        self.add_colliders(scene.colliders)
+        cols = scene.colliders
+        hulls = getattr(scene, "hulls", None) or []
+        if hulls:
+            ids = np.array([self.add_hull(v, f) for v, f in hulls], np.int32)
+            cols = np.array(cols, copy=True)
+            sel = cols["shape"] == scenes.HULL
+            cols["hull_index"][sel] = ids[cols["hull_index"][sel]]
+        self.add_colliders(cols)
         js = getattr(scene, "joints", None)
         if js is not None and len(js):
             self.add_joints(js)
         return self
+
+    def collide_pairs(self, pairs) -> dict:
+        """Narrowphase of collider pairs (a, b) at the current state through the pipeline's
+        binned kernels: points, normal, surface points p_a / p_b and depths per pair, and the
+        GJK/EPA statistics (iterations, statuses; -1 where no GJK ran)."""
+        pr = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        n = len(pr)
+        out = np.zeros((n, 32))
+        gs = np.zeros(n, np.uint32)
+        _check(lib().rbgpu_collide_world_pairs(self.h, n, _ptr(pr, _U), _ptr(out, _D), _ptr(gs, _U)))
+        cnt = out[:, 0].astype(int)
+        pts = out[:, 4:].reshape(n, 4, 7)
+        nogjk = gs == 0xFFFFFFFF
+        g = gs.astype(np.int64)
+        return {"n": cnt, "normal": out[:, 1:4], "p_a": pts[:, :, 0:3], "p_b": pts[:, :, 3:6], "depth": pts[:, :, 6],
+                "gjk_iters": np.where(nogjk, -1, g & 255), "epa_iters": np.where(nogjk, -1, (g >> 8) & 255),
+                "gjk_status": np.where(nogjk, -1, (g >> 16) & 15), "epa_status": np.where(nogjk, -1, (g >> 20) & 15)}
 
     # ---------------------------------------------------------------- step
     def step(self, frames: int = 1):
