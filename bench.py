@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "body-substeps/s on 1 MI355X (learned+batched vs plain), pos err \u22641e-4 m"
 WORKLOAD = "plain GPU pipeline, large pile: 65,536 mixed convex bodies, 10 substeps, 500 steps"
+HULL_WORKLOAD = ("GJK/EPA-heavy: 32,768 random convex hulls (16-64 vertices on ellipsoids with semi-axes "
+                 "U(0.2,0.8), triangulated) in a dense pile over a plane, 10 substeps, plain GJK/EPA")
 
 
 def _cpu_world(sc):
@@ -102,13 +104,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--timing-kernel", default="solve_pos",
+    ap.add_argument("--timing-kernel", default=None,
                     choices=["off", "solve_pos", "solve_vel", "narrowphase", "integrate", "update_vel", "broadphase", "colouring",
-                             "solve_pos_runs", "solve_vel_runs"])
+                             "solve_pos_runs", "solve_vel_runs", "joints", "gjk"],
+                    help="kernel family timed for the roofline (default: solve_pos; gjk for --workload hulls)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="pile", choices=["pile", "batched"],
+    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "hulls"],
                     help="pile: the 65,536-body mixed pile (default); batched: 4,096 independent "
-                         "box-stack + capsule-chain scenes (1.1M bodies) in one world")
+                         "box-stack + capsule-chain scenes (1.1M bodies) in one world; hulls: 32,768 "
+                         "random convex hulls in a dense pile (GJK/EPA)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB ten times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a --pmc run to bytes)")
@@ -122,7 +126,12 @@ def main():
         sys.exit(2)
 
     from rbps_amd import scenes, world
-    if args.workload == "batched":
+    if args.timing_kernel is None:
+        args.timing_kernel = "gjk" if args.workload == "hulls" else "solve_pos"
+    if args.workload == "hulls":
+        scene_fn = lambda: scenes.hull_pile(32768, seed=4)   # noqa: E731
+        workload = HULL_WORKLOAD
+    elif args.workload == "batched":
         scene_fn = scenes.batched_stack_chains
         workload = ("batched scenes: 4,096 independent scenes of 32-512 bodies (box stack + 16-capsule chain "
                     "on spherical joints, compliance 1e-6), 10 substeps, XPBD joint mode")
@@ -168,11 +177,12 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": {"workload": workload, "scene": sc.name, "bodies": sc.n_dynamic, "substeps": sc.substeps, "timestep": sc.timestep,
-                   "shapes": "box/sphere/capsule + static plane", "last_frame": st},
+                   "shapes": "convex hulls + static plane" if args.workload == "hulls" else "box/sphere/capsule + static plane",
+                   "last_frame": st},
         "roofline": roof if args.workload == "pile" else dict(roof or {}, traffic=None),
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_batched() if args.workload == "batched" else cpu_baseline(scene_fn)
+        out["cpu_baseline"] = cpu_baseline_batched() if args.workload == "batched" else cpu_baseline(lambda: sc)
     print(json.dumps(out))
 
 

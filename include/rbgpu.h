@@ -179,8 +179,9 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  * (per-colour launches), 3 narrowphase, 4 integrate, 5 velocity update, 6 broadphase
  * (refit, endpoint sort, sweep, pair list; one span per frame), 7 colouring (one span per
  * substep), 8 position solve runs of small colours (single workgroup), 9 velocity solve
- * runs.  rbgpu_get_timing returns the summed milliseconds and launches since the last
- * reset (synchronises); bytes = algorithmic bytes of the timed launches for 1 and 2. */
+ * runs, 10 joints, 11 GJK/EPA pairs (the wave-per-pair hull kernel).  rbgpu_get_timing
+ * returns the summed milliseconds and launches since the last reset (synchronises);
+ * bytes = algorithmic bytes of the timed launches for 1, 2 and 11. */
 /* Pipeline options (no counterpart in the reference, whose step is sequential).
  *   RBGPU_OPT_PALETTE_MAX  colouring: a pair draws its tentative colour from at most this
  *                          many free colours (default 8).  Changes the colouring, hence the
@@ -252,8 +253,46 @@ int rbgpu_hull_tables(const double *vertices, uint32_t n_vertices, const uint32_
  * out32 per pair: points, normal xyz, then per point the two surface points p_a, p_b
  * (CollisionPipeline.cpp:59-104) and the depth (7 values) x 4.  gjk_stats (optional) per
  * pair: GJK iterations | EPA iterations << 8 | GJK status << 16 | EPA status << 20 (EPA
- * status 15 = not run; 0xffffffff = not a GJK pair).  Replaces the last substep's contacts. */
-int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2, double *out32, uint32_t *gjk_stats);
+ * status 15 = not run; 0xffffffff = not a GJK pair; bit 24: the plain fallback ran).
+ * guess6 (optional, per pair, A's frame): GJK warm-start direction xyz and EPA seed normal
+ * xyz (used when seed_epa) in place of the network's predictions (zero: plain seed).
+ * Replaces the last substep's contacts. */
+int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2, double *out32, uint32_t *gjk_stats,
+                              const float *guess6, int seed_epa);
+
+/* ---- learned warm start of GJK/EPA (no counterpart in the reference) -----------------
+ * A small MLP (32 -> 128 -> 128 -> 128 -> 16, ReLU; BF16 weights on MFMA, FP32
+ * accumulation) maps a GJK pair's relative pose and shape descriptors to a GJK initial
+ * direction and an EPA seed normal.  The reference seeds GJK with tf_b.pos - tf_a.pos
+ * (ContactManifoldGenerator.cpp:423-425) and EPA with nothing; here plain GJK/EPA still
+ * run to the same tolerances and are the fallback when a warm-started run fails, so
+ * predictions change iteration counts (and, within the EPA tolerance, which face EPA
+ * stops on), not the convergence criteria.
+ *   rbgpu_mlp_init        allocate and initialise the network (He-uniform, seeded)
+ *   rbgpu_mlp_set/get_weights  the 39,312 FP32 parameters: per layer W[out][in], bias
+ *   rbgpu_mlp_set_mode    0 plain (default), 1 predicted GJK direction, 2 + EPA seed
+ *   rbgpu_mlp_log         log training samples (features, targets) of the GJK pairs of
+ *                         every substep until `capacity` samples (0: drop the log)
+ *   rbgpu_mlp_logged      samples logged (stop != 0 ends logging); rbgpu_mlp_get_log reads them
+ *   rbgpu_mlp_train       Adam steps on batches drawn from the log; mean batch loss
+ *   rbgpu_mlp_eval        inference on given features (n x 32 float) -> n x 6
+ *   rbgpu_mlp_gradients   gradient of the mean loss on given features and targets
+ *                         (n x 8: GJK target xyz, valid, EPA target xyz, valid)
+ *   rbgpu_get_gjk_totals  since the last reset: GJK pairs, GJK iterations, EPA
+ *                         iterations, overlapping pairs, EPA converged, fallbacks */
+int rbgpu_mlp_init(rbgpu_world *w, uint64_t seed);
+int rbgpu_mlp_set_weights(rbgpu_world *w, const float *params, uint32_t n);
+int rbgpu_mlp_get_weights(rbgpu_world *w, float *params, uint32_t n);
+int rbgpu_mlp_set_mode(rbgpu_world *w, int mode);
+int rbgpu_mlp_log(rbgpu_world *w, uint64_t capacity);
+int rbgpu_mlp_logged(rbgpu_world *w, uint64_t *n, int stop);
+/* Logged samples [start, start + n): features (n x 32) and targets (n x 8), as float. */
+int rbgpu_mlp_get_log(rbgpu_world *w, uint64_t start, uint32_t n, float *features32, float *targets8);
+int rbgpu_mlp_train(rbgpu_world *w, int steps, uint32_t batch, double lr, double *mean_loss);
+int rbgpu_mlp_eval(rbgpu_world *w, uint32_t n, const float *features32, float *out6);
+int rbgpu_mlp_gradients(rbgpu_world *w, uint32_t n, const float *features32, const float *targets8, float *grad,
+                        double *loss);
+int rbgpu_get_gjk_totals(rbgpu_world *w, uint64_t *out6, int reset);
 
 const char *rbgpu_last_error(void);
 

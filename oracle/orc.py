@@ -123,6 +123,8 @@ def oracle_lib():
         lib.orc_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int]
         _bind(lib, "orc_")
         lib.orc_collide.argtypes = [C.c_int, _P, _D, _P, _D, _D]
+        lib.orc_collide_warm.restype = C.c_int
+        lib.orc_collide_warm.argtypes = [C.c_int, _P, _D, _P, _D, _D, _D]
         lib.orc_gjk_epa.restype = C.c_int
         lib.orc_gjk_epa.argtypes = [C.c_int, _P, _D, _P, _D, _D, _D]
         lib.orc_num_solve_pairs.argtypes = [_P]
@@ -304,6 +306,14 @@ def oracle_collide(a, tfa, b, tfb, fp32=False, hulls=None):
     return _collide(oracle_lib().orc_collide, (1 if fp32 else 0,), a, tfa, b, tfb, hulls, "orc_")
 
 
+def oracle_collide_warm(a, tfa, b, tfb, guess6, fp32=True, hulls=None):
+    """Narrowphase of one pair with a warm start for GJK pairs (guess6 in A's frame: GJK
+    direction, EPA seed normal; zeros: plain)."""
+    g = np.ascontiguousarray(guess6, np.float64)
+    fn = lambda f32, pa, ta, pb, tb, out: oracle_lib().orc_collide_warm(f32, pa, ta, pb, tb, _ptr(g, _D), out)  # noqa: E731
+    return _collide(fn, (1 if fp32 else 0,), a, tfa, b, tfb, hulls, "orc_")
+
+
 def ref_collide(a, tfa, b, tfb, hulls=None):
     return _collide(ref_lib().ref_collide, (), a, tfa, b, tfb, hulls, "ref_")
 
@@ -314,12 +324,16 @@ EPA_VALID, EPA_FAILED, EPA_OUT_OF_FACES, EPA_OUT_OF_VERTICES, EPA_NONLINEAR = 0,
 
 def _gjk_epa_out(out, hit):
     return {"hit": bool(hit), "gjk_status": int(out[0]), "gjk_iters": int(out[1]), "epa_status": int(out[2]),
-            "epa_iters": int(out[3]), "normal": out[4:7].copy(), "depth": float(out[7]), "point": out[8:11].copy()}
+            "epa_iters": int(out[3]), "normal": out[4:7].copy(), "depth": float(out[7]), "point": out[8:11].copy(),
+            "gjk_dir": out[11:14].copy(), "fallback": int(out[14])}
 
 
 def oracle_gjk_epa(a, tfa, b, tfb, fp32=False, hulls=None, guess=None):
-    """GJK + EPA of one pair with statuses and iteration counts (guess: GJK seed direction)."""
-    out = np.zeros(11)
+    """GJK + EPA of one pair with statuses, iteration counts and the GJK exit direction.
+    guess (optional, A's frame): GJK direction (3) and EPA seed normal (3, zero = none)."""
+    out = np.zeros(15)
+    if guess is not None and len(guess) == 3:
+        guess = np.concatenate([guess, np.zeros(3)])
     a = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(a), hulls))
     b = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(b), hulls))
     g = None if guess is None else np.ascontiguousarray(guess, np.float64)
@@ -332,7 +346,7 @@ def oracle_gjk_epa(a, tfa, b, tfb, fp32=False, hulls=None, guess=None):
 
 
 def ref_gjk_epa(a, tfa, b, tfb, hulls=None):
-    out = np.zeros(11)
+    out = np.zeros(15)
     a = np.ascontiguousarray(map_hulls("ref_", np.atleast_1d(a), hulls))
     b = np.ascontiguousarray(map_hulls("ref_", np.atleast_1d(b), hulls))
     hit = ref_lib().ref_gjk_epa(a.ctypes.data_as(_P), _ptr(np.ascontiguousarray(tfa, np.float64), _D),

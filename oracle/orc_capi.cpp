@@ -232,29 +232,51 @@ template <class T> orc::TF<T> tf7(const double *p) {
 }
 
 // GJK + EPA of one pair with statistics: out = gjk status, gjk iterations, epa status,
-// epa iterations, normal xyz, depth, EPA witness xyz (11 values).  guess3 may be null
-// (the reference's seed tf_b.pos - tf_a.pos).
+// epa iterations, normal xyz, depth, EPA witness xyz, GJK exit direction xyz, fallback
+// (15 values).  guess6 (optional): warm start in A's frame, GJK direction xyz and EPA
+// seed normal xyz (a zero seed normal: none).
 template <class T> int gjk_epa_t(const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
-                                 const double *tb, const double *guess3, double *out) {
+                                 const double *tb, const double *guess6, double *out) {
     const orc::ShapeP<T> sa = shape_from<T>(*a), sb = shape_from<T>(*b);
     const orc::TF<T> A = tf7<T>(ta), Bt = tf7<T>(tb);
     orc::Manifold<T> m;
     orc::GjkEpaStats st{};
     orc::EpaOut<T> e;
-    orc::V3<T> g;
-    if (guess3) g = orc::V3<T>(T(guess3[0]), T(guess3[1]), T(guess3[2]));
-    const bool hit = orc::gjk_epa_manifold(sa, A, sb, Bt, m, guess3 ? &g : nullptr, &st, &e);
+    orc::V3<T> g, sn, gd;
+    bool seed = false;
+    if (guess6) {
+        g = orc::V3<T>(T(guess6[0]), T(guess6[1]), T(guess6[2]));
+        sn = orc::V3<T>(T(guess6[3]), T(guess6[4]), T(guess6[5]));
+        seed = !(guess6[3] == 0.0 && guess6[4] == 0.0 && guess6[5] == 0.0);
+    }
+    const bool hit = orc::gjk_epa_manifold(sa, A, sb, Bt, m, guess6 ? &g : nullptr, &st, &e, seed ? &sn : nullptr, &gd);
     out[0] = st.gjk_status; out[1] = st.gjk_iters; out[2] = st.epa_status; out[3] = st.epa_iters;
     out[4] = e.n.x; out[5] = e.n.y; out[6] = e.n.z; out[7] = e.depth;
     out[8] = e.p.x; out[9] = e.p.y; out[10] = e.p.z;
+    out[11] = gd.x; out[12] = gd.y; out[13] = gd.z; out[14] = st.fallback;
     return hit ? 1 : 0;
 }
 
-// One narrowphase call (same layout as the reference driver's ref_collide).
+// One narrowphase call (same layout as the reference driver's ref_collide).  guess6
+// (optional): warm start of a GJK pair in A's frame (GJK direction, EPA seed normal; a
+// zero seed normal: none); other pairs ignore it.
 template <class T> int collide_t(const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
-                                        const double *tb, double *out) {
+                                        const double *tb, double *out, const double *guess6 = nullptr) {
     orc::Manifold<T> m;
-    bool hit = orc::dispatch(shape_from<T>(*a), tf7<T>(ta), shape_from<T>(*b), tf7<T>(tb), m);
+    const orc::ShapeP<T> sa = shape_from<T>(*a), sb = shape_from<T>(*b);
+    const bool gjk_pair = (sa.kind == orc::SH_HULL || sb.kind == orc::SH_HULL) && sa.kind != orc::SH_PLANE &&
+                          sb.kind != orc::SH_PLANE;
+    bool hit;
+    if (guess6 && gjk_pair) {
+        const orc::V3<T> g{T(guess6[0]), T(guess6[1]), T(guess6[2])}, sn{T(guess6[3]), T(guess6[4]), T(guess6[5])};
+        const bool seed = !(guess6[3] == 0.0 && guess6[4] == 0.0 && guess6[5] == 0.0);
+        // a zero direction means the plain seed (the GPU's convention)
+        const bool warm = !(guess6[0] == 0.0 && guess6[1] == 0.0 && guess6[2] == 0.0);
+        hit = orc::gjk_epa_manifold<T>(sa, tf7<T>(ta), sb, tf7<T>(tb), m, warm ? &g : nullptr, nullptr, nullptr,
+                                    warm && seed ? &sn : nullptr);
+    } else {
+        hit = orc::dispatch(sa, tf7<T>(ta), sb, tf7<T>(tb), m);
+    }
     out[0] = hit ? m.n : 0;
     out[1] = m.normal.x; out[2] = m.normal.y; out[3] = m.normal.z;
     for (int i = 0; i < 4; ++i) {
@@ -357,10 +379,18 @@ int orc_collide(int fp32, const orc_collider_desc *a, const double *ta, const or
         return -1;
     }
 }
-int orc_gjk_epa(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b, const double *tb,
-                const double *guess3, double *out) {
+int orc_collide_warm(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
+                     const double *tb, const double *guess6, double *out) {
     try {
-        return fp32 ? gjk_epa_t<float>(a, ta, b, tb, guess3, out) : gjk_epa_t<double>(a, ta, b, tb, guess3, out);
+        return fp32 ? collide_t<float>(a, ta, b, tb, out, guess6) : collide_t<double>(a, ta, b, tb, out, guess6);
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+int orc_gjk_epa(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b, const double *tb,
+                const double *guess6, double *out) {
+    try {
+        return fp32 ? gjk_epa_t<float>(a, ta, b, tb, guess6, out) : gjk_epa_t<double>(a, ta, b, tb, guess6, out);
     } catch (const std::exception &) {
         return -1;
     }

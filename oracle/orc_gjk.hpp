@@ -369,6 +369,7 @@ template <class T> inline bool simplex4(Simplex<T> &s, V3<T> &dir) {
 template <class T> struct GjkOut {
     int status = GJK_FAILED, iters = 0;
     Simplex<T> s;
+    V3<T> dir;   // search direction at exit (a separating direction when Valid)
 };
 template <class T> inline GjkOut<T> gjk(const MinkDiff<T> &md, const V3<T> &guess, int max_iter = 128) {
     GjkOut<T> o;
@@ -379,7 +380,7 @@ template <class T> inline GjkOut<T> gjk(const MinkDiff<T> &md, const V3<T> &gues
     for (int it = 0; it < max_iter; ++it) {
         o.iters = it + 1;
         const SVert<T> nx = md.sample(dir);
-        if (dot(nx.w, dir) < T(0)) { o.status = GJK_VALID; return o; }
+        if (dot(nx.w, dir) < T(0)) { o.status = GJK_VALID; o.dir = dir; return o; }
         o.s.push(nx);
         bool inside = false;
         if (o.s.rank == 2) simplex2(o.s, dir);
@@ -433,7 +434,7 @@ template <class T> inline bool face_normal(EpaFace<T> &f, const std::vector<SVer
     return true;
 }
 
-template <class T> inline EpaOut<T> epa(const Simplex<T> &seed, const MinkDiff<T> &md) {
+template <class T> inline EpaOut<T> epa(const Simplex<T> &seed, const MinkDiff<T> &md, const V3<T> *seed_n = nullptr) {
     EpaOut<T> o;
     if (seed.rank != 4) { o.status = EPA_NONLINEAR; return o; }
     std::vector<SVert<T>> vs(seed.v, seed.v + 4);
@@ -475,9 +476,53 @@ template <class T> inline EpaOut<T> epa(const Simplex<T> &seed, const MinkDiff<T
         return o;
     }
     std::vector<std::pair<int, int>> edges;
+    // Adds w to the polytope: faces visible from w become obsolete, the horizon of the
+    // visible region (add_edge toggling) is stitched to w.  false: face pool exhausted.
+    auto expand = [&](const SVert<T> &w) {
+        vs.push_back(w);
+        const int wi = (int)vs.size() - 1;
+        edges.clear();
+        auto add_edge = [&edges](int a, int b) {
+            for (size_t i = 0; i < edges.size(); ++i)
+                if ((edges[i].first == a && edges[i].second == b) || (edges[i].first == b && edges[i].second == a)) {
+                    edges.erase(edges.begin() + (long)i);
+                    return;
+                }
+            edges.push_back({a, b});
+        };
+        for (auto &f : fs) {
+            if (f.obsolete) continue;
+            const V3<T> cen = (vs[f.v[0]].w + vs[f.v[1]].w + vs[f.v[2]].w) * T(1.0 / 3.0);
+            if (dot(f.n, w.w - cen) > T(0)) {
+                f.obsolete = true;
+                add_edge(f.v[0], f.v[1]);
+                add_edge(f.v[1], f.v[2]);
+                add_edge(f.v[2], f.v[0]);
+            }
+        }
+        for (const auto &e : edges) {
+            if ((int)fs.size() >= EPA_MAX_FACES) return false;
+            add_face(e.first, e.second, wi);
+        }
+        return true;
+    };
     const T big = std::numeric_limits<T>::max();
+    int it0 = 0;
+    if (seed_n) {
+        // warm start: one expansion along a predicted normal before the first closest-face
+        // step, when its support is a new vertex that some face sees
+        it0 = 1;
+        const SVert<T> w = md.sample(*seed_n);
+        bool fresh = true, seen = false;
+        for (size_t i = 0; i < vs.size(); ++i) fresh = fresh && !(length_sq(vs[i].w - w.w) < C<T>::EPS * C<T>::EPS);
+        for (const auto &f : fs) {
+            const V3<T> cen = (vs[f.v[0]].w + vs[f.v[1]].w + vs[f.v[2]].w) * T(1.0 / 3.0);
+            seen = seen || (!f.obsolete && dot(f.n, w.w - cen) > T(0));
+        }
+        if (fresh && seen && !expand(w)) { o.iters = 1; o.status = EPA_OUT_OF_FACES; return o; }
+    }
     for (int it = 0; it < EPA_MAX_ITER; ++it) {
-        o.iters = it + 1;
+        o.iters = it0 + it + 1;
         int ci = -1;
         T md_ = big;
         for (size_t i = 0; i < fs.size(); ++i)
@@ -515,31 +560,7 @@ template <class T> inline EpaOut<T> epa(const Simplex<T> &seed, const MinkDiff<T
                 return o;
             }
         if ((int)vs.size() >= EPA_MAX_VERTS) { o.status = EPA_OUT_OF_VERTICES; return o; }
-        vs.push_back(w);
-        const int wi = (int)vs.size() - 1;
-        edges.clear();
-        auto add_edge = [&edges](int a, int b) {
-            for (size_t i = 0; i < edges.size(); ++i)
-                if ((edges[i].first == a && edges[i].second == b) || (edges[i].first == b && edges[i].second == a)) {
-                    edges.erase(edges.begin() + (long)i);
-                    return;
-                }
-            edges.push_back({a, b});
-        };
-        for (auto &f : fs) {
-            if (f.obsolete) continue;
-            const V3<T> cen = (vs[f.v[0]].w + vs[f.v[1]].w + vs[f.v[2]].w) * T(1.0 / 3.0);
-            if (dot(f.n, w.w - cen) > T(0)) {
-                f.obsolete = true;
-                add_edge(f.v[0], f.v[1]);
-                add_edge(f.v[1], f.v[2]);
-                add_edge(f.v[2], f.v[0]);
-            }
-        }
-        for (const auto &e : edges) {
-            if ((int)fs.size() >= EPA_MAX_FACES) { o.status = EPA_OUT_OF_FACES; return o; }
-            add_face(e.first, e.second, wi);
-        }
+        if (!expand(w)) { o.status = EPA_OUT_OF_FACES; return o; }
     }
     o.status = EPA_FAILED;
     return o;
@@ -751,24 +772,41 @@ inline void manifold_hulls(const V3<T> &en, T edep, const V3<T> &ep, const Hull<
     m.n = reduce4_area(kp, kd, m.pos, m.depth);
 }
 
-// GJK + EPA + manifold (ContactManifoldGenerator.cpp:382-438).  guess: the GJK seed
-// direction (the reference's is tf_b.pos - tf_a.pos); stats, when given, receive the
-// GJK/EPA statuses and iteration counts.
-struct GjkEpaStats { int gjk_status, gjk_iters, epa_status, epa_iters; };
+// GJK + EPA + manifold (ContactManifoldGenerator.cpp:382-438).  Warm start (no
+// counterpart in the reference): guess_local / seed_local, in A's frame, replace the
+// GJK seed direction tf_b.pos - tf_a.pos and add one EPA expansion along the seed
+// normal; when the warm-started run fails (GJK cap, EPA not valid) the plain run is the
+// fallback (also when the warm-started GJK is not done within 16 iterations).  stats,
+// when given, receive the statuses and iteration counts of the runs
+// made (both runs' iterations when the fallback ran) and the exit directions.
+struct GjkEpaStats { int gjk_status, gjk_iters, epa_status, epa_iters, fallback; };
 template <class T>
 inline bool gjk_epa_manifold(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, const TF<T> &tb, Manifold<T> &m,
-                             const V3<T> *guess_in = nullptr, GjkEpaStats *st = nullptr, EpaOut<T> *epa_out = nullptr) {
+                             const V3<T> *guess_local = nullptr, GjkEpaStats *st = nullptr, EpaOut<T> *epa_out = nullptr,
+                             const V3<T> *seed_local = nullptr, V3<T> *gjk_dir = nullptr) {
     MinkDiff<T> md{&a, &b, ta, tb};
-    V3<T> guess = tb.pos - ta.pos;
-    if (guess_in) guess = *guess_in;
-    else if (length_sq(guess) < C<T>::EPS) guess = V3<T>(1, 0, 0);
-    const GjkOut<T> g = gjk(md, guess);
-    if (st) { st->gjk_status = g.status; st->gjk_iters = g.iters; st->epa_status = -1; st->epa_iters = 0; }
-    if (g.status != GJK_INSIDE) return false;
-    const EpaOut<T> e = epa(g.s, md);
-    if (st) { st->epa_status = e.status; st->epa_iters = e.iters; }
+    V3<T> plain = tb.pos - ta.pos;
+    if (length_sq(plain) < C<T>::EPS) plain = V3<T>(1, 0, 0);
+    const bool warm = guess_local != nullptr;
+    const V3<T> g = warm ? ta.vec(*guess_local) : plain;
+    const V3<T> sn = (warm && seed_local) ? ta.vec(*seed_local) : V3<T>();
+    GjkOut<T> gr = gjk(md, g, warm ? 16 : 128);   // a warm start gets 16 iterations before the fallback
+    EpaOut<T> e;
+    if (gr.status == GJK_INSIDE) e = epa(gr.s, md, (warm && seed_local) ? &sn : nullptr);
+    int gi = gr.iters, ei = gr.status == GJK_INSIDE ? e.iters : 0, fb = 0;
+    if (warm && (gr.status == GJK_FAILED || (gr.status == GJK_INSIDE && e.status != EPA_VALID))) {
+        fb = 1;
+        gr = gjk(md, plain);
+        e = EpaOut<T>();
+        if (gr.status == GJK_INSIDE) e = epa(gr.s, md);
+        gi += gr.iters;
+        ei += gr.status == GJK_INSIDE ? e.iters : 0;
+    }
+    if (st) { st->gjk_status = gr.status; st->gjk_iters = gi; st->epa_status = gr.status == GJK_INSIDE ? e.status : -1;
+              st->epa_iters = ei; st->fallback = fb; }
+    if (gjk_dir) *gjk_dir = gr.dir;
     if (epa_out) *epa_out = e;
-    if (e.status != EPA_VALID) return false;
+    if (gr.status != GJK_INSIDE || e.status != EPA_VALID) return false;
     m.normal = e.n;
     m.n = 0;
     if (a.kind == SH_HULL && b.kind == SH_HULL && a.hull->adj && b.hull->adj)

@@ -29,6 +29,7 @@ constexpr int HULL_MAX_VERTS = 256;
 constexpr int HULL_VPL = HULL_MAX_VERTS / 64;     // vertices per lane
 constexpr int HULL_MAX_POLY = 64;                 // vertices of one polygon, neighbours of one polygon
 constexpr int EPA_FACES = 256, EPA_VERTS = 128, EPA_ITERS = 128, GJK_ITERS = 128;
+constexpr int GJK_WARM_ITERS = 16;   // a warm-started GJK not done by then falls back to the plain seed
 constexpr int EPA_SEQ = 3 * EPA_FACES;            // directed edges of the visible faces
 constexpr int HCLIP_CAP = 2 * HULL_MAX_POLY + 8;   // Sutherland-Hodgman output bound (n + planes)
 constexpr int HP_MAX_POINTS = 32;                 // hull-plane candidates (PlaneCollision.hpp:118)
@@ -143,6 +144,7 @@ __device__ __forceinline__ SV md_sample(const Side &A, const Side &Bs, const Hul
 struct Simplex {
     SV v[4];   // v[0] newest
     int rank;
+    V3 dir;    // search direction at exit (a separating direction when GJK returns valid)
 };
 __device__ __forceinline__ void sx_push(Simplex &s, const SV &n) {   // DoSimplex.cpp:5-17
     const int r = s.rank;
@@ -231,16 +233,20 @@ __device__ __forceinline__ bool sx_tetra(Simplex &s, V3 &dir) {   // Tetrahedron
     }
     return false;
 }
-__device__ __forceinline__ int gjk(const Side &A, const Side &Bs, const HullSet &hs, V3 guess, Simplex &s, int &iters) {
+__device__ __forceinline__ int gjk(const Side &A, const Side &Bs, const HullSet &hs, V3 guess, Simplex &s, int &iters,
+                                   int max_iter = GJK_ITERS) {
     s.rank = 0;
     V3 dir = (len2(guess) > EPS) ? guess : V3{1.0f, 0.0f, 0.0f};
     const SV v0 = md_sample(A, Bs, hs, dir);
     sx_push(s, v0);
     dir = -v0.w;
-    for (int it = 0; it < GJK_ITERS; ++it) {
+    for (int it = 0; it < max_iter; ++it) {
         iters = it + 1;
         const SV nx = md_sample(A, Bs, hs, dir);
-        if (dot(nx.w, dir) < 0.0f) return GJK_VALID;
+        if (dot(nx.w, dir) < 0.0f) {
+            s.dir = dir;
+            return GJK_VALID;
+        }
         sx_push(s, nx);
         bool inside = false;
         if (s.rank == 2) sx_line(s, dir);
@@ -301,6 +307,103 @@ __device__ __forceinline__ bool epa_face_normal(const EpaLds &L, int i0, int i1,
     return true;
 }
 
+// Adds w to the polytope (EPA.cpp:337-374): faces visible from w become obsolete and
+// the horizon of the visible region is stitched to w.  Returns EPA_OUT_OF_FACES when
+// the face pool runs out, else 0.
+__device__ __forceinline__ int epa_expand(EpaLds &L, const SV &w, int &nvtx, int &nf) {
+    const int lane = lane64();
+    const int wi = nvtx;
+    if (lane == 0) {
+        L.wx[wi] = w.w.x; L.wy[wi] = w.w.y; L.wz[wi] = w.w.z;
+        L.ax[wi] = w.w0.x; L.ay[wi] = w.w0.y; L.az[wi] = w.w0.z;
+    }
+    ++nvtx;
+    // 4. visible faces, listed in face order
+    int nvis = 0;
+    for (int base = 0; base < nf; base += 64) {
+        const int f = base + lane;
+        bool v = false;
+        if (f < nf && !(L.fv[f] >> 24)) {
+            const uint32_t fv = L.fv[f];
+            const V3 cen = ((lds_w(L, (int)(fv & 255u)) + lds_w(L, (int)((fv >> 8) & 255u))) +
+                            lds_w(L, (int)((fv >> 16) & 255u))) * (1.0f / 3.0f);
+            const V3 fn{L.nx[f], L.ny[f], L.nz[f]};
+            v = dot(fn, w.w - cen) > 0.0f;
+        }
+        const uint64_t m = __ballot(v);
+        if (v) {
+            L.fv[f] |= 1u << 24;
+            L.vis[nvis + (int)lane_rank(m)] = (uint16_t)f;
+        }
+        nvis += __popcll(m);
+    }
+    __syncthreads();
+    // directed edges (v0,v1) (v1,v2) (v2,v0) of each visible face
+    const int ns = 3 * nvis;
+    for (int i = lane; i < ns; i += 64) {
+        const uint32_t fv = L.fv[L.vis[i / 3]];
+        const int k = i % 3;
+        const uint32_t a = (fv >> (8 * k)) & 255u, b = (fv >> (8 * ((k + 1) % 3))) & 255u;
+        L.seq[i] = (uint16_t)(a | (b << 8));
+    }
+    __syncthreads();
+    // horizon: odd occurrence count, kept at the last occurrence
+    int nh = 0;
+    for (int base = 0; base < ns; base += 64) {
+        const int i = base + lane;
+        bool keep = false;
+        if (i < ns) {
+            const uint32_t e = L.seq[i];
+            const uint32_t a = e & 255u, b = e >> 8;
+            const uint32_t key = a < b ? (a | (b << 8)) : (b | (a << 8));
+            bool later = false;
+            for (int j = i + 1; j < ns && !later; ++j) {
+                const uint32_t f = L.seq[j];
+                const uint32_t fa = f & 255u, fb = f >> 8;
+                later = (fa < fb ? (fa | (fb << 8)) : (fb | (fa << 8))) == key;
+            }
+            int before = 0;
+            if (!later)
+                for (int j = 0; j < i; ++j) {
+                    const uint32_t f = L.seq[j];
+                    const uint32_t fa = f & 255u, fb = f >> 8;
+                    before += (fa < fb ? (fa | (fb << 8)) : (fb | (fa << 8))) == key;
+                }
+            keep = !later && (before & 1) == 0;
+        }
+        const uint64_t m = __ballot(keep);
+        if (keep) L.hor[nh + (int)lane_rank(m)] = L.seq[i];
+        nh += __popcll(m);
+    }
+    __syncthreads();
+    // stitch: one candidate face per horizon edge, packed in horizon order
+    int made_before_last = 0, made = 0;
+    for (int base = 0; base < nh; base += 64) {
+        const int k = base + lane;
+        bool ok = false;
+        V3 n{0.0f, 0.0f, 0.0f};
+        float d = 0.0f;
+        uint32_t e = 0;
+        if (k < nh) {
+            e = L.hor[k];
+            ok = epa_face_normal(L, (int)(e & 255u), (int)(e >> 8), wi, nvtx, n, d);
+        }
+        const uint64_t m = __ballot(ok);
+        const int slot = nf + made + (int)lane_rank(m);
+        if (ok && k != nh - 1) ++made_before_last;   // counted per lane, summed below
+        if (ok && slot < EPA_FACES) {
+            L.nx[slot] = n.x; L.ny[slot] = n.y; L.nz[slot] = n.z; L.nd[slot] = d;
+            L.fv[slot] = e | ((uint32_t)wi << 16);
+        }
+        made += __popcll(m);
+    }
+    // the reference checks the pool before each new face, failing ones included
+    if (nh > 0 && nf + (int)wave_sum((uint32_t)made_before_last) >= EPA_FACES) return EPA_OUT_OF_FACES;
+    nf += made;
+    __syncthreads();
+    return 0;
+}
+
 struct EpaResult {
     int status, iters;
     V3 n, p;
@@ -308,7 +411,7 @@ struct EpaResult {
 };
 
 __device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side &A, const Side &Bs, const HullSet &hs,
-                                         EpaLds &L) {
+                                         EpaLds &L, const V3 *seed_n) {
     EpaResult r{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
     const int lane = lane64();
     if (seed.rank != 4) { r.status = EPA_NONLINEAR; return r; }
@@ -357,8 +460,28 @@ __device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side &A, con
         nf = 4;
     }
     __syncthreads();
+    int it0 = 0;
+    if (seed_n) {
+        // warm start: one expansion along a predicted normal, when its support is a new
+        // vertex that a face sees
+        it0 = 1;
+        const SV w = md_sample(A, Bs, hs, *seed_n);
+        bool dup = false, seen = false;
+        if (lane < nvtx) dup = len2(lds_w(L, lane) - w.w) < EPS * EPS;
+        if (lane < nf) {
+            const uint32_t fv = L.fv[lane];
+            const V3 cen = ((lds_w(L, (int)(fv & 255u)) + lds_w(L, (int)((fv >> 8) & 255u))) +
+                            lds_w(L, (int)((fv >> 16) & 255u))) * (1.0f / 3.0f);
+            seen = dot(V3{L.nx[lane], L.ny[lane], L.nz[lane]}, w.w - cen) > 0.0f;
+        }
+        if (!__ballot(dup) && __ballot(seen) && epa_expand(L, w, nvtx, nf)) {
+            r.iters = 1;
+            r.status = EPA_OUT_OF_FACES;
+            return r;
+        }
+    }
     for (int it = 0; it < EPA_ITERS; ++it) {
-        r.iters = it + 1;
+        r.iters = it0 + it + 1;
         // 1. closest live face
         float best = 3.402823466e+38f;
         int bi = 0x7fffffff;
@@ -406,98 +529,7 @@ __device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side &A, con
             return r;
         }
         if (nvtx >= EPA_VERTS) { r.status = EPA_OUT_OF_VERTICES; return r; }
-        const int wi = nvtx;
-        if (lane == 0) {
-            L.wx[wi] = w.w.x; L.wy[wi] = w.w.y; L.wz[wi] = w.w.z;
-            L.ax[wi] = w.w0.x; L.ay[wi] = w.w0.y; L.az[wi] = w.w0.z;
-        }
-        ++nvtx;
-        // 4. visible faces, listed in face order
-        int nvis = 0;
-        for (int base = 0; base < nf; base += 64) {
-            const int f = base + lane;
-            bool v = false;
-            if (f < nf && !(L.fv[f] >> 24)) {
-                const uint32_t fv = L.fv[f];
-                const V3 cen = ((lds_w(L, (int)(fv & 255u)) + lds_w(L, (int)((fv >> 8) & 255u))) +
-                                lds_w(L, (int)((fv >> 16) & 255u))) * (1.0f / 3.0f);
-                const V3 fn{L.nx[f], L.ny[f], L.nz[f]};
-                v = dot(fn, w.w - cen) > 0.0f;
-            }
-            const uint64_t m = __ballot(v);
-            if (v) {
-                L.fv[f] |= 1u << 24;
-                L.vis[nvis + (int)lane_rank(m)] = (uint16_t)f;
-            }
-            nvis += __popcll(m);
-        }
-        __syncthreads();
-        // directed edges (v0,v1) (v1,v2) (v2,v0) of each visible face
-        const int ns = 3 * nvis;
-        for (int i = lane; i < ns; i += 64) {
-            const uint32_t fv = L.fv[L.vis[i / 3]];
-            const int k = i % 3;
-            const uint32_t a = (fv >> (8 * k)) & 255u, b = (fv >> (8 * ((k + 1) % 3))) & 255u;
-            L.seq[i] = (uint16_t)(a | (b << 8));
-        }
-        __syncthreads();
-        // horizon: odd occurrence count, kept at the last occurrence
-        int nh = 0;
-        for (int base = 0; base < ns; base += 64) {
-            const int i = base + lane;
-            bool keep = false;
-            if (i < ns) {
-                const uint32_t e = L.seq[i];
-                const uint32_t a = e & 255u, b = e >> 8;
-                const uint32_t key = a < b ? (a | (b << 8)) : (b | (a << 8));
-                bool later = false;
-                for (int j = i + 1; j < ns && !later; ++j) {
-                    const uint32_t f = L.seq[j];
-                    const uint32_t fa = f & 255u, fb = f >> 8;
-                    later = (fa < fb ? (fa | (fb << 8)) : (fb | (fa << 8))) == key;
-                }
-                int before = 0;
-                if (!later)
-                    for (int j = 0; j < i; ++j) {
-                        const uint32_t f = L.seq[j];
-                        const uint32_t fa = f & 255u, fb = f >> 8;
-                        before += (fa < fb ? (fa | (fb << 8)) : (fb | (fa << 8))) == key;
-                    }
-                keep = !later && (before & 1) == 0;
-            }
-            const uint64_t m = __ballot(keep);
-            if (keep) L.hor[nh + (int)lane_rank(m)] = L.seq[i];
-            nh += __popcll(m);
-        }
-        __syncthreads();
-        // stitch: one candidate face per horizon edge, packed in horizon order
-        int made_before_last = 0, made = 0;
-        for (int base = 0; base < nh; base += 64) {
-            const int k = base + lane;
-            bool ok = false;
-            V3 n{0.0f, 0.0f, 0.0f};
-            float d = 0.0f;
-            uint32_t e = 0;
-            if (k < nh) {
-                e = L.hor[k];
-                ok = epa_face_normal(L, (int)(e & 255u), (int)(e >> 8), wi, nvtx, n, d);
-            }
-            const uint64_t m = __ballot(ok);
-            const int slot = nf + made + (int)lane_rank(m);
-            if (ok && k != nh - 1) ++made_before_last;   // counted per lane, summed below
-            if (ok && slot < EPA_FACES) {
-                L.nx[slot] = n.x; L.ny[slot] = n.y; L.nz[slot] = n.z; L.nd[slot] = d;
-                L.fv[slot] = e | ((uint32_t)wi << 16);
-            }
-            made += __popcll(m);
-        }
-        // the reference checks the pool before each new face, failing ones included
-        if (nh > 0 && nf + (int)wave_sum((uint32_t)made_before_last) >= EPA_FACES) {
-            r.status = EPA_OUT_OF_FACES;
-            return r;
-        }
-        nf += made;
-        __syncthreads();
+        if (epa_expand(L, w, nvtx, nf)) { r.status = EPA_OUT_OF_FACES; return r; }
     }
     r.status = EPA_FAILED;
     return r;
@@ -748,9 +780,17 @@ __device__ __forceinline__ void load_side(Side &s, const Colliders &C, const Hul
 // GJK + EPA + manifold for one pair of the hull-hull / hull-primitive bins
 // (ContactManifoldGenerator.cpp:415-438).  One 64-lane workgroup per pair.
 // gstat (optional): per slot, GJK iterations | EPA iterations << 8 | GJK status << 16 |
-// EPA status << 20 (EPA status 15: not run).  guess (optional): GJK seed direction per slot.
+// EPA status << 20 (15: not run) | fallback << 24; iterations of a fallback run are added.
+// guess (optional, per bin position two float4 in A's frame): warm start, the GJK
+// direction and (seed_epa) an EPA seed normal; a zero direction means the plain seed.
+// When the warm-started run fails (GJK not done within 16 iterations, EPA not valid) the
+// plain run follows.
+// tlog (optional, per bin position two float4 in A's frame): training targets, the EPA
+// normal (or, for a separated pair, the reverse of the GJK exit direction) with a valid
+// flag, and the EPA normal with a valid flag.
 __global__ void __launch_bounds__(64) k_narrow_gjk(const uint32_t *bin_slot, uint32_t off, Colliders C, Bodies B,
-                                                   Contacts K, HullSet hs, uint32_t *gstat, const float4 *guess) {
+                                                   Contacts K, HullSet hs, uint32_t *gstat, const float4 *guess,
+                                                   int seed_epa, float4 *tlog) {
     __shared__ PairLds L;
     const uint32_t s = bin_slot[off + blockIdx.x];
     const uint2 cc = K.cc[s], bb = K.bb[s];
@@ -758,17 +798,50 @@ __global__ void __launch_bounds__(64) k_narrow_gjk(const uint32_t *bin_slot, uin
     Side A, Bs;
     load_side(A, C, hs, cc.x, ta);
     load_side(Bs, C, hs, cc.y, tb);
-    V3 g = tb.pos - ta.pos;
-    if (guess) g = xyz(guess[s]);
-    else if (len2(g) < EPS) g = V3{1.0f, 0.0f, 0.0f};
+    V3 plain = tb.pos - ta.pos;
+    if (len2(plain) < EPS) plain = V3{1.0f, 0.0f, 0.0f};
+    V3 g = plain, sn{0.0f, 0.0f, 0.0f};
+    bool warm = false, seeded = false;
+    if (guess) {
+        const V3 gl = xyz(guess[2 * blockIdx.x]);
+        if (len2(gl) > 0.5f) { g = tf_vec(ta, gl); warm = true; }
+        const V3 sl = xyz(guess[2 * blockIdx.x + 1]);
+        if (warm && seed_epa && len2(sl) > 0.5f) { sn = tf_vec(ta, sl); seeded = true; }
+    }
     Simplex sx;
     int giters = 0;
-    const int gs = gjk(A, Bs, hs, g, sx, giters);
+    int gs = gjk(A, Bs, hs, g, sx, giters, warm ? GJK_WARM_ITERS : GJK_ITERS);
     EpaResult e{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
-    if (gs == GJK_INSIDE) e = epa(sx, A, Bs, hs, L.e);
+    if (gs == GJK_INSIDE) e = epa(sx, A, Bs, hs, L.e, seeded ? &sn : nullptr);
+    int gsum = giters, esum = gs == GJK_INSIDE ? e.iters : 0, fb = 0;
+    if (warm && (gs == GJK_FAILED || (gs == GJK_INSIDE && e.status != EPA_VALID))) {
+        fb = 1;
+        __syncthreads();
+        gs = gjk(A, Bs, hs, plain, sx, giters);
+        e = EpaResult{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
+        if (gs == GJK_INSIDE) e = epa(sx, A, Bs, hs, L.e, nullptr);
+        gsum += giters;
+        esum += gs == GJK_INSIDE ? e.iters : 0;
+    }
     if (gstat && lane64() == 0)
-        gstat[s] = (uint32_t)giters | ((uint32_t)e.iters << 8) | ((uint32_t)gs << 16) |
-                   ((uint32_t)(gs == GJK_INSIDE ? e.status : 15) << 20);
+        gstat[s] = (uint32_t)(gsum < 255 ? gsum : 255) | ((uint32_t)(esum < 255 ? esum : 255) << 8) | ((uint32_t)gs << 16) |
+                   ((uint32_t)(gs == GJK_INSIDE ? e.status : 15) << 20) | ((uint32_t)fb << 24);
+    if (tlog && lane64() == 0) {
+        const bool ok = gs == GJK_INSIDE && e.status == EPA_VALID;
+        V3 tg{0.0f, 0.0f, 0.0f};
+        float vg = 0.0f;
+        if (ok) {
+            tg = tf_inv_vec(ta, e.n);
+            vg = 1.0f;
+        } else if (gs == GJK_VALID) {
+            // a seed against the exit direction puts the first support on the far side of
+            // the Minkowski difference, so that -v0 already points across it at the origin
+            const float ln = len(sx.dir);
+            if (ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
+        }
+        tlog[2 * blockIdx.x] = f4(tg, vg);
+        tlog[2 * blockIdx.x + 1] = ok ? f4(tf_inv_vec(ta, e.n), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     Manifold m;
     m.np = 0;
     if (gs == GJK_INSIDE && e.status == EPA_VALID) {
@@ -786,6 +859,24 @@ __global__ void __launch_bounds__(64) k_narrow_gjk(const uint32_t *bin_slot, uin
     if (lane64() != 0) return;
     if (m.np > 0) emit(m, B, K, s, bb);
     else K.cnt[s] = 0;
+}
+
+// Totals over the GJK pairs bin_slot[off .. off + n) from gstat: pairs, GJK iterations,
+// EPA iterations, overlapping pairs, EPA converged, fallback runs.
+__global__ void k_gjk_totals(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *gstat,
+                             unsigned long long *tot) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t v[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const uint32_t g = gstat[bin_slot[off + i]];
+        const uint32_t gs = (g >> 16) & 15u, es = (g >> 20) & 15u;
+        v[0] = 1; v[1] = g & 255u; v[2] = (g >> 8) & 255u; v[3] = gs == GJK_INSIDE; v[4] = gs == GJK_INSIDE && es == EPA_VALID;
+        v[5] = (g >> 24) & 1u;
+    }
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t x = wave_sum(v[k]);
+        if (lane64() == 0 && x) atomicAdd(&tot[k], (unsigned long long)x);
+    }
 }
 
 // Hull against a plane (PlaneCollision.hpp:90-285): deepest support, then up to 32

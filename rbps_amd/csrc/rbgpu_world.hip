@@ -18,6 +18,7 @@
 #include "rbgpu_joints.hpp"
 #include "rbgpu_gjk.hpp"
 #include "rbgpu_hull.hpp"
+#include "rbgpu_mlp.hpp"
 
 using namespace rb;
 
@@ -106,6 +107,20 @@ struct rbgpu_world {
     HullMeta *hmeta = nullptr;
     HullSet hs{};
     uint32_t *gstat = nullptr;
+    // learned warm start: parameters (FP32 master, BF16 copy), gradient, Adam moments, the
+    // logged training set (features, targets) and the per-frame inference buffers
+    float *mlp_wf = nullptr, *mlp_grad = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_loss = nullptr;
+    bf16 *mlp_wb = nullptr;
+    uint64_t mlp_step = 0;
+    int mlp_mode = 0;                   // 0 plain, 1 GJK direction, 2 GJK direction + EPA seed
+    bf16 *ds_feat = nullptr;
+    float4 *ds_tgt = nullptr;
+    uint64_t ds_cap = 0, ds_n = 0;
+    bool ds_logging = false;
+    bf16 *inf_feat = nullptr;
+    float4 *inf_guess = nullptr;
+    uint32_t inf_cap = 0;
+    unsigned long long *gjk_tot = nullptr;   // iterations and counts of the GJK pairs (rbgpu_get_gjk_totals)
     Joints Jt{};
     uint32_t *jorder = nullptr;         // joints by colour
     std::vector<uint32_t> jcol_off;     // joint colour offsets
@@ -155,7 +170,10 @@ struct rbgpu_world {
     double timing_bytes = 0;
     bool timing_points_pending = false;
 
-    ~rbgpu_world() { release(); }
+    ~rbgpu_world() {
+        release();
+        release_mlp();
+    }
 
     void release_pairs() {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
@@ -164,6 +182,12 @@ struct rbgpu_world {
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
         pcap = 0;
+    }
+    void release_mlp() {
+        dfree(mlp_wf); dfree(mlp_grad); dfree(mlp_m); dfree(mlp_v); dfree(mlp_loss); dfree(mlp_wb);
+        dfree(ds_feat); dfree(ds_tgt); dfree(inf_feat); dfree(inf_guess); dfree(gjk_tot);
+        ds_cap = ds_n = 0;
+        inf_cap = 0;
     }
     void release() {
         if (st) (void)hipStreamSynchronize(st);
@@ -220,6 +244,25 @@ struct rbgpu_world {
         return RBGPU_OK;
     }
 
+    // Before the GJK bins [g0, g0 + ng): features of the pairs (for inference and/or the
+    // training log) and the network's warm-start predictions.
+    bool gjk_log = false;
+    bool gjk_explicit = false;   // warm starts given by the caller (rbgpu_collide_world_pairs)
+    bool gjk_guess = false;      // inf_guess holds this narrowphase's warm starts
+    void gjk_prepare(uint32_t g0, uint32_t ng) {
+        gjk_log = !gjk_explicit && ds_logging && ds_feat && ds_n + ng <= ds_cap;
+        if (ds_logging && !gjk_log && !gjk_explicit) ds_logging = false;   // the log is full
+        const bool infer = mlp_mode && mlp_wb && !gjk_explicit && inf_guess && ng <= inf_cap;
+        gjk_guess = gjk_explicit || infer;
+        if (!ng || !(infer || gjk_log)) return;
+        bf16 *feat = gjk_log ? ds_feat + ds_n * MLP_IN : inf_feat;
+        k_gjk_features<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, feat);
+        if (infer) {
+            const uint32_t tiles = (ng + MLP_TILE - 1) / MLP_TILE;
+            k_mlp_infer<<<std::min<uint32_t>(tiles, 1024), 256, 0, st>>>(feat, ng, MlpParams{mlp_wb, mlp_wf}, inf_guess);
+        }
+    }
+
     // Narrowphase of every bin of the current pair list (bin_off): one launch per bin.
     void launch_narrowphase() {
         for (int b = 0; b < NBINS; ++b) {
@@ -240,11 +283,39 @@ struct rbgpu_world {
                 case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                 case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                 case BIN_HH:
-                case BIN_HX: k_narrow_gjk<<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs, gstat, nullptr); break;
+                case BIN_HX: {
+                    // the two GJK bins are adjacent: features, inference and logging cover both
+                    const uint32_t g0 = bin_off[BIN_HH], ng = bin_off[BIN_HP] - g0;
+                    if (b == BIN_HH || bin_off[BIN_HH] == bin_off[BIN_HX]) gjk_prepare(g0, ng);
+                    const uint32_t rel = o - g0;
+                    hipEvent_t *e = timed_begin(11);
+                    k_narrow_gjk<<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs, gstat,
+                                                   gjk_guess ? inf_guess + 2 * (size_t)rel : nullptr,
+                                                   mlp_mode == 2 ? 1 : 0, gjk_log ? ds_tgt + 2 * (ds_n + rel) : nullptr);
+                    timed_end(e);
+                    if (e) timing_bytes += (double)n * gjk_pair_bytes();
+                    if (gjk_tot) k_gjk_totals<<<nblk(n), TPB, 0, st>>>(bin_list, o, n, gstat, gjk_tot);
+                    if (b == BIN_HX || bin_off[BIN_HX] == bin_off[BIN_HP]) {
+                        if (gjk_log) ds_n += ng;
+                        gjk_log = false;
+                    }
+                    break;
+                }
                 case BIN_HP: k_narrow_hull_plane<<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs); break;
                 default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
             }
         }
+    }
+
+    // Algorithmic bytes of one GJK pair (k_narrow_gjk): per side the hull vertices (16 B
+    // each, the world's mean count), hull meta 64 B, collider rows 68 B, body pose 32 B;
+    // the pair's slot, colliders and bodies 20 B; out: count, normal and four contact
+    // points (72 B each) 308 B.
+    double gjk_pair_bytes() const {
+        double nv = 0.0;
+        for (const HostHull &h : hh) nv += (double)h.t.v.size();
+        nv = hh.empty() ? 0.0 : nv / (double)hh.size();
+        return 2.0 * (16.0 * nv + 64.0 + 68.0 + 32.0) + 20.0 + 308.0;
     }
 
     hipEvent_t *timed_begin(int kernel) {
@@ -281,6 +352,11 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
     rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap);
     rc |= dalloc(bb_hits, nblk(cap, 128) * 128); rc |= dalloc(bb_blk, nblk(cap, 128)); rc |= dalloc(gstat, cap);
+    if (mlp_wb) {
+        dfree(inf_feat); dfree(inf_guess);
+        rc |= dalloc(inf_feat, (size_t)cap * MLP_IN); rc |= dalloc(inf_guess, 2 * (size_t)cap);
+        inf_cap = cap;
+    }
     rc |= dalloc(cb.acol, cap); rc |= dalloc(cb.tent, cap); rc |= dalloc(cb.rest, cap); rc |= dalloc(cb.act, cap);
     rc |= dalloc(cb.blk, nblk(cap) + 1);
     uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
@@ -1085,10 +1161,11 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
 }
 
 int rbgpu_set_timing(rbgpu_world *w, int kernel) {
-    if (!w || kernel < 0 || kernel > 10) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (!w || kernel < 0 || kernel > 11) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     if (w->st) HIPCHK(hipStreamSynchronize(w->st));
     w->timing_kernel = kernel;
     w->ev_used = 0;
+    w->timing_bytes = 0.0;
     if (w->d_points) HIPCHK(hipMemset(w->d_points, 0, 8));
     return RBGPU_OK;
 }
@@ -1109,7 +1186,8 @@ int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, doubl
         if (w->d_points) HIPCHK(hipMemcpy(&pts, w->d_points, 8, hipMemcpyDeviceToHost));
         // algorithmic bytes per contact point of the timed per-colour launches: position
         // solve 296 B, velocity solve 280 B
-        *bytes = (double)pts * (w->timing_kernel == 1 ? 296.0 : w->timing_kernel == 2 ? 280.0 : 0.0);
+        *bytes = w->timing_kernel == 11 ? w->timing_bytes
+                                        : (double)pts * (w->timing_kernel == 1 ? 296.0 : w->timing_kernel == 2 ? 280.0 : 0.0);
     }
     return RBGPU_OK;
 }
@@ -1237,7 +1315,8 @@ int rbgpu_hull_tables(const double *vertices, uint32_t n_vertices, const uint32_
     return RBGPU_OK;
 }
 
-int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2, double *out32, uint32_t *gjk_stats) {
+int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2, double *out32, uint32_t *gjk_stats,
+                              const float *guess6, int seed_epa) {
     if (!w || (n && (!pairs2 || !out32))) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     if (!w->built) {
         int rc = w->build();
@@ -1260,7 +1339,28 @@ int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2
     HIPCHK(hipMemcpyAsync(off.data(), w->offs, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     w->bin_off = off;
+    // explicit warm starts (per input pair, A's frame) are permuted into bin order
+    const int mode_saved = w->mlp_mode;
+    float4 *gin = nullptr, *gsave = w->inf_guess;
+    if (guess6) {
+        std::vector<float4> g(2 * (size_t)n);
+        for (uint32_t i = 0; i < n; ++i) {
+            g[2 * i] = make_float4(guess6[6 * i], guess6[6 * i + 1], guess6[6 * i + 2], 0.0f);
+            g[2 * i + 1] = make_float4(guess6[6 * i + 3], guess6[6 * i + 4], guess6[6 * i + 5], 0.0f);
+        }
+        rc = dalloc(gin, 4 * (size_t)n);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(gin + 2 * (size_t)n, g.data(), (size_t)n * 32, hipMemcpyHostToDevice, st));
+        const uint32_t g0 = off[BIN_HH], ng = off[BIN_HP] - off[BIN_HH];
+        if (ng) k_gather_guess<<<nblk(ng), TPB, 0, st>>>(w->bin_list + g0, ng, gin + 2 * (size_t)n, gin);
+        w->inf_guess = gin;
+        w->mlp_mode = seed_epa ? 2 : 1;
+    }
+    w->gjk_explicit = guess6 != nullptr;
     w->launch_narrowphase();
+    w->gjk_explicit = false;
+    w->mlp_mode = mode_saved;
+    w->inf_guess = gsave;
     HIPCHK(hipGetLastError());
     std::vector<int> cnt(n);
     std::vector<float4> nr(n), pa(4 * (size_t)n), pb(4 * (size_t)n), rbv(4 * (size_t)n);
@@ -1272,6 +1372,7 @@ int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2
     HIPCHK(hipMemcpyAsync(rbv.data(), w->K.rb, (size_t)n * 64, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(gs.data(), w->gstat, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    dfree(gin);
     for (uint32_t i = 0; i < n; ++i) {
         double *o = out32 + (size_t)i * 32;
         for (int k = 0; k < 32; ++k) o[k] = 0.0;
@@ -1285,6 +1386,237 @@ int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2
         }
     }
     if (gjk_stats) std::copy(gs.begin(), gs.end(), gjk_stats);
+    return RBGPU_OK;
+}
+
+// ---------------------------------------------------------------- learned warm start
+namespace {
+int need_mlp(rbgpu_world *w) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (!w->mlp_wb) return fail(RBGPU_ERR_INVALID, "warm-start network not initialised (rbgpu_mlp_init)");
+    if (!w->st) HIPCHK(hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking));
+    return RBGPU_OK;
+}
+uint64_t splitmix(uint64_t &x) {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+}  // namespace
+
+int rbgpu_mlp_init(rbgpu_world *w, uint64_t seed) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (!w->st) HIPCHK(hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking));
+    HIPCHK(hipStreamSynchronize(w->st));
+    w->release_mlp();
+    int rc = 0;
+    rc |= dalloc(w->mlp_wf, MLP_NPARAM); rc |= dalloc(w->mlp_wb, MLP_NPARAM); rc |= dalloc(w->mlp_grad, MLP_NPARAM);
+    rc |= dalloc(w->mlp_m, MLP_NPARAM); rc |= dalloc(w->mlp_v, MLP_NPARAM); rc |= dalloc(w->mlp_loss, 1);
+    if (w->pcap) {
+        rc |= dalloc(w->inf_feat, (size_t)w->pcap * MLP_IN); rc |= dalloc(w->inf_guess, 2 * (size_t)w->pcap);
+        w->inf_cap = w->pcap;
+    }
+    if (rc) return RBGPU_ERR_OOM;
+    // He-uniform weights, zero biases
+    std::vector<float> p(MLP_NPARAM, 0.0f);
+    uint64_t x = seed;
+    const int wo[4] = {MLP_W1, MLP_W2, MLP_W3, MLP_W4}, fin[4] = {MLP_IN, MLP_H, MLP_H, MLP_H};
+    const int fout[4] = {MLP_H, MLP_H, MLP_H, MLP_OUT};
+    for (int L = 0; L < 4; ++L) {
+        const double lim = std::sqrt(6.0 / fin[L]);
+        for (int i = 0; i < fin[L] * fout[L]; ++i)
+            p[wo[L] + i] = (float)(lim * (2.0 * (double)(splitmix(x) >> 11) / 9007199254740992.0 - 1.0));
+    }
+    w->mlp_step = 0;
+    return rbgpu_mlp_set_weights(w, p.data(), MLP_NPARAM);
+}
+
+int rbgpu_mlp_set_weights(rbgpu_world *w, const float *params, uint32_t n) {
+    int rc = need_mlp(w);
+    if (rc) return rc;
+    if (!params || n != (uint32_t)MLP_NPARAM) return fail(RBGPU_ERR_INVALID, "parameter count must be 39312");
+    hipStream_t st = w->st;
+    HIPCHK(hipMemcpyAsync(w->mlp_wf, params, MLP_NPARAM * 4, hipMemcpyHostToDevice, st));
+    k_f32_to_bf16<<<nblk(MLP_NPARAM), TPB, 0, st>>>(w->mlp_wf, w->mlp_wb, MLP_NPARAM);
+    HIPCHK(hipMemsetAsync(w->mlp_grad, 0, MLP_NPARAM * 4, st));
+    HIPCHK(hipMemsetAsync(w->mlp_m, 0, MLP_NPARAM * 4, st));
+    HIPCHK(hipMemsetAsync(w->mlp_v, 0, MLP_NPARAM * 4, st));
+    HIPCHK(hipStreamSynchronize(st));
+    w->mlp_step = 0;
+    return RBGPU_OK;
+}
+
+int rbgpu_mlp_get_weights(rbgpu_world *w, float *params, uint32_t n) {
+    int rc = need_mlp(w);
+    if (rc) return rc;
+    if (!params || n != (uint32_t)MLP_NPARAM) return fail(RBGPU_ERR_INVALID, "parameter count must be 39312");
+    HIPCHK(hipStreamSynchronize(w->st));
+    HIPCHK(hipMemcpy(params, w->mlp_wf, MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+    return RBGPU_OK;
+}
+
+int rbgpu_mlp_set_mode(rbgpu_world *w, int mode) {
+    if (!w || mode < 0 || mode > 2) return fail(RBGPU_ERR_INVALID, "mode: 0 plain, 1 GJK direction, 2 + EPA seed");
+    if (mode) {
+        int rc = need_mlp(w);
+        if (rc) return rc;
+    }
+    w->mlp_mode = mode;
+    return RBGPU_OK;
+}
+
+int rbgpu_mlp_log(rbgpu_world *w, uint64_t capacity) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (w->st) HIPCHK(hipStreamSynchronize(w->st));
+    dfree(w->ds_feat); dfree(w->ds_tgt);
+    w->ds_cap = w->ds_n = 0;
+    w->ds_logging = false;
+    if (!capacity) return RBGPU_OK;
+    int rc = 0;
+    rc |= dalloc(w->ds_feat, (size_t)capacity * MLP_IN); rc |= dalloc(w->ds_tgt, 2 * (size_t)capacity);
+    if (rc) return RBGPU_ERR_OOM;
+    w->ds_cap = capacity;
+    w->ds_logging = true;
+    return RBGPU_OK;
+}
+
+int rbgpu_mlp_logged(rbgpu_world *w, uint64_t *n, int stop) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (w->st) HIPCHK(hipStreamSynchronize(w->st));
+    if (n) *n = w->ds_n;
+    if (stop) w->ds_logging = false;
+    return RBGPU_OK;
+}
+
+int rbgpu_mlp_get_log(rbgpu_world *w, uint64_t start, uint32_t n, float *features32, float *targets8) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (start + n > w->ds_n) return fail(RBGPU_ERR_INVALID, "range beyond the logged samples");
+    if (!n) return RBGPU_OK;
+    HIPCHK(hipStreamSynchronize(w->st));
+    float *f = nullptr;
+    if (dalloc(f, (size_t)n * MLP_IN)) return RBGPU_ERR_OOM;
+    k_bf16_to_f32<<<nblk(n * MLP_IN), TPB, 0, w->st>>>(w->ds_feat + start * MLP_IN, f, n * MLP_IN);
+    int rc = RBGPU_OK;
+    if (features32 && hipMemcpyAsync(features32, f, (size_t)n * MLP_IN * 4, hipMemcpyDeviceToHost, w->st) != hipSuccess)
+        rc = fail(RBGPU_ERR_HIP, "copy failed");
+    if (targets8 && hipMemcpyAsync(targets8, w->ds_tgt + 2 * start, (size_t)n * 32, hipMemcpyDeviceToHost, w->st) != hipSuccess)
+        rc = fail(RBGPU_ERR_HIP, "copy failed");
+    if (hipStreamSynchronize(w->st) != hipSuccess) rc = fail(RBGPU_ERR_HIP, "rbgpu_mlp_get_log failed");
+    dfree(f);
+    return rc;
+}
+
+int rbgpu_mlp_train(rbgpu_world *w, int steps, uint32_t batch, double lr, double *mean_loss) {
+    int rc = need_mlp(w);
+    if (rc) return rc;
+    if (steps < 0 || batch == 0 || batch > (1u << 20) || !(lr > 0.0)) return fail(RBGPU_ERR_INVALID, "invalid training arguments");
+    if (!w->ds_n) return fail(RBGPU_ERR_INVALID, "no logged samples (rbgpu_mlp_log, then step)");
+    hipStream_t st = w->st;
+    bf16 *bf = nullptr;
+    float4 *bt = nullptr;
+    rc = dalloc(bf, (size_t)batch * MLP_IN);
+    rc |= dalloc(bt, 2 * (size_t)batch);
+    if (rc) { dfree(bf); dfree(bt); return RBGPU_ERR_OOM; }
+    // sample order: a multiplier coprime to the log size walks it as a permutation
+    const uint64_t N = w->ds_n;
+    uint64_t mult = 2654435761ull % N;
+    auto gcd = [](uint64_t a, uint64_t b) { while (b) { uint64_t t = a % b; a = b; b = t; } return a; };
+    while (N > 1 && (mult == 0 || gcd(mult, N) != 1)) ++mult;
+    if (N == 1) mult = 1;
+    HIPCHK(hipMemsetAsync(w->mlp_loss, 0, 4, st));
+    const uint32_t tiles = (batch + MLP_TILE - 1) / MLP_TILE;
+    for (int k = 0; k < steps; ++k) {
+        const uint64_t start = (w->mlp_step * (uint64_t)batch) % N;
+        k_mlp_gather<<<nblk(batch), TPB, 0, st>>>(w->ds_feat, w->ds_tgt, N, start, mult, batch, bf, bt);
+        k_mlp_grad<<<std::min<uint32_t>(tiles, 512), 256, 0, st>>>(bf, bt, batch, MlpParams{w->mlp_wb, w->mlp_wf},
+                                                                  1.0f / (float)batch, w->mlp_grad, w->mlp_loss);
+        ++w->mlp_step;
+        const float b1c = 1.0f - std::pow(0.9f, (float)w->mlp_step), b2c = 1.0f - std::pow(0.999f, (float)w->mlp_step);
+        k_mlp_adam<<<nblk(MLP_NPARAM), TPB, 0, st>>>(w->mlp_wf, w->mlp_wb, w->mlp_grad, w->mlp_m, w->mlp_v, MLP_NPARAM,
+                                                     (float)lr, b1c, b2c);
+    }
+    float loss = 0.0f;
+    HIPCHK(hipMemcpyAsync(&loss, w->mlp_loss, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(bf);
+    dfree(bt);
+    if (mean_loss) *mean_loss = steps ? (double)loss / steps : 0.0;
+    return RBGPU_OK;
+}
+
+int rbgpu_mlp_eval(rbgpu_world *w, uint32_t n, const float *feat32, float *out6) {
+    int rc = need_mlp(w);
+    if (rc) return rc;
+    if (n && (!feat32 || !out6)) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (!n) return RBGPU_OK;
+    hipStream_t st = w->st;
+    float *f = nullptr;
+    bf16 *fb = nullptr;
+    float4 *o = nullptr;
+    rc = dalloc(f, (size_t)n * MLP_IN);
+    rc |= dalloc(fb, (size_t)n * MLP_IN);
+    rc |= dalloc(o, 2 * (size_t)n);
+    if (!rc) {
+        (void)hipMemcpyAsync(f, feat32, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, st);
+        k_f32_to_bf16<<<nblk(n * MLP_IN), TPB, 0, st>>>(f, fb, n * MLP_IN);
+        k_mlp_infer<<<std::min<uint32_t>((n + MLP_TILE - 1) / MLP_TILE, 1024), 256, 0, st>>>(fb, n, MlpParams{w->mlp_wb, w->mlp_wf}, o);
+        std::vector<float4> h(2 * (size_t)n);
+        (void)hipMemcpyAsync(h.data(), o, 2 * (size_t)n * 16, hipMemcpyDeviceToHost, st);
+        if (hipStreamSynchronize(st) != hipSuccess) rc = fail(RBGPU_ERR_HIP, "rbgpu_mlp_eval failed");
+        for (uint32_t i = 0; i < n && !rc; ++i) {
+            const float4 a = h[2 * i], b = h[2 * i + 1];
+            float *q = out6 + 6 * (size_t)i;
+            q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = b.x; q[4] = b.y; q[5] = b.z;
+        }
+    }
+    dfree(f); dfree(fb); dfree(o);
+    return rc;
+}
+
+int rbgpu_mlp_gradients(rbgpu_world *w, uint32_t n, const float *feat32, const float *tgt8, float *grad, double *loss) {
+    int rc = need_mlp(w);
+    if (rc) return rc;
+    if (!n || !feat32 || !tgt8 || !grad) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    hipStream_t st = w->st;
+    float *f = nullptr, *g = nullptr, *l = nullptr;
+    bf16 *fb = nullptr;
+    float4 *t = nullptr;
+    rc = dalloc(f, (size_t)n * MLP_IN);
+    rc |= dalloc(fb, (size_t)n * MLP_IN);
+    rc |= dalloc(t, 2 * (size_t)n);
+    rc |= dalloc(g, MLP_NPARAM);
+    rc |= dalloc(l, 1);
+    if (!rc) {
+        (void)hipMemcpyAsync(f, feat32, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, st);
+        (void)hipMemcpyAsync(t, tgt8, (size_t)n * 32, hipMemcpyHostToDevice, st);
+        (void)hipMemsetAsync(g, 0, MLP_NPARAM * 4, st);
+        (void)hipMemsetAsync(l, 0, 4, st);
+        k_f32_to_bf16<<<nblk(n * MLP_IN), TPB, 0, st>>>(f, fb, n * MLP_IN);
+        k_mlp_grad<<<std::min<uint32_t>((n + MLP_TILE - 1) / MLP_TILE, 512), 256, 0, st>>>(
+            fb, t, n, MlpParams{w->mlp_wb, w->mlp_wf}, 1.0f / (float)n, g, l);
+        float lh = 0.0f;
+        (void)hipMemcpyAsync(grad, g, MLP_NPARAM * 4, hipMemcpyDeviceToHost, st);
+        (void)hipMemcpyAsync(&lh, l, 4, hipMemcpyDeviceToHost, st);
+        if (hipStreamSynchronize(st) != hipSuccess) rc = fail(RBGPU_ERR_HIP, "rbgpu_mlp_gradients failed");
+        if (loss) *loss = lh;
+    }
+    dfree(f); dfree(fb); dfree(t); dfree(g); dfree(l);
+    return rc;
+}
+
+int rbgpu_get_gjk_totals(rbgpu_world *w, uint64_t *out6, int reset) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (!w->st) HIPCHK(hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking));
+    if (!w->gjk_tot) {
+        if (dalloc(w->gjk_tot, 6)) return RBGPU_ERR_OOM;
+        HIPCHK(hipMemsetAsync(w->gjk_tot, 0, 48, w->st));
+    }
+    unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, w->gjk_tot, 48, hipMemcpyDeviceToHost, w->st));
+    if (reset) HIPCHK(hipMemsetAsync(w->gjk_tot, 0, 48, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
+    if (out6) for (int k = 0; k < 6; ++k) out6[k] = h[k];
     return RBGPU_OK;
 }
 

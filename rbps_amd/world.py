@@ -31,6 +31,8 @@ EXPORTS = [
     "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_collide_pairs",
     "rbgpu_set_option", "rbgpu_get_option", "rbgpu_add_joints", "rbgpu_get_joints", "rbgpu_set_joint_targets",
     "rbgpu_add_hull", "rbgpu_hull_mass_properties", "rbgpu_hull_tables", "rbgpu_collide_world_pairs",
+    "rbgpu_mlp_init", "rbgpu_mlp_set_weights", "rbgpu_mlp_get_weights", "rbgpu_mlp_set_mode", "rbgpu_mlp_log",
+    "rbgpu_mlp_logged", "rbgpu_mlp_get_log", "rbgpu_mlp_train", "rbgpu_mlp_eval", "rbgpu_mlp_gradients", "rbgpu_get_gjk_totals",
     "rbgpu_last_error",
 ]
 
@@ -79,7 +81,19 @@ def lib():
         L.rbgpu_add_hull.argtypes = [_P, _D, C.c_uint32, _U, C.c_uint32, _U]
         L.rbgpu_hull_mass_properties.argtypes = [_D, C.c_uint32, _U, C.c_uint32, _D, _D]
         L.rbgpu_hull_tables.argtypes = [_D, C.c_uint32, _U, C.c_uint32, _U, _D] + [_U] * 8
-        L.rbgpu_collide_world_pairs.argtypes = [_P, C.c_uint32, _U, _D, _U]
+        _F = C.POINTER(C.c_float)
+        L.rbgpu_collide_world_pairs.argtypes = [_P, C.c_uint32, _U, _D, _U, _F, C.c_int]
+        L.rbgpu_mlp_init.argtypes = [_P, C.c_uint64]
+        L.rbgpu_mlp_set_weights.argtypes = [_P, _F, C.c_uint32]
+        L.rbgpu_mlp_get_weights.argtypes = [_P, _F, C.c_uint32]
+        L.rbgpu_mlp_set_mode.argtypes = [_P, C.c_int]
+        L.rbgpu_mlp_log.argtypes = [_P, C.c_uint64]
+        L.rbgpu_mlp_logged.argtypes = [_P, C.POINTER(C.c_uint64), C.c_int]
+        L.rbgpu_mlp_train.argtypes = [_P, C.c_int, C.c_uint32, C.c_double, _D]
+        L.rbgpu_mlp_get_log.argtypes = [_P, C.c_uint64, C.c_uint32, _F, _F]
+        L.rbgpu_mlp_eval.argtypes = [_P, C.c_uint32, _F, _F]
+        L.rbgpu_mlp_gradients.argtypes = [_P, C.c_uint32, _F, _F, _F, _D]
+        L.rbgpu_get_gjk_totals.argtypes = [_P, C.POINTER(C.c_uint64), C.c_int]
         L.rbgpu_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -143,7 +157,7 @@ OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, 
            "joint_mode": 6}
 
 TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "broadphase": 6,
-          "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9, "joints": 10}
+          "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9, "joints": 10, "gjk": 11}
 
 
 class World:
@@ -247,22 +261,87 @@ class World:
             self.add_joints(js)
         return self
 
-    def collide_pairs(self, pairs) -> dict:
+    def collide_pairs(self, pairs, guess=None, seed_epa=False) -> dict:
         """Narrowphase of collider pairs (a, b) at the current state through the pipeline's
         binned kernels: points, normal, surface points p_a / p_b and depths per pair, and the
-        GJK/EPA statistics (iterations, statuses; -1 where no GJK ran)."""
+        GJK/EPA statistics (iterations, statuses, fallback; -1 where no GJK ran).  guess
+        (n x 6, A's frame): explicit warm starts (GJK direction, EPA seed normal)."""
         pr = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
         n = len(pr)
         out = np.zeros((n, 32))
         gs = np.zeros(n, np.uint32)
-        _check(lib().rbgpu_collide_world_pairs(self.h, n, _ptr(pr, _U), _ptr(out, _D), _ptr(gs, _U)))
+        g = None if guess is None else np.ascontiguousarray(guess, np.float32).reshape(n, 6)
+        _check(lib().rbgpu_collide_world_pairs(self.h, n, _ptr(pr, _U), _ptr(out, _D), _ptr(gs, _U),
+                                               _ptr(g, C.POINTER(C.c_float)), 1 if seed_epa else 0))
         cnt = out[:, 0].astype(int)
         pts = out[:, 4:].reshape(n, 4, 7)
         nogjk = gs == 0xFFFFFFFF
         g = gs.astype(np.int64)
         return {"n": cnt, "normal": out[:, 1:4], "p_a": pts[:, :, 0:3], "p_b": pts[:, :, 3:6], "depth": pts[:, :, 6],
                 "gjk_iters": np.where(nogjk, -1, g & 255), "epa_iters": np.where(nogjk, -1, (g >> 8) & 255),
-                "gjk_status": np.where(nogjk, -1, (g >> 16) & 15), "epa_status": np.where(nogjk, -1, (g >> 20) & 15)}
+                "gjk_status": np.where(nogjk, -1, (g >> 16) & 15), "epa_status": np.where(nogjk, -1, (g >> 20) & 15),
+                "fallback": np.where(nogjk, -1, (g >> 24) & 1)}
+
+    # ---------------------------------------------------------------- learned warm start
+    MLP_NPARAM = 39312
+    MLP_MODES = {"plain": 0, "gjk": 1, "gjk+epa": 2}
+
+    def mlp_init(self, seed: int = 1):
+        _check(lib().rbgpu_mlp_init(self.h, seed))
+
+    def mlp_weights(self) -> np.ndarray:
+        p = np.zeros(self.MLP_NPARAM, np.float32)
+        _check(lib().rbgpu_mlp_get_weights(self.h, _ptr(p, C.POINTER(C.c_float)), len(p)))
+        return p
+
+    def mlp_set_weights(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().rbgpu_mlp_set_weights(self.h, _ptr(p, C.POINTER(C.c_float)), len(p)))
+
+    def mlp_mode(self, mode: str):
+        _check(lib().rbgpu_mlp_set_mode(self.h, self.MLP_MODES[mode]))
+
+    def mlp_log(self, capacity: int):
+        _check(lib().rbgpu_mlp_log(self.h, int(capacity)))
+
+    def mlp_logged(self, stop: bool = False) -> int:
+        n = C.c_uint64()
+        _check(lib().rbgpu_mlp_logged(self.h, C.byref(n), 1 if stop else 0))
+        return int(n.value)
+
+    def mlp_log_samples(self, start: int, n: int):
+        """Logged features (n x 32) and targets (n x 8: GJK target xyz, valid, EPA target xyz, valid)."""
+        f = np.zeros((n, 32), np.float32)
+        t = np.zeros((n, 8), np.float32)
+        _F = C.POINTER(C.c_float)
+        _check(lib().rbgpu_mlp_get_log(self.h, start, n, _ptr(f, _F), _ptr(t, _F)))
+        return f, t
+
+    def mlp_train(self, steps: int, batch: int = 4096, lr: float = 1e-3) -> float:
+        loss = np.zeros(1)
+        _check(lib().rbgpu_mlp_train(self.h, steps, batch, lr, _ptr(loss, _D)))
+        return float(loss[0])
+
+    def mlp_eval(self, features) -> np.ndarray:
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 32)
+        out = np.zeros((len(f), 6), np.float32)
+        _check(lib().rbgpu_mlp_eval(self.h, len(f), _ptr(f, C.POINTER(C.c_float)), _ptr(out, C.POINTER(C.c_float))))
+        return out
+
+    def mlp_gradients(self, features, targets):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 32)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 8)
+        g = np.zeros(self.MLP_NPARAM, np.float32)
+        loss = np.zeros(1)
+        _F = C.POINTER(C.c_float)
+        _check(lib().rbgpu_mlp_gradients(self.h, len(f), _ptr(f, _F), _ptr(t, _F), _ptr(g, _F), _ptr(loss, _D)))
+        return g, float(loss[0])
+
+    def gjk_totals(self, reset: bool = True) -> dict:
+        out = (C.c_uint64 * 6)()
+        _check(lib().rbgpu_get_gjk_totals(self.h, out, 1 if reset else 0))
+        k = ("pairs", "gjk_iters", "epa_iters", "overlapping", "epa_valid", "fallbacks")
+        return {a: int(b) for a, b in zip(k, out)}
 
     # ---------------------------------------------------------------- step
     def step(self, frames: int = 1):

@@ -1,0 +1,169 @@
+"""The warm-start network on the GPU (BF16 MFMA inference and training kernels) against
+its CPU restatement (oracle/mlp.py), the warm-started GJK/EPA kernel against the FP32
+oracle, and what the trained network gains over the plain seeds.
+
+Bars: network outputs and gradients agree with the restatement up to BF16 rounding
+(the restatement rounds at the same points; accumulation order differs), warm-started
+GJK/EPA with given seeds is bit-exact with the FP32 oracle, and the trained network
+lowers the GJK + EPA iterations on a held-out pile while the contacts stay within the
+EPA tolerance of the plain run.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp, orc
+from rbps_amd import scenes
+from tests.conftest import golden
+from tests.helpers import unpack_hulls
+
+pytestmark = pytest.mark.gpu
+
+
+def _world(seed=7):
+    from rbps_amd import world
+    w = world.World()
+    w.mlp_init(seed)
+    return w
+
+
+def _unit(x):
+    return x / np.linalg.norm(x, axis=-1, keepdims=True)
+
+
+def test_gpu_mlp_forward_matches_oracle():
+    rng = np.random.default_rng(1)
+    w = _world()
+    p = mlp.init_params(2)
+    w.mlp_set_weights(p)
+    feat = (rng.standard_normal((300, mlp.IN)) * 0.5).astype(np.float32)
+    print("case mlp forward", flush=True)
+    gpu = w.mlp_eval(feat)
+    ref = mlp.predict(p, feat)
+    for h in range(2):
+        cos = np.einsum("ij,ij->i", gpu[:, 3 * h:3 * h + 3], ref[:, 3 * h:3 * h + 3])
+        assert cos.min() > 0.999, (h, cos.min())
+        assert np.median(cos) > 0.99999
+
+
+def test_gpu_mlp_gradients_match_oracle():
+    rng = np.random.default_rng(2)
+    w = _world()
+    p = mlp.init_params(3)
+    w.mlp_set_weights(p)
+    n = 200
+    feat = (rng.standard_normal((n, mlp.IN)) * 0.5).astype(np.float32)
+    tgt = np.zeros((n, 8), np.float32)
+    tgt[:, 0:3], tgt[:, 3] = _unit(rng.standard_normal((n, 3))), 1.0
+    tgt[:, 4:7], tgt[:, 7] = _unit(rng.standard_normal((n, 3))), (rng.random(n) < 0.5)
+    print("case mlp gradients", flush=True)
+    g, loss = w.mlp_gradients(feat, tgt)
+    lr, gr = mlp.loss_and_grad(p, feat, tgt, rounded=True)
+    assert abs(loss - lr) <= 1e-5 + 1e-4 * abs(lr), (loss, lr)
+    for wo, bo, k, nn in mlp.LAYERS:
+        for a, b in ((g[wo:wo + nn * k], gr[wo:wo + nn * k]), (g[bo:bo + nn], gr[bo:bo + nn])):
+            nb = np.linalg.norm(b)
+            if nb == 0:
+                assert not np.any(a)
+                continue
+            assert np.linalg.norm(a - b) <= 2e-2 * nb, (wo, np.linalg.norm(a - b) / nb)
+
+
+def _pair_world(g, hulls):
+    from rbps_amd import world
+    sb = scenes.SceneBuilder()
+    for v, f in hulls:
+        sb.hull(v, f)
+    for i in range(len(g["a"])):
+        for rec, tf in ((g["a"][i], g["tfa"][i]), (g["b"][i], g["tfb"][i])):
+            st = rec["shape"] == scenes.PLANE
+            body = sb.body(scenes.STATIC if st else scenes.DYNAMIC, 0.0 if st else 1.0, pos=tuple(tf[:3]), quat=tuple(tf[3:]))
+            sb.collider(int(rec["shape"]), body, tuple(rec["dims"]), is_static=st, hull_index=int(rec["hull_index"]))
+    w = world.World().load(sb.build("pairs"))
+    return w, np.arange(2 * len(g["a"]), dtype=np.uint32).reshape(-1, 2)
+
+
+def test_gpu_warm_started_gjk_epa_bit_exact_vs_fp32_oracle():
+    """Given seeds (random unit vectors in A's frame) for GJK and EPA: statuses, iteration
+    counts (fallback runs included), fallback flags and manifolds equal the oracle's."""
+    g = golden("hull_pairs")
+    hulls = unpack_hulls(g)
+    rng = np.random.default_rng(9)
+    n = len(g["a"])
+    G = np.concatenate([_unit(rng.standard_normal((n, 3))), _unit(rng.standard_normal((n, 3)))], 1).astype(np.float32)
+    w, pairs = _pair_world(g, hulls)
+    print("case warm-started pairs", n, flush=True)
+    out = w.collide_pairs(pairs, guess=G, seed_epa=True)
+    n_gjk = 0
+    for i in range(n):
+        a, b = g["a"][i], g["b"][i]
+        r = orc.oracle_collide_warm(a, g["tfa"][i], b, g["tfb"][i], G[i].astype(np.float64), fp32=True, hulls=hulls)
+        assert out["n"][i] == r["n"], i
+        if r["n"]:
+            assert np.array_equal(out["normal"][i], np.float32(r["normal"]).astype(np.float64)), i
+            half = (r["depth"].astype(np.float32) * np.float32(0.5)).astype(np.float32)
+            pa = r["points"].astype(np.float32) + np.float32(r["normal"]).astype(np.float32)[None, :] * half[:, None]
+            assert np.array_equal(out["p_a"][i, :r["n"]], pa.astype(np.float64)), i
+            assert np.array_equal(out["depth"][i, :r["n"]], r["depth"].astype(np.float32).astype(np.float64)), i
+        if b["shape"] != scenes.PLANE:
+            s = orc.oracle_gjk_epa(a, g["tfa"][i], b, g["tfb"][i], fp32=True, hulls=hulls, guess=G[i].astype(np.float64))
+            assert out["gjk_status"][i] == s["gjk_status"] and out["gjk_iters"][i] == s["gjk_iters"], i
+            assert out["fallback"][i] == s["fallback"], i
+            if s["gjk_status"] == orc.GJK_INSIDE:
+                assert out["epa_status"][i] == s["epa_status"] and out["epa_iters"][i] == s["epa_iters"], i
+            n_gjk += 1
+    assert n_gjk > 1000
+
+
+def _gjk_mask(w, pairs, sc):
+    k = sc.colliders["shape"][pairs]
+    return ((k == scenes.HULL).any(1)) & ~((k == scenes.PLANE).any(1))
+
+
+def test_gpu_learned_warm_start_gains_on_held_out_pile():
+    """Train on samples logged from one hull pile, evaluate on another (different seed):
+    fewer GJK + EPA iterations than the plain seeds, same contacts within the EPA
+    tolerance (normals, depths), the plain path as fallback."""
+    from rbps_amd import world
+    tr = scenes.hull_pile(1024, seed=31)
+    w = world.World(tr.timestep, tr.substeps).load(tr)
+    w.mlp_init(5)
+    w.step(30)
+    w.mlp_log(600_000)
+    w.step(30)
+    n_log = w.mlp_logged(stop=True)
+    print("case learned: logged", n_log, flush=True)
+    assert n_log > 100_000
+    f, t = w.mlp_log_samples(0, 1000)
+    assert np.allclose(f[:, 24:28].sum(1), 1.0) and np.allclose(f[:, 28:32].sum(1), 1.0)   # kind one-hots
+    assert np.all(t[:, 3] == 1.0) and np.allclose(np.linalg.norm(t[:, 0:3], axis=1), 1.0, atol=1e-5)
+    loss0 = w.mlp_train(1, 8192, 1e-3)
+    loss = w.mlp_train(600, 8192, 2e-3)
+    print("train loss", loss0, "->", loss, flush=True)
+    assert loss < 0.5 * loss0
+    te = scenes.hull_pile(1024, seed=32)
+    w2 = world.World(te.timestep, te.substeps).load(te)
+    w2.step(60)
+    w2.mlp_init(5)
+    w2.mlp_set_weights(w.mlp_weights())
+    pairs = w2.sap_pairs()
+    sel = _gjk_mask(w2, pairs, te)
+    pairs = pairs[sel]
+    plain = w2.collide_pairs(pairs)
+    w2.mlp_mode("gjk")
+    learned = w2.collide_pairs(pairs)
+    w2.mlp_init(5)                     # the same network untrained
+    untrained = w2.collide_pairs(pairs)
+    w2.mlp_mode("plain")
+    it_p = int((plain["gjk_iters"] + plain["epa_iters"]).sum())
+    it_l = int((learned["gjk_iters"] + learned["epa_iters"]).sum())
+    it_u = int((untrained["gjk_iters"] + untrained["epa_iters"]).sum())
+    print("held-out pairs", len(pairs), "iterations plain", it_p, "learned", it_l, "untrained", it_u,
+          "fallbacks", int(learned["fallback"].sum()), flush=True)
+    assert it_l < it_p and it_l < it_u
+    hp, hl = plain["n"] > 0, learned["n"] > 0
+    assert np.mean(hp == hl) > 0.995
+    both = hp & hl
+    cos = np.einsum("ij,ij->i", plain["normal"][both], learned["normal"][both])
+    dd = np.abs(plain["depth"][both, 0] - learned["depth"][both, 0])
+    print("normal cos p1", np.percentile(cos, 1), "depth diff p99", np.percentile(dd, 99), flush=True)
+    assert np.percentile(cos, 1) > 0.999 and np.percentile(dd, 99) < 1e-4
