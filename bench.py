@@ -78,6 +78,51 @@ def cpu_baseline_batched(budget_s=10.0, frames=10):
                       f"reference's order because the reference has no spherical joint)"}
 
 
+def learned_report(w, sc, train_frames=20, capacity=4_000_000, steps=1200, batch=8192):
+    """The learned warm start on the hull workload (after the timed plain run): log the
+    GJK pairs of `train_frames` frames, train, compare on the current state's GJK pairs
+    the iterations and contacts of the plain and learned seeds, then time frames with the
+    learned seeds.  (Errors against FP64 are measured by tests/test_gpu_mlp.py.)"""
+    t0 = time.perf_counter()
+    w.mlp_init(5)
+    w.mlp_log(capacity)
+    w.step(train_frames)
+    logged = w.mlp_logged(stop=True)
+    t1 = time.perf_counter()
+    w.mlp_train(steps // 2, batch, 2e-3)
+    loss = w.mlp_train(steps - steps // 2, batch, 5e-4)
+    w.synchronize()
+    t2 = time.perf_counter()
+    pairs = w.sap_pairs()
+    k = sc.colliders["shape"][pairs]
+    pairs = pairs[(k == 9).any(1) & ~(k == 6).any(1)]
+    w.mlp_mode("plain")
+    rp = w.collide_pairs(pairs)
+    w.mlp_mode("gjk")
+    rl = w.collide_pairs(pairs)
+    both = (rp["n"] > 0) & (rl["n"] > 0)
+    cos = np.einsum("ij,ij->i", rp["normal"][both], rl["normal"][both])
+    dd = np.abs(rp["depth"][both, 0] - rl["depth"][both, 0])
+    frames = 20
+    res = {}
+    for mode in ("plain", "gjk"):
+        w.mlp_mode(mode)
+        w.synchronize()
+        t3 = time.perf_counter()
+        w.step(frames)
+        w.synchronize()
+        res[mode] = sc.n_dynamic * sc.substeps * frames / (time.perf_counter() - t3)
+    w.mlp_mode("plain")
+    it = lambda r: float((r["gjk_iters"] + r["epa_iters"]).mean())   # noqa: E731
+    return {"value": round(res["gjk"], 1), "plain_same_frames": round(res["plain"], 1), "unit": "body-substeps/s",
+            "frames_timed": frames, "logged_samples": logged, "log_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+            "train_loss": round(loss, 4), "gjk_pairs": int(len(pairs)),
+            "iterations_per_pair": {"plain": round(it(rp), 4), "learned": round(it(rl), 4)},
+            "fallbacks": int(rl["fallback"].sum()), "contacts_agree": float(np.mean((rp["n"] > 0) == (rl["n"] > 0))),
+            "normal_deg_max_vs_plain": round(float(np.degrees(np.arccos(np.clip(cos.min(), -1, 1)))), 5) if both.any() else None,
+            "depth_abs_max_vs_plain": float(dd.max()) if both.any() else None}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload
     (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings), or None."""
@@ -109,6 +154,7 @@ def main():
                              "solve_pos_runs", "solve_vel_runs", "joints", "gjk"],
                     help="kernel family timed for the roofline (default: solve_pos; gjk for --workload hulls)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="hulls: skip the learned warm-start report")
     ap.add_argument("--workload", default="pile", choices=["pile", "batched", "hulls"],
                     help="pile: the 65,536-body mixed pile (default); batched: 4,096 independent "
                          "box-stack + capsule-chain scenes (1.1M bodies) in one world; hulls: 32,768 "
@@ -181,6 +227,8 @@ def main():
                    "last_frame": st},
         "roofline": roof if args.workload == "pile" else dict(roof or {}, traffic=None),
     }
+    if args.workload == "hulls" and not args.no_learned:
+        out["learned"] = learned_report(w, sc)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_batched() if args.workload == "batched" else cpu_baseline(lambda: sc)
     print(json.dumps(out))

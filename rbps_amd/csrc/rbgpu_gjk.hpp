@@ -26,7 +26,7 @@
 namespace rb {
 
 constexpr int HULL_MAX_VERTS = 256;
-constexpr int HULL_VPL = HULL_MAX_VERTS / 64;     // vertices per lane
+constexpr int HULL_VPL = HULL_MAX_VERTS / 64;     // vertices per lane (kernels for hulls of up to 64 use 1)
 constexpr int HULL_MAX_POLY = 64;                 // vertices of one polygon, neighbours of one polygon
 constexpr int EPA_FACES = 256, EPA_VERTS = 128, EPA_ITERS = 128, GJK_ITERS = 128;
 constexpr int GJK_WARM_ITERS = 16;   // a warm-started GJK not done by then falls back to the plain seed
@@ -65,58 +65,114 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 // ------------------------------------------------------------------ shapes in a wave
-struct WaveHull {   // this lane's vertices lane + 64k
-    float x[HULL_VPL], y[HULL_VPL], z[HULL_VPL];
+template <int VPL> struct WaveHull {   // this lane's vertices lane + 64k
+    float x[VPL], y[VPL], z[VPL];
     int nv;
     uint32_t v0;
 };
-__device__ __forceinline__ void load_wave_hull(WaveHull &w, const HullSet &hs, uint32_t h) {
+template <int VPL> __device__ __forceinline__ void load_wave_hull(WaveHull<VPL> &w, const HullSet &hs, uint32_t h) {
     const HullMeta m = hs.meta[h];
     w.nv = (int)m.nv;
     w.v0 = m.v0;
 #pragma unroll
-    for (int k = 0; k < HULL_VPL; ++k) {
+    for (int k = 0; k < VPL; ++k) {
         const int i = lane64() + 64 * k;
         float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (i < w.nv) p = hs.v[m.v0 + i];
         w.x[k] = p.x; w.y[k] = p.y; w.z[k] = p.z;
     }
 }
-// Index of the first vertex with the largest dot(v, d) (uniform result).
-__device__ __forceinline__ int wave_support_index(const WaveHull &w, V3 d) {
-    float best = -__builtin_inff();
-    int bi = 0x7fffffff;
+// Cross-lane max / min: DPP within each 16-lane row (quad swaps, half-row and row
+// mirrors), then the four row results through readlane.  Every lane must be active.
+__device__ __forceinline__ uint32_t dpp_row_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));    // quad_perm [1,0,3,2]
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));    // quad_perm [2,3,0,1]
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));   // row_mirror
+    return v;
+}
+__device__ __forceinline__ uint32_t dpp_row_min(uint32_t v) {
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
+    return v;
+}
+__device__ __forceinline__ uint32_t rl(uint32_t v, int lane) { return (uint32_t)__builtin_amdgcn_readlane((int)v, lane); }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    v = dpp_row_max(v);
+    return max(max(rl(v, 0), rl(v, 16)), max(rl(v, 32), rl(v, 48)));
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = dpp_row_min(v);
+    return min(min(rl(v, 0), rl(v, 16)), min(rl(v, 32), rl(v, 48)));
+}
+// Order-preserving map of a non-NaN float to u32 (-0 maps with +0, as the comparisons treat them).
+__device__ __forceinline__ uint32_t ord_f(float x) {
+    const uint32_t u = __float_as_uint(x + 0.0f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Index of the first vertex with the largest dot(v, d) (uniform result); its local
+// coordinates in *vout.  Lane-local scan in index order, wave max of the mapped value,
+// then the lowest index among the lanes holding it.  A NaN at vertex 0 is never beaten
+// in the sequential scan; NaN elsewhere never wins.
+template <int VPL> __device__ __forceinline__ int wave_support_index(const WaveHull<VPL> &w, V3 d, V3 *vout = nullptr) {
+    const int lane = lane64();
+    uint32_t best = 0u;
+    int bk = 0;
     bool nan0 = false;
 #pragma unroll
-    for (int k = 0; k < HULL_VPL; ++k) {
-        const int i = lane64() + 64 * k;
+    for (int k = 0; k < VPL; ++k) {
+        const int i = lane + 64 * k;
         if (i < w.nv) {
             float x = w.x[k] * d.x + w.y[k] * d.y + w.z[k] * d.z;
             if (x != x) {
                 nan0 |= i == 0;
                 x = -__builtin_inff();
             }
-            if (x > best || (x == best && i < bi)) { best = x; bi = i; }
+            const uint32_t o = ord_f(x);
+            if (o > best) { best = o; bk = k; }
         }
     }
-    wave_argmax(best, bi);
-    // a NaN first value is never beaten in the sequential scan
-    return __shfl((int)nan0, 0, 64) ? 0 : bi;
+    const uint32_t m = wave_max_u32(best);
+    const bool cand = best == m;   // lanes without vertices hold 0, below any mapped value
+    int idx;
+    if (w.nv <= 64) {
+        idx = __ffsll((long long)__ballot(cand)) - 1;
+    } else {
+        idx = (int)wave_min_u32(cand ? (uint32_t)(lane + 64 * bk) : 0xffffffffu);
+    }
+    if (rl((uint32_t)nan0, 0)) idx = 0;
+    if (vout) {
+        const int src = idx & 63, slot = idx >> 6;
+        float x = w.x[0], y = w.y[0], z = w.z[0];
+#pragma unroll
+        for (int k = 1; k < VPL; ++k)
+            if (slot == k) { x = w.x[k]; y = w.y[k]; z = w.z[k]; }
+        *vout = V3{__uint_as_float(rl(__float_as_uint(x), src)), __uint_as_float(rl(__float_as_uint(y), src)),
+                   __uint_as_float(rl(__float_as_uint(z), src))};
+    }
+    return idx;
 }
 
-struct Side {
+template <int VPL> struct Side {
     int kind;
     float a, b, c;   // sphere r | box half extents | capsule half_height, radius
     TF tf;
-    WaveHull wh;
+    WaveHull<VPL> wh;
 };
-__device__ __forceinline__ V3 hull_vertex(const HullSet &hs, const WaveHull &w, int i) {
+template <int VPL> __device__ __forceinline__ V3 hull_vertex(const HullSet &hs, const WaveHull<VPL> &w, int i) {
     const float4 p = hs.v[w.v0 + i];
     return V3{p.x, p.y, p.z};
 }
 // Local support (Sphere.hpp:48-51, Box.hpp:49-55, Capsule.hpp:72-80, ConvexHull.hpp:457-473).
-__device__ __forceinline__ V3 local_support(const Side &s, const HullSet &hs, V3 d) {
-    if (s.kind == SH_HULL) return hull_vertex(hs, s.wh, wave_support_index(s.wh, d));
+template <int VPL> __device__ __forceinline__ V3 local_support(const Side<VPL> &s, const HullSet &hs, V3 d) {
+    if (s.kind == SH_HULL) {
+        V3 v;
+        wave_support_index(s.wh, d, &v);
+        return v;
+    }
     if (s.kind == SH_SPHERE) return normalize(d) * s.a;
     if (s.kind == SH_BOX) return V3{d.x > 0.0f ? s.a : -s.a, d.y > 0.0f ? s.b : -s.b, d.z > 0.0f ? s.c : -s.c};
     const V3 e{0.0f, d.y >= 0.0f ? s.a : -s.a, 0.0f};   // capsule
@@ -125,18 +181,17 @@ __device__ __forceinline__ V3 local_support(const Side &s, const HullSet &hs, V3
     return e + vdiv(d, l) * s.b;
 }
 // World support of one side (MinkowskiDiff.hpp:52-63).
-__device__ __forceinline__ V3 side_support(const Side &s, const HullSet &hs, V3 dir) {
+template <int VPL> __device__ __forceinline__ V3 side_support(const Side<VPL> &s, const HullSet &hs, V3 dir) {
     return tf_point(s.tf, local_support(s, hs, tf_inv_vec(s.tf, dir)));
 }
 
 struct SV {
-    V3 w0, w1, w;
+    V3 w0, w;   // A-side support, Minkowski-difference point (the B-side support is not kept)
 };
-__device__ __forceinline__ SV md_sample(const Side &A, const Side &Bs, const HullSet &hs, V3 d) {
+template <int VPL> __device__ __forceinline__ SV md_sample(const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs, V3 d) {
     SV s;
     s.w0 = side_support(A, hs, d);
-    s.w1 = side_support(Bs, hs, -d);
-    s.w = s.w0 - s.w1;
+    s.w = s.w0 - side_support(Bs, hs, -d);
     return s;
 }
 
@@ -146,6 +201,14 @@ struct Simplex {
     int rank;
     V3 dir;    // search direction at exit (a separating direction when GJK returns valid)
 };
+// Vertex i of a simplex by value (selects, so that the simplex stays in registers).
+__device__ __forceinline__ SV sx_vertex(const Simplex &s, int i) {
+    SV r = s.v[0];
+    if (i == 1) r = s.v[1];
+    if (i == 2) r = s.v[2];
+    if (i == 3) r = s.v[3];
+    return r;
+}
 __device__ __forceinline__ void sx_push(Simplex &s, const SV &n) {   // DoSimplex.cpp:5-17
     const int r = s.rank;
     if (r >= 3) s.v[3] = s.v[2];
@@ -222,18 +285,23 @@ __device__ __forceinline__ bool sx_tetra(Simplex &s, V3 &dir) {   // Tetrahedron
         case 1: sx_tet_face(s, dir, va, vb, vc, ab, ac, abc, ao); break;
         case 2: sx_tet_face(s, dir, va, vc, vd, ac, ad, acd, ao); break;
         case 4: sx_tet_face(s, dir, va, vd, vb, ad, ab, adb, ao); break;
-        case 3: case 5: case 6: {
-            const SV &X = pi == 3 ? vc : (pi == 5 ? vb : vd);
-            const V3 ax = pi == 3 ? ac : (pi == 5 ? ab : ad);
-            if (dot(ax, ao) >= 0.0f) { s.v[0] = va; s.v[1] = X; s.rank = 2; dir = triple(ax, ao, ax); }
+        case 3:
+            if (dot(ac, ao) >= 0.0f) { s.v[0] = va; s.v[1] = vc; s.rank = 2; dir = triple(ac, ao, ac); }
             else { s.v[0] = va; s.rank = 1; dir = ao; }
             break;
-        }
+        case 5:
+            if (dot(ab, ao) >= 0.0f) { s.v[0] = va; s.v[1] = vb; s.rank = 2; dir = triple(ab, ao, ab); }
+            else { s.v[0] = va; s.rank = 1; dir = ao; }
+            break;
+        case 6:
+            if (dot(ad, ao) >= 0.0f) { s.v[0] = va; s.v[1] = vd; s.rank = 2; dir = triple(ad, ao, ad); }
+            else { s.v[0] = va; s.rank = 1; dir = ao; }
+            break;
         default: s.v[0] = va; s.rank = 1; dir = ao; break;
     }
     return false;
 }
-__device__ __forceinline__ int gjk(const Side &A, const Side &Bs, const HullSet &hs, V3 guess, Simplex &s, int &iters,
+template <int VPL> __device__ __forceinline__ int gjk(const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs, V3 guess, Simplex &s, int &iters,
                                    int max_iter = GJK_ITERS) {
     s.rank = 0;
     V3 dir = (len2(guess) > EPS) ? guess : V3{1.0f, 0.0f, 0.0f};
@@ -410,7 +478,7 @@ struct EpaResult {
     float depth;
 };
 
-__device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side &A, const Side &Bs, const HullSet &hs,
+template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs,
                                          EpaLds &L, const V3 *seed_n) {
     EpaResult r{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
     const int lane = lane64();
@@ -440,7 +508,7 @@ __device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side &A, con
         }
     }
     if (lane < 4) {
-        const SV &s = lane == 0 ? seed.v[0] : (lane == 1 ? seed.v[1] : (lane == 2 ? seed.v[2] : seed.v[3]));
+        const SV s = sx_vertex(seed, lane);
         L.wx[lane] = s.w.x; L.wy[lane] = s.w.y; L.wz[lane] = s.w.z;
         L.ax[lane] = s.w0.x; L.ay[lane] = s.w0.y; L.az[lane] = s.w0.z;
     }
@@ -482,17 +550,22 @@ __device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side &A, con
     }
     for (int it = 0; it < EPA_ITERS; ++it) {
         r.iters = it0 + it + 1;
-        // 1. closest live face
-        float best = 3.402823466e+38f;
+        // 1. closest live face: lane-local first minimum, wave minimum of the mapped
+        // distance, lowest face index among the lanes holding it; faces at or above
+        // FLT_MAX (and NaN) never qualify, as in the reference's scan from DBL_MAX
+        uint32_t best = 0xffffffffu;
         int bi = 0x7fffffff;
         for (int f = lane; f < nf; f += 64) {
             const float d = L.nd[f];
-            if (!(L.fv[f] >> 24) && d < best) { best = d; bi = f; }
+            if (!(L.fv[f] >> 24) && d < 3.402823466e+38f) {
+                const uint32_t o = ord_f(d);
+                if (o < best) { best = o; bi = f; }
+            }
         }
-        wave_argmin(best, bi);
-        if (bi == 0x7fffffff) { r.status = EPA_FAILED; return r; }
-        const int ci = bi;
-        const float mind = best;
+        const uint32_t mo = wave_min_u32(best);
+        if (mo == 0xffffffffu) { r.status = EPA_FAILED; return r; }
+        const int ci = (int)wave_min_u32(best == mo ? (uint32_t)bi : 0xffffffffu);
+        const float mind = L.nd[ci];
         const V3 cn{L.nx[ci], L.ny[ci], L.nz[ci]};
         const uint32_t cv = L.fv[ci];
         const int c0 = (int)(cv & 255u), c1 = (int)((cv >> 8) & 255u), c2 = (int)((cv >> 16) & 255u);
@@ -667,7 +740,7 @@ __device__ __forceinline__ int reduce4_area(const ClipLds &C, int buf, int n, Ma
 
 // Hull-hull manifold from the EPA result (ContactManifoldGenerator.cpp:176-379).  The
 // result is valid on lane 0.
-__device__ __forceinline__ void manifold_hulls(const EpaResult &e, const Side &A, const Side &Bs, const HullSet &hs,
+template <int VPL> __device__ __forceinline__ void manifold_hulls(const EpaResult &e, const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs,
                                                uint32_t ha_id, uint32_t hb_id, ClipLds &C, Manifold &m) {
     const int lane = lane64();
     const float lowest = -3.402823466e+38f;
@@ -766,7 +839,8 @@ __device__ __forceinline__ void manifold_hulls(const EpaResult &e, const Side &A
 }
 
 // ------------------------------------------------------------------ kernels
-__device__ __forceinline__ void load_side(Side &s, const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf) {
+template <int VPL>
+__device__ __forceinline__ void load_side(Side<VPL> &s, const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf) {
     const int kind = C.meta[c].x;
     const float4 d = C.dims[c];
     s.kind = kind;
@@ -777,88 +851,194 @@ __device__ __forceinline__ void load_side(Side &s, const Colliders &C, const Hul
     if (kind == SH_HULL) load_wave_hull(s.wh, hs, (uint32_t)C.hull[c]);
 }
 
-// GJK + EPA + manifold for one pair of the hull-hull / hull-primitive bins
-// (ContactManifoldGenerator.cpp:415-438).  One 64-lane workgroup per pair.
-// gstat (optional): per slot, GJK iterations | EPA iterations << 8 | GJK status << 16 |
-// EPA status << 20 (15: not run) | fallback << 24; iterations of a fallback run are added.
+// GJK + EPA + manifold of the hull-hull / hull-primitive bins (ContactManifoldGenerator.cpp:415-438)
+// in two launches.  Pass 1 (k_gjk_pass): GJK only, one wavefront per pair, four per
+// workgroup, no LDS; a separated pair is finished there (no contact), an overlapping
+// pair's final simplex is stored and its bin position listed in its workgroup's
+// four-entry segment.  Pass 2 (k_epa_pass): one wavefront per pass-1 workgroup runs EPA
+// (polytope in LDS) and the manifold for the listed pairs.  Most broadphase pairs are
+// separated, so the heavy EPA/manifold code and its LDS only occupy the GPU for the few
+// that overlap.
+// gstat: per slot, GJK iterations | EPA iterations << 8 | GJK status << 16 | EPA status
+// << 20 (15: not run) | fallback << 24; iterations of a fallback run are added.
 // guess (optional, per bin position two float4 in A's frame): warm start, the GJK
 // direction and (seed_epa) an EPA seed normal; a zero direction means the plain seed.
-// When the warm-started run fails (GJK not done within 16 iterations, EPA not valid) the
-// plain run follows.
+// A warm-started GJK gets GJK_WARM_ITERS iterations; when it fails, or EPA from its
+// simplex fails, the plain run follows.
 // tlog (optional, per bin position two float4 in A's frame): training targets, the EPA
 // normal (or, for a separated pair, the reverse of the GJK exit direction) with a valid
 // flag, and the EPA normal with a valid flag.
-__global__ void __launch_bounds__(64) k_narrow_gjk(const uint32_t *bin_slot, uint32_t off, Colliders C, Bodies B,
-                                                   Contacts K, HullSet hs, uint32_t *gstat, const float4 *guess,
-                                                   int seed_epa, float4 *tlog) {
-    __shared__ PairLds L;
-    const uint32_t s = bin_slot[off + blockIdx.x];
-    const uint2 cc = K.cc[s], bb = K.bb[s];
-    const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
-    Side A, Bs;
-    load_side(A, C, hs, cc.x, ta);
-    load_side(Bs, C, hs, cc.y, tb);
-    V3 plain = tb.pos - ta.pos;
-    if (len2(plain) < EPS) plain = V3{1.0f, 0.0f, 0.0f};
-    V3 g = plain, sn{0.0f, 0.0f, 0.0f};
-    bool warm = false, seeded = false;
+constexpr int GJK_PASS_WAVES = 4;
+struct GjkSeeds {
+    V3 plain, g, sn;
+    bool warm, seeded;
+};
+__device__ __forceinline__ GjkSeeds gjk_seeds(const TF &ta, const TF &tb, const float4 *guess, uint32_t t, int seed_epa) {
+    GjkSeeds q;
+    q.plain = tb.pos - ta.pos;
+    if (len2(q.plain) < EPS) q.plain = V3{1.0f, 0.0f, 0.0f};
+    q.g = q.plain;
+    q.sn = V3{0.0f, 0.0f, 0.0f};
+    q.warm = q.seeded = false;
     if (guess) {
-        const V3 gl = xyz(guess[2 * blockIdx.x]);
-        if (len2(gl) > 0.5f) { g = tf_vec(ta, gl); warm = true; }
-        const V3 sl = xyz(guess[2 * blockIdx.x + 1]);
-        if (warm && seed_epa && len2(sl) > 0.5f) { sn = tf_vec(ta, sl); seeded = true; }
+        const V3 gl = xyz(guess[2 * t]);
+        if (len2(gl) > 0.5f) { q.g = tf_vec(ta, gl); q.warm = true; }
+        const V3 sl = xyz(guess[2 * t + 1]);
+        if (q.warm && seed_epa && len2(sl) > 0.5f) { q.sn = tf_vec(ta, sl); q.seeded = true; }
     }
+    return q;
+}
+__device__ __forceinline__ uint32_t pack_gstat(int gsum, int esum, int gs, int es, int fb) {
+    return (uint32_t)(gsum < 255 ? gsum : 255) | ((uint32_t)(esum < 255 ? esum : 255) << 8) | ((uint32_t)gs << 16) |
+           ((uint32_t)es << 20) | ((uint32_t)fb << 24);
+}
+
+#ifndef GJK_PASS_OCCUPANCY
+#define GJK_PASS_OCCUPANCY 4
+#endif
+template <int VPL>
+__global__ void __launch_bounds__(64 * GJK_PASS_WAVES) __attribute__((amdgpu_waves_per_eu(GJK_PASS_OCCUPANCY)))
+k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n,
+                                                                  Colliders C, Bodies B, Contacts K, HullSet hs,
+                                                                  uint32_t *gstat, const float4 *guess, float4 *tlog,
+                                                                  uint32_t *hits, uint32_t *blk_hits, float4 *sxs) {
+    __shared__ uint32_t inside_w[GJK_PASS_WAVES];
+    const int wave = threadIdx.x >> 6, lane = lane64();
+    const uint32_t t = blockIdx.x * GJK_PASS_WAVES + wave;   // bin position
+    bool inside = false;
     Simplex sx;
-    int giters = 0;
-    int gs = gjk(A, Bs, hs, g, sx, giters, warm ? GJK_WARM_ITERS : GJK_ITERS);
-    EpaResult e{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
-    if (gs == GJK_INSIDE) e = epa(sx, A, Bs, hs, L.e, seeded ? &sn : nullptr);
-    int gsum = giters, esum = gs == GJK_INSIDE ? e.iters : 0, fb = 0;
-    if (warm && (gs == GJK_FAILED || (gs == GJK_INSIDE && e.status != EPA_VALID))) {
-        fb = 1;
+    if (t < n) {
+        const uint32_t s = bin_slot[off + t];
+        const uint2 cc = K.cc[s], bb = K.bb[s];
+        const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+        Side<VPL> A, Bs;
+        load_side(A, C, hs, cc.x, ta);
+        load_side(Bs, C, hs, cc.y, tb);
+        const GjkSeeds q = gjk_seeds(ta, tb, guess, t, 0);
+        int it = 0;
+        int gs = gjk(A, Bs, hs, q.g, sx, it, q.warm ? GJK_WARM_ITERS : GJK_ITERS);
+        int gsum = it, fb = 0;
+        if (q.warm && gs == GJK_FAILED) {
+            fb = 1;
+            gs = gjk(A, Bs, hs, q.plain, sx, it);
+            gsum += it;
+        }
+        inside = gs == GJK_INSIDE;
+        if (lane == 0) gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);   // pass 2 completes an overlapping pair
+        if (!inside && lane == 0) {
+            K.cnt[s] = 0;
+            if (tlog) {
+                V3 tg{0.0f, 0.0f, 0.0f};
+                float vg = 0.0f;
+                const float ln = len(sx.dir);
+                // a seed against the exit direction puts the first support on the far side of
+                // the Minkowski difference, so that -v0 already points across it at the origin
+                if (gs == GJK_VALID && ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
+                tlog[2 * t] = f4(tg, vg);
+                tlog[2 * t + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
+    if (lane == 0) inside_w[wave] = inside ? 1u : 0u;
+    __syncthreads();
+    int pos = 0;
+    for (int k = 0; k < wave; ++k) pos += (int)inside_w[k];
+    if (inside) {
+        const uint32_t h = blockIdx.x * GJK_PASS_WAVES + pos;
+        if (lane == 0) hits[h] = t;
+        if (lane < 4) {   // the four simplex vertices: (w.xyz, w0.x) (w0.y, w0.z)
+            const SV v = sx_vertex(sx, lane);
+            sxs[(size_t)h * 8 + 2 * lane] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
+            sxs[(size_t)h * 8 + 2 * lane + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
+        }
+    }
+    if (threadIdx.x == 0) {
+        uint32_t c = 0;
+        for (int k = 0; k < GJK_PASS_WAVES; ++k) c += inside_w[k];
+        blk_hits[blockIdx.x] = c;
+    }
+}
+
+template <int VPL>
+__global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint32_t off, Colliders C, Bodies B, Contacts K,
+                                                 HullSet hs, uint32_t *gstat, const float4 *guess, int seed_epa,
+                                                 float4 *tlog, const uint32_t *hits, const uint32_t *blk_hits,
+                                                 const float4 *sxs) {
+    __shared__ PairLds L;
+    const int lane = lane64();
+    const uint32_t nh = blk_hits[blockIdx.x];
+    for (uint32_t r = 0; r < nh; ++r) {
+        const uint32_t h = blockIdx.x * GJK_PASS_WAVES + r;
+        const uint32_t t = hits[h];
+        const uint32_t s = bin_slot[off + t];
+        const uint2 cc = K.cc[s], bb = K.bb[s];
+        const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+        Side<VPL> A, Bs;
+        load_side(A, C, hs, cc.x, ta);
+        load_side(Bs, C, hs, cc.y, tb);
+        Simplex sx;
+        sx.rank = 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 a = sxs[(size_t)h * 8 + 2 * k], b = sxs[(size_t)h * 8 + 2 * k + 1];
+            sx.v[k].w = V3{a.x, a.y, a.z};
+            sx.v[k].w0 = V3{a.w, b.x, b.y};
+        }
+        const uint32_t g0 = gstat[s];
+        int gsum = (int)(g0 & 255u), fb = (int)((g0 >> 24) & 1u);
+        int gs = GJK_INSIDE;
+        const GjkSeeds q = gjk_seeds(ta, tb, guess, t, seed_epa);
+        const bool warm = q.warm && !fb;   // a pass-1 fallback already ran the plain GJK
         __syncthreads();
-        gs = gjk(A, Bs, hs, plain, sx, giters);
-        e = EpaResult{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
-        if (gs == GJK_INSIDE) e = epa(sx, A, Bs, hs, L.e, nullptr);
-        gsum += giters;
-        esum += gs == GJK_INSIDE ? e.iters : 0;
-    }
-    if (gstat && lane64() == 0)
-        gstat[s] = (uint32_t)(gsum < 255 ? gsum : 255) | ((uint32_t)(esum < 255 ? esum : 255) << 8) | ((uint32_t)gs << 16) |
-                   ((uint32_t)(gs == GJK_INSIDE ? e.status : 15) << 20) | ((uint32_t)fb << 24);
-    if (tlog && lane64() == 0) {
+        EpaResult e = epa(sx, A, Bs, hs, L.e, warm && q.seeded ? &q.sn : nullptr);
+        int esum = e.iters;
+        if (warm && e.status != EPA_VALID) {
+            fb = 1;
+            __syncthreads();
+            int it = 0;
+            gs = gjk(A, Bs, hs, q.plain, sx, it);
+            gsum += it;
+            e = EpaResult{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
+            if (gs == GJK_INSIDE) {
+                e = epa(sx, A, Bs, hs, L.e, nullptr);
+                esum += e.iters;
+            }
+        }
         const bool ok = gs == GJK_INSIDE && e.status == EPA_VALID;
-        V3 tg{0.0f, 0.0f, 0.0f};
-        float vg = 0.0f;
+        if (lane == 0) {
+            gstat[s] = pack_gstat(gsum, esum, gs, gs == GJK_INSIDE ? e.status : 15, fb);
+            if (tlog) {
+                V3 tg{0.0f, 0.0f, 0.0f};
+                float vg = 0.0f;
+                if (ok) { tg = tf_inv_vec(ta, e.n); vg = 1.0f; }
+                else if (gs == GJK_VALID) {
+                    const float ln = len(sx.dir);
+                    if (ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
+                }
+                tlog[2 * t] = f4(tg, vg);
+                tlog[2 * t + 1] = ok ? f4(tf_inv_vec(ta, e.n), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+        Manifold m;
+        m.np = 0;
         if (ok) {
-            tg = tf_inv_vec(ta, e.n);
-            vg = 1.0f;
-        } else if (gs == GJK_VALID) {
-            // a seed against the exit direction puts the first support on the far side of
-            // the Minkowski difference, so that -v0 already points across it at the origin
-            const float ln = len(sx.dir);
-            if (ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
+            const bool adj = A.kind == SH_HULL && Bs.kind == SH_HULL && hs.meta[C.hull[cc.x]].adj && hs.meta[C.hull[cc.y]].adj;
+            if (adj) {
+                __syncthreads();   // the manifold buffers overlay the polytope
+                manifold_hulls(e, A, Bs, hs, (uint32_t)C.hull[cc.x], (uint32_t)C.hull[cc.y], L.c, m);
+            } else {
+                m.n = e.n;
+                m.np = 1;
+                m.p[0] = e.p;
+                m.dep[0] = e.depth;
+            }
         }
-        tlog[2 * blockIdx.x] = f4(tg, vg);
-        tlog[2 * blockIdx.x + 1] = ok ? f4(tf_inv_vec(ta, e.n), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    Manifold m;
-    m.np = 0;
-    if (gs == GJK_INSIDE && e.status == EPA_VALID) {
-        const bool adj = A.kind == SH_HULL && Bs.kind == SH_HULL && hs.meta[C.hull[cc.x]].adj && hs.meta[C.hull[cc.y]].adj;
-        if (adj) {
-            __syncthreads();   // the manifold buffers overlay the polytope
-            manifold_hulls(e, A, Bs, hs, (uint32_t)C.hull[cc.x], (uint32_t)C.hull[cc.y], L.c, m);
-        } else {
-            m.n = e.n;
-            m.np = 1;
-            m.p[0] = e.p;
-            m.dep[0] = e.depth;
+        if (lane == 0) {
+            if (m.np > 0) emit(m, B, K, s, bb);
+            else K.cnt[s] = 0;
         }
+        __syncthreads();
     }
-    if (lane64() != 0) return;
-    if (m.np > 0) emit(m, B, K, s, bb);
-    else K.cnt[s] = 0;
 }
 
 // Totals over the GJK pairs bin_slot[off .. off + n) from gstat: pairs, GJK iterations,
@@ -881,6 +1061,7 @@ __global__ void k_gjk_totals(const uint32_t *bin_slot, uint32_t off, uint32_t n,
 
 // Hull against a plane (PlaneCollision.hpp:90-285): deepest support, then up to 32
 // vertices within 1e-4 of the plane in vertex order, reduced to 4 by area.
+template <int VPL>
 __global__ void __launch_bounds__(64) k_narrow_hull_plane(const uint32_t *bin_slot, uint32_t off, Colliders C, Bodies B,
                                                           Contacts K, HullSet hs) {
     __shared__ float tx[HP_MAX_POINTS], ty[HP_MAX_POINTS], tz[HP_MAX_POINTS], td[HP_MAX_POINTS];
@@ -888,7 +1069,7 @@ __global__ void __launch_bounds__(64) k_narrow_hull_plane(const uint32_t *bin_sl
     const uint32_t s = bin_slot[off + blockIdx.x];
     const uint2 cc = K.cc[s], bb = K.bb[s];
     const TF ta = collider_tf(C, B, cc.x, bb.x), tp = collider_tf(C, B, cc.y, bb.y);
-    WaveHull wh;
+    WaveHull<VPL> wh;
     load_wave_hull(wh, hs, (uint32_t)C.hull[cc.x]);
     const float4 pd = C.dims[cc.y];
     const V3 wn = tf_vec(tp, V3{pd.x, pd.y, pd.z});   // Plane.hpp:47-57
@@ -902,7 +1083,7 @@ __global__ void __launch_bounds__(64) k_narrow_hull_plane(const uint32_t *bin_sl
     }
     int tc = 0;
 #pragma unroll
-    for (int k = 0; k < HULL_VPL; ++k) {
+    for (int k = 0; k < VPL; ++k) {
         const int i = lane + 64 * k;
         float dist = 0.0f;
         bool pen = false;

@@ -106,7 +106,10 @@ struct rbgpu_world {
     uint32_t *htab = nullptr;
     HullMeta *hmeta = nullptr;
     HullSet hs{};
+    int hull_vpl = 1;                   // hull vertices per lane in the wave kernels (1: all hulls <= 64)
     uint32_t *gstat = nullptr;
+    uint32_t *gjk_hits = nullptr, *gjk_blk = nullptr;   // overlapping GJK pairs per 4-pair workgroup
+    float4 *gjk_sx = nullptr;                          // their final simplices (8 float4 each)
     // learned warm start: parameters (FP32 master, BF16 copy), gradient, Adam moments, the
     // logged training set (features, targets) and the per-frame inference buffers
     float *mlp_wf = nullptr, *mlp_grad = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_loss = nullptr;
@@ -178,6 +181,7 @@ struct rbgpu_world {
     void release_pairs() {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
+        dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
@@ -289,9 +293,20 @@ struct rbgpu_world {
                     if (b == BIN_HH || bin_off[BIN_HH] == bin_off[BIN_HX]) gjk_prepare(g0, ng);
                     const uint32_t rel = o - g0;
                     hipEvent_t *e = timed_begin(11);
-                    k_narrow_gjk<<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs, gstat,
-                                                   gjk_guess ? inf_guess + 2 * (size_t)rel : nullptr,
-                                                   mlp_mode == 2 ? 1 : 0, gjk_log ? ds_tgt + 2 * (ds_n + rel) : nullptr);
+                    const float4 *gp = gjk_guess ? inf_guess + 2 * (size_t)rel : nullptr;
+                    float4 *tl = gjk_log ? ds_tgt + 2 * (ds_n + rel) : nullptr;
+                    const uint32_t nb1 = (n + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
+                    if (hull_vpl == 1) {
+                        k_gjk_pass<1><<<nb1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, o, n, C, B, K, hs, gstat, gp, tl,
+                                                                            gjk_hits, gjk_blk, gjk_sx);
+                        k_epa_pass<1><<<nb1, 64, 0, st>>>(bin_list, o, C, B, K, hs, gstat, gp, mlp_mode == 2 ? 1 : 0, tl,
+                                                          gjk_hits, gjk_blk, gjk_sx);
+                    } else {
+                        k_gjk_pass<HULL_VPL><<<nb1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, o, n, C, B, K, hs, gstat, gp,
+                                                                                   tl, gjk_hits, gjk_blk, gjk_sx);
+                        k_epa_pass<HULL_VPL><<<nb1, 64, 0, st>>>(bin_list, o, C, B, K, hs, gstat, gp,
+                                                                 mlp_mode == 2 ? 1 : 0, tl, gjk_hits, gjk_blk, gjk_sx);
+                    }
                     timed_end(e);
                     if (e) timing_bytes += (double)n * gjk_pair_bytes();
                     if (gjk_tot) k_gjk_totals<<<nblk(n), TPB, 0, st>>>(bin_list, o, n, gstat, gjk_tot);
@@ -301,13 +316,16 @@ struct rbgpu_world {
                     }
                     break;
                 }
-                case BIN_HP: k_narrow_hull_plane<<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs); break;
+                case BIN_HP:
+                    if (hull_vpl == 1) k_narrow_hull_plane<1><<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs);
+                    else k_narrow_hull_plane<HULL_VPL><<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs);
+                    break;
                 default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
             }
         }
     }
 
-    // Algorithmic bytes of one GJK pair (k_narrow_gjk): per side the hull vertices (16 B
+    // Algorithmic bytes of one GJK pair (k_gjk_pass + k_epa_pass): per side the hull vertices (16 B
     // each, the world's mean count), hull meta 64 B, collider rows 68 B, body pose 32 B;
     // the pair's slot, colliders and bodies 20 B; out: count, normal and four contact
     // points (72 B each) 308 B.
@@ -352,6 +370,11 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
     rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap);
     rc |= dalloc(bb_hits, nblk(cap, 128) * 128); rc |= dalloc(bb_blk, nblk(cap, 128)); rc |= dalloc(gstat, cap);
+    if (!hh.empty()) {
+        const size_t nb1 = (cap + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
+        rc |= dalloc(gjk_hits, nb1 * GJK_PASS_WAVES); rc |= dalloc(gjk_blk, nb1);
+        rc |= dalloc(gjk_sx, nb1 * GJK_PASS_WAVES * 8);
+    }
     if (mlp_wb) {
         dfree(inf_feat); dfree(inf_guess);
         rc |= dalloc(inf_feat, (size_t)cap * MLP_IN); rc |= dalloc(inf_guess, 2 * (size_t)cap);
@@ -511,6 +534,9 @@ int rbgpu_world::build() {
         HIPCHK(up(hamax, hmaxv.data(), hmaxv.size() * 16));
     }
     hs = HullSet{hv, hpn, htab, hmeta, hamin, hamax};
+    hull_vpl = 1;
+    for (const HostHull &h : hh)
+        if (h.t.v.size() > 64) hull_vpl = HULL_VPL;
     if (S) {
         HIPCHK(up(fmin, h_fmin.data(), S * 16)); HIPCHK(up(fmax, h_fmax.data(), S * 16)); HIPCHK(up(uid, h_uid.data(), S * 4));
         HIPCHK(up(ep_key[0], h_key.data(), NEP * 4)); HIPCHK(up(ep_val[0], h_val.data(), NEP * 4));

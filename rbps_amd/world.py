@@ -18,7 +18,7 @@ from . import scenes
 from .scenes import BODY_DTYPE, COLLIDER_DTYPE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librbgpu.so")
+LIB_PATH = os.environ.get("RBGPU_LIBRARY") or os.path.join(HERE, "librbgpu.so")   # override: another build of the library
 
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)

@@ -167,3 +167,23 @@ def test_gpu_learned_warm_start_gains_on_held_out_pile():
     dd = np.abs(plain["depth"][both, 0] - learned["depth"][both, 0])
     print("normal cos p1", np.percentile(cos, 1), "depth diff p99", np.percentile(dd, 99), flush=True)
     assert np.percentile(cos, 1) > 0.999 and np.percentile(dd, 99) < 1e-4
+    # both against FP64 (the reference's algorithm in double) on a sample of the contacts
+    st = w2.state()
+    idx = np.random.default_rng(0).choice(np.nonzero(both)[0], min(300, int(both.sum())), replace=False)
+    errs = {"plain": [], "learned": []}
+    for i in idx:
+        ca, cb = (int(x) for x in pairs[i])
+        ba, bb = te.colliders["body_id"][ca], te.colliders["body_id"][cb]
+        ta = np.concatenate([st["position"][ba], st["orientation"][ba]])
+        tb = np.concatenate([st["position"][bb], st["orientation"][bb]])
+        r64 = orc.oracle_collide(te.colliders[ca], ta, te.colliders[cb], tb, hulls=te.hulls)
+        if not r64["n"]:
+            continue
+        for name, r in (("plain", plain), ("learned", learned)):
+            ang = np.degrees(np.arccos(np.clip(np.dot(r["normal"][i], r64["normal"]), -1.0, 1.0)))
+            errs[name].append((ang, abs(r["depth"][i, :r["n"][i]].max() - r64["depth"].max())))
+    for name, e in errs.items():
+        e = np.array(e)
+        print(f"vs FP64 ({len(e)} contacts) {name}: normal deg p99 {np.percentile(e[:, 0], 99):.3g} max {e[:, 0].max():.3g}, "
+              f"depth p99 {np.percentile(e[:, 1], 99):.3g} max {e[:, 1].max():.3g}", flush=True)
+        assert np.percentile(e[:, 0], 99) < 0.1 and np.percentile(e[:, 1], 99) < 1e-4
