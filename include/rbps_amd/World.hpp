@@ -76,10 +76,48 @@ struct Plane {
     Plane() = default;
     Plane(const vec3 &n, scalar dd) : normal(n), d(dd) {}
 };
+// rbc::ConvexHullData (include/rbc/shapes/ConvexHull.hpp:79-98): non-owning over the
+// caller's vertices and CCW triangles (triangles optional).  The world copies the data
+// (rbgpu_add_hull) the first time a collider uses it; the arrays must stay valid until then.
+struct ConvexHullData {
+    const vec3 *vertices = nullptr;
+    const uint32_t *face_indices = nullptr;
+    uint32_t vert_count = 0, face_count = 0;
+};
+// convex_hull_data_create / convex_hull_data_destroy (ConvexHull.hpp:149-448)
+inline ConvexHullData *convex_hull_data_create(const vec3 *vertices, uint32_t vert_count,
+                                               const uint32_t *face_indices = nullptr, uint32_t face_count = 0) {
+    return new ConvexHullData{vertices, face_indices, vert_count, face_indices ? face_count : 0u};
+}
+inline void convex_hull_data_destroy(ConvexHullData *d) { delete d; }
+// rbc::ConvexHull (ConvexHull.hpp:101-121)
+struct ConvexHull {
+    const ConvexHullData *data = nullptr;
+    ConvexHull() = default;
+    explicit ConvexHull(const ConvexHullData *d) : data(d) {}
+};
+// compute_volume / compute_inertia_tensor (ConvexHull.hpp:575-745), unit density
+inline scalar compute_volume(const ConvexHull &h) {
+    double vol = 0.0;
+    if (h.data && rbgpu_hull_mass_properties(&h.data->vertices[0].x, h.data->vert_count, h.data->face_indices,
+                                             h.data->face_count, &vol, nullptr) != RBGPU_OK)
+        throw std::runtime_error(std::string("rbgpu: ") + rbgpu_last_error());
+    return vol;
+}
+inline smat3 compute_inertia_tensor(const ConvexHull &h) {
+    double I[6] = {0, 0, 0, 0, 0, 0};
+    if (h.data && rbgpu_hull_mass_properties(&h.data->vertices[0].x, h.data->vert_count, h.data->face_indices,
+                                             h.data->face_count, nullptr, I) != RBGPU_OK)
+        throw std::runtime_error(std::string("rbgpu: ") + rbgpu_last_error());
+    return smat3(I[0], I[1], I[2], I[3], I[4], I[5]);
+}
+
 struct Shape {
     int kind = RBGPU_SPHERE;
     scalar dims[4] = {0.5, 0, 0, 0};
+    const ConvexHullData *hull = nullptr;
     Shape() = default;
+    Shape(const ConvexHull &h) : kind(RBGPU_HULL), dims{0, 0, 0, 0}, hull(h.data) {}
     Shape(const Sphere &s) : kind(RBGPU_SPHERE), dims{s.radius, 0, 0, 0} {}
     Shape(const Box &b) : kind(RBGPU_BOX), dims{b.half_extents.x, b.half_extents.y, b.half_extents.z, 0} {}
     Shape(const Capsule &c) : kind(RBGPU_CAPSULE), dims{c.half_height, c.radius, 0, 0} {}
@@ -194,13 +232,25 @@ struct World {
         return id;
     }
 
+    // The world's copy of a caller's hull data (added on first use).
+    int32_t hull_id(const ConvexHullData *h) {
+        if (!h || !h->vertices || !h->vert_count) throw Error(RBGPU_ERR_INVALID, "convex hull collider without hull data");
+        for (size_t i = 0; i < hull_keys_.size(); ++i)
+            if (hull_keys_[i] == h) return (int32_t)i;
+        uint32_t id = 0;
+        check(rbgpu_add_hull(h_, &h->vertices[0].x, h->vert_count, h->face_indices, h->face_count, &id));
+        hull_keys_.push_back(h);
+        return (int32_t)id;
+    }
+    std::vector<const ConvexHullData *> hull_keys_;
+
     // World::create_collider (World.hpp:124-132)
     uint32_t create_collider(const ColliderParams &p) {
         rbgpu_collider_desc d{};
         d.shape = p.shape.kind;
         d.body_id = p.body_id;
         d.is_static = p.is_static ? 1 : 0;
-        d.hull_index = -1;
+        d.hull_index = p.shape.kind == RBGPU_HULL ? hull_id(p.shape.hull) : -1;
         for (int k = 0; k < 4; ++k) d.dims[k] = p.shape.dims[k];
         d.local_pos[0] = p.local_pos.x; d.local_pos[1] = p.local_pos.y; d.local_pos[2] = p.local_pos.z;
         d.local_rot[0] = p.local_rot.w; d.local_rot[1] = p.local_rot.x; d.local_rot[2] = p.local_rot.y;

@@ -179,6 +179,45 @@ int main() {
         CHECK(end.y < 4.0);
         CHECK(std::sqrt(end.x * end.x + (end.y - 5.0) * (end.y - 5.0) + end.z * end.z) < 2.45);
     }
+    {
+        // a box-shaped convex hull (12 CCW triangles) dropped on the ground plane comes to
+        // rest on a face: centre 0.3 above the plane (its half height), upright
+        std::printf("convex_hull_rests_on_plane\n");
+        static const vec3 verts[8] = {vec3(-0.4, -0.3, -0.5), vec3(0.4, -0.3, -0.5), vec3(0.4, 0.3, -0.5),
+                                      vec3(-0.4, 0.3, -0.5), vec3(-0.4, -0.3, 0.5), vec3(0.4, -0.3, 0.5),
+                                      vec3(0.4, 0.3, 0.5), vec3(-0.4, 0.3, 0.5)};
+        static const uint32_t tris[36] = {0, 3, 2, 0, 2, 1, 4, 5, 6, 4, 6, 7, 0, 1, 5, 0, 5, 4,
+                                          3, 7, 6, 3, 6, 2, 0, 4, 7, 0, 7, 3, 1, 2, 6, 1, 6, 5};
+        ConvexHullData *hd = convex_hull_data_create(verts, 8, tris, 12);
+        ConvexHull hull(hd);
+        CHECK(std::fabs(compute_volume(hull) - 0.48) < 1e-12);
+        World w;
+        BodyParams g;
+        g.type = STATIC;
+        g.mass = 0.0;
+        uint32_t gid = w.create_body(g);
+        ColliderParams gc;
+        gc.shape = Plane(vec3(0, 1, 0), 0.0);
+        gc.body_id = gid;
+        gc.is_static = true;
+        w.create_collider(gc);
+        BodyParams bp;
+        bp.mass = 1000.0 * compute_volume(hull);
+        smat3 I = compute_inertia_tensor(hull);
+        bp.inertia_tensor = smat3(I.xx * 1000.0, I.yy * 1000.0, I.zz * 1000.0, 0, 0, 0);
+        bp.position = vec3(0.0, 1.0, 0.0);
+        uint32_t b = w.create_body(bp);
+        ColliderParams hc;
+        hc.shape = ConvexHull(hd);
+        hc.body_id = b;
+        hc.restitution = 0.0;
+        w.create_collider(hc);
+        for (int f = 0; f < 120; ++f) w.step();
+        vec3 p = w.bodies.position[w.bodies.index_of(b)];
+        CHECK(std::fabs(p.y - 0.3) < 0.01);
+        CHECK(mag(w.bodies.linear_velocity[w.bodies.index_of(b)]) < 0.05);
+        convex_hull_data_destroy(hd);
+    }
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "ALL PASSED", failures);
     return failures ? 1 : 0;
 }
