@@ -29,6 +29,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "body-substeps/s on 1 MI355X (learned+batched vs plain), pos err \u22641e-4 m"
 WORKLOAD = "plain GPU pipeline, large pile: 65,536 mixed convex bodies, 10 substeps, 500 steps"
+COMBINED_WORKLOAD = ("combined at scale: 262,144 bodies in 1,024 uneven scenes (64-512 bodies) of convex hulls, "
+                     "boxes and 8-segment ragdolls (revolute + spherical joints, limits +-60 deg), dt 1/120 s, "
+                     "8 substeps, XPBD joint mode, learned GJK/EPA warm start")
 HULL_WORKLOAD = ("GJK/EPA-heavy: 32,768 random convex hulls (16-64 vertices on ellipsoids with semi-axes "
                  "U(0.2,0.8), triangulated) in a dense pile over a plane, 10 substeps, plain GJK/EPA")
 
@@ -57,25 +60,33 @@ def cpu_baseline(scene_fn, budget_s=10.0, max_frames=30):
             "sample": f"frames 1..{frames} of the same {sc.n_dynamic}-body scene ({dt:.1f} s, FP64, serial)"}
 
 
-def cpu_baseline_batched(budget_s=10.0, frames=10):
+def cpu_baseline_batched(budget_s=10.0, frames=10, combined=False):
     """The reference runs one World per scene: scenes of the same batch (same seeds),
     each for `frames` frames, one after another on one core, until the budget is spent."""
     from rbps_amd import scenes
-    rng = np.random.default_rng(77)
-    counts = rng.integers(32, 513, 4096)
+    if combined:
+        counts = scenes.combined_counts()
+        lib = scenes.hull_library()
+        make = lambda k: scenes.combined_scene(int(counts[k]), seed=2026 + 1 + k, library=lib)   # noqa: E731
+    else:
+        counts = np.random.default_rng(77).integers(32, 513, 4096)
+        make = lambda k: scenes.stack_chain(int(counts[k]), seed=77 + 1 + k)   # noqa: E731
     t0 = time.perf_counter()
     work = done = 0
     while time.perf_counter() - t0 < budget_s and done < len(counts):
-        sc = scenes.stack_chain(int(counts[done]), seed=77 + 1 + done)
+        sc = make(done)
         w, kind = _cpu_world(sc)
+        if combined:
+            w.set_joint_mode(1)
         w.step(frames)
         work += sc.n_dynamic * sc.substeps * frames
         done += 1
     dt = time.perf_counter() - t0
     return {"value": work / dt, "unit": "body-substeps/s", "cores": 1, "kind": kind,
-            "sample": f"scenes 0..{done - 1} of the batch, {frames} frames each, one World per scene "
+            "sample": f"scenes 0..{done - 1} of the batch, frames 1..{frames} each, one World per scene "
                       f"({dt:.1f} s incl. set-up, FP64, serial; the oracle restatement in the "
-                      f"reference's order because the reference has no spherical joint)"}
+                      f"reference's order because the reference has no spherical joint"
+                      f"{'; XPBD joint mode as on the GPU' if combined else ''})"}
 
 
 def learned_report(w, sc, train_frames=20, capacity=4_000_000, steps=1200, batch=8192):
@@ -123,6 +134,36 @@ def learned_report(w, sc, train_frames=20, capacity=4_000_000, steps=1200, batch
             "depth_abs_max_vs_plain": float(dd.max()) if both.any() else None}
 
 
+def run_frames(w, steps, label=""):
+    """Steps `steps` frames (in chunks of 50 only to print progress) and returns the
+    seconds between two synchronisations of the world's stream."""
+    w.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(50, steps - done)
+        w.step(k)
+        done += k
+        print(f"bench: {label + ' ' if label else ''}{done}/{steps} frames", file=sys.stderr, flush=True)
+    w.synchronize()
+    return time.perf_counter() - t0
+
+
+def plain_leg(world, sc, args, pos_learned, value):
+    """The plain GPU pipeline on a fresh world of the same scene: plain GJK/EPA and the
+    same warm-up and timed frames."""
+    w = world.World(sc.timestep, sc.substeps).load(sc)
+    w.set_option("joint_mode", 1)
+    w.step(args.warmup)
+    dt = run_frames(w, args.steps, "plain")
+    plain = sc.n_dynamic * sc.substeps * args.steps / dt
+    d = np.abs(w.state()["position"] - pos_learned).max(axis=1)
+    return {"value": round(plain, 1), "unit": "body-substeps/s", "ms_per_step": round(dt / args.steps * 1e3, 4),
+            "speedup": round(value / plain, 4),
+            "pos_diff_vs_learned_m": {"max": float(d.max()), "p99": float(np.quantile(d, 0.99)),
+                                      "median": float(np.median(d)), "frame": args.warmup + args.steps}}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload
     (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings), or None."""
@@ -154,11 +195,14 @@ def main():
                              "solve_pos_runs", "solve_vel_runs", "joints", "gjk"],
                     help="kernel family timed for the roofline (default: solve_pos; gjk for --workload hulls)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="hulls: skip the learned warm-start report")
-    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "hulls"],
+    ap.add_argument("--no-learned", action="store_true",
+                    help="hulls: skip the learned warm-start report; combined: time the plain GJK/EPA")
+    ap.add_argument("--no-plain", action="store_true", help="combined: skip the plain leg")
+    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "hulls", "combined"],
                     help="pile: the 65,536-body mixed pile (default); batched: 4,096 independent "
                          "box-stack + capsule-chain scenes (1.1M bodies) in one world; hulls: 32,768 "
-                         "random convex hulls in a dense pile (GJK/EPA)")
+                         "random convex hulls in a dense pile (GJK/EPA); combined: 1,024 scenes of hulls, "
+                         "boxes and ragdolls (262,144 bodies), learned warm start, with a plain leg beside it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB ten times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a --pmc run to bytes)")
@@ -181,27 +225,35 @@ def main():
         scene_fn = scenes.batched_stack_chains
         workload = ("batched scenes: 4,096 independent scenes of 32-512 bodies (box stack + 16-capsule chain "
                     "on spherical joints, compliance 1e-6), 10 substeps, XPBD joint mode")
+    elif args.workload == "combined":
+        scene_fn = scenes.combined_batch
+        workload = COMBINED_WORKLOAD
     else:
         scene_fn = scenes.pile_mixed
         workload = WORKLOAD
     sc = scene_fn()
     w = world.World(sc.timestep, sc.substeps).load(sc)
-    if args.workload == "batched":
+    if args.workload in ("batched", "combined"):
         w.set_option("joint_mode", 1)
+    learned = args.workload == "combined" and not args.no_learned
+    if learned:   # train on the GJK/EPA pairs of the warm-up frames, then warm-start from the network
+        w.mlp_init(5)
+        w.mlp_log(4_000_000)
     w.step(args.warmup)
     w.synchronize()
+    train = None
+    if learned:
+        logged = w.mlp_logged(stop=True)
+        t1 = time.perf_counter()
+        w.mlp_train(600, 8192, 2e-3)
+        loss = w.mlp_train(600, 8192, 5e-4)
+        w.synchronize()
+        train = {"logged_samples": logged, "train_s": round(time.perf_counter() - t1, 3), "train_loss": round(loss, 4)}
+        w.mlp_mode("gjk")
     # per-launch HIP events on every 5th frame: events between launches cost time themselves
     w.set_option("timing_stride", 5)
     w.set_timing(args.timing_kernel)
-    t0 = time.perf_counter()
-    done = 0
-    while done < args.steps:   # chunks only to print progress; the work is the same
-        k = min(50, args.steps - done)
-        w.step(k)
-        done += k
-        print(f"bench: {done}/{args.steps} frames", file=sys.stderr, flush=True)
-    w.synchronize()
-    dt = time.perf_counter() - t0
+    dt = run_frames(w, args.steps, "learned" if learned else "")
     tm = w.timing()
     st = w.stats()
     value = sc.n_dynamic * sc.substeps * args.steps / dt
@@ -229,8 +281,16 @@ def main():
     }
     if args.workload == "hulls" and not args.no_learned:
         out["learned"] = learned_report(w, sc)
+    if args.workload == "combined":
+        out["config"]["shapes"] = "convex hulls, boxes, ragdolls (boxes, sphere, capsules) + static planes"
+        out["config"]["scenes"] = int(sc.meta["scenes"])
+        out["config"]["joints"] = int(len(sc.joints))
+        out["learned"] = train
+        if not args.no_plain:
+            out["plain"] = plain_leg(world, sc, args, w.state()["position"], value)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_batched() if args.workload == "batched" else cpu_baseline(lambda: sc)
+        out["cpu_baseline"] = (cpu_baseline_batched(combined=args.workload == "combined")
+                               if args.workload in ("batched", "combined") else cpu_baseline(lambda: sc))
     print(json.dumps(out))
 
 

@@ -865,7 +865,7 @@ __device__ __forceinline__ void load_side(Side<VPL> &s, const Colliders &C, cons
 // direction and (seed_epa) an EPA seed normal; a zero direction means the plain seed.
 // A warm-started GJK gets GJK_WARM_ITERS iterations; when it fails, or EPA from its
 // simplex fails, the plain run follows.
-// tlog (optional, per bin position two float4 in A's frame): training targets, the EPA
+// tlog (optional, per candidate two float4 in A's frame, after *tlog_base rows): training targets, the EPA
 // normal (or, for a separated pair, the reverse of the GJK exit direction) with a valid
 // flag, and the EPA normal with a valid flag.
 constexpr int GJK_PASS_WAVES = 4;
@@ -896,80 +896,97 @@ __device__ __forceinline__ uint32_t pack_gstat(int gsum, int esum, int gs, int e
 #ifndef GJK_PASS_OCCUPANCY
 #define GJK_PASS_OCCUPANCY 4
 #endif
+// Candidate lists (k_gjk_cull): with `cand`, the passes run over the *ncand candidates,
+// candidate tc being bin position cand[tc]; the warm starts are per candidate, the training
+// targets per bin position.  Both passes stride over "virtual blocks" of four candidates
+// (block b takes b, b + gridDim.x, ...), so the grid does not depend on the count.
+__device__ __forceinline__ uint32_t gjk_count(uint32_t n, const uint32_t *ncand) { return ncand ? min(*ncand, n) : n; }
+
 template <int VPL>
 __global__ void __launch_bounds__(64 * GJK_PASS_WAVES) __attribute__((amdgpu_waves_per_eu(GJK_PASS_OCCUPANCY)))
-k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n,
+k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand, const uint32_t *ncand,
                                                                   Colliders C, Bodies B, Contacts K, HullSet hs,
                                                                   uint32_t *gstat, const float4 *guess, float4 *tlog,
-                                                                  uint32_t *hits, uint32_t *blk_hits, float4 *sxs) {
+                                                                  const uint32_t *tlog_base, uint32_t *hits,
+                                                                  uint32_t *blk_hits, float4 *sxs) {
     __shared__ uint32_t inside_w[GJK_PASS_WAVES];
     const int wave = threadIdx.x >> 6, lane = lane64();
-    const uint32_t t = blockIdx.x * GJK_PASS_WAVES + wave;   // bin position
-    bool inside = false;
-    Simplex sx;
-    if (t < n) {
-        const uint32_t s = bin_slot[off + t];
-        const uint2 cc = K.cc[s], bb = K.bb[s];
-        const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
-        Side<VPL> A, Bs;
-        load_side(A, C, hs, cc.x, ta);
-        load_side(Bs, C, hs, cc.y, tb);
-        const GjkSeeds q = gjk_seeds(ta, tb, guess, t, 0);
-        int it = 0;
-        int gs = gjk(A, Bs, hs, q.g, sx, it, q.warm ? GJK_WARM_ITERS : GJK_ITERS);
-        int gsum = it, fb = 0;
-        if (q.warm && gs == GJK_FAILED) {
-            fb = 1;
-            gs = gjk(A, Bs, hs, q.plain, sx, it);
-            gsum += it;
-        }
-        inside = gs == GJK_INSIDE;
-        if (lane == 0) gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);   // pass 2 completes an overlapping pair
-        if (!inside && lane == 0) {
-            K.cnt[s] = 0;
-            if (tlog) {
-                V3 tg{0.0f, 0.0f, 0.0f};
-                float vg = 0.0f;
-                const float ln = len(sx.dir);
-                // a seed against the exit direction puts the first support on the far side of
-                // the Minkowski difference, so that -v0 already points across it at the origin
-                if (gs == GJK_VALID && ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
-                tlog[2 * t] = f4(tg, vg);
-                tlog[2 * t + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t nc = gjk_count(n, ncand);
+    if (tlog && tlog_base) tlog += 2 * (size_t)*tlog_base;
+    for (uint32_t vb = blockIdx.x; vb * GJK_PASS_WAVES < nc; vb += gridDim.x) {
+        const uint32_t tc = vb * GJK_PASS_WAVES + wave;   // candidate
+        bool inside = false;
+        Simplex sx;
+        if (tc < nc) {
+            const uint32_t t = cand ? cand[tc] : tc;   // bin position
+            const uint32_t s = bin_slot[off + t];
+            const uint2 cc = K.cc[s], bb = K.bb[s];
+            const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+            Side<VPL> A, Bs;
+            load_side(A, C, hs, cc.x, ta);
+            load_side(Bs, C, hs, cc.y, tb);
+            const GjkSeeds q = gjk_seeds(ta, tb, guess, tc, 0);
+            int it = 0;
+            int gs = gjk(A, Bs, hs, q.g, sx, it, q.warm ? GJK_WARM_ITERS : GJK_ITERS);
+            int gsum = it, fb = 0;
+            if (q.warm && gs == GJK_FAILED) {
+                fb = 1;
+                gs = gjk(A, Bs, hs, q.plain, sx, it);
+                gsum += it;
+            }
+            inside = gs == GJK_INSIDE;
+            if (lane == 0) gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);   // pass 2 completes an overlapping pair
+            if (!inside && lane == 0) {
+                K.cnt[s] = 0;
+                if (tlog) {
+                    V3 tg{0.0f, 0.0f, 0.0f};
+                    float vg = 0.0f;
+                    const float ln = len(sx.dir);
+                    // a seed against the exit direction puts the first support on the far side of
+                    // the Minkowski difference, so that -v0 already points across it at the origin
+                    if (gs == GJK_VALID && ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
+                    tlog[2 * tc] = f4(tg, vg);
+                    tlog[2 * tc + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
             }
         }
-    }
-    if (lane == 0) inside_w[wave] = inside ? 1u : 0u;
-    __syncthreads();
-    int pos = 0;
-    for (int k = 0; k < wave; ++k) pos += (int)inside_w[k];
-    if (inside) {
-        const uint32_t h = blockIdx.x * GJK_PASS_WAVES + pos;
-        if (lane == 0) hits[h] = t;
-        if (lane < 4) {   // the four simplex vertices: (w.xyz, w0.x) (w0.y, w0.z)
-            const SV v = sx_vertex(sx, lane);
-            sxs[(size_t)h * 8 + 2 * lane] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
-            sxs[(size_t)h * 8 + 2 * lane + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
+        if (lane == 0) inside_w[wave] = inside ? 1u : 0u;
+        __syncthreads();
+        int pos = 0;
+        for (int k = 0; k < wave; ++k) pos += (int)inside_w[k];
+        if (inside) {
+            const uint32_t h = vb * GJK_PASS_WAVES + pos;
+            if (lane == 0) hits[h] = tc;
+            if (lane < 4) {   // the four simplex vertices: (w.xyz, w0.x) (w0.y, w0.z)
+                const SV v = sx_vertex(sx, lane);
+                sxs[(size_t)h * 8 + 2 * lane] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
+                sxs[(size_t)h * 8 + 2 * lane + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
+            }
         }
-    }
-    if (threadIdx.x == 0) {
-        uint32_t c = 0;
-        for (int k = 0; k < GJK_PASS_WAVES; ++k) c += inside_w[k];
-        blk_hits[blockIdx.x] = c;
+        if (threadIdx.x == 0) {
+            uint32_t c = 0;
+            for (int k = 0; k < GJK_PASS_WAVES; ++k) c += inside_w[k];
+            blk_hits[vb] = c;
+        }
+        __syncthreads();   // inside_w is rewritten by the next virtual block
     }
 }
 
 template <int VPL>
-__global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint32_t off, Colliders C, Bodies B, Contacts K,
+__global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand,
+                                                 const uint32_t *ncand, Colliders C, Bodies B, Contacts K,
                                                  HullSet hs, uint32_t *gstat, const float4 *guess, int seed_epa,
-                                                 float4 *tlog, const uint32_t *hits, const uint32_t *blk_hits,
-                                                 const float4 *sxs) {
+                                                 float4 *tlog, const uint32_t *tlog_base, const uint32_t *hits,
+                                                 const uint32_t *blk_hits, const float4 *sxs) {
     __shared__ PairLds L;
     const int lane = lane64();
-    const uint32_t nh = blk_hits[blockIdx.x];
-    for (uint32_t r = 0; r < nh; ++r) {
-        const uint32_t h = blockIdx.x * GJK_PASS_WAVES + r;
-        const uint32_t t = hits[h];
+    const uint32_t nc = gjk_count(n, ncand);
+    if (tlog && tlog_base) tlog += 2 * (size_t)*tlog_base;
+    for (uint32_t vb = blockIdx.x; vb * GJK_PASS_WAVES < nc; vb += gridDim.x)
+    for (uint32_t r = 0, nh = blk_hits[vb]; r < nh; ++r) {
+        const uint32_t h = vb * GJK_PASS_WAVES + r;
+        const uint32_t tc = hits[h];
+        const uint32_t t = cand ? cand[tc] : tc;
         const uint32_t s = bin_slot[off + t];
         const uint2 cc = K.cc[s], bb = K.bb[s];
         const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
@@ -987,7 +1004,7 @@ __global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint3
         const uint32_t g0 = gstat[s];
         int gsum = (int)(g0 & 255u), fb = (int)((g0 >> 24) & 1u);
         int gs = GJK_INSIDE;
-        const GjkSeeds q = gjk_seeds(ta, tb, guess, t, seed_epa);
+        const GjkSeeds q = gjk_seeds(ta, tb, guess, tc, seed_epa);
         const bool warm = q.warm && !fb;   // a pass-1 fallback already ran the plain GJK
         __syncthreads();
         EpaResult e = epa(sx, A, Bs, hs, L.e, warm && q.seeded ? &q.sn : nullptr);
@@ -1015,8 +1032,8 @@ __global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint3
                     const float ln = len(sx.dir);
                     if (ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
                 }
-                tlog[2 * t] = f4(tg, vg);
-                tlog[2 * t + 1] = ok ? f4(tf_inv_vec(ta, e.n), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                tlog[2 * tc] = f4(tg, vg);
+                tlog[2 * tc + 1] = ok ? f4(tf_inv_vec(ta, e.n), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         }
         Manifold m;
@@ -1043,6 +1060,49 @@ __global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint3
 
 // Totals over the GJK pairs bin_slot[off .. off + n) from gstat: pairs, GJK iterations,
 // EPA iterations, overlapping pairs, EPA converged, fallback runs.
+// Exact rejection in front of GJK: a pair whose bounding spheres (radius about each
+// collider's origin in lpos.w, rounded up) are apart by more than a margin cannot
+// overlap, so plain GJK would end separated and emit nothing; the pair is finished here
+// (no contact, status separated after 0 iterations, not logged for training) and the others
+// are flagged for k_gjk_compact.  The margin (1e-4 relative + 1e-3 m + 1e-6 of the
+// largest coordinate) covers the FP32 rounding of the centres and of the test itself.
+__global__ void k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B, Contacts K,
+                           uint32_t *gstat, uint32_t *keep) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = bin_slot[off + i];
+    const uint2 cc = K.cc[s], bb = K.bb[s];
+    const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+    const float ra = C.lpos[cc.x].w, rb = C.lpos[cc.y].w;
+    const V3 d = tb.pos - ta.pos;
+    const float big = fmaxf(fmaxf(fmaxf(fabsf(ta.pos.x), fabsf(ta.pos.y)), fmaxf(fabsf(ta.pos.z), fabsf(tb.pos.x))),
+                            fmaxf(fabsf(tb.pos.y), fabsf(tb.pos.z)));
+    const float rs = (ra + rb) * 1.0001f + 1e-3f + 1e-6f * big;
+    const bool apart = len2(d) > rs * rs;
+    keep[i] = apart ? 0u : 1u;
+    if (apart) {
+        K.cnt[s] = 0;
+        gstat[s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
+    }
+}
+
+// Algorithmic bytes of the rejection test per pair: slot, colliders and bodies 20 B, per side
+// collider offset/rotation 32 B and body pose 32 B, the flag 4 B (a rejected pair also
+// writes its count and status, 8 B).
+constexpr double GJK_CULL_BYTES = 20.0 + 2.0 * 64.0 + 4.0 + 8.0;
+
+__global__ void k_add_count(const uint32_t *n, unsigned long long *acc) { atomicAdd(acc, (unsigned long long)*n); }
+// *cnt += the candidate count (or n_host without a list): rows appended to the training log.
+__global__ void k_log_advance(uint32_t *cnt, const uint32_t *n_dev, uint32_t n_host) {
+    *cnt += n_dev ? min(*n_dev, n_host) : n_host;
+}
+
+// Stable compaction of the kept bin positions (exclusive scan `pos` of `keep`).
+__global__ void k_gjk_compact(const uint32_t *keep, const uint32_t *pos, uint32_t n, uint32_t *cand) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && keep[i]) cand[pos[i]] = i;
+}
+
 __global__ void k_gjk_totals(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *gstat,
                              unsigned long long *tot) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

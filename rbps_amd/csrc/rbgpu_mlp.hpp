@@ -113,10 +113,12 @@ __device__ __forceinline__ void mlp_forward16(const MlpParams &P, const bf16 *x,
 
 // Inference over n samples (features BF16 [n][32]); out: per sample two float4, the
 // normalised GJK direction and EPA seed normal (zero vectors when degenerate).
-__global__ void __launch_bounds__(256) k_mlp_infer(const bf16 *feat, uint32_t n, MlpParams P, float4 *out) {
+__global__ void __launch_bounds__(256) k_mlp_infer(const bf16 *feat, uint32_t n_max, const uint32_t *n_dev, MlpParams P,
+                                                   float4 *out) {
     __shared__ __attribute__((aligned(16))) bf16 x[MLP_TILE * MLP_IN], h1[MLP_TILE * MLP_H], h2[MLP_TILE * MLP_H];
     __shared__ float o[MLP_TILE * MLP_OUT];
     const int wave = threadIdx.x >> 6, l = lane64();
+    const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;   // the count may live on the device
     for (uint32_t t0 = blockIdx.x * MLP_TILE; t0 < n; t0 += gridDim.x * MLP_TILE) {
         for (int i = threadIdx.x; i < MLP_TILE * MLP_IN; i += blockDim.x) {
             const uint32_t s = t0 + i / MLP_IN;
@@ -319,11 +321,13 @@ __device__ __forceinline__ int kind_slot(int kind) {
 }
 
 // Features of the pairs bin_slot[off .. off + n) into feat [n][32] (BF16).
-__global__ void k_gjk_features(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B, Contacts K,
-                               HullSet hs, bf16 *feat) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = bin_slot[off + i];
+__global__ void k_gjk_features(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand,
+                               const uint32_t *ncand, Colliders C, Bodies B, Contacts K, HullSet hs, bf16 *feat,
+                               const uint32_t *row0) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;   // candidate (bin position without a list)
+    if (i >= gjk_count(n, ncand)) return;
+    const uint32_t s = bin_slot[off + (cand ? cand[i] : i)];
+    if (row0) feat += (size_t)*row0 * MLP_IN;   // appended after the rows already logged
     const uint2 cc = K.cc[s], bb = K.bb[s];
     const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
     float f[MLP_IN];

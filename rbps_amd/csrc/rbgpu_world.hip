@@ -62,6 +62,7 @@ struct Readback {
     uint32_t col_off[MAX_COLOURS + 1];
     uint32_t bin_off[NBINS + 1];
     uint64_t points;
+    uint32_t log_rows;
 };
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -108,6 +109,9 @@ struct rbgpu_world {
     HullSet hs{};
     int hull_vpl = 1;                   // hull vertices per lane in the wave kernels (1: all hulls <= 64)
     uint32_t *gstat = nullptr;
+    uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
+    bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
+    uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
     uint32_t *gjk_hits = nullptr, *gjk_blk = nullptr;   // overlapping GJK pairs per 4-pair workgroup
     float4 *gjk_sx = nullptr;                          // their final simplices (8 float4 each)
     // learned warm start: parameters (FP32 master, BF16 copy), gradient, Adam moments, the
@@ -118,7 +122,9 @@ struct rbgpu_world {
     int mlp_mode = 0;                   // 0 plain, 1 GJK direction, 2 GJK direction + EPA seed
     bf16 *ds_feat = nullptr;
     float4 *ds_tgt = nullptr;
-    uint64_t ds_cap = 0, ds_n = 0;
+    uint64_t ds_cap = 0, ds_n = 0;   // log capacity; rows logged (host copy of *ds_cnt after sync_log_count)
+    uint64_t ds_hi = 0;               // upper bound of the rows logged (ds_n + GJK pairs since the last sync)
+    uint32_t *ds_cnt = nullptr;       // rows logged (device)
     bool ds_logging = false;
     bf16 *inf_feat = nullptr;
     float4 *inf_guess = nullptr;
@@ -181,7 +187,8 @@ struct rbgpu_world {
     void release_pairs() {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
-        dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx);
+        dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx); dfree(gjk_keep); dfree(gjk_pos); dfree(gjk_cand);
+        dfree(gjk_ncand);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
@@ -189,8 +196,8 @@ struct rbgpu_world {
     }
     void release_mlp() {
         dfree(mlp_wf); dfree(mlp_grad); dfree(mlp_m); dfree(mlp_v); dfree(mlp_loss); dfree(mlp_wb);
-        dfree(ds_feat); dfree(ds_tgt); dfree(inf_feat); dfree(inf_guess); dfree(gjk_tot);
-        ds_cap = ds_n = 0;
+        dfree(ds_feat); dfree(ds_tgt); dfree(ds_cnt); dfree(inf_feat); dfree(inf_guess); dfree(gjk_tot);
+        ds_cap = ds_n = ds_hi = 0;
         inf_cap = 0;
     }
     void release() {
@@ -248,27 +255,82 @@ struct rbgpu_world {
         return RBGPU_OK;
     }
 
-    // Before the GJK bins [g0, g0 + ng): features of the pairs (for inference and/or the
-    // training log) and the network's warm-start predictions.
+    // The GJK bins [g0, g0 + ng) (hull-hull and hull-primitive, adjacent in the bin order)
+    // in one go: bounding-sphere rejection and compaction (world steps), features for the
+    // training log (every bin position) and for inference (the candidates), the network's
+    // warm starts, then the two GJK/EPA passes over the candidates.
     bool gjk_log = false;
     bool gjk_explicit = false;   // warm starts given by the caller (rbgpu_collide_world_pairs)
-    bool gjk_guess = false;      // inf_guess holds this narrowphase's warm starts
-    void gjk_prepare(uint32_t g0, uint32_t ng) {
-        gjk_log = !gjk_explicit && ds_logging && ds_feat && ds_n + ng <= ds_cap;
-        if (ds_logging && !gjk_log && !gjk_explicit) ds_logging = false;   // the log is full
-        const bool infer = mlp_mode && mlp_wb && !gjk_explicit && inf_guess && ng <= inf_cap;
-        gjk_guess = gjk_explicit || infer;
-        if (!ng || !(infer || gjk_log)) return;
-        bf16 *feat = gjk_log ? ds_feat + ds_n * MLP_IN : inf_feat;
-        k_gjk_features<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, feat);
-        if (infer) {
-            const uint32_t tiles = (ng + MLP_TILE - 1) / MLP_TILE;
-            k_mlp_infer<<<std::min<uint32_t>(tiles, 1024), 256, 0, st>>>(feat, ng, MlpParams{mlp_wb, mlp_wf}, inf_guess);
+    int launch_gjk(uint32_t g0, uint32_t ng) {
+        gjk_log = false;
+        if (ds_logging && !gjk_explicit && ds_feat) {
+            if (ds_hi + ng > ds_cap) {   // near capacity: the exact count decides
+                int rc = sync_log_count();
+                if (rc) return rc;
+            }
+            gjk_log = ds_hi + ng <= ds_cap;
+            if (!gjk_log) ds_logging = false;   // the log is full
         }
+        const bool infer = mlp_mode && mlp_wb && !gjk_explicit && inf_guess && ng <= inf_cap;
+        float4 *tl = gjk_log ? ds_tgt : nullptr;   // rows after *ds_cnt
+        hipEvent_t *e = timed_begin(11);
+        const uint32_t *cand = nullptr, *ncand = nullptr;
+        if (gjk_cull && !gjk_explicit) {
+            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, gstat, gjk_keep);
+            int rc = scan(gjk_keep, gjk_pos, ng, gjk_ncand);
+            if (rc) return rc;
+            k_gjk_compact<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
+            cand = gjk_cand;
+            ncand = gjk_ncand;
+            if (e) k_add_count<<<1, 1, 0, st>>>(gjk_ncand, d_points);
+        }
+        if (gjk_log) k_gjk_features<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, ds_feat, ds_cnt);
+        if (infer) {
+            k_gjk_features<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, inf_feat, nullptr);
+            const uint32_t tiles = (ng + MLP_TILE - 1) / MLP_TILE;
+            k_mlp_infer<<<std::min<uint32_t>(tiles, 1024), 256, 0, st>>>(inf_feat, ng, ncand, MlpParams{mlp_wb, mlp_wf},
+                                                                         inf_guess);
+        }
+        const float4 *gp = (gjk_explicit || infer) ? inf_guess : nullptr;
+        const int seed = mlp_mode == 2 ? 1 : 0;
+        const uint32_t nvb = (ng + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
+        const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb, epa_grid);
+        if (hull_vpl == 1) {
+            k_gjk_pass<1><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, tl,
+                                                              ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+            k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
+                                             gjk_hits, gjk_blk, gjk_sx);
+        } else {
+            k_gjk_pass<HULL_VPL><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
+                                                                     gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+            k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl,
+                                                    ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+        }
+        timed_end(e);
+        if (e) timing_bytes += (double)ng * (cand ? GJK_CULL_BYTES : gjk_pair_bytes());
+        if (gjk_tot) k_gjk_totals<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, gstat, gjk_tot);
+        if (gjk_log) {
+            k_log_advance<<<1, 1, 0, st>>>(ds_cnt, ncand, ng);
+            ds_hi += ng;
+        }
+        gjk_log = false;
+        return RBGPU_OK;
+    }
+
+    // Rows in the training log (device count -> ds_n, ds_hi).
+    int sync_log_count() {
+        uint32_t c = 0;
+        if (ds_cnt) {
+            HIPCHK(hipMemcpyAsync(&rb->log_rows, ds_cnt, 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            c = rb->log_rows;
+        }
+        ds_n = ds_hi = c;
+        return RBGPU_OK;
     }
 
     // Narrowphase of every bin of the current pair list (bin_off): one launch per bin.
-    void launch_narrowphase() {
+    int launch_narrowphase() {
         for (int b = 0; b < NBINS; ++b) {
             uint32_t o = bin_off[b], n = bin_off[b + 1] - bin_off[b];
             if (!n) continue;
@@ -287,35 +349,13 @@ struct rbgpu_world {
                 case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                 case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                 case BIN_HH:
-                case BIN_HX: {
-                    // the two GJK bins are adjacent: features, inference and logging cover both
-                    const uint32_t g0 = bin_off[BIN_HH], ng = bin_off[BIN_HP] - g0;
-                    if (b == BIN_HH || bin_off[BIN_HH] == bin_off[BIN_HX]) gjk_prepare(g0, ng);
-                    const uint32_t rel = o - g0;
-                    hipEvent_t *e = timed_begin(11);
-                    const float4 *gp = gjk_guess ? inf_guess + 2 * (size_t)rel : nullptr;
-                    float4 *tl = gjk_log ? ds_tgt + 2 * (ds_n + rel) : nullptr;
-                    const uint32_t nb1 = (n + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
-                    if (hull_vpl == 1) {
-                        k_gjk_pass<1><<<nb1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, o, n, C, B, K, hs, gstat, gp, tl,
-                                                                            gjk_hits, gjk_blk, gjk_sx);
-                        k_epa_pass<1><<<nb1, 64, 0, st>>>(bin_list, o, C, B, K, hs, gstat, gp, mlp_mode == 2 ? 1 : 0, tl,
-                                                          gjk_hits, gjk_blk, gjk_sx);
-                    } else {
-                        k_gjk_pass<HULL_VPL><<<nb1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, o, n, C, B, K, hs, gstat, gp,
-                                                                                   tl, gjk_hits, gjk_blk, gjk_sx);
-                        k_epa_pass<HULL_VPL><<<nb1, 64, 0, st>>>(bin_list, o, C, B, K, hs, gstat, gp,
-                                                                 mlp_mode == 2 ? 1 : 0, tl, gjk_hits, gjk_blk, gjk_sx);
-                    }
-                    timed_end(e);
-                    if (e) timing_bytes += (double)n * gjk_pair_bytes();
-                    if (gjk_tot) k_gjk_totals<<<nblk(n), TPB, 0, st>>>(bin_list, o, n, gstat, gjk_tot);
-                    if (b == BIN_HX || bin_off[BIN_HX] == bin_off[BIN_HP]) {
-                        if (gjk_log) ds_n += ng;
-                        gjk_log = false;
+                case BIN_HX:
+                    // both GJK bins in one go, at the first non-empty one
+                    if (b == BIN_HH || bin_off[BIN_HH] == bin_off[BIN_HX]) {
+                        const int rc = launch_gjk(bin_off[BIN_HH], bin_off[BIN_HP] - bin_off[BIN_HH]);
+                        if (rc) return rc;
                     }
                     break;
-                }
                 case BIN_HP:
                     if (hull_vpl == 1) k_narrow_hull_plane<1><<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs);
                     else k_narrow_hull_plane<HULL_VPL><<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs);
@@ -323,6 +363,7 @@ struct rbgpu_world {
                 default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
             }
         }
+        return RBGPU_OK;
     }
 
     // Algorithmic bytes of one GJK pair (k_gjk_pass + k_epa_pass): per side the hull vertices (16 B
@@ -353,6 +394,7 @@ struct rbgpu_world {
 
     // ------------------------------------------------------------ upload
     int build();
+    float bounding_radius(const rbgpu_collider_desc &d) const;
     int build_joints();
     int download_state();
     int ensure_pair_capacity(uint32_t need);
@@ -374,6 +416,7 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
         const size_t nb1 = (cap + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
         rc |= dalloc(gjk_hits, nb1 * GJK_PASS_WAVES); rc |= dalloc(gjk_blk, nb1);
         rc |= dalloc(gjk_sx, nb1 * GJK_PASS_WAVES * 8);
+        rc |= dalloc(gjk_keep, cap); rc |= dalloc(gjk_pos, cap); rc |= dalloc(gjk_cand, cap); rc |= dalloc(gjk_ncand, 1);
     }
     if (mlp_wb) {
         dfree(inf_feat); dfree(inf_guess);
@@ -392,6 +435,22 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     HIPCHK(hipMemsetAsync(K.cnt, 0, cap * sizeof(int), st));
     pcap = cap;
     return RBGPU_OK;
+}
+
+// Radius of a sphere about a collider's origin that holds its shape (FP32, rounded up):
+// sphere r, box |half extents|, capsule half-height + r, hull max |vertex| (0 for planes).
+float rbgpu_world::bounding_radius(const rbgpu_collider_desc &d) const {
+    double r = 0.0;
+    switch (d.shape) {
+        case RBGPU_SPHERE: r = d.dims[0]; break;
+        case RBGPU_BOX: r = std::sqrt(d.dims[0] * d.dims[0] + d.dims[1] * d.dims[1] + d.dims[2] * d.dims[2]); break;
+        case RBGPU_CAPSULE: r = d.dims[0] + d.dims[1]; break;
+        case RBGPU_HULL:
+            for (const auto &v : hh[d.hull_index].t.v) r = std::max(r, std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z));
+            break;
+        default: break;
+    }
+    return std::nextafter(std::nextafter((float)r, INFINITY), INFINITY);
 }
 
 int rbgpu_world::build() {
@@ -428,7 +487,10 @@ int rbgpu_world::build() {
         meta[c] = make_int4(d.shape, (int)d.body_id, d.is_static ? 1 : 0, plane ? (int)BP_INVALID : (int)slot);
         if (!plane) ++slot;
         dims[c] = make_float4((float)d.dims[0], (float)d.dims[1], (float)d.dims[2], (float)d.dims[3]);
-        lpos[c] = make_float4((float)d.local_pos[0], (float)d.local_pos[1], (float)d.local_pos[2], 0.0f);
+        // w: radius of a sphere about the collider's origin that holds the shape (rounded up),
+        // for the exact bounding-sphere rejection in front of GJK
+        lpos[c] = make_float4((float)d.local_pos[0], (float)d.local_pos[1], (float)d.local_pos[2],
+                              bounding_radius(d));
         lrot[c] = make_float4((float)d.local_rot[0], (float)d.local_rot[1], (float)d.local_rot[2], (float)d.local_rot[3]);
         mat[c] = make_float4((float)d.restitution, (float)d.static_friction, (float)d.dynamic_friction, 0.0f);
         chull[c] = d.shape == RBGPU_HULL ? d.hull_index : -1;
@@ -732,7 +794,8 @@ int rbgpu_world::frame() {
         runs.clear();
         if (P) {
             hipEvent_t *e = timed_begin(3);
-            launch_narrowphase();
+            int rcn = launch_narrowphase();
+            if (rcn) return rcn;
             timed_end(e);
             // colouring of the pairs with points (conflict = a shared dynamic body)
             hipEvent_t *ec = timed_begin(7);
@@ -1212,7 +1275,8 @@ int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, doubl
         if (w->d_points) HIPCHK(hipMemcpy(&pts, w->d_points, 8, hipMemcpyDeviceToHost));
         // algorithmic bytes per contact point of the timed per-colour launches: position
         // solve 296 B, velocity solve 280 B
-        *bytes = w->timing_kernel == 11 ? w->timing_bytes
+        // GJK: rejection test per pair of the GJK bins (host sum) + full bytes per candidate
+        *bytes = w->timing_kernel == 11 ? w->timing_bytes + (double)pts * w->gjk_pair_bytes()
                                         : (double)pts * (w->timing_kernel == 1 ? 296.0 : w->timing_kernel == 2 ? 280.0 : 0.0);
     }
     return RBGPU_OK;
@@ -1383,10 +1447,14 @@ int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2
         w->mlp_mode = seed_epa ? 2 : 1;
     }
     w->gjk_explicit = guess6 != nullptr;
-    w->launch_narrowphase();
+    const bool cull_saved = w->gjk_cull;
+    w->gjk_cull = false;   // every pair runs GJK here: the entry reports its iterations and statuses
+    const int rcn = w->launch_narrowphase();
+    w->gjk_cull = cull_saved;
     w->gjk_explicit = false;
     w->mlp_mode = mode_saved;
     w->inf_guess = gsave;
+    if (rcn) return rcn;
     HIPCHK(hipGetLastError());
     std::vector<int> cnt(n);
     std::vector<float4> nr(n), pa(4 * (size_t)n), pb(4 * (size_t)n), rbv(4 * (size_t)n);
@@ -1496,12 +1564,15 @@ int rbgpu_mlp_log(rbgpu_world *w, uint64_t capacity) {
     if (!w) return fail(RBGPU_ERR_INVALID, "null world");
     if (w->st) HIPCHK(hipStreamSynchronize(w->st));
     dfree(w->ds_feat); dfree(w->ds_tgt);
-    w->ds_cap = w->ds_n = 0;
+    w->ds_cap = w->ds_n = w->ds_hi = 0;
     w->ds_logging = false;
     if (!capacity) return RBGPU_OK;
+    if (capacity >= (1ull << 31)) return fail(RBGPU_ERR_INVALID, "log capacity must be below 2^31 samples");
     int rc = 0;
     rc |= dalloc(w->ds_feat, (size_t)capacity * MLP_IN); rc |= dalloc(w->ds_tgt, 2 * (size_t)capacity);
+    if (!w->ds_cnt) rc |= dalloc(w->ds_cnt, 1);
     if (rc) return RBGPU_ERR_OOM;
+    HIPCHK(hipMemset(w->ds_cnt, 0, 4));
     w->ds_cap = capacity;
     w->ds_logging = true;
     return RBGPU_OK;
@@ -1509,7 +1580,10 @@ int rbgpu_mlp_log(rbgpu_world *w, uint64_t capacity) {
 
 int rbgpu_mlp_logged(rbgpu_world *w, uint64_t *n, int stop) {
     if (!w) return fail(RBGPU_ERR_INVALID, "null world");
-    if (w->st) HIPCHK(hipStreamSynchronize(w->st));
+    if (w->st) {
+        int rc = w->sync_log_count();
+        if (rc) return rc;
+    }
     if (n) *n = w->ds_n;
     if (stop) w->ds_logging = false;
     return RBGPU_OK;
@@ -1517,6 +1591,10 @@ int rbgpu_mlp_logged(rbgpu_world *w, uint64_t *n, int stop) {
 
 int rbgpu_mlp_get_log(rbgpu_world *w, uint64_t start, uint32_t n, float *features32, float *targets8) {
     if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (w->st) {
+        int rc = w->sync_log_count();
+        if (rc) return rc;
+    }
     if (start + n > w->ds_n) return fail(RBGPU_ERR_INVALID, "range beyond the logged samples");
     if (!n) return RBGPU_OK;
     HIPCHK(hipStreamSynchronize(w->st));
@@ -1537,6 +1615,10 @@ int rbgpu_mlp_train(rbgpu_world *w, int steps, uint32_t batch, double lr, double
     int rc = need_mlp(w);
     if (rc) return rc;
     if (steps < 0 || batch == 0 || batch > (1u << 20) || !(lr > 0.0)) return fail(RBGPU_ERR_INVALID, "invalid training arguments");
+    if (w->st) {
+        rc = w->sync_log_count();
+        if (rc) return rc;
+    }
     if (!w->ds_n) return fail(RBGPU_ERR_INVALID, "no logged samples (rbgpu_mlp_log, then step)");
     hipStream_t st = w->st;
     bf16 *bf = nullptr;
@@ -1586,7 +1668,7 @@ int rbgpu_mlp_eval(rbgpu_world *w, uint32_t n, const float *feat32, float *out6)
     if (!rc) {
         (void)hipMemcpyAsync(f, feat32, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, st);
         k_f32_to_bf16<<<nblk(n * MLP_IN), TPB, 0, st>>>(f, fb, n * MLP_IN);
-        k_mlp_infer<<<std::min<uint32_t>((n + MLP_TILE - 1) / MLP_TILE, 1024), 256, 0, st>>>(fb, n, MlpParams{w->mlp_wb, w->mlp_wf}, o);
+        k_mlp_infer<<<std::min<uint32_t>((n + MLP_TILE - 1) / MLP_TILE, 1024), 256, 0, st>>>(fb, n, nullptr, MlpParams{w->mlp_wb, w->mlp_wf}, o);
         std::vector<float4> h(2 * (size_t)n);
         (void)hipMemcpyAsync(h.data(), o, 2 * (size_t)n * 16, hipMemcpyDeviceToHost, st);
         if (hipStreamSynchronize(st) != hipSuccess) rc = fail(RBGPU_ERR_HIP, "rbgpu_mlp_eval failed");

@@ -179,6 +179,12 @@ class SceneBuilder:
         self.h.append((v, f))
         return len(self.h) - 1
 
+    def shared_hull(self, data) -> int:
+        """Adds hull data from a library (a tuple made by hull_library); `batch` uploads
+        data shared by several scenes once."""
+        self.h.append(data)
+        return len(self.h) - 1
+
     def body(self, type_=DYNAMIC, mass=1.0, inertia=(1.0, 1.0, 1.0, 0.0, 0.0, 0.0), pos=(0, 0, 0),
              quat=(1, 0, 0, 0), vel=(0, 0, 0), angvel=(0, 0, 0)) -> int:
         r = np.zeros((), BODY_DTYPE)
@@ -478,14 +484,22 @@ def batch(parts, name="batch") -> Scene:
     colliders and joints are re-indexed.  Scenes may overlap in space, they never
     interact.  Timestep and substeps come from the first scene."""
     bodies, cols, joints, hulls = [], [], [], []
+    hull_ids = {}   # hull data shared between scenes (the same arrays) is uploaded once
     nb = 0
     for s, sc in enumerate(parts):
         b = sc.bodies.copy()
         b["scene"] = s
         c = sc.colliders.copy()
         c["body_id"] += nb
-        c["hull_index"][c["shape"] == HULL] += len(hulls)
-        hulls.extend(sc.hulls)
+        remap = np.zeros(len(sc.hulls), np.int32)
+        for k, h in enumerate(sc.hulls):
+            key = (id(h[0]), id(h[1]))
+            if key not in hull_ids:
+                hull_ids[key] = len(hulls)
+                hulls.append(h)
+            remap[k] = hull_ids[key]
+        sel = c["shape"] == HULL
+        c["hull_index"][sel] = remap[c["hull_index"][sel]]
         j = sc.joints.copy()
         j["body_1"] += nb
         j["body_2"] += nb
@@ -607,3 +621,107 @@ def hull_zoo(seed=9) -> Scene:
                     _hull_body(sb, p, hs[int(rng.integers(0, len(hs)))], quat=q)
                 k += 1
     return sb.build("hull_zoo", 1.0 / 60.0, 10, 1, seed=seed)
+
+
+# ---------------------------------------------------------------- combined (ragdolls + hulls + boxes)
+def hull_library(n=16, seed=101, axis_min=0.15, axis_max=0.45):
+    """n ellipsoid hulls (16-64 vertices, Qhull triangles) shared by the scenes of a batch."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        v, f = ellipsoid_hull(rng, axis_min=axis_min, axis_max=axis_max, faces=True)
+        out.append((np.ascontiguousarray(v, np.float64), np.ascontiguousarray(f, np.uint32)))
+    return out
+
+
+RAGDOLL_LIMIT = math.radians(60.0)
+
+
+def ragdoll(sb, origin, yaw=0.0, density=1000.0, compliance=1e-6, friction=0.5) -> list:
+    """An 8-segment ragdoll standing at `origin` (pelvis centre), turned by `yaw` about y:
+    pelvis, abdomen and chest boxes, a sphere head, capsule arms and legs.  Joints:
+    pelvis-abdomen, abdomen-chest and the hips are revolute about the body's x axis with
+    limits of +-60 deg; neck and shoulders are spherical with a 60 deg swing limit.  Joint
+    points sit in gaps between the segments, so a straight ragdoll has no self-contacts."""
+    cy, sy = math.cos(0.5 * yaw), math.sin(0.5 * yaw)
+    q = (cy, 0.0, sy, 0.0)
+    ox, oy, oz = origin
+
+    def at(x, y, z):   # body-frame offset (ragdoll frame) -> world
+        c, s = math.cos(yaw), math.sin(yaw)
+        return (ox + c * x + s * z, oy + y, oz - s * x + c * z)
+
+    pelvis = _box(sb, at(0.0, 0.0, 0.0), (0.15, 0.08, 0.1), density, quat=q, friction=friction)
+    belly = _box(sb, at(0.0, 0.2, 0.0), (0.13, 0.08, 0.09), density, quat=q, friction=friction)
+    chest = _box(sb, at(0.0, 0.42, 0.0), (0.17, 0.1, 0.1), density, quat=q, friction=friction)
+    rh = 0.1
+    head = sb.body(DYNAMIC, density * sphere_volume(rh), tuple(density * x for x in sphere_inertia(rh)), at(0.0, 0.68, 0.0), q)
+    sb.collider(SPHERE, head, (rh,), restitution=0.0, static_friction=friction, dynamic_friction=friction)
+    arms = [_capsule(sb, at(sx * 0.27, 0.32, 0.0), 0.18, 0.05, density, quat=q, friction=friction) for sx in (-1.0, 1.0)]
+    legs = [_capsule(sb, at(sx * 0.09, -0.42, 0.0), 0.22, 0.06, density, quat=q, friction=friction) for sx in (-1.0, 1.0)]
+    lim = RAGDOLL_LIMIT
+    hinge = dict(axis=(1, 0, 0), limit_axis=(0, 1, 0), compliance=compliance, limited=True, lower=-lim, upper=lim)
+    sb.joint(REVOLUTE, pelvis, belly, r_1=(0.0, 0.1, 0.0), r_2=(0.0, -0.1, 0.0), **hinge)
+    sb.joint(REVOLUTE, belly, chest, r_1=(0.0, 0.1, 0.0), r_2=(0.0, -0.12, 0.0), **hinge)
+    sb.joint(SPHERICAL, chest, head, r_1=(0.0, 0.13, 0.0), r_2=(0.0, -0.13, 0.0), axis=(0, 1, 0),
+             compliance=compliance, limited=True, upper=lim)
+    for sx, a in zip((-1.0, 1.0), arms):
+        sb.joint(SPHERICAL, chest, a, r_1=(sx * 0.27, 0.16, 0.0), r_2=(0.0, 0.26, 0.0), axis=(0, 1, 0),
+                 compliance=compliance, limited=True, upper=lim)
+    for sx, l in zip((-1.0, 1.0), legs):
+        sb.joint(REVOLUTE, pelvis, l, r_1=(sx * 0.09, -0.11, 0.0), r_2=(0.0, 0.31, 0.0), **hinge)
+    return [pelvis, belly, chest, head] + arms + legs
+
+
+def combined_scene(n_bodies=256, seed=0, library=None, ragdoll_share=0.375, spacing=1.35, y0=2.6, density=1000.0,
+                   frames=1000) -> Scene:
+    """One scene of the combined workload: a ground plane, ragdolls standing on a grid
+    (about `ragdoll_share` of the bodies, 8 per ragdoll), and above them a pile of convex
+    hulls (from the shared `library`) and boxes (half-extents U(0.15, 0.35)), half each,
+    that falls onto the ragdolls.  dt = 1/120 s, 8 substeps."""
+    rng = np.random.default_rng(seed)
+    lib = library if library is not None else hull_library()
+    sb = SceneBuilder()
+    add_ground_plane(sb)
+    hidx = [sb.shared_hull(h) for h in lib]
+    n_rag = int(round(n_bodies * ragdoll_share / 8.0))
+    n_rest = n_bodies - 8 * n_rag
+    side = max(1, int(math.ceil(math.sqrt(max(n_rag, 1)))))
+    for k in range(n_rag):
+        x = (k % side - (side - 1) / 2) * 1.0 + rng.uniform(-0.05, 0.05)
+        z = (k // side - (side - 1) / 2) * 1.0 + rng.uniform(-0.05, 0.05)
+        ragdoll(sb, (x, 0.95, z), yaw=float(rng.uniform(-math.pi, math.pi)), density=density)
+    cols = max(side, int(math.ceil(math.sqrt(n_rest / 6.0))))
+    for k in range(n_rest):
+        c, lvl = k % (cols * cols), k // (cols * cols)
+        p = ((c % cols - (cols - 1) / 2) * spacing + rng.uniform(-0.05, 0.05), y0 + lvl * spacing,
+             (c // cols - (cols - 1) / 2) * spacing + rng.uniform(-0.05, 0.05))
+        if k % 2 == 0:
+            _hull_body(sb, p, hidx[int(rng.integers(0, len(hidx)))], density, _rand_quat(rng))
+        else:
+            _box(sb, p, tuple(rng.uniform(0.15, 0.35, 3)), density, quat=_rand_quat(rng))
+    return sb.build(f"combined_{n_bodies}", 1.0 / 120.0, 8, frames, seed=seed)
+
+
+def combined_counts(n_scenes=1024, total=262144, n_min=64, n_max=512, seed=2026):
+    """Uneven body counts: uniform in [n_min, n_max], then scaled about n_min and
+    adjusted by single bodies so that they sum to `total` (every count stays in range)."""
+    rng = np.random.default_rng(seed)
+    u = rng.integers(n_min, n_max + 1, n_scenes).astype(np.float64)
+    f = (total - n_min * n_scenes) / max((u - n_min).sum(), 1.0)
+    c = np.clip(np.round(n_min + (u - n_min) * f), n_min, n_max).astype(np.int64)
+    while c.sum() != total:
+        i = int(rng.integers(0, n_scenes))
+        d = 1 if c.sum() < total else -1
+        if n_min <= c[i] + d <= n_max:
+            c[i] += d
+    return c
+
+
+def combined_batch(n_scenes=1024, total=262144, seed=2026, n_min=64, n_max=512, frames=1000) -> Scene:
+    """The combined workload: n_scenes independent scenes of uneven size (summing to
+    `total` dynamic bodies) mixing convex hulls, boxes and 8-segment ragdolls, one world."""
+    lib = hull_library()
+    counts = combined_counts(n_scenes, total, n_min, n_max, seed)
+    parts = [combined_scene(int(n), seed=seed + 1 + s, library=lib, frames=frames) for s, n in enumerate(counts)]
+    return batch(parts, f"combined_{n_scenes}_scenes_{total}")
