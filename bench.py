@@ -150,10 +150,11 @@ def run_frames(w, steps, label=""):
 
 
 def plain_leg(world, sc, args, pos_learned, value):
-    """The plain GPU pipeline on a fresh world of the same scene: plain GJK/EPA and the
-    same warm-up and timed frames."""
+    """The plain GPU pipeline on a fresh world of the same scene: plain GJK/EPA, grid-wide
+    colouring and per-colour solve launches, the same warm-up and timed frames."""
     w = world.World(sc.timestep, sc.substeps).load(sc)
     w.set_option("joint_mode", 1)
+    w.set_option("scene_kernels", 0)   # grid-wide colouring rounds and per-colour launches
     w.step(args.warmup)
     dt = run_frames(w, args.steps, "plain")
     plain = sc.n_dynamic * sc.substeps * args.steps / dt
@@ -198,6 +199,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="hulls: skip the learned warm-start report; combined: time the plain GJK/EPA")
     ap.add_argument("--no-plain", action="store_true", help="combined: skip the plain leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every substep directly instead of replaying a captured hipGraph "
+                         "(kernel-trace profiles: rocprofv3's kernel trace does not survive graph replays here)")
     ap.add_argument("--workload", default="pile", choices=["pile", "batched", "hulls", "combined"],
                     help="pile: the 65,536-body mixed pile (default); batched: 4,096 independent "
                          "box-stack + capsule-chain scenes (1.1M bodies) in one world; hulls: 32,768 "
@@ -235,6 +239,8 @@ def main():
     w = world.World(sc.timestep, sc.substeps).load(sc)
     if args.workload in ("batched", "combined"):
         w.set_option("joint_mode", 1)
+    if args.no_graph:
+        w.set_option("graph", 0)
     learned = args.workload == "combined" and not args.no_learned
     if learned:   # train on the GJK/EPA pairs of the warm-up frames, then warm-start from the network
         w.mlp_init(5)

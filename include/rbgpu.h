@@ -100,6 +100,8 @@ typedef struct {
     uint32_t contacts;           /* contact points of the last substep */
     uint32_t bins[16];           /* pairs per narrowphase bin of the last frame (SS SB BB SC CC CB SP BP CP HH HX HP other) */
     uint32_t active_pairs;       /* pairs with contact points in the last substep */
+    uint32_t graph_frames;       /* frames whose substeps ran as a captured hipGraph */
+    uint32_t scene_kernels;      /* 1: the last frame coloured and solved per scene */
 } rbgpu_stats;
 
 typedef struct rbgpu_world rbgpu_world;
@@ -201,7 +203,19 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          row as a1 x a2, which turns the axes apart; chains come apart and
  *                          hinges diverge.  1 = XPBD rows (Mueller et al. 2020): each row's
  *                          error re-evaluated just before it is solved, lambda reset every
- *                          substep, direction v/|v| whenever |v| > 1e-9, alignment a2 x a1. */
+ *                          substep, direction v/|v| whenever |v| > 1e-9, alignment a2 x a1.
+ *   RBGPU_OPT_SCENE_KERNELS  batches: colour and solve each scene in one workgroup (one launch
+ *                          each per substep, no host round trip) instead of grid-wide rounds
+ *                          and per-colour launches; -1 auto (default: on for 16 or more
+ *                          scenes of at most 4096 bodies), 0 off, 1 on.  Same colours, same
+ *                          states.  Colouring errors then surface at the next step.
+ *   RBGPU_OPT_SCENE_THREADS  threads per workgroup of the per-scene solve (64, 128, 256; 64)
+ *   RBGPU_OPT_GJK_CULL     exact bounding-sphere rejection in front of GJK (1, default; 0 runs
+ *                          GJK on every pair of the hull bins)
+ *   RBGPU_OPT_GJK_GRID, RBGPU_OPT_EPA_GRID  workgroups of the two GJK/EPA passes (4096, 8192)
+ *   RBGPU_OPT_GRAPH        with the per-scene kernels: capture one substep per frame in a
+ *                          hipGraph and launch it for every substep (1, default; frames with
+ *                          kernel timing or an open training log run without it) */
 enum {
     RBGPU_OPT_PALETTE_MAX = 0,
     RBGPU_OPT_HEAD_ROUNDS = 1,
@@ -209,7 +223,13 @@ enum {
     RBGPU_OPT_TAIL_MIN = 3,
     RBGPU_OPT_RUN_MIN = 4,
     RBGPU_OPT_TIMING_STRIDE = 5,
-    RBGPU_OPT_JOINT_MODE = 6
+    RBGPU_OPT_JOINT_MODE = 6,
+    RBGPU_OPT_SCENE_KERNELS = 7,
+    RBGPU_OPT_SCENE_THREADS = 8,
+    RBGPU_OPT_GJK_CULL = 9,
+    RBGPU_OPT_GJK_GRID = 10,
+    RBGPU_OPT_EPA_GRID = 11,
+    RBGPU_OPT_GRAPH = 12
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);

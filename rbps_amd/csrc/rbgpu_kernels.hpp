@@ -1003,4 +1003,161 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_solve_vel_run(Bodies B, Contac
     }
 }
 
+// ------------------------------------------------------------------ batches: one workgroup per scene
+// Scenes share no bodies, so a batch's colouring and colour-ordered solve can run scene by
+// scene, each in one workgroup with workgroup barriers instead of kernel boundaries and host
+// round trips.  k_scene_colour runs exactly the rounds of the grid-wide path (rounds 0, 1, ...
+// with the same keys, palettes and masks), so every pair gets the same colour; the solve
+// kernels walk a scene's colours in order, and pairs of one colour share no dynamic body, so
+// the states are bit-identical to the per-colour launches.
+constexpr uint32_t SCENE_THREADS = 256;      // k_scene_colour's workgroup (the bucketing assumes it)
+constexpr uint32_t SCENE_LDS_PAIRS = 6144;   // active pairs of a scene kept in LDS (more: the global slice)
+struct SceneTabs {
+    const uint32_t *sap_start, *sap_end;     // the scene's SAP pairs [start, end) (both 0: none)
+    const uint32_t *plane_off, *plane_idx;   // its plane pairs: plane list entries plane_idx[plane_off[s] ..)
+    const uint32_t *base;                    // exclusive scan of the scenes' pair counts
+    uint32_t *work;                          // per scene at base[s]: active pairs (build order)
+    uint32_t *list;                          // per scene at base[s]: active pairs by colour
+    uint32_t *coff;                          // per scene MAX_COLOURS + 1 offsets into its list
+    uint32_t *ncol;                          // per scene: colours used
+    uint32_t *err;                           // sticky: 1 colour capacity exceeded, 2 no convergence
+};
+
+// Pairs per scene (SAP run + plane pairs), for the slice offsets.
+__global__ void k_scene_sizes(SceneTabs T, uint32_t nsc, uint32_t *size) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsc) size[s] = (T.sap_end[s] - T.sap_start[s]) + (T.plane_off[s + 1] - T.plane_off[s]);
+}
+
+__global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, const uint2 *pbod, const uint32_t *n_sap_dev,
+                                                                Bodies B, ColourBufs cb, SceneTabs T, uint32_t cap_rounds) {
+    __shared__ uint32_t L[SCENE_LDS_PAIRS];
+    __shared__ uint32_t hist[MAX_COLOURS];
+    __shared__ uint32_t sh[SCENE_THREADS / 64 + 1];
+    __shared__ uint32_t nact, won, top;
+    const uint32_t s = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
+    const uint32_t ns = *n_sap_dev;
+    const uint32_t a0 = T.sap_start[s], na = T.sap_end[s] - a0;
+    const uint32_t q0 = T.plane_off[s], total = na + (T.plane_off[s + 1] - q0);
+    uint32_t *G = T.work + T.base[s];
+    if (tid == 0) { nact = 0u; won = 0u; top = 0u; }
+    __syncthreads();
+    // activation: open the pairs with points, count them per dynamic body, list them
+    for (uint32_t j0 = 0; j0 < total; j0 += bd) {
+        const uint32_t j = j0 + tid;
+        bool a = false;
+        uint32_t p = 0u;
+        if (j < total) {
+            p = j < na ? a0 + j : ns + T.plane_idx[q0 + j - na];
+            a = cnt[p] > 0;
+            cb.acol[p] = a ? COLOUR_OPEN : COLOUR_NONE;
+            if (a) {
+                const uint2 bb = pbod[p];
+                if (is_dynamic(B, bb.x)) atomicAdd(&cb.ucnt[bb.x], 1u);
+                if (is_dynamic(B, bb.y)) atomicAdd(&cb.ucnt[bb.y], 1u);
+            }
+        }
+        const uint64_t m = __ballot(a);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t b0 = 0u;
+            if ((int)(tid & 63) == leader) b0 = atomicAdd(&nact, (uint32_t)__popcll(m));
+            b0 = __shfl(b0, leader, 64);
+            if (a) {
+                const uint32_t pos = b0 + lane_rank(m);
+                G[pos] = p;
+                if (pos < SCENE_LDS_PAIRS) L[pos] = p;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t n = nact;
+    const bool in_lds = n <= SCENE_LDS_PAIRS;
+    // rounds 0, 1, ... as the grid-wide path runs them
+    uint32_t left = n, r = 0;
+    for (; left > 0 && r < cap_rounds; ++r) {
+        for (uint32_t j = tid; j < n; j += bd) {
+            const uint32_t p = in_lds ? L[j] : G[j];
+            if (cb.acol[p] == COLOUR_OPEN) colour_claim(cb, pbod, B, p, r);
+        }
+        __syncthreads();
+        uint32_t w = 0;
+        for (uint32_t j = tid; j < n; j += bd) {
+            const uint32_t p = in_lds ? L[j] : G[j];
+            if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r)) ++w;
+        }
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+        if ((tid & 63) == 0 && w) atomicAdd(&won, w);
+        __syncthreads();
+        left = n - won;
+        __syncthreads();
+    }
+    // bucket by colour: sizes, offsets (four colours per thread), scatter
+    for (uint32_t c = tid; c < (uint32_t)MAX_COLOURS; c += bd) hist[c] = 0u;
+    __syncthreads();
+    for (uint32_t j = tid; j < n; j += bd) {
+        const int c = cb.acol[in_lds ? L[j] : G[j]];
+        if (c >= 0) atomicAdd(&hist[c], 1u);
+    }
+    __syncthreads();
+    constexpr uint32_t PER = MAX_COLOURS / SCENE_THREADS;
+    uint32_t v[PER], sum = 0u;
+    for (uint32_t k = 0; k < PER; ++k) {
+        v[k] = hist[tid * PER + k];
+        sum += v[k];
+        if (v[k]) atomicMax(&top, tid * PER + k + 1u);
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(sum, sh, &tot);
+    uint32_t *co = T.coff + (size_t)s * (MAX_COLOURS + 1);
+    for (uint32_t k = 0; k < PER; ++k) {
+        co[tid * PER + k] = run;
+        hist[tid * PER + k] = run;   // scatter cursor
+        run += v[k];
+    }
+    if (tid == 0) co[MAX_COLOURS] = tot;
+    __syncthreads();
+    uint32_t *out = T.list + T.base[s];
+    for (uint32_t j = tid; j < n; j += bd) {
+        const uint32_t p = in_lds ? L[j] : G[j];
+        const int c = cb.acol[p];
+        if (c >= 0) out[atomicAdd(&hist[c], 1u)] = p;
+    }
+    if (tid == 0) {
+        T.ncol[s] = top;
+        if (n) {
+            atomicAdd(&cb.ctr[CTR_ACT], n);
+            atomicMax(&cb.ctr[CTR_NCOL], top);
+            atomicMax(&cb.ctr[CTR_ROUNDS], r);
+        }
+        uint32_t e = ld_shared(&cb.ctr[CTR_ERR]) & 1u;
+        if (left) e |= 2u;
+        if (e) atomicOr(T.err, e);
+    }
+}
+
+// Colour-ordered solve of every scene, one workgroup per scene (pos: XPBD position solve;
+// otherwise the velocity solve).  Lanes iterate alike, so the point counter may reduce per wave.
+template <bool POS>
+__global__ void __launch_bounds__(256) k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pts) {
+    const uint32_t s = blockIdx.x;
+    const uint32_t nc = T.ncol[s];
+    const uint32_t *lst = T.list + T.base[s];
+    const uint32_t *co = T.coff + (size_t)s * (MAX_COLOURS + 1);
+    for (uint32_t c = 0; c < nc; ++c) {
+        const uint32_t o = co[c], e = co[c + 1];
+        for (uint32_t i0 = o; i0 < e; i0 += blockDim.x) {
+            const uint32_t i = i0 + threadIdx.x;
+            const bool v = i < e;
+            const uint32_t p = v ? lst[i] : 0u;
+            if (pts) count_points(K, p, v, pts);
+            if (v) {
+                if (POS) solve_pos_pair(B, K, p);
+                else solve_vel_pair(B, K, p, h);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace rb

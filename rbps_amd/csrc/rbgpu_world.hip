@@ -63,6 +63,7 @@ struct Readback {
     uint32_t bin_off[NBINS + 1];
     uint64_t points;
     uint32_t log_rows;
+    uint32_t scene_err;
 };
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -99,6 +100,18 @@ struct rbgpu_world {
 
     uint32_t N = 0, M = 0, S = 0, NEP = 0, NJ = 0;
     uint32_t nscenes = 1;               // independent scenes (batch) in this world
+    // one workgroup per scene for colouring and solve (option scene_kernels: -1 auto, 0 off, 1 on)
+    int scene_mode = -1;
+    bool scenes_on = false;
+    uint32_t max_scene_bodies = 0, scene_threads = 64;
+    uint32_t *sc_plane_off = nullptr, *sc_plane_idx = nullptr, *sc_size = nullptr, *sc_base = nullptr;
+    uint32_t *sc_work = nullptr, *sc_list = nullptr, *sc_coff = nullptr, *sc_ncol = nullptr, *sc_err = nullptr;
+    SceneTabs scene_tabs() const {
+        return SceneTabs{sap_start, sap_end, sc_plane_off, sc_plane_idx, sc_base, sc_work, sc_list, sc_coff, sc_ncol, sc_err};
+    }
+    bool scene_path() const {
+        return nscenes > 1 && (scene_mode == 1 || (scene_mode == -1 && nscenes >= 16 && max_scene_bodies <= 4096));
+    }
     uint32_t *cscene = nullptr, *slot_scene = nullptr, *sap_start = nullptr, *sap_end = nullptr;
     uint32_t *plane_lid = nullptr, *pair_lid = nullptr;
     uint32_t *bb_hits = nullptr, *bb_blk = nullptr;   // box-box overlap lists (per 128-pair block)
@@ -188,7 +201,7 @@ struct rbgpu_world {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
         dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx); dfree(gjk_keep); dfree(gjk_pos); dfree(gjk_cand);
-        dfree(gjk_ncand);
+        dfree(gjk_ncand); dfree(sc_work); dfree(sc_list);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
@@ -216,8 +229,12 @@ struct rbgpu_world {
         dfree(cb.used); dfree(cb.ucnt); dfree(cb.claim); dfree(cb.hist); dfree(cb.off); dfree(cb.cur); dfree(cb.ctr);
         dfree(plane_pairs); dfree(offs); dfree(d_points);
         dfree(cscene); dfree(slot_scene); dfree(sap_start); dfree(sap_end); dfree(plane_lid);
+        dfree(sc_plane_off); dfree(sc_plane_idx); dfree(sc_size); dfree(sc_base); dfree(sc_coff); dfree(sc_ncol);
+        dfree(sc_err);
         for (auto &e : ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         ev_pool.clear();
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
         if (rb) (void)hipHostFree(rb);
         rb = nullptr;
         if (st) (void)hipStreamDestroy(st);
@@ -317,6 +334,41 @@ struct rbgpu_world {
         return RBGPU_OK;
     }
 
+    // Substep graph: captured from `body` (one substep), then launched `substeps` times.  The
+    // executable graph is kept and updated in place by the next capture when the update
+    // succeeds (same launch shapes), else instantiated again.
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    template <class F> int run_graph(F &&body) {
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = body();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc || ec != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(RBGPU_ERR_HIP, "substep graph capture failed");
+        }
+        if (gexec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            if (hipGraphExecUpdate(gexec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(gexec);
+                gexec = nullptr;
+            }
+        }
+        if (!gexec && hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+            gexec = nullptr;
+            (void)hipGraphDestroy(g);
+            return fail(RBGPU_ERR_HIP, "substep graph instantiation failed");
+        }
+        (void)hipGraphDestroy(g);
+        for (int k = 0; k < substeps; ++k) HIPCHK(hipGraphLaunch(gexec, st));
+        ++graph_frames;
+        return RBGPU_OK;
+    }
+    uint64_t graph_frames = 0;
+
     // Rows in the training log (device count -> ds_n, ds_hi).
     int sync_log_count() {
         uint32_t c = 0;
@@ -410,7 +462,7 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     int rc = 0;
     rc |= dalloc(pairs, cap);
     rc |= dalloc(ka, cap); rc |= dalloc(va, cap); rc |= dalloc(kb, cap); rc |= dalloc(solve_list, cap);
-    rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap);
+    rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap); rc |= dalloc(sc_work, cap); rc |= dalloc(sc_list, cap);
     rc |= dalloc(bb_hits, nblk(cap, 128) * 128); rc |= dalloc(bb_blk, nblk(cap, 128)); rc |= dalloc(gstat, cap);
     if (!hh.empty()) {
         const size_t nb1 = (cap + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
@@ -608,6 +660,23 @@ int rbgpu_world::build() {
     if (n_plane) HIPCHK(up(plane_pairs, planes.data(), n_plane * 8));
     if (n_plane) HIPCHK(up(plane_lid, plids.data(), n_plane * 4));
     nscenes = nsc;
+    {   // per-scene tables: plane-pair lists (constant per build), slice sizes and offsets, colours
+        std::vector<uint32_t> poff(nsc + 1, 0), pidx(n_plane), nb(nsc, 0);
+        for (uint32_t j = 0; j < n_plane; ++j) ++poff[csc[planes[j].y] + 1];
+        for (uint32_t k = 0; k < nsc; ++k) poff[k + 1] += poff[k];
+        std::vector<uint32_t> cur(poff.begin(), poff.end() - 1);
+        for (uint32_t j = 0; j < n_plane; ++j) pidx[cur[csc[planes[j].y]]++] = j;
+        for (const rbgpu_body_desc &d : hb) ++nb[d.scene];
+        max_scene_bodies = nsc ? *std::max_element(nb.begin(), nb.end()) : 0;
+        int rs = 0;
+        rs |= dalloc(sc_plane_off, nsc + 1); rs |= dalloc(sc_plane_idx, std::max<uint32_t>(n_plane, 1));
+        rs |= dalloc(sc_size, nsc + 1); rs |= dalloc(sc_base, nsc + 1); rs |= dalloc(sc_ncol, nsc);
+        rs |= dalloc(sc_coff, (size_t)nsc * (MAX_COLOURS + 1)); rs |= dalloc(sc_err, 1);
+        if (rs) return RBGPU_ERR_OOM;
+        HIPCHK(up(sc_plane_off, poff.data(), (nsc + 1) * 4));
+        if (n_plane) HIPCHK(up(sc_plane_idx, pidx.data(), n_plane * 4));
+        HIPCHK(hipMemsetAsync(sc_err, 0, 4, st));
+    }
     if (m) HIPCHK(up(cscene, csc.data(), m * 4));
     if (S) HIPCHK(up(slot_scene, ssc.data(), S * 4));
     int rcj = build_joints();
@@ -774,7 +843,11 @@ int rbgpu_world::frame() {
         HIPCHK(hipMemcpyAsync(&rb->n_sap, scal, 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&rb->overflow, overflow, 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(rb->bin_off, offs, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
+        if (sc_err) HIPCHK(hipMemcpyAsync(&rb->scene_err, sc_err, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        // errors of the per-scene colouring (sticky; found here, one frame after the fact)
+        if (sc_err && (rb->scene_err & 1u)) return fail(RBGPU_ERR_CAPACITY, "more than 1024 colours needed");
+        if (sc_err && (rb->scene_err & 2u)) return fail(RBGPU_ERR_CAPACITY, "colouring did not converge");
         if (!rb->overflow) break;
         if (attempt > 0) return fail(RBGPU_ERR_CAPACITY, "pair buffer overflow");
         int rc2 = ensure_pair_capacity(rb->n_pairs);
@@ -786,11 +859,20 @@ int rbgpu_world::frame() {
     bin_off.assign(rb->bin_off, rb->bin_off + NBINS + 1);
     const uint32_t P = n_pairs;
     const uint32_t colour_rounds_cap = 1u << 16;
+    const bool by_scene = scene_path();
+    const SceneTabs T = scene_tabs();
+    if (by_scene && P) {   // slice offsets of the scenes' pair lists
+        k_scene_sizes<<<nblk(nscenes), TPB, 0, st>>>(T, nscenes, sc_size);
+        int rcs = scan(sc_size, sc_base, nscenes, nullptr);
+        if (rcs) return rcs;
+    }
 
-    // 4. substeps (World.hpp:203-211)
+    // 4. substeps (World.hpp:203-211).  One substep's launches; with the per-scene kernels
+    //    nothing in it waits for the host, so it is captured once per frame into a hipGraph
+    //    and the graph is launched for every substep.
     cb.pal_max = palette_max;
     cb.lid = pair_lid;
-    for (int s = 0; s < substeps; ++s) {
+    auto substep = [&]() -> int {
         runs.clear();
         if (P) {
             hipEvent_t *e = timed_begin(3);
@@ -800,6 +882,10 @@ int rbgpu_world::frame() {
             // colouring of the pairs with points (conflict = a shared dynamic body)
             hipEvent_t *ec = timed_begin(7);
             k_colour_reset<<<nblk(std::max<uint32_t>(N * (COLOUR_WORDS / 4), MAX_COLOURS)), TPB, 0, st>>>(N, cb);
+            if (by_scene) {   // per scene: rounds, then the colour-ordered lists (no host round trip)
+                k_scene_colour<<<nscenes, SCENE_THREADS, 0, st>>>(K.cnt, K.bb, scal + 0, B, cb, T, colour_rounds_cap);
+                timed_end(ec);
+            } else {
             k_colour_activate<<<nblk(P), TPB, 0, st>>>(K.cnt, P, K.bb, B, cb);
             int rcs = scan(cb.blk, cb.blk, nblk(P), cb.ctr + CTR_ACT);
             if (rcs) return rcs;
@@ -844,10 +930,16 @@ int rbgpu_world::frame() {
                 c = c1;
             }
             timed_end(ec);
+            }
         }
         if (N) {
             hipEvent_t *e = timed_begin(4);
             k_integrate<<<nblk(N), TPB, 0, st>>>(B, N, h);
+            timed_end(e);
+        }
+        if (by_scene && P) {
+            hipEvent_t *e = timed_begin(1);
+            k_scene_solve<true><<<nscenes, scene_threads, 0, st>>>(B, K, T, h, e ? d_points : nullptr);
             timed_end(e);
         }
         for (const Run &u : runs) {
@@ -878,6 +970,11 @@ int rbgpu_world::frame() {
             k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h);
             timed_end(e);
         }
+        if (by_scene && P) {
+            hipEvent_t *e = timed_begin(2);
+            k_scene_solve<false><<<nscenes, scene_threads, 0, st>>>(B, K, T, h, e ? d_points : nullptr);
+            timed_end(e);
+        }
         for (const Run &u : runs) {
             if (u.group) {
                 hipEvent_t *e = timed_begin(9);
@@ -898,6 +995,17 @@ int rbgpu_world::frame() {
                 if (n) k_joint_damp<<<nblk(n), TPB, 0, st>>>(B, Jt, jorder, o, n, h);
             }
             timed_end(e);
+        }
+        return RBGPU_OK;
+    };
+    const bool timed = timing_kernel != 0 && frames % timing_stride == 0;   // per-launch events
+    if (use_graph && by_scene && !timed && !ds_logging && substeps > 1) {
+        int rc = run_graph(substep);
+        if (rc) return rc;
+    } else {
+        for (int s = 0; s < substeps; ++s) {
+            int rc = substep();
+            if (rc) return rc;
         }
     }
     ++frames;
@@ -1187,6 +1295,8 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out) {
     out->frames = w->frames;
     out->sap_pairs = w->n_sap;
     out->pairs = w->n_pairs;
+    out->graph_frames = (uint32_t)w->graph_frames;
+    out->scene_kernels = w->built && w->scene_path() ? 1u : 0u;
     if (w->built) {
         HIPCHK(hipStreamSynchronize(w->st));
         uint32_t sc[CTR_SIZE];
@@ -1231,6 +1341,24 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_JOINT_MODE:
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "joint mode is 0 or 1");
             w->joint_mode = value; return RBGPU_OK;
+        case RBGPU_OPT_SCENE_KERNELS:
+            if (value < -1 || value > 1) return fail(RBGPU_ERR_INVALID, "scene kernels: -1 auto, 0 off, 1 on");
+            w->scene_mode = value; return RBGPU_OK;
+        case RBGPU_OPT_SCENE_THREADS:
+            if (value != 64 && value != 128 && value != 256) return fail(RBGPU_ERR_INVALID, "scene threads: 64, 128 or 256");
+            w->scene_threads = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_GJK_CULL:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "GJK cull is 0 or 1");
+            w->gjk_cull = value != 0; return RBGPU_OK;
+        case RBGPU_OPT_GJK_GRID:
+            if (value < 64 || value > (1 << 20)) return fail(RBGPU_ERR_INVALID, "GJK grid must be in [64, 2^20]");
+            w->gjk_grid = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_EPA_GRID:
+            if (value < 64 || value > (1 << 20)) return fail(RBGPU_ERR_INVALID, "EPA grid must be in [64, 2^20]");
+            w->epa_grid = (uint32_t)value; return RBGPU_OK;
+        case RBGPU_OPT_GRAPH:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "graph is 0 or 1");
+            w->use_graph = value != 0; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
@@ -1245,6 +1373,12 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_RUN_MIN: *value = (int)w->run_min; return RBGPU_OK;
         case RBGPU_OPT_TIMING_STRIDE: *value = (int)w->timing_stride; return RBGPU_OK;
         case RBGPU_OPT_JOINT_MODE: *value = w->joint_mode; return RBGPU_OK;
+        case RBGPU_OPT_SCENE_KERNELS: *value = w->scene_mode; return RBGPU_OK;
+        case RBGPU_OPT_SCENE_THREADS: *value = (int)w->scene_threads; return RBGPU_OK;
+        case RBGPU_OPT_GJK_CULL: *value = w->gjk_cull ? 1 : 0; return RBGPU_OK;
+        case RBGPU_OPT_GJK_GRID: *value = (int)w->gjk_grid; return RBGPU_OK;
+        case RBGPU_OPT_EPA_GRID: *value = (int)w->epa_grid; return RBGPU_OK;
+        case RBGPU_OPT_GRAPH: *value = w->use_graph ? 1 : 0; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }

@@ -44,7 +44,7 @@ class RbgpuError(RuntimeError):
 class Stats(C.Structure):
     _fields_ = [("frames", C.c_uint32), ("sap_pairs", C.c_uint32), ("pairs", C.c_uint32), ("colours", C.c_uint32),
                 ("colour_rounds", C.c_uint32), ("contacts", C.c_uint32), ("bins", C.c_uint32 * 16),
-                ("active_pairs", C.c_uint32)]
+                ("active_pairs", C.c_uint32), ("graph_frames", C.c_uint32), ("scene_kernels", C.c_uint32)]
 
 
 _lib = None
@@ -154,7 +154,8 @@ def hull_tables(verts, faces=None) -> dict:
 
 
 OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, "run_min": 4, "timing_stride": 5,
-           "joint_mode": 6}
+           "joint_mode": 6, "scene_kernels": 7, "scene_threads": 8, "gjk_cull": 9, "gjk_grid": 10, "epa_grid": 11,
+           "graph": 12}
 
 TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "broadphase": 6,
           "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9, "joints": 10, "gjk": 11}
@@ -400,7 +401,7 @@ class World:
         _check(lib().rbgpu_get_stats(self.h, C.byref(s)))
         return {"frames": s.frames, "sap_pairs": s.sap_pairs, "pairs": s.pairs, "colours": s.colours,
                 "colour_rounds": s.colour_rounds, "active_pairs": s.active_pairs, "contacts": s.contacts,
-                "bins": list(s.bins)}
+                "bins": list(s.bins), "graph_frames": s.graph_frames, "scene_kernels": s.scene_kernels}
 
     # ---------------------------------------------------------------- options
     def set_option(self, name: str, value: int):

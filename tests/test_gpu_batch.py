@@ -91,3 +91,52 @@ def test_gpu_combined_batch_matches_fp32_oracle_and_single_worlds():
         for k in ss:
             assert np.array_equal(sb[k][off:off + n], ss[k]), (p.name, k)
         off += n
+
+
+@pytest.mark.parametrize("threads", [64, 256])
+def test_gpu_scene_kernels_equal_grid_path(threads):
+    """Per-scene colouring and solve (one workgroup per scene, no host round trip) against
+    the grid-wide rounds and per-colour launches on the same batch: the same colour for
+    every pair, the same round and colour counts, bit-identical states, every frame."""
+    lib = S.hull_library()
+    parts = _parts() + [S.combined_scene(n, seed=80 + k, library=lib) for k, n in enumerate((64, 120))]
+    bt = S.batch(parts)
+    print("case scene kernels threads", threads, "scenes", len(parts), flush=True)
+    a, b = _gpu(bt), _gpu(bt)
+    a.set_option("scene_kernels", 0)
+    b.set_option("scene_kernels", 1)
+    b.set_option("scene_threads", threads)
+    for f in range(70):
+        a.step(1)
+        b.step(1)
+        sa, sb = a.state(), b.state()
+        for k in sa:
+            assert np.array_equal(sa[k], sb[k]), (f + 1, k)
+        pa, ca = a.pair_colours()
+        pb, cb = b.pair_colours()
+        assert np.array_equal(pa, pb) and np.array_equal(ca, cb), f + 1
+        ta, tb = a.stats(), b.stats()
+        assert (ta["colours"], ta["colour_rounds"], ta["active_pairs"]) == \
+               (tb["colours"], tb["colour_rounds"], tb["active_pairs"]), (f + 1, ta, tb)
+    assert tb["active_pairs"] > 0
+
+
+def test_gpu_substep_graph_equals_direct_launches():
+    """The per-scene substep captured once per frame in a hipGraph and replayed for every
+    substep against the same launches issued directly: bit-identical states each frame."""
+    lib = S.hull_library()
+    parts = [S.combined_scene(n, seed=90 + k, library=lib) for k, n in enumerate((64, 80, 100, 72))] * 5
+    bt = S.batch(parts)
+    print("case substep graph scenes", len(parts), flush=True)
+    a, b = _gpu(bt), _gpu(bt)
+    for w, g in ((a, 0), (b, 1)):
+        w.set_option("scene_kernels", 1)
+        w.set_option("graph", g)
+    for f in range(60):
+        a.step(1)
+        b.step(1)
+        sa, sb = a.state(), b.state()
+        for k in sa:
+            assert np.array_equal(sa[k], sb[k]), (f + 1, k)
+    assert a.stats()["graph_frames"] == 0 and b.stats()["graph_frames"] == 60
+    assert b.stats()["scene_kernels"] == 1
