@@ -1,19 +1,28 @@
 """Throughput of the XPBD substep pipeline on one MI355X: body-substeps/s.
 
-Workload: the mixed convex pile — 65,536 dynamic bodies (boxes, spheres, capsules,
-type = index mod 3) on a seeded 64x16x64 grid over a static plane, dt = 1/60 s,
-10 substeps per World::step, FP32 state.  A "step" is one World::step (one
-frame); --steps frames are timed after --warmup untimed frames, inputs resident
-in HBM.  value = dynamic bodies * substeps * steps / seconds.
+Default workload (--workload combined): 262,144 dynamic bodies in 1,024 independent
+scenes of 64-512 bodies (convex hulls from a shared library, boxes, 8-segment ragdolls on
+revolute + spherical joints with 60 deg limits) in one world, dt = 1/120 s, 8 substeps,
+FP32 state, XPBD joint rows.  The timed leg is the learned + batched pipeline: GJK/EPA
+warm-started by the network (trained on the GJK/EPA pairs of the warm-up frames), one
+workgroup per scene for colouring and solve, each substep replayed from a hipGraph, launch
+options picked by the autotuner during warm-up.  A "step" is one World::step (one frame);
+--steps frames are timed after the warm-up, inputs resident in HBM.
+value = dynamic bodies * substeps * steps / seconds.  Beside it ("plain"): the plain GPU
+pipeline (plain GJK/EPA, grid-wide colouring, per-colour launches, autotuned too) on a
+fresh world of the same scene over the same frames.
+
+Other workloads: pile (65,536 mixed convex bodies, the plain pipeline), batched (4,096
+box-stack + chain scenes), hulls (32,768 random hulls, GJK/EPA).
 
 Also reported:
   roofline     — the dominant kernel's algorithmic bytes per launch / its average
                  launch duration (HIP events on the world's stream around every
-                 launch of that kernel in every 5th frame of the timed region),
+                 launch of that kernel in every 10th frame of the timed region),
                  against 8 TB/s HBM.
   cpu_baseline — the reference engine (compiled from its sources into
-                 oracle/_ref/libref.so; FP64, serial, 1 core) or, without it, the
-                 oracle restatement, on frames 1..k of the same scene (about 10 s).
+                 oracle/_ref/libref.so; FP64, serial, 1 core) or, where it cannot run the
+                 scene (spherical joints), the oracle restatement, on a bounded sample.
 """
 import argparse
 import json
@@ -31,7 +40,7 @@ METRIC = "body-substeps/s on 1 MI355X (learned+batched vs plain), pos err \u2264
 WORKLOAD = "plain GPU pipeline, large pile: 65,536 mixed convex bodies, 10 substeps, 500 steps"
 COMBINED_WORKLOAD = ("combined at scale: 262,144 bodies in 1,024 uneven scenes (64-512 bodies) of convex hulls, "
                      "boxes and 8-segment ragdolls (revolute + spherical joints, limits +-60 deg), dt 1/120 s, "
-                     "8 substeps, XPBD joint mode, learned GJK/EPA warm start")
+                     "8 substeps, XPBD joint rows, capsule-capsule normals from A to B, learned GJK/EPA warm start")
 HULL_WORKLOAD = ("GJK/EPA-heavy: 32,768 random convex hulls (16-64 vertices on ellipsoids with semi-axes "
                  "U(0.2,0.8), triangulated) in a dense pile over a plane, 10 substeps, plain GJK/EPA")
 
@@ -78,6 +87,7 @@ def cpu_baseline_batched(budget_s=10.0, frames=10, combined=False):
         w, kind = _cpu_world(sc)
         if combined:
             w.set_joint_mode(1)
+            w.set_contact_mode(1)
         w.step(frames)
         work += sc.n_dynamic * sc.substeps * frames
         done += 1
@@ -86,7 +96,7 @@ def cpu_baseline_batched(budget_s=10.0, frames=10, combined=False):
             "sample": f"scenes 0..{done - 1} of the batch, frames 1..{frames} each, one World per scene "
                       f"({dt:.1f} s incl. set-up, FP64, serial; the oracle restatement in the "
                       f"reference's order because the reference has no spherical joint"
-                      f"{'; XPBD joint mode as on the GPU' if combined else ''})"}
+                      f"{'; XPBD joint rows and capsule-capsule normals from A to B as on the GPU' if combined else ''})"}
 
 
 def learned_report(w, sc, train_frames=20, capacity=4_000_000, steps=1200, batch=8192):
@@ -149,20 +159,31 @@ def run_frames(w, steps, label=""):
     return time.perf_counter() - t0
 
 
+def tune_and_align(w, start, frames_per_trial=2, settle=40):
+    """Autotune (result-neutral options, real frames), then step up to frame start + settle
+    so that differently tuned legs time the same frames."""
+    rep = w.autotune(frames_per_trial)
+    if rep["frames"] < settle:
+        w.step(settle - rep["frames"])
+    return rep
+
+
 def plain_leg(world, sc, args, pos_learned, value):
     """The plain GPU pipeline on a fresh world of the same scene: plain GJK/EPA, grid-wide
     colouring and per-colour solve launches, the same warm-up and timed frames."""
     w = world.World(sc.timestep, sc.substeps).load(sc)
     w.set_option("joint_mode", 1)
+    w.set_option("contact_mode", 1)
     w.set_option("scene_kernels", 0)   # grid-wide colouring rounds and per-colour launches
     w.step(args.warmup)
+    rep = tune_and_align(w, args.warmup)
     dt = run_frames(w, args.steps, "plain")
     plain = sc.n_dynamic * sc.substeps * args.steps / dt
     d = np.abs(w.state()["position"] - pos_learned).max(axis=1)
     return {"value": round(plain, 1), "unit": "body-substeps/s", "ms_per_step": round(dt / args.steps * 1e3, 4),
-            "speedup": round(value / plain, 4),
+            "speedup": round(value / plain, 4), "tuned": rep,
             "pos_diff_vs_learned_m": {"max": float(d.max()), "p99": float(np.quantile(d, 0.99)),
-                                      "median": float(np.median(d)), "frame": args.warmup + args.steps}}
+                                      "median": float(np.median(d)), "frame": w.stats()["frames"]}}
 
 
 def measured_traffic(kernel):
@@ -189,12 +210,14 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed frames (default: combined 1000, pile and hulls 500, batched 100)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed frames before (default: combined 60 + tuning, others 10)")
     ap.add_argument("--timing-kernel", default=None,
                     choices=["off", "solve_pos", "solve_vel", "narrowphase", "integrate", "update_vel", "broadphase", "colouring",
                              "solve_pos_runs", "solve_vel_runs", "joints", "gjk"],
-                    help="kernel family timed for the roofline (default: solve_pos; gjk for --workload hulls)")
+                    help="kernel family timed for the roofline (default: gjk for hulls and combined, else solve_pos)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="hulls: skip the learned warm-start report; combined: time the plain GJK/EPA")
@@ -202,11 +225,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every substep directly instead of replaying a captured hipGraph "
                          "(kernel-trace profiles: rocprofv3's kernel trace does not survive graph replays here)")
-    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "hulls", "combined"],
-                    help="pile: the 65,536-body mixed pile (default); batched: 4,096 independent "
-                         "box-stack + capsule-chain scenes (1.1M bodies) in one world; hulls: 32,768 "
-                         "random convex hulls in a dense pile (GJK/EPA); combined: 1,024 scenes of hulls, "
-                         "boxes and ragdolls (262,144 bodies), learned warm start, with a plain leg beside it")
+    ap.add_argument("--workload", default="combined", choices=["pile", "batched", "hulls", "combined"],
+                    help="combined (default): 1,024 scenes of hulls, boxes and ragdolls (262,144 bodies), "
+                         "learned + batched, with the plain pipeline beside it; pile: the 65,536-body mixed "
+                         "pile; batched: 4,096 independent box-stack + capsule-chain scenes (1.1M bodies) in "
+                         "one world; hulls: 32,768 random convex hulls in a dense pile (GJK/EPA)")
+    ap.add_argument("--no-tune", action="store_true", help="combined: skip the autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB ten times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a --pmc run to bytes)")
@@ -221,7 +245,11 @@ def main():
 
     from rbps_amd import scenes, world
     if args.timing_kernel is None:
-        args.timing_kernel = "gjk" if args.workload == "hulls" else "solve_pos"
+        args.timing_kernel = "gjk" if args.workload in ("hulls", "combined") else "solve_pos"
+    if args.steps is None:
+        args.steps = {"combined": 1000, "batched": 100}.get(args.workload, 500)
+    if args.warmup is None:
+        args.warmup = 60 if args.workload == "combined" else 10
     if args.workload == "hulls":
         scene_fn = lambda: scenes.hull_pile(32768, seed=4)   # noqa: E731
         workload = HULL_WORKLOAD
@@ -239,6 +267,8 @@ def main():
     w = world.World(sc.timestep, sc.substeps).load(sc)
     if args.workload in ("batched", "combined"):
         w.set_option("joint_mode", 1)
+    if args.workload == "combined":   # capsule-capsule normals from A to B (ragdoll limbs)
+        w.set_option("contact_mode", 1)
     if args.no_graph:
         w.set_option("graph", 0)
     learned = args.workload == "combined" and not args.no_learned
@@ -256,8 +286,14 @@ def main():
         w.synchronize()
         train = {"logged_samples": logged, "train_s": round(time.perf_counter() - t1, 3), "train_loss": round(loss, 4)}
         w.mlp_mode("gjk")
-    # per-launch HIP events on every 5th frame: events between launches cost time themselves
-    w.set_option("timing_stride", 5)
+    tuned = None
+    if args.workload == "combined" and not args.no_tune:
+        tuned = tune_and_align(w, args.warmup)
+        if args.no_graph:
+            w.set_option("graph", 0)
+    # per-launch HIP events on every 10th frame (those frames run without the substep graph):
+    # events between launches cost time themselves
+    w.set_option("timing_stride", 10)
     w.set_timing(args.timing_kernel)
     dt = run_frames(w, args.steps, "learned" if learned else "")
     tm = w.timing()
@@ -285,6 +321,9 @@ def main():
                    "last_frame": st},
         "roofline": roof if args.workload == "pile" else dict(roof or {}, traffic=None),
     }
+    if roof and args.timing_kernel == "gjk":
+        roof["note"] = ("GJK/EPA stage: rejection test + compaction, features and inference, GJK pass, EPA "
+                        "pass; algorithmic bytes = pairs of the hull bins x 160 B + GJK candidates x pair bytes")
     if args.workload == "hulls" and not args.no_learned:
         out["learned"] = learned_report(w, sc)
     if args.workload == "combined":
@@ -292,6 +331,7 @@ def main():
         out["config"]["scenes"] = int(sc.meta["scenes"])
         out["config"]["joints"] = int(len(sc.joints))
         out["learned"] = train
+        out["tuned"] = tuned
         if not args.no_plain:
             out["plain"] = plain_leg(world, sc, args, w.state()["position"], value)
     if not args.no_cpu_baseline:

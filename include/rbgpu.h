@@ -215,7 +215,13 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *   RBGPU_OPT_GJK_GRID, RBGPU_OPT_EPA_GRID  workgroups of the two GJK/EPA passes (4096, 8192)
  *   RBGPU_OPT_GRAPH        with the per-scene kernels: capture one substep per frame in a
  *                          hipGraph and launch it for every substep (1, default; frames with
- *                          kernel timing or an open training log run without it) */
+ *                          kernel timing or an open training log run without it)
+ *   RBGPU_OPT_CONTACT_MODE  0 (default): contacts exactly as the reference makes them.  The
+ *                          reference's capsule-capsule routine returns a normal from B to A
+ *                          (CapsuleCapsule.hpp:68) where every other routine returns one from
+ *                          A to B, so overlapping capsules are pushed into each other and
+ *                          capsule piles gain energy without bound.  1: that normal is turned
+ *                          around (points and depths unchanged). */
 enum {
     RBGPU_OPT_PALETTE_MAX = 0,
     RBGPU_OPT_HEAD_ROUNDS = 1,
@@ -229,10 +235,25 @@ enum {
     RBGPU_OPT_GJK_CULL = 9,
     RBGPU_OPT_GJK_GRID = 10,
     RBGPU_OPT_EPA_GRID = 11,
-    RBGPU_OPT_GRAPH = 12
+    RBGPU_OPT_GRAPH = 12,
+    RBGPU_OPT_CONTACT_MODE = 13
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);
+
+/* Autotuner (no counterpart in the reference).  Picks this world's launch options that
+ * cannot change any result (per-scene solve workgroup size and the substep graph, or the
+ * grid path's run minimum and round batch; the GJK and EPA grids when the world has hulls)
+ * by coordinate descent: each candidate value is timed over `frames_per_trial` real frames
+ * (wall clock between two stream synchronisations) and the fastest is kept.  The trial
+ * frames advance the world exactly as untuned frames would, since every candidate gives
+ * bit-identical states; callers typically tune during warm-up.  Requires timing off. */
+typedef struct {
+    int32_t scene_kernels, scene_threads, graph, run_min, round_batch, gjk_grid, epa_grid;
+    int32_t trials, frames;      /* candidates timed, frames stepped (trials + one settling frame) */
+    double best_ms_per_frame;    /* of the last option tuned */
+} rbgpu_tune_report;
+int rbgpu_autotune(rbgpu_world *w, int frames_per_trial, rbgpu_tune_report *out);
 
 int rbgpu_set_timing(rbgpu_world *w, int kernel);
 int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, double *bytes);

@@ -133,6 +133,7 @@ def oracle_lib():
         lib.orc_colour_rounds.argtypes = [_P]
         lib.orc_set_palette_max.argtypes = [_P, C.c_int]
         lib.orc_set_joint_mode.argtypes = [_P, C.c_int]
+        lib.orc_set_contact_mode.argtypes = [_P, C.c_int]
         lib.orc_joint_errors.argtypes = [C.c_int, _P, _D, _D, C.c_double, _D]
         for f in ("orc_sinf", "orc_cosf"):
             getattr(lib, f).restype = C.c_float
@@ -258,6 +259,10 @@ class OracleWorld(_WorldBase):
     def set_joint_mode(self, mode):
         """0: the reference's joint solver; 1: XPBD rows (see orc_world.hpp)."""
         self.lib.orc_set_joint_mode(self.h, int(mode))
+
+    def set_contact_mode(self, mode):
+        """0: contacts as the reference makes them; 1: capsule-capsule normal from A to B."""
+        self.lib.orc_set_contact_mode(self.h, int(mode))
 
     def set_palette_max(self, m):
         if self.lib.orc_set_palette_max(self.h, m) != 0:

@@ -88,6 +88,7 @@ struct Any {
     virtual int colour_rounds() const = 0;
     virtual void set_palette_max(uint32_t m) = 0;
     virtual void set_joint_mode(int m) = 0;
+    virtual void set_contact_mode(int m) = 0;
     virtual void set_state(const double *p, const double *q, const double *v, const double *w) = 0;
     virtual void add_joint(const orc_joint_desc &d) = 0;
     virtual int njoints() const = 0;
@@ -182,6 +183,7 @@ template <class T> struct Impl : Any {
     int colour_rounds() const override { return (int)w.colour_rounds; }
     void set_palette_max(uint32_t m) override { w.palette_max = m; }
     void set_joint_mode(int m) override { w.joint_mode = m; }
+    void set_contact_mode(int m) override { w.contact_mode = m; }
     void add_joint(const orc_joint_desc &d) override { w.add_joint(joint_from<T>(d), T(d.compliance)); }
     int njoints() const override { return (int)w.joints.size(); }
     void joint_state(double *cur, double *tgt, double *lam4, int32_t *colour) const override {
@@ -357,6 +359,7 @@ void orc_set_joint_targets(void *h, int n, const uint32_t *ids, const double *tg
     static_cast<Any *>(h)->set_joint_targets(n, ids, tgt, speed);
 }
 void orc_set_joint_mode(void *h, int m) { static_cast<Any *>(h)->set_joint_mode(m); }
+void orc_set_contact_mode(void *h, int m) { static_cast<Any *>(h)->set_contact_mode(m); }
 void orc_joint_errors(int fp32, const orc_joint_desc *d, const double *pose1, const double *pose2, double h, double *out) {
     if (fp32) joint_errors_t<float>(d, pose1, pose2, h, out);
     else joint_errors_t<double>(d, pose1, pose2, h, out);

@@ -117,6 +117,14 @@ template <class T> struct World {
     //   before the row is solved, lambda reset every substep, direction v/|v| whenever
     //   |v| > EPS, revolute alignment a2 x a1.  Equal to mode 0 in everything else.
     int joint_mode = 0;
+    // 1: turn the capsule-capsule normal around (the reference's routine returns it from B to
+    // A, CapsuleCapsule.hpp:68, every other routine from A to B)
+    int contact_mode = 0;
+    bool narrow(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, const TF<T> &tb, Manifold<T> &m) const {
+        if (!dispatch(a, ta, b, tb, m)) return false;
+        if (contact_mode == 1 && a.kind == SH_CAPSULE && b.kind == SH_CAPSULE) m.normal = -m.normal;
+        return true;
+    }
 
     // ------------------------------------------------------------ creation
     // BodyAPI.hpp:53-80
@@ -314,7 +322,7 @@ template <class T> struct World {
             if (ca >= shape.size() || cb >= shape.size()) continue;
             if (!pair_ok(ca, cb)) continue;
             Manifold<T> m;
-            if (dispatch(shape[ca], collider_tf(ca), shape[cb], collider_tf(cb), m)) {
+            if (narrow(shape[ca], collider_tf(ca), shape[cb], collider_tf(cb), m)) {
                 make_contacts(m, cbody[ca], cbody[cb], ca, cb, buf);
                 for (int i = 0; i < m.n; ++i) contacts.push_back(buf[i]);
             }
@@ -325,7 +333,7 @@ template <class T> struct World {
                 if (di == pi || shape[di].kind == SH_PLANE) continue;
                 if (!pair_ok(pi, di)) continue;
                 Manifold<T> m;
-                if (dispatch(shape[di], collider_tf(di), shape[pi], collider_tf(pi), m)) {
+                if (narrow(shape[di], collider_tf(di), shape[pi], collider_tf(pi), m)) {
                     make_contacts(m, cbody[di], cbody[pi], di, pi, buf);
                     for (int i = 0; i < m.n; ++i) contacts.push_back(buf[i]);
                 }
@@ -461,7 +469,7 @@ template <class T> struct World {
             const SPair &s = spairs[p];
             Manifold<T> m;
             pcount[p] = 0;
-            if (dispatch(shape[s.ca], collider_tf(s.ca), shape[s.cb], collider_tf(s.cb), m)) {
+            if (narrow(shape[s.ca], collider_tf(s.ca), shape[s.cb], collider_tf(s.cb), m)) {
                 make_contacts(m, s.ba, s.bb, s.ca, s.cb, &pcont[(size_t)p * 4]);
                 pcount[p] = m.n;
             }

@@ -633,9 +633,13 @@ __device__ __forceinline__ bool far_apart(const Shape &a, const TF &ta, const Sh
     return len2(ta.pos - tb.pos) > r * r;
 }
 
+// flip_cc (contact mode 1, capsule-capsule bin only): the reference's capsule-capsule routine
+// returns a normal from B to A (CapsuleCapsule.hpp:68, delta = closest_a - closest_b) where
+// every other routine returns one from A to B, so the solver pushes such a pair into each
+// other; mode 1 turns that normal around (points and depths unchanged).
 template <int BIN>
 __global__ void __launch_bounds__(128) k_narrow(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B,
-                                                Contacts K) {
+                                                Contacts K, int flip_cc = 0) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     uint32_t s = bin_slot[off + t];
@@ -650,8 +654,12 @@ __global__ void __launch_bounds__(128) k_narrow(const uint32_t *bin_slot, uint32
     }
     Manifold m;
     m.np = 0;
-    if (collide_bin<BIN>(a, ta, b, tb, m) && m.np > 0) emit(m, B, K, s, bb);
-    else K.cnt[s] = 0;
+    if (collide_bin<BIN>(a, ta, b, tb, m) && m.np > 0) {
+        if (BIN == BIN_CC && flip_cc) m.n = -m.n;
+        emit(m, B, K, s, bb);
+    } else {
+        K.cnt[s] = 0;
+    }
 }
 
 // Box-box in two launches: the separating-axis test for every pair of the bin (registers

@@ -9,6 +9,7 @@
 //   large colour as its own launch and a run of small colours in one workgroup.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -123,6 +124,7 @@ struct rbgpu_world {
     int hull_vpl = 1;                   // hull vertices per lane in the wave kernels (1: all hulls <= 64)
     uint32_t *gstat = nullptr;
     uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
+    int contact_mode = 0;                               // 1: capsule-capsule normal from A to B
     bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
     uint32_t *gjk_hits = nullptr, *gjk_blk = nullptr;   // overlapping GJK pairs per 4-pair workgroup
@@ -395,7 +397,7 @@ struct rbgpu_world {
                     k_narrow_bb_full<<<g, 128, 0, st>>>(bb_hits, bb_blk, C, B, K);
                     break;
                 case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K, contact_mode); break;
                 case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                 case BIN_SP: k_narrow<BIN_SP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
                 case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
@@ -1163,6 +1165,78 @@ int rbgpu_step(rbgpu_world *w, int n_frames) {
     return RBGPU_OK;
 }
 
+int rbgpu_autotune(rbgpu_world *w, int frames_per_trial, rbgpu_tune_report *out) {
+    if (!w || frames_per_trial < 1) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if (!w->built) {
+        int rc = w->build();
+        if (rc) return rc;
+    }
+    if (w->timing_kernel) return fail(RBGPU_ERR_INVALID, "autotune with kernel timing on (rbgpu_set_timing 0 first)");
+    // wall time of k frames between two synchronisations of the world's stream
+    auto time_frames = [&](int k, double *ms) -> int {
+        HIPCHK(hipStreamSynchronize(w->st));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int f = 0; f < k; ++f) {
+            int rc = w->frame();
+            if (rc) return rc;
+        }
+        HIPCHK(hipStreamSynchronize(w->st));
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return RBGPU_OK;
+    };
+    struct Knob {
+        int opt;
+        std::vector<int> vals;
+    };
+    std::vector<Knob> knobs;
+    if (w->scene_path()) {
+        knobs.push_back({RBGPU_OPT_SCENE_THREADS, {64, 128, 256}});
+        knobs.push_back({RBGPU_OPT_GRAPH, {1, 0}});
+    } else {
+        knobs.push_back({RBGPU_OPT_RUN_MIN, {128, 256, 512, 1024}});
+        knobs.push_back({RBGPU_OPT_ROUND_BATCH, {6, 12, 24}});
+    }
+    if (!w->hh.empty()) {
+        knobs.push_back({RBGPU_OPT_GJK_GRID, {1024, 2048, 4096, 8192, 16384}});
+        knobs.push_back({RBGPU_OPT_EPA_GRID, {2048, 8192, 32768}});
+    }
+    rbgpu_tune_report rep{};
+    double ms = 0.0;
+    int rc = time_frames(1, &ms);   // settles allocations and the first capture
+    if (rc) return rc;
+    rep.frames += 1;
+    double best_all = 0.0;
+    for (const Knob &k : knobs) {
+        int cur = 0;
+        rc = rbgpu_get_option(w, k.opt, &cur);
+        if (rc) return rc;
+        int best = cur;
+        double best_ms = 1e300;
+        for (int v : k.vals) {
+            rc = rbgpu_set_option(w, k.opt, v);
+            if (rc) return rc;
+            rc = time_frames(frames_per_trial, &ms);
+            if (rc) return rc;
+            rep.frames += frames_per_trial;
+            ++rep.trials;
+            if (ms < best_ms) { best_ms = ms; best = v; }
+        }
+        rc = rbgpu_set_option(w, k.opt, best);
+        if (rc) return rc;
+        best_all = best_ms / frames_per_trial;
+    }
+    rbgpu_get_option(w, RBGPU_OPT_SCENE_THREADS, &rep.scene_threads);
+    rbgpu_get_option(w, RBGPU_OPT_GRAPH, &rep.graph);
+    rbgpu_get_option(w, RBGPU_OPT_RUN_MIN, &rep.run_min);
+    rbgpu_get_option(w, RBGPU_OPT_ROUND_BATCH, &rep.round_batch);
+    rbgpu_get_option(w, RBGPU_OPT_GJK_GRID, &rep.gjk_grid);
+    rbgpu_get_option(w, RBGPU_OPT_EPA_GRID, &rep.epa_grid);
+    rep.scene_kernels = w->scene_path() ? 1 : 0;
+    rep.best_ms_per_frame = best_all;
+    if (out) *out = rep;
+    return RBGPU_OK;
+}
+
 int rbgpu_set_step(rbgpu_world *w, double timestep, int substeps) {
     if (!w || substeps <= 0 || !(timestep > 0.0)) return fail(RBGPU_ERR_INVALID, "invalid timestep/substeps");
     w->timestep = timestep;
@@ -1359,6 +1433,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_GRAPH:
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "graph is 0 or 1");
             w->use_graph = value != 0; return RBGPU_OK;
+        case RBGPU_OPT_CONTACT_MODE:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "contact mode is 0 or 1");
+            w->contact_mode = value; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
@@ -1379,6 +1456,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_GJK_GRID: *value = (int)w->gjk_grid; return RBGPU_OK;
         case RBGPU_OPT_EPA_GRID: *value = (int)w->epa_grid; return RBGPU_OK;
         case RBGPU_OPT_GRAPH: *value = w->use_graph ? 1 : 0; return RBGPU_OK;
+        case RBGPU_OPT_CONTACT_MODE: *value = w->contact_mode; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }

@@ -639,42 +639,55 @@ RAGDOLL_LIMIT = math.radians(60.0)
 
 def ragdoll(sb, origin, yaw=0.0, density=1000.0, compliance=1e-6, friction=0.5) -> list:
     """An 8-segment ragdoll standing at `origin` (pelvis centre), turned by `yaw` about y:
-    pelvis, abdomen and chest boxes, a sphere head, capsule arms and legs.  Joints:
-    pelvis-abdomen, abdomen-chest and the hips are revolute about the body's x axis with
-    limits of +-60 deg; neck and shoulders are spherical with a 60 deg swing limit.  Joint
-    points sit in gaps between the segments, so a straight ragdoll has no self-contacts."""
+    pelvis, abdomen and chest boxes, a sphere head, capsule arms held out sideways and
+    capsule legs.  Joints: pelvis-abdomen, abdomen-chest and the hips are revolute about the
+    body's x axis with limits of +-60 deg; neck and shoulders are spherical with a 60 deg
+    swing limit.  Segments are spaced so that no two of one ragdoll touch anywhere inside
+    the limits (stacked boxes of half-depth d stay clear while the gap exceeds 2 d tan 30
+    deg): the reference does not filter contacts between jointed bodies, and a contact
+    fighting a joint limit pumps energy into the pair."""
     cy, sy = math.cos(0.5 * yaw), math.sin(0.5 * yaw)
     q = (cy, 0.0, sy, 0.0)
     ox, oy, oz = origin
 
-    def at(x, y, z):   # body-frame offset (ragdoll frame) -> world
+    def at(x, y, z):   # ragdoll-frame offset -> world
         c, s = math.cos(yaw), math.sin(yaw)
         return (ox + c * x + s * z, oy + y, oz - s * x + c * z)
 
     pelvis = _box(sb, at(0.0, 0.0, 0.0), (0.15, 0.08, 0.1), density, quat=q, friction=friction)
-    belly = _box(sb, at(0.0, 0.2, 0.0), (0.13, 0.08, 0.09), density, quat=q, friction=friction)
-    chest = _box(sb, at(0.0, 0.42, 0.0), (0.17, 0.1, 0.1), density, quat=q, friction=friction)
+    belly = _box(sb, at(0.0, 0.30, 0.0), (0.13, 0.08, 0.09), density, quat=q, friction=friction)
+    chest = _box(sb, at(0.0, 0.62, 0.0), (0.17, 0.1, 0.1), density, quat=q, friction=friction)
     rh = 0.1
-    head = sb.body(DYNAMIC, density * sphere_volume(rh), tuple(density * x for x in sphere_inertia(rh)), at(0.0, 0.68, 0.0), q)
+    head = sb.body(DYNAMIC, density * sphere_volume(rh), tuple(density * x for x in sphere_inertia(rh)), at(0.0, 0.92, 0.0), q)
     sb.collider(SPHERE, head, (rh,), restitution=0.0, static_friction=friction, dynamic_friction=friction)
-    arms = [_capsule(sb, at(sx * 0.27, 0.32, 0.0), 0.18, 0.05, density, quat=q, friction=friction) for sx in (-1.0, 1.0)]
+    # arms: bodies aligned with the chest, capsule colliders turned from y onto x
+    ahh, ar = 0.18, 0.05
+    ixx, iyy, _, _, _, _ = capsule_inertia(ahh, ar)
+    along_x = (math.cos(math.pi / 4), 0.0, 0.0, -math.sin(math.pi / 4))
+    arms = []
+    for sx in (-1.0, 1.0):
+        a = sb.body(DYNAMIC, density * capsule_volume(ahh, ar), (density * iyy, density * ixx, density * ixx, 0.0, 0.0, 0.0),
+                    at(sx * 0.48, 0.66, 0.0), q)
+        sb.collider(CAPSULE, a, (ahh, ar), local_rot=along_x, restitution=0.0, static_friction=friction,
+                    dynamic_friction=friction)
+        arms.append(a)
     legs = [_capsule(sb, at(sx * 0.09, -0.42, 0.0), 0.22, 0.06, density, quat=q, friction=friction) for sx in (-1.0, 1.0)]
     lim = RAGDOLL_LIMIT
     hinge = dict(axis=(1, 0, 0), limit_axis=(0, 1, 0), compliance=compliance, limited=True, lower=-lim, upper=lim)
-    sb.joint(REVOLUTE, pelvis, belly, r_1=(0.0, 0.1, 0.0), r_2=(0.0, -0.1, 0.0), **hinge)
-    sb.joint(REVOLUTE, belly, chest, r_1=(0.0, 0.1, 0.0), r_2=(0.0, -0.12, 0.0), **hinge)
-    sb.joint(SPHERICAL, chest, head, r_1=(0.0, 0.13, 0.0), r_2=(0.0, -0.13, 0.0), axis=(0, 1, 0),
+    sb.joint(REVOLUTE, pelvis, belly, r_1=(0.0, 0.15, 0.0), r_2=(0.0, -0.15, 0.0), **hinge)
+    sb.joint(REVOLUTE, belly, chest, r_1=(0.0, 0.15, 0.0), r_2=(0.0, -0.17, 0.0), **hinge)
+    sb.joint(SPHERICAL, chest, head, r_1=(0.0, 0.13, 0.0), r_2=(0.0, -0.17, 0.0), axis=(0, 1, 0),
              compliance=compliance, limited=True, upper=lim)
     for sx, a in zip((-1.0, 1.0), arms):
-        sb.joint(SPHERICAL, chest, a, r_1=(sx * 0.27, 0.16, 0.0), r_2=(0.0, 0.26, 0.0), axis=(0, 1, 0),
+        sb.joint(SPHERICAL, chest, a, r_1=(sx * 0.20, 0.04, 0.0), r_2=(-sx * 0.28, 0.0, 0.0), axis=(sx, 0, 0),
                  compliance=compliance, limited=True, upper=lim)
     for sx, l in zip((-1.0, 1.0), legs):
         sb.joint(REVOLUTE, pelvis, l, r_1=(sx * 0.09, -0.11, 0.0), r_2=(0.0, 0.31, 0.0), **hinge)
     return [pelvis, belly, chest, head] + arms + legs
 
 
-def combined_scene(n_bodies=256, seed=0, library=None, ragdoll_share=0.375, spacing=1.35, y0=2.6, density=1000.0,
-                   frames=1000) -> Scene:
+def combined_scene(n_bodies=256, seed=0, library=None, ragdoll_share=0.375, spacing=1.35, y0=2.8, density=1000.0,
+                   frames=1000, ragdoll_spacing=1.6) -> Scene:
     """One scene of the combined workload: a ground plane, ragdolls standing on a grid
     (about `ragdoll_share` of the bodies, 8 per ragdoll), and above them a pile of convex
     hulls (from the shared `library`) and boxes (half-extents U(0.15, 0.35)), half each,
@@ -688,8 +701,8 @@ def combined_scene(n_bodies=256, seed=0, library=None, ragdoll_share=0.375, spac
     n_rest = n_bodies - 8 * n_rag
     side = max(1, int(math.ceil(math.sqrt(max(n_rag, 1)))))
     for k in range(n_rag):
-        x = (k % side - (side - 1) / 2) * 1.0 + rng.uniform(-0.05, 0.05)
-        z = (k // side - (side - 1) / 2) * 1.0 + rng.uniform(-0.05, 0.05)
+        x = (k % side - (side - 1) / 2) * ragdoll_spacing + rng.uniform(-0.05, 0.05)
+        z = (k // side - (side - 1) / 2) * ragdoll_spacing + rng.uniform(-0.05, 0.05)
         ragdoll(sb, (x, 0.95, z), yaw=float(rng.uniform(-math.pi, math.pi)), density=density)
     cols = max(side, int(math.ceil(math.sqrt(n_rest / 6.0))))
     for k in range(n_rest):

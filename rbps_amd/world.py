@@ -29,6 +29,7 @@ EXPORTS = [
     "rbgpu_world_create", "rbgpu_world_destroy", "rbgpu_add_bodies", "rbgpu_add_colliders", "rbgpu_step",
     "rbgpu_set_step", "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
     "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_collide_pairs",
+    "rbgpu_autotune",
     "rbgpu_set_option", "rbgpu_get_option", "rbgpu_add_joints", "rbgpu_get_joints", "rbgpu_set_joint_targets",
     "rbgpu_add_hull", "rbgpu_hull_mass_properties", "rbgpu_hull_tables", "rbgpu_collide_world_pairs",
     "rbgpu_mlp_init", "rbgpu_mlp_set_weights", "rbgpu_mlp_get_weights", "rbgpu_mlp_set_mode", "rbgpu_mlp_log",
@@ -39,6 +40,12 @@ EXPORTS = [
 
 class RbgpuError(RuntimeError):
     pass
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("scene_kernels", C.c_int32), ("scene_threads", C.c_int32), ("graph", C.c_int32), ("run_min", C.c_int32),
+                ("round_batch", C.c_int32), ("gjk_grid", C.c_int32), ("epa_grid", C.c_int32), ("trials", C.c_int32),
+                ("frames", C.c_int32), ("best_ms_per_frame", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -75,6 +82,7 @@ def lib():
         L.rbgpu_get_joints.argtypes = [_P, _D, _D, _D, _I]
         L.rbgpu_set_joint_targets.argtypes = [_P, C.c_int, _U, _D, _D]
         L.rbgpu_set_option.argtypes = [_P, C.c_int, C.c_int]
+        L.rbgpu_autotune.argtypes = [_P, C.c_int, C.POINTER(TuneReport)]
         L.rbgpu_get_option.argtypes = [_P, C.c_int, _I]
         L.rbgpu_get_timing.argtypes = [_P, _D, C.POINTER(C.c_uint64), _D]
         L.rbgpu_collide_pairs.argtypes = [C.c_int, _I, _D, _D, _I, _D, _D, _D]
@@ -155,7 +163,7 @@ def hull_tables(verts, faces=None) -> dict:
 
 OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, "run_min": 4, "timing_stride": 5,
            "joint_mode": 6, "scene_kernels": 7, "scene_threads": 8, "gjk_cull": 9, "gjk_grid": 10, "epa_grid": 11,
-           "graph": 12}
+           "graph": 12, "contact_mode": 13}
 
 TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "broadphase": 6,
           "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9, "joints": 10, "gjk": 11}
@@ -414,6 +422,13 @@ class World:
         return v.value
 
     # ---------------------------------------------------------------- timing
+    def autotune(self, frames_per_trial: int = 2) -> dict:
+        """Times the result-neutral launch options over real frames and keeps the fastest
+        (rbgpu_autotune); the world advances as it would untuned."""
+        r = TuneReport()
+        _check(lib().rbgpu_autotune(self.h, int(frames_per_trial), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneReport._fields_}
+
     def set_timing(self, kernel: str):
         _check(lib().rbgpu_set_timing(self.h, TIMING[kernel]))
 

@@ -12,10 +12,11 @@ from tests.helpers import state_close
 pytestmark = pytest.mark.gpu
 
 
-def _gpu(sc, mode=1):
+def _gpu(sc, mode=1, contact_mode=0):
     from rbps_amd import world
     w = world.World(sc.timestep, sc.substeps).load(sc)
     w.set_option("joint_mode", mode)
+    w.set_option("contact_mode", contact_mode)
     return w
 
 
@@ -64,16 +65,18 @@ def test_gpu_full_batch_runs_without_cross_scene_pairs():
 def test_gpu_combined_batch_matches_fp32_oracle_and_single_worlds():
     """Hulls from a shared library, boxes and ragdolls (revolute + spherical joints with
     60 deg limits) in three uneven scenes; 80 frames at dt = 1/120 s, 8 substeps, so the
-    pile has landed on the ragdolls (hull-box, hull-capsule, hull-sphere contacts)."""
+    pile has landed on the ragdolls (hull-box, hull-capsule, hull-sphere contacts), in the
+    contact mode that orients the capsule-capsule normal like the other pairs."""
     lib = S.hull_library()
     parts = [S.combined_scene(n, seed=40 + k, library=lib) for k, n in enumerate((64, 90, 130))]
     bt = S.batch(parts)
     assert len(bt.hulls) == len(lib)
     print("case combined batch bodies", len(bt.bodies), "joints", len(bt.joints), flush=True)
-    g = _gpu(bt)
+    g = _gpu(bt, contact_mode=1)
     o = orc.OracleWorld(bt.timestep, bt.substeps, fp32=True, order=orc.ORDER_COLOUR).load(bt)
     o.set_joint_mode(1)
-    singles = [_gpu(p) for p in parts]
+    o.set_contact_mode(1)
+    singles = [_gpu(p, contact_mode=1) for p in parts]
     hull_pairs = 0
     for f in range(80):
         g.step(1)
@@ -102,7 +105,7 @@ def test_gpu_scene_kernels_equal_grid_path(threads):
     parts = _parts() + [S.combined_scene(n, seed=80 + k, library=lib) for k, n in enumerate((64, 120))]
     bt = S.batch(parts)
     print("case scene kernels threads", threads, "scenes", len(parts), flush=True)
-    a, b = _gpu(bt), _gpu(bt)
+    a, b = _gpu(bt, contact_mode=1), _gpu(bt, contact_mode=1)
     a.set_option("scene_kernels", 0)
     b.set_option("scene_kernels", 1)
     b.set_option("scene_threads", threads)
@@ -140,3 +143,23 @@ def test_gpu_substep_graph_equals_direct_launches():
             assert np.array_equal(sa[k], sb[k]), (f + 1, k)
     assert a.stats()["graph_frames"] == 0 and b.stats()["graph_frames"] == 60
     assert b.stats()["scene_kernels"] == 1
+
+
+def test_gpu_autotune_keeps_the_trajectory():
+    """The autotuner times result-neutral options over real frames: the tuned world's
+    states equal an untuned world's after the same number of frames, and the chosen values
+    are among the candidates."""
+    lib = S.hull_library()
+    parts = [S.combined_scene(n, seed=95 + k, library=lib) for k, n in enumerate((64, 96, 128, 70))] * 5
+    bt = S.batch(parts)
+    print("case autotune scenes", len(parts), flush=True)
+    a, b = _gpu(bt), _gpu(bt)
+    a.step(20)
+    b.step(20)
+    rep = b.autotune(1)
+    assert rep["scene_kernels"] == 1 and rep["scene_threads"] in (64, 128, 256) and rep["graph"] in (0, 1)
+    assert rep["gjk_grid"] in (1024, 2048, 4096, 8192, 16384) and rep["epa_grid"] in (2048, 8192, 32768)
+    a.step(rep["frames"])
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k

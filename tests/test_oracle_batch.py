@@ -52,10 +52,12 @@ def test_combined_scenes_generator_and_batch_equal_single_worlds():
         assert (p.joints["limited"] == 1).all()
     ob = orc.OracleWorld(bt.timestep, bt.substeps, fp32=True, order=orc.ORDER_COLOUR).load(bt)
     ob.set_joint_mode(1)
+    ob.set_contact_mode(1)
     singles = []
     for p in parts:
         o = orc.OracleWorld(bt.timestep, bt.substeps, fp32=True, order=orc.ORDER_COLOUR).load(p)
         o.set_joint_mode(1)
+        o.set_contact_mode(1)
         singles.append(o)
     for f in range(70):
         ob.step(1)
