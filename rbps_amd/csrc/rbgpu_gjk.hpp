@@ -908,7 +908,7 @@ k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *c
                                                                   Colliders C, Bodies B, Contacts K, HullSet hs,
                                                                   uint32_t *gstat, const float4 *guess, float4 *tlog,
                                                                   const uint32_t *tlog_base, uint32_t *hits,
-                                                                  uint32_t *blk_hits, float4 *sxs) {
+                                                                  uint32_t *blk_hits, float4 *sxs, float4 *axis) {
     __shared__ uint32_t inside_w[GJK_PASS_WAVES];
     const int wave = threadIdx.x >> 6, lane = lane64();
     const uint32_t nc = gjk_count(n, ncand);
@@ -936,6 +936,12 @@ k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *c
             }
             inside = gs == GJK_INSIDE;
             if (lane == 0) gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);   // pass 2 completes an overlapping pair
+            if (lane == 0 && axis) {
+                // a separated pair's last direction: the support of A - B along it is
+                // negative, i.e. B lies beyond A along +dir (a hint for the next rejection test)
+                const float ln = len(sx.dir);
+                axis[s] = gs == GJK_VALID && ln > 1e-12f ? f4(vdiv(sx.dir, ln), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
             if (!inside && lane == 0) {
                 K.cnt[s] = 0;
                 if (tlog) {
@@ -1060,14 +1066,38 @@ __global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint3
 
 // Totals over the GJK pairs bin_slot[off .. off + n) from gstat: pairs, GJK iterations,
 // EPA iterations, overlapping pairs, EPA converged, fallback runs.
-// Exact rejection in front of GJK: a pair whose bounding spheres (radius about each
-// collider's origin in lpos.w, rounded up) are apart by more than a margin cannot
-// overlap, so plain GJK would end separated and emit nothing; the pair is finished here
-// (no contact, status separated after 0 iterations, not logged for training) and the others
-// are flagged for k_gjk_compact.  The margin (1e-4 relative + 1e-3 m + 1e-6 of the
-// largest coordinate) covers the FP32 rounding of the centres and of the test itself.
+// Extent of a collider beyond its origin along the unit direction u (world frame): the
+// support distance max over the shape of (p - origin) . u; a hull takes it over its vertices.
+__device__ __forceinline__ float support_extent(const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf, V3 u) {
+    const int4 m = C.meta[c];
+    const float4 dm = C.dims[c];
+    const V3 l = tf_inv_vec(tf, u);
+    if (m.x == SH_SPHERE) return dm.x;
+    if (m.x == SH_BOX) return fabsf(l.x) * dm.x + fabsf(l.y) * dm.y + fabsf(l.z) * dm.z;
+    if (m.x == SH_CAPSULE) return fabsf(l.y) * dm.x + dm.y;
+    const HullMeta hm = hs.meta[C.hull[c]];
+    float e = -3.4e38f;
+    for (uint32_t k = 0; k < hm.nv; ++k) {
+        const float4 v = hs.v[hm.v0 + k];
+        e = fmaxf(e, v.x * l.x + v.y * l.y + v.z * l.z);
+    }
+    return e;
+}
+
+// Exact rejection in front of GJK.  Two tests on the line between the collider origins:
+// bounding spheres (radius about each origin in lpos.w, rounded up) apart, or the two
+// shapes' support extents along that line apart (a separating plane).  Either way the
+// shapes cannot overlap, so plain GJK would end separated and emit nothing; the pair is
+// finished here (no contact, status separated after 0 iterations, not logged for
+// training) and the others are flagged for k_gjk_compact.  The margin (1e-4 relative +
+// 1e-3 m + 1e-6 of the largest coordinate) covers the FP32 rounding of the centres and of
+// the tests themselves.
+// A third test re-checks the axis that separated the pair the last time GJK ran on its
+// slot (`axis`, world frame, written by k_gjk_pass): bodies move little between substeps, so
+// that axis usually still separates.  Any axis that separates the current poses proves the
+// pair apart, however old or whichever pair it came from, so the cache needs no clearing.
 __global__ void k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B, Contacts K,
-                           uint32_t *gstat, uint32_t *keep) {
+                           HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = bin_slot[off + i];
@@ -1077,8 +1107,24 @@ __global__ void k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, C
     const V3 d = tb.pos - ta.pos;
     const float big = fmaxf(fmaxf(fmaxf(fabsf(ta.pos.x), fabsf(ta.pos.y)), fmaxf(fabsf(ta.pos.z), fabsf(tb.pos.x))),
                             fmaxf(fabsf(tb.pos.y), fabsf(tb.pos.z)));
-    const float rs = (ra + rb) * 1.0001f + 1e-3f + 1e-6f * big;
-    const bool apart = len2(d) > rs * rs;
+    const float slack = 1e-3f + 1e-6f * big;
+    const float rs = (ra + rb) * 1.0001f + slack;
+    const float d2 = len2(d);
+    bool apart = d2 > rs * rs;
+    if (!apart && axis) {
+        const float4 a = axis[s];
+        if (a.w != 0.0f) {
+            const V3 u{a.x, a.y, a.z};
+            const float ext = support_extent(C, hs, cc.x, ta, u) + support_extent(C, hs, cc.y, tb, -u);
+            apart = ext * 1.0001f + slack < dot(d, u);
+        }
+    }
+    if (!apart && d2 > 1e-8f) {
+        const float dl = sqrtf(d2);
+        const V3 u = vdiv(d, dl);
+        const float ext = support_extent(C, hs, cc.x, ta, u) + support_extent(C, hs, cc.y, tb, -u);
+        apart = ext * 1.0001f + slack < dl;
+    }
     keep[i] = apart ? 0u : 1u;
     if (apart) {
         K.cnt[s] = 0;

@@ -124,6 +124,7 @@ struct rbgpu_world {
     int hull_vpl = 1;                   // hull vertices per lane in the wave kernels (1: all hulls <= 64)
     uint32_t *gstat = nullptr;
     uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
+    float4 *gjk_axis = nullptr;   // per pair slot: the axis that last separated it (rejection hint)
     int contact_mode = 0;                               // 1: capsule-capsule normal from A to B
     bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
@@ -203,7 +204,7 @@ struct rbgpu_world {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
         dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx); dfree(gjk_keep); dfree(gjk_pos); dfree(gjk_cand);
-        dfree(gjk_ncand); dfree(sc_work); dfree(sc_list);
+        dfree(gjk_ncand); dfree(gjk_axis); dfree(sc_work); dfree(sc_list);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
         dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
@@ -295,7 +296,7 @@ struct rbgpu_world {
         hipEvent_t *e = timed_begin(11);
         const uint32_t *cand = nullptr, *ncand = nullptr;
         if (gjk_cull && !gjk_explicit) {
-            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, gstat, gjk_keep);
+            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep);
             int rc = scan(gjk_keep, gjk_pos, ng, gjk_ncand);
             if (rc) return rc;
             k_gjk_compact<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
@@ -316,12 +317,12 @@ struct rbgpu_world {
         const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb, epa_grid);
         if (hull_vpl == 1) {
             k_gjk_pass<1><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, tl,
-                                                              ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+                                                              ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
             k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
                                              gjk_hits, gjk_blk, gjk_sx);
         } else {
             k_gjk_pass<HULL_VPL><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
-                                                                     gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+                                                                     gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
             k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl,
                                                     ds_cnt, gjk_hits, gjk_blk, gjk_sx);
         }
@@ -471,6 +472,8 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
         rc |= dalloc(gjk_hits, nb1 * GJK_PASS_WAVES); rc |= dalloc(gjk_blk, nb1);
         rc |= dalloc(gjk_sx, nb1 * GJK_PASS_WAVES * 8);
         rc |= dalloc(gjk_keep, cap); rc |= dalloc(gjk_pos, cap); rc |= dalloc(gjk_cand, cap); rc |= dalloc(gjk_ncand, 1);
+        rc |= dalloc(gjk_axis, cap);
+        if (!rc) HIPCHK(hipMemsetAsync(gjk_axis, 0, (size_t)cap * 16, st));   // no hints yet
     }
     if (mlp_wb) {
         dfree(inf_feat); dfree(inf_guess);
