@@ -311,6 +311,10 @@ def main():
                 "avg_launch_us": round(avg_s * 1e6, 3), "launches": tm["launches"],
                 "algorithmic_bytes_per_launch": round(bytes_per_launch, 1)}
 
+    if roof and args.timing_kernel == "gjk":
+        roof["kernel"] = "k_gjk_pass"
+        roof["note"] = ("GJK on the candidates that pass the exact rejection tests, one wavefront per pair; "
+                        "algorithmic bytes = candidates (device count) x (2 x (16 B x mean hull vertices + 164 B) + 40 B)")
     out = {
         "metric": METRIC,
         "value": round(value, 1), "unit": "body-substeps/s", "n_gpus": 1, "steps": args.steps,
@@ -321,9 +325,7 @@ def main():
                    "last_frame": st},
         "roofline": roof if args.workload == "pile" else dict(roof or {}, traffic=None),
     }
-    if roof and args.timing_kernel == "gjk":
-        roof["note"] = ("GJK/EPA stage: rejection test + compaction, features and inference, GJK pass, EPA "
-                        "pass; algorithmic bytes = pairs of the hull bins x 160 B + GJK candidates x pair bytes")
+
     if args.workload == "hulls" and not args.no_learned:
         out["learned"] = learned_report(w, sc)
     if args.workload == "combined":

@@ -181,7 +181,7 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  * (per-colour launches), 3 narrowphase, 4 integrate, 5 velocity update, 6 broadphase
  * (refit, endpoint sort, sweep, pair list; one span per frame), 7 colouring (one span per
  * substep), 8 position solve runs of small colours (single workgroup), 9 velocity solve
- * runs, 10 joints, 11 GJK/EPA pairs (the wave-per-pair hull kernel).  rbgpu_get_timing
+ * runs, 10 joints, 11 the GJK pass (k_gjk_pass, GJK on the candidates of the hull bins).  rbgpu_get_timing
  * returns the summed milliseconds and launches since the last reset (synchronises);
  * bytes = algorithmic bytes of the timed launches for 1, 2 and 11. */
 /* Pipeline options (no counterpart in the reference, whose step is sequential).

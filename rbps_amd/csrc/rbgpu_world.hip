@@ -293,7 +293,7 @@ struct rbgpu_world {
         }
         const bool infer = mlp_mode && mlp_wb && !gjk_explicit && inf_guess && ng <= inf_cap;
         float4 *tl = gjk_log ? ds_tgt : nullptr;   // rows after *ds_cnt
-        hipEvent_t *e = timed_begin(11);
+        const bool timed = timing_kernel == 11 && frames % timing_stride == 0;
         const uint32_t *cand = nullptr, *ncand = nullptr;
         if (gjk_cull && !gjk_explicit) {
             k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep);
@@ -302,7 +302,7 @@ struct rbgpu_world {
             k_gjk_compact<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
             cand = gjk_cand;
             ncand = gjk_ncand;
-            if (e) k_add_count<<<1, 1, 0, st>>>(gjk_ncand, d_points);
+            if (timed) k_add_count<<<1, 1, 0, st>>>(gjk_ncand, d_points);
         }
         if (gjk_log) k_gjk_features<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, ds_feat, ds_cnt);
         if (infer) {
@@ -315,19 +315,21 @@ struct rbgpu_world {
         const int seed = mlp_mode == 2 ? 1 : 0;
         const uint32_t nvb = (ng + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
         const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb, epa_grid);
+        hipEvent_t *e = timed_begin(11);   // the GJK pass alone
         if (hull_vpl == 1) {
             k_gjk_pass<1><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, tl,
                                                               ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            timed_end(e);
             k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
                                              gjk_hits, gjk_blk, gjk_sx);
         } else {
             k_gjk_pass<HULL_VPL><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
                                                                      gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            timed_end(e);
             k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl,
                                                     ds_cnt, gjk_hits, gjk_blk, gjk_sx);
         }
-        timed_end(e);
-        if (e) timing_bytes += (double)ng * (cand ? GJK_CULL_BYTES : gjk_pair_bytes());
+        if (e && !cand) timing_bytes += (double)ng * gjk_pass_bytes();
         if (gjk_tot) k_gjk_totals<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, gstat, gjk_tot);
         if (gjk_log) {
             k_log_advance<<<1, 1, 0, st>>>(ds_cnt, ncand, ng);
@@ -421,15 +423,15 @@ struct rbgpu_world {
         return RBGPU_OK;
     }
 
-    // Algorithmic bytes of one GJK pair (k_gjk_pass + k_epa_pass): per side the hull vertices (16 B
-    // each, the world's mean count), hull meta 64 B, collider rows 68 B, body pose 32 B;
-    // the pair's slot, colliders and bodies 20 B; out: count, normal and four contact
-    // points (72 B each) 308 B.
-    double gjk_pair_bytes() const {
+    // Algorithmic bytes of one pair in the GJK pass: per side the hull vertices (16 B each, the
+    // world's mean count), hull meta 64 B, collider rows 68 B, body pose 32 B; the pair's slot,
+    // colliders and bodies 20 B; out: status 4 B and the separating-axis hint 16 B (an
+    // overlapping pair's simplex, 128 B, is not counted).
+    double gjk_pass_bytes() const {
         double nv = 0.0;
         for (const HostHull &h : hh) nv += (double)h.t.v.size();
         nv = hh.empty() ? 0.0 : nv / (double)hh.size();
-        return 2.0 * (16.0 * nv + 64.0 + 68.0 + 32.0) + 20.0 + 308.0;
+        return 2.0 * (16.0 * nv + 64.0 + 68.0 + 32.0) + 20.0 + 20.0;
     }
 
     hipEvent_t *timed_begin(int kernel) {
@@ -1490,8 +1492,8 @@ int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, doubl
         if (w->d_points) HIPCHK(hipMemcpy(&pts, w->d_points, 8, hipMemcpyDeviceToHost));
         // algorithmic bytes per contact point of the timed per-colour launches: position
         // solve 296 B, velocity solve 280 B
-        // GJK: rejection test per pair of the GJK bins (host sum) + full bytes per candidate
-        *bytes = w->timing_kernel == 11 ? w->timing_bytes + (double)pts * w->gjk_pair_bytes()
+        // GJK pass: bytes per candidate (counted on the device) or per pair without the rejection
+        *bytes = w->timing_kernel == 11 ? w->timing_bytes + (double)pts * w->gjk_pass_bytes()
                                         : (double)pts * (w->timing_kernel == 1 ? 296.0 : w->timing_kernel == 2 ? 280.0 : 0.0);
     }
     return RBGPU_OK;
