@@ -65,18 +65,22 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 // ------------------------------------------------------------------ shapes in a wave
-template <int VPL> struct WaveHull {   // this lane's vertices lane + 64k
+template <int VPL, int LPP = 64> struct WaveHull {   // this lane's vertices r + LPP k (r: lane within the pair's LPP lanes)
     float x[VPL], y[VPL], z[VPL];
     int nv;
     uint32_t v0;
 };
-template <int VPL> __device__ __forceinline__ void load_wave_hull(WaveHull<VPL> &w, const HullSet &hs, uint32_t h) {
+// Lanes per pair (LPP): 64 = one pair per wavefront; 16 = four pairs per wavefront, one per
+// DPP row (hulls of at most 16 * VPL vertices).
+template <int LPP> __device__ __forceinline__ int pair_lane() { return lane64() & (LPP - 1); }
+
+template <int VPL, int LPP> __device__ __forceinline__ void load_wave_hull(WaveHull<VPL, LPP> &w, const HullSet &hs, uint32_t h) {
     const HullMeta m = hs.meta[h];
     w.nv = (int)m.nv;
     w.v0 = m.v0;
 #pragma unroll
     for (int k = 0; k < VPL; ++k) {
-        const int i = lane64() + 64 * k;
+        const int i = pair_lane<LPP>() + LPP * k;
         float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (i < w.nv) p = hs.v[m.v0 + i];
         w.x[k] = p.x; w.y[k] = p.y; w.z[k] = p.z;
@@ -117,7 +121,41 @@ __device__ __forceinline__ uint32_t ord_f(float x) {
 // coordinates in *vout.  Lane-local scan in index order, wave max of the mapped value,
 // then the lowest index among the lanes holding it.  A NaN at vertex 0 is never beaten
 // in the sequential scan; NaN elsewhere never wins.
-template <int VPL> __device__ __forceinline__ int wave_support_index(const WaveHull<VPL> &w, V3 d, V3 *vout = nullptr) {
+template <int VPL, int LPP> __device__ __forceinline__ int wave_support_index(const WaveHull<VPL, LPP> &w, V3 d, V3 *vout = nullptr) {
+    if constexpr (LPP == 16) {
+        // one pair per 16-lane row: vertex r + 16 k on lane r of the row; a lane keeps its
+        // first maximum, the row takes the lowest index among the lanes holding the row
+        // maximum (dpp_row_max / dpp_row_min leave the row result in every lane of the row)
+        const int r = pair_lane<16>(), row = lane64() >> 4;
+        uint32_t best = 0u;
+        int bk = 0;
+        bool nan0 = false;
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) {
+            const int i = r + 16 * k;
+            if (i < w.nv) {
+                float x = w.x[k] * d.x + w.y[k] * d.y + w.z[k] * d.z;
+                if (x != x) {
+                    nan0 |= i == 0;
+                    x = -__builtin_inff();
+                }
+                const uint32_t o = ord_f(x);
+                if (o > best) { best = o; bk = k; }
+            }
+        }
+        const uint32_t m = dpp_row_max(best);
+        int idx = (int)dpp_row_min(best == m ? (uint32_t)(r + 16 * bk) : 0xffffffffu);
+        if ((__ballot(nan0) >> (16 * row)) & 1ull) idx = 0;
+        if (vout) {
+            const int src = (row << 4) | (idx & 15), slot = idx >> 4;
+            float x = w.x[0], y = w.y[0], z = w.z[0];
+#pragma unroll
+            for (int k = 1; k < VPL; ++k)
+                if (slot == k) { x = w.x[k]; y = w.y[k]; z = w.z[k]; }
+            *vout = V3{__shfl(x, src, 64), __shfl(y, src, 64), __shfl(z, src, 64)};
+        }
+        return idx;
+    }
     const int lane = lane64();
     uint32_t best = 0u;
     int bk = 0;
@@ -156,18 +194,18 @@ template <int VPL> __device__ __forceinline__ int wave_support_index(const WaveH
     return idx;
 }
 
-template <int VPL> struct Side {
+template <int VPL, int LPP = 64> struct Side {
     int kind;
     float a, b, c;   // sphere r | box half extents | capsule half_height, radius
     TF tf;
-    WaveHull<VPL> wh;
+    WaveHull<VPL, LPP> wh;
 };
-template <int VPL> __device__ __forceinline__ V3 hull_vertex(const HullSet &hs, const WaveHull<VPL> &w, int i) {
+template <int VPL, int LPP> __device__ __forceinline__ V3 hull_vertex(const HullSet &hs, const WaveHull<VPL, LPP> &w, int i) {
     const float4 p = hs.v[w.v0 + i];
     return V3{p.x, p.y, p.z};
 }
 // Local support (Sphere.hpp:48-51, Box.hpp:49-55, Capsule.hpp:72-80, ConvexHull.hpp:457-473).
-template <int VPL> __device__ __forceinline__ V3 local_support(const Side<VPL> &s, const HullSet &hs, V3 d) {
+template <int VPL, int LPP> __device__ __forceinline__ V3 local_support(const Side<VPL, LPP> &s, const HullSet &hs, V3 d) {
     if (s.kind == SH_HULL) {
         V3 v;
         wave_support_index(s.wh, d, &v);
@@ -181,14 +219,14 @@ template <int VPL> __device__ __forceinline__ V3 local_support(const Side<VPL> &
     return e + vdiv(d, l) * s.b;
 }
 // World support of one side (MinkowskiDiff.hpp:52-63).
-template <int VPL> __device__ __forceinline__ V3 side_support(const Side<VPL> &s, const HullSet &hs, V3 dir) {
+template <int VPL, int LPP> __device__ __forceinline__ V3 side_support(const Side<VPL, LPP> &s, const HullSet &hs, V3 dir) {
     return tf_point(s.tf, local_support(s, hs, tf_inv_vec(s.tf, dir)));
 }
 
 struct SV {
     V3 w0, w;   // A-side support, Minkowski-difference point (the B-side support is not kept)
 };
-template <int VPL> __device__ __forceinline__ SV md_sample(const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs, V3 d) {
+template <int VPL, int LPP> __device__ __forceinline__ SV md_sample(const Side<VPL, LPP> &A, const Side<VPL, LPP> &Bs, const HullSet &hs, V3 d) {
     SV s;
     s.w0 = side_support(A, hs, d);
     s.w = s.w0 - side_support(Bs, hs, -d);
@@ -301,7 +339,7 @@ __device__ __forceinline__ bool sx_tetra(Simplex &s, V3 &dir) {   // Tetrahedron
     }
     return false;
 }
-template <int VPL> __device__ __forceinline__ int gjk(const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs, V3 guess, Simplex &s, int &iters,
+template <int VPL, int LPP> __device__ __forceinline__ int gjk(const Side<VPL, LPP> &A, const Side<VPL, LPP> &Bs, const HullSet &hs, V3 guess, Simplex &s, int &iters,
                                    int max_iter = GJK_ITERS) {
     s.rank = 0;
     V3 dir = (len2(guess) > EPS) ? guess : V3{1.0f, 0.0f, 0.0f};
@@ -839,8 +877,8 @@ template <int VPL> __device__ __forceinline__ void manifold_hulls(const EpaResul
 }
 
 // ------------------------------------------------------------------ kernels
-template <int VPL>
-__device__ __forceinline__ void load_side(Side<VPL> &s, const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf) {
+template <int VPL, int LPP>
+__device__ __forceinline__ void load_side(Side<VPL, LPP> &s, const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf) {
     const int kind = C.meta[c].x;
     const float4 d = C.dims[c];
     s.kind = kind;
@@ -898,23 +936,28 @@ __device__ __forceinline__ uint32_t pack_gstat(int gsum, int esum, int gs, int e
 #endif
 // Candidate lists (k_gjk_cull): with `cand`, the passes run over the *ncand candidates,
 // candidate tc being bin position cand[tc]; the warm starts are per candidate, the training
-// targets per bin position.  Both passes stride over "virtual blocks" of four candidates
-// (block b takes b, b + gridDim.x, ...), so the grid does not depend on the count.
+// targets per bin position.  Both passes stride over "virtual blocks" of gjk_pass_pairs
+// candidates (block b takes b, b + gridDim.x, ...), so the grid does not depend on the count.
 __device__ __forceinline__ uint32_t gjk_count(uint32_t n, const uint32_t *ncand) { return ncand ? min(*ncand, n) : n; }
 
-template <int VPL>
+// Pairs per workgroup of the GJK pass: four wavefronts of 64 / LPP pairs each; the EPA pass
+// reads the overlapping pairs in segments of this size.
+template <int LPP> constexpr int gjk_pass_pairs() { return GJK_PASS_WAVES * (64 / LPP); }
+
+template <int VPL, int LPP>
 __global__ void __launch_bounds__(64 * GJK_PASS_WAVES) __attribute__((amdgpu_waves_per_eu(GJK_PASS_OCCUPANCY)))
 k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand, const uint32_t *ncand,
                                                                   Colliders C, Bodies B, Contacts K, HullSet hs,
                                                                   uint32_t *gstat, const float4 *guess, float4 *tlog,
                                                                   const uint32_t *tlog_base, uint32_t *hits,
                                                                   uint32_t *blk_hits, float4 *sxs, float4 *axis) {
-    __shared__ uint32_t inside_w[GJK_PASS_WAVES];
-    const int wave = threadIdx.x >> 6, lane = lane64();
+    constexpr int PPB = gjk_pass_pairs<LPP>();
+    __shared__ uint32_t inside_g[PPB];
+    const int g = (int)threadIdx.x / LPP, pl = pair_lane<LPP>();   // pair group in the block, lane within it
     const uint32_t nc = gjk_count(n, ncand);
     if (tlog && tlog_base) tlog += 2 * (size_t)*tlog_base;
-    for (uint32_t vb = blockIdx.x; vb * GJK_PASS_WAVES < nc; vb += gridDim.x) {
-        const uint32_t tc = vb * GJK_PASS_WAVES + wave;   // candidate
+    for (uint32_t vb = blockIdx.x; vb * PPB < nc; vb += gridDim.x) {
+        const uint32_t tc = vb * PPB + g;   // candidate
         bool inside = false;
         Simplex sx;
         if (tc < nc) {
@@ -922,7 +965,7 @@ k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *c
             const uint32_t s = bin_slot[off + t];
             const uint2 cc = K.cc[s], bb = K.bb[s];
             const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
-            Side<VPL> A, Bs;
+            Side<VPL, LPP> A, Bs;
             load_side(A, C, hs, cc.x, ta);
             load_side(Bs, C, hs, cc.y, tb);
             const GjkSeeds q = gjk_seeds(ta, tb, guess, tc, 0);
@@ -935,14 +978,14 @@ k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *c
                 gsum += it;
             }
             inside = gs == GJK_INSIDE;
-            if (lane == 0) gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);   // pass 2 completes an overlapping pair
-            if (lane == 0 && axis) {
+            if (pl == 0) gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);   // pass 2 completes an overlapping pair
+            if (pl == 0 && axis) {
                 // a separated pair's last direction: the support of A - B along it is
                 // negative, i.e. B lies beyond A along +dir (a hint for the next rejection test)
                 const float ln = len(sx.dir);
                 axis[s] = gs == GJK_VALID && ln > 1e-12f ? f4(vdiv(sx.dir, ln), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
-            if (!inside && lane == 0) {
+            if (!inside && pl == 0) {
                 K.cnt[s] = 0;
                 if (tlog) {
                     V3 tg{0.0f, 0.0f, 0.0f};
@@ -956,31 +999,34 @@ k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *c
                 }
             }
         }
-        if (lane == 0) inside_w[wave] = inside ? 1u : 0u;
+        if (pl == 0) inside_g[g] = inside ? 1u : 0u;
         __syncthreads();
         int pos = 0;
-        for (int k = 0; k < wave; ++k) pos += (int)inside_w[k];
+        for (int k = 0; k < g; ++k) pos += (int)inside_g[k];
         if (inside) {
-            const uint32_t h = vb * GJK_PASS_WAVES + pos;
-            if (lane == 0) hits[h] = tc;
-            if (lane < 4) {   // the four simplex vertices: (w.xyz, w0.x) (w0.y, w0.z)
-                const SV v = sx_vertex(sx, lane);
-                sxs[(size_t)h * 8 + 2 * lane] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
-                sxs[(size_t)h * 8 + 2 * lane + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
+            const uint32_t h = vb * PPB + pos;
+            if (pl == 0) hits[h] = tc;
+            if (pl < 4) {   // the four simplex vertices: (w.xyz, w0.x) (w0.y, w0.z)
+                const SV v = sx_vertex(sx, pl);
+                sxs[(size_t)h * 8 + 2 * pl] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
+                sxs[(size_t)h * 8 + 2 * pl + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
             }
         }
         if (threadIdx.x == 0) {
             uint32_t c = 0;
-            for (int k = 0; k < GJK_PASS_WAVES; ++k) c += inside_w[k];
+            for (int k = 0; k < PPB; ++k) c += inside_g[k];
             blk_hits[vb] = c;
         }
-        __syncthreads();   // inside_w is rewritten by the next virtual block
+        __syncthreads();   // inside_g is rewritten by the next virtual block
     }
 }
 
+// seg: pairs per GJK-pass workgroup (gjk_pass_pairs of the pass that listed the pairs); a
+// workgroup takes `sub` of a segment's entries at a time (sub divides seg).
 template <int VPL>
 __global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand,
-                                                 const uint32_t *ncand, Colliders C, Bodies B, Contacts K,
+                                                 const uint32_t *ncand, uint32_t seg, uint32_t sub, Colliders C,
+                                                 Bodies B, Contacts K,
                                                  HullSet hs, uint32_t *gstat, const float4 *guess, int seed_epa,
                                                  float4 *tlog, const uint32_t *tlog_base, const uint32_t *hits,
                                                  const uint32_t *blk_hits, const float4 *sxs) {
@@ -988,9 +1034,11 @@ __global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint3
     const int lane = lane64();
     const uint32_t nc = gjk_count(n, ncand);
     if (tlog && tlog_base) tlog += 2 * (size_t)*tlog_base;
-    for (uint32_t vb = blockIdx.x; vb * GJK_PASS_WAVES < nc; vb += gridDim.x)
-    for (uint32_t r = 0, nh = blk_hits[vb]; r < nh; ++r) {
-        const uint32_t h = vb * GJK_PASS_WAVES + r;
+    // work items of `sub` listed pairs: a GJK segment of seg entries splits into seg / sub
+    const uint32_t per = seg / sub, nitems = (nc + seg - 1) / seg * per;
+    for (uint32_t ve = blockIdx.x; ve < nitems; ve += gridDim.x)
+    for (uint32_t vb = ve / per, r = (ve % per) * sub, re = min(blk_hits[vb], r + sub); r < re; ++r) {
+        const uint32_t h = vb * seg + r;
         const uint32_t tc = hits[h];
         const uint32_t t = cand ? cand[tc] : tc;
         const uint32_t s = bin_slot[off + t];

@@ -128,6 +128,8 @@ struct rbgpu_world {
     int contact_mode = 0;                               // 1: capsule-capsule normal from A to B
     bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
+    bool gjk_rows = true;                               // GJK pass: a pair per 16-lane row (hulls <= 64 vertices)
+    uint32_t epa_sub = 16;                              // EPA pass: listed pairs per work item
     uint32_t *gjk_hits = nullptr, *gjk_blk = nullptr;   // overlapping GJK pairs per 4-pair workgroup
     float4 *gjk_sx = nullptr;                          // their final simplices (8 float4 each)
     // learned warm start: parameters (FP32 master, BF16 copy), gradient, Adam moments, the
@@ -313,20 +315,31 @@ struct rbgpu_world {
         }
         const float4 *gp = (gjk_explicit || infer) ? inf_guess : nullptr;
         const int seed = mlp_mode == 2 ? 1 : 0;
-        const uint32_t nvb = (ng + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
-        const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb, epa_grid);
+        // four pairs per wavefront (one per DPP row) when every hull has at most 64 vertices
+        const bool rows = gjk_rows && hull_vpl == 1;
+        const uint32_t ppb = rows ? (uint32_t)gjk_pass_pairs<16>() : (uint32_t)gjk_pass_pairs<64>();
+        const uint32_t nvb = (ng + ppb - 1) / ppb;
+        const uint32_t sub = std::min<uint32_t>(epa_sub, ppb);
+        const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb * (ppb / sub), epa_grid);
         hipEvent_t *e = timed_begin(11);   // the GJK pass alone
-        if (hull_vpl == 1) {
-            k_gjk_pass<1><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, tl,
-                                                              ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+        if (rows) {
+            k_gjk_pass<4, 16><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
+                                                                  tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
             timed_end(e);
-            k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
+            k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
+                                             gjk_hits, gjk_blk, gjk_sx);
+        } else if (hull_vpl == 1) {
+            k_gjk_pass<1, 64><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
+                                                                  tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            timed_end(e);
+            k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
                                              gjk_hits, gjk_blk, gjk_sx);
         } else {
-            k_gjk_pass<HULL_VPL><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
-                                                                     gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            k_gjk_pass<HULL_VPL, 64><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
+                                                                         gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx,
+                                                                         gjk_axis);
             timed_end(e);
-            k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl,
+            k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
                                                     ds_cnt, gjk_hits, gjk_blk, gjk_sx);
         }
         if (e && !cand) timing_bytes += (double)ng * gjk_pass_bytes();
@@ -470,9 +483,11 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     rc |= dalloc(bin_list, cap); rc |= dalloc(pair_lid, cap); rc |= dalloc(sc_work, cap); rc |= dalloc(sc_list, cap);
     rc |= dalloc(bb_hits, nblk(cap, 128) * 128); rc |= dalloc(bb_blk, nblk(cap, 128)); rc |= dalloc(gstat, cap);
     if (!hh.empty()) {
+        // listed pairs: segments of up to gjk_pass_pairs<16>() = 16 per virtual block of 4 or 16 candidates
         const size_t nb1 = (cap + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES;
-        rc |= dalloc(gjk_hits, nb1 * GJK_PASS_WAVES); rc |= dalloc(gjk_blk, nb1);
-        rc |= dalloc(gjk_sx, nb1 * GJK_PASS_WAVES * 8);
+        const size_t nh = ((size_t)cap + 15) / 16 * 16 + 16;
+        rc |= dalloc(gjk_hits, nh); rc |= dalloc(gjk_blk, nb1);
+        rc |= dalloc(gjk_sx, nh * 8);
         rc |= dalloc(gjk_keep, cap); rc |= dalloc(gjk_pos, cap); rc |= dalloc(gjk_cand, cap); rc |= dalloc(gjk_ncand, 1);
         rc |= dalloc(gjk_axis, cap);
         if (!rc) HIPCHK(hipMemsetAsync(gjk_axis, 0, (size_t)cap * 16, st));   // no hints yet
@@ -1202,6 +1217,8 @@ int rbgpu_autotune(rbgpu_world *w, int frames_per_trial, rbgpu_tune_report *out)
         knobs.push_back({RBGPU_OPT_ROUND_BATCH, {6, 12, 24}});
     }
     if (!w->hh.empty()) {
+        if (w->hull_vpl == 1) knobs.push_back({RBGPU_OPT_GJK_ROWS, {1, 0}});
+        knobs.push_back({RBGPU_OPT_EPA_SUB, {16, 4}});
         knobs.push_back({RBGPU_OPT_GJK_GRID, {1024, 2048, 4096, 8192, 16384}});
         knobs.push_back({RBGPU_OPT_EPA_GRID, {2048, 8192, 32768}});
     }
@@ -1441,6 +1458,13 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_CONTACT_MODE:
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "contact mode is 0 or 1");
             w->contact_mode = value; return RBGPU_OK;
+        case RBGPU_OPT_GJK_ROWS:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "GJK rows is 0 or 1");
+            w->gjk_rows = value != 0; return RBGPU_OK;
+        case RBGPU_OPT_EPA_SUB:
+            if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+                return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
+            w->epa_sub = (uint32_t)value; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
@@ -1462,6 +1486,8 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_EPA_GRID: *value = (int)w->epa_grid; return RBGPU_OK;
         case RBGPU_OPT_GRAPH: *value = w->use_graph ? 1 : 0; return RBGPU_OK;
         case RBGPU_OPT_CONTACT_MODE: *value = w->contact_mode; return RBGPU_OK;
+        case RBGPU_OPT_GJK_ROWS: *value = w->gjk_rows ? 1 : 0; return RBGPU_OK;
+        case RBGPU_OPT_EPA_SUB: *value = (int)w->epa_sub; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
