@@ -159,7 +159,7 @@ def run_frames(w, steps, label=""):
     return time.perf_counter() - t0
 
 
-def tune_and_align(w, start, frames_per_trial=2, settle=40):
+def tune_and_align(w, start, frames_per_trial=3, settle=60):
     """Autotune (result-neutral options, real frames), then step up to frame start + settle
     so that differently tuned legs time the same frames."""
     rep = w.autotune(frames_per_trial)

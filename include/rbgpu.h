@@ -248,8 +248,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value);
 
 /* Autotuner (no counterpart in the reference).  Picks this world's launch options that
  * cannot change any result (per-scene solve workgroup size and the substep graph, or the
- * grid path's run minimum and round batch; the GJK and EPA grids when the world has hulls)
- * by coordinate descent: each candidate value is timed over `frames_per_trial` real frames
+ * grid path's run minimum and round batch) by coordinate descent: each candidate value is timed over `frames_per_trial` real frames
  * (wall clock between two stream synchronisations) and the fastest is kept.  The trial
  * frames advance the world exactly as untuned frames would, since every candidate gives
  * bit-identical states; callers typically tune during warm-up.  Requires timing off. */

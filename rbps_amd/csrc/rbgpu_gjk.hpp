@@ -1115,15 +1115,23 @@ __global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint3
 // Totals over the GJK pairs bin_slot[off .. off + n) from gstat: pairs, GJK iterations,
 // EPA iterations, overlapping pairs, EPA converged, fallback runs.
 // Extent of a collider beyond its origin along the unit direction u (world frame): the
-// support distance max over the shape of (p - origin) . u; a hull takes it over its vertices.
-__device__ __forceinline__ float support_extent(const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf, V3 u) {
+// support distance max over the shape of (p - origin) . u; a hull takes it over its vertices,
+// or (bound_only) over the corners of its local AABB, an upper bound.
+__device__ __forceinline__ float support_extent(const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf, V3 u,
+                                                bool bound_only = false) {
     const int4 m = C.meta[c];
     const float4 dm = C.dims[c];
     const V3 l = tf_inv_vec(tf, u);
     if (m.x == SH_SPHERE) return dm.x;
     if (m.x == SH_BOX) return fabsf(l.x) * dm.x + fabsf(l.y) * dm.y + fabsf(l.z) * dm.z;
     if (m.x == SH_CAPSULE) return fabsf(l.y) * dm.x + dm.y;
-    const HullMeta hm = hs.meta[C.hull[c]];
+    const uint32_t h = (uint32_t)C.hull[c];
+    if (bound_only) {
+        const float4 lo = hs.amin[h], hi = hs.amax[h];
+        return (l.x >= 0.0f ? hi.x * l.x : lo.x * l.x) + (l.y >= 0.0f ? hi.y * l.y : lo.y * l.y) +
+               (l.z >= 0.0f ? hi.z * l.z : lo.z * l.z);
+    }
+    const HullMeta hm = hs.meta[h];
     float e = -3.4e38f;
     for (uint32_t k = 0; k < hm.nv; ++k) {
         const float4 v = hs.v[hm.v0 + k];
@@ -1159,19 +1167,24 @@ __global__ void k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, C
     const float rs = (ra + rb) * 1.0001f + slack;
     const float d2 = len2(d);
     bool apart = d2 > rs * rs;
+    // each axis: first the AABB bound of a hull's extent (no vertex loop), then the exact extent
+    auto apart_along = [&](V3 u, float gap) {
+        for (int exact = 0; exact < 2; ++exact) {
+            const float ext = support_extent(C, hs, cc.x, ta, u, !exact) + support_extent(C, hs, cc.y, tb, -u, !exact);
+            if (ext * 1.0001f + slack < gap) return true;
+        }
+        return false;
+    };
     if (!apart && axis) {
         const float4 a = axis[s];
         if (a.w != 0.0f) {
             const V3 u{a.x, a.y, a.z};
-            const float ext = support_extent(C, hs, cc.x, ta, u) + support_extent(C, hs, cc.y, tb, -u);
-            apart = ext * 1.0001f + slack < dot(d, u);
+            apart = apart_along(u, dot(d, u));
         }
     }
     if (!apart && d2 > 1e-8f) {
         const float dl = sqrtf(d2);
-        const V3 u = vdiv(d, dl);
-        const float ext = support_extent(C, hs, cc.x, ta, u) + support_extent(C, hs, cc.y, tb, -u);
-        apart = ext * 1.0001f + slack < dl;
+        apart = apart_along(vdiv(d, dl), dl);
     }
     keep[i] = apart ? 0u : 1u;
     if (apart) {
