@@ -216,6 +216,7 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *   RBGPU_OPT_GJK_ROWS     GJK pass with four pairs per wavefront, one per 16-lane DPP row, when
  *                          every hull has at most 64 vertices (1, default); 0: one pair per wavefront
  *   RBGPU_OPT_EPA_SUB      listed pairs per EPA work item (1, 2, 4, 8, 16; default 16)
+ *   RBGPU_OPT_GJK_OCCUPANCY  waves per EU the row GJK pass budgets its registers for (2, 3, 4)
  *   RBGPU_OPT_GRAPH        with the per-scene kernels: capture one substep per frame in a
  *                          hipGraph and launch it for every substep (1, default; frames with
  *                          kernel timing or an open training log run without it)
@@ -241,7 +242,8 @@ enum {
     RBGPU_OPT_GRAPH = 12,
     RBGPU_OPT_CONTACT_MODE = 13,
     RBGPU_OPT_GJK_ROWS = 14,
-    RBGPU_OPT_EPA_SUB = 15
+    RBGPU_OPT_EPA_SUB = 15,
+    RBGPU_OPT_GJK_OCCUPANCY = 16
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);

@@ -247,11 +247,20 @@ __device__ __forceinline__ SV sx_vertex(const Simplex &s, int i) {
     if (i == 3) r = s.v[3];
     return r;
 }
+// Field-wise select of two simplex vertices (keeps the simplex in registers: conditional
+// whole-struct copies were lowered to stack copies).
+__device__ __forceinline__ SV sv_sel(bool c, const SV &a, const SV &b) {
+    SV r;
+    r.w0 = V3{c ? a.w0.x : b.w0.x, c ? a.w0.y : b.w0.y, c ? a.w0.z : b.w0.z};
+    r.w = V3{c ? a.w.x : b.w.x, c ? a.w.y : b.w.y, c ? a.w.z : b.w.z};
+    return r;
+}
 __device__ __forceinline__ void sx_push(Simplex &s, const SV &n) {   // DoSimplex.cpp:5-17
     const int r = s.rank;
-    if (r >= 3) s.v[3] = s.v[2];
-    if (r >= 2) s.v[2] = s.v[1];
-    if (r >= 1) s.v[1] = s.v[0];
+    const SV v0 = s.v[0], v1 = s.v[1], v2 = s.v[2], v3 = s.v[3];
+    s.v[3] = sv_sel(r >= 3, v2, v3);
+    s.v[2] = sv_sel(r >= 2, v1, v2);
+    s.v[1] = sv_sel(r >= 1, v0, v1);
     s.v[0] = n;
     if (r < 4) s.rank = r + 1;
 }
@@ -545,10 +554,13 @@ template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed,
             }
         }
     }
-    if (lane < 4) {
-        const SV s = sx_vertex(seed, lane);
-        L.wx[lane] = s.w.x; L.wy[lane] = s.w.y; L.wz[lane] = s.w.z;
-        L.ax[lane] = s.w0.x; L.ay[lane] = s.w0.y; L.az[lane] = s.w0.z;
+    if (lane == 0) {   // constant indices (a lane-dependent one put the simplex on the stack)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const SV &s = seed.v[k];
+            L.wx[k] = s.w.x; L.wy[k] = s.w.y; L.wz[k] = s.w.z;
+            L.ax[k] = s.w0.x; L.ay[k] = s.w0.y; L.az[k] = s.w0.z;
+        }
     }
     __syncthreads();
     int nvtx = 4, nf = 0;
@@ -944,8 +956,8 @@ __device__ __forceinline__ uint32_t gjk_count(uint32_t n, const uint32_t *ncand)
 // reads the overlapping pairs in segments of this size.
 template <int LPP> constexpr int gjk_pass_pairs() { return GJK_PASS_WAVES * (64 / LPP); }
 
-template <int VPL, int LPP>
-__global__ void __launch_bounds__(64 * GJK_PASS_WAVES) __attribute__((amdgpu_waves_per_eu(GJK_PASS_OCCUPANCY)))
+template <int VPL, int LPP, int OCC = GJK_PASS_OCCUPANCY>
+__global__ void __launch_bounds__(64 * GJK_PASS_WAVES) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand, const uint32_t *ncand,
                                                                   Colliders C, Bodies B, Contacts K, HullSet hs,
                                                                   uint32_t *gstat, const float4 *guess, float4 *tlog,
@@ -1003,13 +1015,16 @@ k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *c
         __syncthreads();
         int pos = 0;
         for (int k = 0; k < g; ++k) pos += (int)inside_g[k];
-        if (inside) {
+        if (inside && pl == 0) {
             const uint32_t h = vb * PPB + pos;
-            if (pl == 0) hits[h] = tc;
-            if (pl < 4) {   // the four simplex vertices: (w.xyz, w0.x) (w0.y, w0.z)
-                const SV v = sx_vertex(sx, pl);
-                sxs[(size_t)h * 8 + 2 * pl] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
-                sxs[(size_t)h * 8 + 2 * pl + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
+            hits[h] = tc;
+            // the four simplex vertices: (w.xyz, w0.x) (w0.y, w0.z); constant indices keep the
+            // simplex in registers (a lane-dependent index put it on the stack)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const SV &v = sx.v[k];
+                sxs[(size_t)h * 8 + 2 * k] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
+                sxs[(size_t)h * 8 + 2 * k + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
             }
         }
         if (threadIdx.x == 0) {

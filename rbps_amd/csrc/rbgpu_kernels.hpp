@@ -574,7 +574,9 @@ __device__ __forceinline__ void emit(const Manifold &m, const Bodies &B, Contact
     Q qa = conj(quat(B.q[bb.x])), qb = conj(quat(B.q[bb.y]));
     float4 nr = K.nr[s];
     K.nr[s] = f4(m.n, nr.w);
-    for (int i = 0; i < m.np; ++i) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {   // constant indices keep the manifold in registers
+        if (i >= m.np) break;
         float half = m.dep[i] * 0.5f;
         V3 pa = m.p[i] + m.n * half;
         V3 pb = m.p[i] - m.n * half;

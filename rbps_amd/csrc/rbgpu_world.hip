@@ -130,6 +130,7 @@ struct rbgpu_world {
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
     bool gjk_rows = true;                               // GJK pass: a pair per 16-lane row (hulls <= 64 vertices)
     uint32_t epa_sub = 16;                              // EPA pass: listed pairs per work item
+    int gjk_occ = 4;                                    // GJK row pass: waves per EU the registers are budgeted for
     uint32_t *gjk_hits = nullptr, *gjk_blk = nullptr;   // overlapping GJK pairs per 4-pair workgroup
     float4 *gjk_sx = nullptr;                          // their final simplices (8 float4 each)
     // learned warm start: parameters (FP32 master, BF16 copy), gradient, Adam moments, the
@@ -323,8 +324,16 @@ struct rbgpu_world {
         const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb * (ppb / sub), epa_grid);
         hipEvent_t *e = timed_begin(11);   // the GJK pass alone
         if (rows) {
-            k_gjk_pass<4, 16><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
-                                                                  tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            // waves per EU the compiler budgets registers for: 4 caps them at 128 (some spill), 3 and 2 do not
+            if (gjk_occ == 2)
+                k_gjk_pass<4, 16, 2><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
+                                                                         gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            else if (gjk_occ == 3)
+                k_gjk_pass<4, 16, 3><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
+                                                                         gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            else
+                k_gjk_pass<4, 16><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
+                                                                      tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
             timed_end(e);
             k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
                                              gjk_hits, gjk_blk, gjk_sx);
@@ -1458,6 +1467,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_GJK_ROWS:
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "GJK rows is 0 or 1");
             w->gjk_rows = value != 0; return RBGPU_OK;
+        case RBGPU_OPT_GJK_OCCUPANCY:
+            if (value < 2 || value > 4) return fail(RBGPU_ERR_INVALID, "GJK occupancy: 2, 3 or 4 waves per EU");
+            w->gjk_occ = value; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1485,6 +1497,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_CONTACT_MODE: *value = w->contact_mode; return RBGPU_OK;
         case RBGPU_OPT_GJK_ROWS: *value = w->gjk_rows ? 1 : 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB: *value = (int)w->epa_sub; return RBGPU_OK;
+        case RBGPU_OPT_GJK_OCCUPANCY: *value = w->gjk_occ; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
