@@ -186,10 +186,13 @@ def plain_leg(world, sc, args, pos_learned, value):
                                       "median": float(np.median(d)), "frame": w.stats()["frames"]}}
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, workload="pile"):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload
     (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_pile_mixed_65k_hbm_traffic.json")
+    name = {"pile": f"{kernel}_pile_mixed_65k", "combined": "gjk_pass_combined_1024_scenes"}.get(workload)
+    if name is None or (workload == "combined" and kernel != "gjk"):
+        return None
+    path = os.path.join(ROOT, "profiles", f"{name}_hbm_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -306,7 +309,7 @@ def main():
         bytes_per_launch = tm["bytes"] / tm["launches"]
         achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.timing_kernel),
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.timing_kernel, args.workload),
                 "kernel": args.timing_kernel,
                 "avg_launch_us": round(avg_s * 1e6, 3), "launches": tm["launches"],
                 "algorithmic_bytes_per_launch": round(bytes_per_launch, 1)}
@@ -323,7 +326,7 @@ def main():
         "config": {"workload": workload, "scene": sc.name, "bodies": sc.n_dynamic, "substeps": sc.substeps, "timestep": sc.timestep,
                    "shapes": "convex hulls + static plane" if args.workload == "hulls" else "box/sphere/capsule + static plane",
                    "last_frame": st},
-        "roofline": roof if args.workload == "pile" else dict(roof or {}, traffic=None),
+        "roofline": roof if args.workload in ("pile", "combined") else dict(roof or {}, traffic=None),
     }
 
     if args.workload == "hulls" and not args.no_learned:
