@@ -217,6 +217,8 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          every hull has at most 64 vertices (1, default); 0: one pair per wavefront
  *   RBGPU_OPT_EPA_SUB      listed pairs per EPA work item (1, 2, 4, 8, 16; default 16)
  *   RBGPU_OPT_GJK_OCCUPANCY  waves per EU the row GJK pass budgets its registers for (2, 3, 4)
+ *   RBGPU_OPT_EPA_OCCUPANCY, RBGPU_OPT_SOLVE_OCCUPANCY  the same for the EPA pass and the
+ *                          per-scene position solve (0 = no budget, 4)
  *   RBGPU_OPT_GRAPH        with the per-scene kernels: capture one substep per frame in a
  *                          hipGraph and launch it for every substep (1, default; frames with
  *                          kernel timing or an open training log run without it)
@@ -243,7 +245,9 @@ enum {
     RBGPU_OPT_CONTACT_MODE = 13,
     RBGPU_OPT_GJK_ROWS = 14,
     RBGPU_OPT_EPA_SUB = 15,
-    RBGPU_OPT_GJK_OCCUPANCY = 16
+    RBGPU_OPT_GJK_OCCUPANCY = 16,
+    RBGPU_OPT_EPA_OCCUPANCY = 17,
+    RBGPU_OPT_SOLVE_OCCUPANCY = 18
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);

@@ -1038,8 +1038,9 @@ k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *c
 
 // seg: pairs per GJK-pass workgroup (gjk_pass_pairs of the pass that listed the pairs); a
 // workgroup takes `sub` of a segment's entries at a time (sub divides seg).
-template <int VPL>
-__global__ void __launch_bounds__(64) k_epa_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand,
+// OCC: waves per EU the compiler budgets registers for (0: no budget).
+template <int VPL, int OCC = 0>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? OCC : 1))) k_epa_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand,
                                                  const uint32_t *ncand, uint32_t seg, uint32_t sub, Colliders C,
                                                  Bodies B, Contacts K,
                                                  HullSet hs, uint32_t *gstat, const float4 *guess, int seed_epa,

@@ -1148,8 +1148,8 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
 
 // Colour-ordered solve of every scene, one workgroup per scene (pos: XPBD position solve;
 // otherwise the velocity solve).  Lanes iterate alike, so the point counter may reduce per wave.
-template <bool POS>
-__global__ void __launch_bounds__(256) k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pts) {
+template <bool POS, int OCC = 1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pts) {
     const uint32_t s = blockIdx.x;
     const uint32_t nc = T.ncol[s];
     const uint32_t *lst = T.list + T.base[s];

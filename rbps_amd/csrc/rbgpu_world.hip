@@ -131,6 +131,7 @@ struct rbgpu_world {
     bool gjk_rows = true;                               // GJK pass: a pair per 16-lane row (hulls <= 64 vertices)
     uint32_t epa_sub = 16;                              // EPA pass: listed pairs per work item
     int gjk_occ = 4;                                    // GJK row pass: waves per EU the registers are budgeted for
+    int epa_occ = 0, solve_occ = 0;                     // EPA pass / per-scene position solve: 4 = budget for 4 waves
     uint32_t *gjk_hits = nullptr, *gjk_blk = nullptr;   // overlapping GJK pairs per 4-pair workgroup
     float4 *gjk_sx = nullptr;                          // their final simplices (8 float4 each)
     // learned warm start: parameters (FP32 master, BF16 copy), gradient, Adam moments, the
@@ -335,8 +336,12 @@ struct rbgpu_world {
                 k_gjk_pass<4, 16><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
                                                                       tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
             timed_end(e);
-            k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
-                                             gjk_hits, gjk_blk, gjk_sx);
+            if (epa_occ == 4)
+                k_epa_pass<1, 4><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
+                                                    ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+            else
+                k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
+                                                 ds_cnt, gjk_hits, gjk_blk, gjk_sx);
         } else if (hull_vpl == 1) {
             k_gjk_pass<1, 64><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
                                                                   tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
@@ -970,7 +975,8 @@ int rbgpu_world::frame() {
         }
         if (by_scene && P) {
             hipEvent_t *e = timed_begin(1);
-            k_scene_solve<true><<<nscenes, scene_threads, 0, st>>>(B, K, T, h, e ? d_points : nullptr);
+            if (solve_occ == 4) k_scene_solve<true, 4><<<nscenes, scene_threads, 0, st>>>(B, K, T, h, e ? d_points : nullptr);
+            else k_scene_solve<true><<<nscenes, scene_threads, 0, st>>>(B, K, T, h, e ? d_points : nullptr);
             timed_end(e);
         }
         for (const Run &u : runs) {
@@ -1470,6 +1476,12 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_GJK_OCCUPANCY:
             if (value < 2 || value > 4) return fail(RBGPU_ERR_INVALID, "GJK occupancy: 2, 3 or 4 waves per EU");
             w->gjk_occ = value; return RBGPU_OK;
+        case RBGPU_OPT_EPA_OCCUPANCY:
+            if (value != 0 && value != 4) return fail(RBGPU_ERR_INVALID, "EPA occupancy: 0 (no budget) or 4");
+            w->epa_occ = value; return RBGPU_OK;
+        case RBGPU_OPT_SOLVE_OCCUPANCY:
+            if (value != 0 && value != 4) return fail(RBGPU_ERR_INVALID, "solve occupancy: 0 (no budget) or 4");
+            w->solve_occ = value; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1498,6 +1510,8 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_GJK_ROWS: *value = w->gjk_rows ? 1 : 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB: *value = (int)w->epa_sub; return RBGPU_OK;
         case RBGPU_OPT_GJK_OCCUPANCY: *value = w->gjk_occ; return RBGPU_OK;
+        case RBGPU_OPT_EPA_OCCUPANCY: *value = w->epa_occ; return RBGPU_OK;
+        case RBGPU_OPT_SOLVE_OCCUPANCY: *value = w->solve_occ; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
