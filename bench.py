@@ -243,7 +243,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(independent scenes would shard with no per-substep traffic, see DESIGN.md)", file=sys.stderr)
+              "(independent scenes would shard by scene ranges with no per-substep traffic)", file=sys.stderr)
         sys.exit(2)
 
     from rbps_amd import scenes, world

@@ -293,6 +293,20 @@ struct World {
     }
     // Pipeline options (rbgpu_set_option): RBGPU_OPT_* of rbgpu.h.
     void set_option(int option, int value) { check(rbgpu_set_option(h_, option, value)); }
+    // Times the result-neutral launch options over real frames and keeps the fastest
+    // (rbgpu_autotune; no counterpart in the reference): the world advances report.frames
+    // frames exactly as it would untuned.  Refreshes the body mirror like step().
+    rbgpu_tune_report autotune(int frames_per_trial = 3) {
+        if (timestep != ts_ || substeps != sub_) {
+            check(rbgpu_set_step(h_, timestep, substeps));
+            ts_ = timestep;
+            sub_ = substeps;
+        }
+        rbgpu_tune_report r{};
+        check(rbgpu_autotune(h_, frames_per_trial, &r));
+        if (auto_sync_) sync_bodies();
+        return r;
+    }
 
     // JointCollection::target_position / target_speed are public fields in the reference.
     void set_joint_target(uint32_t joint_id, scalar target_position, scalar target_speed) {

@@ -218,6 +218,25 @@ int main() {
         CHECK(mag(w.bodies.linear_velocity[w.bodies.index_of(b)]) < 0.05);
         convex_hull_data_destroy(hd);
     }
+    {
+        // extension: the autotuner steps real frames; a twin stepped as many frames untuned
+        // ends in the same state
+        std::printf("autotune_keeps_the_trajectory\n");
+        World a(1.0 / 60.0, 10), b(1.0 / 60.0, 10);
+        for (World *w : {&a, &b}) {
+            add_static_box(*w, vec3(0.0, -2.0, 0.0));
+            for (int i = 0; i < 6; ++i) add_dynamic_sphere(*w, vec3(0.3 * i, 1.0 + 1.1 * i, 0.0));
+        }
+        a.step();
+        b.step();
+        const rbgpu_tune_report r = b.autotune(1);
+        CHECK(r.trials > 0 && r.frames == r.trials + 1);
+        for (int f = 0; f < r.frames; ++f) a.step();
+        for (uint32_t i = 1; i < 7; ++i) {
+            const vec3 pa = a.bodies.position[a.bodies.index_of(i)], pb = b.bodies.position[b.bodies.index_of(i)];
+            CHECK(pa.x == pb.x && pa.y == pb.y && pa.z == pb.z);
+        }
+    }
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "ALL PASSED", failures);
     return failures ? 1 : 0;
 }
