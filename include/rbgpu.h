@@ -209,7 +209,9 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          and per-colour launches; -1 auto (default: on for 16 or more
  *                          scenes of at most 4096 bodies), 0 off, 1 on.  Same colours, same
  *                          states.  Colouring errors then surface at the next step.
- *   RBGPU_OPT_SCENE_THREADS  threads per workgroup of the per-scene solve (64, 128, 256; 64)
+ *   RBGPU_OPT_SCENE_THREADS  threads per workgroup of the per-scene solve (64, 128, 256; 64).  The
+ *                          per-scene solves also run the scene's joints (timing classes 1 and 2
+ *                          then include them; class 10 stays empty).
  *   RBGPU_OPT_GJK_CULL     exact bounding-sphere rejection in front of GJK (1, default; 0 runs
  *                          GJK on every pair of the hull bins)
  *   RBGPU_OPT_GJK_GRID, RBGPU_OPT_EPA_GRID  workgroups of the two GJK/EPA passes (4096, 8192)

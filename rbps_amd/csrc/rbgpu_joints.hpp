@@ -141,9 +141,7 @@ __device__ __forceinline__ void joint_rows(int4 m, float4 r1, float4 r2, float4 
 }
 
 // compute_joint_errors (Joint.cpp:222-244) for every joint; mode 1 also resets lambda.
-__global__ void k_joint_errors(Bodies B, Joints J, uint32_t nj, float h, int mode) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nj) return;
+__device__ __forceinline__ void joint_errors(Bodies &B, Joints &J, uint32_t i, float h, int mode) {
     const int4 m = J.jm[i];
     const uint32_t b1 = (uint32_t)m.y, b2 = (uint32_t)m.z;
     float4 tg = J.jtg[i];
@@ -157,6 +155,10 @@ __global__ void k_joint_errors(Bodies B, Joints J, uint32_t nj, float h, int mod
         if (mode == 1) J.jlam[(size_t)i * 4 + k] = 0.0f;
     }
     J.jtg[i] = tg;
+}
+__global__ void k_joint_errors(Bodies B, Joints J, uint32_t nj, float h, int mode) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nj) joint_errors(B, J, i, h, mode);
 }
 
 // Body.cpp:103-109 (refreshes the world tensor)
@@ -264,6 +266,52 @@ __global__ void __launch_bounds__(256) k_joint_damp(Bodies B, Joints J, const ui
                                                     uint32_t count, float h) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < count) joint_damp(B, J, order[off + t], h);
+}
+
+// Colour-ordered solve of every scene, one workgroup per scene (POS: XPBD position solve;
+// otherwise the velocity solve).  Lanes iterate alike, so the point counter may reduce per
+// wave.  With per-scene joint tables the scene's joints follow in the same workgroup: after
+// the position solve their errors (Joint.cpp:222-244) and rows colour by colour
+// (Constraint.cpp:89-106), after the velocity solve their damping (Joint.cpp:249-267) — the
+// order of the global joint launches restricted to the scene, whose joints touch no other
+// scene's bodies.
+template <bool POS, int OCC = 1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
+k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pts, Joints J, float inv_h, int jmode) {
+    const uint32_t s = blockIdx.x;
+    const uint32_t nc = T.ncol[s];
+    const uint32_t *lst = T.list + T.base[s];
+    const uint32_t *co = T.coff + (size_t)s * (MAX_COLOURS + 1);
+    for (uint32_t c = 0; c < nc; ++c) {
+        const uint32_t o = co[c], e = co[c + 1];
+        for (uint32_t i0 = o; i0 < e; i0 += blockDim.x) {
+            const uint32_t i = i0 + threadIdx.x;
+            const bool v = i < e;
+            const uint32_t p = v ? lst[i] : 0u;
+            if (pts) count_points(K, p, v, pts);
+            if (v) {
+                if (POS) solve_pos_pair(B, K, p);
+                else solve_vel_pair(B, K, p, h);
+            }
+        }
+        __syncthreads();
+    }
+    if (!T.jorder) return;
+    const uint32_t j0 = T.jbase[s], j1 = T.jbase[s + 1];
+    if (j0 == j1) return;
+    const uint32_t *jc = T.jcoff + (size_t)s * (T.njc + 1);
+    if (POS) {
+        for (uint32_t i = j0 + threadIdx.x; i < j1; i += blockDim.x) joint_errors(B, J, T.jorder[i], h, jmode);
+        __syncthreads();
+    }
+    for (uint32_t c = 0; c < T.njc; ++c) {
+        const uint32_t e = j0 + jc[c + 1];
+        for (uint32_t i = j0 + jc[c] + threadIdx.x; i < e; i += blockDim.x) {
+            if (POS) joint_solve(B, J, T.jorder[i], inv_h, h, jmode);
+            else joint_damp(B, J, T.jorder[i], h);
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace rb

@@ -1031,6 +1031,10 @@ struct SceneTabs {
     uint32_t *coff;                          // per scene MAX_COLOURS + 1 offsets into its list
     uint32_t *ncol;                          // per scene: colours used
     uint32_t *err;                           // sticky: 1 colour capacity exceeded, 2 no convergence
+    // joints solved inside the per-scene solves (null: the global joint launches run instead):
+    // jorder[jbase[s] ..) the scene's joints by joint colour, jcoff[s * (njc + 1) + c] offsets
+    const uint32_t *jorder, *jbase, *jcoff;
+    uint32_t njc;
 };
 
 // Pairs per scene (SAP run + plane pairs), for the slice offsets.
@@ -1143,30 +1147,6 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         uint32_t e = ld_shared(&cb.ctr[CTR_ERR]) & 1u;
         if (left) e |= 2u;
         if (e) atomicOr(T.err, e);
-    }
-}
-
-// Colour-ordered solve of every scene, one workgroup per scene (pos: XPBD position solve;
-// otherwise the velocity solve).  Lanes iterate alike, so the point counter may reduce per wave.
-template <bool POS, int OCC = 1>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pts) {
-    const uint32_t s = blockIdx.x;
-    const uint32_t nc = T.ncol[s];
-    const uint32_t *lst = T.list + T.base[s];
-    const uint32_t *co = T.coff + (size_t)s * (MAX_COLOURS + 1);
-    for (uint32_t c = 0; c < nc; ++c) {
-        const uint32_t o = co[c], e = co[c + 1];
-        for (uint32_t i0 = o; i0 < e; i0 += blockDim.x) {
-            const uint32_t i = i0 + threadIdx.x;
-            const bool v = i < e;
-            const uint32_t p = v ? lst[i] : 0u;
-            if (pts) count_points(K, p, v, pts);
-            if (v) {
-                if (POS) solve_pos_pair(B, K, p);
-                else solve_vel_pair(B, K, p, h);
-            }
-        }
-        __syncthreads();
     }
 }
 

@@ -107,8 +107,10 @@ struct rbgpu_world {
     uint32_t max_scene_bodies = 0, scene_threads = 64;
     uint32_t *sc_plane_off = nullptr, *sc_plane_idx = nullptr, *sc_size = nullptr, *sc_base = nullptr;
     uint32_t *sc_work = nullptr, *sc_list = nullptr, *sc_coff = nullptr, *sc_ncol = nullptr, *sc_err = nullptr;
+    uint32_t *sj_order = nullptr, *sj_base = nullptr, *sj_coff = nullptr, sj_njc = 0;   // per-scene joint tables
     SceneTabs scene_tabs() const {
-        return SceneTabs{sap_start, sap_end, sc_plane_off, sc_plane_idx, sc_base, sc_work, sc_list, sc_coff, sc_ncol, sc_err};
+        return SceneTabs{sap_start, sap_end, sc_plane_off, sc_plane_idx, sc_base, sc_work, sc_list, sc_coff, sc_ncol, sc_err,
+                         sj_order, sj_base, sj_coff, sj_njc};
     }
     bool scene_path() const {
         return nscenes > 1 && (scene_mode == 1 || (scene_mode == -1 && nscenes >= 16 && max_scene_bodies <= 4096));
@@ -229,7 +231,8 @@ struct rbgpu_world {
         dfree(hv); dfree(hpn); dfree(hamin); dfree(hamax); dfree(htab); dfree(hmeta);
         hs = HullSet{};
         dfree(Jt.jm); dfree(Jt.jr1); dfree(Jt.jr2); dfree(Jt.jax); dfree(Jt.jlx); dfree(Jt.jtg); dfree(Jt.jrow);
-        dfree(Jt.jlam); dfree(jorder);
+        dfree(Jt.jlam); dfree(jorder); dfree(sj_order); dfree(sj_base); dfree(sj_coff);
+        sj_njc = 0;
         dfree(fmin); dfree(fmax); dfree(uid); dfree(ep_key[0]); dfree(ep_key[1]); dfree(ep_val[0]); dfree(ep_val[1]);
         dfree(ep_pos); dfree(sw_meta); dfree(sw_a); dfree(sw_b); dfree(ep_cnt); dfree(ep_off); dfree(ep_hist);
         dfree(flags); dfree(scal); dfree(tile_sums);
@@ -786,6 +789,33 @@ int rbgpu_world::build_joints() {
     HIPCHK(up(Jt.jax, jax.data(), NJ * 16)); HIPCHK(up(Jt.jlx, jlx.data(), NJ * 16));
     HIPCHK(up(Jt.jtg, h_jtg.data(), NJ * 16)); HIPCHK(up(Jt.jrow, jrow.data(), (size_t)NJ * 64));
     HIPCHK(up(Jt.jlam, h_jlam.data(), (size_t)NJ * 16)); HIPCHK(up(jorder, order.data(), NJ * 4));
+    // per-scene joint tables for the per-scene solves: the scene's joints by colour (the global
+    // order restricted to the scene); only when every joint stays inside one scene
+    bool inside = nscenes > 1;
+    std::vector<uint32_t> jsc(NJ);
+    for (uint32_t i = 0; i < NJ && inside; ++i) {
+        jsc[i] = hb[hj[i].body_1].scene;
+        inside = hb[hj[i].body_2].scene == jsc[i];
+    }
+    if (inside) {
+        std::vector<uint32_t> so(order);
+        std::stable_sort(so.begin(), so.end(), [&](uint32_t a, uint32_t b) { return jsc[a] < jsc[b]; });
+        std::vector<uint32_t> base(nscenes + 1, 0), coff((size_t)nscenes * (ncolj + 1), 0);
+        for (uint32_t i = 0; i < NJ; ++i) {
+            ++base[jsc[i] + 1];
+            ++coff[(size_t)jsc[i] * (ncolj + 1) + h_jcolour[i] + 1];
+        }
+        for (uint32_t k = 0; k < nscenes; ++k) {
+            base[k + 1] += base[k];
+            for (int c = 0; c < ncolj; ++c) coff[(size_t)k * (ncolj + 1) + c + 1] += coff[(size_t)k * (ncolj + 1) + c];
+        }
+        rc = 0;
+        rc |= dalloc(sj_order, NJ); rc |= dalloc(sj_base, nscenes + 1); rc |= dalloc(sj_coff, coff.size());
+        if (rc) return RBGPU_ERR_OOM;
+        HIPCHK(up(sj_order, so.data(), NJ * 4)); HIPCHK(up(sj_base, base.data(), (nscenes + 1) * 4));
+        HIPCHK(up(sj_coff, coff.data(), coff.size() * 4));
+        sj_njc = (uint32_t)ncolj;
+    }
     HIPCHK(hipStreamSynchronize(st));
     return RBGPU_OK;
 }
@@ -881,9 +911,15 @@ int rbgpu_world::frame() {
         HIPCHK(hipMemcpyAsync(rb->bin_off, offs, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
         if (sc_err) HIPCHK(hipMemcpyAsync(&rb->scene_err, sc_err, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        // errors of the per-scene colouring (sticky; found here, one frame after the fact)
-        if (sc_err && (rb->scene_err & 1u)) return fail(RBGPU_ERR_CAPACITY, "more than 1024 colours needed");
-        if (sc_err && (rb->scene_err & 2u)) return fail(RBGPU_ERR_CAPACITY, "colouring did not converge");
+        // errors of the per-scene colouring (sticky until reported here, one frame after the fact)
+        if (sc_err && rb->scene_err) {
+            const uint32_t e = rb->scene_err;
+            rb->scene_err = 0;
+            HIPCHK(hipMemsetAsync(sc_err, 0, 4, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (e & 1u) return fail(RBGPU_ERR_CAPACITY, "more than 1024 colours needed");
+            return fail(RBGPU_ERR_CAPACITY, "colouring did not converge");
+        }
         if (!rb->overflow) break;
         if (attempt > 0) return fail(RBGPU_ERR_CAPACITY, "pair buffer overflow");
         int rc2 = ensure_pair_capacity(rb->n_pairs);
@@ -908,6 +944,10 @@ int rbgpu_world::frame() {
     //    and the graph is launched for every substep.
     cb.pal_max = palette_max;
     cb.lid = pair_lid;
+    // joints inside the per-scene solves (which run only when the frame has pairs)
+    const bool scene_joints = by_scene && P && NJ && sj_order;
+    SceneTabs TJ = T;
+    if (!scene_joints) TJ.jorder = nullptr;
     auto substep = [&]() -> int {
         runs.clear();
         if (P) {
@@ -975,8 +1015,12 @@ int rbgpu_world::frame() {
         }
         if (by_scene && P) {
             hipEvent_t *e = timed_begin(1);
-            if (solve_occ == 4) k_scene_solve<true, 4><<<nscenes, scene_threads, 0, st>>>(B, K, T, h, e ? d_points : nullptr);
-            else k_scene_solve<true><<<nscenes, scene_threads, 0, st>>>(B, K, T, h, e ? d_points : nullptr);
+            if (solve_occ == 4)
+                k_scene_solve<true, 4><<<nscenes, scene_threads, 0, st>>>(B, K, TJ, h, e ? d_points : nullptr, Jt, inv_h,
+                                                                         joint_mode);
+            else
+                k_scene_solve<true><<<nscenes, scene_threads, 0, st>>>(B, K, TJ, h, e ? d_points : nullptr, Jt, inv_h,
+                                                                      joint_mode);
             timed_end(e);
         }
         for (const Run &u : runs) {
@@ -992,8 +1036,8 @@ int rbgpu_world::frame() {
             }
         }
         // joints: errors from the state after the contact solve, then rows per colour
-        // (World.hpp:166-171)
-        if (NJ) {
+        // (World.hpp:166-171); on the per-scene path they run inside k_scene_solve
+        if (NJ && !scene_joints) {
             hipEvent_t *e = timed_begin(10);
             k_joint_errors<<<nblk(NJ), TPB, 0, st>>>(B, Jt, NJ, h, joint_mode);
             for (size_t c = 0; c + 1 < jcol_off.size(); ++c) {
@@ -1009,7 +1053,8 @@ int rbgpu_world::frame() {
         }
         if (by_scene && P) {
             hipEvent_t *e = timed_begin(2);
-            k_scene_solve<false><<<nscenes, scene_threads, 0, st>>>(B, K, T, h, e ? d_points : nullptr);
+            k_scene_solve<false><<<nscenes, scene_threads, 0, st>>>(B, K, TJ, h, e ? d_points : nullptr, Jt, inv_h,
+                                                                   joint_mode);
             timed_end(e);
         }
         for (const Run &u : runs) {
@@ -1025,7 +1070,7 @@ int rbgpu_world::frame() {
             }
         }
         // joint damping (World.hpp:174-178)
-        if (NJ) {
+        if (NJ && !scene_joints) {
             hipEvent_t *e = timed_begin(10);
             for (size_t c = 0; c + 1 < jcol_off.size(); ++c) {
                 const uint32_t o = jcol_off[c], n = jcol_off[c + 1] - o;
