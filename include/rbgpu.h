@@ -221,6 +221,9 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *   RBGPU_OPT_GJK_OCCUPANCY  waves per EU the row GJK pass budgets its registers for (2, 3, 4)
  *   RBGPU_OPT_EPA_OCCUPANCY, RBGPU_OPT_SOLVE_OCCUPANCY  the same for the EPA pass and the
  *                          per-scene position solve (0 = no budget, 4)
+ *   RBGPU_OPT_SIDE_STREAM  run the narrowphase bins other than GJK/EPA on a second stream of
+ *                          the world, beside the GJK chain, joined before the colouring (1,
+ *                          default; 0: all on the world's stream).  Results do not change.
  *   RBGPU_OPT_GRAPH        with the per-scene kernels: capture one substep per frame in a
  *                          hipGraph and launch it for every substep (1, default; frames with
  *                          kernel timing or an open training log run without it)
@@ -249,7 +252,8 @@ enum {
     RBGPU_OPT_EPA_SUB = 15,
     RBGPU_OPT_GJK_OCCUPANCY = 16,
     RBGPU_OPT_EPA_OCCUPANCY = 17,
-    RBGPU_OPT_SOLVE_OCCUPANCY = 18
+    RBGPU_OPT_SOLVE_OCCUPANCY = 18,
+    RBGPU_OPT_SIDE_STREAM = 19
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);

@@ -134,6 +134,11 @@ struct rbgpu_world {
     uint32_t epa_sub = 16;                              // EPA pass: listed pairs per work item
     int gjk_occ = 4;                                    // GJK row pass: waves per EU the registers are budgeted for
     int epa_occ = 0, solve_occ = 0;                     // EPA pass / per-scene position solve: 4 = budget for 4 waves
+    // the narrowphase bins other than GJK run on a second stream beside the GJK chain (default
+    // priority, joined back by an event before the colouring)
+    bool side_stream = true;
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint32_t *gjk_hits = nullptr, *gjk_blk = nullptr;   // overlapping GJK pairs per 4-pair workgroup
     float4 *gjk_sx = nullptr;                          // their final simplices (8 float4 each)
     // learned warm start: parameters (FP32 master, BF16 copy), gradient, Adam moments, the
@@ -247,6 +252,11 @@ struct rbgpu_world {
         gexec = nullptr;
         if (rb) (void)hipHostFree(rb);
         rb = nullptr;
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        ev_fork = ev_join = nullptr;
+        if (st2) (void)hipStreamDestroy(st2);
+        st2 = nullptr;
         if (st) (void)hipStreamDestroy(st);
         st = nullptr;
         built = false;
@@ -416,39 +426,49 @@ struct rbgpu_world {
         return RBGPU_OK;
     }
 
-    // Narrowphase of every bin of the current pair list (bin_off): one launch per bin.
+    // Narrowphase of every bin of the current pair list (bin_off): one launch per bin.  The
+    // GJK chain (both GJK bins) goes first on the main stream; with side_stream and pairs in
+    // both, the other bins run meanwhile on st2 (their slots and scratch are disjoint from the
+    // chain's) and the main stream waits for them at the end.
     int launch_narrowphase() {
+        const uint32_t g0 = bin_off[BIN_HH], ng = bin_off[BIN_HP] - bin_off[BIN_HH];
+        const bool fork = side_stream && st2 && ng && bin_off[NBINS] - ng > 0;
+        if (fork) {
+            HIPCHK(hipEventRecord(ev_fork, st));
+            HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
+        }
+        if (ng) {
+            const int rc = launch_gjk(g0, ng);
+            if (rc) return rc;
+        }
+        const hipStream_t so = fork ? st2 : st;
         for (int b = 0; b < NBINS; ++b) {
             uint32_t o = bin_off[b], n = bin_off[b + 1] - bin_off[b];
-            if (!n) continue;
+            if (!n || b == BIN_HH || b == BIN_HX) continue;
             const uint32_t g = nblk(n, 128);
             switch (b) {
-                case BIN_SS: k_narrow<BIN_SS><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                case BIN_SB: k_narrow<BIN_SB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                case BIN_SS: k_narrow<BIN_SS><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
+                case BIN_SB: k_narrow<BIN_SB><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
                 case BIN_BB:
-                    k_narrow_bb_sat<<<g, 128, 0, st>>>(bin_list, o, n, C, B, K, bb_hits, bb_blk);
-                    k_narrow_bb_full<<<g, 128, 0, st>>>(bb_hits, bb_blk, C, B, K);
+                    k_narrow_bb_sat<<<g, 128, 0, so>>>(bin_list, o, n, C, B, K, bb_hits, bb_blk);
+                    k_narrow_bb_full<<<g, 128, 0, so>>>(bb_hits, bb_blk, C, B, K);
                     break;
-                case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K, contact_mode); break;
-                case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                case BIN_SP: k_narrow<BIN_SP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
-                case BIN_HH:
-                case BIN_HX:
-                    // both GJK bins in one go, at the first non-empty one
-                    if (b == BIN_HH || bin_off[BIN_HH] == bin_off[BIN_HX]) {
-                        const int rc = launch_gjk(bin_off[BIN_HH], bin_off[BIN_HP] - bin_off[BIN_HH]);
-                        if (rc) return rc;
-                    }
-                    break;
+                case BIN_SC: k_narrow<BIN_SC><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
+                case BIN_CC: k_narrow<BIN_CC><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K, contact_mode); break;
+                case BIN_CB: k_narrow<BIN_CB><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
+                case BIN_SP: k_narrow<BIN_SP><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
+                case BIN_BP: k_narrow<BIN_BP><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
+                case BIN_CP: k_narrow<BIN_CP><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
                 case BIN_HP:
-                    if (hull_vpl == 1) k_narrow_hull_plane<1><<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs);
-                    else k_narrow_hull_plane<HULL_VPL><<<n, 64, 0, st>>>(bin_list, o, C, B, K, hs);
+                    if (hull_vpl == 1) k_narrow_hull_plane<1><<<n, 64, 0, so>>>(bin_list, o, C, B, K, hs);
+                    else k_narrow_hull_plane<HULL_VPL><<<n, 64, 0, so>>>(bin_list, o, C, B, K, hs);
                     break;
-                default: k_narrow<BIN_OTHER><<<g, 128, 0, st>>>(bin_list, o, n, C, B, K); break;
+                default: k_narrow<BIN_OTHER><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
             }
+        }
+        if (fork) {
+            HIPCHK(hipEventRecord(ev_join, st2));
+            HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
         }
         return RBGPU_OK;
     }
@@ -641,6 +661,9 @@ int rbgpu_world::build() {
 
     release();
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void **)&rb, sizeof(Readback), hipHostMallocDefault));
     N = n; M = m; NEP = 2 * S;
     int rc = 0;
@@ -1527,6 +1550,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_SOLVE_OCCUPANCY:
             if (value != 0 && value != 4) return fail(RBGPU_ERR_INVALID, "solve occupancy: 0 (no budget) or 4");
             w->solve_occ = value; return RBGPU_OK;
+        case RBGPU_OPT_SIDE_STREAM:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "side stream is 0 or 1");
+            w->side_stream = value != 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1557,6 +1583,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_GJK_OCCUPANCY: *value = w->gjk_occ; return RBGPU_OK;
         case RBGPU_OPT_EPA_OCCUPANCY: *value = w->epa_occ; return RBGPU_OK;
         case RBGPU_OPT_SOLVE_OCCUPANCY: *value = w->solve_occ; return RBGPU_OK;
+        case RBGPU_OPT_SIDE_STREAM: *value = w->side_stream ? 1 : 0; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }

@@ -145,6 +145,35 @@ def test_gpu_substep_graph_equals_direct_launches():
     assert b.stats()["scene_kernels"] == 1
 
 
+@pytest.mark.parametrize("scene_kernels,graph", [(0, 0), (1, 0), (1, 1)])
+def test_gpu_side_stream_equals_one_stream(scene_kernels, graph):
+    """The narrowphase bins other than GJK/EPA on a second stream beside the GJK chain,
+    against every bin on the world's stream (grid path, per-scene path, per-scene path in a
+    hipGraph): bit-identical states and the same contact counts every frame."""
+    lib = S.hull_library()
+    parts = _parts() + [S.combined_scene(n, seed=70 + k, library=lib) for k, n in enumerate((64, 96, 128))] * 3
+    bt = S.batch(parts)
+    print("case side stream scene_kernels", scene_kernels, "graph", graph, flush=True)
+    a, b = _gpu(bt, contact_mode=1), _gpu(bt, contact_mode=1)
+    for w, side in ((a, 0), (b, 1)):
+        w.set_option("scene_kernels", scene_kernels)
+        w.set_option("graph", graph)
+        w.set_option("side_stream", side)
+    assert b.option("side_stream") == 1 and a.option("side_stream") == 0
+    forked = 0   # frames with pairs both in the GJK bins and in the others
+    for f in range(60):
+        a.step(1)
+        b.step(1)
+        sa, sb = a.state(), b.state()
+        for k in sa:
+            assert np.array_equal(sa[k], sb[k]), (f + 1, k)
+        ta, tb = a.stats(), b.stats()
+        assert (ta["contacts"], ta["active_pairs"]) == (tb["contacts"], tb["active_pairs"]), (f + 1, ta, tb)
+        gjk = tb["bins"][9] + tb["bins"][10]
+        forked += gjk > 0 and sum(tb["bins"]) > gjk
+    assert tb["contacts"] > 0 and forked > 30
+
+
 def test_gpu_autotune_keeps_the_trajectory():
     """The autotuner times result-neutral options over real frames: the tuned world's
     states equal an untuned world's after the same number of frames, and the chosen values
