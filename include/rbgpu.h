@@ -178,10 +178,12 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
 
 /* Kernel timing with HIP events around every launch of one kernel family on the world's
  * stream.  kernel: 0 off, 1 position solve (per-colour launches), 2 velocity solve
- * (per-colour launches), 3 narrowphase, 4 integrate, 5 velocity update, 6 broadphase
+ * (per-colour launches), 3 narrowphase (with the colouring state's reset, which runs
+ * beside it), 4 integrate, 5 velocity update, 6 broadphase
  * (refit, endpoint sort, sweep, pair list; one span per frame), 7 colouring (one span per
  * substep), 8 position solve runs of small colours (single workgroup), 9 velocity solve
- * runs, 10 joints, 11 the GJK pass (k_gjk_pass, GJK on the candidates of the hull bins).  rbgpu_get_timing
+ * runs, 10 joints (empty when the per-scene solves run the joints), 11 the GJK pass
+ * (k_gjk_pass, GJK on the candidates of the hull bins).  rbgpu_get_timing
  * returns the summed milliseconds and launches since the last reset (synchronises);
  * bytes = algorithmic bytes of the timed launches for 1, 2 and 11. */
 /* Pipeline options (no counterpart in the reference, whose step is sequential).
@@ -222,8 +224,10 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *   RBGPU_OPT_EPA_OCCUPANCY, RBGPU_OPT_SOLVE_OCCUPANCY  the same for the EPA pass and the
  *                          per-scene position solve (0 = no budget, 4)
  *   RBGPU_OPT_SIDE_STREAM  run the narrowphase bins other than GJK/EPA on a second stream of
- *                          the world, beside the GJK chain, joined before the colouring (1,
- *                          default; 0: all on the world's stream).  Results do not change.
+ *                          the world, beside the GJK chain, joined before the colouring; the
+ *                          value is where it forks: 1 before the GJK chain, 2 after its
+ *                          rejection tests (default), 3 after the GJK pass; 0: all on the
+ *                          world's stream.  Results do not change.
  *   RBGPU_OPT_GRAPH        with the per-scene kernels: capture one substep per frame in a
  *                          hipGraph and launch it for every substep (1, default; frames with
  *                          kernel timing or an open training log run without it)

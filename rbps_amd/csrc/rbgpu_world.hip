@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 #include "../../include/rbgpu.h"
@@ -136,7 +137,16 @@ struct rbgpu_world {
     int epa_occ = 0, solve_occ = 0;                     // EPA pass / per-scene position solve: 4 = budget for 4 waves
     // the narrowphase bins other than GJK run on a second stream beside the GJK chain (default
     // priority, joined back by an event before the colouring)
-    bool side_stream = true;
+    // 0: off; where the side stream forks: 1 before the GJK chain, 2 after its rejection tests
+    // and compaction, 3 after the GJK pass (beside the EPA pass)
+    int side_stream = 2;
+    std::function<int()> *fork_hook = nullptr;   // set by launch_narrowphase while launch_gjk runs
+    int fork_at(int stage) {
+        if (!fork_hook || stage < side_stream) return RBGPU_OK;
+        std::function<int()> *f = fork_hook;
+        fork_hook = nullptr;
+        return (*f)();
+    }
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint32_t *gjk_hits = nullptr, *gjk_blk = nullptr;   // overlapping GJK pairs per 4-pair workgroup
@@ -312,6 +322,7 @@ struct rbgpu_world {
         float4 *tl = gjk_log ? ds_tgt : nullptr;   // rows after *ds_cnt
         const bool timed = timing_kernel == 11 && frames % timing_stride == 0;
         const uint32_t *cand = nullptr, *ncand = nullptr;
+        if (int rcf = fork_at(1)) return rcf;
         if (gjk_cull && !gjk_explicit) {
             k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep);
             int rc = scan(gjk_keep, gjk_pos, ng, gjk_ncand);
@@ -321,6 +332,7 @@ struct rbgpu_world {
             ncand = gjk_ncand;
             if (timed) k_add_count<<<1, 1, 0, st>>>(gjk_ncand, d_points);
         }
+        if (int rcf = fork_at(2)) return rcf;
         if (gjk_log) k_gjk_features<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, ds_feat, ds_cnt);
         if (infer) {
             k_gjk_features<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, inf_feat, nullptr);
@@ -349,6 +361,7 @@ struct rbgpu_world {
                 k_gjk_pass<4, 16><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
                                                                       tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
             timed_end(e);
+            if (int rcf = fork_at(3)) return rcf;
             if (epa_occ == 4)
                 k_epa_pass<1, 4><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
                                                     ds_cnt, gjk_hits, gjk_blk, gjk_sx);
@@ -359,6 +372,7 @@ struct rbgpu_world {
             k_gjk_pass<1, 64><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
                                                                   tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
             timed_end(e);
+            if (int rcf = fork_at(3)) return rcf;
             k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
                                              gjk_hits, gjk_blk, gjk_sx);
         } else {
@@ -366,6 +380,7 @@ struct rbgpu_world {
                                                                          gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx,
                                                                          gjk_axis);
             timed_end(e);
+            if (int rcf = fork_at(3)) return rcf;
             k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
                                                     ds_cnt, gjk_hits, gjk_blk, gjk_sx);
         }
@@ -429,19 +444,37 @@ struct rbgpu_world {
     // Narrowphase of every bin of the current pair list (bin_off): one launch per bin.  The
     // GJK chain (both GJK bins) goes first on the main stream; with side_stream and pairs in
     // both, the other bins run meanwhile on st2 (their slots and scratch are disjoint from the
-    // chain's) and the main stream waits for them at the end.
-    int launch_narrowphase() {
+    // chain's) and the main stream waits for them at the end.  colour_reset: also clear the
+    // colouring state there (it depends on no narrowphase result).
+    int launch_narrowphase(bool colour_reset = false) {
         const uint32_t g0 = bin_off[BIN_HH], ng = bin_off[BIN_HP] - bin_off[BIN_HH];
         const bool fork = side_stream && st2 && ng && bin_off[NBINS] - ng > 0;
+        std::function<int()> side = [&]() { return launch_side(fork, colour_reset); };
+        if (ng) {
+            fork_hook = fork ? &side : nullptr;
+            int rc = launch_gjk(g0, ng);
+            std::function<int()> *left = fork_hook;
+            fork_hook = nullptr;
+            if (!rc && left) rc = side();   // the chain ended before the fork point
+            if (rc) return rc;
+            if (fork) {   // join once the whole chain is queued
+                HIPCHK(hipEventRecord(ev_join, st2));
+                HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
+                return RBGPU_OK;
+            }
+        }
+        return launch_side(false, colour_reset);
+    }
+    // The narrowphase bins other than GJK (and the colouring state's reset): on st2, forked
+    // from the main stream (launch_narrowphase joins it back), or on the main stream.
+    int launch_side(bool fork, bool colour_reset) {
         if (fork) {
             HIPCHK(hipEventRecord(ev_fork, st));
             HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
         }
-        if (ng) {
-            const int rc = launch_gjk(g0, ng);
-            if (rc) return rc;
-        }
         const hipStream_t so = fork ? st2 : st;
+        if (colour_reset)
+            k_colour_reset<<<nblk(std::max<uint32_t>(N * (COLOUR_WORDS / 4), MAX_COLOURS)), TPB, 0, so>>>(N, cb);
         for (int b = 0; b < NBINS; ++b) {
             uint32_t o = bin_off[b], n = bin_off[b + 1] - bin_off[b];
             if (!n || b == BIN_HH || b == BIN_HX) continue;
@@ -465,10 +498,6 @@ struct rbgpu_world {
                     break;
                 default: k_narrow<BIN_OTHER><<<g, 128, 0, so>>>(bin_list, o, n, C, B, K); break;
             }
-        }
-        if (fork) {
-            HIPCHK(hipEventRecord(ev_join, st2));
-            HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
         }
         return RBGPU_OK;
     }
@@ -975,12 +1004,11 @@ int rbgpu_world::frame() {
         runs.clear();
         if (P) {
             hipEvent_t *e = timed_begin(3);
-            int rcn = launch_narrowphase();
+            int rcn = launch_narrowphase(true);   // with the colouring state cleared beside it
             if (rcn) return rcn;
             timed_end(e);
             // colouring of the pairs with points (conflict = a shared dynamic body)
             hipEvent_t *ec = timed_begin(7);
-            k_colour_reset<<<nblk(std::max<uint32_t>(N * (COLOUR_WORDS / 4), MAX_COLOURS)), TPB, 0, st>>>(N, cb);
             if (by_scene) {   // per scene: rounds, then the colour-ordered lists (no host round trip)
                 k_scene_colour<<<nscenes, SCENE_THREADS, 0, st>>>(K.cnt, K.bb, scal + 0, B, cb, T, colour_rounds_cap);
                 timed_end(ec);
@@ -1551,8 +1579,8 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
             if (value != 0 && value != 4) return fail(RBGPU_ERR_INVALID, "solve occupancy: 0 (no budget) or 4");
             w->solve_occ = value; return RBGPU_OK;
         case RBGPU_OPT_SIDE_STREAM:
-            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "side stream is 0 or 1");
-            w->side_stream = value != 0; return RBGPU_OK;
+            if (value < 0 || value > 3) return fail(RBGPU_ERR_INVALID, "side stream: 0 (off), 1, 2 or 3 (fork point)");
+            w->side_stream = value; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1583,7 +1611,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_GJK_OCCUPANCY: *value = w->gjk_occ; return RBGPU_OK;
         case RBGPU_OPT_EPA_OCCUPANCY: *value = w->epa_occ; return RBGPU_OK;
         case RBGPU_OPT_SOLVE_OCCUPANCY: *value = w->solve_occ; return RBGPU_OK;
-        case RBGPU_OPT_SIDE_STREAM: *value = w->side_stream ? 1 : 0; return RBGPU_OK;
+        case RBGPU_OPT_SIDE_STREAM: *value = w->side_stream; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }

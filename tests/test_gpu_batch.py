@@ -145,21 +145,22 @@ def test_gpu_substep_graph_equals_direct_launches():
     assert b.stats()["scene_kernels"] == 1
 
 
-@pytest.mark.parametrize("scene_kernels,graph", [(0, 0), (1, 0), (1, 1)])
-def test_gpu_side_stream_equals_one_stream(scene_kernels, graph):
-    """The narrowphase bins other than GJK/EPA on a second stream beside the GJK chain,
-    against every bin on the world's stream (grid path, per-scene path, per-scene path in a
-    hipGraph): bit-identical states and the same contact counts every frame."""
+@pytest.mark.parametrize("scene_kernels,graph,side", [(0, 0, 2), (1, 0, 1), (1, 1, 3), (1, 1, 1)])
+def test_gpu_side_stream_equals_one_stream(scene_kernels, graph, side):
+    """The narrowphase bins other than GJK/EPA on a second stream beside the GJK chain
+    (forked before it, after its rejection tests or after the GJK pass), against every bin on
+    the world's stream (grid path, per-scene path, per-scene path in a hipGraph):
+    bit-identical states and the same contact counts every frame."""
     lib = S.hull_library()
     parts = _parts() + [S.combined_scene(n, seed=70 + k, library=lib) for k, n in enumerate((64, 96, 128))] * 3
     bt = S.batch(parts)
-    print("case side stream scene_kernels", scene_kernels, "graph", graph, flush=True)
+    print("case side stream", side, "scene_kernels", scene_kernels, "graph", graph, flush=True)
     a, b = _gpu(bt, contact_mode=1), _gpu(bt, contact_mode=1)
-    for w, side in ((a, 0), (b, 1)):
+    for w, v in ((a, 0), (b, side)):
         w.set_option("scene_kernels", scene_kernels)
         w.set_option("graph", graph)
-        w.set_option("side_stream", side)
-    assert b.option("side_stream") == 1 and a.option("side_stream") == 0
+        w.set_option("side_stream", v)
+    assert b.option("side_stream") == side and a.option("side_stream") == 0
     forked = 0   # frames with pairs both in the GJK bins and in the others
     for f in range(60):
         a.step(1)
