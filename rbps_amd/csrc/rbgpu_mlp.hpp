@@ -99,6 +99,37 @@ __device__ __forceinline__ void dense16(const bf16 *in, int ld_in, int K, const 
     }
 }
 
+// The same layer for RT row tiles of 16 (rows 16 t + ..): each weight fragment is loaded
+// once and used for every tile; each output sees the same MFMA sequence as in dense16.
+template <bool RELU, int RT, class TO>
+__device__ __forceinline__ void dense_rows(const bf16 *in, int ld_in, int K, const bf16 *W, const float *bias, int N,
+                                           TO *out, int ld_out) {
+    const int l = lane64(), r = l & 15, kq = (l >> 4) * 8;
+    for (int nb = 0; nb < N; nb += 16) {
+        floatx4 acc[RT];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(W + (nb + r) * K + k0 + kq);
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const bf16x8 av = *reinterpret_cast<const bf16x8 *>(in + (16 * t + r) * ld_in + k0 + kq);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+        const int n = nb + r;
+        const float bn = bias[n];
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float z = acc[t][j] + bn;
+                if (RELU) z = z > 0.0f ? z : 0.0f;
+                out[(16 * t + 4 * (l >> 4) + j) * ld_out + n] = (TO)z;
+            }
+    }
+}
+
 // Forward of a wave's 16 samples: x [16][32] -> a1, a2, a3 [16][128] (BF16) -> out [16][16].
 __device__ __forceinline__ void mlp_forward16(const MlpParams &P, const bf16 *x, bf16 *a1, bf16 *a2, bf16 *a3, float *out) {
     dense16<true>(x, MLP_IN, MLP_IN, P.wb + MLP_W1, P.wf + MLP_B1, MLP_H, a1, MLP_H);
@@ -112,9 +143,11 @@ __device__ __forceinline__ void mlp_forward16(const MlpParams &P, const bf16 *x,
 }
 
 // Inference over n samples (features BF16 [n][32]); out: per sample two float4, the
-// normalised GJK direction and EPA seed normal (zero vectors when degenerate).
-__global__ void __launch_bounds__(256) k_mlp_infer(const bf16 *feat, uint32_t n_max, const uint32_t *n_dev, MlpParams P,
-                                                   float4 *out) {
+// normalised GJK direction and EPA seed normal (zero vectors when degenerate).  Two
+// wavefronts of MLP_INFER_THREADS, each with two 16-sample tiles of the block's MLP_TILE.
+constexpr int MLP_INFER_RT = 2, MLP_INFER_THREADS = 64 * MLP_TILE / (16 * MLP_INFER_RT);
+__global__ void __launch_bounds__(MLP_INFER_THREADS) k_mlp_infer(const bf16 *feat, uint32_t n_max, const uint32_t *n_dev,
+                                                                 MlpParams P, float4 *out) {
     __shared__ __attribute__((aligned(16))) bf16 x[MLP_TILE * MLP_IN], h1[MLP_TILE * MLP_H], h2[MLP_TILE * MLP_H];
     __shared__ float o[MLP_TILE * MLP_OUT];
     const int wave = threadIdx.x >> 6, l = lane64();
@@ -125,17 +158,18 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const bf16 *feat, uint32_t n_
             x[i] = s < n ? feat[(size_t)t0 * MLP_IN + i] : (bf16)0.0f;
         }
         __syncthreads();
-        const int r0 = wave * 16;
+        constexpr int RT = MLP_INFER_RT;
+        const int r0 = wave * 16 * RT;
         bf16 *xa = x + r0 * MLP_IN, *a1 = h1 + r0 * MLP_H, *a2 = h2 + r0 * MLP_H;
-        dense16<true>(xa, MLP_IN, MLP_IN, P.wb + MLP_W1, P.wf + MLP_B1, MLP_H, a1, MLP_H);
+        dense_rows<true, RT>(xa, MLP_IN, MLP_IN, P.wb + MLP_W1, P.wf + MLP_B1, MLP_H, a1, MLP_H);
         __syncthreads();
-        dense16<true>(a1, MLP_H, MLP_H, P.wb + MLP_W2, P.wf + MLP_B2, MLP_H, a2, MLP_H);
+        dense_rows<true, RT>(a1, MLP_H, MLP_H, P.wb + MLP_W2, P.wf + MLP_B2, MLP_H, a2, MLP_H);
         __syncthreads();
-        dense16<true>(a2, MLP_H, MLP_H, P.wb + MLP_W3, P.wf + MLP_B3, MLP_H, a1, MLP_H);
+        dense_rows<true, RT>(a2, MLP_H, MLP_H, P.wb + MLP_W3, P.wf + MLP_B3, MLP_H, a1, MLP_H);
         __syncthreads();
-        dense16<false>(a1, MLP_H, MLP_H, P.wb + MLP_W4, P.wf + MLP_B4, MLP_OUT, o + r0 * MLP_OUT, MLP_OUT);
+        dense_rows<false, RT>(a1, MLP_H, MLP_H, P.wb + MLP_W4, P.wf + MLP_B4, MLP_OUT, o + r0 * MLP_OUT, MLP_OUT);
         __syncthreads();
-        if (l < 16) {
+        if (l < 16 * RT) {
             const uint32_t s = t0 + r0 + l;
             if (s < n) {
                 const float *q = o + (r0 + l) * MLP_OUT;

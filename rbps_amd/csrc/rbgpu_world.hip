@@ -337,7 +337,8 @@ struct rbgpu_world {
         if (infer) {
             k_gjk_features<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, inf_feat, nullptr);
             const uint32_t tiles = (ng + MLP_TILE - 1) / MLP_TILE;
-            k_mlp_infer<<<std::min<uint32_t>(tiles, 1024), 256, 0, st>>>(inf_feat, ng, ncand, MlpParams{mlp_wb, mlp_wf},
+            k_mlp_infer<<<std::min<uint32_t>(tiles, 1024), MLP_INFER_THREADS, 0, st>>>(inf_feat, ng, ncand,
+                                                                                    MlpParams{mlp_wb, mlp_wf},
                                                                          inf_guess);
         }
         const float4 *gp = (gjk_explicit || infer) ? inf_guess : nullptr;
@@ -2035,7 +2036,8 @@ int rbgpu_mlp_eval(rbgpu_world *w, uint32_t n, const float *feat32, float *out6)
     if (!rc) {
         (void)hipMemcpyAsync(f, feat32, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, st);
         k_f32_to_bf16<<<nblk(n * MLP_IN), TPB, 0, st>>>(f, fb, n * MLP_IN);
-        k_mlp_infer<<<std::min<uint32_t>((n + MLP_TILE - 1) / MLP_TILE, 1024), 256, 0, st>>>(fb, n, nullptr, MlpParams{w->mlp_wb, w->mlp_wf}, o);
+        k_mlp_infer<<<std::min<uint32_t>((n + MLP_TILE - 1) / MLP_TILE, 1024), MLP_INFER_THREADS, 0, st>>>(
+            fb, n, nullptr, MlpParams{w->mlp_wb, w->mlp_wf}, o);
         std::vector<float4> h(2 * (size_t)n);
         (void)hipMemcpyAsync(h.data(), o, 2 * (size_t)n * 16, hipMemcpyDeviceToHost, st);
         if (hipStreamSynchronize(st) != hipSuccess) rc = fail(RBGPU_ERR_HIP, "rbgpu_mlp_eval failed");
