@@ -456,13 +456,15 @@ struct rbgpu_world {
             int rc = launch_gjk(g0, ng);
             std::function<int()> *left = fork_hook;
             fork_hook = nullptr;
-            if (!rc && left) rc = side();   // the chain ended before the fork point
-            if (rc) return rc;
-            if (fork) {   // join once the whole chain is queued
-                HIPCHK(hipEventRecord(ev_join, st2));
-                HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
-                return RBGPU_OK;
+            const bool forked = fork && !left;   // the hook ran (and forked) inside the chain
+            if (!rc && left) rc = side();        // the chain ended before the fork point
+            if (forked || (fork && left && !rc)) {
+                // join once the whole chain is queued, also when the chain failed after the fork
+                const hipError_t e1 = hipEventRecord(ev_join, st2);
+                const hipError_t e2 = hipStreamWaitEvent(st, ev_join, 0);
+                if (!rc && (e1 != hipSuccess || e2 != hipSuccess)) rc = fail(RBGPU_ERR_HIP, "side stream join failed");
             }
+            if (rc || fork) return rc;
         }
         return launch_side(false, colour_reset);
     }
