@@ -228,6 +228,10 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          value is where it forks: 1 before the GJK chain, 2 after its
  *                          rejection tests (default), 3 after the GJK pass; 0: all on the
  *                          world's stream.  Results do not change.
+ *   RBGPU_OPT_AXIS_CARRY   carry the GJK rejection hints (the axis that last separated a pair)
+ *                          across the frame's new pair list, keyed by collider pair (1,
+ *                          default; 0: a new list's slot keeps whatever hint it held).  Results
+ *                          do not change.
  *   RBGPU_OPT_GRAPH        with the per-scene kernels: capture one substep per frame in a
  *                          hipGraph and launch it for every substep (1, default; frames with
  *                          kernel timing or an open training log run without it)
@@ -257,7 +261,8 @@ enum {
     RBGPU_OPT_GJK_OCCUPANCY = 16,
     RBGPU_OPT_EPA_OCCUPANCY = 17,
     RBGPU_OPT_SOLVE_OCCUPANCY = 18,
-    RBGPU_OPT_SIDE_STREAM = 19
+    RBGPU_OPT_SIDE_STREAM = 19,
+    RBGPU_OPT_AXIS_CARRY = 20
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);

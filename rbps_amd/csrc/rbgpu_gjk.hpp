@@ -1156,6 +1156,66 @@ __device__ __forceinline__ float support_extent(const Colliders &C, const HullSe
     return e;
 }
 
+// Separating-axis hints across a frame's new pair list: the old list's hints of its GJK
+// slots go into an open-addressing table keyed by collider pair (linear probing, at most
+// half full), and the new list's GJK slots look their pair up.  A slot whose pair is absent
+// keeps what it held (a hint proves nothing by itself: k_gjk_cull tests it exactly).
+// The slot of a pair changes with the sweep order, so without this the first substep of a
+// frame would mostly test other pairs' axes.
+constexpr unsigned long long AXIS_EMPTY = ~0ull;
+__device__ __forceinline__ uint32_t axis_hash(unsigned long long k, uint32_t mask) {
+    // MurmurHash3's 64-bit finaliser
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return (uint32_t)k & mask;
+}
+__device__ __forceinline__ unsigned long long axis_key(const Contacts &K, uint32_t s) {
+    const uint2 cc = K.cc[s];
+    return ((unsigned long long)cc.x << 32) | cc.y;
+}
+__global__ void k_axis_clear(unsigned long long *keys, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = AXIS_EMPTY;
+}
+// slots: the old list's GJK slots (each < cap, the pair buffers' capacity; others are skipped)
+__global__ void k_axis_save(const uint32_t *slots, uint32_t n, uint32_t cap, Contacts K, const float4 *axis,
+                            unsigned long long *keys, float4 *vals, uint32_t mask) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = slots[i];
+    if (s >= cap) return;
+    const float4 a = axis[s];
+    if (a.w == 0.0f) return;
+    const unsigned long long key = axis_key(K, s);
+    uint32_t h = axis_hash(key, mask);
+    for (uint32_t p = 0; p <= mask; ++p, h = (h + 1u) & mask) {
+        const unsigned long long prev = atomicCAS(&keys[h], AXIS_EMPTY, key);
+        if (prev == AXIS_EMPTY || prev == key) {
+            vals[h] = a;
+            return;
+        }
+    }
+}
+__global__ void k_axis_load(const uint32_t *slots, uint32_t n, Contacts K, const unsigned long long *keys,
+                            const float4 *vals, uint32_t mask, float4 *axis) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = slots[i];
+    const unsigned long long key = axis_key(K, s);
+    uint32_t h = axis_hash(key, mask);
+    for (uint32_t p = 0; p <= mask; ++p, h = (h + 1u) & mask) {
+        const unsigned long long k = keys[h];
+        if (k == key) {
+            axis[s] = vals[h];
+            return;
+        }
+        if (k == AXIS_EMPTY) return;
+    }
+}
+
 // Exact rejection in front of GJK.  Two tests on the line between the collider origins:
 // bounding spheres (radius about each origin in lpos.w, rounded up) apart, or the two
 // shapes' support extents along that line apart (a separating plane).  Either way the

@@ -128,6 +128,11 @@ struct rbgpu_world {
     uint32_t *gstat = nullptr;
     uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
     float4 *gjk_axis = nullptr;   // per pair slot: the axis that last separated it (rejection hint)
+    // the hints carried across a frame's new pair list (table keyed by collider pair)
+    unsigned long long *ax_keys = nullptr;
+    float4 *ax_vals = nullptr;
+    uint32_t ax_cap = 0;
+    bool axis_carry = true;
     int contact_mode = 0;                               // 1: capsule-capsule normal from A to B
     bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
@@ -253,6 +258,10 @@ struct rbgpu_world {
         dfree(flags); dfree(scal); dfree(tile_sums);
         dfree(cb.used); dfree(cb.ucnt); dfree(cb.claim); dfree(cb.hist); dfree(cb.off); dfree(cb.cur); dfree(cb.ctr);
         dfree(plane_pairs); dfree(offs); dfree(d_points);
+        dfree(ax_keys); dfree(ax_vals);
+        ax_cap = 0;
+        n_pairs = n_sap = 0;   // no pair list survives the buffers
+        bin_off.clear();
         dfree(cscene); dfree(slot_scene); dfree(sap_start); dfree(sap_end); dfree(plane_lid);
         dfree(sc_plane_off); dfree(sc_plane_idx); dfree(sc_size); dfree(sc_base); dfree(sc_coff); dfree(sc_ncol);
         dfree(sc_err);
@@ -944,7 +953,26 @@ int rbgpu_world::frame() {
     }
     // 3. pair list: SAP pairs (canonical order) then the plane pairs; per-pair data and
     //    narrowphase bins.  One host synchronisation per frame: the pair count (capacity
-    //    check) and the bin offsets.
+    //    check) and the bin offsets.  The GJK slots' rejection hints are saved by collider
+    //    pair first and restored into the new list's slots after it.
+    const uint32_t ng_old =
+        n_pairs && !hh.empty() && bin_off.size() == NBINS + 1 ? bin_off[BIN_HP] - bin_off[BIN_HH] : 0u;
+    const bool carry = axis_carry && gjk_cull && gjk_axis && ng_old;
+    uint32_t amask = 0;
+    if (carry) {
+        uint32_t H = 1;
+        while (H < 2 * ng_old) H <<= 1;
+        if (H > ax_cap) {
+            dfree(ax_keys); dfree(ax_vals);
+            ax_cap = 0;
+            if (dalloc(ax_keys, H) | dalloc(ax_vals, H)) return RBGPU_ERR_OOM;
+            ax_cap = H;
+        }
+        amask = H - 1;
+        k_axis_clear<<<nblk(H), TPB, 0, st>>>(ax_keys, H);
+        k_axis_save<<<nblk(ng_old), TPB, 0, st>>>(bin_list + bin_off[BIN_HH], ng_old, pcap, K, gjk_axis, ax_keys, ax_vals,
+                                                   amask);
+    }
     for (int attempt = 0;; ++attempt) {
         HIPCHK(hipMemsetAsync(overflow, 0, 4, st));
         if (NEP)
@@ -984,6 +1012,10 @@ int rbgpu_world::frame() {
     n_sap = rb->n_sap;
     n_pairs = rb->n_pairs;
     bin_off.assign(rb->bin_off, rb->bin_off + NBINS + 1);
+    if (carry && bin_off[BIN_HP] > bin_off[BIN_HH])
+        k_axis_load<<<nblk(bin_off[BIN_HP] - bin_off[BIN_HH]), TPB, 0, st>>>(bin_list + bin_off[BIN_HH],
+                                                                               bin_off[BIN_HP] - bin_off[BIN_HH], K, ax_keys,
+                                                                               ax_vals, amask, gjk_axis);
     const uint32_t P = n_pairs;
     const uint32_t colour_rounds_cap = 1u << 16;
     const bool by_scene = scene_path();
@@ -1584,6 +1616,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_SIDE_STREAM:
             if (value < 0 || value > 3) return fail(RBGPU_ERR_INVALID, "side stream: 0 (off), 1, 2 or 3 (fork point)");
             w->side_stream = value; return RBGPU_OK;
+        case RBGPU_OPT_AXIS_CARRY:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "axis carry is 0 or 1");
+            w->axis_carry = value != 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1615,6 +1650,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_EPA_OCCUPANCY: *value = w->epa_occ; return RBGPU_OK;
         case RBGPU_OPT_SOLVE_OCCUPANCY: *value = w->solve_occ; return RBGPU_OK;
         case RBGPU_OPT_SIDE_STREAM: *value = w->side_stream; return RBGPU_OK;
+        case RBGPU_OPT_AXIS_CARRY: *value = w->axis_carry ? 1 : 0; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }

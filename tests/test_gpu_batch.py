@@ -175,6 +175,33 @@ def test_gpu_side_stream_equals_one_stream(scene_kernels, graph, side):
     assert tb["contacts"] > 0 and forked > 30
 
 
+def test_gpu_gjk_rejection_and_carried_hints_change_nothing():
+    """The exact rejection tests in front of GJK, with and without the separating-axis hints
+    carried across each frame's new pair list, against every hull pair through GJK:
+    bit-identical states every frame; the hints only take pairs away from GJK (fewer GJK
+    iterations with the cull, fewer again with the carried hints)."""
+    lib = S.hull_library()
+    parts = [S.combined_scene(n, seed=60 + k, library=lib) for k, n in enumerate((64, 96, 128, 80))] * 2
+    bt = S.batch(parts)
+    print("case gjk rejection scenes", len(parts), flush=True)
+    ws = [_gpu(bt, contact_mode=1) for _ in range(3)]
+    for w, cull, carry in zip(ws, (0, 1, 1), (0, 0, 1)):
+        w.set_option("gjk_cull", cull)
+        w.set_option("axis_carry", carry)
+        w.gjk_totals(reset=True)
+    for f in range(50):
+        for w in ws:
+            w.step(1)
+        s0 = ws[0].state()
+        for w in ws[1:]:
+            sw = w.state()
+            for k in s0:
+                assert np.array_equal(s0[k], sw[k]), (f + 1, k)
+    it = [w.gjk_totals()["gjk_iters"] for w in ws]
+    print("GJK iterations: all pairs, cull, cull + carried hints", it, flush=True)
+    assert it[0] > 0 and it[2] <= it[1] <= it[0]
+
+
 def test_gpu_autotune_keeps_the_trajectory():
     """The autotuner times result-neutral options over real frames: the tuned world's
     states equal an untuned world's after the same number of frames, and the chosen values
