@@ -210,7 +210,13 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          each per substep, no host round trip) instead of grid-wide rounds
  *                          and per-colour launches; -1 auto (default: on for 16 or more
  *                          scenes of at most 4096 bodies), 0 off, 1 on.  Same colours, same
- *                          states.  Colouring errors then surface at the next step.
+ *                          states.  A colouring error (more than 1024 colours, or no
+ *                          convergence) is then read back after the frame that raised it: by
+ *                          the next frame of the same rbgpu_step call, or, for the call's last
+ *                          frames, by the next call that waits for the world (rbgpu_synchronize,
+ *                          rbgpu_get_bodies/_sap_pairs/_pair_colours/_contacts/_stats, or the
+ *                          next rbgpu_step), which returns RBGPU_ERR_CAPACITY.  Past 1024
+ *                          colours the frame's colouring is not valid.
  *   RBGPU_OPT_SCENE_THREADS  threads per workgroup of the per-scene solve (64, 128, 256; 64).  The
  *                          per-scene solves also run the scene's joints (timing classes 1 and 2
  *                          then include them; class 10 stays empty).

@@ -134,6 +134,8 @@ def oracle_lib():
         lib.orc_set_palette_max.argtypes = [_P, C.c_int]
         lib.orc_set_joint_mode.argtypes = [_P, C.c_int]
         lib.orc_set_contact_mode.argtypes = [_P, C.c_int]
+        lib.orc_set_planted_fault.argtypes = [_P, C.c_int]
+        lib.orc_get_solve_order.argtypes = [_P, _U]
         lib.orc_joint_errors.argtypes = [C.c_int, _P, _D, _D, C.c_double, _D]
         for f in ("orc_sinf", "orc_cosf"):
             getattr(lib, f).restype = C.c_float
@@ -264,6 +266,11 @@ class OracleWorld(_WorldBase):
         """0: contacts as the reference makes them; 1: capsule-capsule normal from A to B."""
         self.lib.orc_set_contact_mode(self.h, int(mode))
 
+    def plant_fault(self, kind):
+        """Plants a colour-schedule mistake (1: the last colour's pairs skipped, 2: the
+        first pair solved twice) so tests can show their pins catch it."""
+        self.lib.orc_set_planted_fault(self.h, int(kind))
+
     def set_palette_max(self, m):
         if self.lib.orc_set_palette_max(self.h, m) != 0:
             raise ValueError("palette maximum must be >= 1")
@@ -274,8 +281,11 @@ class OracleWorld(_WorldBase):
         kind = np.zeros(n, np.int32)
         if n:
             self.lib.orc_get_solve_pairs(self.h, _ptr(cc, _U), _ptr(col, _I), _ptr(cnt, _I), _ptr(kind, _I))
+        order = np.zeros(self.lib.orc_get_solve_order(self.h, None), np.uint32)
+        if len(order):
+            self.lib.orc_get_solve_order(self.h, _ptr(order, _U))
         return {"colliders": cc, "colour": col, "count": cnt, "kind": kind, "ncolours": self.lib.orc_num_colours(self.h),
-                "rounds": self.lib.orc_colour_rounds(self.h)}
+                "rounds": self.lib.orc_colour_rounds(self.h), "order": order.astype(np.int64)}
 
 
 class RefWorld(_WorldBase):

@@ -87,6 +87,8 @@ struct Any {
     virtual int ncolours() const = 0;
     virtual int colour_rounds() const = 0;
     virtual void set_palette_max(uint32_t m) = 0;
+    virtual void set_planted_fault(int f) = 0;
+    virtual int solve_order(uint32_t *out) const = 0;
     virtual void set_joint_mode(int m) = 0;
     virtual void set_contact_mode(int m) = 0;
     virtual void set_state(const double *p, const double *q, const double *v, const double *w) = 0;
@@ -182,6 +184,11 @@ template <class T> struct Impl : Any {
     int ncolours() const override { return w.ncolours; }
     int colour_rounds() const override { return (int)w.colour_rounds; }
     void set_palette_max(uint32_t m) override { w.palette_max = m; }
+    void set_planted_fault(int f) override { w.planted_fault = f; }
+    int solve_order(uint32_t *out) const override {
+        if (out) std::copy(w.solve_order.begin(), w.solve_order.end(), out);
+        return (int)w.solve_order.size();
+    }
     void set_joint_mode(int m) override { w.joint_mode = m; }
     void set_contact_mode(int m) override { w.contact_mode = m; }
     void add_joint(const orc_joint_desc &d) override { w.add_joint(joint_from<T>(d), T(d.compliance)); }
@@ -373,6 +380,10 @@ int orc_set_palette_max(void *h, int m) {
     return 0;
 }
 uint32_t orc_mix32(uint32_t x) { return orc::mix32(x); }
+// Test infrastructure: plants a colour-schedule mistake (see World::planted_fault).
+void orc_set_planted_fault(void *h, int f) { static_cast<Any *>(h)->set_planted_fault(f); }
+// The last substep's colour-order solve list (pair indices of the frame's pair list).
+int orc_get_solve_order(void *h, uint32_t *out) { return static_cast<Any *>(h)->solve_order(out); }
 
 int orc_collide(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b, const double *tb,
                 double *out) {

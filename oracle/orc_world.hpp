@@ -801,9 +801,17 @@ template <class T> struct World {
         solve_order.clear();
         for (auto &k : keyed) solve_order.push_back(k.second);
     }
+    // Planted schedule mistakes, for tests that show the colour-order pins catch them:
+    // 1 skips the pairs of the last colour, 2 solves the first pair twice.  0 (default): none.
+    int planted_fault = 0;
     template <class F> void for_each_colour_contact(F &&f) {
-        for (uint32_t p : solve_order)
-            for (int k = 0; k < pcount[p]; ++k) f(pcont[(size_t)p * 4 + k]);
+        const int last = solve_order.empty() ? -1 : colour[solve_order.back()];
+        for (uint32_t p : solve_order) {
+            if (planted_fault == 1 && colour[p] == last) continue;
+            const int reps = planted_fault == 2 && p == solve_order.front() ? 2 : 1;
+            for (int r = 0; r < reps; ++r)
+                for (int k = 0; k < pcount[p]; ++k) f(pcont[(size_t)p * 4 + k]);
+        }
     }
 
     // ------------------------------------------------------------ World::step (World.hpp:192-212)

@@ -66,6 +66,7 @@ struct Readback {
     uint64_t points;
     uint32_t log_rows;
     uint32_t scene_err;
+    uint32_t scene_err_tail;   // sc_err as queued after the last frame of an rbgpu_step call
 };
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -438,6 +439,34 @@ struct rbgpu_world {
         return RBGPU_OK;
     }
     uint64_t graph_frames = 0;
+
+    // Per-scene colouring errors of the last frames of an rbgpu_step call: the call queues a
+    // copy of the sticky flag behind its work; calls that wait for the stream report it.
+    bool err_tail_queued = false;
+    int queue_error_check() {
+        if (!sc_err || !rb || !scene_path()) return RBGPU_OK;
+        HIPCHK(hipMemcpyAsync(&rb->scene_err_tail, sc_err, 4, hipMemcpyDeviceToHost, st));
+        err_tail_queued = true;
+        return RBGPU_OK;
+    }
+    // After a synchronisation of the world's stream.
+    int pending_error() {
+        if (!err_tail_queued) return RBGPU_OK;
+        err_tail_queued = false;
+        const uint32_t e = rb->scene_err_tail;
+        rb->scene_err_tail = 0;
+        if (!e) return RBGPU_OK;
+        HIPCHK(hipMemsetAsync(sc_err, 0, 4, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (e & 1u) return fail(RBGPU_ERR_CAPACITY, "more than 1024 colours needed (a frame of the last step)");
+        return fail(RBGPU_ERR_CAPACITY, "colouring did not converge (a frame of the last step)");
+    }
+    // Waits for the world's stream and reports a pending colouring error.
+    int sync() {
+        if (!st) return RBGPU_OK;
+        HIPCHK(hipStreamSynchronize(st));
+        return pending_error();
+    }
 
     // Rows in the training log (device count -> ds_n, ds_hi).
     int sync_log_count() {
@@ -994,6 +1023,7 @@ int rbgpu_world::frame() {
         HIPCHK(hipMemcpyAsync(rb->bin_off, offs, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
         if (sc_err) HIPCHK(hipMemcpyAsync(&rb->scene_err, sc_err, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        err_tail_queued = false;   // the flag read just now covers the earlier frames
         // errors of the per-scene colouring (sticky until reported here, one frame after the fact)
         if (sc_err && rb->scene_err) {
             const uint32_t e = rb->scene_err;
@@ -1328,7 +1358,7 @@ int rbgpu_step(rbgpu_world *w, int n_frames) {
         if (rc) return rc;
     }
     HIPCHK(hipGetLastError());
-    return RBGPU_OK;
+    return w->queue_error_check();
 }
 
 int rbgpu_autotune(rbgpu_world *w, int frames_per_trial, rbgpu_tune_report *out) {
@@ -1411,8 +1441,7 @@ int rbgpu_set_step(rbgpu_world *w, double timestep, int substeps) {
 
 int rbgpu_synchronize(rbgpu_world *w) {
     if (!w) return fail(RBGPU_ERR_INVALID, "null world");
-    if (w->st) HIPCHK(hipStreamSynchronize(w->st));
-    return RBGPU_OK;
+    return w->sync();
 }
 
 int rbgpu_get_bodies(rbgpu_world *w, double *p, double *q, double *v, double *av) {
@@ -1431,7 +1460,7 @@ int rbgpu_get_bodies(rbgpu_world *w, double *p, double *q, double *v, double *av
         }
         return RBGPU_OK;
     }
-    HIPCHK(hipStreamSynchronize(w->st));
+    if (int rcs = w->sync()) return rcs;
     std::vector<float4> buf(n);
     auto pull = [&](float4 *src, double *dst, bool quat) -> int {
         if (!dst || !n) return RBGPU_OK;
@@ -1471,7 +1500,7 @@ int rbgpu_get_sap_pairs(rbgpu_world *w, uint32_t *pairs2, uint32_t cap, uint32_t
     if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     *n = w->built ? w->n_sap : 0;
     if (!w->built || !pairs2) return RBGPU_OK;
-    HIPCHK(hipStreamSynchronize(w->st));
+    if (int rcs = w->sync()) return rcs;
     uint32_t k = std::min(cap, w->n_sap);
     if (k) HIPCHK(hipMemcpy(pairs2, w->pairs, (size_t)k * 8, hipMemcpyDeviceToHost));
     return RBGPU_OK;
@@ -1481,7 +1510,7 @@ int rbgpu_get_pair_colours(rbgpu_world *w, uint32_t *pairs2, int32_t *colour, ui
     if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     *n = w->built ? w->n_pairs : 0;
     if (!w->built) return RBGPU_OK;
-    HIPCHK(hipStreamSynchronize(w->st));
+    if (int rcs = w->sync()) return rcs;
     const uint32_t k = std::min(cap, w->n_pairs);
     if (k && pairs2) HIPCHK(hipMemcpy(pairs2, w->pairs, (size_t)k * 8, hipMemcpyDeviceToHost));
     if (k && colour) {
@@ -1498,7 +1527,7 @@ int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *n3, double *ra3, 
     if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     *n = 0;
     if (!w->built || !w->n_pairs) return RBGPU_OK;
-    HIPCHK(hipStreamSynchronize(w->st));
+    if (int rcs = w->sync()) return rcs;
     const uint32_t P = w->n_pairs;
     std::vector<int> cnt(P);
     std::vector<uint2> cc(P), bb(P);
@@ -1537,7 +1566,7 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out) {
     out->graph_frames = (uint32_t)w->graph_frames;
     out->scene_kernels = w->built && w->scene_path() ? 1u : 0u;
     if (w->built) {
-        HIPCHK(hipStreamSynchronize(w->st));
+        if (int rcs = w->sync()) return rcs;
         uint32_t sc[CTR_SIZE];
         HIPCHK(hipMemcpy(sc, w->cb.ctr, sizeof(sc), hipMemcpyDeviceToHost));
         out->colours = sc[CTR_NCOL];

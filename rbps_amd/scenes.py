@@ -280,6 +280,50 @@ def pile_small(seed=12345, nx=10, ny=10, nz=10, spacing=1.2, y0=1.0, jitter=0.05
     return sb.build(f"pile_small_{nx}x{ny}x{nz}", 1.0 / 60.0, substeps, frames, seed=seed)
 
 
+def resting_layer(seed=300, n=1000, cols=32, spacing=1.6, drop=(0.02, 0.25), speed=0.2, density=1000.0,
+                  friction=0.5, restitution=0.0, substeps=20, frames=300) -> Scene:
+    """The small plain configuration's bodies (unit boxes, half 0.5, and spheres, r 0.5,
+    by index parity; density 1000; mu 0.5; e 0; dt 1/60; 20 substeps; 300 frames) in a
+    well-posed arrangement: one layer on a `cols`-wide grid (spacing 1.6 m, so no two
+    bodies can touch).  Boxes start flat on the plane (turned about the vertical axis
+    only) and slide with a horizontal velocity of at most `speed` m/s until friction
+    stops them within millimetres; spheres drop from 0.02-0.25 m above their resting
+    height with no horizontal velocity.  A box dropped flat would be ill-posed (rounding
+    decides which corner lands first) and a sphere rolling sideways would never stop, so
+    neither is used.  No contact pair shares a dynamic body, so any solve order gives the
+    same result."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    add_ground_plane(sb, friction, restitution)
+    rows = (n + cols - 1) // cols
+    for k in range(n):
+        i, j = k % cols, k // cols
+        x0, z0 = (i - (cols - 1) / 2) * spacing, (j - (rows - 1) / 2) * spacing
+        drop_k = float(rng.uniform(*drop))
+        yaw = float(rng.uniform(-math.pi, math.pi))
+        vx, vz = (float(x) for x in rng.uniform(-speed, speed, 2))
+        if k % 2 == 0:
+            h = (0.5, 0.5, 0.5)
+            q = (math.cos(yaw / 2), 0.0, math.sin(yaw / 2), 0.0)
+            b = sb.body(DYNAMIC, density * box_volume(h), tuple(density * x for x in box_inertia(h)), (x0, 0.5, z0), q,
+                        (vx, 0.0, vz))
+            sb.collider(BOX, b, h, restitution=restitution, static_friction=friction, dynamic_friction=friction)
+        else:
+            r = 0.5
+            b = sb.body(DYNAMIC, density * sphere_volume(r), tuple(density * x for x in sphere_inertia(r)),
+                        (x0, r + drop_k, z0))
+            sb.collider(SPHERE, b, (r,), restitution=restitution, static_friction=friction, dynamic_friction=friction)
+    return sb.build(f"resting_layer_{n}", 1.0 / 60.0, substeps, frames, seed=seed)
+
+
+def accuracy_batch(n_scenes=50, per_scene=20, cols=5, seed=400) -> Scene:
+    """The well-posed accuracy workload: 1,000 bodies of the small plain configuration as
+    50 independent resting layers of 20 bodies (5 x 4 grid each, so every coordinate stays
+    within a few metres of its scene's origin), one batched world."""
+    return batch([resting_layer(seed=seed + s, n=per_scene, cols=cols) for s in range(n_scenes)],
+                 f"resting_layers_{n_scenes}x{per_scene}")
+
+
 def pile_mixed(seed=2024, nx=64, ny=16, nz=64, spacing=2.2, y0=1.5, jitter=0.05, density=1000.0,
                friction=0.5, restitution=0.0, substeps=10, frames=500) -> Scene:
     """Mixed convex pile: boxes (half extents U(0.2,0.6)), spheres (r U(0.2,0.5)) and
@@ -511,6 +555,31 @@ def batch(parts, name="batch") -> Scene:
     return Scene(name, np.concatenate(bodies), np.concatenate(cols), first.timestep, first.substeps, first.frames,
                  {"scenes": len(parts), "bodies_per_scene": [len(p.bodies) for p in parts]},
                  np.concatenate(joints) if joints else np.zeros(0, JOINT_DTYPE), hulls)
+
+
+def split(sc: Scene, only=None) -> list:
+    """The scenes of a batch (see `batch`) as worlds of their own, in scene order (or the
+    scene indices in `only`): bodies, colliders and joints re-indexed, the batch's hull
+    list kept whole."""
+    out = []
+    scene = sc.bodies["scene"]
+    body_of = sc.colliders["body_id"].astype(np.int64)
+    for s in (range(int(scene.max()) + 1 if len(scene) else 0) if only is None else only):
+        sel = np.nonzero(scene == s)[0]
+        if not len(sel):
+            continue
+        b0, b1 = int(sel[0]), int(sel[-1]) + 1   # a batch keeps each scene's bodies contiguous
+        b = sc.bodies[b0:b1].copy()
+        b["scene"] = 0
+        c = sc.colliders[(body_of >= b0) & (body_of < b1)].copy()
+        c["body_id"] -= np.uint32(b0)
+        j = sc.joints[(sc.joints["body_1"] >= b0) & (sc.joints["body_1"] < b1)].copy() if len(sc.joints) else sc.joints
+        if len(j):
+            j["body_1"] -= np.uint32(b0)
+            j["body_2"] -= np.uint32(b0)
+        out.append(Scene(f"{sc.name}_scene{s}", b, c, sc.timestep, sc.substeps, sc.frames, {"scene": s}, j,
+                         list(sc.hulls)))
+    return out
 
 
 def batched_stack_chains(n_scenes=4096, seed=77, n_min=32, n_max=512) -> Scene:

@@ -228,6 +228,56 @@ def api_scene_fixture(name):
     print(name, "written")
 
 
+ACCURACY_FRAMES = (1, 10, 30, 100, 300)
+
+
+def accuracy_fixture(name):
+    """The reference's FP64 trajectory of the well-posed accuracy batch
+    (scenes.accuracy_batch): every scene run as a World of its own (the reference has no
+    scenes), body positions at ACCURACY_FRAMES in the batch's body order."""
+    sc = scenes.accuracy_batch()
+    parts = scenes.split(sc)
+    out = {"bodies": sc.bodies, "colliders": sc.colliders, "frames": np.array(ACCURACY_FRAMES)}
+    pos = {f: [] for f in ACCURACY_FRAMES}
+    for sub in parts:
+        w = orc.RefWorld(sub.timestep, sub.substeps).load(sub)
+        done = 0
+        for f in ACCURACY_FRAMES:
+            w.step(f - done)
+            done = f
+            pos[f].append(w.state()["position"])
+    for f in ACCURACY_FRAMES:
+        out[f"position_{f}"] = np.concatenate(pos[f])
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+    print(name, "written", out["position_300"].shape)
+
+
+def cpu_ratio_fixture(name, frames=20):
+    """Speed of the oracle restatement (FP64, reference order) against the reference
+    engine compiled from its sources, on the small plain configuration (1,000 bodies,
+    frames 1..frames), one core each: the factor by which an on-box timing of the
+    restatement converts to the reference engine's rate."""
+    import json
+    import time
+    sc = scenes.pile_small()
+    rates = {}
+    for kind in ("reference", "port"):
+        best = 0.0
+        for _ in range(3):
+            w = (orc.RefWorld if kind == "reference" else orc.OracleWorld)(sc.timestep, sc.substeps).load(sc)
+            t0 = time.perf_counter()
+            w.step(frames)
+            best = max(best, sc.n_dynamic * sc.substeps * frames / (time.perf_counter() - t0))
+        rates[kind] = best
+    out = {"workload": "pile_small 10x10x10 (1,000 bodies, 20 substeps), frames 1..%d, one core, best of 3" % frames,
+           "reference_body_substeps_per_s": round(rates["reference"], 1),
+           "port_body_substeps_per_s": round(rates["port"], 1),
+           "port_over_reference": round(rates["port"] / rates["reference"], 4)}
+    with open(os.path.join(OUT, name + ".json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(name, out)
+
+
 if __name__ == "__main__":
     if not orc.ref_available():
         sys.exit("reference build missing: run `make -C oracle ref` first")
@@ -241,3 +291,6 @@ if __name__ == "__main__":
     hull_pairs_fixture("hull_pairs")
     world_fixture("world_hull_zoo", scenes.hull_zoo(), [1, 10, 60])
     world_fixture("world_hull_pile_64", scenes.hull_pile(64, seed=3), [1, 30])
+    world_fixture("world_resting_layer_100", scenes.resting_layer(n=100, cols=10), [1, 30, 60])
+    accuracy_fixture("accuracy_resting_layers")
+    cpu_ratio_fixture("cpu_ratio_pile_small")

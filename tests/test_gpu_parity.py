@@ -75,8 +75,10 @@ def test_gpu_one_frame_vs_fp64():
     """One frame vs FP64.  The precision gap is measured against the FP64 colour-order oracle
     (same order, double arithmetic): bounded by FP32 resolution over 20 substeps of an
     interpenetrating pile (observed 9e-5 m).  The gap to the FP64 *sequential* reference
-    (golden) is algorithmic (Gauss-Seidel order) and only reported; on a non-overlapping
-    start it is zero (tests/test_oracle_invariants.py)."""
+    (golden) is algorithmic (Gauss-Seidel order) and only reported; on a scene where no two
+    contact pairs share a dynamic body the two orders agree bit for bit in FP64
+    (tests/test_oracle_colour_order.py), and tests/test_gpu_accuracy.py reports the GPU's
+    divergence from the reference engine over 300 frames there."""
     g_ = golden("world_pile_small_4x4x4")
     sc = scenes.Scene("p", g_["bodies"], g_["colliders"], float(g_["timestep"]), int(g_["substeps"]))
     w = _world(sc)
