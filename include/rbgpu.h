@@ -296,6 +296,18 @@ int rbgpu_get_timing(rbgpu_world *w, double *total_ms, uint64_t *launches, doubl
 int rbgpu_collide_pairs(int n, const int32_t *kind_a, const double *dims_a, const double *tf_a, const int32_t *kind_b,
                         const double *dims_b, const double *tf_b, double *out);
 
+/* The same narrowphase as rbgpu_collide_pairs, same arguments and output, but it allocates
+ * nothing: no hipMalloc/hipFree or other allocating call.  No counterpart in the reference;
+ * for callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * it).  Inputs and results are staged in one fixed device array per GPU (8.5 MB, set up by
+ * the first call on that GPU), so one call takes at most 16384 pairs (RBGPU_ERR_CAPACITY
+ * beyond that).  Every call on a GPU shares that GPU's array and writes every element it
+ * reads, so nothing carries over between calls; calls that overlap on one GPU (from several
+ * threads) race on it and must be serialised by the caller.  Runs on the null stream and
+ * returns when the result is in `out`. */
+int rbgpu_collide_pairs_noalloc(int n, const int32_t *kind_a, const double *dims_a, const double *tf_a,
+                                const int32_t *kind_b, const double *dims_b, const double *tf_b, double *out);
+
 /* ---- convex hulls ------------------------------------------------------------------
  * rbc::convex_hull_data_create (include/rbc/shapes/ConvexHull.hpp:149-420): vertices
  * (3 doubles each, local frame) and optional CCW outward triangles; the world copies them,

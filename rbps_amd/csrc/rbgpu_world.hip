@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/rbgpu.h"
@@ -1762,6 +1763,96 @@ int rbgpu_collide_pairs(int n, const int32_t *kind_a, const double *dims_a, cons
     if (rc) return rc;
     if (e != hipSuccess) return fail(RBGPU_ERR_HIP, std::string("rbgpu_collide_pairs: ") + hipGetErrorString(e));
     for (size_t k = 0; k < fo.size(); ++k) out[k] = fo[k];
+    return RBGPU_OK;
+}
+
+// ---------------------------------------------------------------- narrowphase without allocation
+// rbgpu_collide_pairs_noalloc stages its inputs and results in `scratch`, a file-scope device
+// array of fixed size, instead of allocating per call.  Host code never passes `scratch`
+// itself to a kernel or a copy (its name is a host-side stand-in there): each device's
+// address of it comes from hipGetSymbolAddress, once per device.
+namespace {
+constexpr uint32_t NOALLOC_MAX_PAIRS = 16384;
+// per pair: kinds 2 x 4 B, dims 2 x 32 B and poses 2 x 56 B as passed in (doubles), six staged
+// float4 rows 96 B, the float result 80 B and its double copy 160 B
+constexpr size_t NA_KA = 0, NA_KB = 4, NA_DA = 8, NA_DB = 40, NA_TA = 72, NA_TB = 128, NA_STAGE = 184, NA_FOUT = 280,
+                 NA_DOUT = 360, NA_PER_PAIR = 520;
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[(size_t)NOALLOC_MAX_PAIRS * NA_PER_PAIR];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+struct NoAllocDevice {
+    std::once_flag once;
+    unsigned char *ptr = nullptr;
+    hipError_t err = hipSuccess;
+};
+NoAllocDevice g_noalloc[NOALLOC_MAX_DEVICES];
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+int noalloc_scratch(unsigned char **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(RBGPU_ERR_UNSUPPORTED, "device index beyond 64");
+    NoAllocDevice &d = g_noalloc[dev];
+    std::call_once(d.once, [&d]() { d.err = hipGetSymbolAddress((void **)&d.ptr, HIP_SYMBOL(scratch)); });
+    if (d.err != hipSuccess || !d.ptr)
+        return fail(RBGPU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(d.err));
+    *out = d.ptr;
+    return RBGPU_OK;
+}
+
+// doubles as passed in -> the float4 rows k_collide_pairs reads (round to nearest, as the
+// host conversion of rbgpu_collide_pairs)
+__global__ void k_stage_pairs(uint32_t n, const double *dims_a, const double *tf_a, const double *dims_b,
+                              const double *tf_b, float4 *da, float4 *db, float4 *pa, float4 *ra, float4 *pb, float4 *rb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *x = dims_a + 4 * (size_t)i, *y = dims_b + 4 * (size_t)i;
+    da[i] = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+    db[i] = make_float4((float)y[0], (float)y[1], (float)y[2], (float)y[3]);
+    const double *a = tf_a + 7 * (size_t)i, *b = tf_b + 7 * (size_t)i;
+    pa[i] = make_float4((float)a[0], (float)a[1], (float)a[2], 0.0f);
+    ra[i] = make_float4((float)a[3], (float)a[4], (float)a[5], (float)a[6]);
+    pb[i] = make_float4((float)b[0], (float)b[1], (float)b[2], 0.0f);
+    rb[i] = make_float4((float)b[3], (float)b[4], (float)b[5], (float)b[6]);
+}
+
+__global__ void k_widen(uint32_t n, const float *in, double *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
+}
+}  // namespace
+
+extern "C" int rbgpu_collide_pairs_noalloc(int n, const int32_t *kind_a, const double *dims_a, const double *tf_a,
+                                           const int32_t *kind_b, const double *dims_b, const double *tf_b,
+                                           double *out) {
+    if (n < 0 || (n && (!kind_a || !dims_a || !tf_a || !kind_b || !dims_b || !tf_b || !out)))
+        return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    if ((uint32_t)n > NOALLOC_MAX_PAIRS)
+        return fail(RBGPU_ERR_CAPACITY, "rbgpu_collide_pairs_noalloc: more than 16384 pairs in one call");
+    if (n == 0) return RBGPU_OK;
+    unsigned char *p = nullptr;
+    int rc = noalloc_scratch(&p);
+    if (rc) return rc;
+    const size_t M = NOALLOC_MAX_PAIRS, un = (size_t)n;
+    int *ka = (int *)(p + NA_KA * M), *kb = (int *)(p + NA_KB * M);
+    double *da = (double *)(p + NA_DA * M), *db = (double *)(p + NA_DB * M);
+    double *ta = (double *)(p + NA_TA * M), *tb = (double *)(p + NA_TB * M);
+    float4 *st = (float4 *)(p + NA_STAGE * M);   // six rows of M float4 each
+    float *fo = (float *)(p + NA_FOUT * M);
+    double *dout = (double *)(p + NA_DOUT * M);
+    HIPCHK(hipMemcpy(ka, kind_a, un * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(kb, kind_b, un * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(da, dims_a, un * 32, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(db, dims_b, un * 32, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ta, tf_a, un * 56, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(tb, tf_b, un * 56, hipMemcpyHostToDevice));
+    k_stage_pairs<<<nblk(n), TPB>>>(n, da, ta, db, tb, st, st + M, st + 2 * M, st + 3 * M, st + 4 * M, st + 5 * M);
+    k_collide_pairs<<<nblk(n), TPB>>>(n, ka, st, st + 2 * M, st + 3 * M, kb, st + M, st + 4 * M, st + 5 * M, fo);
+    k_widen<<<nblk(20 * n), TPB>>>(20 * n, fo, dout);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dout, un * 160, hipMemcpyDeviceToHost));
     return RBGPU_OK;
 }
 

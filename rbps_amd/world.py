@@ -29,6 +29,7 @@ EXPORTS = [
     "rbgpu_world_create", "rbgpu_world_destroy", "rbgpu_add_bodies", "rbgpu_add_colliders", "rbgpu_step",
     "rbgpu_set_step", "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
     "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_collide_pairs",
+    "rbgpu_collide_pairs_noalloc",
     "rbgpu_autotune",
     "rbgpu_set_option", "rbgpu_get_option", "rbgpu_add_joints", "rbgpu_get_joints", "rbgpu_set_joint_targets",
     "rbgpu_add_hull", "rbgpu_hull_mass_properties", "rbgpu_hull_tables", "rbgpu_collide_world_pairs",
@@ -86,6 +87,7 @@ def lib():
         L.rbgpu_get_option.argtypes = [_P, C.c_int, _I]
         L.rbgpu_get_timing.argtypes = [_P, _D, C.POINTER(C.c_uint64), _D]
         L.rbgpu_collide_pairs.argtypes = [C.c_int, _I, _D, _D, _I, _D, _D, _D]
+        L.rbgpu_collide_pairs_noalloc.argtypes = [C.c_int, _I, _D, _D, _I, _D, _D, _D]
         L.rbgpu_add_hull.argtypes = [_P, _D, C.c_uint32, _U, C.c_uint32, _U]
         L.rbgpu_hull_mass_properties.argtypes = [_D, C.c_uint32, _U, C.c_uint32, _D, _D]
         L.rbgpu_hull_tables.argtypes = [_D, C.c_uint32, _U, C.c_uint32, _U, _D] + [_U] * 8
@@ -116,15 +118,20 @@ def _ptr(a, t):
     return a.ctypes.data_as(t) if a is not None else None
 
 
-def collide_pairs(kind_a, dims_a, tf_a, kind_b, dims_b, tf_b) -> np.ndarray:
-    """Single-pair narrowphase of a batch on the GPU; rows = n, normal, (point, depth) x 4."""
+NOALLOC_MAX_PAIRS = 16384
+
+
+def collide_pairs(kind_a, dims_a, tf_a, kind_b, dims_b, tf_b, noalloc=False) -> np.ndarray:
+    """Single-pair narrowphase of a batch on the GPU; rows = n, normal, (point, depth) x 4.
+    noalloc: through rbgpu_collide_pairs_noalloc (fixed device staging, at most
+    NOALLOC_MAX_PAIRS pairs per call)."""
     ka = np.ascontiguousarray(kind_a, np.int32)
     kb = np.ascontiguousarray(kind_b, np.int32)
     da, db = np.ascontiguousarray(dims_a, np.float64), np.ascontiguousarray(dims_b, np.float64)
     ta, tb = np.ascontiguousarray(tf_a, np.float64), np.ascontiguousarray(tf_b, np.float64)
     out = np.zeros((len(ka), 20))
-    _check(lib().rbgpu_collide_pairs(len(ka), _ptr(ka, _I), _ptr(da, _D), _ptr(ta, _D), _ptr(kb, _I), _ptr(db, _D),
-                                     _ptr(tb, _D), _ptr(out, _D)))
+    fn = lib().rbgpu_collide_pairs_noalloc if noalloc else lib().rbgpu_collide_pairs
+    _check(fn(len(ka), _ptr(ka, _I), _ptr(da, _D), _ptr(ta, _D), _ptr(kb, _I), _ptr(db, _D), _ptr(tb, _D), _ptr(out, _D)))
     return out
 
 

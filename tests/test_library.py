@@ -40,6 +40,23 @@ def test_invalid_arguments_fail_loudly():
     assert b"substeps" in lib.rbgpu_last_error()
 
 
+def test_noalloc_narrowphase_rejects_oversized_calls_on_the_host():
+    """The allocation-free narrowphase refuses more pairs than its fixed staging holds
+    before any GPU work (so this runs without a GPU)."""
+    import ctypes as C
+    from rbps_amd import world
+    lib = world.lib()
+    n = world.NOALLOC_MAX_PAIRS + 1
+    k = np.zeros(n, np.int32)
+    d = np.zeros(4 * n)
+    t = np.zeros(7 * n)
+    out = np.zeros(20 * n)
+    P = lambda a, ty: a.ctypes.data_as(C.POINTER(ty))   # noqa: E731
+    rc = lib.rbgpu_collide_pairs_noalloc(n, P(k, C.c_int32), P(d, C.c_double), P(t, C.c_double), P(k, C.c_int32),
+                                         P(d, C.c_double), P(t, C.c_double), P(out, C.c_double))
+    assert rc == -3 and b"16384" in lib.rbgpu_last_error()
+
+
 def test_mass_properties_closed_forms():
     r = 0.5
     assert math.isclose(scenes.sphere_volume(r), 4 / 3 * math.pi * r ** 3)
