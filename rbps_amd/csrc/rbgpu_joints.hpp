@@ -68,42 +68,48 @@ __device__ __forceinline__ float4 row_value(V3 v, int mode) {
     return make_float4(d.x, d.y, d.z, m);
 }
 
+// row[k] for a run-time k by selects: constant indices keep `row` in registers, where a
+// run-time index would put it in scratch memory.
+__device__ __forceinline__ float4 row_at(const float4 row[4], int k) {
+    return k == 0 ? row[0] : k == 1 ? row[1] : k == 2 ? row[2] : row[3];
+}
+
 // Rows of one joint from the two bodies' poses (Joint.cpp:18-182; spherical: attachment row
 // and a swing limit built like the revolute angle limit, Joint.cpp:73-90).  `advance`
 // moves a speed-driven target by speed * h.  tg.z receives the joint's current position.
 __device__ __forceinline__ void joint_rows(int4 m, float4 r1, float4 r2, float4 ax, float4 lx, float4 &tg, V3 x1, Q q1,
                                            V3 x2, Q q2, float h, bool advance, int mode, float4 row[4]) {
+    // the rows are set in locals and stored once at the end: stores to different rows in
+    // different branches could otherwise be merged into one store at a run-time index,
+    // which puts `row` in scratch memory
+    float4 o0 = row[0], o1 = row[1], o2 = row[2], o3 = row[3];
     const int type = jt_type(m), act = jt_act(m);
     const V3 p1 = x1 + rotate(q1, xyz(r1)), p2 = x2 + rotate(q2, xyz(r2));
     if (type == JT_PRISMATIC || type == JT_FIXED) {   // Joint.cpp:27-29, 170-172
         const Q dq = qmul(q1, conj(q2));
-        row[0] = row_value(V3{dq.x, dq.y, dq.z} * 2.0f, mode);
+        o0 = row_value(V3{dq.x, dq.y, dq.z} * 2.0f, mode);
     }
     if (type == JT_FIXED) {
-        row[1] = row_value(p1 - p2, mode);
-        return;
-    }
-    if (type == JT_PRISMATIC) {   // Joint.cpp:32-70
+        o1 = row_value(p1 - p2, mode);
+    } else if (type == JT_PRISMATIC) {   // Joint.cpp:32-70
         const V3 dp = p1 - p2;
         const V3 axis = rotate(q1, xyz(ax));
         const float ps = dot(dp, axis);
         float over = 0.0f;
         if (jt_limited(m)) over = ps - clampf(ps, ax.w, lx.w);
-        row[1] = row_value((dp - axis * ps) + axis * over, mode);
+        o1 = row_value((dp - axis * ps) + axis * over, mode);
         V3 drive{0.0f, 0.0f, 0.0f};
         if (act == JA_SPEED && advance) tg.x = tg.x + tg.y * h;
         if (act == JA_SPEED || act == JA_POSITION) {
             const float curr = dot(p2 - p1, axis);
             drive = axis * (tg.x - curr);
         }
-        row[2] = row_value(drive, mode);
+        o2 = row_value(drive, mode);
         tg.z = ps;
-        return;
-    }
-    if (type == JT_REVOLUTE) {   // Joint.cpp:92-159
+    } else if (type == JT_REVOLUTE) {   // Joint.cpp:92-159
         const V3 a1 = rotate(q1, xyz(ax)), a2 = rotate(q2, xyz(ax));
-        row[0] = row_value(mode == 0 ? cross(a1, a2) : cross(a2, a1), mode);
-        row[1] = row_value(p1 - p2, mode);
+        o0 = row_value(mode == 0 ? cross(a1, a2) : cross(a2, a1), mode);
+        o1 = row_value(p1 - p2, mode);
         const V3 l1 = rotate(q1, xyz(lx)), l2 = rotate(q2, xyz(lx));
         float phi = rb_atan2(dot(cross(l1, l2), a1), dot(l1, l2));
         V3 lim{0.0f, 0.0f, 0.0f};
@@ -112,7 +118,7 @@ __device__ __forceinline__ void joint_rows(int4 m, float4 r1, float4 r2, float4 
             const V3 n1 = rotate(from_axis_angle(a1, phi), l1);
             lim = cross(n1, l2);
         }
-        row[2] = row_value(-lim, mode);
+        o2 = row_value(-lim, mode);
         V3 drive{0.0f, 0.0f, 0.0f};
         if (act == JA_SPEED && advance) tg.x = tg.x + tg.y * h;
         if (act == JA_SPEED || act == JA_POSITION) {
@@ -120,24 +126,27 @@ __device__ __forceinline__ void joint_rows(int4 m, float4 r1, float4 r2, float4 
             const V3 bt = (l1 * rb_cos(theta) + cross(a1, l1) * rb_sin(theta)) + a1 * dot(a1, l1) * (1.0f - rb_cos(theta));
             drive = cross(bt, l2);
         }
-        row[3] = row_value(drive, mode);
+        o3 = row_value(drive, mode);
         tg.z = phi;
-        return;
-    }
-    // spherical
-    row[0] = row_value(p1 - p2, mode);
-    if (jt_limited(m)) {
-        const V3 a1 = rotate(q1, xyz(ax)), a2 = rotate(q2, xyz(ax));
-        const V3 c = cross(a1, a2);
-        const float th = rb_atan2(len(c), dot(a1, a2));
-        V3 lim{0.0f, 0.0f, 0.0f};
-        if (th > lx.w) {
-            const V3 n1 = rotate(from_axis_angle(c, lx.w), a1);
-            lim = cross(n1, a2);
+    } else {   // spherical
+        o0 = row_value(p1 - p2, mode);
+        if (jt_limited(m)) {
+            const V3 a1 = rotate(q1, xyz(ax)), a2 = rotate(q2, xyz(ax));
+            const V3 c = cross(a1, a2);
+            const float th = rb_atan2(len(c), dot(a1, a2));
+            V3 lim{0.0f, 0.0f, 0.0f};
+            if (th > lx.w) {
+                const V3 n1 = rotate(from_axis_angle(c, lx.w), a1);
+                lim = cross(n1, a2);
+            }
+            o1 = row_value(-lim, mode);
+            tg.z = th;
         }
-        row[1] = row_value(-lim, mode);
-        tg.z = th;
     }
+    row[0] = o0;
+    row[1] = o1;
+    row[2] = o2;
+    row[3] = o3;
 }
 
 // compute_joint_errors (Joint.cpp:222-244) for every joint; mode 1 also resets lambda.
@@ -147,11 +156,11 @@ __device__ __forceinline__ void joint_errors(Bodies &B, Joints &J, uint32_t i, f
     float4 tg = J.jtg[i];
     float4 row[4];
     for (int k = 0; k < 4; ++k) row[k] = J.jrow[(size_t)i * 4 + k];
-    joint_rows(m, J.jr1[i], J.jr2[i], J.jax[i], J.jlx[i], tg, xyz(B.x[b1]), quat(B.q[b1]), xyz(B.x[b2]),
-               quat(B.q[b2]), h, true, mode, row);
+    joint_rows(m, J.jr1[i], J.jr2[i], J.jax[i], J.jlx[i], tg, xyz(B.r[b1].x), quat(B.r[b1].q), xyz(B.r[b2].x),
+               quat(B.r[b2].q), h, true, mode, row);
     const int nrows = jt_rows(m);
     for (int k = 0; k < nrows; ++k) {
-        J.jrow[(size_t)i * 4 + k] = row[k];
+        J.jrow[(size_t)i * 4 + k] = row_at(row, k);
         if (mode == 1) J.jlam[(size_t)i * 4 + k] = 0.0f;
     }
     J.jtg[i] = tg;
@@ -185,11 +194,18 @@ __device__ __forceinline__ void joint_solve(Bodies &B, Joints &J, uint32_t i, fl
     BodyReg A, Bd;
     load_body(B, b1, A);
     load_body(B, b2, Bd);
-    float4 row[4];
-    for (int k = 0; k < nrows; ++k) row[k] = J.jrow[(size_t)i * 4 + k];
+    // the rows live in four named values, selected by value: an array indexed at run time
+    // would be placed in scratch memory
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 o0 = nrows > 0 ? J.jrow[(size_t)i * 4] : z4, o1 = nrows > 1 ? J.jrow[(size_t)i * 4 + 1] : z4;
+    float4 o2 = nrows > 2 ? J.jrow[(size_t)i * 4 + 2] : z4, o3 = nrows > 3 ? J.jrow[(size_t)i * 4 + 3] : z4;
     for (int k = 0; k < nrows; ++k) {
-        if (mode == 1) joint_rows(m, r1f, r2f, axf, lxf, tg, A.x, A.q, Bd.x, Bd.q, h, false, mode, row);
-        const float4 rv = row[k];
+        if (mode == 1) {
+            float4 row[4] = {o0, o1, o2, o3};
+            joint_rows(m, r1f, r2f, axf, lxf, tg, A.x, A.q, Bd.x, Bd.q, h, false, mode, row);
+            o0 = row[0]; o1 = row[1]; o2 = row[2]; o3 = row[3];
+        }
+        const float4 rv = k == 0 ? o0 : k == 1 ? o1 : k == 2 ? o2 : o3;
         const V3 n = xyz(rv);
         const float comp = (k == crow) ? r2f.w : 0.0f;
         float lam = J.jlam[(size_t)i * 4 + k];
@@ -214,8 +230,8 @@ __device__ __forceinline__ void joint_solve(Bodies &B, Joints &J, uint32_t i, fl
         J.jlam[(size_t)i * 4 + k] = lam;
     }
     if (mode == 1) J.jtg[i] = tg;
-    if (A.dyn) { B.x[b1] = f4(A.x, A.im); B.q[b1] = f4q(A.q); }
-    if (Bd.dyn) { B.x[b2] = f4(Bd.x, Bd.im); B.q[b2] = f4q(Bd.q); }
+    if (A.dyn) { B.r[b1].x = f4(A.x, A.im); B.r[b1].q = f4q(A.q); }
+    if (Bd.dyn) { B.r[b2].x = f4(Bd.x, Bd.im); B.r[b2].q = f4q(Bd.q); }
 }
 
 __global__ void __launch_bounds__(256) k_joint_solve(Bodies B, Joints J, const uint32_t *order, uint32_t off,
@@ -258,8 +274,8 @@ __device__ __forceinline__ void joint_damp(Bodies &B, Joints &J, uint32_t i, flo
         if (A.dyn) A.w = A.w + smv(A.Iwi, imp);
         if (Bd.dyn) Bd.w = Bd.w + smv(Bd.Iwi, -imp);
     }
-    if (A.dyn) { B.v[b1] = f4(A.v, 0.0f); B.w[b1] = f4(A.w, 0.0f); }
-    if (Bd.dyn) { B.v[b2] = f4(Bd.v, 0.0f); B.w[b2] = f4(Bd.w, 0.0f); }
+    if (A.dyn) { B.r[b1].v = f4(A.v, 0.0f); B.r[b1].w = f4(A.w, 0.0f); }
+    if (Bd.dyn) { B.r[b2].v = f4(Bd.v, 0.0f); B.r[b2].w = f4(Bd.w, 0.0f); }
 }
 
 __global__ void __launch_bounds__(256) k_joint_damp(Bodies B, Joints J, const uint32_t *order, uint32_t off,

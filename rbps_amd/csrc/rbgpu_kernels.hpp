@@ -1,16 +1,15 @@
 // CDNA4 kernels of the XPBD substep pipeline.
 //
-// Data layout (all FP32, structure of arrays, 16-byte rows for coalesced and
-// single-transaction gathers):
-//   bodies     x4 (position, inverse mass)  q4 (w,x,y,z)  v4, w4 (velocities)
-//              px4, pq4 (previous pose)  f4 (force, mass)  t4 (torque)
-//              I0/I1 (local inertia xx yy zz xy | xz yz)
-//              Ii0/Ii1 (local inverse inertia xx yy zz xy | xz yz type -)
+// Data layout (all FP32):
+//   bodies     one 128-byte row (a cache line) per body: x (position, inverse mass),
+//              q (w,x,y,z), v, w (velocities), px, pq (pose at the substep start),
+//              ii0/ii1 (local inverse inertia xx yy zz xy | xz yz dynamic -); beside it
+//              f (force, mass), t (torque), I0/I1 (local inertia), read by integrate only
 //   colliders  meta (kind, body, is_static, bp slot)  dims, lpos, lrot, mat
 //   SAP        fat AABB per slot, endpoint keys/values (stable across frames),
 //              endpoint positions, sorted sweep rows
-//   pairs      frame pair list (canonical), colour, solve slots; contact rows
-//              live in the pair's solve slot, four points per slot.
+//   pairs      frame pair list (canonical), colour, solve slots; a slot's contact
+//              header (32 B) and its four points' blocks (64 B each) are contiguous.
 #pragma once
 #include "rbgpu_collide.hpp"
 #include "rbgpu_sort.hpp"
@@ -24,8 +23,22 @@ constexpr int NBINS = 13;
 // capsule (either order), hull-plane
 enum Bin { BIN_SS = 0, BIN_SB, BIN_BB, BIN_SC, BIN_CC, BIN_CB, BIN_SP, BIN_BP, BIN_CP, BIN_HH, BIN_HX, BIN_HP, BIN_OTHER };
 
+// One 128-byte row per body (one cache line): everything the contact and joint solves
+// gather for a body, so a gather touches one line.
+struct BodyRow {
+    float4 x;     // position, inverse mass
+    float4 q;     // orientation (w, x, y, z)
+    float4 v;     // linear velocity
+    float4 w;     // angular velocity
+    float4 px;    // position at the start of the substep
+    float4 pq;    // orientation at the start of the substep
+    float4 ii0;   // local inverse inertia xx yy zz xy
+    float4 ii1;   // local inverse inertia xz yz, dynamic flag (1 dynamic, 0 static)
+};
+static_assert(sizeof(BodyRow) == 128, "a body row is one cache line");
 struct Bodies {
-    float4 *x, *q, *v, *w, *px, *pq, *f, *t, *I0, *I1, *Ii0, *Ii1;
+    BodyRow *r;
+    float4 *f, *t, *I0, *I1;   // force (w: mass), torque, local inertia (integrate only)
 };
 struct Colliders {
     int4 *meta;      // kind, body, is_static, bp slot
@@ -51,13 +64,19 @@ struct HullSet {
 struct Contacts {
     int *cnt;        // points per slot this substep
     uint2 *cc;       // colliders (a, b) per slot
-    uint2 *bb;       // bodies (a, b) per slot
-    float4 *nr;      // normal, restitution
-    float2 *fr;      // static, dynamic friction
-    float4 *ra, *rb; // body-local lever arms (rb.w = depth), 4 per slot
-    float4 *pa, *pb; // world points (pa.w = pre-solve normal velocity, pb.w = collision flag)
-    float2 *lam;     // normal, tangent lambda
+    uint2 *bb;       // bodies (a, b) per slot (read by the colouring)
+    // per slot, one 32-byte header: (normal, restitution), (bodies a and b as bits, static
+    // friction, dynamic friction)
+    float4 *hd;
+    // per point (4 per slot), one 64-byte block: (r_a body-local, normal lambda),
+    // (r_b body-local, tangent lambda), (p_a world, pre-solve normal velocity),
+    // (p_b world, collision flag)
+    float4 *pt;
+    float *dep;      // per point: penetration depth
 };
+// Rows of a slot's header and of a point's block (point k = slot * 4 + i).
+__device__ __forceinline__ float4 *hd_row(const Contacts &K, uint32_t s) { return K.hd + 2 * (size_t)s; }
+__device__ __forceinline__ float4 *pt_row(const Contacts &K, uint32_t k) { return K.pt + 4 * (size_t)k; }
 
 __device__ __forceinline__ V3 xyz(float4 a) { return V3{a.x, a.y, a.z}; }
 __device__ __forceinline__ Q quat(float4 a) { return Q{a.x, a.y, a.z, a.w}; }
@@ -66,8 +85,8 @@ __device__ __forceinline__ float4 f4q(Q q) { return make_float4(q.w, q.x, q.y, q
 __device__ __forceinline__ SM sm_from(float4 a, float4 b) { return SM{a.x, a.y, a.z, a.w, b.x, b.y}; }
 __device__ __forceinline__ Shape shape_of(int kind, float4 d) { return Shape{kind, d.x, d.y, d.z, d.w}; }
 __device__ __forceinline__ TF collider_tf(const Colliders &C, const Bodies &B, uint32_t c, uint32_t body) {
-    V3 bp = xyz(B.x[body]);
-    Q bq = quat(B.q[body]);
+    V3 bp = xyz(B.r[body].x);
+    Q bq = quat(B.r[body].q);
     return TF{bp + rotate(bq, xyz(C.lpos[c])), qmul(bq, quat(C.lrot[c]))};   // Collider.hpp:73-81
 }
 
@@ -89,7 +108,7 @@ __global__ void k_refit(Colliders C, Bodies B, HullSet hs, uint32_t M, float dt,
     } else {
         t = shape_aabb(shape_of(m.x, C.dims[i]), tf);
     }
-    V3 dv = xyz(B.v[body]) * dt;
+    V3 dv = xyz(B.r[body].v) * dt;
     V3 emn{margin + (dv.x < 0.0f ? -dv.x : 0.0f), margin + (dv.y < 0.0f ? -dv.y : 0.0f), margin + (dv.z < 0.0f ? -dv.z : 0.0f)};
     V3 emx{margin + (dv.x > 0.0f ? dv.x : 0.0f), margin + (dv.y > 0.0f ? dv.y : 0.0f), margin + (dv.z > 0.0f ? dv.z : 0.0f)};
     V3 smn = t.mn - emn, smx = t.mx + emx;
@@ -222,8 +241,10 @@ __global__ void k_pair_gather(const uint2 *pairs, const uint32_t *n_pairs, Colli
     K.cnt[p] = 0;
     float4 ma = C.mat[cc.x], mb = C.mat[cc.y];
     float rest = fmin_ref(ma.x, mb.x);                                     // CollisionPipeline.cpp:75-77
-    K.nr[p] = make_float4(0.0f, 0.0f, 0.0f, rest);
-    K.fr[p] = make_float2((ma.y + mb.y) * 0.5f, (ma.z + mb.z) * 0.5f);
+    float4 *h = hd_row(K, p);
+    h[0] = make_float4(0.0f, 0.0f, 0.0f, rest);
+    h[1] = make_float4(__uint_as_float((uint32_t)ma_.y), __uint_as_float((uint32_t)mb_.y), (ma.y + mb.y) * 0.5f,
+                       (ma.z + mb.z) * 0.5f);
     bin_key[p] = (uint32_t)bin_of(ma_.x, mb_.x);
     bin_val[p] = p;
 }
@@ -306,7 +327,7 @@ __device__ __forceinline__ uint32_t palette(uint32_t k, uint32_t pal_max) {
     const uint32_t m = (k + 3u) / 4u;
     return m < 1u ? 1u : (m > pal_max ? pal_max : m);
 }
-__device__ __forceinline__ bool is_dynamic(const Bodies &B, uint32_t b) { return B.Ii1[b].z != 0.0f; }
+__device__ __forceinline__ bool is_dynamic(const Bodies &B, uint32_t b) { return B.r[b].ii1.z != 0.0f; }
 // Loads of words that other threads update with atomics in the same kernel.
 __device__ __forceinline__ uint32_t ld_shared(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -570,10 +591,10 @@ __global__ void __launch_bounds__(256) k_colour_scatter(ColourBufs cb, uint32_t 
 // Contact emission (CollisionPipeline.cpp:59-104): split the depth across the two
 // surfaces and store body-local lever arms.
 __device__ __forceinline__ void emit(const Manifold &m, const Bodies &B, Contacts &K, uint32_t s, uint2 bb) {
-    V3 xa = xyz(B.x[bb.x]), xb = xyz(B.x[bb.y]);
-    Q qa = conj(quat(B.q[bb.x])), qb = conj(quat(B.q[bb.y]));
-    float4 nr = K.nr[s];
-    K.nr[s] = f4(m.n, nr.w);
+    V3 xa = xyz(B.r[bb.x].x), xb = xyz(B.r[bb.y].x);
+    Q qa = conj(quat(B.r[bb.x].q)), qb = conj(quat(B.r[bb.y].q));
+    float4 *h = hd_row(K, s);
+    h[0] = f4(m.n, h[0].w);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {   // constant indices keep the manifold in registers
         if (i >= m.np) break;
@@ -581,11 +602,12 @@ __device__ __forceinline__ void emit(const Manifold &m, const Bodies &B, Contact
         V3 pa = m.p[i] + m.n * half;
         V3 pb = m.p[i] - m.n * half;
         uint32_t k = s * 4 + i;
-        K.ra[k] = f4(rotate(qa, pa - xa), 0.0f);
-        K.rb[k] = f4(rotate(qb, pb - xb), m.dep[i]);
-        K.pa[k] = f4(pa, 0.0f);
-        K.pb[k] = f4(pb, 0.0f);
-        K.lam[k] = make_float2(0.0f, 0.0f);
+        float4 *r = pt_row(K, k);
+        r[0] = f4(rotate(qa, pa - xa), 0.0f);
+        r[1] = f4(rotate(qb, pb - xb), 0.0f);
+        r[2] = f4(pa, 0.0f);
+        r[3] = f4(pb, 0.0f);
+        K.dep[k] = m.dep[i];
     }
     K.cnt[s] = m.np;
 }
@@ -741,37 +763,39 @@ __global__ void k_collide_pairs(uint32_t n, const int *kind_a, const float4 *dim
 __global__ void k_integrate(Bodies B, uint32_t N, float dt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    float4 ii1 = B.Ii1[i];
+    BodyRow &b = B.r[i];
+    float4 ii1 = b.ii1;
     if (ii1.z == 0.0f) return;   // static
-    float4 x4 = B.x[i], q4 = B.q[i], v4 = B.v[i], w4 = B.w[i], f = B.f[i], tq = B.t[i];
+    float4 x4 = b.x, q4 = b.q, v4 = b.v, w4 = b.w, f = B.f[i], tq = B.t[i];
     Q q = quat(q4);
     M3 R = mat3_cast(q);
     SM Iw = world_tensor(R, sm_from(B.I0[i], B.I1[i]));
-    SM Iwi = world_tensor(R, sm_from(B.Ii0[i], ii1));
-    B.px[i] = x4;
-    B.pq[i] = q4;
+    SM Iwi = world_tensor(R, sm_from(b.ii0, ii1));
+    b.px = x4;
+    b.pq = q4;
     V3 v = xyz(v4) + xyz(f) * x4.w * dt;
     V3 x = xyz(x4) + v * dt;
     V3 w = xyz(w4);
     w = w + smv(sscale(Iwi, dt), xyz(tq) - cross(w, smv(Iw, w)));
     V3 ws = (0.5f * dt) * w;
     q = qstep(q, ws);
-    B.x[i] = f4(x, x4.w);
-    B.q[i] = f4q(q);
-    B.v[i] = f4(v, 0.0f);
-    B.w[i] = f4(w, 0.0f);
+    b.x = f4(x, x4.w);
+    b.q = f4q(q);
+    b.v = f4(v, 0.0f);
+    b.w = f4(w, 0.0f);
 }
 
 __global__ void k_update_vel(Bodies B, uint32_t N, float inv_dt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    if (B.Ii1[i].z == 0.0f) return;
-    V3 x = xyz(B.x[i]), px = xyz(B.px[i]);
-    Q dq = qmul(quat(B.q[i]), conj(quat(B.pq[i])));
+    BodyRow &b = B.r[i];
+    if (b.ii1.z == 0.0f) return;
+    V3 x = xyz(b.x), px = xyz(b.px);
+    Q dq = qmul(quat(b.q), conj(quat(b.pq)));
     V3 v = (x - px) * inv_dt;
     V3 w = (2.0f * inv_dt) * V3{dq.x, dq.y, dq.z};
-    B.v[i] = f4(v, 0.0f);
-    B.w[i] = f4(dq.w >= 0.0f ? w : -w, 0.0f);
+    b.v = f4(v, 0.0f);
+    b.w = f4(dq.w >= 0.0f ? w : -w, 0.0f);
 }
 
 // ------------------------------------------------------------------ contact solve (Contact.cpp)
@@ -785,11 +809,12 @@ struct BodyReg {
 };
 
 __device__ __forceinline__ void load_body(const Bodies &B, uint32_t i, BodyReg &r) {
-    float4 x4 = B.x[i];
-    float4 a = B.Ii0[i], b = B.Ii1[i];
+    const BodyRow &row = B.r[i];
+    float4 x4 = row.x;
+    float4 a = row.ii0, b = row.ii1;
     r.x = xyz(x4);
     r.im = x4.w;
-    r.q = quat(B.q[i]);
+    r.q = quat(row.q);
     r.Ii = sm_from(a, b);
     r.dyn = b.z != 0.0f;
 }
@@ -811,28 +836,31 @@ __device__ __forceinline__ void apply_pos(BodyReg &r, V3 imp, V3 rr) {   // Body
 __device__ __forceinline__ void solve_pos_pair(const Bodies &B, Contacts &K, uint32_t s) {
     const int np = K.cnt[s];
     if (np == 0) return;
-    const uint2 bb = K.bb[s];
-    const float4 nr = K.nr[s];
-    const V3 n = xyz(nr);
-    const float sf = K.fr[s].x;
+    const float4 *hd = hd_row(K, s);
+    const float4 h0 = hd[0], h1 = hd[1];
+    const uint2 bb = make_uint2(__float_as_uint(h1.x), __float_as_uint(h1.y));
+    const V3 n = xyz(h0);
+    const float sf = h1.z;
     BodyReg A, Bd;
     load_body(B, bb.x, A);
     load_body(B, bb.y, Bd);
-    const V3 pxa = xyz(B.px[bb.x]), pxb = xyz(B.px[bb.y]);
-    const Q pqa = quat(B.pq[bb.x]), pqb = quat(B.pq[bb.y]);
-    const V3 va = xyz(B.v[bb.x]), vb = xyz(B.v[bb.y]);
-    const V3 wa = xyz(B.w[bb.x]), wb = xyz(B.w[bb.y]);
+    const BodyRow &ra_ = B.r[bb.x], &rb_ = B.r[bb.y];
+    const V3 pxa = xyz(ra_.px), pxb = xyz(rb_.px);
+    const Q pqa = quat(ra_.pq), pqb = quat(rb_.pq);
+    const V3 va = xyz(ra_.v), vb = xyz(rb_.v);
+    const V3 wa = xyz(ra_.w), wb = xyz(rb_.w);
     for (int i = 0; i < np; ++i) {
-        const uint32_t k = s * 4 + i;
-        const V3 rla = xyz(K.ra[k]), rlb = xyz(K.rb[k]);
-        float2 lam = K.lam[k];
+        float4 *row = pt_row(K, s * 4 + i);
+        const float4 r0 = row[0], r1 = row[1];
+        const V3 rla = xyz(r0), rlb = xyz(r1);
+        float2 lam = make_float2(r0.w, r1.w);
         // Contact.cpp:35-75
         V3 pa = A.x + rotate(A.q, rla);
         V3 pb = Bd.x + rotate(Bd.q, rlb);
         float d = dot(pa - pb, n);
         if (d <= 0.0f) {
-            K.pa[k] = f4(pa, 0.0f);
-            K.pb[k] = f4(pb, 0.0f);
+            row[2] = f4(pa, 0.0f);
+            row[3] = f4(pb, 0.0f);
             continue;
         }
         V3 raw = pa - A.x, rbw = pb - Bd.x;
@@ -854,8 +882,8 @@ __device__ __forceinline__ void solve_pos_pair(const Bodies &B, Contacts &K, uin
         // Contact.cpp:121-195
         V3 ra = rotate(A.q, rla), rb = rotate(Bd.q, rlb);
         V3 pac = A.x + ra, pbc = Bd.x + rb;
-        K.pa[k] = f4(pac, relv);
-        K.pb[k] = f4(pbc, 1.0f);
+        row[2] = f4(pac, relv);
+        row[3] = f4(pbc, 1.0f);
         V3 ppa = pxa + rotate(pqa, rla);
         V3 ppb = pxb + rotate(pqb, rlb);
         V3 dp = (pac - ppa) - (pbc - ppb);
@@ -879,10 +907,11 @@ __device__ __forceinline__ void solve_pos_pair(const Bodies &B, Contacts &K, uin
                 }
             }
         }
-        K.lam[k] = lam;
+        row[0] = f4(rla, lam.x);
+        row[1] = f4(rlb, lam.y);
     }
-    if (A.dyn) { B.x[bb.x] = f4(A.x, A.im); B.q[bb.x] = f4q(A.q); }
-    if (Bd.dyn) { B.x[bb.y] = f4(Bd.x, Bd.im); B.q[bb.y] = f4q(Bd.q); }
+    if (A.dyn) { B.r[bb.x].x = f4(A.x, A.im); B.r[bb.x].q = f4q(A.q); }
+    if (Bd.dyn) { B.r[bb.y].x = f4(Bd.x, Bd.im); B.r[bb.y].q = f4q(Bd.q); }
 }
 
 struct VelReg {
@@ -893,13 +922,14 @@ struct VelReg {
     bool dyn;
 };
 __device__ __forceinline__ void load_vel(const Bodies &B, uint32_t i, VelReg &r) {
-    float4 x4 = B.x[i];
-    float4 a = B.Ii0[i], b = B.Ii1[i];
+    const BodyRow &row = B.r[i];
+    float4 x4 = row.x;
+    float4 a = row.ii0, b = row.ii1;
     r.x = xyz(x4);
     r.im = x4.w;
-    r.q = quat(B.q[i]);
-    r.v = xyz(B.v[i]);
-    r.w = xyz(B.w[i]);
+    r.q = quat(row.q);
+    r.v = xyz(row.v);
+    r.w = xyz(row.w);
     r.Ii = sm_from(a, b);
     r.dyn = b.z != 0.0f;
 }
@@ -918,22 +948,23 @@ __device__ __forceinline__ void apply_vel(VelReg &r, V3 imp, V3 rr) {   // Body.
 __device__ __forceinline__ void solve_vel_pair(const Bodies &B, Contacts &K, uint32_t s, float h) {
     const int np = K.cnt[s];
     if (np == 0) return;
-    const uint2 bb = K.bb[s];
-    const float4 nr = K.nr[s];
-    const V3 n = xyz(nr);
-    const float rest = nr.w, df = K.fr[s].y;
+    const float4 *hd = hd_row(K, s);
+    const float4 h0 = hd[0], h1 = hd[1];
+    const uint2 bb = make_uint2(__float_as_uint(h1.x), __float_as_uint(h1.y));
+    const V3 n = xyz(h0);
+    const float rest = h0.w, df = h1.w;
     VelReg A, Bd;
     load_vel(B, bb.x, A);
     load_vel(B, bb.y, Bd);
     bool any = false;
     for (int i = 0; i < np; ++i) {
-        const uint32_t k = s * 4 + i;
-        const float4 pb4 = K.pb[k];
+        const float4 *row = pt_row(K, s * 4 + i);
+        const float4 pb4 = row[3];
         if (pb4.w == 0.0f) continue;   // no collision at the position level
         any = true;
-        const float4 pa4 = K.pa[k];
+        const float4 pa4 = row[2];
         const float relv = pa4.w;
-        const float lamn = K.lam[k].x;
+        const float lamn = row[0].w;
         // Contact.cpp:206-289
         V3 raw = xyz(pa4) - A.x, rbw = xyz(pb4) - Bd.x;
         V3 vr = (A.v + cross(A.w, raw)) - (Bd.v + cross(Bd.w, rbw));
@@ -968,8 +999,8 @@ __device__ __forceinline__ void solve_vel_pair(const Bodies &B, Contacts &K, uin
         }
     }
     if (!any) return;
-    if (A.dyn) { B.v[bb.x] = f4(A.v, 0.0f); B.w[bb.x] = f4(A.w, 0.0f); }
-    if (Bd.dyn) { B.v[bb.y] = f4(Bd.v, 0.0f); B.w[bb.y] = f4(Bd.w, 0.0f); }
+    if (A.dyn) { B.r[bb.x].v = f4(A.v, 0.0f); B.r[bb.x].w = f4(A.w, 0.0f); }
+    if (Bd.dyn) { B.r[bb.y].v = f4(Bd.v, 0.0f); B.r[bb.y].w = f4(Bd.w, 0.0f); }
 }
 
 // Position / velocity solve of one colour: its pairs touch disjoint dynamic bodies.

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -71,6 +72,12 @@ struct Readback {
 };
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
+
+// One 16-byte field of every body row, copied to a packed host array.
+hipError_t body_column(float4 *dst, const BodyRow *rows, size_t field, uint32_t n) {
+    return hipMemcpy2D(dst, sizeof(float4), reinterpret_cast<const char *>(rows) + field, sizeof(BodyRow),
+                       sizeof(float4), n, hipMemcpyDeviceToHost);
+}
 
 __global__ void k_offsets(const uint32_t *hoff, uint32_t ntiles, uint32_t nd, const uint32_t *n, uint32_t *out) {
     uint32_t d = threadIdx.x + blockIdx.x * blockDim.x;
@@ -234,8 +241,7 @@ struct rbgpu_world {
         dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx); dfree(gjk_keep); dfree(gjk_pos); dfree(gjk_cand);
         dfree(gjk_ncand); dfree(gjk_axis); dfree(sc_work); dfree(sc_list);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
-        dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.nr); dfree(K.fr);
-        dfree(K.ra); dfree(K.rb); dfree(K.pa); dfree(K.pb); dfree(K.lam);
+        dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.hd); dfree(K.pt); dfree(K.dep);
         pcap = 0;
     }
     void release_mlp() {
@@ -247,8 +253,7 @@ struct rbgpu_world {
     void release() {
         if (st) (void)hipStreamSynchronize(st);
         release_pairs();
-        dfree(B.x); dfree(B.q); dfree(B.v); dfree(B.w); dfree(B.px); dfree(B.pq); dfree(B.f); dfree(B.t);
-        dfree(B.I0); dfree(B.I1); dfree(B.Ii0); dfree(B.Ii1);
+        dfree(B.r); dfree(B.f); dfree(B.t); dfree(B.I0); dfree(B.I1);
         dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat); dfree(C.hull);
         dfree(hv); dfree(hpn); dfree(hamin); dfree(hamax); dfree(htab); dfree(hmeta);
         hs = HullSet{};
@@ -609,10 +614,10 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
     rc |= dalloc(cb.blk, nblk(cap) + 1);
     uint32_t nt = (cap + RS_TILE - 1) / RS_TILE;
     rc |= dalloc(p_hist, (size_t)nt * RS_BINS);
-    rc |= dalloc(K.cnt, cap); rc |= dalloc(K.cc, cap); rc |= dalloc(K.bb, cap); rc |= dalloc(K.nr, cap);
-    rc |= dalloc(K.fr, cap);
-    rc |= dalloc(K.ra, 4 * (size_t)cap); rc |= dalloc(K.rb, 4 * (size_t)cap); rc |= dalloc(K.pa, 4 * (size_t)cap);
-    rc |= dalloc(K.pb, 4 * (size_t)cap); rc |= dalloc(K.lam, 4 * (size_t)cap);
+    rc |= dalloc(K.cnt, cap); rc |= dalloc(K.cc, cap); rc |= dalloc(K.bb, cap);
+    rc |= dalloc(K.hd, 2 * (size_t)cap);    // 32-byte header per slot
+    rc |= dalloc(K.pt, 16 * (size_t)cap);   // 64-byte block per point, 4 points per slot
+    rc |= dalloc(K.dep, 4 * (size_t)cap);
     if (rc) return RBGPU_ERR_OOM;
     HIPCHK(hipMemsetAsync(K.cnt, 0, cap * sizeof(int), st));
     pcap = cap;
@@ -738,9 +743,7 @@ int rbgpu_world::build() {
     HIPCHK(hipHostMalloc((void **)&rb, sizeof(Readback), hipHostMallocDefault));
     N = n; M = m; NEP = 2 * S;
     int rc = 0;
-    rc |= dalloc(B.x, n); rc |= dalloc(B.q, n); rc |= dalloc(B.v, n); rc |= dalloc(B.w, n); rc |= dalloc(B.px, n);
-    rc |= dalloc(B.pq, n); rc |= dalloc(B.f, n); rc |= dalloc(B.t, n); rc |= dalloc(B.I0, n); rc |= dalloc(B.I1, n);
-    rc |= dalloc(B.Ii0, n); rc |= dalloc(B.Ii1, n);
+    rc |= dalloc(B.r, n); rc |= dalloc(B.f, n); rc |= dalloc(B.t, n); rc |= dalloc(B.I0, n); rc |= dalloc(B.I1, n);
     rc |= dalloc(C.meta, m); rc |= dalloc(C.dims, m); rc |= dalloc(C.lpos, m); rc |= dalloc(C.lrot, m); rc |= dalloc(C.mat, m);
     rc |= dalloc(C.hull, m);
     rc |= dalloc(hv, hvv.size()); rc |= dalloc(hpn, hpnv.size()); rc |= dalloc(htab, htabv.size());
@@ -762,10 +765,12 @@ int rbgpu_world::build() {
     if (rc) return RBGPU_ERR_OOM;
     auto up = [&](void *dst, const void *src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st); };
     if (n) {
-        HIPCHK(up(B.x, x.data(), n * 16)); HIPCHK(up(B.q, q.data(), n * 16)); HIPCHK(up(B.v, v.data(), n * 16));
-        HIPCHK(up(B.w, w.data(), n * 16)); HIPCHK(up(B.px, x.data(), n * 16)); HIPCHK(up(B.pq, q.data(), n * 16));
+        std::vector<BodyRow> rows(n);
+        for (uint32_t i = 0; i < n; ++i) rows[i] = BodyRow{x[i], q[i], v[i], w[i], x[i], q[i], Ii0[i], Ii1[i]};
+        HIPCHK(up(B.r, rows.data(), n * sizeof(BodyRow)));
+        HIPCHK(hipStreamSynchronize(st));   // `rows` leaves scope
         HIPCHK(up(B.f, f.data(), n * 16)); HIPCHK(up(B.t, t.data(), n * 16)); HIPCHK(up(B.I0, I0.data(), n * 16));
-        HIPCHK(up(B.I1, I1.data(), n * 16)); HIPCHK(up(B.Ii0, Ii0.data(), n * 16)); HIPCHK(up(B.Ii1, Ii1.data(), n * 16));
+        HIPCHK(up(B.I1, I1.data(), n * 16));
     }
     if (m) {
         HIPCHK(up(C.meta, meta.data(), m * 16)); HIPCHK(up(C.dims, dims.data(), m * 16));
@@ -921,10 +926,10 @@ int rbgpu_world::download_state() {
     HIPCHK(hipStreamSynchronize(st));
     std::vector<float4> x(N), q(N), v(N), w(N);
     if (N) {
-        HIPCHK(hipMemcpy(x.data(), B.x, N * 16, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(q.data(), B.q, N * 16, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(v.data(), B.v, N * 16, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(w.data(), B.w, N * 16, hipMemcpyDeviceToHost));
+        HIPCHK(body_column(x.data(), B.r, offsetof(BodyRow, x), N));
+        HIPCHK(body_column(q.data(), B.r, offsetof(BodyRow, q), N));
+        HIPCHK(body_column(v.data(), B.r, offsetof(BodyRow, v), N));
+        HIPCHK(body_column(w.data(), B.r, offsetof(BodyRow, w), N));
     }
     for (uint32_t i = 0; i < N; ++i) {
         rbgpu_body_desc &d = hb[i];
@@ -1463,19 +1468,19 @@ int rbgpu_get_bodies(rbgpu_world *w, double *p, double *q, double *v, double *av
     }
     if (int rcs = w->sync()) return rcs;
     std::vector<float4> buf(n);
-    auto pull = [&](float4 *src, double *dst, bool quat) -> int {
+    auto pull = [&](size_t field, double *dst, bool quat) -> int {
         if (!dst || !n) return RBGPU_OK;
-        HIPCHK(hipMemcpy(buf.data(), src, n * 16, hipMemcpyDeviceToHost));
+        HIPCHK(body_column(buf.data(), w->B.r, field, n));
         for (uint32_t i = 0; i < n; ++i) {
             if (quat) { dst[4 * i] = buf[i].x; dst[4 * i + 1] = buf[i].y; dst[4 * i + 2] = buf[i].z; dst[4 * i + 3] = buf[i].w; }
             else { dst[3 * i] = buf[i].x; dst[3 * i + 1] = buf[i].y; dst[3 * i + 2] = buf[i].z; }
         }
         return RBGPU_OK;
     };
-    int rc = pull(w->B.x, p, false);
-    if (!rc) rc = pull(w->B.q, q, true);
-    if (!rc) rc = pull(w->B.v, v, false);
-    if (!rc) rc = pull(w->B.w, av, false);
+    int rc = pull(offsetof(BodyRow, x), p, false);
+    if (!rc) rc = pull(offsetof(BodyRow, q), q, true);
+    if (!rc) rc = pull(offsetof(BodyRow, v), v, false);
+    if (!rc) rc = pull(offsetof(BodyRow, w), av, false);
     return rc;
 }
 
@@ -1532,26 +1537,26 @@ int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *n3, double *ra3, 
     const uint32_t P = w->n_pairs;
     std::vector<int> cnt(P);
     std::vector<uint2> cc(P), bb(P);
-    std::vector<float4> nr(P), ra(4 * (size_t)P), rbv(4 * (size_t)P);
-    std::vector<float2> lam(4 * (size_t)P);
+    std::vector<float4> hd(2 * (size_t)P), pt(16 * (size_t)P);
+    std::vector<float> dp(4 * (size_t)P);
     HIPCHK(hipMemcpy(cnt.data(), w->K.cnt, P * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(cc.data(), w->K.cc, P * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(bb.data(), w->K.bb, P * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(nr.data(), w->K.nr, P * 16, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ra.data(), w->K.ra, (size_t)P * 64, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(rbv.data(), w->K.rb, (size_t)P * 64, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(lam.data(), w->K.lam, (size_t)P * 32, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hd.data(), w->K.hd, (size_t)P * 32, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pt.data(), w->K.pt, (size_t)P * 256, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dp.data(), w->K.dep, (size_t)P * 16, hipMemcpyDeviceToHost));
     uint32_t k = 0;
     for (uint32_t s = 0; s < P; ++s) {
         for (int i = 0; i < cnt[s]; ++i, ++k) {
             if (k >= cap) continue;
-            size_t j = (size_t)s * 4 + i;
+            const size_t j = (size_t)s * 4 + i;
+            const float4 r0 = pt[4 * j], r1 = pt[4 * j + 1], nr = hd[2 * (size_t)s];
             if (ids4) { ids4[4 * k] = bb[s].x; ids4[4 * k + 1] = bb[s].y; ids4[4 * k + 2] = cc[s].x; ids4[4 * k + 3] = cc[s].y; }
-            if (n3) { n3[3 * k] = nr[s].x; n3[3 * k + 1] = nr[s].y; n3[3 * k + 2] = nr[s].z; }
-            if (ra3) { ra3[3 * k] = ra[j].x; ra3[3 * k + 1] = ra[j].y; ra3[3 * k + 2] = ra[j].z; }
-            if (rb3) { rb3[3 * k] = rbv[j].x; rb3[3 * k + 1] = rbv[j].y; rb3[3 * k + 2] = rbv[j].z; }
-            if (dep) dep[k] = rbv[j].w;
-            if (lam2) { lam2[2 * k] = lam[j].x; lam2[2 * k + 1] = lam[j].y; }
+            if (n3) { n3[3 * k] = nr.x; n3[3 * k + 1] = nr.y; n3[3 * k + 2] = nr.z; }
+            if (ra3) { ra3[3 * k] = r0.x; ra3[3 * k + 1] = r0.y; ra3[3 * k + 2] = r0.z; }
+            if (rb3) { rb3[3 * k] = r1.x; rb3[3 * k + 1] = r1.y; rb3[3 * k + 2] = r1.z; }
+            if (dep) dep[k] = dp[j];
+            if (lam2) { lam2[2 * k] = r0.w; lam2[2 * k + 1] = r1.w; }
         }
     }
     *n = k;
@@ -1983,13 +1988,13 @@ int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2
     if (rcn) return rcn;
     HIPCHK(hipGetLastError());
     std::vector<int> cnt(n);
-    std::vector<float4> nr(n), pa(4 * (size_t)n), pb(4 * (size_t)n), rbv(4 * (size_t)n);
+    std::vector<float4> hd(2 * (size_t)n), pt(16 * (size_t)n);
+    std::vector<float> dp(4 * (size_t)n);
     std::vector<uint32_t> gs(n);
     HIPCHK(hipMemcpyAsync(cnt.data(), w->K.cnt, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(nr.data(), w->K.nr, (size_t)n * 16, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(pa.data(), w->K.pa, (size_t)n * 64, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(pb.data(), w->K.pb, (size_t)n * 64, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(rbv.data(), w->K.rb, (size_t)n * 64, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hd.data(), w->K.hd, (size_t)n * 32, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(pt.data(), w->K.pt, (size_t)n * 256, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(dp.data(), w->K.dep, (size_t)n * 16, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(gs.data(), w->gstat, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     dfree(gin);
@@ -1998,11 +2003,13 @@ int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2
         for (int k = 0; k < 32; ++k) o[k] = 0.0;
         o[0] = cnt[i];
         if (!cnt[i]) continue;
-        o[1] = nr[i].x; o[2] = nr[i].y; o[3] = nr[i].z;
+        const float4 nr = hd[2 * (size_t)i];
+        o[1] = nr.x; o[2] = nr.y; o[3] = nr.z;
         for (int k = 0; k < cnt[i]; ++k) {
-            const float4 a = pa[4 * i + k], b = pb[4 * i + k];
+            const size_t j = 4 * (size_t)i + k;
+            const float4 a = pt[4 * j + 2], b = pt[4 * j + 3];
             double *q = o + 4 + 7 * k;
-            q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = b.x; q[4] = b.y; q[5] = b.z; q[6] = rbv[4 * i + k].w;
+            q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = b.x; q[4] = b.y; q[5] = b.z; q[6] = dp[j];
         }
     }
     if (gjk_stats) std::copy(gs.begin(), gs.end(), gjk_stats);
