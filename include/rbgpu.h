@@ -223,8 +223,9 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *   RBGPU_OPT_GJK_CULL     exact bounding-sphere rejection in front of GJK (1, default; 0 runs
  *                          GJK on every pair of the hull bins)
  *   RBGPU_OPT_GJK_GRID, RBGPU_OPT_EPA_GRID  workgroups of the two GJK/EPA passes (4096, 8192)
- *   RBGPU_OPT_GJK_ROWS     GJK pass with four pairs per wavefront, one per 16-lane DPP row, when
- *                          every hull has at most 64 vertices (1, default); 0: one pair per wavefront
+ *   RBGPU_OPT_GJK_ROWS     layout of the GJK pass: 1 (default) four pairs per wavefront, one per
+ *                          16-lane DPP row, when every hull has at most 64 vertices; 0 one pair per
+ *                          wavefront; 2 one pair per lane (each lane scans its hulls' vertices)
  *   RBGPU_OPT_EPA_SUB      listed pairs per EPA work item (1, 2, 4, 8, 16; default 16)
  *   RBGPU_OPT_GJK_OCCUPANCY  waves per EU the row GJK pass budgets its registers for (2, 3, 4)
  *   RBGPU_OPT_EPA_OCCUPANCY, RBGPU_OPT_SOLVE_OCCUPANCY  the same for the EPA pass and the

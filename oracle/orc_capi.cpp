@@ -8,6 +8,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <vector>
 #include <stdexcept>
 #include "orc_world.hpp"
 
@@ -298,6 +299,52 @@ template <class T> int collide_t(const orc_collider_desc *a, const double *ta, c
 }
 }  // namespace
 
+// Test infrastructure: GJK (plain seed tf_b.pos - tf_a.pos, GJK.cpp:42-96) of one pair with
+// its states recorded; out4 = status, iterations, the first iteration whose state (rank,
+// simplex points w, search direction) equals the state p <= 16 iterations earlier (-1 if
+// none), and that p.  A repeated state repeats forever, since the iteration is a function
+// of that state alone.
+template <class T> int gjk_trace_t(const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
+                                   const double *tb, int *out4) {
+    const orc::ShapeP<T> sa = shape_from<T>(*a), sb = shape_from<T>(*b);
+    orc::MinkDiff<T> md{&sa, &sb, tf7<T>(ta), tf7<T>(tb)};
+    orc::V3<T> guess = md.tb.pos - md.ta.pos;
+    if (orc::length_sq(guess) < orc::C<T>::EPS) guess = orc::V3<T>(1, 0, 0);
+    struct St { int rank; T w[12]; T d[3]; };
+    std::vector<St> hist;
+    orc::Simplex<T> s;
+    orc::V3<T> dir = guess;
+    const orc::SVert<T> v0 = md.sample(dir);
+    s.push(v0);
+    dir = -v0.w;
+    out4[0] = orc::GJK_FAILED; out4[1] = 0; out4[2] = -1; out4[3] = 0;
+    for (int it = 0; it < 128; ++it) {
+        out4[1] = it + 1;
+        const orc::SVert<T> nx = md.sample(dir);
+        if (orc::dot(nx.w, dir) < T(0)) { out4[0] = orc::GJK_VALID; return 0; }
+        s.push(nx);
+        bool inside = false;
+        if (s.rank == 2) orc::simplex2(s, dir);
+        else if (s.rank == 3) orc::simplex3(s, dir);
+        else if (s.rank == 4) inside = orc::simplex4(s, dir);
+        if (inside) { out4[0] = orc::GJK_INSIDE; return 0; }
+        St st{};
+        st.rank = s.rank;
+        for (int k = 0; k < s.rank; ++k) { st.w[3 * k] = s.v[k].w.x; st.w[3 * k + 1] = s.v[k].w.y; st.w[3 * k + 2] = s.v[k].w.z; }
+        st.d[0] = dir.x; st.d[1] = dir.y; st.d[2] = dir.z;
+        if (out4[2] < 0)
+            for (int p = 1; p <= 16 && p <= (int)hist.size(); ++p) {
+                const St &o = hist[hist.size() - p];
+                if (o.rank == st.rank && std::memcmp(o.w, st.w, sizeof(T) * 3 * st.rank) == 0 && std::memcmp(o.d, st.d, sizeof(T) * 3) == 0) {
+                    out4[2] = it + 1;
+                    out4[3] = p;
+                    break;
+                }
+            }
+        hist.push_back(st);
+    }
+    return 0;
+}
 extern "C" {
 
 void *orc_create(int fp32, int order, double timestep, int substeps) {
@@ -408,6 +455,11 @@ int orc_gjk_epa(int fp32, const orc_collider_desc *a, const double *ta, const or
     } catch (const std::exception &) {
         return -1;
     }
+}
+
+int orc_gjk_trace(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b, const double *tb,
+                  int *out4) {
+    return fp32 ? gjk_trace_t<float>(a, ta, b, tb, out4) : gjk_trace_t<double>(a, ta, b, tb, out4);
 }
 
 // Registers a hull (ConvexHull.hpp:149-420 tables); returns its index for

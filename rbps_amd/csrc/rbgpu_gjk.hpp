@@ -195,6 +195,7 @@ template <int VPL, int LPP> __device__ __forceinline__ int wave_support_index(co
 }
 
 template <int VPL, int LPP = 64> struct Side {
+    static constexpr int kLanes = LPP;   // lanes that hold one pair
     int kind;
     float a, b, c;   // sphere r | box half extents | capsule half_height, radius
     TF tf;
@@ -204,29 +205,82 @@ template <int VPL, int LPP> __device__ __forceinline__ V3 hull_vertex(const Hull
     const float4 p = hs.v[w.v0 + i];
     return V3{p.x, p.y, p.z};
 }
-// Local support (Sphere.hpp:48-51, Box.hpp:49-55, Capsule.hpp:72-80, ConvexHull.hpp:457-473).
+// Local support of a sphere, box or capsule (Sphere.hpp:48-51, Box.hpp:49-55, Capsule.hpp:72-80).
+__device__ __forceinline__ V3 prim_support(int kind, float a, float b, float c, V3 d) {
+    if (kind == SH_SPHERE) return normalize(d) * a;
+    if (kind == SH_BOX) return V3{d.x > 0.0f ? a : -a, d.y > 0.0f ? b : -b, d.z > 0.0f ? c : -c};
+    const V3 e{0.0f, d.y >= 0.0f ? a : -a, 0.0f};   // capsule
+    const float l = len(d);
+    if (l < EPS) return e;
+    return e + vdiv(d, l) * b;
+}
+// Local support (and ConvexHull.hpp:457-473 for hulls, over the wave's lanes).
 template <int VPL, int LPP> __device__ __forceinline__ V3 local_support(const Side<VPL, LPP> &s, const HullSet &hs, V3 d) {
     if (s.kind == SH_HULL) {
         V3 v;
         wave_support_index(s.wh, d, &v);
         return v;
     }
-    if (s.kind == SH_SPHERE) return normalize(d) * s.a;
-    if (s.kind == SH_BOX) return V3{d.x > 0.0f ? s.a : -s.a, d.y > 0.0f ? s.b : -s.b, d.z > 0.0f ? s.c : -s.c};
-    const V3 e{0.0f, d.y >= 0.0f ? s.a : -s.a, 0.0f};   // capsule
-    const float l = len(d);
-    if (l < EPS) return e;
-    return e + vdiv(d, l) * s.b;
+    return prim_support(s.kind, s.a, s.b, s.c, d);
 }
 // World support of one side (MinkowskiDiff.hpp:52-63).
 template <int VPL, int LPP> __device__ __forceinline__ V3 side_support(const Side<VPL, LPP> &s, const HullSet &hs, V3 dir) {
     return tf_point(s.tf, local_support(s, hs, tf_inv_vec(s.tf, dir)));
 }
 
+// A side of a pair that one lane holds by itself: the shape and, for a hull, where its
+// vertices start in the world's vertex array.
+struct LaneSide {
+    static constexpr int kLanes = 1;
+    int kind;
+    float a, b, c;
+    TF tf;
+    uint32_t v0;
+    int nv;
+};
+// ConvexHull.hpp:457-473 on one lane: the first vertex (index order) with the largest
+// dot(v, d).  Strict comparisons give the first maximum, treat -0 and +0 alike, never let a
+// NaN win and never let anything beat a NaN at vertex 0 -- the reference's scan.  Loads
+// run four vertices ahead of the ordered comparisons.
+__device__ __forceinline__ V3 lane_hull_support(const HullSet &hs, uint32_t v0, int nv, V3 d) {
+    const float4 p = hs.v[v0];
+    float best = p.x * d.x + p.y * d.y + p.z * d.z;
+    V3 bv{p.x, p.y, p.z};
+    for (int i = 1; i < nv; i += 4) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = i + u < nv ? hs.v[v0 + i + u] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float x = q[u].x * d.x + q[u].y * d.y + q[u].z * d.z;
+            if (i + u < nv && x > best) { best = x; bv = V3{q[u].x, q[u].y, q[u].z}; }
+        }
+    }
+    return bv;
+}
+__device__ __forceinline__ V3 side_support(const LaneSide &s, const HullSet &hs, V3 dir) {
+    const V3 d = tf_inv_vec(s.tf, dir);
+    return tf_point(s.tf, s.kind == SH_HULL ? lane_hull_support(hs, s.v0, s.nv, d) : prim_support(s.kind, s.a, s.b, s.c, d));
+}
+__device__ __forceinline__ void load_lane_side(LaneSide &s, const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf) {
+    const int kind = C.meta[c].x;
+    const float4 d = C.dims[c];
+    s.kind = kind;
+    s.a = d.x; s.b = d.y; s.c = d.z;
+    s.tf = tf;
+    s.v0 = 0;
+    s.nv = 0;
+    if (kind == SH_HULL) {
+        const HullMeta m = hs.meta[(uint32_t)C.hull[c]];
+        s.v0 = m.v0;
+        s.nv = (int)m.nv;
+    }
+}
+
 struct SV {
     V3 w0, w;   // A-side support, Minkowski-difference point (the B-side support is not kept)
 };
-template <int VPL, int LPP> __device__ __forceinline__ SV md_sample(const Side<VPL, LPP> &A, const Side<VPL, LPP> &Bs, const HullSet &hs, V3 d) {
+template <class SideT> __device__ __forceinline__ SV md_sample(const SideT &A, const SideT &Bs, const HullSet &hs, V3 d) {
     SV s;
     s.w0 = side_support(A, hs, d);
     s.w = s.w0 - side_support(Bs, hs, -d);
@@ -348,13 +402,85 @@ __device__ __forceinline__ bool sx_tetra(Simplex &s, V3 &dir) {   // Tetrahedron
     }
     return false;
 }
-template <int VPL, int LPP> __device__ __forceinline__ int gjk(const Side<VPL, LPP> &A, const Side<VPL, LPP> &Bs, const HullSet &hs, V3 guess, Simplex &s, int &iters,
-                                   int max_iter = GJK_ITERS) {
+// Exact cycle detection for GJK (Brent 1980).  An iteration's outcome depends only on the
+// loop state -- rank, the simplex points w below the rank and the search direction (the
+// A-side points w0 ride along and decide nothing) -- so a state equal to an earlier one
+// repeats forever and the loop can only end at its cap, GJK_FAILED.  One earlier state is
+// kept, re-saved at iterations 1, 2, 4, 8, ...; a match ends the loop with the status and
+// iteration count the full run reports.  (In FP32, GJK on touching hulls cycles with
+// periods of 4-15 iterations from about its tenth: such pairs used the full 128.)
+// The state is 16 words: 0 rank, 1-12 the points, 13-15 the direction.  Where a pair spans
+// 16 or 64 lanes (its state uniform over them) lane r keeps word r & 15 and the match is a
+// ballot; a pair on one lane keeps all 16 words.
+// Word r of the state, picked with masks rather than selects or an index: a run-time pick
+// among the simplex's fields was lowered to a stack copy of the simplex.
+__device__ __forceinline__ uint32_t gjk_state_word(const Simplex &s, V3 dir, int r, bool &used) {
+    used = r == 0 || r >= 13 || (r - 1) / 3 < s.rank;
+    const V3 a = s.v[0].w, b = s.v[1].w, c = s.v[2].w, d = s.v[3].w;
+    uint32_t w = (uint32_t)s.rank & (0u - (uint32_t)(r == 0));
+    const float f[15] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, dir.x, dir.y, dir.z};
+#pragma unroll
+    for (int q = 0; q < 15; ++q) w |= __float_as_uint(f[q]) & (0u - (uint32_t)(r == q + 1));
+    return w;
+}
+template <int LANES> struct GjkCycle {
+    uint32_t saved[16];
+    int power = 1, lam = 0;
+    __device__ __forceinline__ bool repeats(const Simplex &s, V3 dir) {
+        bool same = lam > 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            bool used;
+            const uint32_t w = gjk_state_word(s, dir, r, used);
+            same = same && (!used || w == saved[r]);
+            if (power == lam || lam == 0) saved[r] = w;
+        }
+        if (same) return true;
+        if (power == lam) { power *= 2; lam = 0; }
+        ++lam;
+        return false;
+    }
+};
+template <> struct GjkCycle<16> {   // one pair per 16-lane row: lane r keeps word r
+    uint32_t saved = 0u;
+    int power = 1, lam = 0;
+    __device__ __forceinline__ bool repeats(const Simplex &s, V3 dir) {
+        const int r = lane64() & 15;
+        bool used;
+        const uint32_t w = gjk_state_word(s, dir, r, used);
+        const uint32_t diff = (uint32_t)((__ballot(used && w != saved) >> (lane64() & ~15)) & 0xffffull);
+        const bool same = lam > 0 && diff == 0u;
+        if (power == lam || lam == 0) saved = w;
+        if (same) return true;
+        if (power == lam) { power *= 2; lam = 0; }
+        ++lam;
+        return false;
+    }
+};
+template <> struct GjkCycle<64> {   // one pair per wavefront: lanes 0-15 keep the words
+    uint32_t saved = 0u;
+    int power = 1, lam = 0;
+    __device__ __forceinline__ bool repeats(const Simplex &s, V3 dir) {
+        const int r = lane64() & 15;
+        bool used;
+        const uint32_t w = gjk_state_word(s, dir, r, used);
+        const bool same = lam > 0 && (__ballot(used && w != saved) & 0xffffull) == 0ull;
+        if (power == lam || lam == 0) saved = w;
+        if (same) return true;
+        if (power == lam) { power *= 2; lam = 0; }
+        ++lam;
+        return false;
+    }
+};
+
+template <class SideT> __device__ __forceinline__ int gjk(const SideT &A, const SideT &Bs, const HullSet &hs, V3 guess, Simplex &s,
+                                                         int &iters, int max_iter = GJK_ITERS) {
     s.rank = 0;
     V3 dir = (len2(guess) > EPS) ? guess : V3{1.0f, 0.0f, 0.0f};
     const SV v0 = md_sample(A, Bs, hs, dir);
     sx_push(s, v0);
     dir = -v0.w;
+    GjkCycle<SideT::kLanes> cyc;
     for (int it = 0; it < max_iter; ++it) {
         iters = it + 1;
         const SV nx = md_sample(A, Bs, hs, dir);
@@ -368,6 +494,10 @@ template <int VPL, int LPP> __device__ __forceinline__ int gjk(const Side<VPL, L
         else if (s.rank == 3) sx_triangle(s, dir);
         else inside = sx_tetra(s, dir);
         if (inside) return GJK_INSIDE;
+        if (cyc.repeats(s, dir)) {   // the remaining iterations would only go round the cycle
+            iters = max_iter;
+            return GJK_FAILED;
+        }
     }
     return GJK_FAILED;
 }
@@ -1033,6 +1163,76 @@ k_gjk_pass(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *c
             blk_hits[vb] = c;
         }
         __syncthreads();   // inside_g is rewritten by the next virtual block
+    }
+}
+
+// Pass 1 with one pair per lane: each thread runs GJK for one candidate, scanning its hulls'
+// vertices itself (lane_hull_support), so a wavefront serves 64 pairs with one instruction
+// stream and no cross-lane reductions.  Same per-slot results as k_gjk_pass and the same
+// lists of overlapping pairs: candidates [16 j, 16 j + 16) form segment j (hits at 16 j + r,
+// blk_hits[j] entries, simplices beside them), which the EPA pass reads with seg = 16.
+constexpr uint32_t GJK_LANE_SEG = 16;
+__global__ void __launch_bounds__(64) k_gjk_lanes(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand,
+                                                   const uint32_t *ncand, Colliders C, Bodies B, Contacts K, HullSet hs,
+                                                   uint32_t *gstat, const float4 *guess, float4 *tlog,
+                                                   const uint32_t *tlog_base, uint32_t *hits, uint32_t *blk_hits,
+                                                   float4 *sxs, float4 *axis) {
+    const uint32_t nc = gjk_count(n, ncand);
+    if (tlog && tlog_base) tlog += 2 * (size_t)*tlog_base;
+    const int lane = lane64();
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nc; base += gridDim.x * blockDim.x) {   // uniform per block
+        const uint32_t tc = base + threadIdx.x;   // candidate
+        bool inside = false;
+        Simplex sx;
+        if (tc < nc) {
+            const uint32_t t = cand ? cand[tc] : tc;   // bin position
+            const uint32_t s = bin_slot[off + t];
+            const uint2 cc = K.cc[s], bb = K.bb[s];
+            const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+            LaneSide A, Bs;
+            load_lane_side(A, C, hs, cc.x, ta);
+            load_lane_side(Bs, C, hs, cc.y, tb);
+            const GjkSeeds q = gjk_seeds(ta, tb, guess, tc, 0);
+            int it = 0;
+            int gs = gjk(A, Bs, hs, q.g, sx, it, q.warm ? GJK_WARM_ITERS : GJK_ITERS);
+            int gsum = it, fb = 0;
+            if (q.warm && gs == GJK_FAILED) {
+                fb = 1;
+                gs = gjk(A, Bs, hs, q.plain, sx, it);
+                gsum += it;
+            }
+            inside = gs == GJK_INSIDE;
+            gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);   // pass 2 completes an overlapping pair
+            if (axis) {   // as in k_gjk_pass: a separated pair's last direction (rejection hint)
+                const float ln = len(sx.dir);
+                axis[s] = gs == GJK_VALID && ln > 1e-12f ? f4(vdiv(sx.dir, ln), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            if (!inside) {
+                K.cnt[s] = 0;
+                if (tlog) {
+                    V3 tg{0.0f, 0.0f, 0.0f};
+                    float vg = 0.0f;
+                    const float ln = len(sx.dir);
+                    if (gs == GJK_VALID && ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
+                    tlog[2 * tc] = f4(tg, vg);
+                    tlog[2 * tc + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+            }
+        }
+        // segments: this lane's group of 16 lanes holds the candidates of one segment
+        const uint32_t gm = (uint32_t)((__ballot(inside) >> (lane & ~15)) & 0xffffull);
+        const uint32_t seg = tc / GJK_LANE_SEG;
+        if (inside) {
+            const uint32_t h = seg * GJK_LANE_SEG + (uint32_t)__popc(gm & ((1u << (lane & 15)) - 1u));
+            hits[h] = tc;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {   // the simplex: (w.xyz, w0.x) (w0.y, w0.z)
+                const SV &v = sx.v[k];
+                sxs[(size_t)h * 8 + 2 * k] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
+                sxs[(size_t)h * 8 + 2 * k + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
+            }
+        }
+        if ((lane & 15) == 0 && tc < nc) blk_hits[seg] = (uint32_t)__popc(gm);
     }
 }
 

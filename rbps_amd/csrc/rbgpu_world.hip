@@ -145,9 +145,10 @@ struct rbgpu_world {
     int contact_mode = 0;                               // 1: capsule-capsule normal from A to B
     bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
-    bool gjk_rows = true;                               // GJK pass: a pair per 16-lane row (hulls <= 64 vertices)
+    int gjk_layout = 1;                                 // GJK pass: 0 a pair per wavefront, 1 per 16-lane row (hulls <= 64
+                                                        // vertices), 2 per lane
     uint32_t epa_sub = 16;                              // EPA pass: listed pairs per work item
-    int gjk_occ = 4;                                    // GJK row pass: waves per EU the registers are budgeted for
+    int gjk_occ = 3;                                    // GJK row pass: waves per EU the registers are budgeted for
     int epa_occ = 0, solve_occ = 0;                     // EPA pass / per-scene position solve: 4 = budget for 4 waves
     // the narrowphase bins other than GJK run on a second stream beside the GJK chain (default
     // priority, joined back by an event before the colouring)
@@ -360,13 +361,27 @@ struct rbgpu_world {
         const float4 *gp = (gjk_explicit || infer) ? inf_guess : nullptr;
         const int seed = mlp_mode == 2 ? 1 : 0;
         // four pairs per wavefront (one per DPP row) when every hull has at most 64 vertices
-        const bool rows = gjk_rows && hull_vpl == 1;
-        const uint32_t ppb = rows ? (uint32_t)gjk_pass_pairs<16>() : (uint32_t)gjk_pass_pairs<64>();
+        const bool lanes = gjk_layout == 2;
+        const bool rows = !lanes && gjk_layout == 1 && hull_vpl == 1;
+        const uint32_t ppb = lanes ? GJK_LANE_SEG : rows ? (uint32_t)gjk_pass_pairs<16>() : (uint32_t)gjk_pass_pairs<64>();
         const uint32_t nvb = (ng + ppb - 1) / ppb;
         const uint32_t sub = std::min<uint32_t>(epa_sub, ppb);
         const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb * (ppb / sub), epa_grid);
         hipEvent_t *e = timed_begin(11);   // the GJK pass alone
-        if (rows) {
+        if (lanes) {
+            // one pair per lane, 64-thread workgroups; the EPA pass reads 16-candidate segments
+            const uint32_t gl = std::min<uint32_t>(nblk(ng, 64), gjk_grid);
+            k_gjk_lanes<<<gl, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, tl, ds_cnt, gjk_hits,
+                                           gjk_blk, gjk_sx, gjk_axis);
+            timed_end(e);
+            if (int rcf = fork_at(3)) return rcf;
+            if (hull_vpl == 1)
+                k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
+                                                 ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+            else
+                k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp,
+                                                        seed, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+        } else if (rows) {
             // waves per EU the compiler budgets registers for: 4 caps them at 128 (some spill), 3 and 2 do not
             if (gjk_occ == 2)
                 k_gjk_pass<4, 16, 2><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
@@ -1637,8 +1652,8 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "contact mode is 0 or 1");
             w->contact_mode = value; return RBGPU_OK;
         case RBGPU_OPT_GJK_ROWS:
-            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "GJK rows is 0 or 1");
-            w->gjk_rows = value != 0; return RBGPU_OK;
+            if (value < 0 || value > 2) return fail(RBGPU_ERR_INVALID, "GJK layout: 0 wavefront, 1 row, 2 lane per pair");
+            w->gjk_layout = value; return RBGPU_OK;
         case RBGPU_OPT_GJK_OCCUPANCY:
             if (value < 2 || value > 4) return fail(RBGPU_ERR_INVALID, "GJK occupancy: 2, 3 or 4 waves per EU");
             w->gjk_occ = value; return RBGPU_OK;
@@ -1679,7 +1694,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_EPA_GRID: *value = (int)w->epa_grid; return RBGPU_OK;
         case RBGPU_OPT_GRAPH: *value = w->use_graph ? 1 : 0; return RBGPU_OK;
         case RBGPU_OPT_CONTACT_MODE: *value = w->contact_mode; return RBGPU_OK;
-        case RBGPU_OPT_GJK_ROWS: *value = w->gjk_rows ? 1 : 0; return RBGPU_OK;
+        case RBGPU_OPT_GJK_ROWS: *value = w->gjk_layout; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB: *value = (int)w->epa_sub; return RBGPU_OK;
         case RBGPU_OPT_GJK_OCCUPANCY: *value = w->gjk_occ; return RBGPU_OK;
         case RBGPU_OPT_EPA_OCCUPANCY: *value = w->epa_occ; return RBGPU_OK;

@@ -62,17 +62,20 @@ def test_gpu_full_batch_runs_without_cross_scene_pairs():
     assert len(pr) > 0
 
 
-def test_gpu_combined_batch_matches_fp32_oracle_and_single_worlds():
+@pytest.mark.parametrize("layout", [1, 2])
+def test_gpu_combined_batch_matches_fp32_oracle_and_single_worlds(layout):
     """Hulls from a shared library, boxes and ragdolls (revolute + spherical joints with
     60 deg limits) in three uneven scenes; 80 frames at dt = 1/120 s, 8 substeps, so the
     pile has landed on the ragdolls (hull-box, hull-capsule, hull-sphere contacts), in the
-    contact mode that orients the capsule-capsule normal like the other pairs."""
+    contact mode that orients the capsule-capsule normal like the other pairs; GJK pass
+    with a pair per 16-lane row (1) or per lane (2)."""
     lib = S.hull_library()
     parts = [S.combined_scene(n, seed=40 + k, library=lib) for k, n in enumerate((64, 90, 130))]
     bt = S.batch(parts)
     assert len(bt.hulls) == len(lib)
-    print("case combined batch bodies", len(bt.bodies), "joints", len(bt.joints), flush=True)
+    print("case combined batch bodies", len(bt.bodies), "joints", len(bt.joints), "GJK layout", layout, flush=True)
     g = _gpu(bt, contact_mode=1)
+    g.set_option("gjk_rows", layout)
     o = orc.OracleWorld(bt.timestep, bt.substeps, fp32=True, order=orc.ORDER_COLOUR).load(bt)
     o.set_joint_mode(1)
     o.set_contact_mode(1)

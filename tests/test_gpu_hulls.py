@@ -44,12 +44,18 @@ def _expected_surface_points(r):
     return p + n[None, :] * half[:, None], p - n[None, :] * half[:, None]
 
 
-def test_gpu_hull_pairs_bit_exact_vs_fp32_oracle():
+# GJK pass layouts (option gjk_rows): a pair per wavefront, per 16-lane row, per lane
+LAYOUTS = [0, 1, 2]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_gpu_hull_pairs_bit_exact_vs_fp32_oracle(layout):
     g = golden("hull_pairs")
     hulls = unpack_hulls(g)
     a, b, tfa, tfb = g["a"], g["b"], g["tfa"], g["tfb"]
-    print("case hull_pairs", len(a), flush=True)
+    print("case hull_pairs", len(a), "GJK layout", layout, flush=True)
     w, pairs = _pair_world(a, tfa, b, tfb, hulls)
+    w.set_option("gjk_rows", layout)
     out = w.collide_pairs(pairs)
     n_gjk = 0
     for i in range(len(a)):
@@ -71,16 +77,18 @@ def test_gpu_hull_pairs_bit_exact_vs_fp32_oracle():
     assert n_gjk > 1000
 
 
+@pytest.mark.parametrize("layout", [1, 2])
 @pytest.mark.parametrize("make,frames", [
     (lambda: scenes.hull_zoo(), 40),
     (lambda: scenes.hull_pile(96, seed=3), 30),
     (lambda: scenes.hull_pile(96, seed=5, faces=False), 30),
 ])
-def test_gpu_hull_worlds_match_fp32_colour_oracle(make, frames):
+def test_gpu_hull_worlds_match_fp32_colour_oracle(make, frames, layout):
     from rbps_amd import world
     sc = make()
-    print("case", sc.name, flush=True)
+    print("case", sc.name, "GJK layout", layout, flush=True)
     g = world.World(sc.timestep, sc.substeps).load(sc)
+    g.set_option("gjk_rows", layout)
     o = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
     for f in range(frames):
         g.step(1)
