@@ -68,7 +68,8 @@ enum {
     RBGPU_JOINT_PRISMATIC = 0,
     RBGPU_JOINT_REVOLUTE = 1,
     RBGPU_JOINT_FIXED = 2,
-    RBGPU_JOINT_SPHERICAL = 3
+    RBGPU_JOINT_SPHERICAL = 3,
+    RBGPU_JOINT_ROW = 4   /* a constraint row (rbgpu_add_constraints): actuation = its kind, 0 positional, 1 rotational */
 };
 enum { RBGPU_ACT_FREE = 0, RBGPU_ACT_POSITION = 1, RBGPU_ACT_SPEED = 2 };
 
@@ -129,6 +130,27 @@ int rbgpu_add_colliders(rbgpu_world *w, int n, const rbgpu_collider_desc *desc, 
  * spherical extension.  Joint ids are creation indices.  ids_out may be NULL. */
 int rbgpu_add_joints(rbgpu_world *w, int n, const rbgpu_joint_desc *desc, uint32_t *ids_out);
 
+/* World::create_constraint / rbps::create_constraint — include/rbps/API/World.hpp:135-138,
+ * include/rbps/API/ConstraintAPI.hpp:37-64: a constraint row between two bodies (fields of
+ * rbps::ConstraintParams).  A row starts with direction (1, 0, 0) and magnitude 0, keeps the
+ * direction and magnitude its caller writes (rbgpu_set_constraints) and is solved with the
+ * joints' rows, in creation order (Constraint.cpp:89-106; here colour by colour with the
+ * joints).  It shares the joints' id space: ids_out are the ids rbgpu_get_joints reports it
+ * under (its lambda in slot 0). */
+typedef struct {
+    uint32_t body_1, body_2;
+    int32_t type, _pad;          /* 0 POSITIONAL, 1 ROTATIONAL (Constraint.hpp ConstraintType) */
+    double r_1[3], r_2[3];       /* body-local attachment points of a positional row */
+    double compliance;
+} rbgpu_constraint_desc;
+int rbgpu_add_constraints(rbgpu_world *w, int n, const rbgpu_constraint_desc *desc, uint32_t *ids_out);
+/* Writes ConstraintCollection::direction / magnitude of constraint rows (public fields the
+ * reference's callers assign directly); either array may be NULL. */
+int rbgpu_set_constraints(rbgpu_world *w, int n, const uint32_t *ids, const double *direction3, const double *magnitude);
+/* Every joint's / row's constraint rows after the last step (ConstraintCollection::direction,
+ * magnitude): 4 slots per id (unused slots (1, 0, 0), 0); either pointer may be NULL. */
+int rbgpu_get_constraint_rows(rbgpu_world *w, double *direction3, double *magnitude);
+
 /* Public joint state after the last step (JointCollection::current_position,
  * target_position; ConstraintCollection::lambda of the joint's rows, 4 per joint, unused
  * rows 0) and the joint's solve colour.  Any pointer may be NULL. */
@@ -160,6 +182,13 @@ int rbgpu_get_bodies(rbgpu_world *w, double *position, double *orientation, doub
 int rbgpu_set_bodies(rbgpu_world *w, const double *position, const double *orientation,
                      const double *linear_velocity, const double *angular_velocity);
 
+/* BodyCollection::force / torque (Body.hpp:52-69; create_body sets force = (0, -9.8 m, 0)
+ * and torque = 0, BodyAPI.hpp:61-62): the constant external load the integrator applies
+ * every substep (Body.cpp:28-35).  Writes / reads n*3 values each; either pointer may be
+ * NULL.  Held until overwritten; bodies added later start with the defaults. */
+int rbgpu_set_body_forces(rbgpu_world *w, const double *force3, const double *torque3);
+int rbgpu_get_body_forces(rbgpu_world *w, double *force3, double *torque3);
+
 /* Broadphase output of the last frame (rbc::BroadPhaseState::pairs, include/rbc/BroadPhase.hpp:117)
  * with the narrowphase driver's filters applied, in canonical sweep order: (id_a, id_b)
  * with id_a < id_b.  Writes min(cap, n) pairs, *n = total. */
@@ -173,6 +202,19 @@ int rbgpu_get_pair_colours(rbgpu_world *w, uint32_t *pairs2, int32_t *colour, ui
  * collider_a, collider_b; lam2 = normal_lambda, tangent_lambda.  Any pointer may be NULL. */
 int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *normal3, double *r_a_local3, double *r_b_local3,
                        double *depth, double *lam2, uint32_t cap, uint32_t *n);
+
+/* More of the last substep's ContactList (CollisionPipeline.hpp:46-139), in the order of
+ * rbgpu_get_contacts: point_on_a / point_on_b (world points after the position solve),
+ * relative_velocity (pre-solve normal velocity) and collision (0 once the bodies separated
+ * in the position solve).  Any pointer may be NULL. */
+int rbgpu_get_contact_points(rbgpu_world *w, double *point_on_a3, double *point_on_b3, double *relative_velocity,
+                             int32_t *collision, uint32_t cap, uint32_t *n);
+
+/* rbc::BroadPhaseState::objects (BroadPhase.hpp:91-97, 117): per broadphase object (every
+ * collider but planes, in creation order) its fat AABB and user id (the collider id);
+ * writes min(cap, n), *n = total.  Any array may be NULL. */
+int rbgpu_get_broadphase_objects(rbgpu_world *w, double *fat_min3, double *fat_max3, uint32_t *user_id, uint32_t cap,
+                                 uint32_t *n);
 
 int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
 

@@ -45,6 +45,8 @@ def _bind(lib, prefix):
     f("num_joints").argtypes = [_P]
     f("get_joints").argtypes = [_P, _D, _D, _D, _I]
     f("set_joint_targets").argtypes = [_P, C.c_int, _U, _D, _D]
+    f("set_rows").argtypes = [_P, C.c_int, _U, _D, _D]
+    f("set_forces").argtypes = [_P, _D, _D]
     f("hull_register").restype = C.c_int
     f("hull_register").argtypes = [_D, C.c_int, _U, C.c_int]
     f("hull_sizes").argtypes = [C.c_int, _I]
@@ -136,6 +138,7 @@ def oracle_lib():
         lib.orc_set_contact_mode.argtypes = [_P, C.c_int]
         lib.orc_set_planted_fault.argtypes = [_P, C.c_int]
         lib.orc_get_solve_order.argtypes = [_P, _U]
+        lib.orc_gjk_trace.argtypes = [C.c_int, _P, _D, _P, _D, _I]
         lib.orc_joint_errors.argtypes = [C.c_int, _P, _D, _D, C.c_double, _D]
         for f in ("orc_sinf", "orc_cosf"):
             getattr(lib, f).restype = C.c_float
@@ -214,6 +217,21 @@ class _WorldBase:
         tp = None if target_position is None else np.ascontiguousarray(target_position, np.float64)
         ts = None if target_speed is None else np.ascontiguousarray(target_speed, np.float64)
         self._fn("set_joint_targets")(self.h, len(ids), _ptr(ids, _U), _ptr(tp, _D), _ptr(ts, _D))
+
+    def set_rows(self, ids, direction=None, magnitude=None):
+        """Direction / magnitude of constraint rows (joints of type 4), as a caller of the
+        reference writes ConstraintCollection::direction / magnitude."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        d = None if direction is None else np.ascontiguousarray(direction, np.float64).reshape(-1)
+        m = None if magnitude is None else np.ascontiguousarray(magnitude, np.float64)
+        self._fn("set_rows")(self.h, len(ids), _ptr(ids, _U), _ptr(d, _D), _ptr(m, _D))
+
+    def set_forces(self, force=None, torque=None):
+        """BodyCollection::force / torque of every body (n x 3 each), as a caller of the
+        reference writes them."""
+        f = None if force is None else np.ascontiguousarray(force, np.float64).reshape(-1)
+        t = None if torque is None else np.ascontiguousarray(torque, np.float64).reshape(-1)
+        self._fn("set_forces")(self.h, _ptr(f, _D), _ptr(t, _D))
 
     def step(self, frames=1):
         if self._fn("step")(self.h, frames) != 0:
@@ -358,6 +376,18 @@ def oracle_gjk_epa(a, tfa, b, tfb, fp32=False, hulls=None, guess=None):
     if hit < 0:
         raise RuntimeError("gjk_epa: invalid shape")
     return _gjk_epa_out(out, hit)
+
+
+def oracle_gjk_trace(a, tfa, b, tfb, fp32=True, hulls=None):
+    """Plain-seed GJK of one pair with its loop states recorded: status, iterations, the
+    first iteration whose state (rank, simplex points, direction) equals the state `period`
+    iterations earlier (-1: none within 16), and that period."""
+    out = np.zeros(4, np.int32)
+    a = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(a), hulls))
+    b = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(b), hulls))
+    oracle_lib().orc_gjk_trace(1 if fp32 else 0, a.ctypes.data_as(_P), _ptr(np.ascontiguousarray(tfa, np.float64), _D),
+                               b.ctypes.data_as(_P), _ptr(np.ascontiguousarray(tfb, np.float64), _D), _ptr(out, _I))
+    return {"gjk_status": int(out[0]), "gjk_iters": int(out[1]), "first_repeat": int(out[2]), "period": int(out[3])}
 
 
 def ref_gjk_epa(a, tfa, b, tfb, hulls=None):

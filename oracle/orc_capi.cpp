@@ -97,6 +97,8 @@ struct Any {
     virtual int njoints() const = 0;
     virtual void joint_state(double *cur, double *tgt, double *lam4, int32_t *colour) const = 0;
     virtual void set_joint_targets(int n, const uint32_t *ids, const double *tgt, const double *speed) = 0;
+    virtual void set_rows(int n, const uint32_t *ids, const double *dir3, const double *mag) = 0;
+    virtual void set_forces(const double *f3, const double *t3) = 0;
 };
 
 template <class T> typename orc::World<T>::Joint joint_from(const orc_joint_desc &d) {
@@ -208,6 +210,20 @@ template <class T> struct Impl : Any {
             auto &j = w.joints[ids[k]];
             if (tgt) j.target = T(tgt[k]);
             if (speed) j.speed = T(speed[k]);
+        }
+    }
+    void set_rows(int n, const uint32_t *ids, const double *dir3, const double *mag) override {
+        for (int k = 0; k < n; ++k) {
+            auto &j = w.joints[ids[k]];
+            if (j.type != orc::World<T>::J_ROW) continue;
+            if (dir3) j.dir[0] = orc::V3<T>(T(dir3[3 * k]), T(dir3[3 * k + 1]), T(dir3[3 * k + 2]));
+            if (mag) j.mag[0] = T(mag[k]);
+        }
+    }
+    void set_forces(const double *f3, const double *t3) override {
+        for (size_t i = 0; i < w.force.size(); ++i) {
+            if (f3) w.force[i] = orc::V3<T>(T(f3[3 * i]), T(f3[3 * i + 1]), T(f3[3 * i + 2]));
+            if (t3) w.torque[i] = orc::V3<T>(T(t3[3 * i]), T(t3[3 * i + 1]), T(t3[3 * i + 2]));
         }
     }
 };
@@ -400,7 +416,7 @@ int orc_num_colours(void *h) { return static_cast<Any *>(h)->ncolours(); }
 int orc_colour_rounds(void *h) { return static_cast<Any *>(h)->colour_rounds(); }
 int orc_add_joints(void *h, int n, const orc_joint_desc *d) {
     for (int i = 0; i < n; ++i) {
-        if (d[i].type < 0 || d[i].type > 3) return -1;
+        if (d[i].type < 0 || d[i].type > 4) return -1;
         static_cast<Any *>(h)->add_joint(d[i]);
     }
     return 0;
@@ -427,6 +443,12 @@ int orc_set_palette_max(void *h, int m) {
     return 0;
 }
 uint32_t orc_mix32(uint32_t x) { return orc::mix32(x); }
+// Direction and magnitude of constraint rows (joints of type 4), as callers write them.
+void orc_set_rows(void *h, int n, const uint32_t *ids, const double *dir3, const double *mag) {
+    static_cast<Any *>(h)->set_rows(n, ids, dir3, mag);
+}
+// BodyCollection::force / torque of every body (public fields the reference's callers write).
+void orc_set_forces(void *h, const double *f3, const double *t3) { static_cast<Any *>(h)->set_forces(f3, t3); }
 // Test infrastructure: plants a colour-schedule mistake (see World::planted_fault).
 void orc_set_planted_fault(void *h, int f) { static_cast<Any *>(h)->set_planted_fault(f); }
 // The last substep's colour-order solve list (pair indices of the frame's pair list).

@@ -87,7 +87,7 @@ template <class T> struct World {
     std::vector<Contact<T>> pcont;                 // 4 slots per pair
 
     // ---- joints (Joint.hpp:82-106) with their constraint rows (Constraint.hpp:31-53)
-    enum { J_PRISMATIC = 0, J_REVOLUTE = 1, J_FIXED = 2, J_SPHERICAL = 3 };
+    enum { J_PRISMATIC = 0, J_REVOLUTE = 1, J_FIXED = 2, J_SPHERICAL = 3, J_ROW = 4 };
     enum { ACT_FREE = 0, ACT_POSITION = 1, ACT_SPEED = 2 };
     struct Joint {
         int type = 0, act = 0;
@@ -188,6 +188,9 @@ template <class T> struct World {
             case J_FIXED:
                 j.nrows = 2; j.rot[0] = true; j.act = ACT_FREE; j.limited = false; j.lower = j.upper = T(0);
                 j.axis = j.laxis = V3<T>(); j.damping = T(0);
+                break;
+            case J_ROW:   // a constraint row (ConstraintAPI.hpp:37-64): act = its type, direction (1,0,0), magnitude 0
+                j.nrows = 1; j.rot[0] = j.act == 1; j.comp[0] = compliance; j.damping = T(0);
                 break;
             default:
                 j.nrows = j.limited ? 2 : 1; j.rot[1] = true; j.comp[0] = compliance; break;
@@ -555,6 +558,7 @@ template <class T> struct World {
     // Joint.cpp:18-71 (prismatic), 92-159 (revolute), 161-182 (fixed); spherical: attachment
     // row, and a swing limit built like the revolute angle limit (Joint.cpp:73-90).
     void joint_errors(Joint &j, T h, bool advance_target = true) {
+        if (j.type == J_ROW) return;   // a constraint row keeps the direction and magnitude its caller set
         const uint32_t b1 = j.b1, b2 = j.b2;
         const V3<T> r1w = rotate(q[b1], j.r1), r2w = rotate(q[b2], j.r2);
         const V3<T> p1 = pos[b1] + r1w, p2 = pos[b2] + r2w;

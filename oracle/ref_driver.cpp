@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <unordered_map>
 #include <vector>
 #include "rbps/API/World.hpp"
 #include "rbc/Dispatcher.hpp"
@@ -51,11 +52,29 @@ namespace {
 
 m3d::vec3 v3(const double *a) { return m3d::vec3(a[0], a[1], a[2]); }
 
+// The driver's joint list: joints (types 0-2, their JointCollection id) and free constraint
+// rows (type 4, made by World::create_constraint; their ConstraintCollection id), in
+// creation order, per world.
+struct Entry {
+    int type;
+    uint32_t id;
+};
+std::unordered_map<const void *, std::vector<Entry>> g_entries;
+
 // Creates one joint through the reference's World API; target fields are the public
 // joint data a caller sets after creation.  Spherical (3) has no reference counterpart.
+// Type 4: a constraint row (World::create_constraint, ConstraintAPI.hpp:37-64) with
+// actuation = row type (0 POSITIONAL, 1 ROTATIONAL), r_1, r_2 and compliance.
 int add_joint(rbps::World &w, const ref_joint_desc &d) {
     const auto act = static_cast<rbps::JointActuationType>(d.actuation);
     uint32_t id;
+    if (d.type == 4) {
+        rbps::ConstraintParams p;
+        p.body_1 = d.body_1; p.body_2 = d.body_2; p.r_1 = v3(d.r_1); p.r_2 = v3(d.r_2); p.compliance = d.compliance;
+        p.type = d.actuation == 1 ? rbps::ConstraintType::ROTATIONAL : rbps::ConstraintType::POSITIONAL;
+        g_entries[&w].push_back({4, w.create_constraint(p)});
+        return 0;
+    }
     if (d.type == 0) {
         rbps::PrismaticJointParams p;
         p.body_1 = d.body_1; p.body_2 = d.body_2; p.moving_axis = v3(d.axis); p.r_1 = v3(d.r_1); p.r_2 = v3(d.r_2);
@@ -78,6 +97,7 @@ int add_joint(rbps::World &w, const ref_joint_desc &d) {
     const uint32_t i = w.joints.index_of(id);
     w.joints.target_position[i] = d.target_position;
     w.joints.target_speed[i] = d.target_speed;
+    g_entries[&w].push_back({d.type, id});
     return 0;
 }
 
@@ -114,7 +134,10 @@ void *ref_create(double timestep, int substeps) {
     w->broad_phase_state.dirty = false;
     return w;
 }
-void ref_destroy(void *h) { delete static_cast<rbps::World *>(h); }
+void ref_destroy(void *h) {
+    g_entries.erase(h);
+    delete static_cast<rbps::World *>(h);
+}
 
 int ref_add_bodies(void *h, int n, const ref_body_desc *d) {
     auto *w = static_cast<rbps::World *>(h);
@@ -228,25 +251,56 @@ int ref_add_joints(void *h, int n, const ref_joint_desc *d) {
         if (add_joint(*w, d[i])) return -1;
     return 0;
 }
-int ref_num_joints(void *h) { return (int)static_cast<rbps::World *>(h)->joints.count(); }
+int ref_num_joints(void *h) { return (int)g_entries[h].size(); }
 void ref_get_joints(void *h, double *cur, double *tgt, double *lam4, int32_t *colour) {
     auto *w = static_cast<rbps::World *>(h);
     auto &jc = w->joints;
     auto &cc = w->constraints;
-    for (uint32_t i = 0; i < jc.count(); ++i) {
-        if (cur) cur[i] = jc.current_position[i];
-        if (tgt) tgt[i] = jc.target_position[i];
+    const auto &E = g_entries[h];
+    for (uint32_t e = 0; e < E.size(); ++e) {
+        if (E[e].type == 4) {   // a constraint row: its lambda
+            if (cur) cur[e] = 0.0;
+            if (tgt) tgt[e] = 0.0;
+            if (lam4) for (uint32_t k = 0; k < 4; ++k) lam4[4 * e + k] = k == 0 ? cc.lambda[cc.index_of(E[e].id)] : 0.0;
+            if (colour) colour[e] = -1;
+            continue;
+        }
+        const uint32_t i = jc.index_of(E[e].id);
+        if (cur) cur[e] = jc.current_position[i];
+        if (tgt) tgt[e] = jc.target_position[i];
         if (lam4)
             for (uint32_t k = 0; k < 4; ++k)
-                lam4[4 * i + k] = k < jc.constraint_count[i] ? cc.lambda[cc.index_of(jc.constraints[i].ids[k])] : 0.0;
-        if (colour) colour[i] = -1;
+                lam4[4 * e + k] = k < jc.constraint_count[i] ? cc.lambda[cc.index_of(jc.constraints[i].ids[k])] : 0.0;
+        if (colour) colour[e] = -1;
     }
 }
 void ref_set_joint_targets(void *h, int n, const uint32_t *ids, const double *tgt, const double *speed) {
     auto &jc = static_cast<rbps::World *>(h)->joints;
+    const auto &E = g_entries[h];
     for (int k = 0; k < n; ++k) {
-        if (tgt) jc.target_position[ids[k]] = tgt[k];
-        if (speed) jc.target_speed[ids[k]] = speed[k];
+        if (E.at(ids[k]).type == 4) continue;
+        const uint32_t i = jc.index_of(E[ids[k]].id);
+        if (tgt) jc.target_position[i] = tgt[k];
+        if (speed) jc.target_speed[i] = speed[k];
+    }
+}
+// The public ConstraintCollection::direction / magnitude of constraint rows (entries of
+// type 4), which the reference's callers write directly.
+void ref_set_rows(void *h, int n, const uint32_t *ids, const double *dir3, const double *mag) {
+    auto &cc = static_cast<rbps::World *>(h)->constraints;
+    const auto &E = g_entries[h];
+    for (int k = 0; k < n; ++k) {
+        if (E.at(ids[k]).type != 4) continue;
+        const uint32_t i = cc.index_of(E[ids[k]].id);
+        if (dir3) cc.direction[i] = m3d::vec3(dir3[3 * k], dir3[3 * k + 1], dir3[3 * k + 2]);
+        if (mag) cc.magnitude[i] = mag[k];
+    }
+}
+void ref_set_forces(void *h, const double *f3, const double *t3) {
+    auto &b = static_cast<rbps::World *>(h)->bodies;
+    for (uint32_t i = 0; i < b.count(); ++i) {
+        if (f3) b.force[i] = m3d::vec3(f3[3 * i], f3[3 * i + 1], f3[3 * i + 2]);
+        if (t3) b.torque[i] = m3d::vec3(t3[3 * i], t3[3 * i + 1], t3[3 * i + 2]);
     }
 }
 // compute_*_joint_errors (Joint.cpp:18-182) for one joint between two bodies at given poses.
@@ -263,7 +317,9 @@ int ref_joint_errors(const ref_joint_desc *d, const double *pose1, const double 
     }
     ref_joint_desc dd = *d;
     dd.body_1 = 0; dd.body_2 = 1;
-    if (add_joint(w, dd)) return -1;
+    const int rc = add_joint(w, dd);
+    g_entries.erase(&w);
+    if (rc) return -1;
     rbps::compute_joint_errors(w.joints, w.bodies, w.constraints, h);
     auto &jc = w.joints;
     auto &cc = w.constraints;

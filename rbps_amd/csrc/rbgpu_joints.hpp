@@ -19,12 +19,15 @@
 //   never reset, as in the reference).
 // Row kinds per type (JointAPI.hpp): prismatic R P P(compliance), revolute R P R R(c),
 // fixed R P; spherical (this build's extension): P(c) and, when limited, a swing-limit R.
+// A constraint row made by World::create_constraint (ConstraintAPI.hpp:37-64) is a
+// one-row unit of type ROW (actuation = its kind, 1 ROTATIONAL): nothing evaluates its
+// errors, it keeps the direction and magnitude its caller writes.
 #pragma once
 #include "rbgpu_math.hpp"
 
 namespace rb {
 
-enum JointType { JT_PRISMATIC = 0, JT_REVOLUTE = 1, JT_FIXED = 2, JT_SPHERICAL = 3 };
+enum JointType { JT_PRISMATIC = 0, JT_REVOLUTE = 1, JT_FIXED = 2, JT_SPHERICAL = 3, JT_ROW = 4 };
 enum JointAct { JA_FREE = 0, JA_POSITION = 1, JA_SPEED = 2 };
 
 struct Joints {
@@ -39,17 +42,18 @@ __device__ __forceinline__ int jt_act(int4 m) { return (m.x >> 8) & 0xff; }
 __device__ __forceinline__ bool jt_limited(int4 m) { return ((m.x >> 16) & 0xff) != 0; }
 __device__ __forceinline__ int jt_rows(int4 m) { return (m.x >> 24) & 0xff; }
 // ROTATIONAL row?  (POSITIONAL otherwise)
-__device__ __forceinline__ bool jt_rot(int type, int k) {
-    switch (type) {
+__device__ __forceinline__ bool jt_rot(int4 m, int k) {
+    switch (jt_type(m)) {
         case JT_PRISMATIC: return k == 0;
         case JT_REVOLUTE: return k != 1;
         case JT_FIXED: return k == 0;
+        case JT_ROW: return jt_act(m) == 1;
         default: return k == 1;
     }
 }
 // The row carrying the joint's compliance (the others have 0).
 __device__ __forceinline__ int jt_comp_row(int type) {
-    return type == JT_PRISMATIC ? 2 : (type == JT_REVOLUTE ? 3 : (type == JT_SPHERICAL ? 0 : -1));
+    return type == JT_PRISMATIC ? 2 : (type == JT_REVOLUTE ? 3 : (type == JT_SPHERICAL || type == JT_ROW ? 0 : -1));
 }
 
 // Joint solver modes.
@@ -82,8 +86,9 @@ __device__ __forceinline__ void joint_rows(int4 m, float4 r1, float4 r2, float4 
     // the rows are set in locals and stored once at the end: stores to different rows in
     // different branches could otherwise be merged into one store at a run-time index,
     // which puts `row` in scratch memory
-    float4 o0 = row[0], o1 = row[1], o2 = row[2], o3 = row[3];
     const int type = jt_type(m), act = jt_act(m);
+    if (type == JT_ROW) return;   // a constraint row keeps what its caller wrote
+    float4 o0 = row[0], o1 = row[1], o2 = row[2], o3 = row[3];
     const V3 p1 = x1 + rotate(q1, xyz(r1)), p2 = x2 + rotate(q2, xyz(r2));
     if (type == JT_PRISMATIC || type == JT_FIXED) {   // Joint.cpp:27-29, 170-172
         const Q dq = qmul(q1, conj(q2));
@@ -209,7 +214,7 @@ __device__ __forceinline__ void joint_solve(Bodies &B, Joints &J, uint32_t i, fl
         const V3 n = xyz(rv);
         const float comp = (k == crow) ? r2f.w : 0.0f;
         float lam = J.jlam[(size_t)i * 4 + k];
-        if (jt_rot(type, k)) {
+        if (jt_rot(m, k)) {
             const float w1 = w_rot(A, n), w2 = w_rot(Bd, n);
             const float ap = comp * inv_h * inv_h;
             const float dl = (-rv.w - ap * lam) / (w1 + w2 + ap);

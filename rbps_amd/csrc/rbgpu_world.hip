@@ -104,6 +104,21 @@ struct rbgpu_world {
     std::vector<HostHull> hh;           // convex hull data (rbgpu_add_hull)
     std::vector<float4> h_jtg;          // joint target / speed / current (survives rebuilds)
     std::vector<float> h_jlam;          // joint row lambdas (never reset)
+    std::vector<float4> h_jrow_user;    // a constraint row's (direction, magnitude) as its caller wrote it
+    std::vector<float4> h_force, h_torque;   // caller-written force / torque (w = 1), else gravity / zero
+    // BodyCollection::force / torque rows as the integrator reads them: xyz and, in force.w, the mass
+    void body_forces(std::vector<float4> &f, std::vector<float4> &t) const {
+        const size_t n = hb.size();
+        f.resize(n);
+        t.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            const float mass = (float)hb[i].mass;
+            const bool uf = i < h_force.size() && h_force[i].w != 0.0f, ut = i < h_torque.size() && h_torque[i].w != 0.0f;
+            f[i] = uf ? make_float4(h_force[i].x, h_force[i].y, h_force[i].z, mass)
+                      : make_float4(0.0f, -9.8f * mass, 0.0f, mass);   // BodyAPI.hpp:61
+            t[i] = ut ? make_float4(h_torque[i].x, h_torque[i].y, h_torque[i].z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
     std::vector<float4> h_fmin, h_fmax;
     std::vector<uint32_t> h_uid, h_key, h_val;
     std::vector<uint2> h_eppos;
@@ -658,7 +673,8 @@ float rbgpu_world::bounding_radius(const rbgpu_collider_desc &d) const {
 int rbgpu_world::build() {
     // Keep broadphase history; bodies come from the descriptors (or the downloaded state).
     const uint32_t n = (uint32_t)hb.size(), m = (uint32_t)hc.size();
-    std::vector<float4> x(n), q(n), v(n), w(n), f(n), t(n), I0(n), I1(n), Ii0(n), Ii1(n);
+    std::vector<float4> x(n), q(n), v(n), w(n), f, t, I0(n), I1(n), Ii0(n), Ii1(n);
+    body_forces(f, t);
     for (uint32_t i = 0; i < n; ++i) {
         const rbgpu_body_desc &d = hb[i];
         const bool dyn = d.type != RBGPU_STATIC;
@@ -671,8 +687,6 @@ int rbgpu_world::build() {
         q[i] = make_float4((float)d.orientation[0], (float)d.orientation[1], (float)d.orientation[2], (float)d.orientation[3]);
         v[i] = make_float4((float)d.linear_velocity[0], (float)d.linear_velocity[1], (float)d.linear_velocity[2], 0.0f);
         w[i] = make_float4((float)d.angular_velocity[0], (float)d.angular_velocity[1], (float)d.angular_velocity[2], 0.0f);
-        f[i] = make_float4(0.0f, -9.8f * mass, 0.0f, mass);   // BodyAPI.hpp:61
-        t[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         I0[i] = make_float4(I.xx, I.yy, I.zz, I.xy);
         I1[i] = make_float4(I.xz, I.yz, 0.0f, 0.0f);
         Ii0[i] = make_float4(Ii.xx, Ii.yy, Ii.zz, Ii.xy);
@@ -859,11 +873,13 @@ int rbgpu_world::build_joints() {
     int ncolj = 0;
     for (uint32_t i = 0; i < NJ; ++i) {
         const rbgpu_joint_desc &d = hj[i];
+        const bool row = d.type == RBGPU_JOINT_ROW;
         int rows = d.type == RBGPU_JOINT_PRISMATIC ? 3 : d.type == RBGPU_JOINT_REVOLUTE ? 4
-                 : d.type == RBGPU_JOINT_FIXED ? 2 : (d.limited ? 2 : 1);
-        const bool fixed = d.type == RBGPU_JOINT_FIXED;
-        const int act = fixed ? RBGPU_ACT_FREE : d.actuation;
+                 : d.type == RBGPU_JOINT_FIXED ? 2 : row ? 1 : (d.limited ? 2 : 1);
+        const bool fixed = d.type == RBGPU_JOINT_FIXED || row;   // a row has no axes, limits or damping
+        const int act = d.type == RBGPU_JOINT_FIXED ? RBGPU_ACT_FREE : d.actuation;   // a row: its kind
         const int lim = fixed ? 0 : (d.limited ? 1 : 0);
+        if (row) jrow[(size_t)i * 4] = h_jrow_user[i];
         jm[i] = make_int4(d.type | (act << 8) | (lim << 16) | (rows << 24), (int)d.body_1, (int)d.body_2, 0);
         jr1[i] = make_float4((float)d.r_1[0], (float)d.r_1[1], (float)d.r_1[2], fixed ? 0.0f : (float)d.damping);
         jr2[i] = make_float4((float)d.r_2[0], (float)d.r_2[1], (float)d.r_2[2], (float)d.compliance);
@@ -1314,7 +1330,9 @@ int rbgpu_add_joints(rbgpu_world *w, int n, const rbgpu_joint_desc *desc, uint32
     if (!w || n < 0 || (n && !desc)) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     for (int i = 0; i < n; ++i) {
         const rbgpu_joint_desc &d = desc[i];
-        if (d.type < RBGPU_JOINT_PRISMATIC || d.type > RBGPU_JOINT_SPHERICAL) return fail(RBGPU_ERR_INVALID, "unknown joint type");
+        if (d.type < RBGPU_JOINT_PRISMATIC || d.type > RBGPU_JOINT_ROW) return fail(RBGPU_ERR_INVALID, "unknown joint type");
+        if (d.type == RBGPU_JOINT_ROW && d.actuation != 0 && d.actuation != 1)
+            return fail(RBGPU_ERR_INVALID, "constraint row type is 0 (positional) or 1 (rotational)");
         if (d.actuation < RBGPU_ACT_FREE || d.actuation > RBGPU_ACT_SPEED) return fail(RBGPU_ERR_INVALID, "unknown actuation");
         if (d.body_1 >= w->hb.size() || d.body_2 >= w->hb.size()) return fail(RBGPU_ERR_INVALID, "joint body out of range");
         if (d.body_1 == d.body_2) return fail(RBGPU_ERR_INVALID, "joint needs two different bodies");
@@ -1327,6 +1345,7 @@ int rbgpu_add_joints(rbgpu_world *w, int n, const rbgpu_joint_desc *desc, uint32
         w->hj.push_back(desc[i]);
         w->h_jtg.push_back(make_float4((float)desc[i].target_position, (float)desc[i].target_speed, 0.0f, 0.0f));
         for (int k = 0; k < 4; ++k) w->h_jlam.push_back(0.0f);
+        w->h_jrow_user.push_back(make_float4(1.0f, 0.0f, 0.0f, 0.0f));   // ConstraintAPI.hpp:52-54
     }
     w->built = false;
     return RBGPU_OK;
@@ -1364,6 +1383,56 @@ int rbgpu_set_joint_targets(rbgpu_world *w, int n, const uint32_t *ids, const do
     }
     if (w->built && w->NJ) {
         HIPCHK(hipMemcpy(w->Jt.jtg, w->h_jtg.data(), w->NJ * 16, hipMemcpyHostToDevice));
+    }
+    return RBGPU_OK;
+}
+
+int rbgpu_add_constraints(rbgpu_world *w, int n, const rbgpu_constraint_desc *desc, uint32_t *ids_out) {
+    if (!w || n < 0 || (n && !desc)) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    std::vector<rbgpu_joint_desc> d(n);
+    for (int i = 0; i < n; ++i) {
+        if (desc[i].type != 0 && desc[i].type != 1)
+            return fail(RBGPU_ERR_INVALID, "constraint row type is 0 (positional) or 1 (rotational)");
+        d[i] = rbgpu_joint_desc{};
+        d[i].type = RBGPU_JOINT_ROW;
+        d[i].actuation = desc[i].type;
+        d[i].body_1 = desc[i].body_1;
+        d[i].body_2 = desc[i].body_2;
+        for (int k = 0; k < 3; ++k) { d[i].r_1[k] = desc[i].r_1[k]; d[i].r_2[k] = desc[i].r_2[k]; }
+        d[i].compliance = desc[i].compliance;
+    }
+    return rbgpu_add_joints(w, n, d.data(), ids_out);
+}
+
+int rbgpu_set_constraints(rbgpu_world *w, int n, const uint32_t *ids, const double *direction3, const double *magnitude) {
+    if (!w || n < 0 || (n && !ids)) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    for (int k = 0; k < n; ++k) {
+        if (ids[k] >= w->hj.size()) return fail(RBGPU_ERR_INVALID, "constraint id out of range");
+        if (w->hj[ids[k]].type != RBGPU_JOINT_ROW) return fail(RBGPU_ERR_INVALID, "not a constraint row (a joint's rows follow its errors)");
+    }
+    if (w->built && w->NJ) HIPCHK(hipStreamSynchronize(w->st));
+    for (int k = 0; k < n; ++k) {
+        float4 &r = w->h_jrow_user[ids[k]];
+        if (direction3) { r.x = (float)direction3[3 * k]; r.y = (float)direction3[3 * k + 1]; r.z = (float)direction3[3 * k + 2]; }
+        if (magnitude) r.w = (float)magnitude[k];
+        if (w->built && ids[k] < w->NJ) HIPCHK(hipMemcpy(w->Jt.jrow + (size_t)ids[k] * 4, &r, 16, hipMemcpyHostToDevice));
+    }
+    return RBGPU_OK;
+}
+
+int rbgpu_get_constraint_rows(rbgpu_world *w, double *direction3, double *magnitude) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    const size_t nj = w->hj.size();
+    std::vector<float4> rows(nj * 4, make_float4(1.0f, 0.0f, 0.0f, 0.0f));
+    for (size_t i = 0; i < nj; ++i)
+        if (w->hj[i].type == RBGPU_JOINT_ROW) rows[i * 4] = w->h_jrow_user[i];
+    if (w->built && w->NJ) {
+        if (int rcs = w->sync()) return rcs;
+        HIPCHK(hipMemcpy(rows.data(), w->Jt.jrow, (size_t)w->NJ * 64, hipMemcpyDeviceToHost));
+    }
+    for (size_t j = 0; j < nj * 4; ++j) {
+        if (direction3) { direction3[3 * j] = rows[j].x; direction3[3 * j + 1] = rows[j].y; direction3[3 * j + 2] = rows[j].z; }
+        if (magnitude) magnitude[j] = rows[j].w;
     }
     return RBGPU_OK;
 }
@@ -1517,6 +1586,36 @@ int rbgpu_set_bodies(rbgpu_world *w, const double *p, const double *q, const dou
     return RBGPU_OK;
 }
 
+int rbgpu_set_body_forces(rbgpu_world *w, const double *force3, const double *torque3) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    const size_t n = w->hb.size();
+    w->h_force.resize(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    w->h_torque.resize(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (size_t i = 0; i < n; ++i) {
+        if (force3) w->h_force[i] = make_float4((float)force3[3 * i], (float)force3[3 * i + 1], (float)force3[3 * i + 2], 1.0f);
+        if (torque3)
+            w->h_torque[i] = make_float4((float)torque3[3 * i], (float)torque3[3 * i + 1], (float)torque3[3 * i + 2], 1.0f);
+    }
+    if (!w->built || !n) return RBGPU_OK;
+    if (int rcs = w->sync()) return rcs;
+    std::vector<float4> f, t;
+    w->body_forces(f, t);
+    HIPCHK(hipMemcpy(w->B.f, f.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(w->B.t, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    return RBGPU_OK;
+}
+
+int rbgpu_get_body_forces(rbgpu_world *w, double *force3, double *torque3) {
+    if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    std::vector<float4> f, t;
+    w->body_forces(f, t);
+    for (size_t i = 0; i < f.size(); ++i) {
+        if (force3) { force3[3 * i] = f[i].x; force3[3 * i + 1] = f[i].y; force3[3 * i + 2] = f[i].z; }
+        if (torque3) { torque3[3 * i] = t[i].x; torque3[3 * i + 1] = t[i].y; torque3[3 * i + 2] = t[i].z; }
+    }
+    return RBGPU_OK;
+}
+
 int rbgpu_get_sap_pairs(rbgpu_world *w, uint32_t *pairs2, uint32_t cap, uint32_t *n) {
     if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
     *n = w->built ? w->n_sap : 0;
@@ -1575,6 +1674,50 @@ int rbgpu_get_contacts(rbgpu_world *w, uint32_t *ids4, double *n3, double *ra3, 
         }
     }
     *n = k;
+    return RBGPU_OK;
+}
+
+int rbgpu_get_contact_points(rbgpu_world *w, double *point_on_a3, double *point_on_b3, double *relative_velocity,
+                             int32_t *collision, uint32_t cap, uint32_t *n) {
+    if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    *n = 0;
+    if (!w->built || !w->n_pairs) return RBGPU_OK;
+    if (int rcs = w->sync()) return rcs;
+    const uint32_t P = w->n_pairs;
+    std::vector<int> cnt(P);
+    std::vector<float4> pt(16 * (size_t)P);
+    HIPCHK(hipMemcpy(cnt.data(), w->K.cnt, P * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pt.data(), w->K.pt, (size_t)P * 256, hipMemcpyDeviceToHost));
+    uint32_t k = 0;
+    for (uint32_t s = 0; s < P; ++s)
+        for (int i = 0; i < cnt[s]; ++i, ++k) {
+            if (k >= cap) continue;
+            const float4 a = pt[4 * ((size_t)s * 4 + i) + 2], b = pt[4 * ((size_t)s * 4 + i) + 3];
+            if (point_on_a3) { point_on_a3[3 * k] = a.x; point_on_a3[3 * k + 1] = a.y; point_on_a3[3 * k + 2] = a.z; }
+            if (point_on_b3) { point_on_b3[3 * k] = b.x; point_on_b3[3 * k + 1] = b.y; point_on_b3[3 * k + 2] = b.z; }
+            if (relative_velocity) relative_velocity[k] = a.w;
+            if (collision) collision[k] = b.w != 0.0f ? 1 : 0;
+        }
+    *n = k;
+    return RBGPU_OK;
+}
+
+int rbgpu_get_broadphase_objects(rbgpu_world *w, double *fat_min3, double *fat_max3, uint32_t *user_id, uint32_t cap,
+                                 uint32_t *n) {
+    if (!w || !n) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    const uint32_t S = (uint32_t)w->h_uid.size();
+    *n = S;
+    std::vector<float4> mn(w->h_fmin), mx(w->h_fmax);
+    if (w->built && w->S) {
+        if (int rcs = w->sync()) return rcs;
+        HIPCHK(hipMemcpy(mn.data(), w->fmin, (size_t)w->S * 16, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mx.data(), w->fmax, (size_t)w->S * 16, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < S && i < cap; ++i) {
+        if (fat_min3) { fat_min3[3 * i] = mn[i].x; fat_min3[3 * i + 1] = mn[i].y; fat_min3[3 * i + 2] = mn[i].z; }
+        if (fat_max3) { fat_max3[3 * i] = mx[i].x; fat_max3[3 * i + 1] = mx[i].y; fat_max3[3 * i + 2] = mx[i].z; }
+        if (user_id) user_id[i] = w->h_uid[i];
+    }
     return RBGPU_OK;
 }
 

@@ -22,6 +22,8 @@ import numpy as np
 
 STATIC, DYNAMIC = 0, 1
 PRISMATIC, REVOLUTE, FIXED, SPHERICAL = 0, 1, 2, 3      # Joint.hpp JointType (+ spherical)
+ROW = 4   # a constraint row (World::create_constraint): actuation = its type, 0 POSITIONAL, 1 ROTATIONAL
+ROW_POSITIONAL, ROW_ROTATIONAL = 0, 1                    # Constraint.hpp ConstraintType
 ACT_FREE, ACT_POSITION, ACT_SPEED = 0, 1, 2            # Joint.hpp JointActuationType
 SPHERE, BOX, CAPSULE, PLANE, HULL = 0, 1, 3, 6, 9
 
@@ -46,6 +48,12 @@ JOINT_DTYPE = np.dtype([
     ("lower_limit", "<f8"), ("upper_limit", "<f8"), ("target_position", "<f8"), ("target_speed", "<f8"),
 ], align=True)
 assert JOINT_DTYPE.itemsize == 168
+
+CONSTRAINT_DTYPE = np.dtype([   # rbgpu_constraint_desc: the fields of rbps::ConstraintParams
+    ("body_1", "<u4"), ("body_2", "<u4"), ("type", "<i4"), ("_pad", "<i4"),
+    ("r_1", "<f8", (3,)), ("r_2", "<f8", (3,)), ("compliance", "<f8"),
+], align=True)
+assert CONSTRAINT_DTYPE.itemsize == 72
 
 
 # ---------------------------------------------------------------- unit-density mass properties
@@ -521,6 +529,39 @@ def planar_joints(seed=5) -> Scene:
                     (-3.2 + 0.6 * k, 1.5 + 0.5 * k + 0.1 * rng.random(), 0.0))
         sb.collider(SPHERE, b, (rr,), restitution=0.0)
     return sb.build("planar_joints", 1.0 / 60.0, 20, 120, seed=seed)
+
+
+def constraint_rows(seed=11) -> Scene:
+    """Constraint rows made by World::create_constraint (ConstraintAPI.hpp:37-64) between
+    jointed bodies, created between the joints (the reference solves all rows in creation
+    order), planar like planar_joints: two hinged pendulums about z, a POSITIONAL row along
+    x between the bobs, a ROTATIONAL row about z between them and a zero row from a box on
+    the plane to a static anchor; spheres land on that box.  meta["rows"]: (index in
+    `joints`, direction, magnitude) that the caller writes after loading, as reference
+    callers write ConstraintCollection::direction / magnitude."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    add_ground_plane(sb)
+    p1 = _box(sb, (0.0, 5.0, 0.0), (0.1, 0.1, 0.1), type_=STATIC)
+    b1 = _box(sb, (1.0, 5.0, 0.0), (0.4, 0.1, 0.1))
+    p2 = _box(sb, (3.0, 5.0, 0.0), (0.1, 0.1, 0.1), type_=STATIC)
+    b2 = _box(sb, (4.0, 5.0, 0.0), (0.4, 0.1, 0.1))
+    sb.joint(REVOLUTE, p1, b1, r_2=(-1.0, 0.0, 0.0), axis=(0, 0, 1), limit_axis=(1, 0, 0))
+    rows = []
+    rows.append((len(sb.j), (1.0, 0.0, 0.0), 0.01))
+    sb.joint(ROW, b1, b2, r_1=(0.4, 0.0, 0.0), r_2=(-0.4, 0.0, 0.0), actuation=ROW_POSITIONAL, compliance=1e-4)
+    sb.joint(REVOLUTE, p2, b2, r_2=(-1.0, 0.0, 0.0), axis=(0, 0, 1), limit_axis=(1, 0, 0), compliance=0.0)
+    rows.append((len(sb.j), (0.0, 0.0, 1.0), 0.002))
+    sb.joint(ROW, b1, b2, actuation=ROW_ROTATIONAL, compliance=1e-3)
+    base = _box(sb, (-3.0, 0.3, 0.0), (0.6, 0.3, 0.3))
+    anchor = _box(sb, (-3.0, 3.0, 0.0), (0.1, 0.1, 0.1), type_=STATIC)
+    sb.joint(ROW, base, anchor, r_1=(0.0, 0.3, 0.0), actuation=ROW_POSITIONAL)   # direction (1,0,0), magnitude 0
+    for k in range(3):
+        rr = 0.15 + 0.05 * k
+        b = sb.body(DYNAMIC, 1000.0 * sphere_volume(rr), tuple(1000.0 * x for x in sphere_inertia(rr)),
+                    (-3.3 + 0.3 * k, 1.2 + 0.5 * k + 0.1 * rng.random(), 0.0))
+        sb.collider(SPHERE, b, (rr,), restitution=0.0)
+    return sb.build("constraint_rows", 1.0 / 60.0, 20, 60, seed=seed, rows=rows)
 
 
 def batch(parts, name="batch") -> Scene:

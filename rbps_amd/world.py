@@ -29,7 +29,8 @@ EXPORTS = [
     "rbgpu_world_create", "rbgpu_world_destroy", "rbgpu_add_bodies", "rbgpu_add_colliders", "rbgpu_step",
     "rbgpu_set_step", "rbgpu_synchronize", "rbgpu_get_bodies", "rbgpu_set_bodies", "rbgpu_get_sap_pairs", "rbgpu_get_pair_colours",
     "rbgpu_get_contacts", "rbgpu_get_stats", "rbgpu_set_timing", "rbgpu_get_timing", "rbgpu_collide_pairs",
-    "rbgpu_collide_pairs_noalloc",
+    "rbgpu_collide_pairs_noalloc", "rbgpu_add_constraints", "rbgpu_set_constraints", "rbgpu_get_constraint_rows",
+    "rbgpu_get_contact_points", "rbgpu_get_broadphase_objects", "rbgpu_set_body_forces", "rbgpu_get_body_forces",
     "rbgpu_autotune",
     "rbgpu_set_option", "rbgpu_get_option", "rbgpu_add_joints", "rbgpu_get_joints", "rbgpu_set_joint_targets",
     "rbgpu_add_hull", "rbgpu_hull_mass_properties", "rbgpu_hull_tables", "rbgpu_collide_world_pairs",
@@ -74,6 +75,8 @@ def lib():
         L.rbgpu_synchronize.argtypes = [_P]
         L.rbgpu_get_bodies.argtypes = [_P, _D, _D, _D, _D]
         L.rbgpu_set_bodies.argtypes = [_P, _D, _D, _D, _D]
+        L.rbgpu_set_body_forces.argtypes = [_P, _D, _D]
+        L.rbgpu_get_body_forces.argtypes = [_P, _D, _D]
         L.rbgpu_get_sap_pairs.argtypes = [_P, _U, C.c_uint32, _U]
         L.rbgpu_get_pair_colours.argtypes = [_P, _U, _I, C.c_uint32, _U]
         L.rbgpu_get_contacts.argtypes = [_P, _U, _D, _D, _D, _D, _D, C.c_uint32, _U]
@@ -82,6 +85,11 @@ def lib():
         L.rbgpu_add_joints.argtypes = [_P, C.c_int, _P, _U]
         L.rbgpu_get_joints.argtypes = [_P, _D, _D, _D, _I]
         L.rbgpu_set_joint_targets.argtypes = [_P, C.c_int, _U, _D, _D]
+        L.rbgpu_add_constraints.argtypes = [_P, C.c_int, _P, _U]
+        L.rbgpu_set_constraints.argtypes = [_P, C.c_int, _U, _D, _D]
+        L.rbgpu_get_constraint_rows.argtypes = [_P, _D, _D]
+        L.rbgpu_get_contact_points.argtypes = [_P, _D, _D, _D, _I, C.c_uint32, _U]
+        L.rbgpu_get_broadphase_objects.argtypes = [_P, _D, _D, _U, C.c_uint32, _U]
         L.rbgpu_set_option.argtypes = [_P, C.c_int, C.c_int]
         L.rbgpu_autotune.argtypes = [_P, C.c_int, C.POINTER(TuneReport)]
         L.rbgpu_get_option.argtypes = [_P, C.c_int, _I]
@@ -256,6 +264,30 @@ class World:
         ts = None if target_speed is None else np.ascontiguousarray(target_speed, np.float64)
         _check(lib().rbgpu_set_joint_targets(self.h, len(ids), _ptr(ids, _U), _ptr(tp, _D), _ptr(ts, _D)))
 
+    def add_constraints(self, desc) -> np.ndarray:
+        """World::create_constraint (ConstraintAPI.hpp:37-64) from CONSTRAINT_DTYPE rows;
+        returns their ids (shared with the joints)."""
+        desc = np.ascontiguousarray(desc, dtype=scenes.CONSTRAINT_DTYPE)
+        ids = np.zeros(len(desc), np.uint32)
+        _check(lib().rbgpu_add_constraints(self.h, len(desc), desc.ctypes.data_as(_P), _ptr(ids, _U)))
+        self.n_joints += len(desc)
+        return ids
+
+    def set_rows(self, ids, direction=None, magnitude=None):
+        """ConstraintCollection::direction / magnitude of constraint rows, as the reference's
+        callers write them."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        d = None if direction is None else np.ascontiguousarray(direction, np.float64).reshape(-1)
+        m = None if magnitude is None else np.ascontiguousarray(magnitude, np.float64)
+        _check(lib().rbgpu_set_constraints(self.h, len(ids), _ptr(ids, _U), _ptr(d, _D), _ptr(m, _D)))
+
+    def constraint_rows(self) -> dict:
+        n = self.n_joints
+        d, m = np.zeros((n, 4, 3)), np.zeros((n, 4))
+        if n:
+            _check(lib().rbgpu_get_constraint_rows(self.h, _ptr(d, _D), _ptr(m, _D)))
+        return {"direction": d, "magnitude": m}
+
     def add_hull(self, verts, faces=None) -> int:
         """rbc::convex_hull_data_create (ConvexHull.hpp:149-420): the world copies the hull;
         returns the id a hull collider names in hull_index."""
@@ -385,6 +417,22 @@ class World:
         a = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in st}
         _check(lib().rbgpu_set_bodies(self.h, _ptr(a.get("position"), _D), _ptr(a.get("orientation"), _D),
                                       _ptr(a.get("linear_velocity"), _D), _ptr(a.get("angular_velocity"), _D)))
+
+    def set_forces(self, force=None, torque=None):
+        """BodyCollection::force / torque (n x 3 each; Body.hpp:52-69): the constant load the
+        integrator applies each substep, in place of gravity (BodyAPI.hpp:61-62)."""
+        f = None if force is None else np.ascontiguousarray(force, np.float64).reshape(-1)
+        t = None if torque is None else np.ascontiguousarray(torque, np.float64).reshape(-1)
+        for a in (f, t):
+            if a is not None and a.size != 3 * self.n_bodies:
+                raise ValueError(f"expected {self.n_bodies} x 3 values, got {a.size}")
+        _check(lib().rbgpu_set_body_forces(self.h, _ptr(f, _D), _ptr(t, _D)))
+
+    def forces(self) -> dict:
+        n = self.n_bodies
+        f, t = np.zeros((n, 3)), np.zeros((n, 3))
+        _check(lib().rbgpu_get_body_forces(self.h, _ptr(f, _D), _ptr(t, _D)))
+        return {"force": f, "torque": t}
 
     def sap_pairs(self) -> np.ndarray:
         n = C.c_uint32()

@@ -228,6 +228,51 @@ def api_scene_fixture(name):
     print(name, "written")
 
 
+def constraint_rows_fixture(name, frames=(1, 10, 60)):
+    """scenes.constraint_rows through the reference: rows written after loading (their
+    direction and magnitude), states and every joint's / row's lambdas per frame."""
+    sc = scenes.constraint_rows()
+    w = orc.RefWorld(sc.timestep, sc.substeps).load(sc)
+    ids = [r[0] for r in sc.meta["rows"]]
+    w.set_rows(ids, [r[1] for r in sc.meta["rows"]], [r[2] for r in sc.meta["rows"]])
+    out = {"bodies": sc.bodies, "colliders": sc.colliders, "joints": sc.joints, "timestep": sc.timestep,
+           "substeps": sc.substeps, "frames": np.array(frames), "row_ids": np.array(ids, np.uint32),
+           "row_dir": np.array([r[1] for r in sc.meta["rows"]]), "row_mag": np.array([r[2] for r in sc.meta["rows"]])}
+    done = 0
+    for f in frames:
+        w.step(f - done)
+        done = f
+        for k, v in w.state().items():
+            out[f"{k}_{f}"] = v
+        out[f"joint_lambda_{f}"] = w.joints()["lambda"]
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+    print(name, "written")
+
+
+def forces_fixture(name, frames=(1, 10, 40), seed=21):
+    """A small box/sphere pile whose bodies carry caller-written forces and torques
+    (BodyCollection::force / torque, written after loading) through the reference:
+    states per frame."""
+    sc = scenes.pile_small(nx=3, ny=3, nz=2)
+    rng = np.random.default_rng(seed)
+    n = len(sc.bodies)
+    mass = sc.bodies["mass"]
+    force = np.stack([rng.uniform(-2, 2, n) * mass, -9.8 * mass * rng.uniform(0.2, 1.5, n), rng.uniform(-2, 2, n) * mass], 1)
+    torque = rng.uniform(-0.5, 0.5, (n, 3))
+    w = orc.RefWorld(sc.timestep, sc.substeps).load(sc)
+    w.set_forces(force, torque)
+    out = {"bodies": sc.bodies, "colliders": sc.colliders, "timestep": sc.timestep, "substeps": sc.substeps,
+           "frames": np.array(frames), "force": force, "torque": torque}
+    done = 0
+    for f in frames:
+        w.step(f - done)
+        done = f
+        for k, v in w.state().items():
+            out[f"{k}_{f}"] = v
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+    print(name, "written")
+
+
 ACCURACY_FRAMES = (1, 10, 30, 100, 300)
 
 
@@ -292,5 +337,7 @@ if __name__ == "__main__":
     world_fixture("world_hull_zoo", scenes.hull_zoo(), [1, 10, 60])
     world_fixture("world_hull_pile_64", scenes.hull_pile(64, seed=3), [1, 30])
     world_fixture("world_resting_layer_100", scenes.resting_layer(n=100, cols=10), [1, 30, 60])
+    constraint_rows_fixture("world_constraint_rows")
     accuracy_fixture("accuracy_resting_layers")
     cpu_ratio_fixture("cpu_ratio_pile_small")
+    forces_fixture("world_forces")

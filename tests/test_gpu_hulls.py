@@ -101,3 +101,22 @@ def test_gpu_hull_worlds_match_fp32_colour_oracle(make, frames, layout):
         assert np.allclose(cg["normal"], co["normal"], rtol=RTOL, atol=RTOL)
     st = g.stats()
     assert st["bins"][9] + st["bins"][10] > 0 and st["bins"][11] > 0   # GJK pairs and hull-plane pairs ran
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_gpu_gjk_cycling_pairs_match_oracle(layout):
+    """Pairs on which FP32 GJK cycles (tests/golden/gjk_cycling_pairs.npz): the GPU ends
+    the loop at the repeated state and reports what the capped run reports -- GJK_FAILED
+    after 128 iterations, no contact -- as the FP32 oracle runs it to the cap."""
+    g = golden("gjk_cycling_pairs")
+    hulls = unpack_hulls(g)
+    a, b, tfa, tfb = g["a"], g["b"], g["tfa"], g["tfb"]
+    print("case gjk_cycling_pairs", len(a), "GJK layout", layout, flush=True)
+    w, pairs = _pair_world(a, tfa, b, tfb, hulls)
+    w.set_option("gjk_rows", layout)
+    out = w.collide_pairs(pairs)
+    for i in range(len(a)):
+        s = orc.oracle_gjk_epa(a[i], tfa[i], b[i], tfb[i], fp32=True, hulls=hulls)
+        assert out["gjk_status"][i] == s["gjk_status"] == orc.GJK_FAILED, i
+        assert out["gjk_iters"][i] == s["gjk_iters"] == 128, i
+        assert out["n"][i] == 0, i

@@ -80,3 +80,30 @@ def test_gpu_joint_targets_and_api_errors():
     w.add_bodies(bad.build("x").bodies)
     with pytest.raises(world.RbgpuError):   # a joint needs two different bodies
         w.add_joints(np.array(bad.j, dtype=scenes.JOINT_DTYPE))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gpu_constraint_rows_match_fp32_colour_oracle(mode):
+    """Constraint rows (World::create_constraint) created between joints and written after
+    loading: GPU = FP32 colour-order oracle (states, lambdas of joints and rows); a row
+    keeps the direction and magnitude written to it; writing a joint's row is refused."""
+    from rbps_amd import world
+    sc = scenes.constraint_rows()
+    print("case constraint_rows mode", mode, flush=True)
+    g, o = _pair(sc, mode)
+    ids = np.array([r[0] for r in sc.meta["rows"]], np.uint32)
+    dirs, mags = np.array([r[1] for r in sc.meta["rows"]]), np.array([r[2] for r in sc.meta["rows"]])
+    g.set_rows(ids, dirs, mags)
+    o.set_rows(ids, dirs, mags)
+    for f in range(40):
+        g.step(1)
+        o.step(1)
+        ok, worst = _nan_safe_close(g.state(), o.state())
+        assert ok, f"mode {mode} frame {f + 1}: state {worst}"
+        ok, worst = _nan_safe_close({"lambda": g.joints()["lambda"]}, {"lambda": o.joints()["lambda"]})
+        assert ok, f"mode {mode} frame {f + 1}: lambdas {worst}"
+    rows = g.constraint_rows()
+    assert np.allclose(rows["direction"][ids, 0], dirs) and np.allclose(rows["magnitude"][ids, 0], mags)
+    assert (g.joints()["lambda"][ids, 0] != 0.0).all()
+    with pytest.raises(world.RbgpuError, match="not a constraint row"):
+        g.set_rows([0], [(0.0, 1.0, 0.0)], [1.0])

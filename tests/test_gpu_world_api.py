@@ -91,3 +91,29 @@ def test_contacts_bounded():                            # test_world.cpp:165-176
     add_dynamic_sphere(w, (0, 0, 0), 0.5)
     w.step(10)
     assert w.stats()["contacts"] <= 16
+
+
+def test_caller_forces_match_fp32_colour_oracle():
+    """Caller-written force / torque (rbgpu_set_body_forces; Body.hpp:52-69) on the pile of
+    tests/golden/world_forces.npz: the GPU equals the FP32 colour-order oracle given the same
+    loads (states within 1e-5 relative, bit-identical in practice), before the first step
+    and again rewritten after 10 frames (the built world's device copy)."""
+    from oracle import orc
+    from tests.helpers import state_close
+    g = golden("world_forces")
+    sc = scenes.Scene("forces", g["bodies"], g["colliders"], float(g["timestep"]), int(g["substeps"]))
+    w = W(sc.timestep, sc.substeps).load(sc)
+    o = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
+    w.set_forces(g["force"], g["torque"])
+    o.set_forces(g["force"], g["torque"])
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    assert np.array_equal(w.forces()["force"], f32(g["force"]))
+    for f in range(30):
+        if f == 10:
+            w.set_forces(-0.5 * g["force"], None)
+            o.set_forces(-0.5 * g["force"], None)
+        w.step(1)
+        o.step(1)
+        ok, worst = state_close(w.state(), o.state(), 1e-5)
+        assert ok, f"frame {f + 1}: {worst}"
+    assert np.array_equal(w.forces()["torque"], f32(g["torque"]))

@@ -82,3 +82,23 @@ def test_live_reference_bit_exact(seed):
         for k in sr:
             assert np.array_equal(sr[k], so[k])
         assert np.array_equal(r.pairs(), o.pairs())
+
+
+def test_caller_forces_bit_exact_vs_reference_golden():
+    """Caller-written BodyCollection::force / torque (Body.hpp:52-69) on a small pile: the
+    FP64 restatement equals the reference (tests/golden/world_forces.npz) bit for bit, and
+    the forces matter (gravity-only states differ)."""
+    g = golden("world_forces")
+    sc = scenes.Scene("forces", g["bodies"], g["colliders"], float(g["timestep"]), int(g["substeps"]))
+    w = orc.OracleWorld(sc.timestep, sc.substeps).load(sc)
+    w.set_forces(g["force"], g["torque"])
+    plain = orc.OracleWorld(sc.timestep, sc.substeps).load(sc)
+    done = 0
+    for f in (int(x) for x in g["frames"]):
+        w.step(f - done)
+        plain.step(f - done)
+        done = f
+        st = w.state()
+        for k in ("position", "orientation", "linear_velocity", "angular_velocity"):
+            assert np.array_equal(st[k], g[f"{k}_{f}"]), (f, k)
+        assert not np.array_equal(st["position"], plain.state()["position"])

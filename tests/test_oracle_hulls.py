@@ -169,3 +169,25 @@ def test_live_reference_hull_pairs():
         assert r1["n"] == r2["n"] and np.array_equal(r1["points"], r2["points"]), t
         if r1["n"]:
             assert np.array_equal(r1["normal"], r2["normal"]) and np.array_equal(r1["depth"], r2["depth"]), t
+
+
+def test_fp32_gjk_cycles_on_touching_hulls():
+    """tests/golden/gjk_cycling_pairs.npz: touching hull pairs of the combined workload
+    (frame 120) on which FP32 GJK never decides.  Each one's loop state (rank, simplex
+    points, direction) repeats exactly after a few iterations, so the capped run ends
+    GJK_FAILED after 128 iterations -- the premise of the GPU's cycle detection, which ends
+    such a loop at the repeat with the same status and count.  GJK with the plain seed,
+    FP32 restatement; the FP64 restatement decides most of the same pairs."""
+    g = golden("gjk_cycling_pairs")
+    hulls = unpack_hulls(g)
+    a, b, tfa, tfb = g["a"], g["b"], g["tfa"], g["tfb"]
+    decided64 = 0
+    for i in range(len(a)):
+        t = orc.oracle_gjk_trace(a[i], tfa[i], b[i], tfb[i], fp32=True, hulls=hulls)
+        assert t["gjk_status"] == orc.GJK_FAILED and t["gjk_iters"] == 128, i
+        assert 0 < t["first_repeat"] < 128 and 1 <= t["period"] <= 16, (i, t)
+        s = orc.oracle_gjk_epa(a[i], tfa[i], b[i], tfb[i], fp32=True, hulls=hulls)
+        assert s["gjk_status"] == orc.GJK_FAILED and s["gjk_iters"] == 128, i
+        decided64 += orc.oracle_gjk_trace(a[i], tfa[i], b[i], tfb[i], fp32=False, hulls=hulls)["gjk_status"] != orc.GJK_FAILED
+    print(f"FP64 decides {decided64} of {len(a)}")
+    assert len(a) >= 40

@@ -245,3 +245,26 @@ def test_joint_colouring_valid():
         bodies = [int(b) for j in sc.joints[col == c] for b in (j["body_1"], j["body_2"]) if dyn[int(b)]]
         assert len(bodies) == len(set(bodies)), c
     assert col.max() + 1 == 2   # a chain needs two colours
+
+
+def test_constraint_rows_fp64_equal_reference_golden():
+    """Constraint rows made by World::create_constraint (ConstraintAPI.hpp:37-64), created
+    between the joints and written by the caller after loading (direction, magnitude):
+    the FP64 restatement in the reference's order equals the reference engine
+    (tests/golden/world_constraint_rows.npz) bit for bit -- states and every joint's and
+    row's lambda.  A row keeps the direction and magnitude its caller wrote; the zero row
+    stays at lambda 0."""
+    g = golden("world_constraint_rows")
+    sc = S.Scene("rows", g["bodies"], g["colliders"], float(g["timestep"]), int(g["substeps"]), joints=g["joints"])
+    w = orc.OracleWorld(sc.timestep, sc.substeps).load(sc)
+    w.set_rows(g["row_ids"], g["row_dir"], g["row_mag"])
+    done = 0
+    for f in (int(x) for x in g["frames"]):
+        w.step(f - done)
+        done = f
+        st = w.state()
+        for k in st:
+            assert np.array_equal(st[k], g[f"{k}_{f}"]), (f, k)
+        assert np.array_equal(w.joints()["lambda"], g[f"joint_lambda_{f}"]), f
+    lam = g["joint_lambda_60"]
+    assert lam[1, 0] != 0.0 and lam[3, 0] != 0.0 and lam[4, 0] == 0.0
