@@ -421,6 +421,32 @@ int rbgpu_mlp_gradients(rbgpu_world *w, uint32_t n, const float *features32, con
                         double *loss);
 int rbgpu_get_gjk_totals(rbgpu_world *w, uint64_t *out6, int reset);
 
+/* Device-resident snapshot and restore of a world's simulation state (the reference's
+ * WorldSnapshot design, docs/design/world-snapshot-restore/world-snapshot-restore.md:29-45,
+ * held in device memory).  A snapshot holds what carries from one frame to the next: body
+ * poses, velocities and previous poses, forces and torques, the broadphase's fat AABBs and
+ * endpoint order, joint targets, current positions and lambdas, constraint rows, the GJK
+ * rejection hints, the frame count, timestep / substeps and every option.  Restoring it and
+ * stepping repeats the original run bit for bit.  Contacts, pair lists and colours are
+ * rebuilt by the next frame (their getters describe the last frame stepped until then);
+ * the learned warm start's weights are parameters, not state (rbgpu_mlp_get_weights).
+ *   create   allocates the snapshot's device block and captures the world (builds it first
+ *            if needed); capture re-captures into an existing snapshot, reusing its block
+ *            when it is large enough.
+ *   restore  needs a world with the same bodies, colliders, joints, scenes and hulls (the
+ *            world it came from, or one loaded from the same scene), waits for it, and
+ *            copies device to device.
+ *   size / save / load  the serialised form for a checkpoint file: header + the section
+ *            block, with a checksum that load verifies. */
+typedef struct rbgpu_snapshot rbgpu_snapshot;
+int rbgpu_snapshot_create(rbgpu_world *w, rbgpu_snapshot **out);
+int rbgpu_snapshot_capture(rbgpu_world *w, rbgpu_snapshot *s);
+int rbgpu_snapshot_restore(rbgpu_world *w, const rbgpu_snapshot *s);
+int rbgpu_snapshot_size(const rbgpu_snapshot *s, uint64_t *bytes);
+int rbgpu_snapshot_save(const rbgpu_snapshot *s, void *host, uint64_t cap);
+int rbgpu_snapshot_load(const void *host, uint64_t n, rbgpu_snapshot **out);
+int rbgpu_snapshot_destroy(rbgpu_snapshot *s);
+
 const char *rbgpu_last_error(void);
 
 #ifdef __cplusplus

@@ -1399,21 +1399,28 @@ __global__ void k_axis_save(const uint32_t *slots, uint32_t n, uint32_t cap, Con
         }
     }
 }
+// Every slot of the new list gets its pair's saved hint, or none (w = 0): a slot's hint then
+// depends only on the table, never on which pair held the slot before.  keys == nullptr: no
+// table (nothing carried), every slot starts without a hint.
 __global__ void k_axis_load(const uint32_t *slots, uint32_t n, Contacts K, const unsigned long long *keys,
                             const float4 *vals, uint32_t mask, float4 *axis) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = slots[i];
-    const unsigned long long key = axis_key(K, s);
-    uint32_t h = axis_hash(key, mask);
-    for (uint32_t p = 0; p <= mask; ++p, h = (h + 1u) & mask) {
-        const unsigned long long k = keys[h];
-        if (k == key) {
-            axis[s] = vals[h];
-            return;
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (keys) {
+        const unsigned long long key = axis_key(K, s);
+        uint32_t h = axis_hash(key, mask);
+        for (uint32_t p = 0; p <= mask; ++p, h = (h + 1u) & mask) {
+            const unsigned long long k = keys[h];
+            if (k == key) {
+                a = vals[h];
+                break;
+            }
+            if (k == AXIS_EMPTY) break;
         }
-        if (k == AXIS_EMPTY) return;
     }
+    axis[s] = a;
 }
 
 // Exact rejection in front of GJK.  Two tests on the line between the collider origins:

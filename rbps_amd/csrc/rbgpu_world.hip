@@ -157,6 +157,7 @@ struct rbgpu_world {
     float4 *ax_vals = nullptr;
     uint32_t ax_cap = 0;
     bool axis_carry = true;
+    int64_t ax_restore = -1;   // >= 0: the next frame's hint table came from a snapshot (its size; 0 empty)
     int contact_mode = 0;                               // 1: capsule-capsule normal from A to B
     bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
@@ -1021,11 +1022,17 @@ int rbgpu_world::frame() {
     //    narrowphase bins.  One host synchronisation per frame: the pair count (capacity
     //    check) and the bin offsets.  The GJK slots' rejection hints are saved by collider
     //    pair first and restored into the new list's slots after it.
-    const uint32_t ng_old =
-        n_pairs && !hh.empty() && bin_off.size() == NBINS + 1 ? bin_off[BIN_HP] - bin_off[BIN_HH] : 0u;
-    const bool carry = axis_carry && gjk_cull && gjk_axis && ng_old;
+    //    After a snapshot restore the table comes from the snapshot instead.
+    const bool restored = ax_restore >= 0;
+    const uint32_t ng_old = !restored && n_pairs && !hh.empty() && bin_off.size() == NBINS + 1
+                                ? bin_off[BIN_HP] - bin_off[BIN_HH] : 0u;
+    bool carry = axis_carry && gjk_cull && gjk_axis && ng_old;
     uint32_t amask = 0;
-    if (carry) {
+    if (restored) {
+        carry = axis_carry && gjk_cull && ax_restore > 0;
+        amask = ax_restore > 0 ? (uint32_t)ax_restore - 1 : 0u;
+        ax_restore = -1;
+    } else if (carry) {
         uint32_t H = 1;
         while (H < 2 * ng_old) H <<= 1;
         if (H > ax_cap) {
@@ -1079,10 +1086,11 @@ int rbgpu_world::frame() {
     n_sap = rb->n_sap;
     n_pairs = rb->n_pairs;
     bin_off.assign(rb->bin_off, rb->bin_off + NBINS + 1);
-    if (carry && bin_off[BIN_HP] > bin_off[BIN_HH])
+    if (gjk_axis && bin_off[BIN_HP] > bin_off[BIN_HH])
         k_axis_load<<<nblk(bin_off[BIN_HP] - bin_off[BIN_HH]), TPB, 0, st>>>(bin_list + bin_off[BIN_HH],
-                                                                               bin_off[BIN_HP] - bin_off[BIN_HH], K, ax_keys,
-                                                                               ax_vals, amask, gjk_axis);
+                                                                               bin_off[BIN_HP] - bin_off[BIN_HH], K,
+                                                                               carry ? ax_keys : nullptr, ax_vals, amask,
+                                                                               gjk_axis);
     const uint32_t P = n_pairs;
     const uint32_t colour_rounds_cap = 1u << 16;
     const bool by_scene = scene_path();
@@ -2417,6 +2425,245 @@ int rbgpu_get_gjk_totals(rbgpu_world *w, uint64_t *out6, int reset) {
     if (reset) HIPCHK(hipMemsetAsync(w->gjk_tot, 0, 48, w->st));
     HIPCHK(hipStreamSynchronize(w->st));
     if (out6) for (int k = 0; k < 6; ++k) out6[k] = h[k];
+    return RBGPU_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ snapshot / restore
+// The state that carries from one frame to the next, copied device to device on the
+// world's stream: body rows (pose, velocity, previous pose), force and torque, fat AABBs,
+// the endpoint order the next frame's stable sort starts from, endpoint positions, joint
+// targets / current positions, joint lambdas (never reset in joint mode 0), constraint
+// rows, and the GJK rejection hints keyed by collider pair.  Everything else a frame
+// rebuilds from these.  The header holds the counts (the layout check), the frame count,
+// the step and every option.
+struct rbgpu_snapshot {
+    struct Header {
+        char magic[8];
+        uint32_t version;
+        uint32_t N, M, S, NEP, NJ, nscenes, nhulls;
+        uint32_t frames;
+        uint32_t ax_H;            // hint table entries (a power of two; 0: none)
+        int32_t substeps;
+        int32_t n_options;
+        int32_t options[32];
+        double timestep;
+        uint64_t payload;         // bytes of the section block
+        uint64_t checksum;        // FNV-1a 64 of the section block (serialised form)
+    } hd{};
+    unsigned char *dev = nullptr;  // the section block (device)
+    size_t cap = 0;
+    ~rbgpu_snapshot() { dfree(dev); }
+};
+
+namespace {
+
+constexpr char SNAP_MAGIC[8] = {'R', 'B', 'G', 'P', 'U', 'S', 'N', 'P'};
+constexpr uint32_t SNAP_VERSION = 1;
+constexpr int SNAP_OPTIONS = RBGPU_OPT_AXIS_CARRY + 1;
+constexpr size_t SNAP_ALIGN = 256;
+
+struct SnapSec {
+    void *world;   // the world's buffer (nullptr: the hint table, handled apart)
+    size_t bytes;
+};
+// Sections in block order; the last two are the hint table's keys and values.
+std::vector<SnapSec> snap_sections(rbgpu_world *w, const rbgpu_snapshot::Header &h) {
+    return {{w->B.r, (size_t)h.N * sizeof(BodyRow)}, {w->B.f, (size_t)h.N * 16}, {w->B.t, (size_t)h.N * 16},
+            {w->fmin, (size_t)h.S * 16}, {w->fmax, (size_t)h.S * 16}, {w->ep_key[0], (size_t)h.NEP * 4},
+            {w->ep_val[0], (size_t)h.NEP * 4}, {w->ep_pos, (size_t)h.S * 8}, {w->Jt.jtg, (size_t)h.NJ * 16},
+            {w->Jt.jlam, (size_t)h.NJ * 16}, {w->Jt.jrow, (size_t)h.NJ * 64}, {nullptr, (size_t)h.ax_H * 8},
+            {nullptr, (size_t)h.ax_H * 16}};
+}
+std::vector<size_t> snap_offsets(const std::vector<SnapSec> &sec) {
+    std::vector<size_t> off(sec.size() + 1, 0);
+    for (size_t k = 0; k < sec.size(); ++k) off[k + 1] = off[k] + (sec[k].bytes + SNAP_ALIGN - 1) / SNAP_ALIGN * SNAP_ALIGN;
+    return off;
+}
+uint64_t fnv1a(const unsigned char *p, size_t n) {
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
+    return h;
+}
+int snap_reserve(rbgpu_snapshot *s, size_t bytes) {
+    if (bytes <= s->cap && s->dev) return RBGPU_OK;
+    dfree(s->dev);
+    s->cap = 0;
+    if (dalloc(s->dev, bytes)) return RBGPU_ERR_OOM;
+    s->cap = bytes;
+    return RBGPU_OK;
+}
+
+int snap_capture(rbgpu_world *w, rbgpu_snapshot *s) {
+    if (!w->built) {
+        int rc = w->build();
+        if (rc) return rc;
+    }
+    if (int rc = w->sync()) return rc;
+    rbgpu_snapshot::Header h{};
+    std::memcpy(h.magic, SNAP_MAGIC, 8);
+    h.version = SNAP_VERSION;
+    h.N = w->N; h.M = w->M; h.S = w->S; h.NEP = w->NEP; h.NJ = w->NJ;
+    h.nscenes = w->nscenes; h.nhulls = (uint32_t)w->hh.size(); h.frames = w->frames;
+    h.timestep = w->timestep; h.substeps = w->substeps;
+    h.n_options = SNAP_OPTIONS;
+    for (int k = 0; k < SNAP_OPTIONS; ++k)
+        if (int rc = rbgpu_get_option(w, k, &h.options[k])) return rc;
+    // the hint table the next frame would carry: the one restored and not yet used, or the
+    // one saved from the last frame's GJK slots
+    const uint32_t ng = w->n_pairs && !w->hh.empty() && w->bin_off.size() == NBINS + 1
+                            ? w->bin_off[BIN_HP] - w->bin_off[BIN_HH] : 0u;
+    const bool from_restore = w->ax_restore >= 0;
+    if (from_restore) {
+        h.ax_H = (uint32_t)w->ax_restore;
+    } else if (w->axis_carry && w->gjk_cull && w->gjk_axis && ng) {
+        h.ax_H = 1;
+        while (h.ax_H < 2 * ng) h.ax_H <<= 1;
+    }
+    const std::vector<SnapSec> sec = snap_sections(w, h);
+    const std::vector<size_t> off = snap_offsets(sec);
+    h.payload = off.back();
+    if (int rc = snap_reserve(s, h.payload)) return rc;
+    for (size_t k = 0; k + 2 < sec.size(); ++k)
+        if (sec[k].bytes)
+            HIPCHK(hipMemcpyAsync(s->dev + off[k], sec[k].world, sec[k].bytes, hipMemcpyDeviceToDevice, w->st));
+    if (h.ax_H) {
+        auto *keys = (unsigned long long *)(s->dev + off[sec.size() - 2]);
+        auto *vals = (float4 *)(s->dev + off[sec.size() - 1]);
+        if (from_restore) {
+            HIPCHK(hipMemcpyAsync(keys, w->ax_keys, (size_t)h.ax_H * 8, hipMemcpyDeviceToDevice, w->st));
+            HIPCHK(hipMemcpyAsync(vals, w->ax_vals, (size_t)h.ax_H * 16, hipMemcpyDeviceToDevice, w->st));
+        } else {
+            k_axis_clear<<<nblk(h.ax_H), TPB, 0, w->st>>>(keys, h.ax_H);
+            k_axis_save<<<nblk(ng), TPB, 0, w->st>>>(w->bin_list + w->bin_off[BIN_HH], ng, w->pcap, w->K, w->gjk_axis,
+                                                      keys, vals, h.ax_H - 1);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    HIPCHK(hipStreamSynchronize(w->st));
+    s->hd = h;
+    return RBGPU_OK;
+}
+
+int snap_check(const rbgpu_snapshot::Header &h) {
+    if (std::memcmp(h.magic, SNAP_MAGIC, 8) != 0) return fail(RBGPU_ERR_INVALID, "not a world snapshot");
+    if (h.version != SNAP_VERSION) return fail(RBGPU_ERR_INVALID, "snapshot version mismatch");
+    if (h.n_options != SNAP_OPTIONS) return fail(RBGPU_ERR_INVALID, "snapshot option count mismatch");
+    if (h.ax_H & (h.ax_H - 1)) return fail(RBGPU_ERR_INVALID, "corrupt snapshot (hint table size)");
+    return RBGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbgpu_snapshot_create(rbgpu_world *w, rbgpu_snapshot **out) {
+    if (!w || !out) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    *out = nullptr;
+    auto *s = new rbgpu_snapshot();
+    int rc = snap_capture(w, s);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return RBGPU_OK;
+}
+
+int rbgpu_snapshot_capture(rbgpu_world *w, rbgpu_snapshot *s) {
+    if (!w || !s) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    return snap_capture(w, s);
+}
+
+int rbgpu_snapshot_restore(rbgpu_world *w, const rbgpu_snapshot *s) {
+    if (!w || !s || !s->dev) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    const rbgpu_snapshot::Header &h = s->hd;
+    if (int rc = snap_check(h)) return rc;
+    if (!w->built) {
+        int rc = w->build();
+        if (rc) return rc;
+    }
+    if (h.N != w->N || h.M != w->M || h.S != w->S || h.NEP != w->NEP || h.NJ != w->NJ || h.nscenes != w->nscenes ||
+        h.nhulls != (uint32_t)w->hh.size())
+        return fail(RBGPU_ERR_INVALID, "snapshot of a world with other bodies, colliders, joints, scenes or hulls");
+    if (int rc = w->sync()) return rc;
+    for (int k = 0; k < SNAP_OPTIONS; ++k)
+        if (int rc = rbgpu_set_option(w, k, h.options[k])) return rc;
+    if (int rc = rbgpu_set_step(w, h.timestep, h.substeps)) return rc;
+    const std::vector<SnapSec> sec = snap_sections(w, h);
+    const std::vector<size_t> off = snap_offsets(sec);
+    if (off.back() != h.payload || h.payload > s->cap) return fail(RBGPU_ERR_INVALID, "corrupt snapshot (layout)");
+    for (size_t k = 0; k + 2 < sec.size(); ++k)
+        if (sec[k].bytes)
+            HIPCHK(hipMemcpyAsync(sec[k].world, s->dev + off[k], sec[k].bytes, hipMemcpyDeviceToDevice, w->st));
+    if (h.ax_H > w->ax_cap) {
+        HIPCHK(hipStreamSynchronize(w->st));
+        dfree(w->ax_keys); dfree(w->ax_vals);
+        w->ax_cap = 0;
+        if (dalloc(w->ax_keys, h.ax_H) | dalloc(w->ax_vals, h.ax_H)) return RBGPU_ERR_OOM;
+        w->ax_cap = h.ax_H;
+    }
+    if (h.ax_H) {
+        HIPCHK(hipMemcpyAsync(w->ax_keys, s->dev + off[sec.size() - 2], (size_t)h.ax_H * 8, hipMemcpyDeviceToDevice, w->st));
+        HIPCHK(hipMemcpyAsync(w->ax_vals, s->dev + off[sec.size() - 1], (size_t)h.ax_H * 16, hipMemcpyDeviceToDevice,
+                              w->st));
+    }
+    w->ax_restore = h.ax_H;
+    w->frames = h.frames;
+    // the sweep's per-endpoint arrays derive from the restored boxes and order only when the
+    // frame marks them dirty: mark it (the stable sort of the restored, sorted order keeps it)
+    k_set_int<<<1, 1, 0, w->st>>>(w->flags, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(w->st));
+    return RBGPU_OK;
+}
+
+int rbgpu_snapshot_size(const rbgpu_snapshot *s, uint64_t *bytes) {
+    if (!s || !bytes) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    *bytes = sizeof(rbgpu_snapshot::Header) + s->hd.payload;
+    return RBGPU_OK;
+}
+
+int rbgpu_snapshot_save(const rbgpu_snapshot *s, void *host, uint64_t cap) {
+    if (!s || !host || !s->dev) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    const uint64_t need = sizeof(rbgpu_snapshot::Header) + s->hd.payload;
+    if (cap < need) return fail(RBGPU_ERR_CAPACITY, "host buffer smaller than rbgpu_snapshot_size");
+    auto *p = (unsigned char *)host;
+    HIPCHK(hipMemcpy(p + sizeof(rbgpu_snapshot::Header), s->dev, s->hd.payload, hipMemcpyDeviceToHost));
+    rbgpu_snapshot::Header h = s->hd;
+    h.checksum = fnv1a(p + sizeof(h), h.payload);
+    std::memcpy(p, &h, sizeof(h));
+    return RBGPU_OK;
+}
+
+int rbgpu_snapshot_load(const void *host, uint64_t n, rbgpu_snapshot **out) {
+    if (!host || !out) return fail(RBGPU_ERR_INVALID, "invalid arguments");
+    *out = nullptr;
+    rbgpu_snapshot::Header h;
+    if (n < sizeof(h)) return fail(RBGPU_ERR_INVALID, "snapshot bytes shorter than the header");
+    std::memcpy(&h, host, sizeof(h));
+    if (int rc = snap_check(h)) return rc;
+    if (n != sizeof(h) + h.payload) return fail(RBGPU_ERR_INVALID, "snapshot bytes of the wrong length");
+    const auto *p = (const unsigned char *)host + sizeof(h);
+    if (fnv1a(p, h.payload) != h.checksum) return fail(RBGPU_ERR_INVALID, "snapshot checksum mismatch");
+    auto *s = new rbgpu_snapshot();
+    if (int rc = snap_reserve(s, h.payload)) {
+        delete s;
+        return rc;
+    }
+    hipError_t e = hipMemcpy(s->dev, p, h.payload, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(RBGPU_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    s->hd = h;
+    *out = s;
+    return RBGPU_OK;
+}
+
+int rbgpu_snapshot_destroy(rbgpu_snapshot *s) {
+    delete s;
     return RBGPU_OK;
 }
 

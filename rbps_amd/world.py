@@ -36,6 +36,8 @@ EXPORTS = [
     "rbgpu_add_hull", "rbgpu_hull_mass_properties", "rbgpu_hull_tables", "rbgpu_collide_world_pairs",
     "rbgpu_mlp_init", "rbgpu_mlp_set_weights", "rbgpu_mlp_get_weights", "rbgpu_mlp_set_mode", "rbgpu_mlp_log",
     "rbgpu_mlp_logged", "rbgpu_mlp_get_log", "rbgpu_mlp_train", "rbgpu_mlp_eval", "rbgpu_mlp_gradients", "rbgpu_get_gjk_totals",
+    "rbgpu_snapshot_create", "rbgpu_snapshot_capture", "rbgpu_snapshot_restore", "rbgpu_snapshot_size",
+    "rbgpu_snapshot_save", "rbgpu_snapshot_load", "rbgpu_snapshot_destroy",
     "rbgpu_last_error",
 ]
 
@@ -112,6 +114,13 @@ def lib():
         L.rbgpu_mlp_eval.argtypes = [_P, C.c_uint32, _F, _F]
         L.rbgpu_mlp_gradients.argtypes = [_P, C.c_uint32, _F, _F, _F, _D]
         L.rbgpu_get_gjk_totals.argtypes = [_P, C.POINTER(C.c_uint64), C.c_int]
+        L.rbgpu_snapshot_create.argtypes = [_P, C.POINTER(_P)]
+        L.rbgpu_snapshot_capture.argtypes = [_P, _P]
+        L.rbgpu_snapshot_restore.argtypes = [_P, _P]
+        L.rbgpu_snapshot_size.argtypes = [_P, C.POINTER(C.c_uint64)]
+        L.rbgpu_snapshot_save.argtypes = [_P, _P, C.c_uint64]
+        L.rbgpu_snapshot_load.argtypes = [_P, C.c_uint64, C.POINTER(_P)]
+        L.rbgpu_snapshot_destroy.argtypes = [_P]
         L.rbgpu_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -185,6 +194,44 @@ OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, 
 
 TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "broadphase": 6,
           "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9, "joints": 10, "gjk": 11}
+
+
+class Snapshot:
+    """A device-resident snapshot of a world's state (World.snapshot(); the reference's
+    WorldSnapshot design, docs/design/world-snapshot-restore/world-snapshot-restore.md:29-66).
+    ``bytes(snap)`` / ``Snapshot.load(data)`` are the checkpoint form."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def load(cls, data: bytes) -> "Snapshot":
+        buf = C.create_string_buffer(bytes(data), len(data))
+        h = _P()
+        _check(lib().rbgpu_snapshot_load(C.cast(buf, _P), len(data), C.byref(h)))
+        return cls(h)
+
+    def __len__(self):
+        n = C.c_uint64()
+        _check(lib().rbgpu_snapshot_size(self.h, C.byref(n)))
+        return n.value
+
+    def __bytes__(self):
+        n = len(self)
+        buf = C.create_string_buffer(n)
+        _check(lib().rbgpu_snapshot_save(self.h, C.cast(buf, _P), n))
+        return buf.raw
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rbgpu_snapshot_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class World:
@@ -433,6 +480,21 @@ class World:
         f, t = np.zeros((n, 3)), np.zeros((n, 3))
         _check(lib().rbgpu_get_body_forces(self.h, _ptr(f, _D), _ptr(t, _D)))
         return {"force": f, "torque": t}
+
+    def snapshot(self, into: "Snapshot" = None) -> "Snapshot":
+        """Captures the state that carries between frames into device memory (a new
+        snapshot, or `into`, reusing its block)."""
+        if into is not None:
+            _check(lib().rbgpu_snapshot_capture(self.h, into.h))
+            return into
+        h = _P()
+        _check(lib().rbgpu_snapshot_create(self.h, C.byref(h)))
+        return Snapshot(h)
+
+    def restore(self, snap: "Snapshot"):
+        """Puts the world back into the snapshot's state (same bodies, colliders, joints,
+        scenes and hulls); the following steps repeat the original run bit for bit."""
+        _check(lib().rbgpu_snapshot_restore(self.h, snap.h))
 
     def sap_pairs(self) -> np.ndarray:
         n = C.c_uint32()
