@@ -556,3 +556,62 @@ class World:
         ms, n, b = C.c_double(), C.c_uint64(), C.c_double()
         _check(lib().rbgpu_get_timing(self.h, C.byref(ms), C.byref(n), C.byref(b)))
         return {"ms": ms.value, "launches": n.value, "bytes": b.value}
+
+
+class WorldPool:
+    """The reference's multi-world design (docs/design/multi-world/multi-world.md:40-50) in
+    Python: independent worlds, one per scene, stepped together as the scenes of one batched
+    GPU world (every kernel runs once over all worlds; each world evolves exactly as it would
+    alone).  ``worlds`` are the member scenes; states come back per world, or stacked
+    ``(n_worlds, ...)`` when every world has the same body count."""
+
+    def __init__(self, worlds, timestep: float = None, substeps: int = None):
+        self.worlds = list(worlds)
+        if not self.worlds:
+            raise ValueError("a WorldPool needs at least one world")
+        sc = scenes.batch(self.worlds, "pool")
+        self.timestep = sc.timestep if timestep is None else timestep
+        self.substeps = sc.substeps if substeps is None else substeps
+        self.batch = World(self.timestep, self.substeps).load(sc)
+        self._sizes = [len(w.bodies) for w in self.worlds]
+        self._off = np.concatenate([[0], np.cumsum(self._sizes)]).astype(np.int64)
+
+    def __len__(self):
+        return len(self.worlds)
+
+    def parallel_step(self, dt: float = None, frames: int = 1):
+        """`frames` frames of dt (default: the pool's timestep) for every world."""
+        if dt is not None and dt != self.timestep:
+            self.batch.set_step(dt, self.substeps)
+            self.timestep = dt
+        self.batch.step(frames)
+
+    def parallel_each(self, fn):
+        """fn(index, state) for every world's state (host side), in world order; returns
+        the list of results."""
+        return [fn(i, st) for i, st in enumerate(self.states())]
+
+    def states(self) -> list:
+        st = self.batch.state()
+        return [{k: v[self._off[i]:self._off[i + 1]] for k, v in st.items()} for i in range(len(self.worlds))]
+
+    def stacked(self) -> dict:
+        """Every world's state as (n_worlds, bodies, ...) arrays (equal body counts only)."""
+        if len(set(self._sizes)) != 1:
+            raise ValueError("stacked states need worlds of equal body count")
+        st = self.batch.state()
+        return {k: v.reshape(len(self.worlds), self._sizes[0], *v.shape[1:]) for k, v in st.items()}
+
+    def set_states(self, states):
+        """Writes every world's state (a list of per-world dicts, as states() returns)."""
+        st = {k: np.concatenate([s[k] for s in states]) for k in states[0]}
+        self.batch.set_state(st)
+
+    def snapshot(self, into: "Snapshot" = None) -> "Snapshot":
+        return self.batch.snapshot(into)
+
+    def restore(self, snap: "Snapshot"):
+        self.batch.restore(snap)
+
+    def set_option(self, name: str, value: int):
+        self.batch.set_option(name, value)

@@ -223,3 +223,36 @@ def test_gpu_autotune_keeps_the_trajectory():
     sa, sb = a.state(), b.state()
     for k in sa:
         assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_gpu_world_pool_equals_worlds_alone():
+    """rbps_amd.world.WorldPool (multi-world.md:40-50): 8 piles of equal size stepped
+    together equal the same piles stepped as worlds of their own, bit for bit; stacked
+    states come back as (worlds, bodies, ...) arrays; a state written per world reaches
+    only that world."""
+    from rbps_amd import world
+    parts = [S.pile_small(nx=3, ny=3, nz=2, seed=100 + k) for k in range(8)]
+    pool = world.WorldPool(parts)
+    alone = [_gpu(p) for p in parts]
+    pool.parallel_step(frames=20)
+    for w in alone:
+        w.step(20)
+    for st, w in zip(pool.states(), alone):
+        ws = w.state()
+        for k in ws:
+            assert np.array_equal(st[k], ws[k]), k
+    stk = pool.stacked()
+    assert stk["position"].shape == (8, len(parts[0].bodies), 3)
+    sts = pool.states()
+    sts[5]["linear_velocity"] = sts[5]["linear_velocity"] + np.array([0.0, 3.0, 0.0])
+    pool.set_states(sts)
+    ws = alone[5].state()
+    ws["linear_velocity"] = ws["linear_velocity"] + np.array([0.0, 3.0, 0.0])
+    alone[5].set_state(ws)
+    pool.parallel_step(frames=10)
+    for w in alone:
+        w.step(10)
+    for st, w in zip(pool.states(), alone):
+        ws = w.state()
+        for k in ws:
+            assert np.array_equal(st[k], ws[k]), k
