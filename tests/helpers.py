@@ -173,3 +173,15 @@ def sat_depth(va, fa, vb, fb):
     d = (va @ n.T).max(0) - (vb @ n.T).min(0)
     k = int(np.argmin(d))
     return float(d[k]), n[k]
+
+
+def colouring_valid_np(pairs_bodies, colour, dynamic):
+    """colouring_valid for large pair lists: no (dynamic body, colour) key twice."""
+    pb = np.asarray(pairs_bodies, np.int64).reshape(-1, 2)
+    col = np.asarray(colour, np.int64)
+    keys = []
+    for side in (0, 1):
+        sel = (col >= 0) & dynamic[pb[:, side]]
+        keys.append(pb[sel, side] * (1 << 20) + col[sel])
+    k = np.concatenate(keys)
+    return len(np.unique(k)) == len(k)
