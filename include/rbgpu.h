@@ -279,8 +279,19 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          world's stream.  Results do not change.
  *   RBGPU_OPT_AXIS_CARRY   carry the GJK rejection hints (the axis that last separated a pair)
  *                          across the frame's new pair list, keyed by collider pair (1,
- *                          default; 0: a new list's slot keeps whatever hint it held).  Results
- *                          do not change.
+ *                          default; 0: every pair starts a frame without a hint).  Results do
+ *                          not change.
+ *   RBGPU_OPT_EPA_GUARD    0 (default): EPA depths and normals as the reference computes them.
+ *                          Any direction n bounds the penetration depth from above by the
+ *                          shapes' overlap along n (the Minkowski difference's support along
+ *                          n); when EPA's face is deeper than the smallest overlap over a set of
+ *                          directions by more than 1e-4 m, the FP32 EPA has settled on a face
+ *                          that is not the minimum, and the contact takes that direction and
+ *                          overlap instead.  1: the set is the pair's contact normal of the
+ *                          previous substep (same frame); 2: that and, while the depth in
+ *                          question exceeds 2 mm, the face normals of both shapes (hull
+ *                          polygons, box faces).  Changes results (of pairs where FP32 EPA goes
+ *                          wrong, typically touching hulls).
  *   RBGPU_OPT_GRAPH        with the per-scene kernels: capture one substep per frame in a
  *                          hipGraph and launch it for every substep (1, default; frames with
  *                          kernel timing or an open training log run without it)
@@ -311,7 +322,8 @@ enum {
     RBGPU_OPT_EPA_OCCUPANCY = 17,
     RBGPU_OPT_SOLVE_OCCUPANCY = 18,
     RBGPU_OPT_SIDE_STREAM = 19,
-    RBGPU_OPT_AXIS_CARRY = 20
+    RBGPU_OPT_AXIS_CARRY = 20,
+    RBGPU_OPT_EPA_GUARD = 21
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);

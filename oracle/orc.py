@@ -136,6 +136,10 @@ def oracle_lib():
         lib.orc_set_palette_max.argtypes = [_P, C.c_int]
         lib.orc_set_joint_mode.argtypes = [_P, C.c_int]
         lib.orc_set_contact_mode.argtypes = [_P, C.c_int]
+        lib.orc_set_epa_guard.argtypes = [_P, C.c_int]
+        lib.orc_guard_stats.argtypes = [_P, _D]
+        lib.orc_gjk_epa_guarded.restype = C.c_int
+        lib.orc_gjk_epa_guarded.argtypes = [C.c_int, _P, _D, _P, _D, _D, _D]
         lib.orc_set_planted_fault.argtypes = [_P, C.c_int]
         lib.orc_get_solve_order.argtypes = [_P, _U]
         lib.orc_gjk_trace.argtypes = [C.c_int, _P, _D, _P, _D, _I]
@@ -284,6 +288,17 @@ class OracleWorld(_WorldBase):
         """0: contacts as the reference makes them; 1: capsule-capsule normal from A to B."""
         self.lib.orc_set_contact_mode(self.h, int(mode))
 
+    def set_epa_guard(self, on):
+        """1: bound EPA depths by the overlap along the pair's previous-substep normal
+        (colour order; see orc_gjk.hpp gjk_epa_manifold)."""
+        self.lib.orc_set_epa_guard(self.h, int(on))
+
+    def guard_stats(self):
+        """(EPA results the guard replaced, largest depth it cut in m)."""
+        out = np.zeros(2)
+        self.lib.orc_guard_stats(self.h, _ptr(out, _D))
+        return int(out[0]), float(out[1])
+
     def plant_fault(self, kind):
         """Plants a colour-schedule mistake (1: the last colour's pairs skipped, 2: the
         first pair solved twice) so tests can show their pins catch it."""
@@ -361,10 +376,25 @@ def _gjk_epa_out(out, hit):
             "gjk_dir": out[11:14].copy(), "fallback": int(out[14])}
 
 
-def oracle_gjk_epa(a, tfa, b, tfb, fp32=False, hulls=None, guess=None):
+def oracle_gjk_epa(a, tfa, b, tfb, fp32=False, hulls=None, guess=None, guard=None, guard_faces=False):
     """GJK + EPA of one pair with statuses, iteration counts and the GJK exit direction.
-    guess (optional, A's frame): GJK direction (3) and EPA seed normal (3, zero = none)."""
+    guess (optional, A's frame): GJK direction (3) and EPA seed normal (3, zero = none).
+    guard (optional, world unit vector) / guard_faces: the EPA guard's directions (no warm
+    start then)."""
     out = np.zeros(15)
+    if guard is not None or guard_faces:
+        guard = np.concatenate([np.zeros(3) if guard is None else np.asarray(guard, np.float64),
+                                [1.0 if guard_faces else 0.0]])
+        a = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(a), hulls))
+        b = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(b), hulls))
+        gd = np.ascontiguousarray(guard, np.float64)
+        hit = oracle_lib().orc_gjk_epa_guarded(1 if fp32 else 0, a.ctypes.data_as(_P),
+                                               _ptr(np.ascontiguousarray(tfa, np.float64), _D), b.ctypes.data_as(_P),
+                                               _ptr(np.ascontiguousarray(tfb, np.float64), _D), _ptr(gd, _D),
+                                               _ptr(out, _D))
+        if hit < 0:
+            raise RuntimeError("gjk_epa: invalid shape")
+        return _gjk_epa_out(out, hit)
     if guess is not None and len(guess) == 3:
         guess = np.concatenate([guess, np.zeros(3)])
     a = np.ascontiguousarray(map_hulls("orc_", np.atleast_1d(a), hulls))

@@ -92,6 +92,8 @@ struct Any {
     virtual int solve_order(uint32_t *out) const = 0;
     virtual void set_joint_mode(int m) = 0;
     virtual void set_contact_mode(int m) = 0;
+    virtual void set_epa_guard(int g) = 0;
+    virtual void guard_stats(double *out2) const = 0;
     virtual void set_state(const double *p, const double *q, const double *v, const double *w) = 0;
     virtual void add_joint(const orc_joint_desc &d) = 0;
     virtual int njoints() const = 0;
@@ -194,6 +196,8 @@ template <class T> struct Impl : Any {
     }
     void set_joint_mode(int m) override { w.joint_mode = m; }
     void set_contact_mode(int m) override { w.contact_mode = m; }
+    void set_epa_guard(int g) override { w.epa_guard = g; }
+    void guard_stats(double *out2) const override { out2[0] = double(w.guard_hits); out2[1] = w.guard_max_cut; }
     void add_joint(const orc_joint_desc &d) override { w.add_joint(joint_from<T>(d), T(d.compliance)); }
     int njoints() const override { return (int)w.joints.size(); }
     void joint_state(double *cur, double *tgt, double *lam4, int32_t *colour) const override {
@@ -262,7 +266,7 @@ template <class T> orc::TF<T> tf7(const double *p) {
 // (15 values).  guess6 (optional): warm start in A's frame, GJK direction xyz and EPA
 // seed normal xyz (a zero seed normal: none).
 template <class T> int gjk_epa_t(const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
-                                 const double *tb, const double *guess6, double *out) {
+                                 const double *tb, const double *guess6, double *out, const double *guard3 = nullptr) {
     const orc::ShapeP<T> sa = shape_from<T>(*a), sb = shape_from<T>(*b);
     const orc::TF<T> A = tf7<T>(ta), Bt = tf7<T>(tb);
     orc::Manifold<T> m;
@@ -275,7 +279,14 @@ template <class T> int gjk_epa_t(const orc_collider_desc *a, const double *ta, c
         sn = orc::V3<T>(T(guess6[3]), T(guess6[4]), T(guess6[5]));
         seed = !(guess6[3] == 0.0 && guess6[4] == 0.0 && guess6[5] == 0.0);
     }
-    const bool hit = orc::gjk_epa_manifold(sa, A, sb, Bt, m, guess6 ? &g : nullptr, &st, &e, seed ? &sn : nullptr, &gd);
+    orc::EpaGuard<T> eg;
+    if (guard3) {   // guard3: a direction (has_prev) and, in guard3[3], the face-normal flag
+        eg.has_prev = !(guard3[0] == 0.0 && guard3[1] == 0.0 && guard3[2] == 0.0);
+        eg.prev = orc::V3<T>(T(guard3[0]), T(guard3[1]), T(guard3[2]));
+        eg.faces = guard3[3] != 0.0;
+    }
+    const bool hit = orc::gjk_epa_manifold(sa, A, sb, Bt, m, guess6 ? &g : nullptr, &st, &e, seed ? &sn : nullptr, &gd,
+                                           guard3 ? &eg : nullptr);
     out[0] = st.gjk_status; out[1] = st.gjk_iters; out[2] = st.epa_status; out[3] = st.epa_iters;
     out[4] = e.n.x; out[5] = e.n.y; out[6] = e.n.z; out[7] = e.depth;
     out[8] = e.p.x; out[9] = e.p.y; out[10] = e.p.z;
@@ -430,6 +441,9 @@ void orc_set_joint_targets(void *h, int n, const uint32_t *ids, const double *tg
 }
 void orc_set_joint_mode(void *h, int m) { static_cast<Any *>(h)->set_joint_mode(m); }
 void orc_set_contact_mode(void *h, int m) { static_cast<Any *>(h)->set_contact_mode(m); }
+void orc_set_epa_guard(void *h, int g) { static_cast<Any *>(h)->set_epa_guard(g); }
+// EPA results the guard replaced so far and the largest depth it cut (m).
+void orc_guard_stats(void *h, double *out2) { static_cast<const Any *>(h)->guard_stats(out2); }
 void orc_joint_errors(int fp32, const orc_joint_desc *d, const double *pose1, const double *pose2, double h, double *out) {
     if (fp32) joint_errors_t<float>(d, pose1, pose2, h, out);
     else joint_errors_t<double>(d, pose1, pose2, h, out);
@@ -474,6 +488,17 @@ int orc_gjk_epa(int fp32, const orc_collider_desc *a, const double *ta, const or
                 const double *guess6, double *out) {
     try {
         return fp32 ? gjk_epa_t<float>(a, ta, b, tb, guess6, out) : gjk_epa_t<double>(a, ta, b, tb, guess6, out);
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+// orc_gjk_epa with the EPA guard (see gjk_epa_manifold): guard4 = a direction (world; zero:
+// none) and a face-normal flag.
+int orc_gjk_epa_guarded(int fp32, const orc_collider_desc *a, const double *ta, const orc_collider_desc *b,
+                        const double *tb, const double *guard3, double *out) {
+    try {
+        return fp32 ? gjk_epa_t<float>(a, ta, b, tb, nullptr, out, guard3) : gjk_epa_t<double>(a, ta, b, tb, nullptr, out, guard3);
     } catch (const std::exception &) {
         return -1;
     }

@@ -544,8 +544,11 @@ inline bool capsule_plane(const ShapeP<T> &cp, const TF<T> &tc, const ShapeP<T> 
 #include "orc_gjk.hpp"
 namespace orc {
 
+template <class T> struct EpaGuard;   // orc_gjk.hpp
+// guard: the EPA guard of GJK/EPA pairs (gjk_epa_manifold), else unused.
 template <class T>
-inline bool dispatch(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, const TF<T> &tb, Manifold<T> &m) {
+inline bool dispatch(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, const TF<T> &tb, Manifold<T> &m,
+                     const EpaGuard<T> *guard = nullptr, T *guard_cut = nullptr) {
     const int ka = a.kind, kb = b.kind;
     auto sym = [&](bool hit) { if (hit) m.normal = -m.normal; return hit; };
     if (ka == SH_SPHERE && kb == SH_SPHERE) return sphere_sphere(a, ta, b, tb, m);
@@ -561,7 +564,8 @@ inline bool dispatch(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, co
     // takes the GJK/EPA primary template, in the given order (Dispatcher.hpp:69-91)
     if (ka == SH_HULL && kb == SH_PLANE) return hull_plane(a, ta, b, tb, m);
     if (ka == SH_PLANE && kb == SH_HULL) return sym(hull_plane(b, tb, a, ta, m));
-    if ((ka == SH_HULL || kb == SH_HULL) && ka != SH_PLANE && kb != SH_PLANE) return gjk_epa_manifold(a, ta, b, tb, m);
+    if ((ka == SH_HULL || kb == SH_HULL) && ka != SH_PLANE && kb != SH_PLANE)
+        return gjk_epa_manifold<T>(a, ta, b, tb, m, nullptr, nullptr, nullptr, nullptr, nullptr, guard, guard_cut);
     if (kb == SH_PLANE) {
         if (ka == SH_SPHERE) return sphere_plane(a, ta, b, tb, m);
         if (ka == SH_BOX) return box_plane(a, ta, b, tb, m);

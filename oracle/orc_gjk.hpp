@@ -779,11 +779,39 @@ inline void manifold_hulls(const V3<T> &en, T edep, const V3<T> &ep, const Hull<
 // fallback (also when the warm-started GJK is not done within 16 iterations).  stats,
 // when given, receive the statuses and iteration counts of the runs
 // made (both runs' iterations when the fallback ran) and the exit directions.
+//
+// EPA guard (no counterpart in the reference; off unless `guard` is given): any unit
+// direction n bounds the penetration depth from above by the overlap of the shapes along n,
+// h(n) = support_{A-B}(n) . n.  An FP32 EPA that settles on a face deeper than the smallest
+// h over the guard's directions by more than EPA_GUARD_TOL has not found the minimum; the
+// result is replaced by (that direction, its h) with A's support point along it as the
+// witness.  Directions, in this order (the first smallest wins): the pair's contact normal of
+// the previous substep (has_prev), then with `faces` (when the depth in question exceeds
+// EPA_GUARD_SCAN) the face normals of A (a hull's merged polygons, a box's +-x +-y +-z) and
+// the reversed face normals of B.
+constexpr float EPA_GUARD_TOL = 1e-4f;
+// The face normals are tried only when the depth still in question exceeds this (m): resting
+// contacts are a fraction of a millimetre deep, the faces FP32 EPA settles on wrongly are
+// centimetres deep, and the scan costs a support query per face.
+constexpr float EPA_GUARD_SCAN = 2e-3f;
+template <class T> struct EpaGuard {
+    bool has_prev = false, faces = false;
+    V3<T> prev;
+};
+template <class T> inline int face_normal_count(const ShapeP<T> &s) {
+    return s.kind == SH_HULL ? (int)s.hull->pn.size() : s.kind == SH_BOX ? 6 : 0;
+}
+template <class T> inline V3<T> face_normal_local(const ShapeP<T> &s, int k) {
+    if (s.kind == SH_HULL) return s.hull->pn[k];
+    const T sg = (k & 1) ? T(-1) : T(1);
+    return k < 2 ? V3<T>(sg, T(0), T(0)) : k < 4 ? V3<T>(T(0), sg, T(0)) : V3<T>(T(0), T(0), sg);
+}
 struct GjkEpaStats { int gjk_status, gjk_iters, epa_status, epa_iters, fallback; };
 template <class T>
 inline bool gjk_epa_manifold(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, const TF<T> &tb, Manifold<T> &m,
                              const V3<T> *guess_local = nullptr, GjkEpaStats *st = nullptr, EpaOut<T> *epa_out = nullptr,
-                             const V3<T> *seed_local = nullptr, V3<T> *gjk_dir = nullptr) {
+                             const V3<T> *seed_local = nullptr, V3<T> *gjk_dir = nullptr,
+                             const EpaGuard<T> *guard = nullptr, T *guard_cut = nullptr) {
     MinkDiff<T> md{&a, &b, ta, tb};
     V3<T> plain = tb.pos - ta.pos;
     if (length_sq(plain) < C<T>::EPS) plain = V3<T>(1, 0, 0);
@@ -804,6 +832,27 @@ inline bool gjk_epa_manifold(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T
     }
     if (st) { st->gjk_status = gr.status; st->gjk_iters = gi; st->epa_status = gr.status == GJK_INSIDE ? e.status : -1;
               st->epa_iters = ei; st->fallback = fb; }
+    if (guard && gr.status == GJK_INSIDE && e.status == EPA_VALID) {
+        bool any = false;
+        V3<T> bn, bp;
+        T bh = T(0);
+        auto consider = [&](const V3<T> &n) {
+            const SVert<T> w = md.sample(n);
+            const T h = dot(w.w, n);
+            if (!any || h < bh) { any = true; bh = h; bn = n; bp = w.w0; }
+        };
+        if (guard->has_prev) consider(guard->prev);
+        if (guard->faces && e.depth > T(EPA_GUARD_SCAN) && (!any || bh > T(EPA_GUARD_SCAN))) {
+            for (int k = 0, nf = face_normal_count(a); k < nf; ++k) consider(ta.vec(face_normal_local(a, k)));
+            for (int k = 0, nf = face_normal_count(b); k < nf; ++k) consider(-tb.vec(face_normal_local(b, k)));
+        }
+        if (any && bh < e.depth - T(EPA_GUARD_TOL)) {
+            if (guard_cut) *guard_cut = e.depth - bh;
+            e.n = bn;
+            e.depth = bh;
+            e.p = bp;
+        }
+    }
     if (gjk_dir) *gjk_dir = gr.dir;
     if (epa_out) *epa_out = e;
     if (gr.status != GJK_INSIDE || e.status != EPA_VALID) return false;

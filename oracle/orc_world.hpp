@@ -120,8 +120,14 @@ template <class T> struct World {
     // 1: turn the capsule-capsule normal around (the reference's routine returns it from B to
     // A, CapsuleCapsule.hpp:68, every other routine from A to B)
     int contact_mode = 0;
-    bool narrow(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, const TF<T> &tb, Manifold<T> &m) const {
-        if (!dispatch(a, ta, b, tb, m)) return false;
+    // EPA guard (gjk_epa_manifold), colour order only: 1 the pair's contact normal of the
+    // previous substep of the same frame; 2 that and the face normals of both shapes
+    int epa_guard = 0;
+    uint64_t guard_hits = 0;   // EPA results the guard replaced, and the largest depth it cut
+    double guard_max_cut = 0.0;
+    bool narrow(const ShapeP<T> &a, const TF<T> &ta, const ShapeP<T> &b, const TF<T> &tb, Manifold<T> &m,
+                const EpaGuard<T> *guard = nullptr, T *guard_cut = nullptr) const {
+        if (!dispatch(a, ta, b, tb, m, guard, guard_cut)) return false;
         if (contact_mode == 1 && a.kind == SH_CAPSULE && b.kind == SH_CAPSULE) m.normal = -m.normal;
         return true;
     }
@@ -471,8 +477,19 @@ template <class T> struct World {
         for (uint32_t p = 0; p < spairs.size(); ++p) {
             const SPair &s = spairs[p];
             Manifold<T> m;
+            EpaGuard<T> g;
+            g.has_prev = epa_guard && pcount[p] > 0;   // none in a frame's first substep
+            if (g.has_prev) g.prev = pcont[(size_t)p * 4].n;
+            g.faces = epa_guard == 2;
             pcount[p] = 0;
-            if (narrow(shape[s.ca], collider_tf(s.ca), shape[s.cb], collider_tf(s.cb), m)) {
+            T cut = T(0);
+            const bool hit = narrow(shape[s.ca], collider_tf(s.ca), shape[s.cb], collider_tf(s.cb), m,
+                                    (g.has_prev || g.faces) ? &g : nullptr, &cut);
+            if (cut > T(0)) {
+                ++guard_hits;
+                if (double(cut) > guard_max_cut) guard_max_cut = double(cut);
+            }
+            if (hit) {
                 make_contacts(m, s.ba, s.bb, s.ca, s.cb, &pcont[(size_t)p * 4]);
                 pcount[p] = m.n;
             }

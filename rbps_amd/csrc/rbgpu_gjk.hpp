@@ -649,6 +649,23 @@ __device__ __forceinline__ int epa_expand(EpaLds &L, const SV &w, int &nvtx, int
     return 0;
 }
 
+constexpr float EPA_GUARD_TOL = 1e-4f;   // depth excess (m) over the guard direction's overlap that counts
+// Face normals are tried only when the depth in question exceeds this (m): resting contacts
+// are a fraction of a millimetre deep, the faces FP32 EPA settles on wrongly centimetres.
+constexpr float EPA_GUARD_SCAN = 2e-3f;
+// Face normals the EPA guard tries (local frame): a hull's merged polygons, a box's
+// +x -x +y -y +z -z; other shapes none.
+template <class SideT> __device__ __forceinline__ int face_count(const SideT &S, const HullSet &hs, int hull) {
+    return S.kind == SH_HULL ? (int)hs.meta[hull].np : S.kind == SH_BOX ? 6 : 0;
+}
+template <class SideT> __device__ __forceinline__ V3 face_normal_local(const SideT &S, const HullSet &hs, int hull, int k) {
+    if (S.kind == SH_HULL) {
+        const float4 p = hs.pn[hs.meta[hull].pn0 + k];
+        return V3{p.x, p.y, p.z};
+    }
+    const float sg = (k & 1) ? -1.0f : 1.0f;
+    return k < 2 ? V3{sg, 0.0f, 0.0f} : k < 4 ? V3{0.0f, sg, 0.0f} : V3{0.0f, 0.0f, sg};
+}
 struct EpaResult {
     int status, iters;
     V3 n, p;
@@ -1245,7 +1262,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? O
                                                  Bodies B, Contacts K,
                                                  HullSet hs, uint32_t *gstat, const float4 *guess, int seed_epa,
                                                  float4 *tlog, const uint32_t *tlog_base, const uint32_t *hits,
-                                                 const uint32_t *blk_hits, const float4 *sxs) {
+                                                 const uint32_t *blk_hits, const float4 *sxs, int guard) {
     __shared__ PairLds L;
     const int lane = lane64();
     const uint32_t nc = gjk_count(n, ncand);
@@ -1292,6 +1309,38 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? O
             }
         }
         const bool ok = gs == GJK_INSIDE && e.status == EPA_VALID;
+        // EPA guard: the overlap along any direction bounds the depth from above; an EPA
+        // face deeper than the smallest overlap over the guard's directions by more than the
+        // tolerance is not the minimum and gives way to that direction.  Directions, first
+        // smallest wins: the pair's previous-substep normal (K still holds the previous
+        // substep's contacts of this slot; none in a frame's first substep, the pair gather
+        // clears them), then with guard 2, while the depth in question exceeds
+        // EPA_GUARD_SCAN, A's face normals and B's reversed.  The loops are uniform over the
+        // wavefront (one pair).
+        if (ok && guard) {
+            bool any = false;
+            V3 bn{0.0f, 0.0f, 0.0f}, bp{0.0f, 0.0f, 0.0f};
+            float bh = 0.0f;
+            auto consider = [&](V3 gn) {
+                const SV w = md_sample(A, Bs, hs, gn);
+                const float hov = dot(w.w, gn);
+                if (!any || hov < bh) { any = true; bh = hov; bn = gn; bp = w.w0; }
+            };
+            if (K.cnt[s] > 0) {
+                const float4 hp = hd_row(K, s)[0];
+                consider(V3{hp.x, hp.y, hp.z});
+            }
+            if (guard == 2 && e.depth > EPA_GUARD_SCAN && (!any || bh > EPA_GUARD_SCAN)) {
+                const int nfa = face_count(A, hs, C.hull[cc.x]), nfb = face_count(Bs, hs, C.hull[cc.y]);
+                for (int k = 0; k < nfa; ++k) consider(tf_vec(A.tf, face_normal_local(A, hs, C.hull[cc.x], k)));
+                for (int k = 0; k < nfb; ++k) consider(-tf_vec(Bs.tf, face_normal_local(Bs, hs, C.hull[cc.y], k)));
+            }
+            if (any && bh < e.depth - EPA_GUARD_TOL) {
+                e.n = bn;
+                e.depth = bh;
+                e.p = bp;
+            }
+        }
         if (lane == 0) {
             gstat[s] = pack_gstat(gsum, esum, gs, gs == GJK_INSIDE ? e.status : 15, fb);
             if (tlog) {

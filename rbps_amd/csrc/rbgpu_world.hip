@@ -73,10 +73,11 @@ struct Readback {
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
-// One 16-byte field of every body row, copied to a packed host array.
-hipError_t body_column(float4 *dst, const BodyRow *rows, size_t field, uint32_t n) {
-    return hipMemcpy2D(dst, sizeof(float4), reinterpret_cast<const char *>(rows) + field, sizeof(BodyRow),
-                       sizeof(float4), n, hipMemcpyDeviceToHost);
+// Every body row, in one contiguous copy to the host (a strided copy of single fields moves
+// 16 bytes per row and is far slower).
+hipError_t body_rows(std::vector<BodyRow> &dst, const BodyRow *rows, uint32_t n) {
+    dst.resize(n);
+    return n ? hipMemcpy(dst.data(), rows, (size_t)n * sizeof(BodyRow), hipMemcpyDeviceToHost) : hipSuccess;
 }
 
 __global__ void k_offsets(const uint32_t *hoff, uint32_t ntiles, uint32_t nd, const uint32_t *n, uint32_t *out) {
@@ -159,6 +160,7 @@ struct rbgpu_world {
     bool axis_carry = true;
     int64_t ax_restore = -1;   // >= 0: the next frame's hint table came from a snapshot (its size; 0 empty)
     int contact_mode = 0;                               // 1: capsule-capsule normal from A to B
+    int epa_guard = 0;                                  // EPA depth bounded along the previous normal (1), and faces (2)
     bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
     int gjk_layout = 1;                                 // GJK pass: 0 a pair per wavefront, 1 per 16-lane row (hulls <= 64
@@ -342,6 +344,7 @@ struct rbgpu_world {
     bool gjk_log = false;
     bool gjk_explicit = false;   // warm starts given by the caller (rbgpu_collide_world_pairs)
     int launch_gjk(uint32_t g0, uint32_t ng) {
+        const int guard = epa_guard;
         gjk_log = false;
         if (ds_logging && !gjk_explicit && ds_feat) {
             if (ds_hi + ng > ds_cap) {   // near capacity: the exact count decides
@@ -393,10 +396,10 @@ struct rbgpu_world {
             if (int rcf = fork_at(3)) return rcf;
             if (hull_vpl == 1)
                 k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
-                                                 ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+                                                 ds_cnt, gjk_hits, gjk_blk, gjk_sx, guard);
             else
                 k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp,
-                                                        seed, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+                                                        seed, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, guard);
         } else if (rows) {
             // waves per EU the compiler budgets registers for: 4 caps them at 128 (some spill), 3 and 2 do not
             if (gjk_occ == 2)
@@ -412,17 +415,17 @@ struct rbgpu_world {
             if (int rcf = fork_at(3)) return rcf;
             if (epa_occ == 4)
                 k_epa_pass<1, 4><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
-                                                    ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+                                                    ds_cnt, gjk_hits, gjk_blk, gjk_sx, guard);
             else
                 k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
-                                                 ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+                                                 ds_cnt, gjk_hits, gjk_blk, gjk_sx, guard);
         } else if (hull_vpl == 1) {
             k_gjk_pass<1, 64><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
                                                                   tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
             timed_end(e);
             if (int rcf = fork_at(3)) return rcf;
             k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl, ds_cnt,
-                                             gjk_hits, gjk_blk, gjk_sx);
+                                             gjk_hits, gjk_blk, gjk_sx, guard);
         } else {
             k_gjk_pass<HULL_VPL, 64><<<g1, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat,
                                                                          gp, tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx,
@@ -430,7 +433,7 @@ struct rbgpu_world {
             timed_end(e);
             if (int rcf = fork_at(3)) return rcf;
             k_epa_pass<HULL_VPL><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
-                                                    ds_cnt, gjk_hits, gjk_blk, gjk_sx);
+                                                    ds_cnt, gjk_hits, gjk_blk, gjk_sx, guard);
         }
         if (e && !cand) timing_bytes += (double)ng * gjk_pass_bytes();
         if (gjk_tot) k_gjk_totals<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, gstat, gjk_tot);
@@ -956,19 +959,15 @@ int rbgpu_world::build_joints() {
 int rbgpu_world::download_state() {
     if (!built) return RBGPU_OK;
     HIPCHK(hipStreamSynchronize(st));
-    std::vector<float4> x(N), q(N), v(N), w(N);
-    if (N) {
-        HIPCHK(body_column(x.data(), B.r, offsetof(BodyRow, x), N));
-        HIPCHK(body_column(q.data(), B.r, offsetof(BodyRow, q), N));
-        HIPCHK(body_column(v.data(), B.r, offsetof(BodyRow, v), N));
-        HIPCHK(body_column(w.data(), B.r, offsetof(BodyRow, w), N));
-    }
+    std::vector<BodyRow> rows;
+    HIPCHK(body_rows(rows, B.r, N));
     for (uint32_t i = 0; i < N; ++i) {
         rbgpu_body_desc &d = hb[i];
-        d.position[0] = x[i].x; d.position[1] = x[i].y; d.position[2] = x[i].z;
-        d.orientation[0] = q[i].x; d.orientation[1] = q[i].y; d.orientation[2] = q[i].z; d.orientation[3] = q[i].w;
-        d.linear_velocity[0] = v[i].x; d.linear_velocity[1] = v[i].y; d.linear_velocity[2] = v[i].z;
-        d.angular_velocity[0] = w[i].x; d.angular_velocity[1] = w[i].y; d.angular_velocity[2] = w[i].z;
+        const BodyRow &r = rows[i];
+        d.position[0] = r.x.x; d.position[1] = r.x.y; d.position[2] = r.x.z;
+        d.orientation[0] = r.q.x; d.orientation[1] = r.q.y; d.orientation[2] = r.q.z; d.orientation[3] = r.q.w;
+        d.linear_velocity[0] = r.v.x; d.linear_velocity[1] = r.v.y; d.linear_velocity[2] = r.v.z;
+        d.angular_velocity[0] = r.w.x; d.angular_velocity[1] = r.w.y; d.angular_velocity[2] = r.w.z;
     }
     if (S) {
         HIPCHK(hipMemcpy(h_fmin.data(), fmin, S * 16, hipMemcpyDeviceToHost));
@@ -1559,21 +1558,16 @@ int rbgpu_get_bodies(rbgpu_world *w, double *p, double *q, double *v, double *av
         return RBGPU_OK;
     }
     if (int rcs = w->sync()) return rcs;
-    std::vector<float4> buf(n);
-    auto pull = [&](size_t field, double *dst, bool quat) -> int {
-        if (!dst || !n) return RBGPU_OK;
-        HIPCHK(body_column(buf.data(), w->B.r, field, n));
-        for (uint32_t i = 0; i < n; ++i) {
-            if (quat) { dst[4 * i] = buf[i].x; dst[4 * i + 1] = buf[i].y; dst[4 * i + 2] = buf[i].z; dst[4 * i + 3] = buf[i].w; }
-            else { dst[3 * i] = buf[i].x; dst[3 * i + 1] = buf[i].y; dst[3 * i + 2] = buf[i].z; }
-        }
-        return RBGPU_OK;
-    };
-    int rc = pull(offsetof(BodyRow, x), p, false);
-    if (!rc) rc = pull(offsetof(BodyRow, q), q, true);
-    if (!rc) rc = pull(offsetof(BodyRow, v), v, false);
-    if (!rc) rc = pull(offsetof(BodyRow, w), av, false);
-    return rc;
+    std::vector<BodyRow> rows;
+    HIPCHK(body_rows(rows, w->B.r, n));
+    for (uint32_t i = 0; i < n; ++i) {
+        const BodyRow &r = rows[i];
+        if (p) { p[3 * i] = r.x.x; p[3 * i + 1] = r.x.y; p[3 * i + 2] = r.x.z; }
+        if (q) { q[4 * i] = r.q.x; q[4 * i + 1] = r.q.y; q[4 * i + 2] = r.q.z; q[4 * i + 3] = r.q.w; }
+        if (v) { v[3 * i] = r.v.x; v[3 * i + 1] = r.v.y; v[3 * i + 2] = r.v.z; }
+        if (av) { av[3 * i] = r.w.x; av[3 * i + 1] = r.w.y; av[3 * i + 2] = r.w.z; }
+    }
+    return RBGPU_OK;
 }
 
 int rbgpu_set_bodies(rbgpu_world *w, const double *p, const double *q, const double *v, const double *av) {
@@ -1820,6 +1814,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_AXIS_CARRY:
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "axis carry is 0 or 1");
             w->axis_carry = value != 0; return RBGPU_OK;
+        case RBGPU_OPT_EPA_GUARD:
+            if (value < 0 || value > 2) return fail(RBGPU_ERR_INVALID, "EPA guard is 0, 1 or 2");
+            w->epa_guard = value; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1852,6 +1849,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_SOLVE_OCCUPANCY: *value = w->solve_occ; return RBGPU_OK;
         case RBGPU_OPT_SIDE_STREAM: *value = w->side_stream; return RBGPU_OK;
         case RBGPU_OPT_AXIS_CARRY: *value = w->axis_carry ? 1 : 0; return RBGPU_OK;
+        case RBGPU_OPT_EPA_GUARD: *value = w->epa_guard; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
@@ -2145,9 +2143,12 @@ int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2
     }
     w->gjk_explicit = guess6 != nullptr;
     const bool cull_saved = w->gjk_cull;
+    const int guard_saved = w->epa_guard;
     w->gjk_cull = false;   // every pair runs GJK here: the entry reports its iterations and statuses
+    w->epa_guard = 0;      // the listed pairs have no previous substep
     const int rcn = w->launch_narrowphase();
     w->gjk_cull = cull_saved;
+    w->epa_guard = guard_saved;
     w->gjk_explicit = false;
     w->mlp_mode = mode_saved;
     w->inf_guess = gsave;
@@ -2461,7 +2462,7 @@ namespace {
 
 constexpr char SNAP_MAGIC[8] = {'R', 'B', 'G', 'P', 'U', 'S', 'N', 'P'};
 constexpr uint32_t SNAP_VERSION = 1;
-constexpr int SNAP_OPTIONS = RBGPU_OPT_AXIS_CARRY + 1;
+constexpr int SNAP_OPTIONS = RBGPU_OPT_EPA_GUARD + 1;
 constexpr size_t SNAP_ALIGN = 256;
 
 struct SnapSec {

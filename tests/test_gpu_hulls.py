@@ -120,3 +120,27 @@ def test_gpu_gjk_cycling_pairs_match_oracle(layout):
         assert out["gjk_status"][i] == s["gjk_status"] == orc.GJK_FAILED, i
         assert out["gjk_iters"][i] == s["gjk_iters"] == 128, i
         assert out["n"][i] == 0, i
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_gpu_epa_guard_matches_fp32_oracle(mode):
+    """The EPA guard (RBGPU_OPT_EPA_GUARD; 1: the previous substep's normal, 2: and both
+    shapes' face normals) on a settling pile of 256 hulls: the GPU equals the FP32
+    colour-order oracle with the same guard every frame, through the frames where the guard
+    replaces FP32 EPA results (the oracle counts them: at least one)."""
+    from rbps_amd import world
+    sc = scenes.hull_pile(256, seed=3)
+    print("case hull pile 256, EPA guard", mode, flush=True)
+    g = world.World(sc.timestep, sc.substeps).load(sc)
+    g.set_option("epa_guard", mode)
+    o = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
+    o.set_epa_guard(mode)
+    for f in range(60):
+        g.step(1)
+        o.step(1)
+        ok, worst = state_close(g.state(), o.state(), 1e-5)
+        assert ok, f"frame {f + 1}: {worst}"
+    cg, co = g.contacts(), o.contacts()
+    assert np.array_equal(cg["ids"], co["ids"])
+    assert np.allclose(cg["normal"], co["normal"], rtol=1e-5, atol=1e-5)
+    assert o.guard_stats()[0] > 0

@@ -191,3 +191,86 @@ def test_fp32_gjk_cycles_on_touching_hulls():
         decided64 += orc.oracle_gjk_trace(a[i], tfa[i], b[i], tfb[i], fp32=False, hulls=hulls)["gjk_status"] != orc.GJK_FAILED
     print(f"FP64 decides {decided64} of {len(a)}")
     assert len(a) >= 40
+
+
+def _guard_pair():
+    g = golden("epa_guard_pairs")
+    hulls = [(g["va"], g["fa"]), (g["vb"], g["fb"])]
+    a, b = np.zeros((), scenes.COLLIDER_DTYPE), np.zeros((), scenes.COLLIDER_DTYPE)
+    for r, i in ((a, 0), (b, 1)):
+        r["shape"], r["hull_index"], r["local_rot"] = scenes.HULL, i, (1, 0, 0, 0)
+    return g, hulls, a, b
+
+
+def _overlap(verts_a, tfa, verts_b, tfb, n):
+    """Overlap of two posed hulls along unit n: max over A of x.n minus min over B of x.n
+    (the Minkowski difference's support along n), in FP64 from the vertices."""
+    def pose(v, tf):
+        w, x, y, z = tf[3:]
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        return v @ R.T + tf[:3]
+    return float((pose(verts_a, tfa) @ n).max() - (pose(verts_b, tfb) @ n).min())
+
+
+def test_epa_guard_on_the_traced_touching_pair():
+    """Two library hulls face to face at zero penetration (tests/golden/epa_guard_pairs.npz,
+    tests/golden/make_epa_guard.py): FP64 EPA finds the contact (depth < 1e-6), the FP32
+    restatement settles on a face 4 cm deep along another normal.  Any direction's overlap
+    bounds the depth from above; with the face normal as the guard's direction the guarded
+    FP32 result is (that normal, its overlap), within 1e-5 m of zero.  Pressed 1 mm together
+    the pair is found right by FP32 and the guard leaves the result unchanged."""
+    g, hulls, a, b = _guard_pair()
+    n = np.asarray(g["guard"], np.float64)
+    r64 = orc.oracle_gjk_epa(a, g["tfa"], b, g["tfb"], hulls=hulls)
+    r32 = orc.oracle_gjk_epa(a, g["tfa"], b, g["tfb"], fp32=True, hulls=hulls)
+    assert r64["hit"] and r64["depth"] < 1e-6
+    assert r32["hit"] and r32["depth"] > 0.04   # the failure the guard is for
+    # the bound itself (FP64, from the vertices): the FP64 depth is below every direction's
+    # overlap, the face normal's overlap is about zero, the bogus FP32 face's far above it
+    rng = np.random.default_rng(3)
+    for d in rng.normal(size=(200, 3)):
+        d /= np.linalg.norm(d)
+        assert _overlap(g["va"], g["tfa"], g["vb"], g["tfb"], d) >= r64["depth"] - 1e-12
+    assert abs(_overlap(g["va"], g["tfa"], g["vb"], g["tfb"], n)) < 1e-5
+    gd = orc.oracle_gjk_epa(a, g["tfa"], b, g["tfb"], fp32=True, hulls=hulls, guard=n)
+    assert gd["hit"] and abs(gd["depth"]) < 1e-5
+    assert np.allclose(gd["normal"], n.astype(np.float32), atol=0)
+    assert gd["depth"] <= r32["depth"] - 1e-4
+    # without a previous normal the face normals of the two hulls find the contact too
+    gf = orc.oracle_gjk_epa(a, g["tfa"], b, g["tfb"], fp32=True, hulls=hulls, guard_faces=True)
+    assert gf["hit"] and abs(gf["depth"]) < 1e-5
+    # FP64 with the guard: nothing to correct
+    gd64 = orc.oracle_gjk_epa(a, g["tfa"], b, g["tfb"], hulls=hulls, guard=n)
+    assert gd64["depth"] == r64["depth"] and np.array_equal(gd64["normal"], r64["normal"])
+    # pressed together: FP32 right, the guard changes nothing
+    p32 = orc.oracle_gjk_epa(a, g["tfa"], b, g["tfb_pressed"], fp32=True, hulls=hulls)
+    p64 = orc.oracle_gjk_epa(a, g["tfa"], b, g["tfb_pressed"], hulls=hulls)
+    assert abs(p32["depth"] - p64["depth"]) < 1e-5 and abs(p64["depth"] - 1e-3) < 1e-4
+    pg = orc.oracle_gjk_epa(a, g["tfa"], b, g["tfb_pressed"], fp32=True, hulls=hulls, guard=n)
+    assert pg["depth"] == p32["depth"] and np.array_equal(pg["normal"], p32["normal"])
+
+
+def test_epa_guard_in_the_world_changes_only_what_it_should():
+    """The guard in the colour-order world (the pair's contact normal of the previous
+    substep as its direction): until it first replaces an EPA result the run is
+    bit-identical to the unguarded one; on a settling pile of 256 hulls it does act within
+    60 frames, cutting a bogus centimetre-deep EPA face (resting contacts are a fraction of
+    a millimetre deep)."""
+    sc = scenes.hull_pile(256, seed=3)
+    plain = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
+    guard = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
+    guard.set_epa_guard(1)
+    for f in range(60):
+        plain.step(1)
+        guard.step(1)
+        hits, cut = guard.guard_stats()
+        if hits:
+            break
+        sp, sg = plain.state(), guard.state()
+        for k in sp:
+            assert np.array_equal(sp[k], sg[k]), f"frame {f + 1}: {k}"
+    assert hits > 0 and f >= 10
+    assert cut > 0.01
+    assert plain.guard_stats() == (0, 0.0)
