@@ -266,8 +266,10 @@ def plain_leg(world, sc, args, log_frames, pos_learned, value):
 def measured_traffic(kernel, workload="pile"):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload
     (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings), or None."""
-    name = {"pile": f"{kernel}_pile_mixed_65k", "combined": "gjk_pass_combined_1024_scenes"}.get(workload)
-    if name is None or (workload == "combined" and kernel != "gjk"):
+    name = {"pile": f"{kernel}_pile_mixed_65k",
+            "combined": {"gjk": "gjk_pass_combined_1024_scenes",
+                         "solve_pos": "scene_solve_pos_combined_1024_scenes"}.get(kernel)}.get(workload)
+    if name is None:
         return None
     path = os.path.join(ROOT, "profiles", f"{name}_hbm_traffic.json")
     if not os.path.exists(path):
@@ -296,9 +298,10 @@ def solve_roofline(w, frames=20):
     r = roofline_of(w.timing(), "k_scene_solve<pos> / k_solve_pos")
     w.set_timing("off")
     if r:
-        r["traffic"] = None
+        r["traffic"] = measured_traffic("solve_pos", "combined")
         r["note"] = (f"frames after the timed region ({frames}); algorithmic bytes = 296 B per contact point "
-                     "solved (contact row, two bodies gathered and written, lambda)")
+                     "solved (contact row, two bodies gathered and written, lambda); traffic: rocprofv3 PMC "
+                     "of the timed frames (it also carries the scenes' joint rows)")
     return r
 
 
