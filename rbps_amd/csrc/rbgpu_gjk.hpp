@@ -673,7 +673,7 @@ struct EpaResult {
 };
 
 template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs,
-                                         EpaLds &L, const V3 *seed_n) {
+                                         EpaLds &L, bool seeded, V3 seed_n) {
     EpaResult r{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
     const int lane = lane64();
     if (seed.rank != 4) { r.status = EPA_NONLINEAR; return r; }
@@ -726,11 +726,11 @@ template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed,
     }
     __syncthreads();
     int it0 = 0;
-    if (seed_n) {
+    if (seeded) {
         // warm start: one expansion along a predicted normal, when its support is a new
         // vertex that a face sees
         it0 = 1;
-        const SV w = md_sample(A, Bs, hs, *seed_n);
+        const SV w = md_sample(A, Bs, hs, seed_n);
         bool dup = false, seen = false;
         if (lane < nvtx) dup = len2(lds_w(L, lane) - w.w) < EPS * EPS;
         if (lane < nf) {
@@ -896,7 +896,9 @@ __device__ __forceinline__ bool skew_lines(V3 p1, V3 d1, V3 p2, V3 d2, V3 &l1, V
 __device__ __forceinline__ int reduce4_area(const ClipLds &C, int buf, int n, Manifold &m) {
     auto P = [&](int i) { return V3{C.px[buf][i], C.py[buf][i], C.pz[buf][i]}; };
     if (n <= 4) {
-        for (int i = 0; i < n; ++i) { m.p[i] = P(i); m.dep[i] = C.dep[i]; }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)   // constant indices keep the manifold in registers
+            if (i < n) { m.p[i] = P(i); m.dep[i] = C.dep[i]; }
         return n;
     }
     int i0 = 0;
@@ -987,10 +989,11 @@ template <int VPL> __device__ __forceinline__ void manifold_hulls(const EpaResul
         return;
     }
     const bool ref_a = al_a > al_b;
-    const HullTabs &hr = ref_a ? ha : hb;
-    const HullTabs &hi = ref_a ? hb : ha;
-    const TF &tr = ref_a ? A.tf : Bs.tf;
-    const TF &ti = ref_a ? Bs.tf : A.tf;
+    // by value: a reference to one of two locals would put both on the stack
+    const HullTabs hr = ref_a ? ha : hb;
+    const HullTabs hi = ref_a ? hb : ha;
+    const TF tr = ref_a ? A.tf : Bs.tf;
+    const TF ti = ref_a ? Bs.tf : A.tf;
     const uint32_t rp = ref_a ? pa : pb, ip = ref_a ? pb : pa;
     const V3 rn = tf_vec(tr, tv(hr.pn, rp));
     const uint32_t is = hi.poff[ip], ie = hi.poff[ip + 1];
@@ -1253,6 +1256,140 @@ __global__ void __launch_bounds__(64) k_gjk_lanes(const uint32_t *bin_slot, uint
     }
 }
 
+// Pass 1 with one pair per 16-lane DPP row and rows refilled as they finish: a wavefront
+// owns a range of GJK_REFILL_SEG candidates and runs GJK one iteration at a time on its four
+// rows; a row whose pair is done takes the range's next candidate at once, so a wavefront
+// does not wait for the slowest of four pairs (k_gjk_pass does, every four).  Each pair runs
+// exactly the iterations gjk() runs (same seeds, warm-start budget, fallback, cycle test),
+// so statuses, iteration counts and simplices are those of k_gjk_pass.  Overlapping pairs of
+// range j are listed at GJK_REFILL_SEG * j + r (in completion order), blk_hits[j] of them,
+// the simplices beside them: the EPA pass reads them with seg = GJK_REFILL_SEG.
+constexpr uint32_t GJK_REFILL_SEG = 16;
+template <int OCC>
+__global__ void __launch_bounds__(64 * GJK_PASS_WAVES) __attribute__((amdgpu_waves_per_eu(OCC)))
+k_gjk_refill(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand, const uint32_t *ncand, Colliders C,
+             Bodies B, Contacts K, HullSet hs, uint32_t *gstat, const float4 *guess, float4 *tlog, const uint32_t *tlog_base,
+             uint32_t *hits, uint32_t *blk_hits, float4 *sxs, float4 *axis) {
+    constexpr int LPP = 16;
+    const uint32_t nc = gjk_count(n, ncand);
+    if (tlog && tlog_base) tlog += 2 * (size_t)*tlog_base;
+    const int lane = lane64(), row = lane >> 4, pl = lane & 15;
+    const uint32_t wave = blockIdx.x * GJK_PASS_WAVES + threadIdx.x / 64, nwaves = gridDim.x * GJK_PASS_WAVES;
+    for (uint32_t rg = wave; rg * GJK_REFILL_SEG < nc; rg += nwaves) {   // uniform per wavefront
+        const uint32_t end = min(nc, (rg + 1) * GJK_REFILL_SEG);
+        uint32_t next = rg * GJK_REFILL_SEG, nhit = 0;   // wavefront-uniform
+        // this row's pair
+        bool active = false;
+        uint32_t tc = 0, s = 0;
+        Side<4, LPP> A, Bs;   // hulls of at most 64 vertices: four per lane
+        Simplex sx;
+        V3 dir{0.0f, 0.0f, 0.0f}, plain{0.0f, 0.0f, 0.0f};
+        GjkCycle<LPP> cyc;
+        int it = 0, max_it = GJK_ITERS, gsum = 0, fb = 0;
+        bool warm = false;
+        for (;;) {
+            // rows without a pair take the next candidates, in row order
+            const uint32_t need = (uint32_t)__popcll(__ballot(!active && pl == 0) & 0x0001000100010001ull);
+            const uint32_t before = (uint32_t)__popcll(__ballot(!active && pl == 0) & ((1ull << (row * 16)) - 1ull));
+            if (!active && next + before < end) {
+                tc = next + before;
+                const uint32_t t = cand ? cand[tc] : tc;   // bin position
+                s = bin_slot[off + t];
+                const uint2 cc = K.cc[s], bb = K.bb[s];
+                const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+                load_side(A, C, hs, cc.x, ta);
+                load_side(Bs, C, hs, cc.y, tb);
+                const GjkSeeds q = gjk_seeds(ta, tb, guess, tc, 0);
+                warm = q.warm;
+                plain = q.plain;
+                max_it = warm ? GJK_WARM_ITERS : GJK_ITERS;
+                gsum = 0;
+                fb = 0;
+                // the start of gjk(): the first support along the seed
+                sx.rank = 0;
+                dir = (len2(q.g) > EPS) ? q.g : V3{1.0f, 0.0f, 0.0f};
+                const SV v0 = md_sample(A, Bs, hs, dir);
+                sx_push(sx, v0);
+                dir = -v0.w;
+                cyc = GjkCycle<LPP>();
+                it = 0;
+                active = true;
+            }
+            next = min(end, next + need);
+            if (__ballot(active) == 0ull) break;   // the range is done
+            int gs = -1;   // this row's pair ended with status gs (>= 0)
+            if (active) {
+                // one iteration of gjk()'s loop
+                const SV nx = md_sample(A, Bs, hs, dir);
+                if (dot(nx.w, dir) < 0.0f) {
+                    sx.dir = dir;
+                    gs = GJK_VALID;
+                } else {
+                    sx_push(sx, nx);
+                    bool inside = false;
+                    if (sx.rank == 2) sx_line(sx, dir);
+                    else if (sx.rank == 3) sx_triangle(sx, dir);
+                    else inside = sx_tetra(sx, dir);
+                    if (inside) gs = GJK_INSIDE;
+                    else if (cyc.repeats(sx, dir)) { it = max_it - 1; gs = GJK_FAILED; }
+                    else if (it + 1 == max_it) gs = GJK_FAILED;
+                }
+                ++it;
+                if (gs >= 0) {
+                    gsum += it;
+                    if (warm && gs == GJK_FAILED) {   // the warm-started run failed: the plain run follows
+                        fb = 1;
+                        warm = false;
+                        max_it = GJK_ITERS;
+                        sx.rank = 0;
+                        dir = (len2(plain) > EPS) ? plain : V3{1.0f, 0.0f, 0.0f};
+                        const SV v0 = md_sample(A, Bs, hs, dir);
+                        sx_push(sx, v0);
+                        dir = -v0.w;
+                        cyc = GjkCycle<LPP>();
+                        it = 0;
+                        gs = -1;
+                    }
+                }
+                if (gs >= 0 && pl == 0) {
+                    gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);   // pass 2 completes an overlapping pair
+                    if (axis) {
+                        const float ln = len(sx.dir);
+                        axis[s] = gs == GJK_VALID && ln > 1e-12f ? f4(vdiv(sx.dir, ln), 1.0f)
+                                                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
+                    if (gs != GJK_INSIDE) {
+                        K.cnt[s] = 0;
+                        if (tlog) {
+                            V3 tg{0.0f, 0.0f, 0.0f};
+                            float vg = 0.0f;
+                            const float ln = len(sx.dir);
+                            if (gs == GJK_VALID && ln > 1e-12f) { tg = tf_inv_vec(A.tf, vdiv(-sx.dir, ln)); vg = 1.0f; }
+                            tlog[2 * tc] = f4(tg, vg);
+                            tlog[2 * tc + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        }
+                    }
+                }
+            }
+            // list this iteration's overlapping pairs, in row order
+            const uint64_t ins = __ballot(gs == GJK_INSIDE && pl == 0);
+            if (gs == GJK_INSIDE && pl == 0) {
+                const uint32_t h = rg * GJK_REFILL_SEG + nhit + (uint32_t)__popcll(ins & ((1ull << lane) - 1ull));
+                hits[h] = tc;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const SV &v = sx.v[k];
+                    sxs[(size_t)h * 8 + 2 * k] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
+                    sxs[(size_t)h * 8 + 2 * k + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
+                }
+            }
+            nhit += (uint32_t)__popcll(ins);
+            if (gs >= 0) active = false;
+        }
+        if (lane == 0) blk_hits[rg] = nhit;
+    }
+}
+
 // seg: pairs per GJK-pass workgroup (gjk_pass_pairs of the pass that listed the pairs); a
 // workgroup takes `sub` of a segment's entries at a time (sub divides seg).
 // OCC: waves per EU the compiler budgets registers for (0: no budget).
@@ -1294,7 +1431,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? O
         const GjkSeeds q = gjk_seeds(ta, tb, guess, tc, seed_epa);
         const bool warm = q.warm && !fb;   // a pass-1 fallback already ran the plain GJK
         __syncthreads();
-        EpaResult e = epa(sx, A, Bs, hs, L.e, warm && q.seeded ? &q.sn : nullptr);
+        // the seed by value: a pointer into q would keep q in scratch memory
+        EpaResult e = epa(sx, A, Bs, hs, L.e, warm && q.seeded, q.sn);
         int esum = e.iters;
         if (warm && e.status != EPA_VALID) {
             fb = 1;
@@ -1304,7 +1442,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? O
             gsum += it;
             e = EpaResult{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
             if (gs == GJK_INSIDE) {
-                e = epa(sx, A, Bs, hs, L.e, nullptr);
+                e = epa(sx, A, Bs, hs, L.e, false, V3{0.0f, 0.0f, 0.0f});
                 esum += e.iters;
             }
         }

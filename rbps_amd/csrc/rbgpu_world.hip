@@ -381,13 +381,36 @@ struct rbgpu_world {
         const int seed = mlp_mode == 2 ? 1 : 0;
         // four pairs per wavefront (one per DPP row) when every hull has at most 64 vertices
         const bool lanes = gjk_layout == 2;
+        const bool refill = gjk_layout == 3 && hull_vpl == 1;
         const bool rows = !lanes && gjk_layout == 1 && hull_vpl == 1;
-        const uint32_t ppb = lanes ? GJK_LANE_SEG : rows ? (uint32_t)gjk_pass_pairs<16>() : (uint32_t)gjk_pass_pairs<64>();
+        const uint32_t ppb = lanes ? GJK_LANE_SEG
+                             : refill ? GJK_REFILL_SEG
+                             : rows ? (uint32_t)gjk_pass_pairs<16>() : (uint32_t)gjk_pass_pairs<64>();
         const uint32_t nvb = (ng + ppb - 1) / ppb;
         const uint32_t sub = std::min<uint32_t>(epa_sub, ppb);
         const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb * (ppb / sub), epa_grid);
         hipEvent_t *e = timed_begin(11);   // the GJK pass alone
-        if (lanes) {
+        if (refill) {
+            // a wavefront per range of GJK_REFILL_SEG candidates, its rows refilled as they finish
+            const uint32_t gr = std::min<uint32_t>((nvb + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES, gjk_grid);
+            if (gjk_occ == 2)
+                k_gjk_refill<2><<<gr, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
+                                                                   tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            else if (gjk_occ == 3)
+                k_gjk_refill<3><<<gr, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
+                                                                   tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            else
+                k_gjk_refill<4><<<gr, 64 * GJK_PASS_WAVES, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp,
+                                                                   tl, ds_cnt, gjk_hits, gjk_blk, gjk_sx, gjk_axis);
+            timed_end(e);
+            if (int rcf = fork_at(3)) return rcf;
+            if (epa_occ == 4)
+                k_epa_pass<1, 4><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
+                                                    ds_cnt, gjk_hits, gjk_blk, gjk_sx, guard);
+            else
+                k_epa_pass<1><<<g2, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, ppb, sub, C, B, K, hs, gstat, gp, seed, tl,
+                                                 ds_cnt, gjk_hits, gjk_blk, gjk_sx, guard);
+        } else if (lanes) {
             // one pair per lane, 64-thread workgroups; the EPA pass reads 16-candidate segments
             const uint32_t gl = std::min<uint32_t>(nblk(ng, 64), gjk_grid);
             k_gjk_lanes<<<gl, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, tl, ds_cnt, gjk_hits,
@@ -1797,7 +1820,8 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "contact mode is 0 or 1");
             w->contact_mode = value; return RBGPU_OK;
         case RBGPU_OPT_GJK_ROWS:
-            if (value < 0 || value > 2) return fail(RBGPU_ERR_INVALID, "GJK layout: 0 wavefront, 1 row, 2 lane per pair");
+            if (value < 0 || value > 3)
+                return fail(RBGPU_ERR_INVALID, "GJK layout: 0 wavefront, 1 row, 2 lane per pair, 3 refilled rows");
             w->gjk_layout = value; return RBGPU_OK;
         case RBGPU_OPT_GJK_OCCUPANCY:
             if (value < 2 || value > 4) return fail(RBGPU_ERR_INVALID, "GJK occupancy: 2, 3 or 4 waves per EU");

@@ -62,7 +62,7 @@ def test_gpu_full_batch_runs_without_cross_scene_pairs():
     assert len(pr) > 0
 
 
-@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("layout", [1, 2, 3])
 def test_gpu_combined_batch_matches_fp32_oracle_and_single_worlds(layout):
     """Hulls from a shared library, boxes and ragdolls (revolute + spherical joints with
     60 deg limits) in three uneven scenes; 80 frames at dt = 1/120 s, 8 substeps, so the

@@ -44,8 +44,8 @@ def _expected_surface_points(r):
     return p + n[None, :] * half[:, None], p - n[None, :] * half[:, None]
 
 
-# GJK pass layouts (option gjk_rows): a pair per wavefront, per 16-lane row, per lane
-LAYOUTS = [0, 1, 2]
+# GJK pass layouts (option gjk_rows): a pair per wavefront, per 16-lane row, per lane, per refilled row
+LAYOUTS = [0, 1, 2, 3]
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
@@ -77,7 +77,7 @@ def test_gpu_hull_pairs_bit_exact_vs_fp32_oracle(layout):
     assert n_gjk > 1000
 
 
-@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("layout", [1, 2, 3])
 @pytest.mark.parametrize("make,frames", [
     (lambda: scenes.hull_zoo(), 40),
     (lambda: scenes.hull_pile(96, seed=3), 30),
