@@ -1545,8 +1545,8 @@ __device__ __forceinline__ float support_extent(const Colliders &C, const HullSe
 
 // Separating-axis hints across a frame's new pair list: the old list's hints of its GJK
 // slots go into an open-addressing table keyed by collider pair (linear probing, at most
-// half full), and the new list's GJK slots look their pair up.  A slot whose pair is absent
-// keeps what it held (a hint proves nothing by itself: k_gjk_cull tests it exactly).
+// half full), and the new list's GJK slots look their pair up (a slot whose pair is absent
+// starts without a hint; a hint proves nothing by itself: k_gjk_cull tests it exactly).
 // The slot of a pair changes with the sweep order, so without this the first substep of a
 // frame would mostly test other pairs' axes.
 constexpr unsigned long long AXIS_EMPTY = ~0ull;
@@ -1622,44 +1622,126 @@ __global__ void k_axis_load(const uint32_t *slots, uint32_t n, Contacts K, const
 // slot (`axis`, world frame, written by k_gjk_pass): bodies move little between substeps, so
 // that axis usually still separates.  Any axis that separates the current poses proves the
 // pair apart, however old or whichever pair it came from, so the cache needs no clearing.
-__global__ void k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B, Contacts K,
-                           HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep) {
+//
+// Work split for the wavefront: every lane first runs the cheap tests of its own pair (the
+// bounding spheres, and each axis against the hulls' local-AABB bound of their extent).  The
+// exact extents of hulls (a maximum over their vertices) then run for the lanes still open,
+// four pairs at a time, one 16-lane row each (each lane a quarter of the vertices, a DPP row
+// maximum): a per-lane vertex loop left ~87 % of the lanes idle.  The tests and their
+// outcome are the same either way (a maximum does not depend on the order it is taken in).
+struct CullPair {
+    uint32_t s;
+    uint2 cc;
+    TF ta, tb;
+    V3 d;
+    float d2, slack;
+};
+__device__ __forceinline__ CullPair cull_pair(const uint32_t *bin_slot, uint32_t off, uint32_t i, const Colliders &C,
+                                              const Bodies &B, const Contacts &K) {
+    CullPair p;
+    p.s = bin_slot[off + i];
+    p.cc = K.cc[p.s];
+    const uint2 bb = K.bb[p.s];
+    p.ta = collider_tf(C, B, p.cc.x, bb.x);
+    p.tb = collider_tf(C, B, p.cc.y, bb.y);
+    p.d = p.tb.pos - p.ta.pos;
+    const float big = fmaxf(fmaxf(fmaxf(fabsf(p.ta.pos.x), fabsf(p.ta.pos.y)), fmaxf(fabsf(p.ta.pos.z), fabsf(p.tb.pos.x))),
+                            fmaxf(fabsf(p.tb.pos.y), fabsf(p.tb.pos.z)));
+    p.slack = 1e-3f + 1e-6f * big;
+    p.d2 = len2(p.d);
+    return p;
+}
+__device__ __forceinline__ float dpp_row_fmax(float v) {
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false)));
+    return v;
+}
+// support_extent of one side, a hull's maximum taken by the 16 lanes of a row (all active).
+__device__ __forceinline__ float row_support_extent(const Colliders &C, const HullSet &hs, uint32_t c, const TF &tf, V3 u) {
+    const int4 m = C.meta[c];
+    if (m.x != SH_HULL) return support_extent(C, hs, c, tf, u);
+    const V3 l = tf_inv_vec(tf, u);
+    const HullMeta hm = hs.meta[(uint32_t)C.hull[c]];
+    float e = -3.4e38f;
+    for (uint32_t k = (uint32_t)(lane64() & 15); k < hm.nv; k += 16) {
+        const float4 v = hs.v[hm.v0 + k];
+        e = fmaxf(e, v.x * l.x + v.y * l.y + v.z * l.z);
+    }
+    return dpp_row_fmax(e);
+}
+__global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B,
+                                                  Contacts K, HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = bin_slot[off + i];
-    const uint2 cc = K.cc[s], bb = K.bb[s];
-    const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
-    const float ra = C.lpos[cc.x].w, rb = C.lpos[cc.y].w;
-    const V3 d = tb.pos - ta.pos;
-    const float big = fmaxf(fmaxf(fmaxf(fabsf(ta.pos.x), fabsf(ta.pos.y)), fmaxf(fabsf(ta.pos.z), fabsf(tb.pos.x))),
-                            fmaxf(fabsf(tb.pos.y), fabsf(tb.pos.z)));
-    const float slack = 1e-3f + 1e-6f * big;
-    const float rs = (ra + rb) * 1.0001f + slack;
-    const float d2 = len2(d);
-    bool apart = d2 > rs * rs;
-    // each axis: first the AABB bound of a hull's extent (no vertex loop), then the exact extent
-    auto apart_along = [&](V3 u, float gap) {
-        for (int exact = 0; exact < 2; ++exact) {
-            const float ext = support_extent(C, hs, cc.x, ta, u, !exact) + support_extent(C, hs, cc.y, tb, -u, !exact);
-            if (ext * 1.0001f + slack < gap) return true;
+    const int lane = lane64(), row = lane >> 4, pl = lane & 15;
+    bool need = false;
+    if (i < n) {
+        const CullPair p = cull_pair(bin_slot, off, i, C, B, K);
+        const float ra = C.lpos[p.cc.x].w, rb = C.lpos[p.cc.y].w;
+        const float rs = (ra + rb) * 1.0001f + p.slack;
+        bool apart = p.d2 > rs * rs;
+        auto bound_apart = [&](V3 u, float gap) {
+            const float ext = support_extent(C, hs, p.cc.x, p.ta, u, true) + support_extent(C, hs, p.cc.y, p.tb, -u, true);
+            return ext * 1.0001f + p.slack < gap;
+        };
+        bool axes = false;
+        if (!apart && axis) {
+            const float4 a = axis[p.s];
+            if (a.w != 0.0f) {
+                const V3 u{a.x, a.y, a.z};
+                apart = bound_apart(u, dot(p.d, u));
+                axes = true;
+            }
         }
-        return false;
-    };
-    if (!apart && axis) {
-        const float4 a = axis[s];
-        if (a.w != 0.0f) {
-            const V3 u{a.x, a.y, a.z};
-            apart = apart_along(u, dot(d, u));
+        if (!apart && p.d2 > 1e-8f) {
+            const float dl = sqrtf(p.d2);
+            apart = bound_apart(vdiv(p.d, dl), dl);
+            axes = true;
+        }
+        // the exact extents can only differ from the bounds for hulls
+        need = !apart && axes && (C.meta[p.cc.x].x == SH_HULL || C.meta[p.cc.y].x == SH_HULL);
+        if (!need) {
+            keep[i] = apart ? 0u : 1u;
+            if (apart) {
+                K.cnt[p.s] = 0;
+                gstat[p.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
+            }
         }
     }
-    if (!apart && d2 > 1e-8f) {
-        const float dl = sqrtf(d2);
-        apart = apart_along(vdiv(d, dl), dl);
-    }
-    keep[i] = apart ? 0u : 1u;
-    if (apart) {
-        K.cnt[s] = 0;
-        gstat[s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
+    // exact extents of the open pairs, a row per pair
+    uint64_t todo = __ballot(need);
+    while (todo) {   // uniform over the wavefront
+        uint64_t m = todo;
+        for (int k = 0; k < row; ++k) m &= m - 1ull;
+        for (int k = 0; k < 4; ++k) todo &= todo - 1ull;
+        if (m == 0ull) continue;   // fewer than four pairs left: this row waits (whole row inactive)
+        const int src = __ffsll((long long)m) - 1;
+        const uint32_t j = blockIdx.x * blockDim.x + (threadIdx.x & ~63u) + (uint32_t)src;   // the open pair's index
+        const CullPair p = cull_pair(bin_slot, off, j, C, B, K);
+        auto exact_apart = [&](V3 u, float gap) {
+            const float ext = row_support_extent(C, hs, p.cc.x, p.ta, u) + row_support_extent(C, hs, p.cc.y, p.tb, -u);
+            return ext * 1.0001f + p.slack < gap;
+        };
+        bool apart = false;
+        if (axis) {
+            const float4 a = axis[p.s];
+            if (a.w != 0.0f) {
+                const V3 u{a.x, a.y, a.z};
+                apart = exact_apart(u, dot(p.d, u));
+            }
+        }
+        if (!apart && p.d2 > 1e-8f) {
+            const float dl = sqrtf(p.d2);
+            apart = exact_apart(vdiv(p.d, dl), dl);
+        }
+        if (pl == 0) {
+            keep[j] = apart ? 0u : 1u;
+            if (apart) {
+                K.cnt[p.s] = 0;
+                gstat[p.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
+            }
+        }
     }
 }
 
@@ -1678,6 +1760,19 @@ __global__ void k_log_advance(uint32_t *cnt, const uint32_t *n_dev, uint32_t n_h
 __global__ void k_gjk_compact(const uint32_t *keep, const uint32_t *pos, uint32_t n, uint32_t *cand) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && keep[i]) cand[pos[i]] = i;
+}
+// The same compaction with a sort key per candidate: the GJK iterations its slot took in the
+// previous substep (capped at 15; a frame's first substep reads the slots' old contents).
+// Sorted by this key, the four pairs of a GJK-pass wavefront need similar iteration counts,
+// so fewer rows idle under the slowest one.  The order of the candidates changes no result.
+__global__ void k_gjk_compact_keyed(const uint32_t *keep, const uint32_t *pos, uint32_t n, const uint32_t *bin_slot,
+                                    uint32_t off, const uint32_t *gstat, uint32_t *cand, uint32_t *key) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && keep[i]) {
+        const uint32_t p = pos[i];
+        cand[p] = i;
+        key[p] = min(gstat[bin_slot[off + i]] & 255u, 15u);
+    }
 }
 
 __global__ void k_gjk_totals(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *gstat,

@@ -152,6 +152,8 @@ struct rbgpu_world {
     int hull_vpl = 1;                   // hull vertices per lane in the wave kernels (1: all hulls <= 64)
     uint32_t *gstat = nullptr;
     uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
+    uint32_t *gjk_ckey = nullptr, *gjk_ckey2 = nullptr, *gjk_cand2 = nullptr;   // candidates sorted by last iterations
+    bool gjk_sort = false;   // measured slower on the combined workload (bin order keeps pairs of a region together)
     float4 *gjk_axis = nullptr;   // per pair slot: the axis that last separated it (rejection hint)
     // the hints carried across a frame's new pair list (table keyed by collider pair)
     unsigned long long *ax_keys = nullptr;
@@ -258,6 +260,7 @@ struct rbgpu_world {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
         dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx); dfree(gjk_keep); dfree(gjk_pos); dfree(gjk_cand);
+        dfree(gjk_ckey); dfree(gjk_ckey2); dfree(gjk_cand2);
         dfree(gjk_ncand); dfree(gjk_axis); dfree(sc_work); dfree(sc_list);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.hd); dfree(K.pt); dfree(K.dep);
@@ -363,8 +366,16 @@ struct rbgpu_world {
             k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep);
             int rc = scan(gjk_keep, gjk_pos, ng, gjk_ncand);
             if (rc) return rc;
-            k_gjk_compact<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
-            cand = gjk_cand;
+            if (gjk_sort) {   // candidates by their previous GJK iteration count (stable)
+                k_gjk_compact_keyed<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, bin_list, g0, gstat, gjk_cand,
+                                                               gjk_ckey);
+                rc = radix_pass(gjk_ckey, gjk_cand, gjk_ckey2, gjk_cand2, ng, 0, p_hist, nullptr, gjk_ncand);
+                if (rc) return rc;
+                cand = gjk_cand2;
+            } else {
+                k_gjk_compact<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
+                cand = gjk_cand;
+            }
             ncand = gjk_ncand;
             if (timed) k_add_count<<<1, 1, 0, st>>>(gjk_ncand, d_points);
         }
@@ -659,6 +670,7 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
         rc |= dalloc(gjk_hits, nh); rc |= dalloc(gjk_blk, nb1);
         rc |= dalloc(gjk_sx, nh * 8);
         rc |= dalloc(gjk_keep, cap); rc |= dalloc(gjk_pos, cap); rc |= dalloc(gjk_cand, cap); rc |= dalloc(gjk_ncand, 1);
+        rc |= dalloc(gjk_ckey, cap); rc |= dalloc(gjk_ckey2, cap); rc |= dalloc(gjk_cand2, cap);
         rc |= dalloc(gjk_axis, cap);
         if (!rc) HIPCHK(hipMemsetAsync(gjk_axis, 0, (size_t)cap * 16, st));   // no hints yet
     }
@@ -1841,6 +1853,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_EPA_GUARD:
             if (value < 0 || value > 2) return fail(RBGPU_ERR_INVALID, "EPA guard is 0, 1 or 2");
             w->epa_guard = value; return RBGPU_OK;
+        case RBGPU_OPT_GJK_SORT:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "GJK sort is 0 or 1");
+            w->gjk_sort = value != 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1874,6 +1889,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_SIDE_STREAM: *value = w->side_stream; return RBGPU_OK;
         case RBGPU_OPT_AXIS_CARRY: *value = w->axis_carry ? 1 : 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_GUARD: *value = w->epa_guard; return RBGPU_OK;
+        case RBGPU_OPT_GJK_SORT: *value = w->gjk_sort ? 1 : 0; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
@@ -2486,7 +2502,7 @@ namespace {
 
 constexpr char SNAP_MAGIC[8] = {'R', 'B', 'G', 'P', 'U', 'S', 'N', 'P'};
 constexpr uint32_t SNAP_VERSION = 1;
-constexpr int SNAP_OPTIONS = RBGPU_OPT_EPA_GUARD + 1;
+constexpr int SNAP_OPTIONS = RBGPU_OPT_GJK_SORT + 1;
 constexpr size_t SNAP_ALIGN = 256;
 
 struct SnapSec {

@@ -256,3 +256,23 @@ def test_gpu_world_pool_equals_worlds_alone():
         ws = w.state()
         for k in ws:
             assert np.array_equal(st[k], ws[k]), k
+
+
+def test_gpu_gjk_candidate_order_keeps_results():
+    """RBGPU_OPT_GJK_SORT (GJK candidates ordered by their previous iteration count) and the
+    GJK layouts change only which wavefront runs which pair: the combined batch's states are
+    bit-identical with the sort on and off, for the row layout and the refilled rows."""
+    lib = S.hull_library()
+    parts = [S.combined_scene(n, seed=90 + k, library=lib) for k, n in enumerate((80, 120))]
+    bt = S.batch(parts)
+    print("case GJK candidate order", len(bt.bodies), "bodies", flush=True)
+    runs = []
+    for sort, layout in ((0, 1), (1, 1), (1, 3)):
+        g = _gpu(bt, contact_mode=1)
+        g.set_option("gjk_sort", sort)
+        g.set_option("gjk_rows", layout)
+        g.step(60)
+        runs.append(g.state())
+    for st in runs[1:]:
+        for k in st:
+            assert np.array_equal(st[k], runs[0][k]), k
