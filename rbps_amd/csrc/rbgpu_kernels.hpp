@@ -157,41 +157,60 @@ __device__ __forceinline__ bool pair_ok(const Colliders &C, uint32_t ca, uint32_
     return a.y != b.y;
 }
 
-// Forward scan from each min endpoint to its own max endpoint: every pair is
-// found once, from the object that opens first (same set as the active-set sweep
-// of BroadPhase.cpp:104-152, inclusive 3-D fat-AABB test AABB.hpp:54-59).
+// Sweep: the range of every broadphase slot, from its min endpoint to its own max
+// endpoint, is scanned for the min endpoints it contains, so every pair is found once,
+// from the object that opens first (same set as the active-set sweep of
+// BroadPhase.cpp:104-152, inclusive 3-D fat-AABB test AABB.hpp:54-59).
+// One 16-lane row (a quarter wavefront) per slot: a step tests the 16 endpoints that
+// follow, loaded together, and the row's ballot ranks the hits, so each slot's pairs
+// are written in endpoint order at the offset its count gave (the count pass, EMIT
+// false, writes the slot's count at its min endpoint and 0 at its max endpoint).
+constexpr int SWEEP_ROW = 16;
 template <bool EMIT>
-__global__ void k_sweep(const uint32_t *sw_meta, const float4 *sw_a, const float2 *sw_b, const uint2 *ep_pos,
-                        const uint32_t *uid, Colliders C, uint32_t nep, uint32_t *cnt, const uint32_t *off,
-                        uint2 *pairs, uint32_t cap, int *overflow) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nep) return;
-    uint32_t v = sw_meta[i];
-    if (v & 1u) { if (!EMIT) cnt[i] = 0; return; }
-    const uint32_t slot = v >> 1;
-    const uint32_t end = ep_pos[slot].y;
+__global__ void __launch_bounds__(256) k_sweep(const uint32_t *sw_meta, const float4 *sw_a, const float2 *sw_b,
+                                               const uint2 *ep_pos, const uint32_t *uid, Colliders C, uint32_t nslot,
+                                               uint32_t *cnt, const uint32_t *off, uint2 *pairs, uint32_t cap,
+                                               int *overflow) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = t / SWEEP_ROW, r = t % SWEEP_ROW;
+    const uint32_t sh = (threadIdx.x & 63u) & ~(uint32_t)(SWEEP_ROW - 1);   // the row's first lane
+    if (s >= nslot) return;   // whole rows: nslot rows of SWEEP_ROW threads
+    const uint2 p = ep_pos[s];
+    const uint32_t i = p.x, end = p.y;
     const float4 A = sw_a[i];
     const float2 Az = sw_b[i];
-    const uint32_t ua = uid[slot];
-    uint32_t n = 0, o = EMIT ? off[i] : 0;
-    for (uint32_t j = i + 1; j < end; ++j) {
-        uint32_t w = sw_meta[j];
-        if (w & 1u) continue;
-        float4 Bv = sw_a[j];
-        float2 Bz = sw_b[j];
-        bool ov = (A.x <= Bv.y && A.y >= Bv.x) && (A.z <= Bv.w && A.w >= Bv.z) && (Az.x <= Bz.y && Az.y >= Bz.x);
-        if (!ov) continue;
-        uint32_t ub = uid[w >> 1];
-        uint32_t lo = ua < ub ? ua : ub, hi = ua < ub ? ub : ua;
-        if (!pair_ok(C, lo, hi)) continue;
-        if (EMIT) {
-            uint32_t k = o + n;
+    const uint32_t ua = uid[s];
+    uint32_t n = 0;
+    const uint32_t o = EMIT ? off[i] : 0u;
+    for (uint32_t base = i + 1; base < end; base += SWEEP_ROW) {   // row-uniform trip count
+        const uint32_t j = base + r;
+        bool hit = false;
+        uint32_t lo = 0, hi = 0;
+        if (j < end) {
+            const uint32_t w = sw_meta[j];
+            if (!(w & 1u)) {
+                const float4 Bv = sw_a[j];
+                const float2 Bz = sw_b[j];
+                if ((A.x <= Bv.y && A.y >= Bv.x) && (A.z <= Bv.w && A.w >= Bv.z) && (Az.x <= Bz.y && Az.y >= Bz.x)) {
+                    const uint32_t ub = uid[w >> 1];
+                    lo = ua < ub ? ua : ub;
+                    hi = ua < ub ? ub : ua;
+                    hit = pair_ok(C, lo, hi);
+                }
+            }
+        }
+        const uint32_t m = (uint32_t)(__ballot(hit) >> sh) & ((1u << SWEEP_ROW) - 1u);
+        if (EMIT && hit) {
+            const uint32_t k = o + n + (uint32_t)__popc(m & ((1u << r) - 1u));
             if (k < cap) pairs[k] = make_uint2(lo, hi);
             else *overflow = 1;
         }
-        ++n;
+        n += (uint32_t)__popc(m);
     }
-    if (!EMIT) cnt[i] = n;
+    if (!EMIT && r == 0) {
+        cnt[i] = n;
+        cnt[end] = 0u;
+    }
 }
 
 __global__ void k_append(uint2 *pairs, const uint32_t *n_sap, const uint2 *extra, uint32_t n_extra, uint32_t cap,

@@ -802,6 +802,7 @@ int rbgpu_world::build() {
         }
     }
     if (slot != S) return fail(RBGPU_ERR_INVALID, "broadphase slot bookkeeping mismatch");
+    if (S > (1u << 26)) return fail(RBGPU_ERR_INVALID, "more than 2^26 broadphase colliders");   // sweep rows: 16 threads each
 
     release();
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -1044,8 +1045,8 @@ int rbgpu_world::frame() {
         }
         k_sweep_prep<<<nblk(NEP), TPB, 0, st>>>(ep_val[0], NEP, fmin, fmax, ep_pos, sw_meta, sw_a, sw_b, ep_key[0], dirty);
         // 3. sweep: count, scan, readback of the pair count
-        k_sweep<false><<<nblk(NEP), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, NEP, ep_cnt, nullptr, nullptr, 0,
-                                                  overflow);
+        k_sweep<false><<<nblk(S * SWEEP_ROW), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, S, ep_cnt, nullptr,
+                                                            nullptr, 0, overflow);
         int rc = scan(ep_cnt, ep_off, NEP, scal + 0);
         if (rc) return rc;
         k_set_int<<<1, 1, 0, st>>>(dirty, 0);
@@ -1083,8 +1084,8 @@ int rbgpu_world::frame() {
     for (int attempt = 0;; ++attempt) {
         HIPCHK(hipMemsetAsync(overflow, 0, 4, st));
         if (NEP)
-            k_sweep<true><<<nblk(NEP), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, NEP, nullptr, ep_off, pairs,
-                                                     pcap, overflow);
+            k_sweep<true><<<nblk(S * SWEEP_ROW), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, S, nullptr, ep_off,
+                                                               pairs, pcap, overflow);
         k_append<<<nblk(std::max<uint32_t>(n_plane, 1)), TPB, 0, st>>>(pairs, scal + 0, plane_pairs, n_plane, pcap,
                                                                         scal + 1, overflow);
         k_pair_gather<<<nblk(pcap), TPB, 0, st>>>(pairs, scal + 1, C, K, ka, va);
