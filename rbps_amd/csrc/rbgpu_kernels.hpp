@@ -134,9 +134,12 @@ __global__ void k_ep_scene_key(const uint32_t *ep_val, const uint32_t *slot_scen
 
 // Sorted sweep rows, endpoint positions and endpoint keys after a sort (the batch sort
 // reorders values without their keys; a key is float_key of its fat-box bound).
-// row (float4): min endpoint -> (x_max, y_min, y_max, z_min) and zmx separately.
+// Row of endpoint j, two float4: (x_min, x_max, y_min, y_max) and (z_min, z_max, filter
+// word 0, filter word 1) of its slot's fat box, the words as bits: collider | static << 31
+// and body | is_max << 31 (everything the sweep's pair filter reads).
 __global__ void k_sweep_prep(const uint32_t *ep_val, uint32_t nep, const float4 *fmin, const float4 *fmax, uint2 *ep_pos,
-                             uint32_t *sw_meta, float4 *sw_a, float2 *sw_b, uint32_t *ep_key, const int *dirty) {
+                             const uint32_t *uid, Colliders C, float4 *sw_a, float4 *sw_b, uint32_t *ep_key,
+                             const int *dirty) {
     if (*dirty == 0) return;
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nep) return;
@@ -144,17 +147,12 @@ __global__ void k_sweep_prep(const uint32_t *ep_val, uint32_t nep, const float4 
     uint32_t slot = v >> 1, is_max = v & 1u;
     if (is_max) ep_pos[slot].y = j; else ep_pos[slot].x = j;
     ep_key[j] = float_key(is_max ? fmax[slot].x : fmin[slot].x);
-    sw_meta[j] = v;
     float4 a = fmin[slot], b = fmax[slot];
+    const uint32_t c = uid[slot];
+    const int4 m = C.meta[c];
+    const uint32_t w0 = c | (m.z != 0 ? 0x80000000u : 0u), w1 = ((uint32_t)m.y & 0x7fffffffu) | (is_max << 31);
     sw_a[j] = make_float4(a.x, b.x, a.y, b.y);
-    sw_b[j] = make_float2(a.z, b.z);
-}
-
-// Pair filter of the narrowphase driver (CollisionPipeline.cpp:121-133).
-__device__ __forceinline__ bool pair_ok(const Colliders &C, uint32_t ca, uint32_t cb) {
-    int4 a = C.meta[ca], b = C.meta[cb];
-    if (a.z != 0 && b.z != 0) return false;
-    return a.y != b.y;
+    sw_b[j] = make_float4(a.z, b.z, __uint_as_float(w0), __uint_as_float(w1));
 }
 
 // Sweep: the range of every broadphase slot, from its min endpoint to its own max
@@ -166,40 +164,40 @@ __device__ __forceinline__ bool pair_ok(const Colliders &C, uint32_t ca, uint32_
 // are written in endpoint order at the offset its count gave (the count pass, EMIT
 // false, writes the slot's count at its min endpoint and 0 at its max endpoint).
 constexpr int SWEEP_ROW = 16;
-template <bool EMIT>
-__global__ void __launch_bounds__(256) k_sweep(const uint32_t *sw_meta, const float4 *sw_a, const float2 *sw_b,
-                                               const uint2 *ep_pos, const uint32_t *uid, Colliders C, uint32_t nslot,
-                                               uint32_t *cnt, const uint32_t *off, uint2 *pairs, uint32_t cap,
-                                               int *overflow) {
+template <bool EMIT, int W = SWEEP_ROW>
+__global__ void __launch_bounds__(256) k_sweep(const float4 *sw_a, const float4 *sw_b, const uint2 *ep_pos,
+                                               uint32_t nslot, uint32_t *cnt, const uint32_t *off, uint2 *pairs,
+                                               uint32_t cap, int *overflow) {
+    static_assert(W == 4 || W == 8 || W == 16 || W == 32, "sweep row width");
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t s = t / SWEEP_ROW, r = t % SWEEP_ROW;
-    const uint32_t sh = (threadIdx.x & 63u) & ~(uint32_t)(SWEEP_ROW - 1);   // the row's first lane
-    if (s >= nslot) return;   // whole rows: nslot rows of SWEEP_ROW threads
+    const uint32_t s = t / W, r = t % W;
+    const uint32_t sh = (threadIdx.x & 63u) & ~(uint32_t)(W - 1);   // the row's first lane
+    if (s >= nslot) return;   // whole rows: nslot rows of W threads
     const uint2 p = ep_pos[s];
     const uint32_t i = p.x, end = p.y;
-    const float4 A = sw_a[i];
-    const float2 Az = sw_b[i];
-    const uint32_t ua = uid[s];
+    const float4 A = sw_a[i], Ab = sw_b[i];
+    const uint32_t a0 = __float_as_uint(Ab.z), a1 = __float_as_uint(Ab.w);
+    const uint32_t ua = a0 & 0x7fffffffu;
     uint32_t n = 0;
     const uint32_t o = EMIT ? off[i] : 0u;
-    for (uint32_t base = i + 1; base < end; base += SWEEP_ROW) {   // row-uniform trip count
+    for (uint32_t base = i + 1; base < end; base += W) {   // row-uniform trip count
         const uint32_t j = base + r;
         bool hit = false;
         uint32_t lo = 0, hi = 0;
         if (j < end) {
-            const uint32_t w = sw_meta[j];
-            if (!(w & 1u)) {
-                const float4 Bv = sw_a[j];
-                const float2 Bz = sw_b[j];
-                if ((A.x <= Bv.y && A.y >= Bv.x) && (A.z <= Bv.w && A.w >= Bv.z) && (Az.x <= Bz.y && Az.y >= Bz.x)) {
-                    const uint32_t ub = uid[w >> 1];
-                    lo = ua < ub ? ua : ub;
-                    hi = ua < ub ? ub : ua;
-                    hit = pair_ok(C, lo, hi);
-                }
+            const float4 Bv = sw_a[j], Bb = sw_b[j];
+            const uint32_t b0 = __float_as_uint(Bb.z), b1 = __float_as_uint(Bb.w);
+            // a min endpoint whose box overlaps; pair filter (CollisionPipeline.cpp:121-133):
+            // not both static, not the same body
+            if (!(b1 >> 31) && (A.x <= Bv.y && A.y >= Bv.x) && (A.z <= Bv.w && A.w >= Bv.z) &&
+                (Ab.x <= Bb.y && Ab.y >= Bb.x) && !((a0 & b0) >> 31) && ((a1 ^ b1) & 0x7fffffffu) != 0u) {
+                const uint32_t ub = b0 & 0x7fffffffu;
+                lo = ua < ub ? ua : ub;
+                hi = ua < ub ? ub : ua;
+                hit = true;
             }
         }
-        const uint32_t m = (uint32_t)(__ballot(hit) >> sh) & ((1u << SWEEP_ROW) - 1u);
+        const uint32_t m = (uint32_t)(__ballot(hit) >> sh) & (uint32_t)((1ull << W) - 1ull);
         if (EMIT && hit) {
             const uint32_t k = o + n + (uint32_t)__popc(m & ((1u << r) - 1u));
             if (k < cap) pairs[k] = make_uint2(lo, hi);

@@ -213,9 +213,8 @@ struct rbgpu_world {
     uint32_t *uid = nullptr;
     uint32_t *ep_key[2] = {nullptr, nullptr}, *ep_val[2] = {nullptr, nullptr};
     uint2 *ep_pos = nullptr;
-    uint32_t *sw_meta = nullptr;
     float4 *sw_a = nullptr;
-    float2 *sw_b = nullptr;
+    float4 *sw_b = nullptr;
     uint32_t *ep_cnt = nullptr, *ep_off = nullptr;
     uint32_t *ep_hist = nullptr;
     int *flags = nullptr;            // [0] dirty [1] overflow
@@ -283,7 +282,7 @@ struct rbgpu_world {
         dfree(Jt.jlam); dfree(jorder); dfree(sj_order); dfree(sj_base); dfree(sj_coff);
         sj_njc = 0;
         dfree(fmin); dfree(fmax); dfree(uid); dfree(ep_key[0]); dfree(ep_key[1]); dfree(ep_val[0]); dfree(ep_val[1]);
-        dfree(ep_pos); dfree(sw_meta); dfree(sw_a); dfree(sw_b); dfree(ep_cnt); dfree(ep_off); dfree(ep_hist);
+        dfree(ep_pos); dfree(sw_a); dfree(sw_b); dfree(ep_cnt); dfree(ep_off); dfree(ep_hist);
         dfree(flags); dfree(scal); dfree(tile_sums);
         dfree(cb.used); dfree(cb.ucnt); dfree(cb.claim); dfree(cb.hist); dfree(cb.off); dfree(cb.cur); dfree(cb.ctr);
         dfree(plane_pairs); dfree(offs); dfree(d_points);
@@ -819,7 +818,7 @@ int rbgpu_world::build() {
     rc |= dalloc(hmeta, hmv.size()); rc |= dalloc(hamin, hminv.size()); rc |= dalloc(hamax, hmaxv.size());
     rc |= dalloc(fmin, S); rc |= dalloc(fmax, S); rc |= dalloc(uid, S);
     rc |= dalloc(ep_key[0], NEP); rc |= dalloc(ep_key[1], NEP); rc |= dalloc(ep_val[0], NEP); rc |= dalloc(ep_val[1], NEP);
-    rc |= dalloc(ep_pos, S); rc |= dalloc(sw_meta, NEP); rc |= dalloc(sw_a, NEP); rc |= dalloc(sw_b, NEP);
+    rc |= dalloc(ep_pos, S); rc |= dalloc(sw_a, NEP); rc |= dalloc(sw_b, NEP);
     rc |= dalloc(ep_cnt, NEP + 1); rc |= dalloc(ep_off, NEP + 1);
     rc |= dalloc(ep_hist, (size_t)((NEP + RS_TILE - 1) / RS_TILE) * RS_BINS);
     rc |= dalloc(flags, 4); rc |= dalloc(scal, 8);
@@ -1043,10 +1042,11 @@ int rbgpu_world::frame() {
             if (!rc) rc = radix_pass(ep_key[1], ep_val[1], ep_cnt, ep_val[0], NEP, 8, ep_hist, dirty);
             if (rc) return rc;
         }
-        k_sweep_prep<<<nblk(NEP), TPB, 0, st>>>(ep_val[0], NEP, fmin, fmax, ep_pos, sw_meta, sw_a, sw_b, ep_key[0], dirty);
+        k_sweep_prep<<<nblk(NEP), TPB, 0, st>>>(ep_val[0], NEP, fmin, fmax, ep_pos, uid, C, sw_a, sw_b, ep_key[0],
+                                                dirty);
         // 3. sweep: count, scan, readback of the pair count
-        k_sweep<false><<<nblk(S * SWEEP_ROW), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, S, ep_cnt, nullptr,
-                                                            nullptr, 0, overflow);
+        k_sweep<false><<<nblk(S * SWEEP_ROW), TPB, 0, st>>>(sw_a, sw_b, ep_pos, S, ep_cnt, nullptr, nullptr, 0,
+                                                            overflow);
         int rc = scan(ep_cnt, ep_off, NEP, scal + 0);
         if (rc) return rc;
         k_set_int<<<1, 1, 0, st>>>(dirty, 0);
@@ -1084,8 +1084,8 @@ int rbgpu_world::frame() {
     for (int attempt = 0;; ++attempt) {
         HIPCHK(hipMemsetAsync(overflow, 0, 4, st));
         if (NEP)
-            k_sweep<true><<<nblk(S * SWEEP_ROW), TPB, 0, st>>>(sw_meta, sw_a, sw_b, ep_pos, uid, C, S, nullptr, ep_off,
-                                                               pairs, pcap, overflow);
+            k_sweep<true><<<nblk(S * SWEEP_ROW), TPB, 0, st>>>(sw_a, sw_b, ep_pos, S, nullptr, ep_off, pairs, pcap,
+                                                               overflow);
         k_append<<<nblk(std::max<uint32_t>(n_plane, 1)), TPB, 0, st>>>(pairs, scal + 0, plane_pairs, n_plane, pcap,
                                                                         scal + 1, overflow);
         k_pair_gather<<<nblk(pcap), TPB, 0, st>>>(pairs, scal + 1, C, K, ka, va);
