@@ -265,9 +265,15 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *   RBGPU_OPT_GJK_CULL     exact bounding-sphere rejection in front of GJK (1, default; 0 runs
  *                          GJK on every pair of the hull bins)
  *   RBGPU_OPT_GJK_GRID, RBGPU_OPT_EPA_GRID  workgroups of the two GJK/EPA passes (4096, 8192)
- *   RBGPU_OPT_GJK_ROWS     layout of the GJK pass: 1 (default) four pairs per wavefront, one per
- *                          16-lane DPP row, when every hull has at most 64 vertices; 0 one pair per
- *                          wavefront; 2 one pair per lane (each lane scans its hulls' vertices)
+ *   RBGPU_OPT_GJK_ROWS     layout of the GJK pass when every hull has at most 64 vertices: 4
+ *                          (default) GJK, EPA and the manifold in one pass, four pairs per
+ *                          wavefront, one per 16-lane DPP row, polytope and clip buffers per row in
+ *                          LDS (a pair that outgrows them is finished by the full-size EPA pass);
+ *                          5 the same with row buffers most pairs outgrow (tests); 1 GJK alone per
+ *                          row, EPA in a second pass (one pair per wavefront); 0 one pair per
+ *                          wavefront; 2 one pair per lane (each lane scans its hulls' vertices); 3
+ *                          rows refilled one candidate at a time.  Larger hulls: one pair per
+ *                          wavefront.  Every layout gives the same results
  *   RBGPU_OPT_EPA_SUB      listed pairs per EPA work item (1, 2, 4, 8, 16; default 16)
  *   RBGPU_OPT_GJK_OCCUPANCY  waves per EU the row GJK pass budgets its registers for (2, 3, 4)
  *   RBGPU_OPT_EPA_OCCUPANCY, RBGPU_OPT_SOLVE_OCCUPANCY  the same for the EPA pass and the
@@ -286,6 +292,11 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          numbers of iterations; 0 (default): bin order, which measured faster
  *                          (288 vs 310 us per GJK pass on the combined workload).  Results do not
  *                          change.
+ *   RBGPU_OPT_CULL_MOTION  1: within a frame, a pair the rejection tests proved apart
+ *                          stays rejected without testing again while the margin its test left
+ *                          exceeds how far its colliders can have moved since the frame started
+ *                          (|x - x0| + 2 (|offset| + radius) |q - q0|); 0 (default) tests every substep.
+ *                          Results do not change.
  *   RBGPU_OPT_EPA_GUARD    0 (default): EPA depths and normals as the reference computes them.
  *                          Any direction n bounds the penetration depth from above by the
  *                          shapes' overlap along n (the Minkowski difference's support along
@@ -329,7 +340,8 @@ enum {
     RBGPU_OPT_SIDE_STREAM = 19,
     RBGPU_OPT_AXIS_CARRY = 20,
     RBGPU_OPT_EPA_GUARD = 21,
-    RBGPU_OPT_GJK_SORT = 22
+    RBGPU_OPT_GJK_SORT = 22,
+    RBGPU_OPT_CULL_MOTION = 23
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);
@@ -424,7 +436,10 @@ int rbgpu_collide_world_pairs(rbgpu_world *w, uint32_t n, const uint32_t *pairs2
  *   rbgpu_mlp_gradients   gradient of the mean loss on given features and targets
  *                         (n x 8: GJK target xyz, valid, EPA target xyz, valid)
  *   rbgpu_get_gjk_totals  since the last reset: GJK pairs, GJK iterations, EPA
- *                         iterations, overlapping pairs, EPA converged, fallbacks */
+ *                         iterations, overlapping pairs, EPA converged, fallbacks
+ *   rbgpu_get_gjk_counts  the same n <= RBGPU_GJK_COUNTS counters, the seventh the pairs the
+ *                         GJK + EPA row pass (RBGPU_OPT_GJK_ROWS 4, 5) handed to the full-size
+ *                         EPA pass (no reference counterpart: instrumentation) */
 int rbgpu_mlp_init(rbgpu_world *w, uint64_t seed);
 int rbgpu_mlp_set_weights(rbgpu_world *w, const float *params, uint32_t n);
 int rbgpu_mlp_get_weights(rbgpu_world *w, float *params, uint32_t n);
@@ -438,6 +453,8 @@ int rbgpu_mlp_eval(rbgpu_world *w, uint32_t n, const float *features32, float *o
 int rbgpu_mlp_gradients(rbgpu_world *w, uint32_t n, const float *features32, const float *targets8, float *grad,
                         double *loss);
 int rbgpu_get_gjk_totals(rbgpu_world *w, uint64_t *out6, int reset);
+#define RBGPU_GJK_COUNTS 7
+int rbgpu_get_gjk_counts(rbgpu_world *w, uint64_t *out, uint32_t n, int reset);
 
 /* Device-resident snapshot and restore of a world's simulation state (the reference's
  * WorldSnapshot design, docs/design/world-snapshot-restore/world-snapshot-restore.md:29-45,

@@ -36,6 +36,9 @@ constexpr int HP_MAX_POINTS = 32;                 // hull-plane candidates (Plan
 
 enum { GJK_VALID = 0, GJK_INSIDE = 1, GJK_FAILED = 2 };
 enum { EPA_VALID = 0, EPA_FAILED = 1, EPA_OUT_OF_FACES = 2, EPA_OUT_OF_VERTICES = 3, EPA_NONLINEAR = 4 };
+// Not a reference status: a polytope or clip buffer smaller than the reference's limits ran
+// out; the pair is run again with the full ones (never stored).
+constexpr int EPA_DEFER = 14;
 
 __device__ __forceinline__ int lane64() { return (int)(threadIdx.x & 63); }
 
@@ -63,6 +66,58 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+__device__ __forceinline__ uint32_t dpp_row_max(uint32_t v);
+__device__ __forceinline__ uint32_t dpp_row_min(uint32_t v);
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v);
+
+// The lanes that hold one pair: LPP = 64 (a pair per wavefront, one-wave workgroups) or 16
+// (a pair per DPP row, four per wavefront).  Ballots come back shifted to the group's first
+// lane; reductions stay inside the group.  Every lane of a group must be active at each call
+// (the group's control flow is uniform); other groups of the wavefront may be inactive.
+template <int LPP> struct Grp {
+    static_assert(LPP == 16 || LPP == 64, "a pair spans a DPP row or a wavefront");
+    __device__ static __forceinline__ int lane() { return lane64() & (LPP - 1); }
+    __device__ static __forceinline__ uint64_t ballot(bool v) {
+        const uint64_t m = __ballot(v);
+        if constexpr (LPP == 64) return m;
+        else return (m >> (lane64() & ~(LPP - 1))) & ((1ull << LPP) - 1ull);
+    }
+    // set bits of a group ballot below this lane
+    __device__ static __forceinline__ uint32_t rank(uint64_t m) {
+        if constexpr (LPP == 64) return lane_rank(m);
+        else return (uint32_t)__popcll(m & ((1ull << lane()) - 1ull));
+    }
+    __device__ static __forceinline__ uint32_t min_u32(uint32_t v) {
+        if constexpr (LPP == 64) return wave_min_u32(v);
+        else return dpp_row_min(v);
+    }
+    __device__ static __forceinline__ uint32_t sum(uint32_t v) {
+#pragma unroll
+        for (int o = LPP / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    }
+    // arg-max over (value, index): larger value, then lower index
+    __device__ static __forceinline__ void argmax(float &d, int &i) {
+#pragma unroll
+        for (int o = LPP / 2; o > 0; o >>= 1) {
+            const float od = __shfl_xor(d, o, 64);
+            const int oi = __shfl_xor(i, o, 64);
+            if (od > d || (od == d && oi < i)) { d = od; i = oi; }
+        }
+    }
+    // Orders the group's LDS accesses: a wavefront runs its LDS instructions in order, so
+    // within one wavefront a compiler barrier suffices (a workgroup barrier there would need
+    // every group of the workgroup to arrive).
+    __device__ static __forceinline__ void sync() {
+        if constexpr (LPP == 64) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+};
 
 // ------------------------------------------------------------------ shapes in a wave
 template <int VPL, int LPP = 64> struct WaveHull {   // this lane's vertices r + LPP k (r: lane within the pair's LPP lanes)
@@ -503,33 +558,40 @@ template <class SideT> __device__ __forceinline__ int gjk(const SideT &A, const 
 }
 
 // ------------------------------------------------------------------ EPA (EPA.cpp:141-378)
-struct EpaLds {
-    float wx[EPA_VERTS], wy[EPA_VERTS], wz[EPA_VERTS];     // polytope vertices (w)
-    float ax[EPA_VERTS], ay[EPA_VERTS], az[EPA_VERTS];     // their A-side supports (w0)
-    float nx[EPA_FACES], ny[EPA_FACES], nz[EPA_FACES], nd[EPA_FACES];
-    uint32_t fv[EPA_FACES];                                // v0 | v1 << 8 | v2 << 16 | obsolete << 24
-    uint16_t vis[EPA_FACES];                               // visible faces, face order
-    uint16_t seq[EPA_SEQ];                                 // their directed edges (a | b << 8)
-    uint16_t hor[EPA_SEQ];                                 // horizon edges, in order
+// Polytope of NV vertices and NF faces.  The reference's limits (EPA_VERTS, EPA_FACES) give
+// its OUT_OF_VERTICES / OUT_OF_FACES statuses; a smaller polytope ends with EPA_DEFER where
+// the reference's would still grow, and the pair is run again with the full one.  Up to that
+// point the two runs are the same instructions on the same data.
+template <int NV, int NF> struct EpaLdsT {
+    float wx[NV], wy[NV], wz[NV];     // polytope vertices (w)
+    float ax[NV], ay[NV], az[NV];     // their A-side supports (w0)
+    float nx[NF], ny[NF], nz[NF], nd[NF];
+    uint32_t fv[NF];                  // v0 | v1 << 8 | v2 << 16 | obsolete << 24
+    uint16_t vis[NF];                 // visible faces, face order
+    uint16_t seq[3 * NF];             // their directed edges (a | b << 8)
+    uint16_t hor[3 * NF];             // horizon edges, in order
 };
-// Manifold buffers, overlaid on the EPA polytope once EPA is done.
-struct ClipLds {
-    float px[2][HCLIP_CAP], py[2][HCLIP_CAP], pz[2][HCLIP_CAP];
-    float qx[HULL_MAX_POLY], qy[HULL_MAX_POLY], qz[HULL_MAX_POLY];   // clip planes: point
-    float mx[HULL_MAX_POLY], my[HULL_MAX_POLY], mz[HULL_MAX_POLY];   // and normal
-    float dep[HCLIP_CAP];
+using EpaLds = EpaLdsT<EPA_VERTS, EPA_FACES>;
+// Manifold buffers, overlaid on the EPA polytope once EPA is done: two clip buffers of HC
+// points and the kept points' depths.
+template <int HC> struct ClipLdsT {
+    float px[2][HC], py[2][HC], pz[2][HC];
+    float dep[HC];
 };
-union PairLds {
-    EpaLds e;
-    ClipLds c;
+using ClipLds = ClipLdsT<HCLIP_CAP>;
+template <int NV, int NF, int HC> union PairLdsT {
+    EpaLdsT<NV, NF> e;
+    ClipLdsT<HC> c;
 };
+using PairLds = PairLdsT<EPA_VERTS, EPA_FACES, HCLIP_CAP>;
 
-__device__ __forceinline__ V3 lds_w(const EpaLds &L, int i) { return V3{L.wx[i], L.wy[i], L.wz[i]}; }
-__device__ __forceinline__ V3 lds_a(const EpaLds &L, int i) { return V3{L.ax[i], L.ay[i], L.az[i]}; }
+template <class L> __device__ __forceinline__ V3 lds_w(const L &l, int i) { return V3{l.wx[i], l.wy[i], l.wz[i]}; }
+template <class L> __device__ __forceinline__ V3 lds_a(const L &l, int i) { return V3{l.ax[i], l.ay[i], l.az[i]}; }
 
 // compute_face_normal (EPA.cpp:37-87) over the first nall polytope vertices.
-__device__ __forceinline__ bool epa_face_normal(const EpaLds &L, int i0, int i1, int i2, int nall, V3 &n_out, float &d_out) {
-    const V3 a = lds_w(L, i0), b = lds_w(L, i1), c = lds_w(L, i2);
+template <class L> __device__ __forceinline__ bool epa_face_normal(const L &l, int i0, int i1, int i2, int nall, V3 &n_out,
+                                                                    float &d_out) {
+    const V3 a = lds_w(l, i0), b = lds_w(l, i1), c = lds_w(l, i2);
     V3 n = cross(b - a, c - a);
     const float nl = len(n);
     if (nl < EPS) return false;
@@ -541,7 +603,7 @@ __device__ __forceinline__ bool epa_face_normal(const EpaLds &L, int i0, int i1,
     } else if (d <= 1e-12f) {
         bool fixed = false;
         for (int i = 0; i < nall; ++i) {
-            const float t = dot(n, lds_w(L, i));
+            const float t = dot(n, lds_w(l, i));
             if (t > 1e-12f) { n = -n; d = 0.0f; fixed = true; break; }
             if (t < -1e-12f) { d = 0.0f; fixed = true; break; }
         }
@@ -554,9 +616,10 @@ __device__ __forceinline__ bool epa_face_normal(const EpaLds &L, int i0, int i1,
 
 // Adds w to the polytope (EPA.cpp:337-374): faces visible from w become obsolete and
 // the horizon of the visible region is stitched to w.  Returns EPA_OUT_OF_FACES when
-// the face pool runs out, else 0.
-__device__ __forceinline__ int epa_expand(EpaLds &L, const SV &w, int &nvtx, int &nf) {
-    const int lane = lane64();
+// the face pool runs out (EPA_DEFER for a pool smaller than the reference's), else 0.
+template <int LPP, int NV, int NF> __device__ __forceinline__ int epa_expand(EpaLdsT<NV, NF> &L, const SV &w, int &nvtx, int &nf) {
+    using G = Grp<LPP>;
+    const int lane = G::lane();
     const int wi = nvtx;
     if (lane == 0) {
         L.wx[wi] = w.w.x; L.wy[wi] = w.w.y; L.wz[wi] = w.w.z;
@@ -565,7 +628,7 @@ __device__ __forceinline__ int epa_expand(EpaLds &L, const SV &w, int &nvtx, int
     ++nvtx;
     // 4. visible faces, listed in face order
     int nvis = 0;
-    for (int base = 0; base < nf; base += 64) {
+    for (int base = 0; base < nf; base += LPP) {
         const int f = base + lane;
         bool v = false;
         if (f < nf && !(L.fv[f] >> 24)) {
@@ -575,55 +638,52 @@ __device__ __forceinline__ int epa_expand(EpaLds &L, const SV &w, int &nvtx, int
             const V3 fn{L.nx[f], L.ny[f], L.nz[f]};
             v = dot(fn, w.w - cen) > 0.0f;
         }
-        const uint64_t m = __ballot(v);
+        const uint64_t m = G::ballot(v);
         if (v) {
             L.fv[f] |= 1u << 24;
-            L.vis[nvis + (int)lane_rank(m)] = (uint16_t)f;
+            L.vis[nvis + (int)G::rank(m)] = (uint16_t)f;
         }
         nvis += __popcll(m);
     }
-    __syncthreads();
+    G::sync();
     // directed edges (v0,v1) (v1,v2) (v2,v0) of each visible face
     const int ns = 3 * nvis;
-    for (int i = lane; i < ns; i += 64) {
+    for (int i = lane; i < ns; i += LPP) {
         const uint32_t fv = L.fv[L.vis[i / 3]];
         const int k = i % 3;
         const uint32_t a = (fv >> (8 * k)) & 255u, b = (fv >> (8 * ((k + 1) % 3))) & 255u;
         L.seq[i] = (uint16_t)(a | (b << 8));
     }
-    __syncthreads();
-    // horizon: odd occurrence count, kept at the last occurrence
+    G::sync();
+    // horizon: odd occurrence count, kept at the last occurrence.  Every lane scans the whole
+    // sequence (no early exit: the reads do not wait for one another)
+    auto ukey = [](uint32_t e) {
+        const uint32_t a = e & 255u, b = e >> 8;
+        return a < b ? (a | (b << 8)) : (b | (a << 8));
+    };
     int nh = 0;
-    for (int base = 0; base < ns; base += 64) {
+    for (int base = 0; base < ns; base += LPP) {
         const int i = base + lane;
         bool keep = false;
         if (i < ns) {
-            const uint32_t e = L.seq[i];
-            const uint32_t a = e & 255u, b = e >> 8;
-            const uint32_t key = a < b ? (a | (b << 8)) : (b | (a << 8));
-            bool later = false;
-            for (int j = i + 1; j < ns && !later; ++j) {
-                const uint32_t f = L.seq[j];
-                const uint32_t fa = f & 255u, fb = f >> 8;
-                later = (fa < fb ? (fa | (fb << 8)) : (fb | (fa << 8))) == key;
+            const uint32_t key = ukey(L.seq[i]);
+            int later = 0, before = 0;
+#pragma unroll 8
+            for (int j = 0; j < ns; ++j) {
+                const bool same = ukey(L.seq[j]) == key;
+                later += (int)(same & (j > i));
+                before += (int)(same & (j < i));
             }
-            int before = 0;
-            if (!later)
-                for (int j = 0; j < i; ++j) {
-                    const uint32_t f = L.seq[j];
-                    const uint32_t fa = f & 255u, fb = f >> 8;
-                    before += (fa < fb ? (fa | (fb << 8)) : (fb | (fa << 8))) == key;
-                }
-            keep = !later && (before & 1) == 0;
+            keep = later == 0 && (before & 1) == 0;
         }
-        const uint64_t m = __ballot(keep);
-        if (keep) L.hor[nh + (int)lane_rank(m)] = L.seq[i];
+        const uint64_t m = G::ballot(keep);
+        if (keep) L.hor[nh + (int)G::rank(m)] = L.seq[i];
         nh += __popcll(m);
     }
-    __syncthreads();
+    G::sync();
     // stitch: one candidate face per horizon edge, packed in horizon order
     int made_before_last = 0, made = 0;
-    for (int base = 0; base < nh; base += 64) {
+    for (int base = 0; base < nh; base += LPP) {
         const int k = base + lane;
         bool ok = false;
         V3 n{0.0f, 0.0f, 0.0f};
@@ -633,19 +693,20 @@ __device__ __forceinline__ int epa_expand(EpaLds &L, const SV &w, int &nvtx, int
             e = L.hor[k];
             ok = epa_face_normal(L, (int)(e & 255u), (int)(e >> 8), wi, nvtx, n, d);
         }
-        const uint64_t m = __ballot(ok);
-        const int slot = nf + made + (int)lane_rank(m);
+        const uint64_t m = G::ballot(ok);
+        const int slot = nf + made + (int)G::rank(m);
         if (ok && k != nh - 1) ++made_before_last;   // counted per lane, summed below
-        if (ok && slot < EPA_FACES) {
+        if (ok && slot < NF) {
             L.nx[slot] = n.x; L.ny[slot] = n.y; L.nz[slot] = n.z; L.nd[slot] = d;
             L.fv[slot] = e | ((uint32_t)wi << 16);
         }
         made += __popcll(m);
     }
     // the reference checks the pool before each new face, failing ones included
-    if (nh > 0 && nf + (int)wave_sum((uint32_t)made_before_last) >= EPA_FACES) return EPA_OUT_OF_FACES;
+    if (nh > 0 && nf + (int)G::sum((uint32_t)made_before_last) >= NF)
+        return NF < EPA_FACES ? EPA_DEFER : EPA_OUT_OF_FACES;
     nf += made;
-    __syncthreads();
+    G::sync();
     return 0;
 }
 
@@ -672,10 +733,13 @@ struct EpaResult {
     float depth;
 };
 
-template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed, const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs,
-                                         EpaLds &L, bool seeded, V3 seed_n) {
+template <class SideT, int NV, int NF>
+__device__ __forceinline__ EpaResult epa(const Simplex &seed, const SideT &A, const SideT &Bs, const HullSet &hs,
+                                         EpaLdsT<NV, NF> &L, bool seeded, V3 seed_n) {
+    constexpr int LPP = SideT::kLanes;
+    using G = Grp<LPP>;
     EpaResult r{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
-    const int lane = lane64();
+    const int lane = G::lane();
     if (seed.rank != 4) { r.status = EPA_NONLINEAR; return r; }
     {   // coplanar seed: a touching contact (EPA.cpp:174-212)
         const V3 va = seed.v[0].w, vb = seed.v[1].w, vc = seed.v[2].w, vd = seed.v[3].w;
@@ -709,7 +773,7 @@ template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed,
             L.ax[k] = s.w0.x; L.ay[k] = s.w0.y; L.az[k] = s.w0.z;
         }
     }
-    __syncthreads();
+    G::sync();
     int nvtx = 4, nf = 0;
     {   // seed faces ABC, ACD, ADB, BCD (every lane evaluates them alike)
         const int fa[4] = {0, 0, 0, 1}, fb[4] = {1, 2, 3, 2}, fc[4] = {2, 3, 1, 3};
@@ -724,7 +788,7 @@ template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed,
         }
         nf = 4;
     }
-    __syncthreads();
+    G::sync();
     int it0 = 0;
     if (seeded) {
         // warm start: one expansion along a predicted normal, when its support is a new
@@ -739,10 +803,13 @@ template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed,
                             lds_w(L, (int)((fv >> 16) & 255u))) * (1.0f / 3.0f);
             seen = dot(V3{L.nx[lane], L.ny[lane], L.nz[lane]}, w.w - cen) > 0.0f;
         }
-        if (!__ballot(dup) && __ballot(seen) && epa_expand(L, w, nvtx, nf)) {
-            r.iters = 1;
-            r.status = EPA_OUT_OF_FACES;
-            return r;
+        if (!G::ballot(dup) && G::ballot(seen)) {
+            const int rc = epa_expand<LPP>(L, w, nvtx, nf);
+            if (rc) {
+                r.iters = 1;
+                r.status = rc;
+                return r;
+            }
         }
     }
     for (int it = 0; it < EPA_ITERS; ++it) {
@@ -752,16 +819,16 @@ template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed,
         // FLT_MAX (and NaN) never qualify, as in the reference's scan from DBL_MAX
         uint32_t best = 0xffffffffu;
         int bi = 0x7fffffff;
-        for (int f = lane; f < nf; f += 64) {
+        for (int f = lane; f < nf; f += LPP) {
             const float d = L.nd[f];
             if (!(L.fv[f] >> 24) && d < 3.402823466e+38f) {
                 const uint32_t o = ord_f(d);
                 if (o < best) { best = o; bi = f; }
             }
         }
-        const uint32_t mo = wave_min_u32(best);
+        const uint32_t mo = G::min_u32(best);
         if (mo == 0xffffffffu) { r.status = EPA_FAILED; return r; }
-        const int ci = (int)wave_min_u32(best == mo ? (uint32_t)bi : 0xffffffffu);
+        const int ci = (int)G::min_u32(best == mo ? (uint32_t)bi : 0xffffffffu);
         const float mind = L.nd[ci];
         const V3 cn{L.nx[ci], L.ny[ci], L.nz[ci]};
         const uint32_t cv = L.fv[ci];
@@ -790,30 +857,31 @@ template <int VPL> __device__ __forceinline__ EpaResult epa(const Simplex &seed,
         }
         // duplicate support: stop on the closest face
         bool dup = false;
-        for (int i = lane; i < nvtx; i += 64) dup |= len2(lds_w(L, i) - w.w) < EPS * EPS;
-        if (__ballot(dup)) {
+        for (int i = lane; i < nvtx; i += LPP) dup |= len2(lds_w(L, i) - w.w) < EPS * EPS;
+        if (G::ballot(dup)) {
             r.n = cn;
             r.depth = mind;
             r.p = lds_a(L, c0);
             r.status = EPA_VALID;
             return r;
         }
-        if (nvtx >= EPA_VERTS) { r.status = EPA_OUT_OF_VERTICES; return r; }
-        if (epa_expand(L, w, nvtx, nf)) { r.status = EPA_OUT_OF_FACES; return r; }
+        if (nvtx >= NV) { r.status = NV < EPA_VERTS ? EPA_DEFER : EPA_OUT_OF_VERTICES; return r; }
+        if (const int rc = epa_expand<LPP>(L, w, nvtx, nf)) { r.status = rc; return r; }
     }
     r.status = EPA_FAILED;
     return r;
 }
 
 // ------------------------------------------------------------------ manifolds
-struct HullTabs {   // one hull's tables
-    const float4 *v, *pn;
-    const uint32_t *poff, *pidx, *v2p_o, *v2p_i, *v2n_o, *v2n_i, *p2p_o, *p2p_i;
+// One hull's tables as offsets into the world's arrays (vertices, polygon normals, the u32
+// table): 32-bit offsets rather than pointers keep the per-pair copies (two hulls, and the
+// reference/incident pick) at half the registers.
+struct HullTabs {
+    uint32_t v, pn, poff, pidx, v2p_o, v2p_i, v2n_o, v2n_i, p2p_o, p2p_i;
 };
 __device__ __forceinline__ HullTabs hull_tabs(const HullSet &hs, uint32_t h) {
     const HullMeta m = hs.meta[h];
-    return HullTabs{hs.v + m.v0, hs.pn + m.pn0, hs.tab + m.poff, hs.tab + m.pidx, hs.tab + m.v2p_o,
-                    hs.tab + m.v2p_i, hs.tab + m.v2n_o, hs.tab + m.v2n_i, hs.tab + m.p2p_o, hs.tab + m.p2p_i};
+    return HullTabs{m.v0, m.pn0, m.poff, m.pidx, m.v2p_o, m.v2p_i, m.v2n_o, m.v2n_i, m.p2p_o, m.p2p_i};
 }
 __device__ __forceinline__ V3 tv(const float4 *p, uint32_t i) {
     const float4 q = p[i];
@@ -821,13 +889,14 @@ __device__ __forceinline__ V3 tv(const float4 *p, uint32_t i) {
 }
 
 // Polygon at a support vertex most aligned with wd (ContactManifoldGenerator.cpp:73-95).
-__device__ __forceinline__ uint32_t best_polygon(const HullTabs &h, const TF &tf, uint32_t sv, V3 wd, float &al) {
-    const uint32_t s = h.v2p_o[sv], e = h.v2p_o[sv + 1];
-    uint32_t best = (s < e) ? h.v2p_i[s] : 0u;
+__device__ __forceinline__ uint32_t best_polygon(const HullSet &hs, const HullTabs &h, const TF &tf, uint32_t sv, V3 wd,
+                                                 float &al) {
+    const uint32_t s = hs.tab[h.v2p_o + sv], e = hs.tab[h.v2p_o + sv + 1];
+    uint32_t best = (s < e) ? hs.tab[h.v2p_i + s] : 0u;
     float bd = -3.402823466e+38f;
     for (uint32_t i = s; i < e; ++i) {
-        const uint32_t p = h.v2p_i[i];
-        const float x = dot(tf_vec(tf, tv(h.pn, p)), wd);
+        const uint32_t p = hs.tab[h.v2p_i + i];
+        const float x = dot(tf_vec(tf, tv(hs.pn + h.pn, p)), wd);
         if (x > bd) { bd = x; best = p; }
     }
     al = bd;
@@ -850,12 +919,15 @@ __device__ __forceinline__ bool clip_edge(V3 pp, V3 pn, V3 s, V3 e, V3 &out) {
 }
 // Sutherland-Hodgman of buffer `src` (n points) against one plane into buffer 1 - src.
 // A convex n-gon gains at most one point per plane, so HCLIP_CAP never binds; the bound
-// only keeps a numerically non-convex input inside the buffer.
-__device__ __forceinline__ int clip_pass(ClipLds &C, int src, int n, V3 pp, V3 pn, bool remove_only) {
+// only keeps a numerically non-convex input inside the buffer.  A smaller buffer (HC below
+// HCLIP_CAP) sets `full` where the reference-sized one would take the point.
+template <int HC>
+__device__ __forceinline__ int clip_pass(ClipLdsT<HC> &C, int src, int n, V3 pp, V3 pn, bool remove_only, bool &full) {
     const int dst = 1 - src;
     int m = 0;
     auto put = [&](V3 p) {
-        if (m < HCLIP_CAP) { C.px[dst][m] = p.x; C.py[dst][m] = p.y; C.pz[dst][m] = p.z; ++m; }
+        if (m < HC) { C.px[dst][m] = p.x; C.py[dst][m] = p.y; C.pz[dst][m] = p.z; ++m; }
+        else if (HC < HCLIP_CAP) full = true;
     };
     V3 sp{C.px[src][n - 1], C.py[src][n - 1], C.pz[src][n - 1]};
     for (int j = 0; j < n; ++j) {
@@ -893,7 +965,7 @@ __device__ __forceinline__ bool skew_lines(V3 p1, V3 d1, V3 p2, V3 d2, V3 &l1, V
     return true;
 }
 // ContactManifoldGenerator.hpp:88-154 over the kept points in C.px[buf]/C.dep (lane 0).
-__device__ __forceinline__ int reduce4_area(const ClipLds &C, int buf, int n, Manifold &m) {
+template <int HC> __device__ __forceinline__ int reduce4_area(const ClipLdsT<HC> &C, int buf, int n, Manifold &m) {
     auto P = [&](int i) { return V3{C.px[buf][i], C.py[buf][i], C.pz[buf][i]}; };
     if (n <= 4) {
 #pragma unroll
@@ -938,10 +1010,14 @@ __device__ __forceinline__ int reduce4_area(const ClipLds &C, int buf, int n, Ma
 }
 
 // Hull-hull manifold from the EPA result (ContactManifoldGenerator.cpp:176-379).  The
-// result is valid on lane 0.
-template <int VPL> __device__ __forceinline__ void manifold_hulls(const EpaResult &e, const Side<VPL> &A, const Side<VPL> &Bs, const HullSet &hs,
-                                               uint32_t ha_id, uint32_t hb_id, ClipLds &C, Manifold &m) {
-    const int lane = lane64();
+// result is valid on the group's lane 0, which returns false when a clip buffer smaller than
+// the reference's ran out (the pair is then run again with the full buffers).
+template <class SideT, int HC>
+__device__ __forceinline__ bool manifold_hulls(const EpaResult &e, const SideT &A, const SideT &Bs, const HullSet &hs,
+                                               uint32_t ha_id, uint32_t hb_id, ClipLdsT<HC> &C, Manifold &m) {
+    constexpr int LPP = SideT::kLanes;
+    using G = Grp<LPP>;
+    const int lane = G::lane();
     const float lowest = -3.402823466e+38f;
     m.n = e.n;
     m.np = 0;
@@ -949,19 +1025,21 @@ template <int VPL> __device__ __forceinline__ void manifold_hulls(const EpaResul
     const uint32_t sa = (uint32_t)wave_support_index(A.wh, tf_inv_vec(A.tf, e.n));
     const uint32_t sb = (uint32_t)wave_support_index(Bs.wh, tf_inv_vec(Bs.tf, -e.n));
     float al_a, al_b;
-    const uint32_t pa = best_polygon(ha, A.tf, sa, e.n, al_a);
-    const uint32_t pb = best_polygon(hb, Bs.tf, sb, -e.n, al_b);
+    const uint32_t pa = best_polygon(hs, ha, A.tf, sa, e.n, al_a);
+    const uint32_t pb = best_polygon(hs, hb, Bs.tf, sb, -e.n, al_b);
     // edge pairs at the two support vertices, in (i, j, sign) order
-    const V3 wsa = tf_point(A.tf, tv(ha.v, sa)), wsb = tf_point(Bs.tf, tv(hb.v, sb));
-    const uint32_t as = ha.v2n_o[sa], ae = ha.v2n_o[sa + 1], bs = hb.v2n_o[sb], be = hb.v2n_o[sb + 1];
+    const float4 *va = hs.v + ha.v, *vb = hs.v + hb.v;
+    const V3 wsa = tf_point(A.tf, tv(va, sa)), wsb = tf_point(Bs.tf, tv(vb, sb));
+    const uint32_t as = hs.tab[ha.v2n_o + sa], ae = hs.tab[ha.v2n_o + sa + 1];
+    const uint32_t bs = hs.tab[hb.v2n_o + sb], be = hs.tab[hb.v2n_o + sb + 1];
     const int dega = (int)(ae - as), degb = (int)(be - bs);
     float al_e = lowest;
     int key = 0x7fffffff;
     if (dega > 0 && degb > 0) {
-        for (int c = lane; c < dega * degb; c += 64) {
-            const uint32_t na = ha.v2n_i[as + c / degb], nb = hb.v2n_i[bs + c % degb];
-            const V3 ea = wsa - tf_point(A.tf, tv(ha.v, na));
-            const V3 eb = wsb - tf_point(Bs.tf, tv(hb.v, nb));
+        for (int c = lane; c < dega * degb; c += LPP) {
+            const uint32_t na = hs.tab[ha.v2n_i + as + c / degb], nb = hs.tab[hb.v2n_i + bs + c % degb];
+            const V3 ea = wsa - tf_point(A.tf, tv(va, na));
+            const V3 eb = wsb - tf_point(Bs.tf, tv(vb, nb));
             const V3 cc = cross(ea, eb);
             const float cl = len(cc);
             if (cl < EPS) continue;
@@ -972,53 +1050,60 @@ template <int VPL> __device__ __forceinline__ void manifold_hulls(const EpaResul
             if (dn > al_e) { al_e = dn; key = 2 * c + 1; }
         }
     }
-    wave_argmax(al_e, key);
+    G::argmax(al_e, key);
     const bool has_edge = key != 0x7fffffff;
-    if (lane != 0) return;
+    if (lane != 0) return true;
     if (has_edge && al_e > al_a + 1e-4f && al_e > al_b + 1e-4f) {
         const int c = key / 2;
-        const uint32_t na = ha.v2n_i[as + c / degb], nb = hb.v2n_i[bs + c % degb];
-        const V3 p1 = tf_point(A.tf, tv(ha.v, sa));
-        const V3 d1 = tf_point(A.tf, tv(ha.v, na)) - p1;
-        const V3 p2 = tf_point(Bs.tf, tv(hb.v, sb));
-        const V3 d2 = tf_point(Bs.tf, tv(hb.v, nb)) - p2;
+        const uint32_t na = hs.tab[ha.v2n_i + as + c / degb], nb = hs.tab[hb.v2n_i + bs + c % degb];
+        const V3 p1 = tf_point(A.tf, tv(va, sa));
+        const V3 d1 = tf_point(A.tf, tv(va, na)) - p1;
+        const V3 p2 = tf_point(Bs.tf, tv(vb, sb));
+        const V3 d2 = tf_point(Bs.tf, tv(vb, nb)) - p2;
         V3 l1, l2;
         m.np = 1;
         m.dep[0] = e.depth;
         m.p[0] = skew_lines(p1, d1, p2, d2, l1, l2) ? (l1 + l2) * 0.5f : e.p;
-        return;
+        return true;
     }
     const bool ref_a = al_a > al_b;
     // by value: a reference to one of two locals would put both on the stack
-    const HullTabs hr = ref_a ? ha : hb;
-    const HullTabs hi = ref_a ? hb : ha;
+    // the reference / incident hull's tables read again (copies of ha / hb by value held four
+    // table sets in registers at once)
+    const HullTabs hr = hull_tabs(hs, ref_a ? ha_id : hb_id);
+    const HullTabs hi = hull_tabs(hs, ref_a ? hb_id : ha_id);
     const TF tr = ref_a ? A.tf : Bs.tf;
     const TF ti = ref_a ? Bs.tf : A.tf;
     const uint32_t rp = ref_a ? pa : pb, ip = ref_a ? pb : pa;
-    const V3 rn = tf_vec(tr, tv(hr.pn, rp));
-    const uint32_t is = hi.poff[ip], ie = hi.poff[ip + 1];
+    const float4 *vr = hs.v + hr.v, *vi = hs.v + hi.v;
+    const V3 rn = tf_vec(tr, tv(hs.pn + hr.pn, rp));
+    const uint32_t is = hs.tab[hi.poff + ip], ie = hs.tab[hi.poff + ip + 1];
     const int inc_n = (int)(ie - is);
     if (inc_n < 3) {
         m.np = 1; m.p[0] = e.p; m.dep[0] = e.depth;
-        return;
+        return true;
     }
+    if (HC < HCLIP_CAP && inc_n > HC) return false;
     for (int i = 0; i < inc_n; ++i) {
-        const V3 p = tf_point(ti, tv(hi.v, hi.pidx[is + i]));
+        const V3 p = tf_point(ti, tv(vi, hs.tab[hi.pidx + is + i]));
         C.px[0][i] = p.x; C.py[0][i] = p.y; C.pz[0][i] = p.z;
     }
-    const V3 rpt = tf_point(tr, tv(hr.v, hr.pidx[hr.poff[rp]]));
+    const V3 rpt = tf_point(tr, tv(vr, hs.tab[hr.pidx + hs.tab[hr.poff + rp]]));
     int n = inc_n, buf = 0;
-    for (uint32_t k = hr.p2p_o[rp]; k < hr.p2p_o[rp + 1] && n > 0; ++k) {
-        const uint32_t q = hr.p2p_i[k];
-        const V3 pp = tf_point(tr, tv(hr.v, hr.pidx[hr.poff[q]]));
-        const V3 pn = -tf_vec(tr, tv(hr.pn, q));
-        n = clip_pass(C, buf, n, pp, pn, false);
+    bool full = false;
+    const uint32_t k1 = hs.tab[hr.p2p_o + rp + 1];
+    for (uint32_t k = hs.tab[hr.p2p_o + rp]; k < k1 && n > 0; ++k) {
+        const uint32_t q = hs.tab[hr.p2p_i + k];
+        const V3 pp = tf_point(tr, tv(vr, hs.tab[hr.pidx + hs.tab[hr.poff + q]]));
+        const V3 pn = -tf_vec(tr, tv(hs.pn + hr.pn, q));
+        n = clip_pass(C, buf, n, pp, pn, false, full);
         buf = 1 - buf;
     }
     if (n > 0) {
-        n = clip_pass(C, buf, n, rpt, -rn, true);
+        n = clip_pass(C, buf, n, rpt, -rn, true, full);
         buf = 1 - buf;
     }
+    if (full) return false;
     int kept = 0;   // compacted in place
     for (int i = 0; i < n; ++i) {
         const V3 p{C.px[buf][i], C.py[buf][i], C.pz[buf][i]};
@@ -1033,9 +1118,10 @@ template <int VPL> __device__ __forceinline__ void manifold_hulls(const EpaResul
     }
     if (!kept) {
         m.np = 1; m.p[0] = e.p; m.dep[0] = e.depth;
-        return;
+        return true;
     }
     m.np = reduce4_area(C, buf, kept, m);
+    return true;
 }
 
 // ------------------------------------------------------------------ kernels
@@ -1399,17 +1485,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? O
                                                  Bodies B, Contacts K,
                                                  HullSet hs, uint32_t *gstat, const float4 *guess, int seed_epa,
                                                  float4 *tlog, const uint32_t *tlog_base, const uint32_t *hits,
-                                                 const uint32_t *blk_hits, const float4 *sxs, int guard) {
+                                                 const uint32_t *blk_hits, const float4 *sxs, int guard,
+                                                 const uint32_t *nflat = nullptr) {
     __shared__ PairLds L;
     const int lane = lane64();
     const uint32_t nc = gjk_count(n, ncand);
     if (tlog && tlog_base) tlog += 2 * (size_t)*tlog_base;
-    // work items of `sub` listed pairs: a GJK segment of seg entries splits into seg / sub
-    const uint32_t per = seg / sub, nitems = (nc + seg - 1) / seg * per;
+    // work items of `sub` listed pairs: a GJK segment of seg entries splits into seg / sub;
+    // with nflat, items are the *nflat entries of one flat list whose simplices are stored
+    // per candidate (the pairs the fused pass handed over)
+    const uint32_t per = seg / sub, nitems = nflat ? *nflat : (nc + seg - 1) / seg * per;
     for (uint32_t ve = blockIdx.x; ve < nitems; ve += gridDim.x)
-    for (uint32_t vb = ve / per, r = (ve % per) * sub, re = min(blk_hits[vb], r + sub); r < re; ++r) {
-        const uint32_t h = vb * seg + r;
+    for (uint32_t vb = nflat ? 0u : ve / per, r = nflat ? 0u : (ve % per) * sub,
+                  re = nflat ? 1u : min(blk_hits[vb], r + sub); r < re; ++r) {
+        const uint32_t h = nflat ? ve : vb * seg + r;
         const uint32_t tc = hits[h];
+        const size_t hx = nflat ? (size_t)tc : (size_t)h;   // simplex row
         const uint32_t t = cand ? cand[tc] : tc;
         const uint32_t s = bin_slot[off + t];
         const uint2 cc = K.cc[s], bb = K.bb[s];
@@ -1421,7 +1512,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? O
         sx.rank = 4;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const float4 a = sxs[(size_t)h * 8 + 2 * k], b = sxs[(size_t)h * 8 + 2 * k + 1];
+            const float4 a = sxs[hx * 8 + 2 * k], b = sxs[hx * 8 + 2 * k + 1];
             sx.v[k].w = V3{a.x, a.y, a.z};
             sx.v[k].w0 = V3{a.w, b.x, b.y};
         }
@@ -1512,6 +1603,163 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? O
             else K.cnt[s] = 0;
         }
         __syncthreads();
+    }
+}
+
+// GJK, EPA and the manifold in one pass, one pair per 16-lane DPP row (four per one-wave
+// workgroup, hulls of at most 64 vertices): an overlapping pair goes on to EPA in the row
+// that ran its GJK, with the hull vertices, transforms and simplex still in registers and a
+// polytope of NV vertices / NF faces and clip buffers of HC points in LDS per row.  The
+// steps and their order are those of k_gjk_pass followed by k_epa_pass, so every per-slot
+// result is the same.  A pair whose polytope or clip buffer runs out (EPA_DEFER) keeps its
+// GJK outcome (status, the simplex stored per candidate) and is listed in dhits (*dcount
+// entries, any order) for k_epa_pass in flat-list mode, which runs EPA and the manifold again
+// with the reference-sized buffers; nothing else of the pair is written here.
+// Row polytope and clip buffer sizes of the fused pass; workgroups of the full-size EPA pass
+// that takes the pairs they could not hold.
+constexpr int FUSED_NV = 32, FUSED_NF = 64, FUSED_HC = 40;
+constexpr uint32_t FUSED_DEFER_GRID = 256;
+template <int NV, int NF, int HC, int OCC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC)))
+k_gjk_epa_rows(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *cand, const uint32_t *ncand, Colliders C,
+               Bodies B, Contacts K, HullSet hs, uint32_t *gstat, const float4 *guess, int seed_epa, float4 *tlog,
+               const uint32_t *tlog_base, uint32_t *dcount, uint32_t *dhits, float4 *sxs, float4 *axis, int guard) {
+    using G = Grp<16>;
+    __shared__ PairLdsT<NV, NF, HC> Ls[4];
+    const int row = lane64() >> 4, pl = G::lane();
+    PairLdsT<NV, NF, HC> &L = Ls[row];
+    const uint32_t nc = gjk_count(n, ncand);
+    if (tlog && tlog_base) tlog += 2 * (size_t)*tlog_base;
+    for (uint32_t vb = blockIdx.x; vb * 4 < nc; vb += gridDim.x) {
+        const uint32_t tc = vb * 4 + (uint32_t)row;   // candidate
+        if (tc >= nc) continue;
+        const uint32_t t = cand ? cand[tc] : tc;   // bin position
+        const uint32_t s = bin_slot[off + t];
+        const uint2 cc = K.cc[s], bb = K.bb[s];
+        const TF ta = collider_tf(C, B, cc.x, bb.x), tb = collider_tf(C, B, cc.y, bb.y);
+        Side<4, 16> A, Bs;
+        load_side(A, C, hs, cc.x, ta);
+        load_side(Bs, C, hs, cc.y, tb);
+        const GjkSeeds q = gjk_seeds(ta, tb, guess, tc, seed_epa);
+        // GJK (k_gjk_pass)
+        Simplex sx;
+        int it = 0;
+        int gs = gjk(A, Bs, hs, q.g, sx, it, q.warm ? GJK_WARM_ITERS : GJK_ITERS);
+        int gsum = it, fb = 0;
+        if (q.warm && gs == GJK_FAILED) {
+            fb = 1;
+            gs = gjk(A, Bs, hs, q.plain, sx, it);
+            gsum += it;
+        }
+        if (pl == 0 && axis) {
+            const float ln = len(sx.dir);
+            axis[s] = gs == GJK_VALID && ln > 1e-12f ? f4(vdiv(sx.dir, ln), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        if (gs != GJK_INSIDE) {
+            if (pl == 0) {
+                gstat[s] = pack_gstat(gsum, 0, gs, 15, fb);
+                K.cnt[s] = 0;
+                if (tlog) {
+                    V3 tg{0.0f, 0.0f, 0.0f};
+                    float vg = 0.0f;
+                    const float ln = len(sx.dir);
+                    if (gs == GJK_VALID && ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
+                    tlog[2 * tc] = f4(tg, vg);
+                    tlog[2 * tc + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+            }
+            continue;
+        }
+        if (pl == 0) {   // the simplex, for a pair handed over (constant indices keep it in registers)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const SV &v = sx.v[k];
+                sxs[(size_t)tc * 8 + 2 * k] = make_float4(v.w.x, v.w.y, v.w.z, v.w0.x);
+                sxs[(size_t)tc * 8 + 2 * k + 1] = make_float4(v.w0.y, v.w0.z, 0.0f, 0.0f);
+            }
+        }
+        // EPA and the manifold (k_epa_pass)
+        const uint32_t gsum1 = (uint32_t)gsum, fb1 = (uint32_t)fb;
+        const bool warm = q.warm && !fb;   // a fallback already ran the plain GJK
+        bool defer = false;
+        G::sync();
+        EpaResult e = epa(sx, A, Bs, hs, L.e, warm && q.seeded, q.sn);
+        int esum = e.iters;
+        defer = e.status == EPA_DEFER;
+        if (!defer && warm && e.status != EPA_VALID) {
+            fb = 1;
+            G::sync();
+            gs = gjk(A, Bs, hs, q.plain, sx, it);
+            gsum += it;
+            e = EpaResult{EPA_FAILED, 0, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, 0.0f};
+            if (gs == GJK_INSIDE) {
+                e = epa(sx, A, Bs, hs, L.e, false, V3{0.0f, 0.0f, 0.0f});
+                esum += e.iters;
+                defer = e.status == EPA_DEFER;
+            }
+        }
+        const bool ok = !defer && gs == GJK_INSIDE && e.status == EPA_VALID;
+        if (ok && guard) {   // the EPA guard of k_epa_pass
+            bool any = false;
+            V3 bn{0.0f, 0.0f, 0.0f}, bp{0.0f, 0.0f, 0.0f};
+            float bh = 0.0f;
+            auto consider = [&](V3 gn) {
+                const SV w = md_sample(A, Bs, hs, gn);
+                const float hov = dot(w.w, gn);
+                if (!any || hov < bh) { any = true; bh = hov; bn = gn; bp = w.w0; }
+            };
+            if (K.cnt[s] > 0) {
+                const float4 hp = hd_row(K, s)[0];
+                consider(V3{hp.x, hp.y, hp.z});
+            }
+            if (guard == 2 && e.depth > EPA_GUARD_SCAN && (!any || bh > EPA_GUARD_SCAN)) {
+                const int nfa = face_count(A, hs, C.hull[cc.x]), nfb = face_count(Bs, hs, C.hull[cc.y]);
+                for (int k = 0; k < nfa; ++k) consider(tf_vec(A.tf, face_normal_local(A, hs, C.hull[cc.x], k)));
+                for (int k = 0; k < nfb; ++k) consider(-tf_vec(Bs.tf, face_normal_local(Bs, hs, C.hull[cc.y], k)));
+            }
+            if (any && bh < e.depth - EPA_GUARD_TOL) {
+                e.n = bn;
+                e.depth = bh;
+                e.p = bp;
+            }
+        }
+        Manifold m;
+        m.np = 0;
+        if (ok) {
+            const bool adj = A.kind == SH_HULL && Bs.kind == SH_HULL && hs.meta[C.hull[cc.x]].adj && hs.meta[C.hull[cc.y]].adj;
+            if (adj) {
+                G::sync();   // the manifold buffers overlay the polytope
+                defer = !manifold_hulls(e, A, Bs, hs, (uint32_t)C.hull[cc.x], (uint32_t)C.hull[cc.y], L.c, m);
+                defer = G::ballot(defer) != 0ull;   // lane 0 decides
+            } else {
+                m.n = e.n;
+                m.np = 1;
+                m.p[0] = e.p;
+                m.dep[0] = e.depth;
+            }
+        }
+        if (pl == 0) {
+            if (defer) {
+                gstat[s] = pack_gstat((int)gsum1, 0, GJK_INSIDE, 15, (int)fb1);
+                dhits[atomicAdd(dcount, 1u)] = tc;
+            } else {
+                gstat[s] = pack_gstat(gsum, esum, gs, gs == GJK_INSIDE ? e.status : 15, fb);
+                if (tlog) {
+                    V3 tg{0.0f, 0.0f, 0.0f};
+                    float vg = 0.0f;
+                    if (ok) { tg = tf_inv_vec(ta, e.n); vg = 1.0f; }
+                    else if (gs == GJK_VALID) {
+                        const float ln = len(sx.dir);
+                        if (ln > 1e-12f) { tg = tf_inv_vec(ta, vdiv(-sx.dir, ln)); vg = 1.0f; }
+                    }
+                    tlog[2 * tc] = f4(tg, vg);
+                    tlog[2 * tc + 1] = ok ? f4(tf_inv_vec(ta, e.n), 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                if (m.np > 0) emit(m, B, K, s, bb);
+                else K.cnt[s] = 0;
+            }
+        }
+        G::sync();   // the next pair of this row rewrites its LDS
     }
 }
 
@@ -1631,7 +1879,7 @@ __global__ void k_axis_load(const uint32_t *slots, uint32_t n, Contacts K, const
 // outcome are the same either way (a maximum does not depend on the order it is taken in).
 struct CullPair {
     uint32_t s;
-    uint2 cc;
+    uint2 cc, bb;
     TF ta, tb;
     V3 d;
     float d2, slack;
@@ -1641,9 +1889,9 @@ __device__ __forceinline__ CullPair cull_pair(const uint32_t *bin_slot, uint32_t
     CullPair p;
     p.s = bin_slot[off + i];
     p.cc = K.cc[p.s];
-    const uint2 bb = K.bb[p.s];
-    p.ta = collider_tf(C, B, p.cc.x, bb.x);
-    p.tb = collider_tf(C, B, p.cc.y, bb.y);
+    p.bb = K.bb[p.s];
+    p.ta = collider_tf(C, B, p.cc.x, p.bb.x);
+    p.tb = collider_tf(C, B, p.cc.y, p.bb.y);
     p.d = p.tb.pos - p.ta.pos;
     const float big = fmaxf(fmaxf(fmaxf(fabsf(p.ta.pos.x), fabsf(p.ta.pos.y)), fmaxf(fabsf(p.ta.pos.z), fabsf(p.tb.pos.x))),
                             fmaxf(fabsf(p.tb.pos.y), fabsf(p.tb.pos.z)));
@@ -1671,19 +1919,69 @@ __device__ __forceinline__ float row_support_extent(const Colliders &C, const Hu
     }
     return dpp_row_fmax(e);
 }
+
+// Motion bound of the rejection tests within a frame.  A test that proves a pair apart leaves
+// a margin: how far the shapes could still approach each other along the test's direction
+// with the test passing.  No point of a collider has moved by more than
+//   D = |x - x0| + 2 (|local offset| + bounding radius) |q - q0|
+// since the frame started (x0, q0: its body's pose then; k_update_vel keeps both terms per
+// body), so a pair proven apart with margin m at a moment of bound D' is apart, by at least
+// what that test demanded, while m - D' > D: the record keeps m - D' (less a rounding
+// allowance) per bin position, and later substeps of the frame skip the tests while it holds.
+// Records are rewritten at the frame's first substep; they bound separations, never decide a
+// contact, so the verdicts (and every result) stay those of the full tests.
+struct CullMotion {
+    const float2 *mot;   // per body: |x - x0|, |q - q0| (null: no motion bound)
+    float4 *rec;         // per bin position: margin less the bound of its moment (<= 0: none), Ra, Rb
+    uint2 *rb;           // per bin position: the two bodies
+    const int *later;    // 0 in the frame's first substep
+};
+constexpr float CULL_MOT_REL = 1.0f + 1e-5f, CULL_MOT_ABS = 1e-6f;
+__device__ __forceinline__ float cull_reach(const Colliders &C, uint32_t c) {   // |local offset| + bounding radius
+    const float4 l = C.lpos[c];
+    return sqrtf(l.x * l.x + l.y * l.y + l.z * l.z) + l.w;
+}
+__device__ __forceinline__ void cull_record(const CullMotion &M, const Colliders &C, uint32_t i, const CullPair &p, bool apart,
+                                            float margin) {
+    if (!M.mot) return;
+    const float ra = cull_reach(C, p.cc.x), rb = cull_reach(C, p.cc.y);
+    float m = -1.0f;
+    if (apart) {
+        const float2 ma = M.mot[p.bb.x], mb = M.mot[p.bb.y];
+        const float d = (ma.x + mb.x) + 2.0f * (ra * ma.y + rb * mb.y);
+        m = margin - d * CULL_MOT_REL - CULL_MOT_ABS;
+    }
+    M.rec[i] = make_float4(m, ra, rb, 0.0f);
+    M.rb[i] = p.bb;
+}
+
 __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B,
-                                                  Contacts K, HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep) {
+                                                  Contacts K, HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep,
+                                                  CullMotion M) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = lane64(), row = lane >> 4, pl = lane & 15;
-    bool need = false;
-    if (i < n) {
+    bool need = false, held = false;
+    if (i < n && M.mot && *M.later) {   // still apart by the record of an earlier substep
+        const float4 r = M.rec[i];
+        if (r.x > 0.0f) {
+            const uint2 b = M.rb[i];
+            const float2 ma = M.mot[b.x], mb = M.mot[b.y];
+            const float d = (ma.x + mb.x) + 2.0f * (r.y * ma.y + r.z * mb.y);
+            held = r.x > d * CULL_MOT_REL + CULL_MOT_ABS;
+            if (held) keep[i] = 0u;   // count and status are as the test that proved it left them
+        }
+    }
+    if (i < n && !held) {
         const CullPair p = cull_pair(bin_slot, off, i, C, B, K);
         const float ra = C.lpos[p.cc.x].w, rb = C.lpos[p.cc.y].w;
         const float rs = (ra + rb) * 1.0001f + p.slack;
         bool apart = p.d2 > rs * rs;
+        float margin = apart ? sqrtf(p.d2) - rs : 0.0f;
         auto bound_apart = [&](V3 u, float gap) {
             const float ext = support_extent(C, hs, p.cc.x, p.ta, u, true) + support_extent(C, hs, p.cc.y, p.tb, -u, true);
-            return ext * 1.0001f + p.slack < gap;
+            const float lim = ext * 1.0001f + p.slack;
+            margin = gap - lim;
+            return lim < gap;
         };
         bool axes = false;
         if (!apart && axis) {
@@ -1707,6 +2005,7 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
                 K.cnt[p.s] = 0;
                 gstat[p.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
             }
+            cull_record(M, C, i, p, apart, margin);
         }
     }
     // exact extents of the open pairs, a row per pair
@@ -1719,9 +2018,12 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
         const int src = __ffsll((long long)m) - 1;
         const uint32_t j = blockIdx.x * blockDim.x + (threadIdx.x & ~63u) + (uint32_t)src;   // the open pair's index
         const CullPair p = cull_pair(bin_slot, off, j, C, B, K);
+        float margin = 0.0f;
         auto exact_apart = [&](V3 u, float gap) {
             const float ext = row_support_extent(C, hs, p.cc.x, p.ta, u) + row_support_extent(C, hs, p.cc.y, p.tb, -u);
-            return ext * 1.0001f + p.slack < gap;
+            const float lim = ext * 1.0001f + p.slack;
+            margin = gap - lim;
+            return lim < gap;
         };
         bool apart = false;
         if (axis) {
@@ -1741,6 +2043,7 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
                 K.cnt[p.s] = 0;
                 gstat[p.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
             }
+            cull_record(M, C, j, p, apart, margin);
         }
     }
 }

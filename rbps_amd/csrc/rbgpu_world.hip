@@ -152,6 +152,14 @@ struct rbgpu_world {
     int hull_vpl = 1;                   // hull vertices per lane in the wave kernels (1: all hulls <= 64)
     uint32_t *gstat = nullptr;
     uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
+    uint32_t *gjk_dn = nullptr;   // pairs the fused GJK/EPA pass hands to the full-size EPA pass
+    // motion bound of the rejection tests (RBGPU_OPT_CULL_MOTION): frame-start poses, per-body
+    // displacement, per-bin-position records, the later-substep flag
+    bool cull_motion = false, frame_motion = false;
+    float4 *pose0 = nullptr, *cull_rec = nullptr;
+    float2 *mot = nullptr;
+    uint2 *cull_rb = nullptr;
+    int *cull_later = nullptr;
     uint32_t *gjk_ckey = nullptr, *gjk_ckey2 = nullptr, *gjk_cand2 = nullptr;   // candidates sorted by last iterations
     bool gjk_sort = false;   // measured slower on the combined workload (bin order keeps pairs of a region together)
     float4 *gjk_axis = nullptr;   // per pair slot: the axis that last separated it (rejection hint)
@@ -165,8 +173,9 @@ struct rbgpu_world {
     int epa_guard = 0;                                  // EPA depth bounded along the previous normal (1), and faces (2)
     bool gjk_cull = true;                               // bounding-sphere rejection in front of GJK
     uint32_t gjk_grid = 4096, epa_grid = 8192;          // workgroups of the two GJK/EPA passes
-    int gjk_layout = 1;                                 // GJK pass: 0 a pair per wavefront, 1 per 16-lane row (hulls <= 64
-                                                        // vertices), 2 per lane
+    int gjk_layout = 4;                                 // GJK pass: 0 a pair per wavefront, 1 per 16-lane row (hulls <= 64
+                                                        // vertices), 2 per lane, 3 refilled rows, 4 GJK + EPA per row
+                                                        // (5: with row buffers most pairs outgrow, for tests)
     uint32_t epa_sub = 16;                              // EPA pass: listed pairs per work item
     int gjk_occ = 3;                                    // GJK row pass: waves per EU the registers are budgeted for
     int epa_occ = 0, solve_occ = 0;                     // EPA pass / per-scene position solve: 4 = budget for 4 waves
@@ -260,7 +269,7 @@ struct rbgpu_world {
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
         dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx); dfree(gjk_keep); dfree(gjk_pos); dfree(gjk_cand);
         dfree(gjk_ckey); dfree(gjk_ckey2); dfree(gjk_cand2);
-        dfree(gjk_ncand); dfree(gjk_axis); dfree(sc_work); dfree(sc_list);
+        dfree(gjk_ncand); dfree(gjk_dn); dfree(gjk_axis); dfree(cull_rec); dfree(cull_rb); dfree(sc_work); dfree(sc_list);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.hd); dfree(K.pt); dfree(K.dep);
         pcap = 0;
@@ -274,7 +283,7 @@ struct rbgpu_world {
     void release() {
         if (st) (void)hipStreamSynchronize(st);
         release_pairs();
-        dfree(B.r); dfree(B.f); dfree(B.t); dfree(B.I0); dfree(B.I1);
+        dfree(B.r); dfree(B.f); dfree(B.t); dfree(B.I0); dfree(B.I1); dfree(pose0); dfree(mot); dfree(cull_later);
         dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat); dfree(C.hull);
         dfree(hv); dfree(hpn); dfree(hamin); dfree(hamax); dfree(htab); dfree(hmeta);
         hs = HullSet{};
@@ -362,7 +371,10 @@ struct rbgpu_world {
         const uint32_t *cand = nullptr, *ncand = nullptr;
         if (int rcf = fork_at(1)) return rcf;
         if (gjk_cull && !gjk_explicit) {
-            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep);
+            const bool motion = frame_motion && cull_rec;
+            const CullMotion cm{motion ? mot : nullptr, cull_rec, cull_rb, cull_later};
+            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep, cm);
+            if (motion) k_set_int<<<1, 1, 0, st>>>(cull_later, 1);   // the frame's later substeps
             int rc = scan(gjk_keep, gjk_pos, ng, gjk_ncand);
             if (rc) return rc;
             if (gjk_sort) {   // candidates by their previous GJK iteration count (stable)
@@ -392,6 +404,7 @@ struct rbgpu_world {
         // four pairs per wavefront (one per DPP row) when every hull has at most 64 vertices
         const bool lanes = gjk_layout == 2;
         const bool refill = gjk_layout == 3 && hull_vpl == 1;
+        const bool fused = gjk_layout >= 4 && hull_vpl == 1;
         const bool rows = !lanes && gjk_layout == 1 && hull_vpl == 1;
         const uint32_t ppb = lanes ? GJK_LANE_SEG
                              : refill ? GJK_REFILL_SEG
@@ -400,7 +413,31 @@ struct rbgpu_world {
         const uint32_t sub = std::min<uint32_t>(epa_sub, ppb);
         const uint32_t g1 = std::min<uint32_t>(nvb, gjk_grid), g2 = std::min<uint32_t>(nvb * (ppb / sub), epa_grid);
         hipEvent_t *e = timed_begin(11);   // the GJK pass alone
-        if (refill) {
+        if (fused) {
+            // GJK, EPA and the manifold per 16-lane row (four pairs per one-wave workgroup);
+            // the pairs whose polytope outgrew the row's go to the full-size EPA pass
+            k_set_int<<<1, 1, 0, st>>>((int *)gjk_dn, 0);
+            const uint32_t gf = std::min<uint32_t>((ng + 3) / 4, 4 * gjk_grid);
+            if (gjk_layout == 5)   // row buffers that most pairs outgrow: exercises the hand-over
+                k_gjk_epa_rows<8, 16, 8, 3><<<gf, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl,
+                                                               ds_cnt, gjk_dn, gjk_hits, gjk_sx, gjk_axis, guard);
+            else if (gjk_occ == 4)
+                k_gjk_epa_rows<FUSED_NV, FUSED_NF, FUSED_HC, 4><<<gf, 64, 0, st>>>(
+                    bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl, ds_cnt, gjk_dn, gjk_hits, gjk_sx, gjk_axis,
+                    guard);
+            else if (gjk_occ == 2)
+                k_gjk_epa_rows<FUSED_NV, FUSED_NF, FUSED_HC, 2><<<gf, 64, 0, st>>>(
+                    bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl, ds_cnt, gjk_dn, gjk_hits, gjk_sx, gjk_axis,
+                    guard);
+            else
+                k_gjk_epa_rows<FUSED_NV, FUSED_NF, FUSED_HC, 3><<<gf, 64, 0, st>>>(
+                    bin_list, g0, ng, cand, ncand, C, B, K, hs, gstat, gp, seed, tl, ds_cnt, gjk_dn, gjk_hits, gjk_sx, gjk_axis,
+                    guard);
+            timed_end(e);
+            if (int rcf = fork_at(3)) return rcf;
+            k_epa_pass<1><<<FUSED_DEFER_GRID, 64, 0, st>>>(bin_list, g0, ng, cand, ncand, 1, 1, C, B, K, hs, gstat, gp, seed, tl,
+                                                          ds_cnt, gjk_hits, nullptr, gjk_sx, guard, gjk_dn);
+        } else if (refill) {
             // a wavefront per range of GJK_REFILL_SEG candidates, its rows refilled as they finish
             const uint32_t gr = std::min<uint32_t>((nvb + GJK_PASS_WAVES - 1) / GJK_PASS_WAVES, gjk_grid);
             if (gjk_occ == 2)
@@ -470,6 +507,7 @@ struct rbgpu_world {
         }
         if (e && !cand) timing_bytes += (double)ng * gjk_pass_bytes();
         if (gjk_tot) k_gjk_totals<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, gstat, gjk_tot);
+        if (gjk_tot && fused) k_add_count<<<1, 1, 0, st>>>(gjk_dn, gjk_tot + 6);
         if (gjk_log) {
             k_log_advance<<<1, 1, 0, st>>>(ds_cnt, ncand, ng);
             ds_hi += ng;
@@ -669,6 +707,8 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
         rc |= dalloc(gjk_hits, nh); rc |= dalloc(gjk_blk, nb1);
         rc |= dalloc(gjk_sx, nh * 8);
         rc |= dalloc(gjk_keep, cap); rc |= dalloc(gjk_pos, cap); rc |= dalloc(gjk_cand, cap); rc |= dalloc(gjk_ncand, 1);
+        rc |= dalloc(gjk_dn, 1);
+        rc |= dalloc(cull_rec, cap); rc |= dalloc(cull_rb, cap);
         rc |= dalloc(gjk_ckey, cap); rc |= dalloc(gjk_ckey2, cap); rc |= dalloc(gjk_cand2, cap);
         rc |= dalloc(gjk_axis, cap);
         if (!rc) HIPCHK(hipMemsetAsync(gjk_axis, 0, (size_t)cap * 16, st));   // no hints yet
@@ -812,6 +852,7 @@ int rbgpu_world::build() {
     N = n; M = m; NEP = 2 * S;
     int rc = 0;
     rc |= dalloc(B.r, n); rc |= dalloc(B.f, n); rc |= dalloc(B.t, n); rc |= dalloc(B.I0, n); rc |= dalloc(B.I1, n);
+    rc |= dalloc(pose0, 2 * (size_t)n); rc |= dalloc(mot, n); rc |= dalloc(cull_later, 1);
     rc |= dalloc(C.meta, m); rc |= dalloc(C.dims, m); rc |= dalloc(C.lpos, m); rc |= dalloc(C.lrot, m); rc |= dalloc(C.mat, m);
     rc |= dalloc(C.hull, m);
     rc |= dalloc(hv, hvv.size()); rc |= dalloc(hpn, hpnv.size()); rc |= dalloc(htab, htabv.size());
@@ -1145,6 +1186,10 @@ int rbgpu_world::frame() {
     const bool scene_joints = by_scene && P && NJ && sj_order;
     SceneTabs TJ = T;
     if (!scene_joints) TJ.jorder = nullptr;
+    // motion bound of the GJK rejection tests: poses at the frame's start
+    frame_motion = cull_motion && gjk_cull && N && !hh.empty() && P && bin_off.size() == NBINS + 1 &&
+                   bin_off[BIN_HP] > bin_off[BIN_HH];
+    if (frame_motion) k_pose_save<<<nblk(N), TPB, 0, st>>>(B, N, pose0, mot, cull_later);
     auto substep = [&]() -> int {
         runs.clear();
         if (P) {
@@ -1244,7 +1289,7 @@ int rbgpu_world::frame() {
         }
         if (N) {
             hipEvent_t *e = timed_begin(5);
-            k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h);
+            k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h, frame_motion ? pose0 : nullptr, frame_motion ? mot : nullptr);
             timed_end(e);
         }
         if (by_scene && P) {
@@ -1833,8 +1878,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "contact mode is 0 or 1");
             w->contact_mode = value; return RBGPU_OK;
         case RBGPU_OPT_GJK_ROWS:
-            if (value < 0 || value > 3)
-                return fail(RBGPU_ERR_INVALID, "GJK layout: 0 wavefront, 1 row, 2 lane per pair, 3 refilled rows");
+            if (value < 0 || value > 5)
+                return fail(RBGPU_ERR_INVALID, "GJK layout: 0 wavefront, 1 row, 2 lane per pair, 3 refilled rows, "
+                                               "4 GJK + EPA per row, 5 the same with row buffers most pairs outgrow");
             w->gjk_layout = value; return RBGPU_OK;
         case RBGPU_OPT_GJK_OCCUPANCY:
             if (value < 2 || value > 4) return fail(RBGPU_ERR_INVALID, "GJK occupancy: 2, 3 or 4 waves per EU");
@@ -1857,6 +1903,9 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_GJK_SORT:
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "GJK sort is 0 or 1");
             w->gjk_sort = value != 0; return RBGPU_OK;
+        case RBGPU_OPT_CULL_MOTION:
+            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "cull motion is 0 or 1");
+            w->cull_motion = value != 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1891,6 +1940,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_AXIS_CARRY: *value = w->axis_carry ? 1 : 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_GUARD: *value = w->epa_guard; return RBGPU_OK;
         case RBGPU_OPT_GJK_SORT: *value = w->gjk_sort ? 1 : 0; return RBGPU_OK;
+        case RBGPU_OPT_CULL_MOTION: *value = w->cull_motion ? 1 : 0; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
@@ -2455,20 +2505,24 @@ int rbgpu_mlp_gradients(rbgpu_world *w, uint32_t n, const float *feat32, const f
     return rc;
 }
 
-int rbgpu_get_gjk_totals(rbgpu_world *w, uint64_t *out6, int reset) {
+int rbgpu_get_gjk_counts(rbgpu_world *w, uint64_t *out, uint32_t n, int reset) {
     if (!w) return fail(RBGPU_ERR_INVALID, "null world");
+    if (n > RBGPU_GJK_COUNTS) return fail(RBGPU_ERR_INVALID, "at most RBGPU_GJK_COUNTS counters");
     if (!w->st) HIPCHK(hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking));
+    constexpr size_t bytes = sizeof(unsigned long long) * RBGPU_GJK_COUNTS;
     if (!w->gjk_tot) {
-        if (dalloc(w->gjk_tot, 6)) return RBGPU_ERR_OOM;
-        HIPCHK(hipMemsetAsync(w->gjk_tot, 0, 48, w->st));
+        if (dalloc(w->gjk_tot, RBGPU_GJK_COUNTS)) return RBGPU_ERR_OOM;
+        HIPCHK(hipMemsetAsync(w->gjk_tot, 0, bytes, w->st));
     }
-    unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
-    HIPCHK(hipMemcpyAsync(h, w->gjk_tot, 48, hipMemcpyDeviceToHost, w->st));
-    if (reset) HIPCHK(hipMemsetAsync(w->gjk_tot, 0, 48, w->st));
+    unsigned long long h[RBGPU_GJK_COUNTS] = {};
+    HIPCHK(hipMemcpyAsync(h, w->gjk_tot, bytes, hipMemcpyDeviceToHost, w->st));
+    if (reset) HIPCHK(hipMemsetAsync(w->gjk_tot, 0, bytes, w->st));
     HIPCHK(hipStreamSynchronize(w->st));
-    if (out6) for (int k = 0; k < 6; ++k) out6[k] = h[k];
+    if (out) for (uint32_t k = 0; k < n; ++k) out[k] = h[k];
     return RBGPU_OK;
 }
+
+int rbgpu_get_gjk_totals(rbgpu_world *w, uint64_t *out6, int reset) { return rbgpu_get_gjk_counts(w, out6, 6, reset); }
 
 }  // extern "C"
 
@@ -2502,8 +2556,8 @@ struct rbgpu_snapshot {
 namespace {
 
 constexpr char SNAP_MAGIC[8] = {'R', 'B', 'G', 'P', 'U', 'S', 'N', 'P'};
-constexpr uint32_t SNAP_VERSION = 1;
-constexpr int SNAP_OPTIONS = RBGPU_OPT_GJK_SORT + 1;
+constexpr uint32_t SNAP_VERSION = 2;
+constexpr int SNAP_OPTIONS = RBGPU_OPT_CULL_MOTION + 1;
 constexpr size_t SNAP_ALIGN = 256;
 
 struct SnapSec {

@@ -35,7 +35,7 @@ EXPORTS = [
     "rbgpu_set_option", "rbgpu_get_option", "rbgpu_add_joints", "rbgpu_get_joints", "rbgpu_set_joint_targets",
     "rbgpu_add_hull", "rbgpu_hull_mass_properties", "rbgpu_hull_tables", "rbgpu_collide_world_pairs",
     "rbgpu_mlp_init", "rbgpu_mlp_set_weights", "rbgpu_mlp_get_weights", "rbgpu_mlp_set_mode", "rbgpu_mlp_log",
-    "rbgpu_mlp_logged", "rbgpu_mlp_get_log", "rbgpu_mlp_train", "rbgpu_mlp_eval", "rbgpu_mlp_gradients", "rbgpu_get_gjk_totals",
+    "rbgpu_mlp_logged", "rbgpu_mlp_get_log", "rbgpu_mlp_train", "rbgpu_mlp_eval", "rbgpu_mlp_gradients", "rbgpu_get_gjk_totals", "rbgpu_get_gjk_counts",
     "rbgpu_snapshot_create", "rbgpu_snapshot_capture", "rbgpu_snapshot_restore", "rbgpu_snapshot_size",
     "rbgpu_snapshot_save", "rbgpu_snapshot_load", "rbgpu_snapshot_destroy",
     "rbgpu_last_error",
@@ -114,6 +114,7 @@ def lib():
         L.rbgpu_mlp_eval.argtypes = [_P, C.c_uint32, _F, _F]
         L.rbgpu_mlp_gradients.argtypes = [_P, C.c_uint32, _F, _F, _F, _D]
         L.rbgpu_get_gjk_totals.argtypes = [_P, C.POINTER(C.c_uint64), C.c_int]
+        L.rbgpu_get_gjk_counts.argtypes = [_P, C.POINTER(C.c_uint64), C.c_uint32, C.c_int]
         L.rbgpu_snapshot_create.argtypes = [_P, C.POINTER(_P)]
         L.rbgpu_snapshot_capture.argtypes = [_P, _P]
         L.rbgpu_snapshot_restore.argtypes = [_P, _P]
@@ -190,7 +191,8 @@ OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, 
            "graph": 12, "contact_mode": 13, "gjk_rows": 14,
            "epa_sub": 15, "gjk_occupancy": 16,
            "epa_occupancy": 17, "solve_occupancy": 18,
-           "side_stream": 19, "axis_carry": 20, "epa_guard": 21, "gjk_sort": 22}
+           "side_stream": 19, "axis_carry": 20, "epa_guard": 21, "gjk_sort": 22,
+           "cull_motion": 23}
 
 TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "broadphase": 6,
           "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9, "joints": 10, "gjk": 11}
@@ -436,9 +438,9 @@ class World:
         return g, float(loss[0])
 
     def gjk_totals(self, reset: bool = True) -> dict:
-        out = (C.c_uint64 * 6)()
-        _check(lib().rbgpu_get_gjk_totals(self.h, out, 1 if reset else 0))
-        k = ("pairs", "gjk_iters", "epa_iters", "overlapping", "epa_valid", "fallbacks")
+        out = (C.c_uint64 * 7)()
+        _check(lib().rbgpu_get_gjk_counts(self.h, out, 7, 1 if reset else 0))
+        k = ("pairs", "gjk_iters", "epa_iters", "overlapping", "epa_valid", "fallbacks", "handed_over")
         return {a: int(b) for a, b in zip(k, out)}
 
     # ---------------------------------------------------------------- step

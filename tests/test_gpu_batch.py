@@ -62,13 +62,14 @@ def test_gpu_full_batch_runs_without_cross_scene_pairs():
     assert len(pr) > 0
 
 
-@pytest.mark.parametrize("layout", [1, 2, 3])
+@pytest.mark.parametrize("layout", [1, 2, 3, 4, 5])
 def test_gpu_combined_batch_matches_fp32_oracle_and_single_worlds(layout):
     """Hulls from a shared library, boxes and ragdolls (revolute + spherical joints with
     60 deg limits) in three uneven scenes; 80 frames at dt = 1/120 s, 8 substeps, so the
     pile has landed on the ragdolls (hull-box, hull-capsule, hull-sphere contacts), in the
     contact mode that orients the capsule-capsule normal like the other pairs; GJK pass
-    with a pair per 16-lane row (1) or per lane (2)."""
+    with a pair per 16-lane row (1), per lane (2), per refilled row (3), or GJK and EPA per
+    16-lane row (4; 5 with row buffers that hand most overlapping pairs to the full EPA pass)."""
     lib = S.hull_library()
     parts = [S.combined_scene(n, seed=40 + k, library=lib) for k, n in enumerate((64, 90, 130))]
     bt = S.batch(parts)
@@ -76,6 +77,7 @@ def test_gpu_combined_batch_matches_fp32_oracle_and_single_worlds(layout):
     print("case combined batch bodies", len(bt.bodies), "joints", len(bt.joints), "GJK layout", layout, flush=True)
     g = _gpu(bt, contact_mode=1)
     g.set_option("gjk_rows", layout)
+    g.gjk_totals()   # counts from here on
     o = orc.OracleWorld(bt.timestep, bt.substeps, fp32=True, order=orc.ORDER_COLOUR).load(bt)
     o.set_joint_mode(1)
     o.set_contact_mode(1)
@@ -91,6 +93,10 @@ def test_gpu_combined_batch_matches_fp32_oracle_and_single_worlds(layout):
         bins = g.stats()["bins"]
         hull_pairs = max(hull_pairs, bins[9] + bins[10])
     assert hull_pairs > 0
+    tot = g.gjk_totals()
+    print("GJK totals", tot)
+    if layout == 5:   # the small row buffers hand pairs to the full-size EPA pass
+        assert tot["handed_over"] > 0
     sb, off = g.state(), 0
     for p, s in zip(parts, singles):
         ss, n = s.state(), len(p.bodies)
@@ -261,13 +267,14 @@ def test_gpu_world_pool_equals_worlds_alone():
 def test_gpu_gjk_candidate_order_keeps_results():
     """RBGPU_OPT_GJK_SORT (GJK candidates ordered by their previous iteration count) and the
     GJK layouts change only which wavefront runs which pair: the combined batch's states are
-    bit-identical with the sort on and off, for the row layout and the refilled rows."""
+    bit-identical with the sort on and off, for the row layout, the refilled rows and GJK + EPA
+    per row (with row buffers of both sizes)."""
     lib = S.hull_library()
     parts = [S.combined_scene(n, seed=90 + k, library=lib) for k, n in enumerate((80, 120))]
     bt = S.batch(parts)
     print("case GJK candidate order", len(bt.bodies), "bodies", flush=True)
     runs = []
-    for sort, layout in ((0, 1), (1, 1), (1, 3)):
+    for sort, layout in ((0, 1), (1, 1), (1, 3), (0, 4), (1, 5)):
         g = _gpu(bt, contact_mode=1)
         g.set_option("gjk_sort", sort)
         g.set_option("gjk_rows", layout)

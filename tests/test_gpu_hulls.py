@@ -44,8 +44,10 @@ def _expected_surface_points(r):
     return p + n[None, :] * half[:, None], p - n[None, :] * half[:, None]
 
 
-# GJK pass layouts (option gjk_rows): a pair per wavefront, per 16-lane row, per lane, per refilled row
-LAYOUTS = [0, 1, 2, 3]
+# GJK pass layouts (option gjk_rows): a pair per wavefront, per 16-lane row, per lane, per refilled row;
+# GJK + EPA per 16-lane row (4), the same with row buffers most pairs outgrow, so that the full-size
+# EPA pass takes them over (5)
+LAYOUTS = [0, 1, 2, 3, 4, 5]
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
@@ -77,7 +79,7 @@ def test_gpu_hull_pairs_bit_exact_vs_fp32_oracle(layout):
     assert n_gjk > 1000
 
 
-@pytest.mark.parametrize("layout", [1, 2, 3])
+@pytest.mark.parametrize("layout", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("make,frames", [
     (lambda: scenes.hull_zoo(), 40),
     (lambda: scenes.hull_pile(96, seed=3), 30),
@@ -122,17 +124,19 @@ def test_gpu_gjk_cycling_pairs_match_oracle(layout):
         assert out["n"][i] == 0, i
 
 
+@pytest.mark.parametrize("layout", [1, 4])
 @pytest.mark.parametrize("mode", [1, 2])
-def test_gpu_epa_guard_matches_fp32_oracle(mode):
+def test_gpu_epa_guard_matches_fp32_oracle(mode, layout):
     """The EPA guard (RBGPU_OPT_EPA_GUARD; 1: the previous substep's normal, 2: and both
     shapes' face normals) on a settling pile of 256 hulls: the GPU equals the FP32
     colour-order oracle with the same guard every frame, through the frames where the guard
     replaces FP32 EPA results (the oracle counts them: at least one)."""
     from rbps_amd import world
     sc = scenes.hull_pile(256, seed=3)
-    print("case hull pile 256, EPA guard", mode, flush=True)
+    print("case hull pile 256, EPA guard", mode, "GJK layout", layout, flush=True)
     g = world.World(sc.timestep, sc.substeps).load(sc)
     g.set_option("epa_guard", mode)
+    g.set_option("gjk_rows", layout)
     o = orc.OracleWorld(sc.timestep, sc.substeps, fp32=True, order=orc.ORDER_COLOUR).load(sc)
     o.set_epa_guard(mode)
     for f in range(60):
