@@ -292,11 +292,6 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          numbers of iterations; 0 (default): bin order, which measured faster
  *                          (288 vs 310 us per GJK pass on the combined workload).  Results do not
  *                          change.
- *   RBGPU_OPT_CULL_MOTION  1: within a frame, a pair the rejection tests proved apart
- *                          stays rejected without testing again while the margin its test left
- *                          exceeds how far its colliders can have moved since the frame started
- *                          (|x - x0| + 2 (|offset| + radius) |q - q0|); 0 (default) tests every substep.
- *                          Results do not change.
  *   RBGPU_OPT_EPA_GUARD    0 (default): EPA depths and normals as the reference computes them.
  *                          Any direction n bounds the penetration depth from above by the
  *                          shapes' overlap along n (the Minkowski difference's support along
@@ -340,8 +335,7 @@ enum {
     RBGPU_OPT_SIDE_STREAM = 19,
     RBGPU_OPT_AXIS_CARRY = 20,
     RBGPU_OPT_EPA_GUARD = 21,
-    RBGPU_OPT_GJK_SORT = 22,
-    RBGPU_OPT_CULL_MOTION = 23
+    RBGPU_OPT_GJK_SORT = 22
 };
 int rbgpu_set_option(rbgpu_world *w, int option, int value);
 int rbgpu_get_option(rbgpu_world *w, int option, int *value);

@@ -1879,7 +1879,7 @@ __global__ void k_axis_load(const uint32_t *slots, uint32_t n, Contacts K, const
 // outcome are the same either way (a maximum does not depend on the order it is taken in).
 struct CullPair {
     uint32_t s;
-    uint2 cc, bb;
+    uint2 cc;
     TF ta, tb;
     V3 d;
     float d2, slack;
@@ -1889,9 +1889,9 @@ __device__ __forceinline__ CullPair cull_pair(const uint32_t *bin_slot, uint32_t
     CullPair p;
     p.s = bin_slot[off + i];
     p.cc = K.cc[p.s];
-    p.bb = K.bb[p.s];
-    p.ta = collider_tf(C, B, p.cc.x, p.bb.x);
-    p.tb = collider_tf(C, B, p.cc.y, p.bb.y);
+    const uint2 bb = K.bb[p.s];
+    p.ta = collider_tf(C, B, p.cc.x, bb.x);
+    p.tb = collider_tf(C, B, p.cc.y, bb.y);
     p.d = p.tb.pos - p.ta.pos;
     const float big = fmaxf(fmaxf(fmaxf(fabsf(p.ta.pos.x), fabsf(p.ta.pos.y)), fmaxf(fabsf(p.ta.pos.z), fabsf(p.tb.pos.x))),
                             fmaxf(fabsf(p.tb.pos.y), fabsf(p.tb.pos.z)));
@@ -1919,69 +1919,19 @@ __device__ __forceinline__ float row_support_extent(const Colliders &C, const Hu
     }
     return dpp_row_fmax(e);
 }
-
-// Motion bound of the rejection tests within a frame.  A test that proves a pair apart leaves
-// a margin: how far the shapes could still approach each other along the test's direction
-// with the test passing.  No point of a collider has moved by more than
-//   D = |x - x0| + 2 (|local offset| + bounding radius) |q - q0|
-// since the frame started (x0, q0: its body's pose then; k_update_vel keeps both terms per
-// body), so a pair proven apart with margin m at a moment of bound D' is apart, by at least
-// what that test demanded, while m - D' > D: the record keeps m - D' (less a rounding
-// allowance) per bin position, and later substeps of the frame skip the tests while it holds.
-// Records are rewritten at the frame's first substep; they bound separations, never decide a
-// contact, so the verdicts (and every result) stay those of the full tests.
-struct CullMotion {
-    const float2 *mot;   // per body: |x - x0|, |q - q0| (null: no motion bound)
-    float4 *rec;         // per bin position: margin less the bound of its moment (<= 0: none), Ra, Rb
-    uint2 *rb;           // per bin position: the two bodies
-    const int *later;    // 0 in the frame's first substep
-};
-constexpr float CULL_MOT_REL = 1.0f + 1e-5f, CULL_MOT_ABS = 1e-6f;
-__device__ __forceinline__ float cull_reach(const Colliders &C, uint32_t c) {   // |local offset| + bounding radius
-    const float4 l = C.lpos[c];
-    return sqrtf(l.x * l.x + l.y * l.y + l.z * l.z) + l.w;
-}
-__device__ __forceinline__ void cull_record(const CullMotion &M, const Colliders &C, uint32_t i, const CullPair &p, bool apart,
-                                            float margin) {
-    if (!M.mot) return;
-    const float ra = cull_reach(C, p.cc.x), rb = cull_reach(C, p.cc.y);
-    float m = -1.0f;
-    if (apart) {
-        const float2 ma = M.mot[p.bb.x], mb = M.mot[p.bb.y];
-        const float d = (ma.x + mb.x) + 2.0f * (ra * ma.y + rb * mb.y);
-        m = margin - d * CULL_MOT_REL - CULL_MOT_ABS;
-    }
-    M.rec[i] = make_float4(m, ra, rb, 0.0f);
-    M.rb[i] = p.bb;
-}
-
 __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B,
-                                                  Contacts K, HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep,
-                                                  CullMotion M) {
+                                                  Contacts K, HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = lane64(), row = lane >> 4, pl = lane & 15;
-    bool need = false, held = false;
-    if (i < n && M.mot && *M.later) {   // still apart by the record of an earlier substep
-        const float4 r = M.rec[i];
-        if (r.x > 0.0f) {
-            const uint2 b = M.rb[i];
-            const float2 ma = M.mot[b.x], mb = M.mot[b.y];
-            const float d = (ma.x + mb.x) + 2.0f * (r.y * ma.y + r.z * mb.y);
-            held = r.x > d * CULL_MOT_REL + CULL_MOT_ABS;
-            if (held) keep[i] = 0u;   // count and status are as the test that proved it left them
-        }
-    }
-    if (i < n && !held) {
+    bool need = false;
+    if (i < n) {
         const CullPair p = cull_pair(bin_slot, off, i, C, B, K);
         const float ra = C.lpos[p.cc.x].w, rb = C.lpos[p.cc.y].w;
         const float rs = (ra + rb) * 1.0001f + p.slack;
         bool apart = p.d2 > rs * rs;
-        float margin = apart ? sqrtf(p.d2) - rs : 0.0f;
         auto bound_apart = [&](V3 u, float gap) {
             const float ext = support_extent(C, hs, p.cc.x, p.ta, u, true) + support_extent(C, hs, p.cc.y, p.tb, -u, true);
-            const float lim = ext * 1.0001f + p.slack;
-            margin = gap - lim;
-            return lim < gap;
+            return ext * 1.0001f + p.slack < gap;
         };
         bool axes = false;
         if (!apart && axis) {
@@ -2005,7 +1955,6 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
                 K.cnt[p.s] = 0;
                 gstat[p.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
             }
-            cull_record(M, C, i, p, apart, margin);
         }
     }
     // exact extents of the open pairs, a row per pair
@@ -2018,12 +1967,9 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
         const int src = __ffsll((long long)m) - 1;
         const uint32_t j = blockIdx.x * blockDim.x + (threadIdx.x & ~63u) + (uint32_t)src;   // the open pair's index
         const CullPair p = cull_pair(bin_slot, off, j, C, B, K);
-        float margin = 0.0f;
         auto exact_apart = [&](V3 u, float gap) {
             const float ext = row_support_extent(C, hs, p.cc.x, p.ta, u) + row_support_extent(C, hs, p.cc.y, p.tb, -u);
-            const float lim = ext * 1.0001f + p.slack;
-            margin = gap - lim;
-            return lim < gap;
+            return ext * 1.0001f + p.slack < gap;
         };
         bool apart = false;
         if (axis) {
@@ -2043,7 +1989,6 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
                 K.cnt[p.s] = 0;
                 gstat[p.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
             }
-            cull_record(M, C, j, p, apart, margin);
         }
     }
 }

@@ -802,10 +802,7 @@ __global__ void k_integrate(Bodies B, uint32_t N, float dt) {
     b.w = f4(w, 0.0f);
 }
 
-// With mot: each dynamic body's displacement since the frame's start (pose0) for the GJK
-// rejection tests' motion bound -- |x - x0| and min(|q - q0|, |q + q0|); a point at distance r
-// from the body's origin has moved at most |x - x0| + 2 r |q - q0|.
-__global__ void k_update_vel(Bodies B, uint32_t N, float inv_dt, const float4 *pose0 = nullptr, float2 *mot = nullptr) {
+__global__ void k_update_vel(Bodies B, uint32_t N, float inv_dt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     BodyRow &b = B.r[i];
@@ -816,25 +813,6 @@ __global__ void k_update_vel(Bodies B, uint32_t N, float inv_dt, const float4 *p
     V3 w = (2.0f * inv_dt) * V3{dq.x, dq.y, dq.z};
     b.v = f4(v, 0.0f);
     b.w = f4(dq.w >= 0.0f ? w : -w, 0.0f);
-    if (mot) {
-        const float4 x0 = pose0[2 * i], q0 = pose0[2 * i + 1], q = b.q;
-        const float dm = (q.x - q0.x) * (q.x - q0.x) + (q.y - q0.y) * (q.y - q0.y) + (q.z - q0.z) * (q.z - q0.z) +
-                         (q.w - q0.w) * (q.w - q0.w);
-        const float dp = (q.x + q0.x) * (q.x + q0.x) + (q.y + q0.y) * (q.y + q0.y) + (q.z + q0.z) * (q.z + q0.z) +
-                         (q.w + q0.w) * (q.w + q0.w);
-        mot[i] = make_float2(len(x - xyz(x0)), sqrtf(fminf(dm, dp)));
-    }
-}
-// Frame start for the motion bound: every body's pose, its displacement cleared, the flag
-// that tells the rejection tests a later substep is running cleared.
-__global__ void k_pose_save(Bodies B, uint32_t N, float4 *pose0, float2 *mot, int *later) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *later = 0;
-    if (i >= N) return;
-    const BodyRow &b = B.r[i];
-    pose0[2 * i] = b.x;
-    pose0[2 * i + 1] = b.q;
-    mot[i] = make_float2(0.0f, 0.0f);
 }
 
 // ------------------------------------------------------------------ contact solve (Contact.cpp)

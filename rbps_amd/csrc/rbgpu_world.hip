@@ -153,13 +153,6 @@ struct rbgpu_world {
     uint32_t *gstat = nullptr;
     uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
     uint32_t *gjk_dn = nullptr;   // pairs the fused GJK/EPA pass hands to the full-size EPA pass
-    // motion bound of the rejection tests (RBGPU_OPT_CULL_MOTION): frame-start poses, per-body
-    // displacement, per-bin-position records, the later-substep flag
-    bool cull_motion = false, frame_motion = false;
-    float4 *pose0 = nullptr, *cull_rec = nullptr;
-    float2 *mot = nullptr;
-    uint2 *cull_rb = nullptr;
-    int *cull_later = nullptr;
     uint32_t *gjk_ckey = nullptr, *gjk_ckey2 = nullptr, *gjk_cand2 = nullptr;   // candidates sorted by last iterations
     bool gjk_sort = false;   // measured slower on the combined workload (bin order keeps pairs of a region together)
     float4 *gjk_axis = nullptr;   // per pair slot: the axis that last separated it (rejection hint)
@@ -269,7 +262,7 @@ struct rbgpu_world {
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
         dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx); dfree(gjk_keep); dfree(gjk_pos); dfree(gjk_cand);
         dfree(gjk_ckey); dfree(gjk_ckey2); dfree(gjk_cand2);
-        dfree(gjk_ncand); dfree(gjk_dn); dfree(gjk_axis); dfree(cull_rec); dfree(cull_rb); dfree(sc_work); dfree(sc_list);
+        dfree(gjk_ncand); dfree(gjk_dn); dfree(gjk_axis); dfree(sc_work); dfree(sc_list);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
         dfree(K.cnt); dfree(K.cc); dfree(K.bb); dfree(K.hd); dfree(K.pt); dfree(K.dep);
         pcap = 0;
@@ -283,7 +276,7 @@ struct rbgpu_world {
     void release() {
         if (st) (void)hipStreamSynchronize(st);
         release_pairs();
-        dfree(B.r); dfree(B.f); dfree(B.t); dfree(B.I0); dfree(B.I1); dfree(pose0); dfree(mot); dfree(cull_later);
+        dfree(B.r); dfree(B.f); dfree(B.t); dfree(B.I0); dfree(B.I1);
         dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat); dfree(C.hull);
         dfree(hv); dfree(hpn); dfree(hamin); dfree(hamax); dfree(htab); dfree(hmeta);
         hs = HullSet{};
@@ -371,10 +364,7 @@ struct rbgpu_world {
         const uint32_t *cand = nullptr, *ncand = nullptr;
         if (int rcf = fork_at(1)) return rcf;
         if (gjk_cull && !gjk_explicit) {
-            const bool motion = frame_motion && cull_rec;
-            const CullMotion cm{motion ? mot : nullptr, cull_rec, cull_rb, cull_later};
-            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep, cm);
-            if (motion) k_set_int<<<1, 1, 0, st>>>(cull_later, 1);   // the frame's later substeps
+            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep);
             int rc = scan(gjk_keep, gjk_pos, ng, gjk_ncand);
             if (rc) return rc;
             if (gjk_sort) {   // candidates by their previous GJK iteration count (stable)
@@ -708,7 +698,6 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
         rc |= dalloc(gjk_sx, nh * 8);
         rc |= dalloc(gjk_keep, cap); rc |= dalloc(gjk_pos, cap); rc |= dalloc(gjk_cand, cap); rc |= dalloc(gjk_ncand, 1);
         rc |= dalloc(gjk_dn, 1);
-        rc |= dalloc(cull_rec, cap); rc |= dalloc(cull_rb, cap);
         rc |= dalloc(gjk_ckey, cap); rc |= dalloc(gjk_ckey2, cap); rc |= dalloc(gjk_cand2, cap);
         rc |= dalloc(gjk_axis, cap);
         if (!rc) HIPCHK(hipMemsetAsync(gjk_axis, 0, (size_t)cap * 16, st));   // no hints yet
@@ -852,7 +841,6 @@ int rbgpu_world::build() {
     N = n; M = m; NEP = 2 * S;
     int rc = 0;
     rc |= dalloc(B.r, n); rc |= dalloc(B.f, n); rc |= dalloc(B.t, n); rc |= dalloc(B.I0, n); rc |= dalloc(B.I1, n);
-    rc |= dalloc(pose0, 2 * (size_t)n); rc |= dalloc(mot, n); rc |= dalloc(cull_later, 1);
     rc |= dalloc(C.meta, m); rc |= dalloc(C.dims, m); rc |= dalloc(C.lpos, m); rc |= dalloc(C.lrot, m); rc |= dalloc(C.mat, m);
     rc |= dalloc(C.hull, m);
     rc |= dalloc(hv, hvv.size()); rc |= dalloc(hpn, hpnv.size()); rc |= dalloc(htab, htabv.size());
@@ -1186,10 +1174,6 @@ int rbgpu_world::frame() {
     const bool scene_joints = by_scene && P && NJ && sj_order;
     SceneTabs TJ = T;
     if (!scene_joints) TJ.jorder = nullptr;
-    // motion bound of the GJK rejection tests: poses at the frame's start
-    frame_motion = cull_motion && gjk_cull && N && !hh.empty() && P && bin_off.size() == NBINS + 1 &&
-                   bin_off[BIN_HP] > bin_off[BIN_HH];
-    if (frame_motion) k_pose_save<<<nblk(N), TPB, 0, st>>>(B, N, pose0, mot, cull_later);
     auto substep = [&]() -> int {
         runs.clear();
         if (P) {
@@ -1289,7 +1273,7 @@ int rbgpu_world::frame() {
         }
         if (N) {
             hipEvent_t *e = timed_begin(5);
-            k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h, frame_motion ? pose0 : nullptr, frame_motion ? mot : nullptr);
+            k_update_vel<<<nblk(N), TPB, 0, st>>>(B, N, inv_h);
             timed_end(e);
         }
         if (by_scene && P) {
@@ -1903,9 +1887,6 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
         case RBGPU_OPT_GJK_SORT:
             if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "GJK sort is 0 or 1");
             w->gjk_sort = value != 0; return RBGPU_OK;
-        case RBGPU_OPT_CULL_MOTION:
-            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "cull motion is 0 or 1");
-            w->cull_motion = value != 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1940,7 +1921,6 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_AXIS_CARRY: *value = w->axis_carry ? 1 : 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_GUARD: *value = w->epa_guard; return RBGPU_OK;
         case RBGPU_OPT_GJK_SORT: *value = w->gjk_sort ? 1 : 0; return RBGPU_OK;
-        case RBGPU_OPT_CULL_MOTION: *value = w->cull_motion ? 1 : 0; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
@@ -2556,8 +2536,8 @@ struct rbgpu_snapshot {
 namespace {
 
 constexpr char SNAP_MAGIC[8] = {'R', 'B', 'G', 'P', 'U', 'S', 'N', 'P'};
-constexpr uint32_t SNAP_VERSION = 2;
-constexpr int SNAP_OPTIONS = RBGPU_OPT_CULL_MOTION + 1;
+constexpr uint32_t SNAP_VERSION = 1;
+constexpr int SNAP_OPTIONS = RBGPU_OPT_GJK_SORT + 1;
 constexpr size_t SNAP_ALIGN = 256;
 
 struct SnapSec {

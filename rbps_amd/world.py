@@ -191,8 +191,7 @@ OPTIONS = {"palette_max": 0, "head_rounds": 1, "round_batch": 2, "tail_min": 3, 
            "graph": 12, "contact_mode": 13, "gjk_rows": 14,
            "epa_sub": 15, "gjk_occupancy": 16,
            "epa_occupancy": 17, "solve_occupancy": 18,
-           "side_stream": 19, "axis_carry": 20, "epa_guard": 21, "gjk_sort": 22,
-           "cull_motion": 23}
+           "side_stream": 19, "axis_carry": 20, "epa_guard": 21, "gjk_sort": 22}
 
 TIMING = {"off": 0, "solve_pos": 1, "solve_vel": 2, "narrowphase": 3, "integrate": 4, "update_vel": 5, "broadphase": 6,
           "colouring": 7, "solve_pos_runs": 8, "solve_vel_runs": 9, "joints": 10, "gjk": 11}
