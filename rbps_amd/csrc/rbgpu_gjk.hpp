@@ -2023,19 +2023,29 @@ __global__ void k_gjk_compact_keyed(const uint32_t *keep, const uint32_t *pos, u
     }
 }
 
-__global__ void k_gjk_totals(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *gstat,
-                             unsigned long long *tot) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// A grid of at most GJK_TOTALS_BLOCKS workgroups of 256 threads strides over the pairs; one
+// atomic per counter per workgroup (per-wavefront atomics on six addresses serialised into
+// milliseconds per frame).
+constexpr uint32_t GJK_TOTALS_BLOCKS = 256;
+__global__ void __launch_bounds__(256) k_gjk_totals(const uint32_t *bin_slot, uint32_t off, uint32_t n, const uint32_t *gstat,
+                                                    unsigned long long *tot) {
+    __shared__ uint32_t part[4][6];
     uint32_t v[6] = {0, 0, 0, 0, 0, 0};
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t g = gstat[bin_slot[off + i]];
         const uint32_t gs = (g >> 16) & 15u, es = (g >> 20) & 15u;
-        v[0] = 1; v[1] = g & 255u; v[2] = (g >> 8) & 255u; v[3] = gs == GJK_INSIDE; v[4] = gs == GJK_INSIDE && es == EPA_VALID;
-        v[5] = (g >> 24) & 1u;
+        v[0] += 1u; v[1] += g & 255u; v[2] += (g >> 8) & 255u; v[3] += gs == GJK_INSIDE;
+        v[4] += gs == GJK_INSIDE && es == EPA_VALID; v[5] += (g >> 24) & 1u;
     }
     for (int k = 0; k < 6; ++k) {
         const uint32_t x = wave_sum(v[k]);
-        if (lane64() == 0 && x) atomicAdd(&tot[k], (unsigned long long)x);
+        if (lane64() == 0) part[threadIdx.x >> 6][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const unsigned long long x = (unsigned long long)part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                     part[3][threadIdx.x];
+        if (x) atomicAdd(&tot[threadIdx.x], x);
     }
 }
 

@@ -1069,7 +1069,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_solve_vel_run(Bodies B, Contac
 // kernels walk a scene's colours in order, and pairs of one colour share no dynamic body, so
 // the states are bit-identical to the per-colour launches.
 constexpr uint32_t SCENE_THREADS = 256;      // k_scene_colour's workgroup (the bucketing assumes it)
-constexpr uint32_t SCENE_LDS_PAIRS = 6144;   // active pairs of a scene kept in LDS (more: the global slice)
+constexpr uint32_t SCENE_LDS_PAIRS = 1024;   // active pairs of a scene kept in LDS (more: the global slice)
 struct SceneTabs {
     const uint32_t *sap_start, *sap_end;     // the scene's SAP pairs [start, end) (both 0: none)
     const uint32_t *plane_off, *plane_idx;   // its plane pairs: plane list entries plane_idx[plane_off[s] ..)
@@ -1091,9 +1091,19 @@ __global__ void k_scene_sizes(SceneTabs T, uint32_t nsc, uint32_t *size) {
     if (s < nsc) size[s] = (T.sap_end[s] - T.sap_start[s]) + (T.plane_off[s + 1] - T.plane_off[s]);
 }
 
+// The rounds keep a scene's active pairs in LDS when they fit (SCENE_LDS_PAIRS): pair id,
+// bodies with their dynamic flags, local id, colour and tentative colour, so that a round
+// reads only the bodies' counts, masks and slot claims from global memory (a chain of one or
+// two loads in place of six); larger scenes read the same fields from the global arrays.
+// Either way the rounds, keys, palettes and masks are those of colour_claim / colour_settle.
+constexpr uint32_t SCENE_BDYN = 0x80000000u;   // body word: index | dynamic flag
 __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, const uint2 *pbod, const uint32_t *n_sap_dev,
                                                                 Bodies B, ColourBufs cb, SceneTabs T, uint32_t cap_rounds) {
-    __shared__ uint32_t L[SCENE_LDS_PAIRS];
+    __shared__ uint32_t L[SCENE_LDS_PAIRS];    // pair id
+    __shared__ uint2 LB[SCENE_LDS_PAIRS];      // bodies | SCENE_BDYN
+    __shared__ uint32_t LQ[SCENE_LDS_PAIRS];   // local pair id (the hashes' input)
+    __shared__ int LA[SCENE_LDS_PAIRS];        // colour, COLOUR_OPEN
+    __shared__ uint16_t LT[SCENE_LDS_PAIRS];   // tentative colour of the round
     __shared__ uint32_t hist[MAX_COLOURS];
     __shared__ uint32_t sh[SCENE_THREADS / 64 + 1];
     __shared__ uint32_t nact, won, top;
@@ -1104,31 +1114,56 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     uint32_t *G = T.work + T.base[s];
     if (tid == 0) { nact = 0u; won = 0u; top = 0u; }
     __syncthreads();
-    // activation: open the pairs with points, count them per dynamic body, list them
-    for (uint32_t j0 = 0; j0 < total; j0 += bd) {
-        const uint32_t j = j0 + tid;
-        bool a = false;
-        uint32_t p = 0u;
-        if (j < total) {
-            p = j < na ? a0 + j : ns + T.plane_idx[q0 + j - na];
-            a = cnt[p] > 0;
-            cb.acol[p] = a ? COLOUR_OPEN : COLOUR_NONE;
-            if (a) {
-                const uint2 bb = pbod[p];
-                if (is_dynamic(B, bb.x)) atomicAdd(&cb.ucnt[bb.x], 1u);
-                if (is_dynamic(B, bb.y)) atomicAdd(&cb.ucnt[bb.y], 1u);
+    // activation, in two parts.  (1) The scan: the point counts of ACT_AHEAD pairs per thread
+    // at once, the pairs without points marked, the others listed (no atomics in this loop: a
+    // global atomic between loads made every step wait a round trip).  (2) The listed pairs:
+    // their bodies, dynamic flags and uncoloured counts.
+    constexpr uint32_t ACT_AHEAD = 8;
+    for (uint32_t c0 = 0; c0 < total; c0 += ACT_AHEAD * bd) {
+        uint32_t pid[ACT_AHEAD];
+        bool act[ACT_AHEAD];
+#pragma unroll
+        for (uint32_t u = 0; u < ACT_AHEAD; ++u) {
+            const uint32_t j = c0 + u * bd + tid;
+            pid[u] = j < total ? (j < na ? a0 + j : ns + T.plane_idx[q0 + j - na]) : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ACT_AHEAD; ++u) act[u] = c0 + u * bd + tid < total && cnt[pid[u]] > 0;
+#pragma unroll
+        for (uint32_t u = 0; u < ACT_AHEAD; ++u) {
+            if (c0 + u * bd >= total) break;   // uniform
+            const uint32_t j = c0 + u * bd + tid;
+            const bool a = act[u];
+            if (j < total && !a) cb.acol[pid[u]] = COLOUR_NONE;
+            const uint64_t m = __ballot(a);
+            if (m) {
+                const int leader = __ffsll((long long)m) - 1;
+                uint32_t b0 = 0u;
+                if ((int)(tid & 63) == leader) b0 = atomicAdd(&nact, (uint32_t)__popcll(m));
+                b0 = __shfl(b0, leader, 64);
+                if (a) {
+                    const uint32_t pos = b0 + lane_rank(m);
+                    G[pos] = pid[u];
+                    if (pos < SCENE_LDS_PAIRS) L[pos] = pid[u];
+                }
             }
         }
-        const uint64_t m = __ballot(a);
-        if (m) {
-            const int leader = __ffsll((long long)m) - 1;
-            uint32_t b0 = 0u;
-            if ((int)(tid & 63) == leader) b0 = atomicAdd(&nact, (uint32_t)__popcll(m));
-            b0 = __shfl(b0, leader, 64);
-            if (a) {
-                const uint32_t pos = b0 + lane_rank(m);
-                G[pos] = p;
-                if (pos < SCENE_LDS_PAIRS) L[pos] = p;
+    }
+    __syncthreads();
+    {
+        const uint32_t na_ = nact;
+        for (uint32_t j = tid; j < na_; j += bd) {
+            const uint32_t p = j < SCENE_LDS_PAIRS ? L[j] : G[j];
+            const uint2 bb = pbod[p];
+            const uint32_t q = cb.lid[p];
+            const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
+            cb.acol[p] = COLOUR_OPEN;
+            if (da) atomicAdd(&cb.ucnt[bb.x], 1u);
+            if (db) atomicAdd(&cb.ucnt[bb.y], 1u);
+            if (j < SCENE_LDS_PAIRS) {
+                LB[j] = make_uint2(bb.x | (da ? SCENE_BDYN : 0u), bb.y | (db ? SCENE_BDYN : 0u));
+                LQ[j] = q;
+                LA[j] = COLOUR_OPEN;
             }
         }
     }
@@ -1138,15 +1173,62 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     // rounds 0, 1, ... as the grid-wide path runs them
     uint32_t left = n, r = 0;
     for (; left > 0 && r < cap_rounds; ++r) {
-        for (uint32_t j = tid; j < n; j += bd) {
-            const uint32_t p = in_lds ? L[j] : G[j];
-            if (cb.acol[p] == COLOUR_OPEN) colour_claim(cb, pbod, B, p, r);
+        if (in_lds) {
+            for (uint32_t j = tid; j < n; j += bd) {   // claim (colour_claim)
+                if (LA[j] != COLOUR_OPEN) continue;
+                const uint2 bw = LB[j];
+                const bool da = bw.x & SCENE_BDYN, db = bw.y & SCENE_BDYN;
+                const uint32_t ba = bw.x & ~SCENE_BDYN, bb = bw.y & ~SCENE_BDYN;
+                const uint32_t k = (da ? ld_shared(&cb.ucnt[ba]) : 0u) + (db ? ld_shared(&cb.ucnt[bb]) : 0u);
+                const uint32_t q = LQ[j];
+                uint32_t idx = tent_hash(q, r) % palette(k, cb.pal_max);
+                uint32_t t = MAX_COLOURS;
+                for (uint32_t w = 0; w < COLOUR_WORDS; ++w) {
+                    uint32_t fr = ~((da ? ld_shared(&cb.used[ba * COLOUR_WORDS + w]) : 0u) |
+                                    (db ? ld_shared(&cb.used[bb * COLOUR_WORDS + w]) : 0u));
+                    const uint32_t nf = (uint32_t)__popc(fr);
+                    if (idx < nf) {
+                        for (; idx; --idx) fr &= fr - 1u;
+                        t = w * 32u + (uint32_t)__builtin_ctz(fr);
+                        break;
+                    }
+                    idx -= nf;
+                }
+                if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&cb.ctr[CTR_ERR], 1u); t = MAX_COLOURS - 1; }
+                LT[j] = (uint16_t)t;
+                const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(q, r);
+                if (da) atomicMax(&cb.claim[ba * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
+                if (db) atomicMax(&cb.claim[bb * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
+            }
+        } else {
+            for (uint32_t j = tid; j < n; j += bd) {
+                const uint32_t p = G[j];
+                if (cb.acol[p] == COLOUR_OPEN) colour_claim(cb, pbod, B, p, r);
+            }
         }
         __syncthreads();
         uint32_t w = 0;
-        for (uint32_t j = tid; j < n; j += bd) {
-            const uint32_t p = in_lds ? L[j] : G[j];
-            if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r)) ++w;
+        if (in_lds) {
+            for (uint32_t j = tid; j < n; j += bd) {   // settle (colour_settle)
+                if (LA[j] != COLOUR_OPEN) continue;
+                const uint2 bw = LB[j];
+                const bool da = bw.x & SCENE_BDYN, db = bw.y & SCENE_BDYN;
+                const uint32_t ba = bw.x & ~SCENE_BDYN, bb = bw.y & ~SCENE_BDYN;
+                const uint32_t t = LT[j];
+                const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(LQ[j], r);
+                if (da && ld_shared(&cb.claim[ba * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) continue;
+                if (db && ld_shared(&cb.claim[bb * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) continue;
+                LA[j] = (int)t;
+                const uint32_t bit = 1u << (t % 32u);
+                if (da) { atomicOr(&cb.used[ba * COLOUR_WORDS + t / 32u], bit); atomicSub(&cb.ucnt[ba], 1u); }
+                if (db) { atomicOr(&cb.used[bb * COLOUR_WORDS + t / 32u], bit); atomicSub(&cb.ucnt[bb], 1u); }
+                ++w;
+            }
+        } else {
+            for (uint32_t j = tid; j < n; j += bd) {
+                const uint32_t p = G[j];
+                if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r)) ++w;
+            }
         }
         for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
         if ((tid & 63) == 0 && w) atomicAdd(&won, w);
@@ -1154,11 +1236,14 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         left = n - won;
         __syncthreads();
     }
+    if (in_lds)   // the colours to the global array (pair colours are read from it)
+        for (uint32_t j = tid; j < n; j += bd) cb.acol[L[j]] = LA[j];
+    auto colour_of = [&](uint32_t j) { return in_lds ? LA[j] : cb.acol[G[j]]; };
     // bucket by colour: sizes, offsets (four colours per thread), scatter
     for (uint32_t c = tid; c < (uint32_t)MAX_COLOURS; c += bd) hist[c] = 0u;
     __syncthreads();
     for (uint32_t j = tid; j < n; j += bd) {
-        const int c = cb.acol[in_lds ? L[j] : G[j]];
+        const int c = colour_of(j);
         if (c >= 0) atomicAdd(&hist[c], 1u);
     }
     __syncthreads();
@@ -1181,9 +1266,8 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     __syncthreads();
     uint32_t *out = T.list + T.base[s];
     for (uint32_t j = tid; j < n; j += bd) {
-        const uint32_t p = in_lds ? L[j] : G[j];
-        const int c = cb.acol[p];
-        if (c >= 0) out[atomicAdd(&hist[c], 1u)] = p;
+        const int c = colour_of(j);
+        if (c >= 0) out[atomicAdd(&hist[c], 1u)] = in_lds ? L[j] : G[j];
     }
     if (tid == 0) {
         T.ncol[s] = top;

@@ -496,7 +496,8 @@ struct rbgpu_world {
                                                     ds_cnt, gjk_hits, gjk_blk, gjk_sx, guard);
         }
         if (e && !cand) timing_bytes += (double)ng * gjk_pass_bytes();
-        if (gjk_tot) k_gjk_totals<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, gstat, gjk_tot);
+        if (gjk_tot)
+            k_gjk_totals<<<std::min<uint32_t>(nblk(ng, 256), GJK_TOTALS_BLOCKS), 256, 0, st>>>(bin_list, g0, ng, gstat, gjk_tot);
         if (gjk_tot && fused) k_add_count<<<1, 1, 0, st>>>(gjk_dn, gjk_tot + 6);
         if (gjk_log) {
             k_log_advance<<<1, 1, 0, st>>>(ds_cnt, ncand, ng);
