@@ -267,7 +267,7 @@ def measured_traffic(kernel, workload="pile"):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload
     (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings), or None."""
     name = {"pile": f"{kernel}_pile_mixed_65k",
-            "combined": {"gjk": "gjk_pass_combined_1024_scenes",
+            "combined": {"gjk": "gjk_epa_rows_combined_1024_scenes",
                          "solve_pos": "scene_solve_pos_combined_1024_scenes"}.get(kernel)}.get(workload)
     if name is None:
         return None
@@ -421,9 +421,11 @@ def main():
     if roof:
         roof["traffic"] = measured_traffic(args.timing_kernel, args.workload)
     if roof and args.timing_kernel == "gjk":
-        roof["kernel"] = "k_gjk_pass"
-        roof["note"] = ("GJK on the candidates that pass the exact rejection tests, four pairs per wavefront; "
-                        "algorithmic bytes = candidates (device count) x (2 x (16 B x mean hull vertices + 164 B) + 40 B)")
+        roof["kernel"] = "k_gjk_epa_rows"
+        roof["note"] = ("GJK, then EPA and the manifold for the overlapping pairs, on the candidates that pass the "
+                        "exact rejection tests, one pair per 16-lane row; algorithmic bytes = candidates (device "
+                        "count) x (2 x (16 B x mean hull vertices + 164 B) + 40 B), the inputs (the contacts written "
+                        "for the overlapping pairs are not counted); VALU-issue bound, not HBM bound")
     out = {
         "metric": METRIC,
         "value": round(value, 1), "unit": "body-substeps/s", "n_gpus": 1, "steps": args.steps,

@@ -154,6 +154,91 @@ __device__ __forceinline__ void joint_rows(int4 m, float4 r1, float4 r2, float4 
     row[3] = o3;
 }
 
+// Row k of joint_rows alone, from the same expressions in the same order (the XPBD mode
+// re-evaluates one row just before solving it; evaluating all four there repeated the angle
+// functions four times per row).  With `cur`, tg.z also receives the joint's current
+// position as joint_rows leaves it.  A constraint row (type ROW) keeps `old`.
+__device__ __forceinline__ float4 joint_row(int4 m, float4 r1, float4 r2, float4 ax, float4 lx, float4 &tg, V3 x1, Q q1,
+                                            V3 x2, Q q2, int mode, int k, bool cur, float4 old) {
+    const int type = jt_type(m), act = jt_act(m);
+    if (type == JT_ROW) return old;
+    if ((type == JT_PRISMATIC || type == JT_FIXED) && k == 0) {
+        const Q dq = qmul(q1, conj(q2));
+        return row_value(V3{dq.x, dq.y, dq.z} * 2.0f, mode);
+    }
+    const V3 p1 = x1 + rotate(q1, xyz(r1)), p2 = x2 + rotate(q2, xyz(r2));
+    if (type == JT_FIXED) return k == 1 ? row_value(p1 - p2, mode) : old;
+    if (type == JT_PRISMATIC) {
+        const V3 dp = p1 - p2;
+        const V3 axis = rotate(q1, xyz(ax));
+        const float ps = dot(dp, axis);
+        if (cur) tg.z = ps;
+        if (k == 1) {
+            float over = 0.0f;
+            if (jt_limited(m)) over = ps - clampf(ps, ax.w, lx.w);
+            return row_value((dp - axis * ps) + axis * over, mode);
+        }
+        if (k == 2) {
+            V3 drive{0.0f, 0.0f, 0.0f};
+            if (act == JA_SPEED || act == JA_POSITION) {
+                const float curr = dot(p2 - p1, axis);
+                drive = axis * (tg.x - curr);
+            }
+            return row_value(drive, mode);
+        }
+        return old;
+    }
+    if (type == JT_REVOLUTE) {
+        if (k == 1) {
+            if (!cur) return row_value(p1 - p2, mode);
+        }
+        const V3 a1 = rotate(q1, xyz(ax)), a2 = rotate(q2, xyz(ax));
+        if (k == 0 && !cur) return row_value(mode == 0 ? cross(a1, a2) : cross(a2, a1), mode);
+        const V3 l1 = rotate(q1, xyz(lx)), l2 = rotate(q2, xyz(lx));
+        float4 out = old;
+        if (k == 0) out = row_value(mode == 0 ? cross(a1, a2) : cross(a2, a1), mode);
+        if (k == 1) out = row_value(p1 - p2, mode);
+        if (k == 2 || cur) {
+            float phi = rb_atan2(dot(cross(l1, l2), a1), dot(l1, l2));
+            V3 lim{0.0f, 0.0f, 0.0f};
+            if (jt_limited(m) && (phi < ax.w || phi > lx.w)) {
+                phi = clampf(phi, ax.w, lx.w);
+                if (k == 2) {
+                    const V3 n1 = rotate(from_axis_angle(a1, phi), l1);
+                    lim = cross(n1, l2);
+                }
+            }
+            if (k == 2) out = row_value(-lim, mode);
+            if (cur) tg.z = phi;
+        }
+        if (k == 3) {
+            V3 drive{0.0f, 0.0f, 0.0f};
+            if (act == JA_SPEED || act == JA_POSITION) {
+                const float theta = tg.x + 3.14159265358979323846f;
+                const V3 bt = (l1 * rb_cos(theta) + cross(a1, l1) * rb_sin(theta)) + a1 * dot(a1, l1) * (1.0f - rb_cos(theta));
+                drive = cross(bt, l2);
+            }
+            out = row_value(drive, mode);
+        }
+        return out;
+    }
+    // spherical
+    if (k == 0) return row_value(p1 - p2, mode);
+    if (k == 1 && jt_limited(m)) {
+        const V3 a1 = rotate(q1, xyz(ax)), a2 = rotate(q2, xyz(ax));
+        const V3 c = cross(a1, a2);
+        const float th = rb_atan2(len(c), dot(a1, a2));
+        V3 lim{0.0f, 0.0f, 0.0f};
+        if (th > lx.w) {
+            const V3 n1 = rotate(from_axis_angle(c, lx.w), a1);
+            lim = cross(n1, a2);
+        }
+        if (cur) tg.z = th;
+        return row_value(-lim, mode);
+    }
+    return old;
+}
+
 // compute_joint_errors (Joint.cpp:222-244) for every joint; mode 1 also resets lambda.
 __device__ __forceinline__ void joint_errors(Bodies &B, Joints &J, uint32_t i, float h, int mode) {
     const int4 m = J.jm[i];
@@ -205,12 +290,9 @@ __device__ __forceinline__ void joint_solve(Bodies &B, Joints &J, uint32_t i, fl
     float4 o0 = nrows > 0 ? J.jrow[(size_t)i * 4] : z4, o1 = nrows > 1 ? J.jrow[(size_t)i * 4 + 1] : z4;
     float4 o2 = nrows > 2 ? J.jrow[(size_t)i * 4 + 2] : z4, o3 = nrows > 3 ? J.jrow[(size_t)i * 4 + 3] : z4;
     for (int k = 0; k < nrows; ++k) {
-        if (mode == 1) {
-            float4 row[4] = {o0, o1, o2, o3};
-            joint_rows(m, r1f, r2f, axf, lxf, tg, A.x, A.q, Bd.x, Bd.q, h, false, mode, row);
-            o0 = row[0]; o1 = row[1]; o2 = row[2]; o3 = row[3];
-        }
-        const float4 rv = k == 0 ? o0 : k == 1 ? o1 : k == 2 ? o2 : o3;
+        float4 rv = k == 0 ? o0 : k == 1 ? o1 : k == 2 ? o2 : o3;
+        if (mode == 1)   // this row from the current poses; the joint's position at its last row
+            rv = joint_row(m, r1f, r2f, axf, lxf, tg, A.x, A.q, Bd.x, Bd.q, mode, k, k == nrows - 1, rv);
         const V3 n = xyz(rv);
         const float comp = (k == crow) ? r2f.w : 0.0f;
         float lam = J.jlam[(size_t)i * 4 + k];
@@ -296,19 +378,39 @@ __global__ void __launch_bounds__(256) k_joint_damp(Bodies B, Joints J, const ui
 // (Constraint.cpp:89-106), after the velocity solve their damping (Joint.cpp:249-267) — the
 // order of the global joint launches restricted to the scene, whose joints touch no other
 // scene's bodies.
+// The scene's colour offsets, its pair list and joint order are read into LDS once (up to
+// SOLVE_LDS_PAIRS pairs, SOLVE_LDS_JOINTS joints; larger scenes read them from the global
+// lists), so that each colour step starts with the contact and body loads.
+constexpr uint32_t SOLVE_LDS_PAIRS = 2048, SOLVE_LDS_JOINTS = 1024, SOLVE_LDS_COLOURS = 256, SOLVE_LDS_JCOLS = 64;
 template <bool POS, int OCC = 1>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pts, Joints J, float inv_h, int jmode) {
+    __shared__ uint32_t s_co[SOLVE_LDS_COLOURS + 1], s_lst[SOLVE_LDS_PAIRS];
+    __shared__ uint32_t s_jc[SOLVE_LDS_JCOLS + 1], s_jo[SOLVE_LDS_JOINTS];
     const uint32_t s = blockIdx.x;
     const uint32_t nc = T.ncol[s];
     const uint32_t *lst = T.list + T.base[s];
     const uint32_t *co = T.coff + (size_t)s * (MAX_COLOURS + 1);
+    const bool jn = T.jorder != nullptr;
+    const uint32_t j0 = jn ? T.jbase[s] : 0u, j1 = jn ? T.jbase[s + 1] : 0u;
+    const uint32_t *jc = jn ? T.jcoff + (size_t)s * (T.njc + 1) : nullptr;
+    const bool co_l = nc <= SOLVE_LDS_COLOURS, jc_l = jn && T.njc <= SOLVE_LDS_JCOLS;
+    for (uint32_t c = threadIdx.x; co_l && c <= nc; c += blockDim.x) s_co[c] = co[c];
+    for (uint32_t c = threadIdx.x; jc_l && c <= T.njc; c += blockDim.x) s_jc[c] = jc[c];
+    const uint32_t nj = j1 - j0;
+    const bool jo_l = nj <= SOLVE_LDS_JOINTS;
+    for (uint32_t i = threadIdx.x; jo_l && i < nj; i += blockDim.x) s_jo[i] = T.jorder[j0 + i];
+    __syncthreads();
+    const uint32_t np = co_l ? s_co[nc] : co[nc];
+    const bool ls_l = np <= SOLVE_LDS_PAIRS;
+    for (uint32_t i = threadIdx.x; ls_l && i < np; i += blockDim.x) s_lst[i] = lst[i];
+    __syncthreads();
     for (uint32_t c = 0; c < nc; ++c) {
-        const uint32_t o = co[c], e = co[c + 1];
+        const uint32_t o = co_l ? s_co[c] : co[c], e = co_l ? s_co[c + 1] : co[c + 1];
         for (uint32_t i0 = o; i0 < e; i0 += blockDim.x) {
             const uint32_t i = i0 + threadIdx.x;
             const bool v = i < e;
-            const uint32_t p = v ? lst[i] : 0u;
+            const uint32_t p = v ? (ls_l ? s_lst[i] : lst[i]) : 0u;
             if (pts) count_points(K, p, v, pts);
             if (v) {
                 if (POS) solve_pos_pair(B, K, p);
@@ -317,19 +419,17 @@ k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pt
         }
         __syncthreads();
     }
-    if (!T.jorder) return;
-    const uint32_t j0 = T.jbase[s], j1 = T.jbase[s + 1];
-    if (j0 == j1) return;
-    const uint32_t *jc = T.jcoff + (size_t)s * (T.njc + 1);
+    if (!jn || j0 == j1) return;
+    auto jorder = [&](uint32_t i) { return jo_l ? s_jo[i - j0] : T.jorder[i]; };
     if (POS) {
-        for (uint32_t i = j0 + threadIdx.x; i < j1; i += blockDim.x) joint_errors(B, J, T.jorder[i], h, jmode);
+        for (uint32_t i = j0 + threadIdx.x; i < j1; i += blockDim.x) joint_errors(B, J, jorder(i), h, jmode);
         __syncthreads();
     }
     for (uint32_t c = 0; c < T.njc; ++c) {
-        const uint32_t e = j0 + jc[c + 1];
-        for (uint32_t i = j0 + jc[c] + threadIdx.x; i < e; i += blockDim.x) {
-            if (POS) joint_solve(B, J, T.jorder[i], inv_h, h, jmode);
-            else joint_damp(B, J, T.jorder[i], h);
+        const uint32_t e = j0 + (jc_l ? s_jc[c + 1] : jc[c + 1]);
+        for (uint32_t i = j0 + (jc_l ? s_jc[c] : jc[c]) + threadIdx.x; i < e; i += blockDim.x) {
+            if (POS) joint_solve(B, J, jorder(i), inv_h, h, jmode);
+            else joint_damp(B, J, jorder(i), h);
         }
         __syncthreads();
     }
