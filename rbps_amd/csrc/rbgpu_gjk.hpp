@@ -1777,10 +1777,14 @@ __device__ __forceinline__ float support_extent(const Colliders &C, const HullSe
     if (m.x == SH_BOX) return fabsf(l.x) * dm.x + fabsf(l.y) * dm.y + fabsf(l.z) * dm.z;
     if (m.x == SH_CAPSULE) return fabsf(l.y) * dm.x + dm.y;
     const uint32_t h = (uint32_t)C.hull[c];
-    if (bound_only) {
+    if (bound_only) {   // the smaller of the local AABB's and the bounding ellipsoid's extents
         const float4 lo = hs.amin[h], hi = hs.amax[h];
-        return (l.x >= 0.0f ? hi.x * l.x : lo.x * l.x) + (l.y >= 0.0f ? hi.y * l.y : lo.y * l.y) +
-               (l.z >= 0.0f ? hi.z * l.z : lo.z * l.z);
+        const float box = (l.x >= 0.0f ? hi.x * l.x : lo.x * l.x) + (l.y >= 0.0f ? hi.y * l.y : lo.y * l.y) +
+                          (l.z >= 0.0f ? hi.z * l.z : lo.z * l.z);
+        if (!hs.ell) return box;
+        const float4 c = hs.ell[2 * h], a = hs.ell[2 * h + 1];
+        const float ax = a.x * l.x, ay = a.y * l.y, az = a.z * l.z;
+        return fminf(box, (c.x * l.x + c.y * l.y + c.z * l.z) + sqrtf(ax * ax + ay * ay + az * az));
     }
     const HullMeta hm = hs.meta[h];
     float e = -3.4e38f;
@@ -1924,8 +1928,9 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = lane64(), row = lane >> 4, pl = lane & 15;
     bool need = false;
+    CullPair p{};   // this lane's pair (the exact stage takes the open pairs' from their lanes)
     if (i < n) {
-        const CullPair p = cull_pair(bin_slot, off, i, C, B, K);
+        p = cull_pair(bin_slot, off, i, C, B, K);
         const float ra = C.lpos[p.cc.x].w, rb = C.lpos[p.cc.y].w;
         const float rs = (ra + rb) * 1.0001f + p.slack;
         bool apart = p.d2 > rs * rs;
@@ -1963,31 +1968,44 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
         uint64_t m = todo;
         for (int k = 0; k < row; ++k) m &= m - 1ull;
         for (int k = 0; k < 4; ++k) todo &= todo - 1ull;
+        // the open pair as its lane computed it, shuffled (not loaded again) while every lane of
+        // the wavefront is active; a row without a pair this round takes lane 0's and waits
+        const int src = m ? __ffsll((long long)m) - 1 : 0;
+        auto shf = [&](float x) { return __shfl(x, src, 64); };
+        auto shu = [&](uint32_t x) { return (uint32_t)__shfl((int)x, src, 64); };
+        CullPair q;
+        q.s = shu(p.s);
+        q.cc = make_uint2(shu(p.cc.x), shu(p.cc.y));
+        q.ta.pos = V3{shf(p.ta.pos.x), shf(p.ta.pos.y), shf(p.ta.pos.z)};
+        q.ta.rot = Q{shf(p.ta.rot.w), shf(p.ta.rot.x), shf(p.ta.rot.y), shf(p.ta.rot.z)};
+        q.tb.pos = V3{shf(p.tb.pos.x), shf(p.tb.pos.y), shf(p.tb.pos.z)};
+        q.tb.rot = Q{shf(p.tb.rot.w), shf(p.tb.rot.x), shf(p.tb.rot.y), shf(p.tb.rot.z)};
+        q.d = V3{shf(p.d.x), shf(p.d.y), shf(p.d.z)};
+        q.d2 = shf(p.d2);
+        q.slack = shf(p.slack);
         if (m == 0ull) continue;   // fewer than four pairs left: this row waits (whole row inactive)
-        const int src = __ffsll((long long)m) - 1;
         const uint32_t j = blockIdx.x * blockDim.x + (threadIdx.x & ~63u) + (uint32_t)src;   // the open pair's index
-        const CullPair p = cull_pair(bin_slot, off, j, C, B, K);
         auto exact_apart = [&](V3 u, float gap) {
-            const float ext = row_support_extent(C, hs, p.cc.x, p.ta, u) + row_support_extent(C, hs, p.cc.y, p.tb, -u);
-            return ext * 1.0001f + p.slack < gap;
+            const float ext = row_support_extent(C, hs, q.cc.x, q.ta, u) + row_support_extent(C, hs, q.cc.y, q.tb, -u);
+            return ext * 1.0001f + q.slack < gap;
         };
         bool apart = false;
         if (axis) {
-            const float4 a = axis[p.s];
+            const float4 a = axis[q.s];
             if (a.w != 0.0f) {
                 const V3 u{a.x, a.y, a.z};
-                apart = exact_apart(u, dot(p.d, u));
+                apart = exact_apart(u, dot(q.d, u));
             }
         }
-        if (!apart && p.d2 > 1e-8f) {
-            const float dl = sqrtf(p.d2);
-            apart = exact_apart(vdiv(p.d, dl), dl);
+        if (!apart && q.d2 > 1e-8f) {
+            const float dl = sqrtf(q.d2);
+            apart = exact_apart(vdiv(q.d, dl), dl);
         }
         if (pl == 0) {
             keep[j] = apart ? 0u : 1u;
             if (apart) {
-                K.cnt[p.s] = 0;
-                gstat[p.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
+                K.cnt[q.s] = 0;
+                gstat[q.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
             }
         }
     }

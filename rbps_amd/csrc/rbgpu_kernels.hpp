@@ -60,6 +60,7 @@ struct HullSet {
     const uint32_t *tab;   // polygon loops and adjacency lists (CSR, hull-local indices)
     const HullMeta *meta;
     const float4 *amin, *amax;   // local AABB
+    const float4 *ell;           // per hull: bounding ellipsoid centre, then its semi-axes (local frame)
 };
 struct Contacts {
     int *cnt;        // points per slot this substep

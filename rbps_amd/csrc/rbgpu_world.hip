@@ -101,6 +101,10 @@ struct rbgpu_world {
     struct HostHull {
         rbh::HullTables t;
         float amin[3], amax[3];   // local AABB of the FP32 vertices
+        // bounding ellipsoid of the FP32 vertices (local frame, axes along the local axes): centre
+        // and semi-axes, every vertex v satisfying sum_k ((v_k - c_k) / a_k)^2 <= 1 with room for
+        // FP32 rounding; extent along a unit u at most c . u + |diag(a) u|
+        float ec[3], ea[3];
     };
     std::vector<HostHull> hh;           // convex hull data (rbgpu_add_hull)
     std::vector<float4> h_jtg;          // joint target / speed / current (survives rebuilds)
@@ -145,7 +149,7 @@ struct rbgpu_world {
     uint32_t *plane_lid = nullptr, *pair_lid = nullptr;
     uint32_t *bb_hits = nullptr, *bb_blk = nullptr;   // box-box overlap lists (per 128-pair block)
     // hull tables on the device and GJK/EPA statistics per pair slot
-    float4 *hv = nullptr, *hpn = nullptr, *hamin = nullptr, *hamax = nullptr;
+    float4 *hv = nullptr, *hpn = nullptr, *hamin = nullptr, *hamax = nullptr, *hell = nullptr;
     uint32_t *htab = nullptr;
     HullMeta *hmeta = nullptr;
     HullSet hs{};
@@ -278,7 +282,7 @@ struct rbgpu_world {
         release_pairs();
         dfree(B.r); dfree(B.f); dfree(B.t); dfree(B.I0); dfree(B.I1);
         dfree(C.meta); dfree(C.dims); dfree(C.lpos); dfree(C.lrot); dfree(C.mat); dfree(C.hull);
-        dfree(hv); dfree(hpn); dfree(hamin); dfree(hamax); dfree(htab); dfree(hmeta);
+        dfree(hv); dfree(hpn); dfree(hamin); dfree(hamax); dfree(hell); dfree(htab); dfree(hmeta);
         hs = HullSet{};
         dfree(Jt.jm); dfree(Jt.jr1); dfree(Jt.jr2); dfree(Jt.jax); dfree(Jt.jlx); dfree(Jt.jtg); dfree(Jt.jrow);
         dfree(Jt.jlam); dfree(jorder); dfree(sj_order); dfree(sj_base); dfree(sj_coff);
@@ -782,7 +786,7 @@ int rbgpu_world::build() {
         chull[c] = d.shape == RBGPU_HULL ? d.hull_index : -1;
     }
     // hull tables, concatenated: vertices, polygon normals (rounded from double), index tables
-    std::vector<float4> hvv, hpnv, hminv, hmaxv;
+    std::vector<float4> hvv, hpnv, hminv, hmaxv, hellv;
     std::vector<uint32_t> htabv;
     std::vector<HullMeta> hmv;
     for (const HostHull &h : hh) {
@@ -804,6 +808,8 @@ int rbgpu_world::build() {
         hmv.push_back(mt);
         hminv.push_back(make_float4(h.amin[0], h.amin[1], h.amin[2], 0.0f));
         hmaxv.push_back(make_float4(h.amax[0], h.amax[1], h.amax[2], 0.0f));
+        hellv.push_back(make_float4(h.ec[0], h.ec[1], h.ec[2], 0.0f));
+        hellv.push_back(make_float4(h.ea[0], h.ea[1], h.ea[2], 0.0f));
     }
     // Plane loop of the narrowphase driver (CollisionPipeline.cpp:156-197): constant per scene;
     // a plane pairs with the colliders of its own scene.  plane_lid: index of the pair among
@@ -848,6 +854,7 @@ int rbgpu_world::build() {
     rc |= dalloc(C.hull, m);
     rc |= dalloc(hv, hvv.size()); rc |= dalloc(hpn, hpnv.size()); rc |= dalloc(htab, htabv.size());
     rc |= dalloc(hmeta, hmv.size()); rc |= dalloc(hamin, hminv.size()); rc |= dalloc(hamax, hmaxv.size());
+    rc |= dalloc(hell, hellv.size());
     rc |= dalloc(fmin, S); rc |= dalloc(fmax, S); rc |= dalloc(uid, S);
     rc |= dalloc(ep_key[0], NEP); rc |= dalloc(ep_key[1], NEP); rc |= dalloc(ep_val[0], NEP); rc |= dalloc(ep_val[1], NEP);
     rc |= dalloc(ep_pos, S); rc |= dalloc(sw_a, NEP); rc |= dalloc(sw_b, NEP);
@@ -884,8 +891,9 @@ int rbgpu_world::build() {
         HIPCHK(up(hmeta, hmv.data(), hmv.size() * sizeof(HullMeta)));
         HIPCHK(up(hamin, hminv.data(), hminv.size() * 16));
         HIPCHK(up(hamax, hmaxv.data(), hmaxv.size() * 16));
+        HIPCHK(up(hell, hellv.data(), hellv.size() * 16));
     }
-    hs = HullSet{hv, hpn, htab, hmeta, hamin, hamax};
+    hs = HullSet{hv, hpn, htab, hmeta, hamin, hamax, hell};
     hull_vpl = 1;
     for (const HostHull &h : hh)
         if (h.t.v.size() > 64) hull_vpl = HULL_VPL;
@@ -2131,6 +2139,29 @@ int rbgpu_add_hull(rbgpu_world *w, const double *vertices, uint32_t n_vertices, 
         for (int k = 0; k < 3; ++k) {
             h.amin[k] = c[k] < h.amin[k] ? c[k] : h.amin[k];
             h.amax[k] = h.amax[k] < c[k] ? c[k] : h.amax[k];
+        }
+    }
+    {   // bounding ellipsoid: the AABB's centre and half extents scaled to hold every vertex,
+        // widened by 1e-4 relative and 1e-6 absolute (far above the FP32 rounding of the bound)
+        double c[3], e[3], emax = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            c[k] = (double)(float)(0.5 * ((double)h.amin[k] + (double)h.amax[k]));   // the stored centre
+            e[k] = 0.5 * ((double)h.amax[k] - (double)h.amin[k]);
+            emax = std::max(emax, e[k]);
+        }
+        for (int k = 0; k < 3; ++k) e[k] = std::max(e[k], 1e-3 * emax + 1e-9);
+        double s2 = 0.0;
+        for (const auto &p : h.t.v) {
+            const double v[3] = {(double)(float)p.x, (double)(float)p.y, (double)(float)p.z};
+            double q = 0.0;
+            for (int k = 0; k < 3; ++k) q += ((v[k] - c[k]) / e[k]) * ((v[k] - c[k]) / e[k]);
+            s2 = std::max(s2, q);
+        }
+        const double sc = std::sqrt(s2);
+        for (int k = 0; k < 3; ++k) {
+            h.ec[k] = (float)c[k];
+            const double a = sc * e[k] * (1.0 + 1e-4) + 1e-6;
+            h.ea[k] = std::nextafter((float)a, 3.0e38f);
         }
     }
     rc = w->download_state();
