@@ -563,12 +563,13 @@ template <class SideT> __device__ __forceinline__ int gjk(const SideT &A, const 
 // the reference's would still grow, and the pair is run again with the full one.  Up to that
 // point the two runs are the same instructions on the same data.
 template <int NV, int NF> struct EpaLdsT {
+    static_assert(NF % 4 == 0, "the horizon scan reads the edge list four entries at a time");
     float wx[NV], wy[NV], wz[NV];     // polytope vertices (w)
     float ax[NV], ay[NV], az[NV];     // their A-side supports (w0)
     float nx[NF], ny[NF], nz[NF], nd[NF];
     uint32_t fv[NF];                  // v0 | v1 << 8 | v2 << 16 | obsolete << 24
     uint16_t vis[NF];                 // visible faces, face order
-    uint16_t seq[3 * NF];             // their directed edges (a | b << 8)
+    alignas(8) uint16_t seq[3 * NF];  // their directed edges (a | b << 8); read four at a time
     uint16_t hor[3 * NF];             // horizon edges, in order
 };
 using EpaLds = EpaLdsT<EPA_VERTS, EPA_FACES>;
@@ -640,21 +641,20 @@ template <int LPP, int NV, int NF> __device__ __forceinline__ int epa_expand(Epa
         }
         const uint64_t m = G::ballot(v);
         if (v) {
-            L.fv[f] |= 1u << 24;
-            L.vis[nvis + (int)G::rank(m)] = (uint16_t)f;
+            const uint32_t fv = L.fv[f];
+            L.fv[f] = fv | (1u << 24);
+            const int pos = nvis + (int)G::rank(m);
+            L.vis[pos] = (uint16_t)f;
+            // its directed edges (v0,v1) (v1,v2) (v2,v0), in visible-face order
+            const uint32_t a = fv & 255u, b = (fv >> 8) & 255u, c = (fv >> 16) & 255u;
+            L.seq[3 * pos] = (uint16_t)(a | (b << 8));
+            L.seq[3 * pos + 1] = (uint16_t)(b | (c << 8));
+            L.seq[3 * pos + 2] = (uint16_t)(c | (a << 8));
         }
         nvis += __popcll(m);
     }
     G::sync();
-    // directed edges (v0,v1) (v1,v2) (v2,v0) of each visible face
     const int ns = 3 * nvis;
-    for (int i = lane; i < ns; i += LPP) {
-        const uint32_t fv = L.fv[L.vis[i / 3]];
-        const int k = i % 3;
-        const uint32_t a = (fv >> (8 * k)) & 255u, b = (fv >> (8 * ((k + 1) % 3))) & 255u;
-        L.seq[i] = (uint16_t)(a | (b << 8));
-    }
-    G::sync();
     // horizon: odd occurrence count, kept at the last occurrence.  Every lane scans the whole
     // sequence (no early exit: the reads do not wait for one another)
     auto ukey = [](uint32_t e) {
@@ -668,11 +668,17 @@ template <int LPP, int NV, int NF> __device__ __forceinline__ int epa_expand(Epa
         if (i < ns) {
             const uint32_t key = ukey(L.seq[i]);
             int later = 0, before = 0;
-#pragma unroll 8
-            for (int j = 0; j < ns; ++j) {
-                const bool same = ukey(L.seq[j]) == key;
-                later += (int)(same & (j > i));
-                before += (int)(same & (j < i));
+            const uint64_t *sq = reinterpret_cast<const uint64_t *>(L.seq);   // 3 NF entries: a multiple of four
+#pragma unroll 4
+            for (int j4 = 0; j4 < ns; j4 += 4) {
+                const uint64_t q = sq[j4 >> 2];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int j = j4 + t;
+                    const bool same = j < ns && ukey((uint32_t)(q >> (16 * t)) & 0xffffu) == key;
+                    later += (int)(same & (j > i));
+                    before += (int)(same & (j < i));
+                }
             }
             keep = later == 0 && (before & 1) == 0;
         }

@@ -181,12 +181,17 @@ struct rbgpu_world {
     // 0: off; where the side stream forks: 1 before the GJK chain, 2 after its rejection tests
     // and compaction, 3 after the GJK pass (beside the EPA pass)
     int side_stream = 2;
-    std::function<int()> *fork_hook = nullptr;   // set by launch_narrowphase while launch_gjk runs
+    // The fork point: the main stream records the fork event there (the side stream waits on
+    // it); the side kernels themselves are queued once the whole GJK chain is, so that the
+    // host's launches of the side bins never hold back the main stream's next kernel.
+    bool fork_armed = false, fork_recorded = false;   // set by launch_narrowphase while launch_gjk runs
     int fork_at(int stage) {
-        if (!fork_hook || stage < side_stream) return RBGPU_OK;
-        std::function<int()> *f = fork_hook;
-        fork_hook = nullptr;
-        return (*f)();
+        if (!fork_armed || stage < side_stream) return RBGPU_OK;
+        fork_armed = false;
+        HIPCHK(hipEventRecord(ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
+        fork_recorded = true;
+        return RBGPU_OK;
     }
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -596,16 +601,18 @@ struct rbgpu_world {
     int launch_narrowphase(bool colour_reset = false) {
         const uint32_t g0 = bin_off[BIN_HH], ng = bin_off[BIN_HP] - bin_off[BIN_HH];
         const bool fork = side_stream && st2 && ng && bin_off[NBINS] - ng > 0;
-        std::function<int()> side = [&]() { return launch_side(fork, colour_reset); };
         if (ng) {
-            fork_hook = fork ? &side : nullptr;
+            fork_armed = fork;
+            fork_recorded = false;
             int rc = launch_gjk(g0, ng);
-            std::function<int()> *left = fork_hook;
-            fork_hook = nullptr;
-            const bool forked = fork && !left;   // the hook ran (and forked) inside the chain
-            if (!rc && left) rc = side();        // the chain ended before the fork point
-            if (forked || (fork && left && !rc)) {
-                // join once the whole chain is queued, also when the chain failed after the fork
+            const bool recorded = fork_recorded;   // the chain reached the fork point
+            fork_armed = fork_recorded = false;
+            if (fork && (!rc || recorded)) {
+                // the side bins once the whole chain is queued (behind the fork event, recorded at
+                // the fork point or, when the chain ended before it, now), also when the chain
+                // failed after the fork point; then the join
+                const int rs = launch_side(true, colour_reset, recorded);
+                if (!rc) rc = rs;
                 const hipError_t e1 = hipEventRecord(ev_join, st2);
                 const hipError_t e2 = hipStreamWaitEvent(st, ev_join, 0);
                 if (!rc && (e1 != hipSuccess || e2 != hipSuccess)) rc = fail(RBGPU_ERR_HIP, "side stream join failed");
@@ -616,8 +623,8 @@ struct rbgpu_world {
     }
     // The narrowphase bins other than GJK (and the colouring state's reset): on st2, forked
     // from the main stream (launch_narrowphase joins it back), or on the main stream.
-    int launch_side(bool fork, bool colour_reset) {
-        if (fork) {
+    int launch_side(bool fork, bool colour_reset, bool recorded = false) {
+        if (fork && !recorded) {
             HIPCHK(hipEventRecord(ev_fork, st));
             HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
         }
