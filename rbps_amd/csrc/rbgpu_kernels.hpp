@@ -337,7 +337,18 @@ struct ColourBufs {
     uint32_t *ctr;              // CTR_*
     const uint32_t *lid;        // local pair id per pair (the hashes' input)
     uint32_t pal_max;           // largest palette a pair draws its tentative colour from
+    uint32_t *epoch;            // per-scene path: substeps since the last full clear (claim keys)
 };
+// High word of a round's slot-claim key.  The grid-wide path clears the claims every substep
+// and uses r + 1; the per-scene path puts a per-substep epoch above it (stale claims of earlier
+// substeps are then smaller than any key of the running round, so only the used masks need a
+// clear, which each scene does for its own bodies).  Only keys of one round are ever compared,
+// so the outcome is the same either way.
+constexpr uint32_t COLOUR_EPOCH_SHIFT = 17;   // r + 1 <= 2^16 (the rounds cap)
+constexpr uint32_t COLOUR_EPOCH_MAX = (1u << (32 - COLOUR_EPOCH_SHIFT)) - 1u;
+__device__ __forceinline__ unsigned long long claim_key(uint32_t kh, uint32_t lo) {
+    return ((unsigned long long)kh << 32) | lo;
+}
 
 __device__ __forceinline__ uint32_t tent_hash(uint32_t p, uint32_t r) { return mix32(p ^ (r * 0x9e3779b9u)); }
 __device__ __forceinline__ uint32_t rank_hash(uint32_t p, uint32_t r) { return mix32(p ^ (r * 0x85ebca6bu) ^ 0xc2b2ae35u); }
@@ -383,6 +394,12 @@ __global__ void k_colour_reset(uint32_t N, ColourBufs cb) {
     if (i < N) cb.ucnt[i] = 0u;
     if (i < (uint32_t)MAX_COLOURS) cb.hist[i] = 0u;
     if (i < (uint32_t)CTR_SIZE) cb.ctr[i] = 0u;
+    if (i == 0 && cb.epoch) *cb.epoch = 1u;
+}
+// Per-scene path, the substeps after a full clear: the counters only, the next epoch.
+__global__ void __launch_bounds__(512) k_colour_ctr_reset(ColourBufs cb) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)CTR_SIZE; i += blockDim.x) cb.ctr[i] = 0u;
+    if (threadIdx.x == 0) *cb.epoch += 1u;
 }
 
 // Marks the pairs with points as open, counts them per dynamic body and per block.
@@ -421,7 +438,7 @@ __global__ void __launch_bounds__(256) k_colour_compact(uint32_t P, ColourBufs c
 }
 
 __device__ __forceinline__ void colour_claim(const ColourBufs &cb, const uint2 *pbod, const Bodies &B, uint32_t p,
-                                             uint32_t r) {
+                                             uint32_t r, uint32_t kh) {
     const uint2 bb = pbod[p];
     const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
     const uint32_t k = (da ? ld_shared(&cb.ucnt[bb.x]) : 0u) + (db ? ld_shared(&cb.ucnt[bb.y]) : 0u);
@@ -441,17 +458,17 @@ __device__ __forceinline__ void colour_claim(const ColourBufs &cb, const uint2 *
     }
     if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&cb.ctr[CTR_ERR], 1u); t = MAX_COLOURS - 1; }
     cb.tent[p] = t;
-    const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(q, r);
+    const unsigned long long key = claim_key(kh, rank_hash(q, r));
     if (da) atomicMax(&cb.claim[bb.x * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
     if (db) atomicMax(&cb.claim[bb.y * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
 }
 
 __device__ __forceinline__ bool colour_settle(const ColourBufs &cb, const uint2 *pbod, const Bodies &B, uint32_t p,
-                                              uint32_t r) {
+                                              uint32_t r, uint32_t kh) {
     const uint2 bb = pbod[p];
     const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
     const uint32_t t = cb.tent[p];
-    const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(cb.lid[p], r);
+    const unsigned long long key = claim_key(kh, rank_hash(cb.lid[p], r));
     if (da && ld_shared(&cb.claim[bb.x * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) return false;
     if (db && ld_shared(&cb.claim[bb.y * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) return false;
     cb.acol[p] = (int)t;
@@ -473,7 +490,7 @@ __global__ void __launch_bounds__(256) k_colour_claim(const uint2 *pbod, Bodies 
         const uint32_t p = cb.act[i];
         if (cb.acol[p] == COLOUR_OPEN) {
             any = true;
-            colour_claim(cb, pbod, B, p, r);
+            colour_claim(cb, pbod, B, p, r, r + 1u);
         }
     }
     if (__ballot(any) && (threadIdx.x & 63) == 0) cb.ctr[CTR_FLAGS + r] = 1u;
@@ -486,7 +503,7 @@ __global__ void __launch_bounds__(256) k_colour_settle(const uint2 *pbod, Bodies
     uint32_t won = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t p = cb.act[i];
-        if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r)) ++won;
+        if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r, r + 1u)) ++won;
     }
     for (int o = 32; o > 0; o >>= 1) won += __shfl_xor(won, o, 64);
     if (threadIdx.x == 0) sh = 0u;
@@ -522,13 +539,13 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_colour_tail(const uint2 *pbod,
         if (threadIdx.x == 0) won = 0u;
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
             const uint32_t p = cb.rest[j];
-            if (cb.acol[p] == COLOUR_OPEN) colour_claim(cb, pbod, B, p, r);
+            if (cb.acol[p] == COLOUR_OPEN) colour_claim(cb, pbod, B, p, r, r + 1u);
         }
         __syncthreads();
         uint32_t w = 0;
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
             const uint32_t p = cb.rest[j];
-            if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r)) ++w;
+            if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r, r + 1u)) ++w;
         }
         for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
         if ((threadIdx.x & 63) == 0 && w) atomicAdd(&won, w);
@@ -1171,9 +1188,11 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     __syncthreads();
     const uint32_t n = nact;
     const bool in_lds = n <= SCENE_LDS_PAIRS;
+    const uint32_t ep = cb.epoch ? *cb.epoch : 0u;   // 0: the claims were cleared this substep
     // rounds 0, 1, ... as the grid-wide path runs them
     uint32_t left = n, r = 0;
     for (; left > 0 && r < cap_rounds; ++r) {
+        const uint32_t kh = (ep << COLOUR_EPOCH_SHIFT) | (r + 1u);
         if (in_lds) {
             for (uint32_t j = tid; j < n; j += bd) {   // claim (colour_claim)
                 if (LA[j] != COLOUR_OPEN) continue;
@@ -1197,14 +1216,14 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
                 }
                 if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&cb.ctr[CTR_ERR], 1u); t = MAX_COLOURS - 1; }
                 LT[j] = (uint16_t)t;
-                const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(q, r);
+                const unsigned long long key = claim_key(kh, rank_hash(q, r));
                 if (da) atomicMax(&cb.claim[ba * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
                 if (db) atomicMax(&cb.claim[bb * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
             }
         } else {
             for (uint32_t j = tid; j < n; j += bd) {
                 const uint32_t p = G[j];
-                if (cb.acol[p] == COLOUR_OPEN) colour_claim(cb, pbod, B, p, r);
+                if (cb.acol[p] == COLOUR_OPEN) colour_claim(cb, pbod, B, p, r, kh);
             }
         }
         __syncthreads();
@@ -1216,7 +1235,7 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
                 const bool da = bw.x & SCENE_BDYN, db = bw.y & SCENE_BDYN;
                 const uint32_t ba = bw.x & ~SCENE_BDYN, bb = bw.y & ~SCENE_BDYN;
                 const uint32_t t = LT[j];
-                const unsigned long long key = ((unsigned long long)(r + 1u) << 32) | rank_hash(LQ[j], r);
+                const unsigned long long key = claim_key(kh, rank_hash(LQ[j], r));
                 if (da && ld_shared(&cb.claim[ba * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) continue;
                 if (db && ld_shared(&cb.claim[bb * COLOUR_SLOTS + t % COLOUR_SLOTS]) != key) continue;
                 LA[j] = (int)t;
@@ -1228,7 +1247,7 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         } else {
             for (uint32_t j = tid; j < n; j += bd) {
                 const uint32_t p = G[j];
-                if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r)) ++w;
+                if (cb.acol[p] == COLOUR_OPEN && colour_settle(cb, pbod, B, p, r, kh)) ++w;
             }
         }
         for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
@@ -1269,6 +1288,28 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     for (uint32_t j = tid; j < n; j += bd) {
         const int c = colour_of(j);
         if (c >= 0) out[atomicAdd(&hist[c], 1u)] = in_lds ? L[j] : G[j];
+    }
+    if (ep) {   // what this scene set: its bodies' used-mask words of its colours and their counts
+        const uint32_t nw = (top + 31u) / 32u;   // final since the barrier before the offsets
+        for (uint32_t j = tid; j < n; j += bd) {
+            uint32_t ba, bb;
+            bool da, db;
+            if (in_lds) {
+                const uint2 bw = LB[j];
+                da = bw.x & SCENE_BDYN; db = bw.y & SCENE_BDYN;
+                ba = bw.x & ~SCENE_BDYN; bb = bw.y & ~SCENE_BDYN;
+            } else {
+                const uint2 b2 = pbod[G[j]];
+                ba = b2.x; bb = b2.y;
+                da = is_dynamic(B, ba); db = is_dynamic(B, bb);
+            }
+            for (uint32_t w = 0; w < nw; ++w) {
+                if (da) cb.used[ba * COLOUR_WORDS + w] = 0u;
+                if (db) cb.used[bb * COLOUR_WORDS + w] = 0u;
+            }
+            if (da) cb.ucnt[ba] = 0u;
+            if (db) cb.ucnt[bb] = 0u;
+        }
     }
     if (tid == 0) {
         T.ncol[s] = top;

@@ -180,6 +180,11 @@ struct rbgpu_world {
     // priority, joined back by an event before the colouring)
     // 0: off; where the side stream forks: 1 before the GJK chain, 2 after its rejection tests
     // and compaction, 3 after the GJK pass (beside the EPA pass)
+    // Colouring state between substeps (per-scene path): clean after a frame whose scenes cleared
+    // what they set; colour_fast: this frame clears only the counters; colour_epochs: the claim
+    // epoch reached on the device since its last full clear.
+    bool colour_clean = false, colour_fast = false;
+    uint32_t colour_epochs = 0;
     int side_stream = 2;
     // The fork point: the main stream records the fork event there (the side stream waits on
     // it); the side kernels themselves are queued once the whole GJK chain is, so that the
@@ -295,7 +300,8 @@ struct rbgpu_world {
         dfree(fmin); dfree(fmax); dfree(uid); dfree(ep_key[0]); dfree(ep_key[1]); dfree(ep_val[0]); dfree(ep_val[1]);
         dfree(ep_pos); dfree(sw_a); dfree(sw_b); dfree(ep_cnt); dfree(ep_off); dfree(ep_hist);
         dfree(flags); dfree(scal); dfree(tile_sums);
-        dfree(cb.used); dfree(cb.ucnt); dfree(cb.claim); dfree(cb.hist); dfree(cb.off); dfree(cb.cur); dfree(cb.ctr);
+        dfree(cb.used); dfree(cb.ucnt); dfree(cb.claim); dfree(cb.hist); dfree(cb.off); dfree(cb.cur); dfree(cb.ctr); dfree(cb.epoch);
+        colour_clean = false;
         dfree(plane_pairs); dfree(offs); dfree(d_points);
         dfree(ax_keys); dfree(ax_vals);
         ax_cap = 0;
@@ -569,6 +575,7 @@ struct rbgpu_world {
         const uint32_t e = rb->scene_err_tail;
         rb->scene_err_tail = 0;
         if (!e) return RBGPU_OK;
+        colour_clean = false;
         HIPCHK(hipMemsetAsync(sc_err, 0, 4, st));
         HIPCHK(hipStreamSynchronize(st));
         if (e & 1u) return fail(RBGPU_ERR_CAPACITY, "more than 1024 colours needed (a frame of the last step)");
@@ -629,7 +636,9 @@ struct rbgpu_world {
             HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
         }
         const hipStream_t so = fork ? st2 : st;
-        if (colour_reset)
+        if (colour_reset && colour_fast)
+            k_colour_ctr_reset<<<1, 512, 0, so>>>(cb);
+        else if (colour_reset)
             k_colour_reset<<<nblk(std::max<uint32_t>(N * (COLOUR_WORDS / 4), MAX_COLOURS)), TPB, 0, so>>>(N, cb);
         for (int b = 0; b < NBINS; ++b) {
             uint32_t o = bin_off[b], n = bin_off[b + 1] - bin_off[b];
@@ -872,6 +881,7 @@ int rbgpu_world::build() {
     rc |= dalloc(cb.used, (size_t)n * COLOUR_WORDS); rc |= dalloc(cb.ucnt, n);
     rc |= dalloc(cb.claim, (size_t)n * COLOUR_SLOTS); rc |= dalloc(cb.hist, MAX_COLOURS);
     rc |= dalloc(cb.off, MAX_COLOURS + 1); rc |= dalloc(cb.cur, MAX_COLOURS); rc |= dalloc(cb.ctr, CTR_SIZE);
+    rc |= dalloc(cb.epoch, 1);
     rc |= dalloc(plane_pairs, planes.size()); rc |= dalloc(offs, NBINS + 4);
     rc |= dalloc(cscene, m); rc |= dalloc(slot_scene, S); rc |= dalloc(sap_start, nsc); rc |= dalloc(sap_end, nsc);
     rc |= dalloc(plane_lid, planes.size());
@@ -1154,6 +1164,7 @@ int rbgpu_world::frame() {
         if (sc_err && rb->scene_err) {
             const uint32_t e = rb->scene_err;
             rb->scene_err = 0;
+            colour_clean = false;
             HIPCHK(hipMemsetAsync(sc_err, 0, 4, st));
             HIPCHK(hipStreamSynchronize(st));
             if (e & 1u) return fail(RBGPU_ERR_CAPACITY, "more than 1024 colours needed");
@@ -1192,6 +1203,7 @@ int rbgpu_world::frame() {
     const bool scene_joints = by_scene && P && NJ && sj_order;
     SceneTabs TJ = T;
     if (!scene_joints) TJ.jorder = nullptr;
+    colour_fast = by_scene && colour_clean;   // this frame's colouring-state clear: counters only
     auto substep = [&]() -> int {
         runs.clear();
         if (P) {
@@ -1332,6 +1344,15 @@ int rbgpu_world::frame() {
             int rc = substep();
             if (rc) return rc;
         }
+    }
+    // colouring state for the next frame: after per-scene frames only the counters need a
+    // clear (the claims' epoch rises by one per substep); a full clear before the epoch wraps
+    // and after the grid-wide path (its claims carry no epoch, its masks are not cleared)
+    if (P && by_scene) {
+        colour_epochs = colour_fast ? colour_epochs + (uint32_t)substeps : 1u;
+        colour_clean = colour_epochs + 2u * (uint32_t)substeps < COLOUR_EPOCH_MAX;
+    } else if (P) {
+        colour_clean = false;
     }
     ++frames;
     return RBGPU_OK;
