@@ -30,6 +30,7 @@ constexpr int HULL_VPL = HULL_MAX_VERTS / 64;     // vertices per lane (kernels 
 constexpr int HULL_MAX_POLY = 64;                 // vertices of one polygon, neighbours of one polygon
 constexpr int EPA_FACES = 256, EPA_VERTS = 128, EPA_ITERS = 128, GJK_ITERS = 128;
 constexpr int GJK_WARM_ITERS = 16;   // a warm-started GJK not done by then falls back to the plain seed
+constexpr int GJK_CYCLE_FROM = 8;    // first iteration whose loop state the cycle detection saves
 constexpr int EPA_SEQ = 3 * EPA_FACES;            // directed edges of the visible faces
 constexpr int HCLIP_CAP = 2 * HULL_MAX_POLY + 8;   // Sutherland-Hodgman output bound (n + planes)
 constexpr int HP_MAX_POINTS = 32;                 // hull-plane candidates (PlaneCollision.hpp:118)
@@ -461,7 +462,8 @@ __device__ __forceinline__ bool sx_tetra(Simplex &s, V3 &dir) {   // Tetrahedron
 // loop state -- rank, the simplex points w below the rank and the search direction (the
 // A-side points w0 ride along and decide nothing) -- so a state equal to an earlier one
 // repeats forever and the loop can only end at its cap, GJK_FAILED.  One earlier state is
-// kept, re-saved at iterations 1, 2, 4, 8, ...; a match ends the loop with the status and
+// kept, re-saved at the 1st, 2nd, 4th, 8th, ... call (calls from iteration GJK_CYCLE_FROM on);
+// a match ends the loop with the status and
 // iteration count the full run reports.  (In FP32, GJK on touching hulls cycles with
 // periods of 4-15 iterations from about its tenth: such pairs used the full 128.)
 // The state is 16 words: 0 rank, 1-12 the points, 13-15 the direction.  Where a pair spans
@@ -549,7 +551,10 @@ template <class SideT> __device__ __forceinline__ int gjk(const SideT &A, const 
         else if (s.rank == 3) sx_triangle(s, dir);
         else inside = sx_tetra(s, dir);
         if (inside) return GJK_INSIDE;
-        if (cyc.repeats(s, dir)) {   // the remaining iterations would only go round the cycle
+        // the remaining iterations would only go round the cycle; the detection starts at
+        // iteration GJK_CYCLE_FROM (a state that repeats from an earlier iteration keeps
+        // repeating from there, and most pairs end before it)
+        if (it >= GJK_CYCLE_FROM && cyc.repeats(s, dir)) {
             iters = max_iter;
             return GJK_FAILED;
         }
