@@ -607,11 +607,18 @@ template <class L> __device__ __forceinline__ bool epa_face_normal(const L &l, i
         n = -n;
         d = -d;
     } else if (d <= 1e-12f) {
+        // the first vertex off the plane decides (reads four vertices ahead, in order)
         bool fixed = false;
-        for (int i = 0; i < nall; ++i) {
-            const float t = dot(n, lds_w(l, i));
-            if (t > 1e-12f) { n = -n; d = 0.0f; fixed = true; break; }
-            if (t < -1e-12f) { d = 0.0f; fixed = true; break; }
+        for (int i0 = 0; i0 < nall && !fixed; i0 += 4) {
+            float t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) t[u] = i0 + u < nall ? dot(n, lds_w(l, i0 + u)) : 0.0f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (fixed) break;
+                if (t[u] > 1e-12f) { n = -n; d = 0.0f; fixed = true; }
+                else if (t[u] < -1e-12f) { d = 0.0f; fixed = true; }
+            }
         }
         if (!fixed) return false;
     }
@@ -623,7 +630,8 @@ template <class L> __device__ __forceinline__ bool epa_face_normal(const L &l, i
 // Adds w to the polytope (EPA.cpp:337-374): faces visible from w become obsolete and
 // the horizon of the visible region is stitched to w.  Returns EPA_OUT_OF_FACES when
 // the face pool runs out (EPA_DEFER for a pool smaller than the reference's), else 0.
-template <int LPP, int NV, int NF> __device__ __forceinline__ int epa_expand(EpaLdsT<NV, NF> &L, const SV &w, int &nvtx, int &nf) {
+template <int LPP, int NV, int NF>
+__device__ __forceinline__ int epa_expand(EpaLdsT<NV, NF> &L, const SV &w, int &nvtx, int &nf, int &nf_tot) {
     using G = Grp<LPP>;
     const int lane = G::lane();
     const int wi = nvtx;
@@ -632,21 +640,36 @@ template <int LPP, int NV, int NF> __device__ __forceinline__ int epa_expand(Epa
         L.ax[wi] = w.w0.x; L.ay[wi] = w.w0.y; L.az[wi] = w.w0.z;
     }
     ++nvtx;
-    // 4. visible faces, listed in face order
-    int nvis = 0;
-    for (int base = 0; base < nf; base += LPP) {
-        const int f = base + lane;
-        bool v = false;
-        if (f < nf && !(L.fv[f] >> 24)) {
+    // 4. visible faces, listed in face order: every face of this lane tested first (the reads
+    // need not wait for the list's writes), then listed
+    constexpr int FPL = (NF + LPP - 1) / LPP;   // faces per lane
+    bool vf[FPL];
+    uint32_t fvr[FPL];
+#pragma unroll
+    for (int u = 0; u < FPL; ++u) {
+        const int f = u * LPP + lane;
+        vf[u] = false;
+        fvr[u] = 0u;
+        if (f < nf) {
             const uint32_t fv = L.fv[f];
-            const V3 cen = ((lds_w(L, (int)(fv & 255u)) + lds_w(L, (int)((fv >> 8) & 255u))) +
-                            lds_w(L, (int)((fv >> 16) & 255u))) * (1.0f / 3.0f);
-            const V3 fn{L.nx[f], L.ny[f], L.nz[f]};
-            v = dot(fn, w.w - cen) > 0.0f;
+            fvr[u] = fv;
+            if (!(fv >> 24)) {
+                const V3 cen = ((lds_w(L, (int)(fv & 255u)) + lds_w(L, (int)((fv >> 8) & 255u))) +
+                                lds_w(L, (int)((fv >> 16) & 255u))) * (1.0f / 3.0f);
+                const V3 fn{L.nx[f], L.ny[f], L.nz[f]};
+                vf[u] = dot(fn, w.w - cen) > 0.0f;
+            }
         }
+    }
+    int nvis = 0;
+#pragma unroll
+    for (int u = 0; u < FPL; ++u) {
+        if (u * LPP >= nf) break;   // uniform over the group
+        const int f = u * LPP + lane;
+        const bool v = vf[u];
         const uint64_t m = G::ballot(v);
         if (v) {
-            const uint32_t fv = L.fv[f];
+            const uint32_t fv = fvr[u];
             L.fv[f] = fv | (1u << 24);
             const int pos = nvis + (int)G::rank(m);
             L.vis[pos] = (uint16_t)f;
@@ -692,6 +715,37 @@ template <int LPP, int NV, int NF> __device__ __forceinline__ int epa_expand(Epa
         nh += __popcll(m);
     }
     G::sync();
+    // A pool smaller than the reference's makes room when the new faces might not fit: the live
+    // faces move down, in order (the closest-face ties and the visible-face lists depend only on
+    // the live faces' order); the reference's own pool, which keeps obsolete faces, is counted
+    // in nf_tot.
+    if constexpr (NF < EPA_FACES) {
+        if (nf + nh > NF) {   // uniform over the group
+            float rx[FPL], ry[FPL], rz[FPL], rd[FPL];
+            uint32_t rf[FPL];
+            int rp[FPL];
+            int live = 0;
+#pragma unroll
+            for (int u = 0; u < FPL; ++u) {
+                const int f = u * LPP + lane;
+                const bool lv = f < nf && !(L.fv[f] >> 24);
+                const uint64_t m = G::ballot(lv);
+                rp[u] = lv ? live + (int)G::rank(m) : -1;
+                live += __popcll(m);
+                rx[u] = lv ? L.nx[f] : 0.0f; ry[u] = lv ? L.ny[f] : 0.0f; rz[u] = lv ? L.nz[f] : 0.0f;
+                rd[u] = lv ? L.nd[f] : 0.0f; rf[u] = lv ? L.fv[f] : 0u;
+            }
+            G::sync();
+#pragma unroll
+            for (int u = 0; u < FPL; ++u)
+                if (rp[u] >= 0) {
+                    L.nx[rp[u]] = rx[u]; L.ny[rp[u]] = ry[u]; L.nz[rp[u]] = rz[u]; L.nd[rp[u]] = rd[u];
+                    L.fv[rp[u]] = rf[u];
+                }
+            nf = live;
+            G::sync();
+        }
+    }
     // stitch: one candidate face per horizon edge, packed in horizon order
     int made_before_last = 0, made = 0;
     for (int base = 0; base < nh; base += LPP) {
@@ -713,10 +767,13 @@ template <int LPP, int NV, int NF> __device__ __forceinline__ int epa_expand(Epa
         }
         made += __popcll(m);
     }
-    // the reference checks the pool before each new face, failing ones included
-    if (nh > 0 && nf + (int)G::sum((uint32_t)made_before_last) >= NF)
-        return NF < EPA_FACES ? EPA_DEFER : EPA_OUT_OF_FACES;
+    // the reference checks its pool (every face made) before each new face, failing ones
+    // included; a smaller pool hands the pair over where its live faces would not fit
+    const int mb = (int)G::sum((uint32_t)made_before_last);
+    if (nh > 0 && nf_tot + mb >= EPA_FACES) return EPA_OUT_OF_FACES;
+    if (NF < EPA_FACES && nh > 0 && nf + mb >= NF) return EPA_DEFER;
     nf += made;
+    nf_tot += made;
     G::sync();
     return 0;
 }
@@ -785,7 +842,7 @@ __device__ __forceinline__ EpaResult epa(const Simplex &seed, const SideT &A, co
         }
     }
     G::sync();
-    int nvtx = 4, nf = 0;
+    int nvtx = 4, nf = 0, nf_tot = 0;
     {   // seed faces ABC, ACD, ADB, BCD (every lane evaluates them alike)
         const int fa[4] = {0, 0, 0, 1}, fb[4] = {1, 2, 3, 2}, fc[4] = {2, 3, 1, 3};
         for (int f = 0; f < 4; ++f) {
@@ -797,7 +854,7 @@ __device__ __forceinline__ EpaResult epa(const Simplex &seed, const SideT &A, co
                 L.fv[f] = (uint32_t)fa[f] | ((uint32_t)fb[f] << 8) | ((uint32_t)fc[f] << 16);
             }
         }
-        nf = 4;
+        nf = nf_tot = 4;
     }
     G::sync();
     int it0 = 0;
@@ -815,7 +872,7 @@ __device__ __forceinline__ EpaResult epa(const Simplex &seed, const SideT &A, co
             seen = dot(V3{L.nx[lane], L.ny[lane], L.nz[lane]}, w.w - cen) > 0.0f;
         }
         if (!G::ballot(dup) && G::ballot(seen)) {
-            const int rc = epa_expand<LPP>(L, w, nvtx, nf);
+            const int rc = epa_expand<LPP>(L, w, nvtx, nf, nf_tot);
             if (rc) {
                 r.iters = 1;
                 r.status = rc;
@@ -877,7 +934,7 @@ __device__ __forceinline__ EpaResult epa(const Simplex &seed, const SideT &A, co
             return r;
         }
         if (nvtx >= NV) { r.status = NV < EPA_VERTS ? EPA_DEFER : EPA_OUT_OF_VERTICES; return r; }
-        if (const int rc = epa_expand<LPP>(L, w, nvtx, nf)) { r.status = rc; return r; }
+        if (const int rc = epa_expand<LPP>(L, w, nvtx, nf, nf_tot)) { r.status = rc; return r; }
     }
     r.status = EPA_FAILED;
     return r;
