@@ -287,11 +287,11 @@ int rbgpu_get_stats(rbgpu_world *w, rbgpu_stats *out);
  *                          across the frame's new pair list, keyed by collider pair (1,
  *                          default; 0: every pair starts a frame without a hint).  Results do
  *                          not change.
- *   RBGPU_OPT_GJK_SORT     1: order the GJK candidates by the iterations their slot took in the
- *                          previous substep, so the four pairs of a GJK wavefront take similar
- *                          numbers of iterations; 0 (default): bin order, which measured faster
- *                          (288 vs 310 us per GJK pass on the combined workload).  Results do not
- *                          change.
+ *   RBGPU_OPT_GJK_SORT     order of the GJK candidates, from their slots' previous substep:
+ *                          0 (default) bin order; 1 by GJK iterations, so the four pairs of a
+ *                          GJK wavefront take similar numbers of iterations; 2 overlapping pairs
+ *                          after separated ones (stable), so the rows of a wavefront mostly all
+ *                          run EPA or none does; 3 both.  Results do not change.
  *   RBGPU_OPT_EPA_GUARD    0 (default): EPA depths and normals as the reference computes them.
  *                          Any direction n bounds the penetration depth from above by the
  *                          shapes' overlap along n (the Minkowski difference's support along

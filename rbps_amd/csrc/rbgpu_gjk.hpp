@@ -2095,17 +2095,20 @@ __global__ void k_gjk_compact(const uint32_t *keep, const uint32_t *pos, uint32_
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && keep[i]) cand[pos[i]] = i;
 }
-// The same compaction with a sort key per candidate: the GJK iterations its slot took in the
-// previous substep (capped at 15; a frame's first substep reads the slots' old contents).
-// Sorted by this key, the four pairs of a GJK-pass wavefront need similar iteration counts,
-// so fewer rows idle under the slowest one.  The order of the candidates changes no result.
+// The same compaction with a sort key per candidate from its slot's previous substep (a
+// frame's first substep reads the slots' old contents): mode 1 the GJK iterations (capped at
+// 15), so the four pairs of a wavefront need similar iteration counts; mode 2 whether GJK
+// found the pair overlapping, so the rows of a wavefront mostly all run EPA or none does;
+// mode 3 both.  The order of the candidates changes no result.
 __global__ void k_gjk_compact_keyed(const uint32_t *keep, const uint32_t *pos, uint32_t n, const uint32_t *bin_slot,
-                                    uint32_t off, const uint32_t *gstat, uint32_t *cand, uint32_t *key) {
+                                    uint32_t off, const uint32_t *gstat, uint32_t *cand, uint32_t *key, int mode) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && keep[i]) {
         const uint32_t p = pos[i];
+        const uint32_t g = gstat[bin_slot[off + i]];
+        const uint32_t in = ((g >> 16) & 15u) == (uint32_t)GJK_INSIDE ? 1u : 0u;
         cand[p] = i;
-        key[p] = min(gstat[bin_slot[off + i]] & 255u, 15u);
+        key[p] = mode == 1 ? min(g & 255u, 15u) : mode == 2 ? in : (in << 4) | min(g & 255u, 15u);
     }
 }
 

@@ -382,9 +382,10 @@ __global__ void __launch_bounds__(256) k_joint_damp(Bodies B, Joints J, const ui
 // SOLVE_LDS_PAIRS pairs, SOLVE_LDS_JOINTS joints; larger scenes read them from the global
 // lists), so that each colour step starts with the contact and body loads.
 constexpr uint32_t SOLVE_LDS_PAIRS = 2048, SOLVE_LDS_JOINTS = 1024, SOLVE_LDS_COLOURS = 256, SOLVE_LDS_JCOLS = 64;
-template <bool POS, int OCC = 1>
+template <bool POS, int OCC = 1, bool ALL = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pts, Joints J, float inv_h, int jmode) {
+    static_assert(POS || !ALL, "the whole substep solve starts with the positions");
     __shared__ uint32_t s_co[SOLVE_LDS_COLOURS + 1], s_lst[SOLVE_LDS_PAIRS];
     __shared__ uint32_t s_jc[SOLVE_LDS_JCOLS + 1], s_jo[SOLVE_LDS_JOINTS];
     const uint32_t s = blockIdx.x;
@@ -405,33 +406,46 @@ k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pt
     const bool ls_l = np <= SOLVE_LDS_PAIRS;
     for (uint32_t i = threadIdx.x; ls_l && i < np; i += blockDim.x) s_lst[i] = lst[i];
     __syncthreads();
-    for (uint32_t c = 0; c < nc; ++c) {
-        const uint32_t o = co_l ? s_co[c] : co[c], e = co_l ? s_co[c + 1] : co[c + 1];
-        for (uint32_t i0 = o; i0 < e; i0 += blockDim.x) {
-            const uint32_t i = i0 + threadIdx.x;
-            const bool v = i < e;
-            const uint32_t p = v ? (ls_l ? s_lst[i] : lst[i]) : 0u;
-            if (pts) count_points(K, p, v, pts);
-            if (v) {
-                if (POS) solve_pos_pair(B, K, p);
-                else solve_vel_pair(B, K, p, h);
-            }
-        }
-        __syncthreads();
-    }
-    if (!jn || j0 == j1) return;
     auto jorder = [&](uint32_t i) { return jo_l ? s_jo[i - j0] : T.jorder[i]; };
-    if (POS) {
-        for (uint32_t i = j0 + threadIdx.x; i < j1; i += blockDim.x) joint_errors(B, J, jorder(i), h, jmode);
-        __syncthreads();
-    }
-    for (uint32_t c = 0; c < T.njc; ++c) {
-        const uint32_t e = j0 + (jc_l ? s_jc[c + 1] : jc[c + 1]);
-        for (uint32_t i = j0 + (jc_l ? s_jc[c] : jc[c]) + threadIdx.x; i < e; i += blockDim.x) {
-            if (POS) joint_solve(B, J, jorder(i), inv_h, h, jmode);
-            else joint_damp(B, J, jorder(i), h);
+    // one solve: the contacts colour by colour, then the scene's joints (errors and rows after
+    // the position solve, damping after the velocity solve)
+    auto solve = [&](bool pos) {
+        for (uint32_t c = 0; c < nc; ++c) {
+            const uint32_t o = co_l ? s_co[c] : co[c], e = co_l ? s_co[c + 1] : co[c + 1];
+            for (uint32_t i0 = o; i0 < e; i0 += blockDim.x) {
+                const uint32_t i = i0 + threadIdx.x;
+                const bool v = i < e;
+                const uint32_t p = v ? (ls_l ? s_lst[i] : lst[i]) : 0u;
+                if (pts && pos == POS) count_points(K, p, v, pts);
+                if (v) {
+                    if (pos) solve_pos_pair(B, K, p);
+                    else solve_vel_pair(B, K, p, h);
+                }
+            }
+            __syncthreads();
         }
+        if (!jn || j0 == j1) return;
+        if (pos) {
+            for (uint32_t i = j0 + threadIdx.x; i < j1; i += blockDim.x) joint_errors(B, J, jorder(i), h, jmode);
+            __syncthreads();
+        }
+        for (uint32_t c = 0; c < T.njc; ++c) {
+            const uint32_t e = j0 + (jc_l ? s_jc[c + 1] : jc[c + 1]);
+            for (uint32_t i = j0 + (jc_l ? s_jc[c] : jc[c]) + threadIdx.x; i < e; i += blockDim.x) {
+                if (pos) joint_solve(B, J, jorder(i), inv_h, h, jmode);
+                else joint_damp(B, J, jorder(i), h);
+            }
+            __syncthreads();
+        }
+    };
+    solve(POS);
+    if (ALL) {
+        // the rest of the substep for this scene (World.hpp:203-211): velocities from the
+        // solved positions (k_update_vel on the scene's bodies), then the velocity solve
+        const uint32_t b1 = T.body_off[s + 1];
+        for (uint32_t i = T.body_off[s] + threadIdx.x; i < b1; i += blockDim.x) update_vel_body(B, T.body_idx[i], inv_h);
         __syncthreads();
+        solve(false);
     }
 }
 

@@ -795,9 +795,7 @@ __global__ void k_collide_pairs(uint32_t n, const int *kind_a, const float4 *dim
 }
 
 // ------------------------------------------------------------------ bodies (Body.cpp)
-__global__ void k_integrate(Bodies B, uint32_t N, float dt) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+__device__ __forceinline__ void integrate_body(const Bodies &B, uint32_t i, float dt) {
     BodyRow &b = B.r[i];
     float4 ii1 = b.ii1;
     if (ii1.z == 0.0f) return;   // static
@@ -820,9 +818,12 @@ __global__ void k_integrate(Bodies B, uint32_t N, float dt) {
     b.w = f4(w, 0.0f);
 }
 
-__global__ void k_update_vel(Bodies B, uint32_t N, float inv_dt) {
+__global__ void k_integrate(Bodies B, uint32_t N, float dt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    if (i < N) integrate_body(B, i, dt);
+}
+
+__device__ __forceinline__ void update_vel_body(const Bodies &B, uint32_t i, float inv_dt) {
     BodyRow &b = B.r[i];
     if (b.ii1.z == 0.0f) return;
     V3 x = xyz(b.x), px = xyz(b.px);
@@ -831,6 +832,11 @@ __global__ void k_update_vel(Bodies B, uint32_t N, float inv_dt) {
     V3 w = (2.0f * inv_dt) * V3{dq.x, dq.y, dq.z};
     b.v = f4(v, 0.0f);
     b.w = f4(dq.w >= 0.0f ? w : -w, 0.0f);
+}
+
+__global__ void k_update_vel(Bodies B, uint32_t N, float inv_dt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) update_vel_body(B, i, inv_dt);
 }
 
 // ------------------------------------------------------------------ contact solve (Contact.cpp)
@@ -1101,6 +1107,10 @@ struct SceneTabs {
     // jorder[jbase[s] ..) the scene's joints by joint colour, jcoff[s * (njc + 1) + c] offsets
     const uint32_t *jorder, *jbase, *jcoff;
     uint32_t njc;
+    // the scene's bodies: body_idx[body_off[s] ..) in ascending order; body_loc: a body's
+    // index in its scene's list
+    const uint32_t *body_off, *body_idx, *body_loc;
+    const uint32_t *corder;   // the colouring's scene per workgroup (largest scenes first)
 };
 
 // Pairs per scene (SAP run + plane pairs), for the slice offsets.
@@ -1115,17 +1125,29 @@ __global__ void k_scene_sizes(SceneTabs T, uint32_t nsc, uint32_t *size) {
 // two loads in place of six); larger scenes read the same fields from the global arrays.
 // Either way the rounds, keys, palettes and masks are those of colour_claim / colour_settle.
 constexpr uint32_t SCENE_BDYN = 0x80000000u;   // body word: index | dynamic flag
+constexpr uint32_t SCENE_LDYN = 0x8000u;       // local body half-word: local id | dynamic flag
+constexpr uint32_t SCENE_LDS_BODIES = 512, SCENE_LDS_UWORDS = 2;
+static_assert(SCENE_LDS_BODIES * COLOUR_SLOTS >= (uint32_t)MAX_COLOURS, "the histogram lives in the claim table");
+static_assert(SCENE_LDS_BODIES < SCENE_LDYN, "local ids fit in 15 bits");
 __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, const uint2 *pbod, const uint32_t *n_sap_dev,
-                                                                Bodies B, ColourBufs cb, SceneTabs T, uint32_t cap_rounds) {
+                                                                Bodies B, ColourBufs cb, SceneTabs T, uint32_t cap_rounds,
+                                                                uint32_t n_int, float dt) {
     __shared__ uint32_t L[SCENE_LDS_PAIRS];    // pair id
     __shared__ uint2 LB[SCENE_LDS_PAIRS];      // bodies | SCENE_BDYN
     __shared__ uint32_t LQ[SCENE_LDS_PAIRS];   // local pair id (the hashes' input)
     __shared__ int LA[SCENE_LDS_PAIRS];        // colour, COLOUR_OPEN
     __shared__ uint16_t LT[SCENE_LDS_PAIRS];   // tentative colour of the round
-    __shared__ uint32_t hist[MAX_COLOURS];
+    __shared__ uint32_t LL[SCENE_LDS_PAIRS];   // bodies' local ids | SCENE_LDYN, a in the low half
+    // per local body (scenes of up to SCENE_LDS_BODIES bodies): the round's slot claims (the
+    // keys' low words; the bucketing's histogram after the rounds), uncoloured counts and
+    // colours 0 .. 32 SCENE_LDS_UWORDS - 1 of the used masks (higher colours stay global)
+    __shared__ uint32_t LC[SCENE_LDS_BODIES * COLOUR_SLOTS];
+    __shared__ uint32_t LU[SCENE_LDS_BODIES];
+    __shared__ uint32_t LW[SCENE_LDS_BODIES * SCENE_LDS_UWORDS];
+    uint32_t *const hist = LC;
     __shared__ uint32_t sh[SCENE_THREADS / 64 + 1];
     __shared__ uint32_t nact, won, top;
-    const uint32_t s = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
+    const uint32_t s = T.corder ? T.corder[blockIdx.x] : blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
     const uint32_t ns = *n_sap_dev;
     const uint32_t a0 = T.sap_start[s], na = T.sap_end[s] - a0;
     const uint32_t q0 = T.plane_off[s], total = na + (T.plane_off[s + 1] - q0);
@@ -1168,32 +1190,82 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         }
     }
     __syncthreads();
-    {
-        const uint32_t na_ = nact;
-        for (uint32_t j = tid; j < na_; j += bd) {
-            const uint32_t p = j < SCENE_LDS_PAIRS ? L[j] : G[j];
-            const uint2 bb = pbod[p];
-            const uint32_t q = cb.lid[p];
-            const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
-            cb.acol[p] = COLOUR_OPEN;
+    const uint32_t n = nact;
+    const bool in_lds = n <= SCENE_LDS_PAIRS;
+    // the bodies' colouring state in LDS when the pairs are there and the scene is small enough
+    const uint32_t nbs = T.body_loc ? T.body_off[s + 1] - T.body_off[s] : 0u;
+    const bool lbod = in_lds && T.body_loc && nbs <= SCENE_LDS_BODIES;
+    if (lbod) {
+        for (uint32_t i = tid; i < nbs; i += bd) LU[i] = 0u;
+        for (uint32_t i = tid; i < nbs * SCENE_LDS_UWORDS; i += bd) LW[i] = 0u;
+        uint4 *c4 = reinterpret_cast<uint4 *>(LC);
+        for (uint32_t i = tid; i < nbs * (COLOUR_SLOTS / 4); i += bd) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+    }
+    for (uint32_t j = tid; j < n; j += bd) {
+        const uint32_t p = j < SCENE_LDS_PAIRS ? L[j] : G[j];
+        const uint2 bb = pbod[p];
+        const uint32_t q = cb.lid[p];
+        const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
+        cb.acol[p] = COLOUR_OPEN;
+        if (lbod) {   // every body of a scene's pair is in the scene
+            const uint32_t la = T.body_loc[bb.x], lb = T.body_loc[bb.y];
+            if (da) atomicAdd(&LU[la], 1u);
+            if (db) atomicAdd(&LU[lb], 1u);
+            LL[j] = (la | (da ? SCENE_LDYN : 0u)) | ((lb | (db ? SCENE_LDYN : 0u)) << 16);
+        } else {
             if (da) atomicAdd(&cb.ucnt[bb.x], 1u);
             if (db) atomicAdd(&cb.ucnt[bb.y], 1u);
-            if (j < SCENE_LDS_PAIRS) {
-                LB[j] = make_uint2(bb.x | (da ? SCENE_BDYN : 0u), bb.y | (db ? SCENE_BDYN : 0u));
-                LQ[j] = q;
-                LA[j] = COLOUR_OPEN;
-            }
+        }
+        if (j < SCENE_LDS_PAIRS) {
+            LB[j] = make_uint2(bb.x | (da ? SCENE_BDYN : 0u), bb.y | (db ? SCENE_BDYN : 0u));
+            LQ[j] = q;
+            LA[j] = COLOUR_OPEN;
         }
     }
     __syncthreads();
-    const uint32_t n = nact;
-    const bool in_lds = n <= SCENE_LDS_PAIRS;
     const uint32_t ep = cb.epoch ? *cb.epoch : 0u;   // 0: the claims were cleared this substep
     // rounds 0, 1, ... as the grid-wide path runs them
     uint32_t left = n, r = 0;
     for (; left > 0 && r < cap_rounds; ++r) {
         const uint32_t kh = (ep << COLOUR_EPOCH_SHIFT) | (r + 1u);
-        if (in_lds) {
+        if (lbod) {
+            for (uint32_t j = tid; j < n; j += bd) {   // claim (colour_claim on the LDS state)
+                if (LA[j] != COLOUR_OPEN) continue;
+                const uint32_t lw = LL[j];
+                const bool da = lw & SCENE_LDYN, db = (lw >> 16) & SCENE_LDYN;
+                const uint32_t la = lw & (SCENE_LDYN - 1u), lb = (lw >> 16) & (SCENE_LDYN - 1u);
+                const uint32_t k = (da ? LU[la] : 0u) + (db ? LU[lb] : 0u);
+                const uint32_t q = LQ[j];
+                uint32_t idx = tent_hash(q, r) % palette(k, cb.pal_max);
+                uint32_t t = MAX_COLOURS;
+                for (uint32_t w = 0; w < COLOUR_WORDS; ++w) {
+                    uint32_t ua = 0u, ub = 0u;
+                    if (w < SCENE_LDS_UWORDS) {
+                        ua = da ? LW[la * SCENE_LDS_UWORDS + w] : 0u;
+                        ub = db ? LW[lb * SCENE_LDS_UWORDS + w] : 0u;
+                    } else {
+                        const uint2 bw = LB[j];
+                        ua = da ? ld_shared(&cb.used[(bw.x & ~SCENE_BDYN) * COLOUR_WORDS + w]) : 0u;
+                        ub = db ? ld_shared(&cb.used[(bw.y & ~SCENE_BDYN) * COLOUR_WORDS + w]) : 0u;
+                    }
+                    uint32_t fr = ~(ua | ub);
+                    const uint32_t nf = (uint32_t)__popc(fr);
+                    if (idx < nf) {
+                        for (; idx; --idx) fr &= fr - 1u;
+                        t = w * 32u + (uint32_t)__builtin_ctz(fr);
+                        break;
+                    }
+                    idx -= nf;
+                }
+                if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&cb.ctr[CTR_ERR], 1u); t = MAX_COLOURS - 1; }
+                LT[j] = (uint16_t)t;
+                // only this round's claims are in the table, so the keys' low words decide
+                const uint32_t lo = rank_hash(q, r);
+                if (da) atomicMax(&LC[la * COLOUR_SLOTS + t % COLOUR_SLOTS], lo);
+                if (db) atomicMax(&LC[lb * COLOUR_SLOTS + t % COLOUR_SLOTS], lo);
+            }
+        } else if (in_lds) {
             for (uint32_t j = tid; j < n; j += bd) {   // claim (colour_claim)
                 if (LA[j] != COLOUR_OPEN) continue;
                 const uint2 bw = LB[j];
@@ -1228,7 +1300,31 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         }
         __syncthreads();
         uint32_t w = 0;
-        if (in_lds) {
+        if (lbod) {
+            for (uint32_t j = tid; j < n; j += bd) {   // settle (colour_settle on the LDS state)
+                if (LA[j] != COLOUR_OPEN) continue;
+                const uint32_t lw = LL[j];
+                const bool da = lw & SCENE_LDYN, db = (lw >> 16) & SCENE_LDYN;
+                const uint32_t la = lw & (SCENE_LDYN - 1u), lb = (lw >> 16) & (SCENE_LDYN - 1u);
+                const uint32_t t = LT[j];
+                const uint32_t lo = rank_hash(LQ[j], r);
+                if (da && LC[la * COLOUR_SLOTS + t % COLOUR_SLOTS] != lo) continue;
+                if (db && LC[lb * COLOUR_SLOTS + t % COLOUR_SLOTS] != lo) continue;
+                LA[j] = (int)t;
+                const uint32_t bit = 1u << (t % 32u), tw = t / 32u;
+                if (tw < SCENE_LDS_UWORDS) {
+                    if (da) atomicOr(&LW[la * SCENE_LDS_UWORDS + tw], bit);
+                    if (db) atomicOr(&LW[lb * SCENE_LDS_UWORDS + tw], bit);
+                } else {
+                    const uint2 bw = LB[j];
+                    if (da) atomicOr(&cb.used[(bw.x & ~SCENE_BDYN) * COLOUR_WORDS + tw], bit);
+                    if (db) atomicOr(&cb.used[(bw.y & ~SCENE_BDYN) * COLOUR_WORDS + tw], bit);
+                }
+                if (da) atomicSub(&LU[la], 1u);
+                if (db) atomicSub(&LU[lb], 1u);
+                ++w;
+            }
+        } else if (in_lds) {
             for (uint32_t j = tid; j < n; j += bd) {   // settle (colour_settle)
                 if (LA[j] != COLOUR_OPEN) continue;
                 const uint2 bw = LB[j];
@@ -1254,6 +1350,10 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         if ((tid & 63) == 0 && w) atomicAdd(&won, w);
         __syncthreads();
         left = n - won;
+        if (lbod) {   // the next round's claim table
+            uint4 *c4 = reinterpret_cast<uint4 *>(LC);
+            for (uint32_t i = tid; i < nbs * (COLOUR_SLOTS / 4); i += bd) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+        }
         __syncthreads();
     }
     if (in_lds)   // the colours to the global array (pair colours are read from it)
@@ -1291,7 +1391,14 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     }
     if (ep) {   // what this scene set: its bodies' used-mask words of its colours and their counts
         const uint32_t nw = (top + 31u) / 32u;   // final since the barrier before the offsets
-        for (uint32_t j = tid; j < n; j += bd) {
+        for (uint32_t j = tid; j < n && lbod; j += bd) {   // LDS state: only the global words
+            const uint2 bw = LB[j];
+            for (uint32_t w = SCENE_LDS_UWORDS; w < nw; ++w) {
+                if (bw.x & SCENE_BDYN) cb.used[(bw.x & ~SCENE_BDYN) * COLOUR_WORDS + w] = 0u;
+                if (bw.y & SCENE_BDYN) cb.used[(bw.y & ~SCENE_BDYN) * COLOUR_WORDS + w] = 0u;
+            }
+        }
+        for (uint32_t j = tid; j < n && !lbod; j += bd) {
             uint32_t ba, bb;
             bool da, db;
             if (in_lds) {
@@ -1322,6 +1429,9 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         if (left) e |= 2u;
         if (e) atomicOr(T.err, e);
     }
+    // the substep's integration (k_integrate) of n_int bodies, spread over the workgroups: the
+    // colouring reads only the bodies' dynamic flags, which the integration does not write
+    for (uint32_t i = s * bd + tid; i < n_int; i += gridDim.x * bd) integrate_body(B, i, dt);
 }
 
 }  // namespace rb

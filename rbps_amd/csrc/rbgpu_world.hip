@@ -138,9 +138,11 @@ struct rbgpu_world {
     uint32_t *sc_plane_off = nullptr, *sc_plane_idx = nullptr, *sc_size = nullptr, *sc_base = nullptr;
     uint32_t *sc_work = nullptr, *sc_list = nullptr, *sc_coff = nullptr, *sc_ncol = nullptr, *sc_err = nullptr;
     uint32_t *sj_order = nullptr, *sj_base = nullptr, *sj_coff = nullptr, sj_njc = 0;   // per-scene joint tables
+    // per-scene body lists, each body's index in its scene's list, the colouring's scene order
+    uint32_t *sb_off = nullptr, *sb_idx = nullptr, *sb_loc = nullptr, *sc_order = nullptr;
     SceneTabs scene_tabs() const {
         return SceneTabs{sap_start, sap_end, sc_plane_off, sc_plane_idx, sc_base, sc_work, sc_list, sc_coff, sc_ncol, sc_err,
-                         sj_order, sj_base, sj_coff, sj_njc};
+                         sj_order, sj_base, sj_coff, sj_njc, sb_off, sb_idx, sb_loc, sc_order};
     }
     bool scene_path() const {
         return nscenes > 1 && (scene_mode == 1 || (scene_mode == -1 && nscenes >= 16 && max_scene_bodies <= 4096));
@@ -158,7 +160,7 @@ struct rbgpu_world {
     uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
     uint32_t *gjk_dn = nullptr;   // pairs the fused GJK/EPA pass hands to the full-size EPA pass
     uint32_t *gjk_ckey = nullptr, *gjk_ckey2 = nullptr, *gjk_cand2 = nullptr;   // candidates sorted by last iterations
-    bool gjk_sort = false;   // measured slower on the combined workload (bin order keeps pairs of a region together)
+    int gjk_sort = 0;   // candidate order: 0 bin order, 1 iterations, 2 overlap, 3 both
     float4 *gjk_axis = nullptr;   // per pair slot: the axis that last separated it (rejection hint)
     // the hints carried across a frame's new pair list (table keyed by collider pair)
     unsigned long long *ax_keys = nullptr;
@@ -309,7 +311,7 @@ struct rbgpu_world {
         bin_off.clear();
         dfree(cscene); dfree(slot_scene); dfree(sap_start); dfree(sap_end); dfree(plane_lid);
         dfree(sc_plane_off); dfree(sc_plane_idx); dfree(sc_size); dfree(sc_base); dfree(sc_coff); dfree(sc_ncol);
-        dfree(sc_err);
+        dfree(sc_err); dfree(sb_off); dfree(sb_idx); dfree(sb_loc); dfree(sc_order);
         for (auto &e : ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         ev_pool.clear();
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -384,7 +386,7 @@ struct rbgpu_world {
             if (rc) return rc;
             if (gjk_sort) {   // candidates by their previous GJK iteration count (stable)
                 k_gjk_compact_keyed<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, bin_list, g0, gstat, gjk_cand,
-                                                               gjk_ckey);
+                                                               gjk_ckey, gjk_sort);
                 rc = radix_pass(gjk_ckey, gjk_cand, gjk_ckey2, gjk_cand2, ng, 0, p_hist, nullptr, gjk_ncand);
                 if (rc) return rc;
                 cand = gjk_cand2;
@@ -935,9 +937,29 @@ int rbgpu_world::build() {
         rs |= dalloc(sc_plane_off, nsc + 1); rs |= dalloc(sc_plane_idx, std::max<uint32_t>(n_plane, 1));
         rs |= dalloc(sc_size, nsc + 1); rs |= dalloc(sc_base, nsc + 1); rs |= dalloc(sc_ncol, nsc);
         rs |= dalloc(sc_coff, (size_t)nsc * (MAX_COLOURS + 1)); rs |= dalloc(sc_err, 1);
+        // the scenes' bodies in ascending order (the per-scene solve's velocity update, the
+        // colouring's local body ids); the colouring takes the scenes largest first, so that
+        // the longest workgroups start early
+        std::vector<uint32_t> boff(nsc + 1, 0), bidx(N), bloc(N), sord(nsc);
+        for (uint32_t k = 0; k < nsc; ++k) boff[k + 1] = boff[k] + nb[k];
+        {
+            std::vector<uint32_t> bcur(boff.begin(), boff.end() - 1);
+            for (uint32_t i = 0; i < N; ++i) {
+                bloc[i] = bcur[hb[i].scene] - boff[hb[i].scene];
+                bidx[bcur[hb[i].scene]++] = i;
+            }
+        }
+        for (uint32_t k = 0; k < nsc; ++k) sord[k] = k;
+        std::stable_sort(sord.begin(), sord.end(), [&](uint32_t a, uint32_t b) { return nb[a] > nb[b]; });
+        rs |= dalloc(sb_off, nsc + 1); rs |= dalloc(sb_idx, std::max<uint32_t>(N, 1));
+        rs |= dalloc(sb_loc, std::max<uint32_t>(N, 1)); rs |= dalloc(sc_order, nsc);
         if (rs) return RBGPU_ERR_OOM;
         HIPCHK(up(sc_plane_off, poff.data(), (nsc + 1) * 4));
         if (n_plane) HIPCHK(up(sc_plane_idx, pidx.data(), n_plane * 4));
+        HIPCHK(up(sb_off, boff.data(), (nsc + 1) * 4));
+        if (N) HIPCHK(up(sb_idx, bidx.data(), N * 4));
+        if (N) HIPCHK(up(sb_loc, bloc.data(), N * 4));
+        HIPCHK(up(sc_order, sord.data(), nsc * 4));
         HIPCHK(hipMemsetAsync(sc_err, 0, 4, st));
     }
     if (m) HIPCHK(up(cscene, csc.data(), m * 4));
@@ -1204,6 +1226,18 @@ int rbgpu_world::frame() {
     SceneTabs TJ = T;
     if (!scene_joints) TJ.jorder = nullptr;
     colour_fast = by_scene && colour_clean;   // this frame's colouring-state clear: counters only
+    // On the per-scene path the body passes ride in the per-scene kernels: the integration in
+    // the colouring's workgroups (it is independent of the colouring), the velocity update and
+    // the velocity solve in the position solve's (a scene's bodies are touched by its own
+    // pairs and joints only) -- unless one of those launches is timed this frame.
+    const bool timed_frame = timing_kernel != 0 && frames % timing_stride == 0;
+    auto timed_k = [&](std::initializer_list<int> ks) {
+        for (int k : ks)
+            if (timed_frame && timing_kernel == k) return true;
+        return false;
+    };
+    const bool fold_int = by_scene && P && N && !timed_k({4, 7});
+    const bool fold_vel = by_scene && P && N && (NJ == 0 || scene_joints) && solve_occ != 4 && !timed_k({1, 2, 5});
     auto substep = [&]() -> int {
         runs.clear();
         if (P) {
@@ -1214,7 +1248,8 @@ int rbgpu_world::frame() {
             // colouring of the pairs with points (conflict = a shared dynamic body)
             hipEvent_t *ec = timed_begin(7);
             if (by_scene) {   // per scene: rounds, then the colour-ordered lists (no host round trip)
-                k_scene_colour<<<nscenes, SCENE_THREADS, 0, st>>>(K.cnt, K.bb, scal + 0, B, cb, T, colour_rounds_cap);
+                k_scene_colour<<<nscenes, SCENE_THREADS, 0, st>>>(K.cnt, K.bb, scal + 0, B, cb, T, colour_rounds_cap,
+                                                                  fold_int ? N : 0u, h);
                 timed_end(ec);
             } else {
             k_colour_activate<<<nblk(P), TPB, 0, st>>>(K.cnt, P, K.bb, B, cb);
@@ -1263,10 +1298,14 @@ int rbgpu_world::frame() {
             timed_end(ec);
             }
         }
-        if (N) {
+        if (N && !fold_int) {
             hipEvent_t *e = timed_begin(4);
             k_integrate<<<nblk(N), TPB, 0, st>>>(B, N, h);
             timed_end(e);
+        }
+        if (fold_vel) {   // positions, velocities, velocity solve: one launch
+            k_scene_solve<true, 1, true><<<nscenes, scene_threads, 0, st>>>(B, K, TJ, h, nullptr, Jt, inv_h, joint_mode);
+            return RBGPU_OK;
         }
         if (by_scene && P) {
             hipEvent_t *e = timed_begin(1);
@@ -1924,8 +1963,8 @@ int rbgpu_set_option(rbgpu_world *w, int option, int value) {
             if (value < 0 || value > 2) return fail(RBGPU_ERR_INVALID, "EPA guard is 0, 1 or 2");
             w->epa_guard = value; return RBGPU_OK;
         case RBGPU_OPT_GJK_SORT:
-            if (value != 0 && value != 1) return fail(RBGPU_ERR_INVALID, "GJK sort is 0 or 1");
-            w->gjk_sort = value != 0; return RBGPU_OK;
+            if (value < 0 || value > 3) return fail(RBGPU_ERR_INVALID, "GJK sort is 0, 1, 2 or 3");
+            w->gjk_sort = value; return RBGPU_OK;
         case RBGPU_OPT_EPA_SUB:
             if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 return fail(RBGPU_ERR_INVALID, "EPA work item: 1, 2, 4, 8 or 16 pairs");
@@ -1959,7 +1998,7 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value) {
         case RBGPU_OPT_SIDE_STREAM: *value = w->side_stream; return RBGPU_OK;
         case RBGPU_OPT_AXIS_CARRY: *value = w->axis_carry ? 1 : 0; return RBGPU_OK;
         case RBGPU_OPT_EPA_GUARD: *value = w->epa_guard; return RBGPU_OK;
-        case RBGPU_OPT_GJK_SORT: *value = w->gjk_sort ? 1 : 0; return RBGPU_OK;
+        case RBGPU_OPT_GJK_SORT: *value = w->gjk_sort; return RBGPU_OK;
         default: return fail(RBGPU_ERR_INVALID, "unknown option");
     }
 }
