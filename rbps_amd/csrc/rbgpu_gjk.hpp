@@ -1991,11 +1991,16 @@ __device__ __forceinline__ float row_support_extent(const Colliders &C, const Hu
     }
     return dpp_row_fmax(e);
 }
+// keep[i]: the pair stays a GJK candidate; bsum (optional): kept pairs per 256-pair workgroup.
 __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B,
-                                                  Contacts K, HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep) {
+                                                  Contacts K, HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep,
+                                                  uint32_t *bsum) {
+    __shared__ uint32_t bkept;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = lane64(), row = lane >> 4, pl = lane & 15;
-    bool need = false;
+    if (threadIdx.x == 0) bkept = 0u;
+    __syncthreads();
+    bool need = false, kept = false;
     CullPair p{};   // this lane's pair (the exact stage takes the open pairs' from their lanes)
     if (i < n) {
         p = cull_pair(bin_slot, off, i, C, B, K);
@@ -2024,11 +2029,16 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
         need = !apart && axes && (C.meta[p.cc.x].x == SH_HULL || C.meta[p.cc.y].x == SH_HULL);
         if (!need) {
             keep[i] = apart ? 0u : 1u;
+            kept = !apart;
             if (apart) {
                 K.cnt[p.s] = 0;
                 gstat[p.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
             }
         }
+    }
+    {
+        const uint64_t km = __ballot(kept);
+        if (bsum && lane == 0 && km) atomicAdd(&bkept, (uint32_t)__popcll(km));
     }
     // exact extents of the open pairs, a row per pair
     uint64_t todo = __ballot(need);
@@ -2074,8 +2084,14 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
             if (apart) {
                 K.cnt[q.s] = 0;
                 gstat[q.s] = pack_gstat(0, 0, GJK_VALID, 15, 0);
+            } else if (bsum) {
+                atomicAdd(&bkept, 1u);
             }
         }
+    }
+    if (bsum) {
+        __syncthreads();
+        if (threadIdx.x == 0) bsum[blockIdx.x] = bkept;
     }
 }
 
@@ -2094,6 +2110,20 @@ __global__ void k_log_advance(uint32_t *cnt, const uint32_t *n_dev, uint32_t n_h
 __global__ void k_gjk_compact(const uint32_t *keep, const uint32_t *pos, uint32_t n, uint32_t *cand) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && keep[i]) cand[pos[i]] = i;
+}
+// The same from the cull's per-workgroup counts (boff: their exclusive scan; 256-pair
+// workgroups as in k_gjk_cull): the rank inside the workgroup from the wavefronts' ballots.
+__global__ void __launch_bounds__(256) k_gjk_compact_blk(const uint32_t *keep, const uint32_t *boff, uint32_t n,
+                                                         uint32_t *cand) {
+    __shared__ uint32_t ws[4];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x, w = threadIdx.x >> 6;
+    const bool k = i < n && keep[i];
+    const uint64_t m = __ballot(k);
+    if ((threadIdx.x & 63u) == 0u) ws[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = boff[blockIdx.x];
+    for (uint32_t u = 0; u < w; ++u) base += ws[u];
+    if (k) cand[base + lane_rank(m)] = i;
 }
 // The same compaction with a sort key per candidate from its slot's previous substep (a
 // frame's first substep reads the slots' old contents): mode 1 the GJK iterations (capped at

@@ -1146,13 +1146,13 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     __shared__ uint32_t LW[SCENE_LDS_BODIES * SCENE_LDS_UWORDS];
     uint32_t *const hist = LC;
     __shared__ uint32_t sh[SCENE_THREADS / 64 + 1];
-    __shared__ uint32_t nact, won, top;
+    __shared__ uint32_t nact, won, top, serr;
     const uint32_t s = T.corder ? T.corder[blockIdx.x] : blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
     const uint32_t ns = *n_sap_dev;
     const uint32_t a0 = T.sap_start[s], na = T.sap_end[s] - a0;
     const uint32_t q0 = T.plane_off[s], total = na + (T.plane_off[s + 1] - q0);
     uint32_t *G = T.work + T.base[s];
-    if (tid == 0) { nact = 0u; won = 0u; top = 0u; }
+    if (tid == 0) { nact = 0u; won = 0u; top = 0u; serr = 0u; }
     __syncthreads();
     // activation, in two parts.  (1) The scan: the point counts of ACT_AHEAD pairs per thread
     // at once, the pairs without points marked, the others listed (no atomics in this loop: a
@@ -1258,7 +1258,7 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
                     }
                     idx -= nf;
                 }
-                if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&cb.ctr[CTR_ERR], 1u); t = MAX_COLOURS - 1; }
+                if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&serr, 1u); t = MAX_COLOURS - 1; }
                 LT[j] = (uint16_t)t;
                 // only this round's claims are in the table, so the keys' low words decide
                 const uint32_t lo = rank_hash(q, r);
@@ -1286,7 +1286,7 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
                     }
                     idx -= nf;
                 }
-                if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&cb.ctr[CTR_ERR], 1u); t = MAX_COLOURS - 1; }
+                if (t >= (uint32_t)MAX_COLOURS) { atomicOr(&serr, 1u); t = MAX_COLOURS - 1; }
                 LT[j] = (uint16_t)t;
                 const unsigned long long key = claim_key(kh, rank_hash(q, r));
                 if (da) atomicMax(&cb.claim[ba * COLOUR_SLOTS + t % COLOUR_SLOTS], key);
@@ -1376,13 +1376,16 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     }
     uint32_t tot;
     uint32_t run = block_exclusive_scan(sum, sh, &tot);
+    const uint32_t tp = top;   // final: the scan ends with a barrier
+    // offsets of colours 0 .. top (the solve reads no others; colours from top on are empty, so
+    // the offset at top is the total)
     uint32_t *co = T.coff + (size_t)s * (MAX_COLOURS + 1);
     for (uint32_t k = 0; k < PER; ++k) {
-        co[tid * PER + k] = run;
+        if (tid * PER + k <= tp) co[tid * PER + k] = run;
         hist[tid * PER + k] = run;   // scatter cursor
         run += v[k];
     }
-    if (tid == 0) co[MAX_COLOURS] = tot;
+    if (tid == 0 && tp == (uint32_t)MAX_COLOURS) co[MAX_COLOURS] = tot;
     __syncthreads();
     uint32_t *out = T.list + T.base[s];
     for (uint32_t j = tid; j < n; j += bd) {
@@ -1420,18 +1423,19 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     }
     if (tid == 0) {
         T.ncol[s] = top;
-        if (n) {
-            atomicAdd(&cb.ctr[CTR_ACT], n);
-            atomicMax(&cb.ctr[CTR_NCOL], top);
-            atomicMax(&cb.ctr[CTR_ROUNDS], r);
-        }
-        uint32_t e = ld_shared(&cb.ctr[CTR_ERR]) & 1u;
+        // colour capacity: this scene's LDS flag (the global word only where colour_claim ran)
+        uint32_t e = serr | (in_lds ? 0u : ld_shared(&cb.ctr[CTR_ERR]) & 1u);
         if (left) e |= 2u;
         if (e) atomicOr(T.err, e);
     }
     // the substep's integration (k_integrate) of n_int bodies, spread over the workgroups: the
     // colouring reads only the bodies' dynamic flags, which the integration does not write
     for (uint32_t i = s * bd + tid; i < n_int; i += gridDim.x * bd) integrate_body(B, i, dt);
+    if (tid == 0 && n) {   // statistics (no wait on these atomics)
+        atomicAdd(&cb.ctr[CTR_ACT], n);
+        atomicMax(&cb.ctr[CTR_NCOL], top);
+        atomicMax(&cb.ctr[CTR_ROUNDS], r);
+    }
 }
 
 }  // namespace rb

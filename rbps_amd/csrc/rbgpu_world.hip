@@ -381,18 +381,28 @@ struct rbgpu_world {
         const uint32_t *cand = nullptr, *ncand = nullptr;
         if (int rcf = fork_at(1)) return rcf;
         if (gjk_cull && !gjk_explicit) {
-            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep);
-            int rc = scan(gjk_keep, gjk_pos, ng, gjk_ncand);
-            if (rc) return rc;
-            if (gjk_sort) {   // candidates by their previous GJK iteration count (stable)
-                k_gjk_compact_keyed<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, bin_list, g0, gstat, gjk_cand,
-                                                               gjk_ckey, gjk_sort);
-                rc = radix_pass(gjk_ckey, gjk_cand, gjk_ckey2, gjk_cand2, ng, 0, p_hist, nullptr, gjk_ncand);
-                if (rc) return rc;
-                cand = gjk_cand2;
-            } else {
-                k_gjk_compact<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
+            static_assert(TPB == 256, "k_gjk_compact_blk takes the cull's 256-pair workgroups");
+            // bin order from the cull's per-workgroup counts (their scan in one workgroup)
+            const bool blk = !gjk_sort && nblk(ng) <= (uint32_t)(SCAN_TOP_THREADS * SCAN_TOP_ITEMS);
+            k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep,
+                                                 blk ? gjk_pos : nullptr);
+            if (blk) {
+                k_scan_top<<<1, SCAN_TOP_THREADS, 0, st>>>(gjk_pos, nblk(ng), gjk_ncand);
+                k_gjk_compact_blk<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
                 cand = gjk_cand;
+            } else {
+                int rc = scan(gjk_keep, gjk_pos, ng, gjk_ncand);
+                if (rc) return rc;
+                if (gjk_sort) {   // candidates by a key from their slot's previous substep (stable)
+                    k_gjk_compact_keyed<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, bin_list, g0, gstat, gjk_cand,
+                                                                   gjk_ckey, gjk_sort);
+                    rc = radix_pass(gjk_ckey, gjk_cand, gjk_ckey2, gjk_cand2, ng, 0, p_hist, nullptr, gjk_ncand);
+                    if (rc) return rc;
+                    cand = gjk_cand2;
+                } else {
+                    k_gjk_compact<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
+                    cand = gjk_cand;
+                }
             }
             ncand = gjk_ncand;
             if (timed) k_add_count<<<1, 1, 0, st>>>(gjk_ncand, d_points);
