@@ -1126,33 +1126,48 @@ __global__ void k_scene_sizes(SceneTabs T, uint32_t nsc, uint32_t *size) {
 // Either way the rounds, keys, palettes and masks are those of colour_claim / colour_settle.
 constexpr uint32_t SCENE_BDYN = 0x80000000u;   // body word: index | dynamic flag
 constexpr uint32_t SCENE_LDYN = 0x8000u;       // local body half-word: local id | dynamic flag
-constexpr uint32_t SCENE_LDS_BODIES = 512, SCENE_LDS_UWORDS = 2;
-static_assert(SCENE_LDS_BODIES * COLOUR_SLOTS >= (uint32_t)MAX_COLOURS, "the histogram lives in the claim table");
+// LDS colouring state: scenes of up to SCENE_LDS_BODIES bodies of which at most
+// SCENE_LDS_ACTIVE dynamic ones have active pairs (numbered in scene order for the tables)
+constexpr uint32_t SCENE_LDS_BODIES = 512, SCENE_LDS_ACTIVE = 256, SCENE_LDS_UWORDS = 2;
+static_assert(SCENE_LDS_PAIRS >= (uint32_t)MAX_COLOURS, "the histogram lives in the hash-input array");
 static_assert(SCENE_LDS_BODIES < SCENE_LDYN, "local ids fit in 15 bits");
+static_assert(SCENE_LDS_BODIES == 2 * SCENE_THREADS, "the body numbering takes two bodies per thread");
+static_assert(SCENE_LDS_ACTIVE * COLOUR_SLOTS * 4 >= SCENE_LDS_PAIRS * 8, "the claim table overlays the body pairs");
 __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, const uint2 *pbod, const uint32_t *n_sap_dev,
                                                                 Bodies B, ColourBufs cb, SceneTabs T, uint32_t cap_rounds,
                                                                 uint32_t n_int, float dt) {
     __shared__ uint32_t L[SCENE_LDS_PAIRS];    // pair id
-    __shared__ uint2 LB[SCENE_LDS_PAIRS];      // bodies | SCENE_BDYN
-    __shared__ uint32_t LQ[SCENE_LDS_PAIRS];   // local pair id (the hashes' input)
+    __shared__ uint32_t LQ[SCENE_LDS_PAIRS];   // local pair id (the hashes' input); the histogram after the rounds
     __shared__ int LA[SCENE_LDS_PAIRS];        // colour, COLOUR_OPEN
     __shared__ uint16_t LT[SCENE_LDS_PAIRS];   // tentative colour of the round
-    __shared__ uint32_t LL[SCENE_LDS_PAIRS];   // bodies' local ids | SCENE_LDYN, a in the low half
-    // per local body (scenes of up to SCENE_LDS_BODIES bodies): the round's slot claims (the
-    // keys' low words; the bucketing's histogram after the rounds), uncoloured counts and
-    // colours 0 .. 32 SCENE_LDS_UWORDS - 1 of the used masks (higher colours stay global)
-    __shared__ uint32_t LC[SCENE_LDS_BODIES * COLOUR_SLOTS];
-    __shared__ uint32_t LU[SCENE_LDS_BODIES];
-    __shared__ uint32_t LW[SCENE_LDS_BODIES * SCENE_LDS_UWORDS];
-    uint32_t *const hist = LC;
+    __shared__ uint32_t LL[SCENE_LDS_PAIRS];   // bodies' scene-local, then active ids | SCENE_LDYN, a in the low half
+    // the pairs' bodies | SCENE_BDYN; with the LDS state, once its bodies are numbered, the
+    // rounds' slot claims per active body (the keys' low words) in the same place
+    __shared__ union {
+        uint2 b[SCENE_LDS_PAIRS];
+        uint32_t c[SCENE_LDS_ACTIVE * COLOUR_SLOTS];
+    } LBC;
+    uint2 *const LB = LBC.b;
+    uint32_t *const LC = LBC.c;
+    // per active body: uncoloured counts, colours 0 .. 32 SCENE_LDS_UWORDS - 1 of the used
+    // masks (higher colours stay global), body index; per scene body: its active id
+    __shared__ uint32_t LU[SCENE_LDS_ACTIVE];
+    __shared__ uint32_t LW[SCENE_LDS_ACTIVE * SCENE_LDS_UWORDS];
+    __shared__ uint32_t LG[SCENE_LDS_ACTIVE];
+    __shared__ uint16_t LM[SCENE_LDS_BODIES];
+    uint32_t *const hist = LQ;
     __shared__ uint32_t sh[SCENE_THREADS / 64 + 1];
     __shared__ uint32_t nact, won, top, serr;
+    uint32_t nbs_active = 0u;
     const uint32_t s = T.corder ? T.corder[blockIdx.x] : blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
     const uint32_t ns = *n_sap_dev;
     const uint32_t a0 = T.sap_start[s], na = T.sap_end[s] - a0;
     const uint32_t q0 = T.plane_off[s], total = na + (T.plane_off[s + 1] - q0);
     uint32_t *G = T.work + T.base[s];
+    const uint32_t nbs = T.body_loc ? T.body_off[s + 1] - T.body_off[s] : 0u;
+    const bool lfit = T.body_loc && nbs <= SCENE_LDS_BODIES;   // bodies numbered in LDS
     if (tid == 0) { nact = 0u; won = 0u; top = 0u; serr = 0u; }
+    for (uint32_t i = tid; lfit && i < nbs; i += bd) LM[i] = 0u;
     __syncthreads();
     // activation, in two parts.  (1) The scan: the point counts of ACT_AHEAD pairs per thread
     // at once, the pairs without points marked, the others listed (no atomics in this loop: a
@@ -1192,26 +1207,19 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     __syncthreads();
     const uint32_t n = nact;
     const bool in_lds = n <= SCENE_LDS_PAIRS;
-    // the bodies' colouring state in LDS when the pairs are there and the scene is small enough
-    const uint32_t nbs = T.body_loc ? T.body_off[s + 1] - T.body_off[s] : 0u;
-    const bool lbod = in_lds && T.body_loc && nbs <= SCENE_LDS_BODIES;
-    if (lbod) {
-        for (uint32_t i = tid; i < nbs; i += bd) LU[i] = 0u;
-        for (uint32_t i = tid; i < nbs * SCENE_LDS_UWORDS; i += bd) LW[i] = 0u;
-        uint4 *c4 = reinterpret_cast<uint4 *>(LC);
-        for (uint32_t i = tid; i < nbs * (COLOUR_SLOTS / 4); i += bd) c4[i] = make_uint4(0u, 0u, 0u, 0u);
-        __syncthreads();
-    }
+    // the bodies' colouring state in LDS when the pairs are there and the scene's active
+    // dynamic bodies fit: mark them (scene-local ids), number them in scene order, then count
+    const bool lmark = in_lds && lfit;
     for (uint32_t j = tid; j < n; j += bd) {
         const uint32_t p = j < SCENE_LDS_PAIRS ? L[j] : G[j];
         const uint2 bb = pbod[p];
         const uint32_t q = cb.lid[p];
         const bool da = is_dynamic(B, bb.x), db = is_dynamic(B, bb.y);
         cb.acol[p] = COLOUR_OPEN;
-        if (lbod) {   // every body of a scene's pair is in the scene
+        if (lmark) {   // every body of a scene's pair is in the scene
             const uint32_t la = T.body_loc[bb.x], lb = T.body_loc[bb.y];
-            if (da) atomicAdd(&LU[la], 1u);
-            if (db) atomicAdd(&LU[lb], 1u);
+            if (da) LM[la] = 1u;
+            if (db) LM[lb] = 1u;
             LL[j] = (la | (da ? SCENE_LDYN : 0u)) | ((lb | (db ? SCENE_LDYN : 0u)) << 16);
         } else {
             if (da) atomicAdd(&cb.ucnt[bb.x], 1u);
@@ -1224,6 +1232,41 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         }
     }
     __syncthreads();
+    bool lbod = false;
+    if (lmark) {   // uniform
+        const uint32_t i0 = 2u * tid;
+        const uint32_t m0 = i0 < nbs ? LM[i0] : 0u, m1 = i0 + 1u < nbs ? LM[i0 + 1u] : 0u;
+        uint32_t nact_b;
+        const uint32_t base = block_exclusive_scan(m0 + m1, sh, &nact_b);
+        lbod = nact_b <= SCENE_LDS_ACTIVE;
+        if (i0 < nbs) LM[i0] = (uint16_t)base;
+        if (i0 + 1u < nbs) LM[i0 + 1u] = (uint16_t)(base + m0);
+        for (uint32_t i = tid; lbod && i < nact_b; i += bd) LU[i] = 0u;
+        for (uint32_t i = tid; lbod && i < nact_b * SCENE_LDS_UWORDS; i += bd) LW[i] = 0u;
+        __syncthreads();
+        for (uint32_t j = tid; j < n; j += bd) {
+            const uint32_t lw = LL[j];
+            const bool da = lw & SCENE_LDYN, db = (lw >> 16) & SCENE_LDYN;
+            const uint2 bw = LB[j];
+            if (lbod) {   // active ids, their bodies and uncoloured counts
+                const uint32_t ca = da ? LM[lw & (SCENE_LDYN - 1u)] : 0u;
+                const uint32_t cc = db ? LM[(lw >> 16) & (SCENE_LDYN - 1u)] : 0u;
+                LL[j] = (ca | (da ? SCENE_LDYN : 0u)) | ((cc | (db ? SCENE_LDYN : 0u)) << 16);
+                if (da) { LG[ca] = bw.x & ~SCENE_BDYN; atomicAdd(&LU[ca], 1u); }
+                if (db) { LG[cc] = bw.y & ~SCENE_BDYN; atomicAdd(&LU[cc], 1u); }
+            } else {   // too many active bodies: the global state
+                if (da) atomicAdd(&cb.ucnt[bw.x & ~SCENE_BDYN], 1u);
+                if (db) atomicAdd(&cb.ucnt[bw.y & ~SCENE_BDYN], 1u);
+            }
+        }
+        __syncthreads();
+        if (lbod) {   // the claim table over the body pairs (read above, not after)
+            uint4 *c4 = reinterpret_cast<uint4 *>(LC);
+            for (uint32_t i = tid; i < nact_b * (COLOUR_SLOTS / 4); i += bd) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+            nbs_active = nact_b;
+        }
+        __syncthreads();
+    }
     const uint32_t ep = cb.epoch ? *cb.epoch : 0u;   // 0: the claims were cleared this substep
     // rounds 0, 1, ... as the grid-wide path runs them
     uint32_t left = n, r = 0;
@@ -1245,9 +1288,8 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
                         ua = da ? LW[la * SCENE_LDS_UWORDS + w] : 0u;
                         ub = db ? LW[lb * SCENE_LDS_UWORDS + w] : 0u;
                     } else {
-                        const uint2 bw = LB[j];
-                        ua = da ? ld_shared(&cb.used[(bw.x & ~SCENE_BDYN) * COLOUR_WORDS + w]) : 0u;
-                        ub = db ? ld_shared(&cb.used[(bw.y & ~SCENE_BDYN) * COLOUR_WORDS + w]) : 0u;
+                        ua = da ? ld_shared(&cb.used[LG[la] * COLOUR_WORDS + w]) : 0u;
+                        ub = db ? ld_shared(&cb.used[LG[lb] * COLOUR_WORDS + w]) : 0u;
                     }
                     uint32_t fr = ~(ua | ub);
                     const uint32_t nf = (uint32_t)__popc(fr);
@@ -1316,9 +1358,8 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
                     if (da) atomicOr(&LW[la * SCENE_LDS_UWORDS + tw], bit);
                     if (db) atomicOr(&LW[lb * SCENE_LDS_UWORDS + tw], bit);
                 } else {
-                    const uint2 bw = LB[j];
-                    if (da) atomicOr(&cb.used[(bw.x & ~SCENE_BDYN) * COLOUR_WORDS + tw], bit);
-                    if (db) atomicOr(&cb.used[(bw.y & ~SCENE_BDYN) * COLOUR_WORDS + tw], bit);
+                    if (da) atomicOr(&cb.used[LG[la] * COLOUR_WORDS + tw], bit);
+                    if (db) atomicOr(&cb.used[LG[lb] * COLOUR_WORDS + tw], bit);
                 }
                 if (da) atomicSub(&LU[la], 1u);
                 if (db) atomicSub(&LU[lb], 1u);
@@ -1352,7 +1393,7 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
         left = n - won;
         if (lbod) {   // the next round's claim table
             uint4 *c4 = reinterpret_cast<uint4 *>(LC);
-            for (uint32_t i = tid; i < nbs * (COLOUR_SLOTS / 4); i += bd) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+            for (uint32_t i = tid; i < nbs_active * (COLOUR_SLOTS / 4); i += bd) c4[i] = make_uint4(0u, 0u, 0u, 0u);
         }
         __syncthreads();
     }
@@ -1394,13 +1435,8 @@ __global__ void __launch_bounds__(SCENE_THREADS) k_scene_colour(const int *cnt, 
     }
     if (ep) {   // what this scene set: its bodies' used-mask words of its colours and their counts
         const uint32_t nw = (top + 31u) / 32u;   // final since the barrier before the offsets
-        for (uint32_t j = tid; j < n && lbod; j += bd) {   // LDS state: only the global words
-            const uint2 bw = LB[j];
-            for (uint32_t w = SCENE_LDS_UWORDS; w < nw; ++w) {
-                if (bw.x & SCENE_BDYN) cb.used[(bw.x & ~SCENE_BDYN) * COLOUR_WORDS + w] = 0u;
-                if (bw.y & SCENE_BDYN) cb.used[(bw.y & ~SCENE_BDYN) * COLOUR_WORDS + w] = 0u;
-            }
-        }
+        for (uint32_t i = tid; i < nbs_active && lbod; i += bd)   // LDS state: only the global words
+            for (uint32_t w = SCENE_LDS_UWORDS; w < nw; ++w) cb.used[LG[i] * COLOUR_WORDS + w] = 0u;
         for (uint32_t j = tid; j < n && !lbod; j += bd) {
             uint32_t ba, bb;
             bool da, db;
