@@ -240,8 +240,10 @@ __device__ __forceinline__ float4 joint_row(int4 m, float4 r1, float4 r2, float4
 }
 
 // compute_joint_errors (Joint.cpp:222-244) for every joint; mode 1 also resets lambda.
-__device__ __forceinline__ void joint_errors(Bodies &B, Joints &J, uint32_t i, float h, int mode) {
-    const int4 m = J.jm[i];
+// KNOWN (here and below): the joint's meta row comes with the call (staged by the caller).
+template <bool KNOWN = false>
+__device__ __forceinline__ void joint_errors(Bodies &B, Joints &J, uint32_t i, float h, int mode, int4 mk = {}) {
+    const int4 m = KNOWN ? mk : J.jm[i];
     const uint32_t b1 = (uint32_t)m.y, b2 = (uint32_t)m.z;
     float4 tg = J.jtg[i];
     float4 row[4];
@@ -274,8 +276,10 @@ __device__ __forceinline__ void apply_rot(BodyReg &r, V3 imp) {   // Body.cpp:70
 
 // solve_constraints over one joint's rows (Constraint.cpp:14-46, 89-106); mode 1 first
 // re-evaluates the rows from the bodies' current poses.
-__device__ __forceinline__ void joint_solve(Bodies &B, Joints &J, uint32_t i, float inv_h, float h, int mode) {
-    const int4 m = J.jm[i];
+template <bool KNOWN = false>
+__device__ __forceinline__ void joint_solve(Bodies &B, Joints &J, uint32_t i, float inv_h, float h, int mode,
+                                            int4 mk = {}) {
+    const int4 m = KNOWN ? mk : J.jm[i];
     const int type = jt_type(m), nrows = jt_rows(m), crow = jt_comp_row(type);
     const uint32_t b1 = (uint32_t)m.y, b2 = (uint32_t)m.z;
     const float4 r1f = J.jr1[i], r2f = J.jr2[i], axf = J.jax[i], lxf = J.jlx[i];
@@ -333,8 +337,9 @@ __device__ __forceinline__ float w_rot_vel(VelReg &r, V3 n) {
     r.Iwi = world_tensor(mat3_cast(r.q), r.Ii);
     return dot(n, smv(r.Iwi, n));
 }
-__device__ __forceinline__ void joint_damp(Bodies &B, Joints &J, uint32_t i, float h) {
-    const int4 m = J.jm[i];
+template <bool KNOWN = false>
+__device__ __forceinline__ void joint_damp(Bodies &B, Joints &J, uint32_t i, float h, int4 mk = {}) {
+    const int4 m = KNOWN ? mk : J.jm[i];
     const int type = jt_type(m);
     if (type != JT_PRISMATIC && type != JT_REVOLUTE) return;
     const uint32_t b1 = (uint32_t)m.y, b2 = (uint32_t)m.z;
@@ -381,13 +386,16 @@ __global__ void __launch_bounds__(256) k_joint_damp(Bodies B, Joints J, const ui
 // The scene's colour offsets, its pair list and joint order are read into LDS once (up to
 // SOLVE_LDS_PAIRS pairs, SOLVE_LDS_JOINTS joints; larger scenes read them from the global
 // lists), so that each colour step starts with the contact and body loads.
-constexpr uint32_t SOLVE_LDS_PAIRS = 2048, SOLVE_LDS_JOINTS = 1024, SOLVE_LDS_COLOURS = 256, SOLVE_LDS_JCOLS = 64;
+constexpr uint32_t SOLVE_LDS_PAIRS = 1024, SOLVE_LDS_JOINTS = 512, SOLVE_LDS_COLOURS = 256, SOLVE_LDS_JCOLS = 64;
 template <bool POS, int OCC = 1, bool ALL = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pts, Joints J, float inv_h, int jmode) {
     static_assert(POS || !ALL, "the whole substep solve starts with the positions");
     __shared__ uint32_t s_co[SOLVE_LDS_COLOURS + 1], s_lst[SOLVE_LDS_PAIRS];
+    __shared__ uint2 s_bb[SOLVE_LDS_PAIRS];       // the listed pairs' bodies
+    __shared__ uint8_t s_np[SOLVE_LDS_PAIRS];     // and point counts
     __shared__ uint32_t s_jc[SOLVE_LDS_JCOLS + 1], s_jo[SOLVE_LDS_JOINTS];
+    __shared__ int4 s_jm[SOLVE_LDS_JOINTS];       // the scene's joints' meta rows (type, bodies)
     const uint32_t s = blockIdx.x;
     const uint32_t nc = T.ncol[s];
     const uint32_t *lst = T.list + T.base[s];
@@ -400,13 +408,21 @@ k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pt
     for (uint32_t c = threadIdx.x; jc_l && c <= T.njc; c += blockDim.x) s_jc[c] = jc[c];
     const uint32_t nj = j1 - j0;
     const bool jo_l = nj <= SOLVE_LDS_JOINTS;
-    for (uint32_t i = threadIdx.x; jo_l && i < nj; i += blockDim.x) s_jo[i] = T.jorder[j0 + i];
+    for (uint32_t i = threadIdx.x; jo_l && i < nj; i += blockDim.x) {
+        const uint32_t q = T.jorder[j0 + i];
+        s_jo[i] = q;
+        s_jm[i] = J.jm[q];
+    }
     __syncthreads();
     const uint32_t np = co_l ? s_co[nc] : co[nc];
     const bool ls_l = np <= SOLVE_LDS_PAIRS;
-    for (uint32_t i = threadIdx.x; ls_l && i < np; i += blockDim.x) s_lst[i] = lst[i];
+    for (uint32_t i = threadIdx.x; ls_l && i < np; i += blockDim.x) {
+        const uint32_t p = lst[i];
+        s_lst[i] = p;
+        s_bb[i] = K.bb[p];
+        s_np[i] = (uint8_t)K.cnt[p];
+    }
     __syncthreads();
-    auto jorder = [&](uint32_t i) { return jo_l ? s_jo[i - j0] : T.jorder[i]; };
     // one solve: the contacts colour by colour, then the scene's joints (errors and rows after
     // the position solve, damping after the velocity solve)
     auto solve = [&](bool pos) {
@@ -417,23 +433,30 @@ k_scene_solve(Bodies B, Contacts K, SceneTabs T, float h, unsigned long long *pt
                 const bool v = i < e;
                 const uint32_t p = v ? (ls_l ? s_lst[i] : lst[i]) : 0u;
                 if (pts && pos == POS) count_points(K, p, v, pts);
-                if (v) {
-                    if (pos) solve_pos_pair(B, K, p);
-                    else solve_vel_pair(B, K, p, h);
+                if (v) {   // bodies and point count staged in LDS (or read here)
+                    const int npp = ls_l ? (int)s_np[i] : K.cnt[p];
+                    const uint2 bbp = ls_l ? s_bb[i] : K.bb[p];
+                    if (pos) solve_pos_pair<true>(B, K, p, npp, bbp);
+                    else solve_vel_pair<true>(B, K, p, h, npp, bbp);
                 }
             }
             __syncthreads();
         }
         if (!jn || j0 == j1) return;
         if (pos) {
-            for (uint32_t i = j0 + threadIdx.x; i < j1; i += blockDim.x) joint_errors(B, J, jorder(i), h, jmode);
+            for (uint32_t i = j0 + threadIdx.x; i < j1; i += blockDim.x) {
+                const uint32_t q = jo_l ? s_jo[i - j0] : T.jorder[i];
+                joint_errors<true>(B, J, q, h, jmode, jo_l ? s_jm[i - j0] : J.jm[q]);
+            }
             __syncthreads();
         }
         for (uint32_t c = 0; c < T.njc; ++c) {
             const uint32_t e = j0 + (jc_l ? s_jc[c + 1] : jc[c + 1]);
             for (uint32_t i = j0 + (jc_l ? s_jc[c] : jc[c]) + threadIdx.x; i < e; i += blockDim.x) {
-                if (pos) joint_solve(B, J, jorder(i), inv_h, h, jmode);
-                else joint_damp(B, J, jorder(i), h);
+                const uint32_t q = jo_l ? s_jo[i - j0] : T.jorder[i];
+                const int4 mq = jo_l ? s_jm[i - j0] : J.jm[q];   // the meta row (staged in LDS)
+                if (pos) joint_solve<true>(B, J, q, inv_h, h, jmode, mq);
+                else joint_damp<true>(B, J, q, h, mq);
             }
             __syncthreads();
         }

@@ -874,12 +874,15 @@ __device__ __forceinline__ void apply_pos(BodyReg &r, V3 imp, V3 rr) {   // Body
     r.q = qstep(r.q, rot);
 }
 
-__device__ __forceinline__ void solve_pos_pair(const Bodies &B, Contacts &K, uint32_t s) {
-    const int np = K.cnt[s];
+// KNOWN: the point count and bodies come with the call (a caller that staged them), so the
+// body loads need not wait for the pair's own rows.
+template <bool KNOWN = false>
+__device__ __forceinline__ void solve_pos_pair(const Bodies &B, Contacts &K, uint32_t s, int npk = 0, uint2 bbk = {}) {
+    const int np = KNOWN ? npk : K.cnt[s];
     if (np == 0) return;
     const float4 *hd = hd_row(K, s);
     const float4 h0 = hd[0], h1 = hd[1];
-    const uint2 bb = make_uint2(__float_as_uint(h1.x), __float_as_uint(h1.y));
+    const uint2 bb = KNOWN ? bbk : make_uint2(__float_as_uint(h1.x), __float_as_uint(h1.y));
     const V3 n = xyz(h0);
     const float sf = h1.z;
     BodyReg A, Bd;
@@ -986,12 +989,14 @@ __device__ __forceinline__ void apply_vel(VelReg &r, V3 imp, V3 rr) {   // Body.
     r.w = r.w + smv(r.Iwi, cross(rr, imp));
 }
 
-__device__ __forceinline__ void solve_vel_pair(const Bodies &B, Contacts &K, uint32_t s, float h) {
-    const int np = K.cnt[s];
+template <bool KNOWN = false>
+__device__ __forceinline__ void solve_vel_pair(const Bodies &B, Contacts &K, uint32_t s, float h, int npk = 0,
+                                               uint2 bbk = {}) {
+    const int np = KNOWN ? npk : K.cnt[s];
     if (np == 0) return;
     const float4 *hd = hd_row(K, s);
     const float4 h0 = hd[0], h1 = hd[1];
-    const uint2 bb = make_uint2(__float_as_uint(h1.x), __float_as_uint(h1.y));
+    const uint2 bb = KNOWN ? bbk : make_uint2(__float_as_uint(h1.x), __float_as_uint(h1.y));
     const V3 n = xyz(h0);
     const float rest = h0.w, df = h1.w;
     VelReg A, Bd;
