@@ -1685,7 +1685,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC ? O
 // with the reference-sized buffers; nothing else of the pair is written here.
 // Row polytope and clip buffer sizes of the fused pass; workgroups of the full-size EPA pass
 // that takes the pairs they could not hold.
-constexpr int FUSED_NV = 32, FUSED_NF = 64, FUSED_HC = 40;
+constexpr int FUSED_NV = 36, FUSED_NF = 72, FUSED_HC = 40;
 constexpr uint32_t FUSED_DEFER_GRID = 256;
 template <int NV, int NF, int HC, int OCC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC)))
