@@ -342,13 +342,14 @@ int rbgpu_get_option(rbgpu_world *w, int option, int *value);
 
 /* Autotuner (no counterpart in the reference).  Picks this world's launch options that
  * cannot change any result (per-scene solve workgroup size and the substep graph, or the
- * grid path's run minimum and round batch) by coordinate descent: each candidate value is timed over `frames_per_trial` real frames
- * (wall clock between two stream synchronisations) and the fastest is kept.  The trial
+ * grid path's run minimum and round batch) by coordinate descent: each candidate value is timed twice over `frames_per_trial` real
+ * frames (wall clock between two stream synchronisations; two passes over the values, the second in reverse order, since the scene
+ * moves between trials) and the value with the smallest total is kept.  The trial
  * frames advance the world exactly as untuned frames would, since every candidate gives
  * bit-identical states; callers typically tune during warm-up.  Requires timing off. */
 typedef struct {
     int32_t scene_kernels, scene_threads, graph, run_min, round_batch, gjk_grid, epa_grid;
-    int32_t trials, frames;      /* candidates timed, frames stepped (trials + one settling frame) */
+    int32_t trials, frames;      /* timings taken (two per candidate), frames stepped (trials + one settling frame) */
     double best_ms_per_frame;    /* of the last option tuned */
 } rbgpu_tune_report;
 int rbgpu_autotune(rbgpu_world *w, int frames_per_trial, rbgpu_tune_report *out);

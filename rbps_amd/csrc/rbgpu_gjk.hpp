@@ -1957,11 +1957,19 @@ struct CullPair {
     float d2, slack;
 };
 __device__ __forceinline__ CullPair cull_pair(const uint32_t *bin_slot, uint32_t off, uint32_t i, const Colliders &C,
-                                              const Bodies &B, const Contacts &K) {
+                                              const Bodies &B, const Contacts &K, const uint4 *rec, const uint32_t *recb) {
     CullPair p;
-    p.s = bin_slot[off + i];
-    p.cc = K.cc[p.s];
-    const uint2 bb = K.bb[p.s];
+    uint2 bb;
+    if (rec) {   // the frame's record: slot, colliders and bodies in one round of loads
+        const uint4 r = rec[i];
+        p.s = r.x;
+        p.cc = make_uint2(r.y, r.z);
+        bb = make_uint2(r.w, recb[i]);
+    } else {
+        p.s = bin_slot[off + i];
+        p.cc = K.cc[p.s];
+        bb = K.bb[p.s];
+    }
     p.ta = collider_tf(C, B, p.cc.x, bb.x);
     p.tb = collider_tf(C, B, p.cc.y, bb.y);
     p.d = p.tb.pos - p.ta.pos;
@@ -1991,10 +1999,21 @@ __device__ __forceinline__ float row_support_extent(const Colliders &C, const Hu
     }
     return dpp_row_fmax(e);
 }
+// The frame's GJK-bin records (per bin position: slot, colliders, bodies), written once per
+// frame after the pair list is binned, so that each substep's rejection pass reaches the poses
+// after one round of loads instead of two.
+__global__ void k_gjk_records(const uint32_t *bin_slot, uint32_t n, Contacts K, uint4 *rec, uint32_t *recb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = bin_slot[i];
+    const uint2 cc = K.cc[s], bb = K.bb[s];
+    rec[i] = make_uint4(s, cc.x, cc.y, bb.x);
+    recb[i] = bb.y;
+}
 // keep[i]: the pair stays a GJK candidate; bsum (optional): kept pairs per 256-pair workgroup.
 __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint32_t off, uint32_t n, Colliders C, Bodies B,
                                                   Contacts K, HullSet hs, const float4 *axis, uint32_t *gstat, uint32_t *keep,
-                                                  uint32_t *bsum) {
+                                                  uint32_t *bsum, const uint4 *rec = nullptr, const uint32_t *recb = nullptr) {
     __shared__ uint32_t bkept;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = lane64(), row = lane >> 4, pl = lane & 15;
@@ -2003,7 +2022,7 @@ __global__ void __launch_bounds__(256) k_gjk_cull(const uint32_t *bin_slot, uint
     bool need = false, kept = false;
     CullPair p{};   // this lane's pair (the exact stage takes the open pairs' from their lanes)
     if (i < n) {
-        p = cull_pair(bin_slot, off, i, C, B, K);
+        p = cull_pair(bin_slot, off, i, C, B, K, rec, recb);
         const float ra = C.lpos[p.cc.x].w, rb = C.lpos[p.cc.y].w;
         const float rs = (ra + rb) * 1.0001f + p.slack;
         bool apart = p.d2 > rs * rs;

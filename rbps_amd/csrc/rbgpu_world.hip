@@ -158,6 +158,8 @@ struct rbgpu_world {
     int hull_vpl = 1;                   // hull vertices per lane in the wave kernels (1: all hulls <= 64)
     uint32_t *gstat = nullptr;
     uint32_t *gjk_keep = nullptr, *gjk_pos = nullptr, *gjk_cand = nullptr, *gjk_ncand = nullptr;   // cull -> candidates
+    uint4 *gjk_rec = nullptr;   // per GJK-bin position, this frame: slot, colliders, body a (the cull's first load)
+    uint32_t *gjk_recb = nullptr;   // and body b
     uint32_t *gjk_dn = nullptr;   // pairs the fused GJK/EPA pass hands to the full-size EPA pass
     uint32_t *gjk_ckey = nullptr, *gjk_ckey2 = nullptr, *gjk_cand2 = nullptr;   // candidates sorted by last iterations
     int gjk_sort = 0;   // candidate order: 0 bin order, 1 iterations, 2 overlap, 3 both
@@ -277,6 +279,7 @@ struct rbgpu_world {
         dfree(pairs); dfree(ka); dfree(va); dfree(kb);
         dfree(solve_list); dfree(bin_list); dfree(p_hist); dfree(pair_lid); dfree(bb_hits); dfree(bb_blk); dfree(gstat);
         dfree(gjk_hits); dfree(gjk_blk); dfree(gjk_sx); dfree(gjk_keep); dfree(gjk_pos); dfree(gjk_cand);
+        dfree(gjk_rec); dfree(gjk_recb);
         dfree(gjk_ckey); dfree(gjk_ckey2); dfree(gjk_cand2);
         dfree(gjk_ncand); dfree(gjk_dn); dfree(gjk_axis); dfree(sc_work); dfree(sc_list);
         dfree(cb.acol); dfree(cb.tent); dfree(cb.rest); dfree(cb.act); dfree(cb.blk);
@@ -384,8 +387,10 @@ struct rbgpu_world {
             static_assert(TPB == 256, "k_gjk_compact_blk takes the cull's 256-pair workgroups");
             // bin order from the cull's per-workgroup counts (their scan in one workgroup)
             const bool blk = !gjk_sort && nblk(ng) <= (uint32_t)(SCAN_TOP_THREADS * SCAN_TOP_ITEMS);
+            // records of this frame's list (the GJK bins start at g0 every substep)
+            const bool rec = gjk_rec && g0 == bin_off[BIN_HH] && ng == bin_off[BIN_HP] - bin_off[BIN_HH];
             k_gjk_cull<<<nblk(ng), TPB, 0, st>>>(bin_list, g0, ng, C, B, K, hs, gjk_axis, gstat, gjk_keep,
-                                                 blk ? gjk_pos : nullptr);
+                                                 blk ? gjk_pos : nullptr, rec ? gjk_rec : nullptr, gjk_recb);
             if (blk) {
                 k_scan_top<<<1, SCAN_TOP_THREADS, 0, st>>>(gjk_pos, nblk(ng), gjk_ncand);
                 k_gjk_compact_blk<<<nblk(ng), TPB, 0, st>>>(gjk_keep, gjk_pos, ng, gjk_cand);
@@ -732,6 +737,7 @@ int rbgpu_world::ensure_pair_capacity(uint32_t need) {
         rc |= dalloc(gjk_hits, nh); rc |= dalloc(gjk_blk, nb1);
         rc |= dalloc(gjk_sx, nh * 8);
         rc |= dalloc(gjk_keep, cap); rc |= dalloc(gjk_pos, cap); rc |= dalloc(gjk_cand, cap); rc |= dalloc(gjk_ncand, 1);
+        rc |= dalloc(gjk_rec, cap); rc |= dalloc(gjk_recb, cap);
         rc |= dalloc(gjk_dn, 1);
         rc |= dalloc(gjk_ckey, cap); rc |= dalloc(gjk_ckey2, cap); rc |= dalloc(gjk_cand2, cap);
         rc |= dalloc(gjk_axis, cap);
@@ -1211,6 +1217,10 @@ int rbgpu_world::frame() {
     n_sap = rb->n_sap;
     n_pairs = rb->n_pairs;
     bin_off.assign(rb->bin_off, rb->bin_off + NBINS + 1);
+    if (gjk_rec && gjk_cull && bin_off[BIN_HP] > bin_off[BIN_HH])   // the frame's GJK-bin records
+        k_gjk_records<<<nblk(bin_off[BIN_HP] - bin_off[BIN_HH]), TPB, 0, st>>>(bin_list + bin_off[BIN_HH],
+                                                                                 bin_off[BIN_HP] - bin_off[BIN_HH], K,
+                                                                                 gjk_rec, gjk_recb);
     if (gjk_axis && bin_off[BIN_HP] > bin_off[BIN_HH])
         k_axis_load<<<nblk(bin_off[BIN_HP] - bin_off[BIN_HH]), TPB, 0, st>>>(bin_list + bin_off[BIN_HH],
                                                                                bin_off[BIN_HP] - bin_off[BIN_HH], K,
@@ -1655,20 +1665,29 @@ int rbgpu_autotune(rbgpu_world *w, int frames_per_trial, rbgpu_tune_report *out)
         int cur = 0;
         rc = rbgpu_get_option(w, k.opt, &cur);
         if (rc) return rc;
+        // two passes over the values, the second in reverse order, times summed per value: the
+        // scene moves between trials, so a single trial per value mostly measures which frames
+        // it got
         int best = cur;
         double best_ms = 1e300;
-        for (int v : k.vals) {
-            rc = rbgpu_set_option(w, k.opt, v);
+        std::vector<double> acc(k.vals.size(), 0.0);
+        const size_t nv = k.vals.size();
+        for (size_t t = 0; t < 2 * nv; ++t) {
+            const size_t j = t < nv ? t : 2 * nv - 1 - t;
+            rc = rbgpu_set_option(w, k.opt, k.vals[j]);
             if (rc) return rc;
             rc = time_frames(frames_per_trial, &ms);
             if (rc) return rc;
             rep.frames += frames_per_trial;
             ++rep.trials;
-            if (ms < best_ms) { best_ms = ms; best = v; }
+            acc[j] += ms;
         }
+        for (size_t j = 0; j < nv; ++j)
+            if (acc[j] < best_ms) { best_ms = acc[j]; best = k.vals[j]; }
+        (void)cur;
         rc = rbgpu_set_option(w, k.opt, best);
         if (rc) return rc;
-        best_all = best_ms / frames_per_trial;
+        best_all = best_ms / (2.0 * frames_per_trial);
     }
     rbgpu_get_option(w, RBGPU_OPT_SCENE_THREADS, &rep.scene_threads);
     rbgpu_get_option(w, RBGPU_OPT_GRAPH, &rep.graph);
